@@ -1,0 +1,134 @@
+"""Benchmark of the spin-projection grid kernel on one MI355X.
+
+Workload (the plain-FP64 GPU configuration): synthetic factored Hamiltonian
+n=64 spatial orbitals (NSO=128), N=32 electrons, L=192, Lebedev degree 11
+(50 points) x 16 gamma = 800 grid points, complex128, seed 1002.
+
+One step = one projected energy + gradient evaluation (EandG, PHF.py:55-91):
+phftools.buildhsg -> EigenSolver -> phftools.localg, Hamiltonian resident on
+the device.  value = grid points / s over the timed steps.
+
+roofline: the ERI GEMM (the dominant kernel), algorithmic FLOPs
+4*NSO^4 per grid point (real A, re/im density columns) over its average
+launch time measured with HIP events on the library's stream.
+
+cpu_baseline: the oracle's CPU restatement on a bounded sample of grid points,
+scaled to grid points / s.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"fp64": 78.6e12}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_workload(n=64, nocc=32, ngrid=(50, 16), seed=1002):
+    from phf_pcc_amd.hamiltonian import synthetic_problem
+    from tests._cases import grid_args
+    t0 = time.time()
+    prob = synthetic_problem(n, nocc, seed=seed, build_dense=True, ngrid=ngrid)
+    log(f"[bench] built synthetic Hamiltonian n={n} NSO={2*n} in {time.time()-t0:.1f}s")
+    ga = grid_args(n, ngrid, 0)
+    return prob, ga
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--nocc", type=int, default=32)
+    ap.add_argument("--cpu-points", type=int, default=24)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the grid-point sharding over GPUs "
+              "is a design in DESIGN.md, not a measured path", file=sys.stderr)
+        sys.exit(2)
+
+    from oracle import phf_oracle as O
+    from phf_pcc_amd.PHFTools_F import phftools
+
+    prob, ga = make_workload(args.n, args.nocc)
+    nso, nocc = prob.nso, prob.nocc
+    G = int(ga["ngrid"][0] * ga["ngrid"][1])
+    ref = prob.ref.astype(complex)
+    t0 = time.time()
+    phftools.set_hamiltonian(prob.hone, prob.htwo)
+    log(f"[bench] uploaded Hamiltonian in {time.time()-t0:.1f}s")
+    ctx = phftools.context()
+    ctx.set_profiling(True)
+
+    call = (prob.hone, prob.htwo, ref, prob.z, ga["ngrid"], ga["nci"], 1, ga["ncisp"], nocc, ga["r1"], ga["r2"],
+            ga["rpg"], ga["weightsp"], ga["weightpg"], ga["roota"], ga["rootb"], ga["rooty"], 0, 1, 0)
+
+    def step():
+        h, s, gm, rm = phftools.buildhsg(*call)
+        ev, vec = O.eigen_solver(h, s)
+        g = phftools.localg(gm, rm, s, ev[0], vec[:, 0] / vec[0, 0], np.ones(1), np.ones(1), 1)
+        return ev[0].real, g
+
+    for i in range(args.warmup):
+        e, _ = step()
+        log(f"[bench] warmup {i}: E={e:.12f}")
+    gemm_ms, dev_ms = [], []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        e, g = step()
+        st = ctx.stage_times()
+        dev_ms.append(st[0])
+        gemm_ms.append(st[2])
+    dt = time.perf_counter() - t0
+    log(f"[bench] {args.steps} steps in {dt:.3f}s; E={e:.12f}; device {np.mean(dev_ms):.3f} ms, "
+        f"ERI GEMM {np.mean(gemm_ms):.3f} ms")
+    ms_step = dt / args.steps * 1e3
+    value = G * args.steps / dt
+    flops = 4.0 * nso ** 4 * G
+    t_gemm = np.mean(gemm_ms) * 1e-3
+    achieved = flops / t_gemm
+    roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / PEAK["fp64"], "traffic": None, "kernel": "eri_gemm_f64_kernel",
+            "flops_per_launch": flops}
+
+    cpu = None
+    if not args.no_cpu:
+        npts = min(args.cpu_points, G)
+        pts = [(il, iy, 0) for il in range(int(ga["ngrid"][0])) for iy in range(int(ga["ngrid"][1]))][:npts]
+        ga_cpu = dict(ga)
+        t0 = time.perf_counter()
+        O.buildhsg(prob.hone, prob.htwo, ref, prob.z, cmplxconj=1, nocc=nocc, points=pts, check_hermitian=False,
+                   **ga_cpu)
+        tc = time.perf_counter() - t0
+        cpu = {"value": npts / tc, "unit": "grid pts/s", "cores": int(os.environ.get("OMP_NUM_THREADS", "1")),
+               "kind": "port",
+               "sample": f"numpy restatement of BuildHSG on {npts} of {G} grid points (same Hamiltonian), "
+                         f"{tc:.1f}s"}
+        log(f"[bench] cpu baseline {cpu['value']:.2f} grid pts/s")
+
+    out = {"metric": "projected E+grad evals/s (grid pts/s) vs plain FP64 GPU at 1e-6 Eh; % MFMA peak",
+           "value": value, "unit": "grid pts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"plain FP64 projected E+grad, synthetic n={args.n} (NSO={nso}), N={nocc}, "
+                                  f"L={3*args.n}, grid 50x16={G}", "form": "plain-fp64",
+                      "evals_per_s": value / G, "grid_points": G},
+           "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

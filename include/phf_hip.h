@@ -1,0 +1,100 @@
+/*
+ * phf_hip.h -- C ABI of the MI355X spin-projection grid kernel (libphf_hip.so).
+ *
+ * Drop-in replacement for the f2py module PHFTools_F built from
+ * PHF_fort/PHFTools.f90 (PHF_fort/makefile:7) and called from PHF.py:70-76.
+ * Every entry point returns PHF_OK (0) or one of the error codes below; it
+ * never terminates the process (the reference STOPs, e.g. Wrappers.f90:311,
+ * PHFTools.f90:199).  phf_last_error() gives the message of the last failure.
+ *
+ * Array conventions: plain host pointers; complex arrays are complex128
+ * (interleaved re, im doubles); multi-index arrays are C (row-major) order,
+ * i.e. exactly numpy's default layout of the arrays the reference passes.
+ * Element (i,j,k,l) of an [a][b][c][d] array sits at ((i*b+j)*c+k)*d+l.
+ * Index meanings are those of the Fortran dummies of the same name.
+ */
+#ifndef PHF_HIP_H
+#define PHF_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct phf_ctx_s phf_ctx;
+
+enum {
+  PHF_OK = 0,
+  PHF_ERR_ARG = 1,            /* bad argument / shape (f2py raises ValueError) */
+  PHF_ERR_HIP = 2,            /* HIP runtime failure */
+  PHF_ERR_SINGULAR = 3,       /* exactly singular overlap at a grid point (Wrappers.f90:311) */
+  PHF_ERR_NOT_HERMITIAN = 4,  /* Hmat/Smat fail the Hermiticity test (PHFTools.f90:195-200) */
+  PHF_ERR_NO_HAMILTONIAN = 5, /* phf_buildhsg before phf_set_hamiltonian */
+  PHF_ERR_UNSUPPORTED = 6,    /* option not implemented on this path */
+  PHF_ERR_NOMEM = 7           /* device allocation failed */
+};
+
+/* Precision of the ERI contraction (the overlap LU, determinant, densities
+ * and the grid sum are always FP64). */
+enum {
+  PHF_PREC_FP64 = 0
+};
+
+/* Context: one GPU, one HIP stream, a cached device Hamiltonian and a
+ * workspace arena.  No counterpart in the reference (its state lives in
+ * Fortran locals). */
+int phf_ctx_create(int device, phf_ctx** out);
+int phf_ctx_destroy(phf_ctx* ctx);
+/* Message of the last failure on ctx (or of the last failed phf_ctx_create
+ * when ctx is NULL). */
+const char* phf_last_error(const phf_ctx* ctx);
+
+/* Upload the MO-basis Hamiltonian that BuildHSG receives as HOne / HTwo
+ * (PHFTools.f90:17): hone complex [nso][nso], htwo complex [nso]^4 with
+ * htwo[i][j][k][l] = <ij||kl>.  The device keeps A[(i,k),(j,l)] = V(i,j,k,l)
+ * (real plane only when every imaginary part is exactly zero) until the next
+ * call.  flags: bit 0 forces the complex form. */
+int phf_set_hamiltonian(phf_ctx* ctx, const double* hone, const double* htwo, int nso, int flags);
+
+/* phftools.buildhsg (PHFTools.f90:10-201; call site PHF.py:70-71).
+ *   ref      complex [nso][nso]         Ref (MO coefficients, AO rows)
+ *   z        complex [nso-nocc][nocc]   Thouless Z
+ *   ngrid    int[2]                     (nl, ny)
+ *   r1       complex [nr1][nso][nso], r2 complex [nr2][nso][nso],
+ *   rpg      complex [npg][nso][nso]
+ *   weightsp double [nl][ny]; weightpg double [npg][ncipg][ncipg]
+ *   roota/rootb/rooty double[na]/[nb]/[ny_] (Wigner angles)
+ *   qj, sp   J and spin-projection type (SP=2 swaps the angle indices)
+ *   precision  PHF_PREC_*
+ * Outputs (caller-allocated): hmat, smat complex [nci][nci];
+ *   gradmat, rdmmat complex [nci][nci][nso][nso].
+ * The MPI communicator argument of the reference (PHFTools.f90:41) has no
+ * counterpart: one context is one GPU. */
+int phf_buildhsg(phf_ctx* ctx, const double* ref, const double* z, const int* ngrid, int nci,
+                 int cmplxconj, int ncisp, int nocc, const double* r1, int nr1, const double* r2,
+                 int nr2, const double* rpg, int npg, const double* weightsp,
+                 const double* weightpg, int ncipg, const double* roota, int na,
+                 const double* rootb, int nb, const double* rooty, int ny_, int qj, int sp,
+                 int precision, double* hmat, double* smat, double* gradmat, double* rdmmat);
+
+/* phftools.localg (PHFTools.f90:203-247; call site PHF.py:76).
+ *   gradmat, rdmmat complex [nci][nci][nso][nso]; smat complex [nci][nci];
+ *   e0 complex scalar (2 doubles); fsp complex[ncisp]; fpg complex[ncipg];
+ *   fk complex[nfk] (nfk >= 2 when cmplxconj == 2).
+ * Output g: complex [nso][nso]. */
+int phf_localg(phf_ctx* ctx, const double* gradmat, const double* rdmmat, const double* smat,
+               const double* e0, const double* fsp, int ncisp, const double* fpg, int ncipg,
+               const double* fk, int nfk, int cmplxconj, int nso, int nci, double* g);
+
+/* Stage timing with HIP events on the context's stream (no counterpart in the
+ * reference, which only prints wall-clock times, Main_PHF.py:15,40,44).
+ * After a profiled phf_buildhsg, phf_stage_times fills up to nmax values in
+ * milliseconds: [0] all device work of the call, [1] rotation + overlap +
+ * inverse + density pack, [2] ERI GEMM, [3] epilogue + grid sums; it returns
+ * how many it wrote. */
+int phf_set_profiling(phf_ctx* ctx, int on);
+int phf_stage_times(const phf_ctx* ctx, double* ms, int nmax);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PHF_HIP_H */

@@ -1,0 +1,164 @@
+"""Drop-in for the f2py extension ``PHFTools_F`` (PHF_fort/makefile:7).
+
+``from phf_pcc_amd.PHFTools_F import phftools`` replaces
+``from PHFTools_F import phftools`` (PHF.py:10).  The calling convention is
+the one f2py gives the Fortran module procedures (call sites PHF.py:70-71,76):
+
+  hmat, smat, gradmat, rdmmat = phftools.buildhsg(
+      hone, htwo, ref, z, ngrid, nci, cmplxconj, ncisp, nocc, r1, r2, rpg,
+      weightsp, weightpg, roota, rootb, rooty, qj, sp, comm, [npg, ncipg, nso])
+  g = phftools.localg(gradmat, rdmmat, smat, e0, fsp, fpg, fk, cmplxconj,
+                      [nso, nci, ncisp, ncipg])
+
+Inputs are cast to complex128 / float64 / int as f2py would; a shape that
+disagrees with the explicit dimension arguments raises ValueError.  Numerical
+failures that STOP the reference process raise instead:
+``SingularOverlapError`` (a numpy LinAlgError; Wrappers.f90:311) and
+``NotHermitianError`` (PHFTools.f90:195-200).  ``comm`` (an MPI handle) is
+accepted and ignored: one context drives one GPU.
+
+The device keeps the last Hamiltonian.  It is re-uploaded when ``htwo`` or
+``hone`` differs from the cached pair by identity, address, shape or a
+content fingerprint (a sample of elements); ``phftools.set_hamiltonian``
+uploads explicitly and ``phftools.invalidate()`` drops the cache.
+
+``phftools.precision`` selects the ERI-contraction precision (default
+"fp64", the reference's arithmetic).
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+
+import numpy as np
+
+from . import _lib
+
+
+def _fingerprint(a: np.ndarray) -> tuple:
+    flat = a.reshape(-1)
+    n = flat.size
+    h = hashlib.blake2b(digest_size=16)
+    if n <= 1 << 16:
+        h.update(np.ascontiguousarray(flat).tobytes())
+    else:
+        idx = np.unique(np.concatenate([np.arange(4096), np.linspace(0, n - 1, 1 << 16).astype(np.int64),
+                                        np.arange(n - 4096, n)]))
+        h.update(np.ascontiguousarray(flat[idx]).tobytes())
+    return (a.__array_interface__["data"][0], a.shape, a.dtype.str, h.hexdigest())
+
+
+class _PhfTools:
+    def __init__(self):
+        self._ctx = None
+        self.precision = "fp64"
+        self.device = int(os.environ.get("PHF_DEVICE", "0"))
+
+    # ------------------------------------------------------------------
+    def context(self) -> _lib.Context:
+        if self._ctx is None:
+            self._ctx = _lib.Context(self.device)
+        return self._ctx
+
+    def invalidate(self):
+        if self._ctx is not None:
+            self._ctx.ham_key = None
+
+    def set_hamiltonian(self, hone, htwo):
+        key = (_fingerprint(np.asarray(hone)), _fingerprint(np.asarray(htwo)))
+        self.context().set_hamiltonian(hone, htwo, key=key)
+
+    def _ensure_ham(self, hone, htwo):
+        ctx = self.context()
+        key = (_fingerprint(hone), _fingerprint(htwo))
+        if ctx.ham_key != key:
+            ctx.set_hamiltonian(hone, htwo, key=key)
+
+    # ------------------------------------------------------------------
+    def buildhsg(self, hone, htwo, ref, z, ngrid, nci, cmplxconj, ncisp, nocc, r1, r2, rpg,
+                 weightsp, weightpg, roota, rootb, rooty, qj, sp, comm=None,
+                 npg=None, ncipg=None, nso=None):
+        """BuildHSG (PHFTools.f90:10-201) on the GPU."""
+        hone = np.asarray(hone)
+        htwo = np.asarray(htwo)
+        nso = hone.shape[0] if nso is None else int(nso)
+        nocc = int(nocc)
+        rpg = np.ascontiguousarray(rpg, dtype=np.complex128)
+        weightpg = np.ascontiguousarray(weightpg, dtype=np.float64)
+        npg = rpg.shape[0] if npg is None else int(npg)
+        ncipg = weightpg.shape[1] if ncipg is None else int(ncipg)
+        ngrid = np.ascontiguousarray(ngrid, dtype=np.int32).reshape(-1)
+        if ngrid.size != 2:
+            raise ValueError("ngrid must have 2 entries")
+        ref = np.ascontiguousarray(ref, dtype=np.complex128)
+        z = np.ascontiguousarray(z, dtype=np.complex128)
+        r1 = np.ascontiguousarray(r1, dtype=np.complex128)
+        r2 = np.ascontiguousarray(r2, dtype=np.complex128)
+        weightsp = np.ascontiguousarray(weightsp, dtype=np.float64)
+        roota = np.ascontiguousarray(roota, dtype=np.float64).reshape(-1)
+        rootb = np.ascontiguousarray(rootb, dtype=np.float64).reshape(-1)
+        rooty = np.ascontiguousarray(rooty, dtype=np.float64).reshape(-1)
+        nl, ny = int(ngrid[0]), int(ngrid[1])
+        checks = [
+            (hone.shape == (nso, nso), "hone must be (nso, nso)"),
+            (htwo.shape == (nso,) * 4, "htwo must be (nso, nso, nso, nso)"),
+            (ref.shape == (nso, nso), "ref must be (nso, nso)"),
+            (z.shape == (nso - nocc, nocc), "z must be (nso-nocc, nocc)"),
+            (r1.ndim == 3 and r1.shape[1:] == (nso, nso), "r1 must be (*, nso, nso)"),
+            (r2.ndim == 3 and r2.shape[1:] == (nso, nso), "r2 must be (*, nso, nso)"),
+            (rpg.shape == (npg, nso, nso), "rpg must be (npg, nso, nso)"),
+            (weightpg.shape == (npg, ncipg, ncipg), "weightpg must be (npg, ncipg, ncipg)"),
+            (weightsp.ndim == 2 and weightsp.shape[0] >= nl and weightsp.shape[1] >= ny,
+             "weightsp must cover (ngrid[0], ngrid[1])"),
+        ]
+        for ok, msg in checks:
+            if not ok:
+                raise ValueError(f"buildhsg: {msg}")
+        if weightsp.shape != (nl, ny):
+            weightsp = np.ascontiguousarray(weightsp[:nl, :ny])
+        self._ensure_ham(hone, htwo)
+        ctx = self.context()
+        nci = int(nci)
+        hmat = np.zeros((nci, nci), dtype=np.complex128)
+        smat = np.zeros((nci, nci), dtype=np.complex128)
+        gradmat = np.zeros((nci, nci, nso, nso), dtype=np.complex128)
+        rdmmat = np.zeros((nci, nci, nso, nso), dtype=np.complex128)
+        prec = _lib.PREC[self.precision]
+        lib = _lib.load()
+        rc = lib.phf_buildhsg(ctx.handle, _lib.ptr(ref), _lib.ptr(z), ngrid.ctypes.data_as(_lib._ip), nci,
+                              int(cmplxconj), int(ncisp), nocc, _lib.ptr(r1), r1.shape[0], _lib.ptr(r2),
+                              r2.shape[0], _lib.ptr(rpg), npg, _lib.ptr(weightsp), _lib.ptr(weightpg), ncipg,
+                              _lib.ptr(roota), roota.size, _lib.ptr(rootb), rootb.size, _lib.ptr(rooty),
+                              rooty.size, int(qj), int(sp), prec, _lib.ptr(hmat), _lib.ptr(smat),
+                              _lib.ptr(gradmat), _lib.ptr(rdmmat))
+        _lib.check(rc, ctx.handle)
+        return hmat, smat, gradmat, rdmmat
+
+    def localg(self, gradmat, rdmmat, smat, e0, fsp, fpg, fk, cmplxconj,
+               nso=None, nci=None, ncisp=None, ncipg=None):
+        """LocalG (PHFTools.f90:203-247) on the GPU."""
+        gradmat = np.ascontiguousarray(gradmat, dtype=np.complex128)
+        rdmmat = np.ascontiguousarray(rdmmat, dtype=np.complex128)
+        smat = np.ascontiguousarray(smat, dtype=np.complex128)
+        fsp = np.ascontiguousarray(fsp, dtype=np.complex128).reshape(-1)
+        fpg = np.ascontiguousarray(fpg, dtype=np.complex128).reshape(-1)
+        fk = np.ascontiguousarray(fk, dtype=np.complex128).reshape(-1)
+        nso = gradmat.shape[2] if nso is None else int(nso)
+        nci = gradmat.shape[0] if nci is None else int(nci)
+        ncisp = fsp.size if ncisp is None else int(ncisp)
+        ncipg = fpg.size if ncipg is None else int(ncipg)
+        if gradmat.shape != (nci, nci, nso, nso) or rdmmat.shape != gradmat.shape or smat.shape != (nci, nci):
+            raise ValueError("localg: gradmat/rdmmat must be (nci, nci, nso, nso) and smat (nci, nci)")
+        if fsp.size < ncisp or fpg.size < ncipg:
+            raise ValueError("localg: fsp/fpg shorter than ncisp/ncipg")
+        e0a = np.array([complex(e0)], dtype=np.complex128)
+        g = np.zeros((nso, nso), dtype=np.complex128)
+        ctx = self.context()
+        rc = _lib.load().phf_localg(ctx.handle, _lib.ptr(gradmat), _lib.ptr(rdmmat), _lib.ptr(smat),
+                                    _lib.ptr(e0a), _lib.ptr(fsp), ncisp, _lib.ptr(fpg), ncipg, _lib.ptr(fk),
+                                    fk.size, int(cmplxconj), nso, nci, _lib.ptr(g))
+        _lib.check(rc, ctx.handle)
+        return g
+
+
+phftools = _PhfTools()

@@ -1,0 +1,143 @@
+"""ctypes binding of libphf_hip.so (declarations in include/phf_hip.h).
+
+The library is required: importing the product without it raises, there is
+no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("PHF_HIP_LIB", os.path.join(HERE, "lib", "libphf_hip.so"))
+
+PHF_OK = 0
+PHF_ERR_ARG = 1
+PHF_ERR_HIP = 2
+PHF_ERR_SINGULAR = 3
+PHF_ERR_NOT_HERMITIAN = 4
+PHF_ERR_NO_HAMILTONIAN = 5
+PHF_ERR_UNSUPPORTED = 6
+PHF_ERR_NOMEM = 7
+
+PREC = {"fp64": 0}
+
+# every symbol declared in include/phf_hip.h
+EXPORTS = ["phf_ctx_create", "phf_ctx_destroy", "phf_last_error", "phf_set_hamiltonian",
+           "phf_buildhsg", "phf_localg", "phf_set_profiling", "phf_stage_times"]
+
+
+class PhfError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[phf error {code}] {msg}")
+        self.code = code
+
+
+class SingularOverlapError(PhfError, np.linalg.LinAlgError):
+    pass
+
+
+class NotHermitianError(PhfError):
+    pass
+
+
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libphf_hip.so not found at {LIB_PATH}; run phf_pcc_amd.build_lib.build() first")
+    lib = C.CDLL(LIB_PATH)
+    lib.phf_ctx_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
+    lib.phf_ctx_destroy.argtypes = [C.c_void_p]
+    lib.phf_last_error.argtypes = [C.c_void_p]
+    lib.phf_last_error.restype = C.c_char_p
+    lib.phf_set_hamiltonian.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_int]
+    lib.phf_buildhsg.argtypes = [C.c_void_p, _dp, _dp, _ip, C.c_int, C.c_int, C.c_int, C.c_int,
+                                 _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp, C.c_int,
+                                 _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                 _dp, _dp, _dp, _dp]
+    lib.phf_localg.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int,
+                               C.c_int, C.c_int, C.c_int, _dp]
+    lib.phf_set_profiling.argtypes = [C.c_void_p, C.c_int]
+    lib.phf_stage_times.argtypes = [C.c_void_p, _dp, C.c_int]
+    for name in EXPORTS:
+        if name != "phf_last_error":
+            getattr(lib, name).restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp)
+
+
+def check(rc: int, ctx) -> None:
+    if rc == PHF_OK:
+        return
+    msg = load().phf_last_error(ctx).decode(errors="replace")
+    if rc == PHF_ERR_SINGULAR:
+        raise SingularOverlapError(rc, msg)
+    if rc == PHF_ERR_NOT_HERMITIAN:
+        raise NotHermitianError(rc, msg)
+    if rc == PHF_ERR_ARG:
+        raise ValueError(msg)
+    raise PhfError(rc, msg)
+
+
+class Context:
+    """One GPU, one HIP stream, one cached device Hamiltonian."""
+
+    def __init__(self, device: int = 0):
+        lib = load()
+        h = C.c_void_p()
+        rc = lib.phf_ctx_create(int(device), C.byref(h))
+        if rc != PHF_OK:
+            raise PhfError(rc, lib.phf_last_error(None).decode(errors="replace"))
+        self._h = h
+        self.device = device
+        self.ham_key = None
+        self.nso = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().phf_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_profiling(self, on: bool = True):
+        check(load().phf_set_profiling(self._h, 1 if on else 0), self._h)
+
+    def stage_times(self) -> np.ndarray:
+        out = np.zeros(8)
+        n = load().phf_stage_times(self._h, ptr(out), 8)
+        return out[:n].copy()
+
+    def set_hamiltonian(self, hone: np.ndarray, htwo: np.ndarray, force_complex: bool = False, key=None):
+        hone = np.ascontiguousarray(hone, dtype=np.complex128)
+        htwo = np.ascontiguousarray(htwo, dtype=np.complex128)
+        nso = hone.shape[0]
+        if hone.shape != (nso, nso) or htwo.shape != (nso,) * 4:
+            raise ValueError("hone must be (nso, nso) and htwo (nso, nso, nso, nso)")
+        check(load().phf_set_hamiltonian(self._h, ptr(hone), ptr(htwo), nso, 1 if force_complex else 0), self._h)
+        self.nso = nso
+        self.ham_key = key

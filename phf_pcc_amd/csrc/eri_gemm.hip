@@ -1,0 +1,113 @@
+// The ERI contraction of all grid points as one matrix-core GEMM.
+//
+// MakeGamma (PHFTools.f90:310-337) computes, per grid point,
+//   Gamma_g(i,k) = sum_{j,l} V(i,j,k,l) rho_g(l,j).
+// With A[(i,k),(j,l)] = V(i,j,k,l) (NSO^2 x NSO^2, cached on the device) and
+// P[(j,l), g] = rho_g(l,j), all grid points together are Gamma = A P.  The
+// kernel computes the transposed product
+//   C[row][r] = sum_c Pt[row][c] * A[r][c]
+// where row runs over (grid point, re/im) and both operands are stored with
+// the contraction index c contiguous ("NT"), so one loader serves both.  For a
+// complex A the contraction runs over [A_re | A_im] and the density rows are
+// [P_re | -P_im] and [P_im | P_re].
+//
+// Layout contract: Pt is [Mp][K], A is [Nr][K], C is [Mp][ldc]; Mp % BM == 0,
+// Nr % BN == 0, K % 16 == 0 (the host pads with zeros).
+#include "phf_common.h"
+
+namespace {
+
+template <int BM, int BN>
+__global__ __launch_bounds__(256, 2) void eri_gemm_f64_kernel(const double* __restrict__ P,
+                                                               const double* __restrict__ A,
+                                                               double* __restrict__ C, int K, long ldc,
+                                                               int tiles_m) {
+  constexpr int BK = 16, LDSW = BK + 2;
+  constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+  constexpr int PL = BM / 32, AL = BN / 32;
+  __shared__ double sP[2][BM * LDSW];
+  __shared__ double sA[2][BN * LDSW];
+
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int mt = t % tiles_m, nt = t / tiles_m;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const double* Pg = P + (long)mt * BM * K;
+  const double* Ag = A + (long)nt * BN * K;
+  const int lr = tid >> 3, lc = (tid & 7) * 2;
+
+  d4_t acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+
+  double2 rp[PL], ra[AL];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < PL; ++e)
+      rp[e] = *reinterpret_cast<const double2*>(Pg + (long)(lr + 32 * e) * K + k0 + lc);
+#pragma unroll
+    for (int e = 0; e < AL; ++e)
+      ra[e] = *reinterpret_cast<const double2*>(Ag + (long)(lr + 32 * e) * K + k0 + lc);
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < PL; ++e)
+      *reinterpret_cast<double2*>(&sP[buf][(lr + 32 * e) * LDSW + lc]) = rp[e];
+#pragma unroll
+    for (int e = 0; e < AL; ++e)
+      *reinterpret_cast<double2*>(&sA[buf][(lr + 32 * e) * LDSW + lc]) = ra[e];
+  };
+
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  const int nk = K / BK;
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) gload((kt + 1) * BK);
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 4) {
+      double a[FM], b[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) a[i] = sP[cur][(wm * WM + i * 16 + (lane & 15)) * LDSW + ks + (lane >> 4)];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) b[j] = sA[cur][(wn * WN + j * 16 + (lane & 15)) * LDSW + ks + (lane >> 4)];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    if (kt + 1 < nk) lstore(cur ^ 1);
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long row = (long)mt * BM + wm * WM + i * 16 + 4 * (lane >> 4) + r;
+        const long col = (long)nt * BN + wn * WN + j * 16 + (lane & 15);
+        C[row * ldc + col] = acc[i][j][r];
+      }
+}
+
+}  // namespace
+
+// Tile height for the grid-point dimension: the host pads Mp to this.
+int eri_gemm_f64_bm(int rows) { return rows <= 64 * 12 ? 64 : 128; }
+
+void eri_gemm_f64(hipStream_t s, const double* P, const double* A, double* C, int Mp, int Nr, int K,
+                  long ldc, int bm) {
+  if (bm == 64) {
+    const int tm = Mp / 64, tn = Nr / 128;
+    eri_gemm_f64_kernel<64, 128><<<tm * tn, 256, 0, s>>>(P, A, C, K, ldc, tm);
+  } else {
+    const int tm = Mp / 128, tn = Nr / 128;
+    eri_gemm_f64_kernel<128, 128><<<tm * tn, 256, 0, s>>>(P, A, C, K, ldc, tm);
+  }
+}

@@ -1,0 +1,773 @@
+// C ABI and host orchestration of the spin-projection grid pipeline.
+//
+// Per call of phf_buildhsg (BuildHSG, PHFTools.f90:10-201), with grid points
+// g = (il, iy, ipg), g = (il*ny + iy)*npg + ipg (the reference's loop order):
+//   K1  rotation chain   T_g = Ref^H R1[il] R2[iy] Rpg[ipg] Ref C0   (zgemm, right to left)
+//       overlap          M_g = C0^H T_g
+//   K2  inverse + det    M_g^{-1}, d_g = det M_g                     (lu_inverse_batched)
+//       solve            X_g = T_g M_g^{-1},  rho_g = X_g C0^H
+//       density pack     Pt rows (g,re),(g,im) = rho_g^T            (zgemm, split planes)
+//   K3  ERI GEMM         Gamma_g = A rho_g for all g at once         (eri_gemm_*)
+//   K4  epilogue         F = h + Gamma_g; FX = F X_g; Q = C0^H FX;
+//                        e_g = (tr(C0^H h X) + tr Q)/2 = Makeh;
+//                        Y_g = FX - X_g Q + e_g X_g, so that the reference's
+//                        Gradient (I-rho)(h+Gamma)rho + e rho = Y_g C0^H
+//   K5  grid sum         Wd = w * Wigner * weightpg * d_g;
+//                        Hmat = sum Wd e, Smat = sum Wd, Ysum = sum Wd Y,
+//                        Xsum = sum Wd X (fixed order, deterministic);
+//                        Gradmat = Ysum C0^H, Rdmmat = Xsum C0^H.
+// The MPI reduce/bcast of the reference (PHFTools.f90:181-188) has no
+// counterpart on one GPU.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "phf_common.h"
+#include "../../include/phf_hip.h"
+
+void lu_inverse_batched(hipStream_t s, int n, const double* M, long mbs, double* Minv, long ibs,
+                        double* det, int* info, int batch);
+int lu_inverse_max_n();
+void eri_gemm_f64(hipStream_t s, const double* P, const double* A, double* C, int Mp, int Nr, int K,
+                  long ldc, int bm);
+int eri_gemm_f64_bm(int rows);
+
+struct phf_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // cached Hamiltonian
+  int nso = 0;
+  bool has_ham = false;
+  bool cplx = false;
+  int kpad = 0, nrpad = 0, kt = 0;
+  double* dA = nullptr;     // [nrpad][kt] FP64
+  double* dH1 = nullptr;    // hone, row-major complex128
+  // workspace arena
+  char* ws = nullptr;
+  size_t ws_size = 0;
+  // per-stage timing with HIP events on `stream` (phf_set_profiling)
+  bool profiling = false;
+  hipEvent_t ev[8] = {};
+  double stage_ms[8] = {};
+  int nstage = 0;
+};
+
+static thread_local std::string g_create_err;
+
+namespace {
+
+typedef std::complex<double> cplx;
+
+long round_up(long x, long m) { return (x + m - 1) / m * m; }
+
+int fail(phf_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+#define HIPCK(expr)                                                                   \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(ctx, e_ == hipErrorOutOfMemory ? PHF_ERR_NOMEM : PHF_ERR_HIP,       \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                 \
+  } while (0)
+
+struct Arena {
+  char* base;
+  size_t off = 0;
+  template <class T>
+  T* take(size_t count) {
+    off = (off + 255) & ~size_t(255);
+    T* p = reinterpret_cast<T*>(base ? base + off : nullptr);
+    off += count * sizeof(T);
+    return p;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------------
+
+__global__ void fill_zero_kernel(double* p, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+void fill_zero(hipStream_t s, double* p, long n) {
+  if (n <= 0) return;
+  long blocks = std::min<long>((n + 255) / 256, 65536);
+  fill_zero_kernel<<<blocks, 256, 0, s>>>(p, n);
+}
+
+__global__ void imag_nonzero_kernel(const double* stage, long nelem, int* flag) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nelem; i += (long)gridDim.x * blockDim.x)
+    if (stage[2 * i + 1] != 0.0) atomicOr(flag, 1);
+}
+
+// A[r][c'] for r < nrpad, c' < kt, from the C-order complex V(i,j,k,l):
+// r = i*nso + k, c = j*nso + l, c' = half*kpad + c (half 1 = imaginary plane).
+__global__ void ham_permute_kernel(const double* __restrict__ stage, double* __restrict__ A, int nso,
+                                   int kpad, int kt, long nrpad) {
+  const long n2 = (long)nso * nso;
+  const long total = nrpad * kt;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const long r = idx / kt;
+    const int cp = (int)(idx % kt);
+    const int half = cp / kpad, c = cp % kpad;
+    double v = 0.0;
+    if (r < n2 && c < n2) {
+      const long i = r / nso, k = r % nso, j = c / nso, l = c % nso;
+      v = stage[((((i * nso + j) * nso + k) * nso) + l) * 2 + half];
+    }
+    A[idx] = v;
+  }
+}
+
+// complex A: rows (g,re) = [P_re | -P_im], (g,im) = [P_im | P_re]
+__global__ void pack_cplx_kernel(double* Pt, int G, int n2, int kpad, int kt) {
+  const int g = blockIdx.y;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n2; c += gridDim.x * blockDim.x) {
+    double* re = Pt + (long)(2 * g) * kt;
+    double* im = Pt + (long)(2 * g + 1) * kt;
+    const double pr = re[c], pi = im[c];
+    re[kpad + c] = -pi;
+    im[kpad + c] = pr;
+  }
+}
+
+__global__ void wd_kernel(int G, int nmn, const double* W, const double* det, double* Wd) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G * nmn) return;
+  const int g = i / nmn;
+  const double wr = W[2 * i], wi = W[2 * i + 1], dr = det[2 * g], di = det[2 * g + 1];
+  Wd[2 * i] = wr * dr - wi * di;
+  Wd[2 * i + 1] = wr * di + wi * dr;
+}
+
+// e_g = (sum_{a,p} Hc(a,p) X_g(p,a) + sum_a Q_g(a,a)) / 2; one workgroup per g.
+__global__ __launch_bounds__(256) void energy_kernel(int nso, int nocc, const double* __restrict__ Hc,
+                                                     const double* __restrict__ X, const double* __restrict__ Q,
+                                                     double* __restrict__ E) {
+  __shared__ double sr[256], si[256];
+  const long g = blockIdx.x;
+  const double* Xg = X + g * 2L * nso * nocc;
+  const double* Qg = Q + g * 2L * nocc * nocc;
+  double ar = 0.0, ai = 0.0;
+  for (int idx = threadIdx.x; idx < nso * nocc; idx += 256) {
+    const int p = idx % nso, a = idx / nso;
+    const double hr = Hc[2 * (a + (long)p * nocc)], hi = Hc[2 * (a + (long)p * nocc) + 1];
+    const double xr = Xg[2 * idx], xi = Xg[2 * idx + 1];
+    ar += hr * xr - hi * xi;
+    ai += hr * xi + hi * xr;
+  }
+  for (int a = threadIdx.x; a < nocc; a += 256) {
+    ar += Qg[2 * (a + a * nocc)];
+    ai += Qg[2 * (a + a * nocc) + 1];
+  }
+  sr[threadIdx.x] = ar;
+  si[threadIdx.x] = ai;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      sr[threadIdx.x] += sr[threadIdx.x + s];
+      si[threadIdx.x] += si[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    E[2 * g] = 0.5 * sr[0];
+    E[2 * g + 1] = 0.5 * si[0];
+  }
+}
+
+// Y_g = FX_g + e_g X_g
+__global__ void yinit_kernel(long per_g, int G, const double* FX, const double* X, const double* E,
+                             double* Y) {
+  const long total = per_g * G;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const long g = i / per_g;
+    const double er = E[2 * g], ei = E[2 * g + 1];
+    const double xr = X[2 * i], xi = X[2 * i + 1];
+    Y[2 * i] = FX[2 * i] + er * xr - ei * xi;
+    Y[2 * i + 1] = FX[2 * i + 1] + er * xi + ei * xr;
+  }
+}
+
+// partial[c][mn][e] = sum_{g in chunk c} Wd[g][mn] * V[g][e]   (complex)
+__global__ void wreduce_partial_kernel(int G, int gchunk, int len, int nmn, const double* __restrict__ Wd,
+                                       const double* __restrict__ V, double* __restrict__ part) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (e >= len) return;
+  const int g0 = c * gchunk, g1 = min(G, g0 + gchunk);
+  for (int mn = 0; mn < nmn; ++mn) {
+    double ar = 0.0, ai = 0.0;
+    for (int g = g0; g < g1; ++g) {
+      const double wr = Wd[2 * ((long)g * nmn + mn)], wi = Wd[2 * ((long)g * nmn + mn) + 1];
+      const double vr = V[2 * ((long)g * len + e)], vi = V[2 * ((long)g * len + e) + 1];
+      ar += wr * vr - wi * vi;
+      ai += wr * vi + wi * vr;
+    }
+    const long o = ((long)c * nmn + mn) * len + e;
+    part[2 * o] = ar;
+    part[2 * o + 1] = ai;
+  }
+}
+
+__global__ void wreduce_final_kernel(int nchunk, int len, int nmn, const double* __restrict__ part,
+                                     double* __restrict__ out) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= (long)len * nmn) return;
+  double ar = 0.0, ai = 0.0;
+  for (int c = 0; c < nchunk; ++c) {
+    ar += part[2 * ((long)c * nmn * len + i)];
+    ai += part[2 * ((long)c * nmn * len + i) + 1];
+  }
+  out[2 * i] = ar;
+  out[2 * i + 1] = ai;
+}
+
+// HS[mn] = sum_g Wd[g][mn] E[g],  HS[nmn + mn] = sum_g Wd[g][mn]; one workgroup.
+__global__ __launch_bounds__(256) void hs_reduce_kernel(int G, int nmn, const double* Wd, const double* E,
+                                                        double* HS) {
+  __shared__ double s[4][256];
+  for (int mn = 0; mn < nmn; ++mn) {
+    double hr = 0, hi = 0, sr = 0, si = 0;
+    for (int g = threadIdx.x; g < G; g += 256) {
+      const double wr = Wd[2 * ((long)g * nmn + mn)], wi = Wd[2 * ((long)g * nmn + mn) + 1];
+      const double er = E[2 * g], ei = E[2 * g + 1];
+      hr += wr * er - wi * ei;
+      hi += wr * ei + wi * er;
+      sr += wr;
+      si += wi;
+    }
+    s[0][threadIdx.x] = hr;
+    s[1][threadIdx.x] = hi;
+    s[2][threadIdx.x] = sr;
+    s[3][threadIdx.x] = si;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+      if (threadIdx.x < st)
+        for (int q = 0; q < 4; ++q) s[q][threadIdx.x] += s[q][threadIdx.x + st];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      HS[2 * mn] = s[0][0];
+      HS[2 * mn + 1] = s[1][0];
+      HS[2 * (nmn + mn)] = s[2][0];
+      HS[2 * (nmn + mn) + 1] = s[3][0];
+    }
+    __syncthreads();
+  }
+}
+
+// LocalG: g(p,q) = (sum_t c_t op_t(Grad_t) - E0 sum_t c_t op_t(Rdm_t)) / ovlp
+__global__ void localg_kernel(long n2, int nterm, const int* __restrict__ tmn, const int* __restrict__ tconj,
+                              const double* __restrict__ tcoef, const double* __restrict__ grad,
+                              const double* __restrict__ rdm, double e0r, double e0i, double ovr,
+                              double ovi, double* __restrict__ out) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= n2) return;
+  double gr = 0, gi = 0, rr = 0, ri = 0;
+  for (int t = 0; t < nterm; ++t) {
+    const long off = (long)tmn[t] * n2 + e;
+    double ar = grad[2 * off], ai = grad[2 * off + 1];
+    double br = rdm[2 * off], bi = rdm[2 * off + 1];
+    if (tconj[t]) { ai = -ai; bi = -bi; }
+    const double cr = tcoef[2 * t], ci = tcoef[2 * t + 1];
+    gr += cr * ar - ci * ai;
+    gi += cr * ai + ci * ar;
+    rr += cr * br - ci * bi;
+    ri += cr * bi + ci * br;
+  }
+  const double nr = gr - (e0r * rr - e0i * ri), ni = gi - (e0r * ri + e0i * rr);
+  const double d = ovr * ovr + ovi * ovi;
+  out[2 * e] = (nr * ovr + ni * ovi) / d;
+  out[2 * e + 1] = (ni * ovr - nr * ovi) / d;
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+
+// Wignerfac (PHFTools.f90:350-376)
+cplx wignerfac(int j, int n, int m, double alpha, double beta, double gama) {
+  const int smax = std::min(j + m, j - n), smin = std::max(0, m - n);
+  double fac = std::tgamma(j + n + 1.0) * std::tgamma(j - n + 1.0) * std::tgamma(j + m + 1.0) *
+               std::tgamma(j - m + 1.0);
+  fac = std::sqrt(fac);
+  double d = 0.0;
+  for (int s = smin; s <= smax; ++s) {
+    const int p1 = n - m + s, p2 = 2 * j + m - n - 2 * s, p3 = n - m + 2 * s;
+    const double numer = ((p1 % 2) ? -1.0 : 1.0) * std::pow(std::cos(beta / 2), p2) * std::pow(std::sin(beta / 2), p3);
+    const double denom = std::tgamma(j + m - s + 1.0) * std::tgamma(s + 1.0) * std::tgamma(n - m + s + 1.0) *
+                         std::tgamma(j - n - s + 1.0);
+    d += numer / denom;
+  }
+  d *= fac;
+  return std::exp(cplx(0.0, -alpha * n)) * d * std::exp(cplx(0.0, -gama * m));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" int phf_ctx_create(int device, phf_ctx** out) {
+  if (!out) {
+    g_create_err = "phf_ctx_create: out is NULL";
+    return PHF_ERR_ARG;
+  }
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) {
+    g_create_err = std::string("phf_ctx_create: no HIP device (") + hipGetErrorString(e) + ")";
+    return PHF_ERR_HIP;
+  }
+  if (device < 0 || device >= ndev) {
+    g_create_err = "phf_ctx_create: device index out of range";
+    return PHF_ERR_ARG;
+  }
+  e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    g_create_err = std::string("hipSetDevice: ") + hipGetErrorString(e);
+    return PHF_ERR_HIP;
+  }
+  phf_ctx* ctx = new phf_ctx_s();
+  ctx->device = device;
+  e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    g_create_err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+    delete ctx;
+    return PHF_ERR_HIP;
+  }
+  *out = ctx;
+  return PHF_OK;
+}
+
+extern "C" int phf_ctx_destroy(phf_ctx* ctx) {
+  if (!ctx) return PHF_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->dA) (void)hipFree(ctx->dA);
+  if (ctx->dH1) (void)hipFree(ctx->dH1);
+  if (ctx->ws) (void)hipFree(ctx->ws);
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return PHF_OK;
+}
+
+extern "C" const char* phf_last_error(const phf_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : g_create_err.c_str();
+}
+
+static int ensure_ws(phf_ctx* ctx, size_t bytes) {
+  if (ctx->ws_size >= bytes) return PHF_OK;
+  if (ctx->ws) {
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    HIPCK(hipFree(ctx->ws));
+    ctx->ws = nullptr;
+    ctx->ws_size = 0;
+  }
+  HIPCK(hipMalloc(&ctx->ws, bytes));
+  ctx->ws_size = bytes;
+  return PHF_OK;
+}
+
+extern "C" int phf_set_hamiltonian(phf_ctx* ctx, const double* hone, const double* htwo, int nso, int flags) {
+  if (!ctx) return PHF_ERR_ARG;
+  if (!hone || !htwo || nso <= 0 || nso > 512)
+    return fail(ctx, PHF_ERR_ARG, "phf_set_hamiltonian: bad arguments");
+  HIPCK(hipSetDevice(ctx->device));
+  ctx->has_ham = false;
+  if (ctx->dA) { HIPCK(hipFree(ctx->dA)); ctx->dA = nullptr; }
+  if (ctx->dH1) { HIPCK(hipFree(ctx->dH1)); ctx->dH1 = nullptr; }
+  const long n2 = (long)nso * nso;
+  const long n4 = n2 * n2;
+  HIPCK(hipMalloc(&ctx->dH1, sizeof(double) * 2 * n2));
+  HIPCK(hipMemcpyAsync(ctx->dH1, hone, sizeof(double) * 2 * n2, hipMemcpyHostToDevice, ctx->stream));
+
+  double* stage = nullptr;
+  int* dflag = nullptr;
+  HIPCK(hipMalloc(&stage, sizeof(double) * 2 * n4));
+  HIPCK(hipMalloc(&dflag, sizeof(int)));
+  HIPCK(hipMemsetAsync(dflag, 0, sizeof(int), ctx->stream));
+  HIPCK(hipMemcpyAsync(stage, htwo, sizeof(double) * 2 * n4, hipMemcpyHostToDevice, ctx->stream));
+  imag_nonzero_kernel<<<4096, 256, 0, ctx->stream>>>(stage, n4, dflag);
+  int hflag = 0;
+  HIPCK(hipMemcpyAsync(&hflag, dflag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCK(hipStreamSynchronize(ctx->stream));
+  ctx->cplx = hflag != 0 || (flags & 1);
+  ctx->nso = nso;
+  ctx->kpad = (int)round_up(n2, 16);
+  ctx->nrpad = (int)round_up(n2, 128);
+  ctx->kt = ctx->cplx ? 2 * ctx->kpad : ctx->kpad;
+  const long asz = (long)ctx->nrpad * ctx->kt;
+  hipError_t e = hipMalloc(&ctx->dA, sizeof(double) * asz);
+  if (e != hipSuccess) {
+    (void)hipFree(stage);
+    (void)hipFree(dflag);
+    ctx->dA = nullptr;
+    return fail(ctx, PHF_ERR_NOMEM, std::string("phf_set_hamiltonian: ") + hipGetErrorString(e));
+  }
+  ham_permute_kernel<<<65536, 256, 0, ctx->stream>>>(stage, ctx->dA, nso, ctx->kpad, ctx->kt, ctx->nrpad);
+  HIPCK(hipGetLastError());
+  HIPCK(hipStreamSynchronize(ctx->stream));
+  HIPCK(hipFree(stage));
+  HIPCK(hipFree(dflag));
+  ctx->has_ham = true;
+  return PHF_OK;
+}
+
+extern "C" int phf_buildhsg(phf_ctx* ctx, const double* ref, const double* z, const int* ngrid, int nci,
+                            int cmplxconj, int ncisp, int nocc, const double* r1, int nr1, const double* r2,
+                            int nr2, const double* rpg, int npg, const double* weightsp,
+                            const double* weightpg, int ncipg, const double* roota, int na,
+                            const double* rootb, int nb, const double* rooty, int ny_, int qj, int sp,
+                            int precision, double* hmat, double* smat, double* gradmat, double* rdmmat) {
+  if (!ctx) return PHF_ERR_ARG;
+  if (!ctx->has_ham) return fail(ctx, PHF_ERR_NO_HAMILTONIAN, "phf_buildhsg: no Hamiltonian set");
+  const int nso = ctx->nso;
+  if (!ref || !ngrid || !r1 || !r2 || !rpg || !weightsp || !weightpg || !hmat || !smat || !gradmat || !rdmmat)
+    return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: NULL pointer argument");
+  const int nl = ngrid[0], ny = ngrid[1];
+  if (nl <= 0 || ny <= 0 || npg <= 0 || nci <= 0 || ncisp <= 0 || ncipg <= 0 || qj < 0)
+    return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: non-positive grid or CI dimension");
+  if (nocc <= 0 || nocc > nso) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: nocc out of range");
+  if (nocc > lu_inverse_max_n())
+    return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: nocc above the LDS inverse limit");
+  if (nocc < nso && !z) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: z is NULL");
+  if (nl > nr1 || ny > nr2) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: ngrid exceeds R1/R2 extent");
+  if (cmplxconj != 0 && cmplxconj != 1)
+    return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: CmplxConj=2 is not implemented on the GPU path");
+  if (precision != PHF_PREC_FP64) return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: unknown precision");
+  if (qj + qj + (ncipg - 1) * ncisp >= nci)
+    return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: nci too small for QJ/ncisp/ncipg");
+  if (qj > 0) {
+    const bool ok = (sp == 2) ? (nl <= na && ny <= nb && nl <= ny_) : (nl <= na && nl <= nb && ny <= ny_);
+    if (!roota || !rootb || !rooty || !ok)
+      return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: Wigner angle arrays too short");
+  }
+  HIPCK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+
+  const int N = nocc;
+  const long G = (long)nl * ny * npg;
+  const int nmn = nci * nci;
+  const long n2 = (long)nso * nso;
+
+  // ---- host precompute: C0 = [I; Z] (column-major) and the grid weight table
+  std::vector<double> c0(2 * (size_t)nso * N, 0.0);
+  for (int a = 0; a < N; ++a) c0[2 * ((size_t)a + (size_t)a * nso)] = 1.0;
+  for (int i = N; i < nso; ++i)
+    for (int a = 0; a < N; ++a) {
+      const size_t src = 2 * ((size_t)(i - N) * N + a);
+      c0[2 * ((size_t)i + (size_t)a * nso)] = z[src];
+      c0[2 * ((size_t)i + (size_t)a * nso) + 1] = (cmplxconj == 0) ? 0.0 : z[src + 1];
+    }
+  std::vector<double> wtab(2 * (size_t)G * nmn, 0.0);
+  for (int il = 0; il < nl; ++il)
+    for (int iy = 0; iy < ny; ++iy)
+      for (int ipg = 0; ipg < npg; ++ipg) {
+        const long g = ((long)il * ny + iy) * npg + ipg;
+        const double w = weightsp[(long)il * ny + iy];
+        for (int ni = -qj; ni <= qj; ++ni)
+          for (int nj = -qj; nj <= qj; ++nj) {
+            cplx wig(1.0, 0.0);
+            if (qj > 0)
+              wig = (sp == 2) ? wignerfac(qj, ni, nj, roota[il], rootb[iy], rooty[il])
+                              : wignerfac(qj, ni, nj, roota[il], rootb[il], rooty[iy]);
+            for (int p = 0; p < ncipg; ++p)
+              for (int q = 0; q < ncipg; ++q) {
+                const cplx ww = weightpg[((long)ipg * ncipg + p) * ncipg + q] * wig;
+                const int m = ni + qj + p * ncisp, n = nj + qj + q * ncisp;
+                const size_t o = 2 * ((size_t)g * nmn + (size_t)m * nci + n);
+                const cplx v = w * ww;
+                wtab[o] += v.real();
+                wtab[o + 1] += v.imag();
+              }
+          }
+      }
+
+  // ---- workspace layout
+  const int bm = eri_gemm_f64_bm((int)(2 * G));
+  const long Mp = round_up(2 * G, bm);
+  const int kt = ctx->kt, nrpad = ctx->nrpad;
+  const int gchunk = 32;
+  const int nchunk = (int)((G + gchunk - 1) / gchunk);
+  const long lenX = (long)nso * N;
+  Arena A0{nullptr};
+  auto carve = [&](Arena& a, auto&& sink) {
+    sink(a.template take<double>(2 * n2), "ref");
+    sink(a.template take<double>(2 * lenX), "c0");
+    sink(a.template take<double>(2 * (size_t)nl * n2), "r1");
+    sink(a.template take<double>(2 * (size_t)ny * n2), "r2");
+    sink(a.template take<double>(2 * (size_t)npg * n2), "rpg");
+    sink(a.template take<double>(2 * (size_t)G * nmn), "w");
+    sink(a.template take<double>(2 * lenX), "w0");
+    sink(a.template take<double>(2 * lenX * npg), "wpg");
+    sink(a.template take<double>(2 * lenX * npg * ny), "w2");
+    sink(a.template take<double>(2 * (size_t)nl * n2), "r1p");
+    sink(a.template take<double>(2 * lenX * G), "t");
+    sink(a.template take<double>(2 * (size_t)N * N * G), "m");
+    sink(a.template take<double>(2 * (size_t)N * N * G), "minv");
+    sink(a.template take<double>(2 * (size_t)G), "det");
+    sink(a.template take<double>((size_t)G), "info");
+    sink(a.template take<double>(2 * lenX * G), "x");
+    sink(a.template take<double>((size_t)Mp * kt), "pt");
+    sink(a.template take<double>((size_t)Mp * nrpad), "gam");
+    sink(a.template take<double>(2 * lenX * G), "fx");
+    sink(a.template take<double>(2 * (size_t)N * N * G), "q");
+    sink(a.template take<double>(2 * lenX), "hc");
+    sink(a.template take<double>(2 * (size_t)G), "e");
+    sink(a.template take<double>(2 * lenX * G), "y");
+    sink(a.template take<double>(2 * (size_t)G * nmn), "wd");
+    sink(a.template take<double>(2 * (size_t)nchunk * nmn * lenX), "part");
+    sink(a.template take<double>(2 * (size_t)nmn * lenX), "ysum");
+    sink(a.template take<double>(2 * (size_t)nmn * lenX), "xsum");
+    sink(a.template take<double>(4 * (size_t)nmn), "hs");
+    sink(a.template take<double>(2 * (size_t)nmn * n2), "grad");
+    sink(a.template take<double>(2 * (size_t)nmn * n2), "rdm");
+  };
+  carve(A0, [](double*, const char*) {});
+  {
+    int rc = ensure_ws(ctx, A0.off + 256);
+    if (rc) return rc;
+  }
+  std::vector<double*> ptrs;
+  Arena A1{ctx->ws};
+  carve(A1, [&](double* p, const char*) { ptrs.push_back(p); });
+  int pi = 0;
+  double *dRef = ptrs[pi++], *dC0 = ptrs[pi++], *dR1 = ptrs[pi++], *dR2 = ptrs[pi++], *dRpg = ptrs[pi++];
+  double *dW = ptrs[pi++], *dW0 = ptrs[pi++], *dWpg = ptrs[pi++], *dW2 = ptrs[pi++], *dR1p = ptrs[pi++];
+  double *dT = ptrs[pi++], *dM = ptrs[pi++], *dMinv = ptrs[pi++], *dDet = ptrs[pi++];
+  int* dInfo = reinterpret_cast<int*>(ptrs[pi++]);
+  double *dX = ptrs[pi++], *dPt = ptrs[pi++], *dGam = ptrs[pi++], *dFX = ptrs[pi++], *dQ = ptrs[pi++];
+  double *dHc = ptrs[pi++], *dE = ptrs[pi++], *dY = ptrs[pi++], *dWd = ptrs[pi++], *dPart = ptrs[pi++];
+  double *dYsum = ptrs[pi++], *dXsum = ptrs[pi++], *dHS = ptrs[pi++], *dGrad = ptrs[pi++], *dRdm = ptrs[pi++];
+
+  // ---- uploads
+  auto up = [&](double* d, const double* h, size_t nd) {
+    return hipMemcpyAsync(d, h, nd * sizeof(double), hipMemcpyHostToDevice, s);
+  };
+  HIPCK(up(dRef, ref, 2 * n2));
+  HIPCK(up(dC0, c0.data(), c0.size()));
+  HIPCK(up(dR1, r1, 2 * (size_t)nl * n2));
+  HIPCK(up(dR2, r2, 2 * (size_t)ny * n2));
+  HIPCK(up(dRpg, rpg, 2 * (size_t)npg * n2));
+  HIPCK(up(dW, wtab.data(), wtab.size()));
+
+  if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[0], s));
+  const long sT = 2 * lenX;          // batch stride of NSO x N column-major blocks
+  const long sNN = 2L * N * N;
+  // K1: rotation chain, right to left
+  // W0 = Ref C0     (Ref row-major host layout)
+  zgemm_batched(s, nso, N, nso, 1, 0, ZMat{dRef, 2L * nso, 2, 0, 1, 0}, zmat_colmajor(dC0, nso, 0), 0, 0,
+                zout_colmajor(dW0, nso, 0), 1);
+  // Wpg[ipg] = Rpg[ipg] W0
+  zgemm_batched(s, nso, N, nso, 1, 0, ZMat{dRpg, 2L * nso, 2, 2 * n2, 1, 0}, zmat_colmajor(dW0, nso, 0), 0, 0,
+                zout_colmajor(dWpg, nso, sT), npg);
+  // W2[iy] = R2[iy] [Wpg_0 .. Wpg_{npg-1}]
+  zgemm_batched(s, nso, N * npg, nso, 1, 0, ZMat{dR2, 2L * nso, 2, 2 * n2, 1, 0},
+                zmat_colmajor(dWpg, nso, 0), 0, 0, zout_colmajor(dW2, nso, sT * npg), ny);
+  // R1p[il] = Ref^H R1[il]
+  zgemm_batched(s, nso, nso, nso, 1, 0, ZMat{dRef, 2, 2L * nso, 0, 1, 1}, ZMat{dR1, 2L * nso, 2, 2 * n2, 1, 0}, 0,
+                0, zout_colmajor(dR1p, nso, 2 * n2), nl);
+  // T[il] = R1p[il] W2   -> T_g = columns g*N .. g*N+N-1
+  zgemm_batched(s, nso, N * npg * ny, nso, 1, 0, zmat_colmajor(dR1p, nso, 2 * n2), zmat_colmajor(dW2, nso, 0), 0, 0,
+                zout_colmajor(dT, nso, sT * npg * ny), nl);
+  // M_g = C0^H T_g
+  zgemm_batched(s, N, N, nso, 1, 0, ZMat{dC0, 2L * nso, 2, 0, 1, 1}, zmat_colmajor(dT, nso, sT), 0, 0,
+                zout_colmajor(dM, N, sNN), (int)G);
+  // K2: inverse and determinant
+  lu_inverse_batched(s, N, dM, sNN, dMinv, sNN, dDet, dInfo, (int)G);
+  // X_g = T_g M_g^{-1}
+  zgemm_batched(s, nso, N, N, 1, 0, zmat_colmajor(dT, nso, sT), zmat_colmajor(dMinv, N, sNN), 0, 0,
+                zout_colmajor(dX, nso, sT), (int)G);
+  // density pack: Pt row 2g = Re rho_g^T, row 2g+1 = Im rho_g^T (flattened j*nso + l)
+  fill_zero(s, dPt, Mp * kt);
+  zgemm_batched(s, nso, nso, N, 1, 0, ZMat{dC0, 2, 2L * nso, 0, 1, 1}, ZMat{dX, 2L * nso, 2, sT, 1, 0}, 0, 0,
+                ZOut{dPt, nso, 1, 2L * kt, kt}, (int)G);
+  if (ctx->cplx) {
+    dim3 grid((unsigned)std::min<long>((n2 + 255) / 256, 1024), (unsigned)G);
+    pack_cplx_kernel<<<grid, 256, 0, s>>>(dPt, (int)G, (int)n2, ctx->kpad, kt);
+  }
+  // K3: Gamma for all grid points
+  if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[1], s));
+  eri_gemm_f64(s, dPt, ctx->dA, dGam, (int)Mp, nrpad, kt, nrpad, bm);
+  if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[2], s));
+  // K4: FX = h X + Gamma X
+  zgemm_batched(s, nso, N, nso, 1, 0, ZMat{ctx->dH1, 2L * nso, 2, 0, 1, 0}, zmat_colmajor(dX, nso, sT), 0, 0,
+                zout_colmajor(dFX, nso, sT), (int)G);
+  zgemm_batched(s, nso, N, nso, 1, 0, ZMat{dGam, nso, 1, 2L * nrpad, nrpad, 0}, zmat_colmajor(dX, nso, sT), 1, 0,
+                zout_colmajor(dFX, nso, sT), (int)G);
+  // Q_g = C0^H FX_g ; Hc = C0^H h
+  zgemm_batched(s, N, N, nso, 1, 0, ZMat{dC0, 2L * nso, 2, 0, 1, 1}, zmat_colmajor(dFX, nso, sT), 0, 0,
+                zout_colmajor(dQ, N, sNN), (int)G);
+  zgemm_batched(s, N, nso, nso, 1, 0, ZMat{dC0, 2L * nso, 2, 0, 1, 1}, ZMat{ctx->dH1, 2L * nso, 2, 0, 1, 0}, 0, 0,
+                zout_colmajor(dHc, N, 0), 1);
+  energy_kernel<<<(unsigned)G, 256, 0, s>>>(nso, N, dHc, dX, dQ, dE);
+  // Y_g = FX_g + e_g X_g - X_g Q_g
+  yinit_kernel<<<(unsigned)std::min<long>((lenX * G + 255) / 256, 65536), 256, 0, s>>>(lenX, (int)G, dFX, dX, dE, dY);
+  zgemm_batched(s, nso, N, N, -1, 0, zmat_colmajor(dX, nso, sT), zmat_colmajor(dQ, N, sNN), 1, 0,
+                zout_colmajor(dY, nso, sT), (int)G);
+  // K5: weighted grid sums
+  wd_kernel<<<(unsigned)((G * nmn + 255) / 256), 256, 0, s>>>((int)G, nmn, dW, dDet, dWd);
+  hs_reduce_kernel<<<1, 256, 0, s>>>((int)G, nmn, dWd, dE, dHS);
+  {
+    dim3 grid((unsigned)((lenX + 255) / 256), (unsigned)nchunk);
+    const unsigned fin = (unsigned)((lenX * nmn + 255) / 256);
+    wreduce_partial_kernel<<<grid, 256, 0, s>>>((int)G, gchunk, (int)lenX, nmn, dWd, dY, dPart);
+    wreduce_final_kernel<<<fin, 256, 0, s>>>(nchunk, (int)lenX, nmn, dPart, dYsum);
+    wreduce_partial_kernel<<<grid, 256, 0, s>>>((int)G, gchunk, (int)lenX, nmn, dWd, dX, dPart);
+    wreduce_final_kernel<<<fin, 256, 0, s>>>(nchunk, (int)lenX, nmn, dPart, dXsum);
+  }
+  // Gradmat[mn] = Ysum[mn] C0^H, Rdmmat[mn] = Xsum[mn] C0^H (C-order [nso][nso])
+  zgemm_batched(s, nso, nso, N, 1, 0, zmat_colmajor(dYsum, nso, sT), ZMat{dC0, 2L * nso, 2, 0, 1, 1}, 0, 0,
+                ZOut{dGrad, 2L * nso, 2, 2 * n2, 1}, nmn);
+  zgemm_batched(s, nso, nso, N, 1, 0, zmat_colmajor(dXsum, nso, sT), ZMat{dC0, 2L * nso, 2, 0, 1, 1}, 0, 0,
+                ZOut{dRdm, 2L * nso, 2, 2 * n2, 1}, nmn);
+  HIPCK(hipGetLastError());
+  if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[3], s));
+
+  std::vector<double> hs(4 * (size_t)nmn);
+  std::vector<int> info((size_t)G);
+  HIPCK(hipMemcpyAsync(hs.data(), dHS, hs.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCK(hipMemcpyAsync(info.data(), dInfo, info.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCK(hipMemcpyAsync(gradmat, dGrad, 2 * (size_t)nmn * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCK(hipMemcpyAsync(rdmmat, dRdm, 2 * (size_t)nmn * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCK(hipStreamSynchronize(s));
+  if (ctx->profiling) {
+    float t01, t12, t23, t03;
+    HIPCK(hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]));
+    HIPCK(hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]));
+    HIPCK(hipEventElapsedTime(&t23, ctx->ev[2], ctx->ev[3]));
+    HIPCK(hipEventElapsedTime(&t03, ctx->ev[0], ctx->ev[3]));
+    ctx->stage_ms[0] = t03;   // all device work of the call
+    ctx->stage_ms[1] = t01;   // rotation, overlap, inverse, density pack
+    ctx->stage_ms[2] = t12;   // ERI GEMM
+    ctx->stage_ms[3] = t23;   // epilogue and grid sums
+    ctx->nstage = 4;
+  }
+  for (long g = 0; g < G; ++g)
+    if (info[g] != 0) {
+      const long il = g / ((long)ny * npg), iy = (g / npg) % ny, ipg = g % npg;
+      char buf[160];
+      snprintf(buf, sizeof buf, "Error in ZGetRF: singular overlap at grid point (il=%ld, iy=%ld, ipg=%ld)",
+               il + 1, iy + 1, ipg + 1);
+      return fail(ctx, PHF_ERR_SINGULAR, buf);
+    }
+  std::memcpy(hmat, hs.data(), 2 * (size_t)nmn * sizeof(double));
+  std::memcpy(smat, hs.data() + 2 * nmn, 2 * (size_t)nmn * sizeof(double));
+  // Hermiticity test (PHFTools.f90:195-200)
+  double w1 = 0, w2 = 0, hmax = 0, smax = 0;
+  for (int m = 0; m < nci; ++m)
+    for (int n = 0; n < nci; ++n) {
+      const size_t a = 2 * ((size_t)m * nci + n), b = 2 * ((size_t)n * nci + m);
+      w1 = std::max(w1, std::fabs(-hmat[b + 1] - hmat[a + 1]));
+      w2 = std::max(w2, std::fabs(-smat[b + 1] - smat[a + 1]));
+      hmax = std::max(hmax, std::hypot(hmat[a], hmat[a + 1]));
+      smax = std::max(smax, std::hypot(smat[a], smat[a + 1]));
+    }
+  const double tol = 1e-5;
+  if (w1 > hmax * tol || w2 > smax * tol) return fail(ctx, PHF_ERR_NOT_HERMITIAN, "Not Hermitian");
+  return PHF_OK;
+}
+
+extern "C" int phf_set_profiling(phf_ctx* ctx, int on) {
+  if (!ctx) return PHF_ERR_ARG;
+  HIPCK(hipSetDevice(ctx->device));
+  if (on && !ctx->ev[0])
+    for (auto& e : ctx->ev) HIPCK(hipEventCreate(&e));
+  ctx->profiling = on != 0;
+  ctx->nstage = 0;
+  return PHF_OK;
+}
+
+extern "C" int phf_stage_times(const phf_ctx* ctx, double* ms, int nmax) {
+  if (!ctx || !ms) return 0;
+  const int n = std::min(nmax, ctx->nstage);
+  for (int i = 0; i < n; ++i) ms[i] = ctx->stage_ms[i];
+  return n;
+}
+
+extern "C" int phf_localg(phf_ctx* ctx, const double* gradmat, const double* rdmmat, const double* smat,
+                          const double* e0, const double* fsp, int ncisp, const double* fpg, int ncipg,
+                          const double* fk, int nfk, int cmplxconj, int nso, int nci, double* g) {
+  if (!ctx) return PHF_ERR_ARG;
+  if (!gradmat || !rdmmat || !smat || !e0 || !fsp || !fpg || !fk || !g || nso <= 0 || nci <= 0 || ncisp <= 0 ||
+      ncipg <= 0 || nfk < 1)
+    return fail(ctx, PHF_ERR_ARG, "phf_localg: bad arguments");
+  const int d = ncisp * ncipg;
+  if (cmplxconj == 2 && (nfk < 2 || 2 * d > nci))
+    return fail(ctx, PHF_ERR_ARG, "phf_localg: CmplxConj=2 needs fk(2) and nci >= 2*ncisp*ncipg");
+  if (d > nci) return fail(ctx, PHF_ERR_ARG, "phf_localg: nci < ncisp*ncipg");
+  HIPCK(hipSetDevice(ctx->device));
+  auto C = [](const double* p, int i) { return cplx(p[2 * i], p[2 * i + 1]); };
+  std::vector<int> tmn, tconj;
+  std::vector<double> tcoef;
+  cplx ovlp(0, 0);
+  auto term = [&](int m, int n, cplx c, int cj) {
+    tmn.push_back(m * nci + n);
+    tconj.push_back(cj);
+    tcoef.push_back(c.real());
+    tcoef.push_back(c.imag());
+  };
+  const cplx fk0 = C(fk, 0), fk1 = nfk > 1 ? C(fk, 1) : cplx(0, 0);
+  for (int ni = 0; ni < ncisp; ++ni)
+    for (int nj = 0; nj < ncisp; ++nj)
+      for (int p = 0; p < ncipg; ++p)
+        for (int q = 0; q < ncipg; ++q) {
+          const int m = ni + p * ncisp, n = nj + q * ncisp;
+          const cplx fac = std::conj(C(fsp, ni)) * C(fsp, nj) * std::conj(C(fpg, p)) * C(fpg, q);
+          const cplx f11 = fac * std::conj(fk0) * fk0;
+          term(m, n, f11, 0);
+          ovlp += f11 * C(smat, m * nci + n);
+          if (cmplxconj == 2) {
+            term(m + d, n + d, fac * std::conj(fk1) * fk1, 1);
+            ovlp += fac * std::conj(fk1) * fk1 * C(smat, (m + d) * nci + n + d);
+            term(m, n + d, fac * std::conj(fk0) * fk1, 0);
+            ovlp += fac * std::conj(fk0) * fk1 * C(smat, m * nci + n + d);
+            term(m + d, n, fac * std::conj(fk0) * fk1, 1);
+            ovlp += fac * std::conj(fk1) * fk0 * C(smat, (m + d) * nci + n);
+          }
+        }
+  const long n2 = (long)nso * nso;
+  const int nterm = (int)tmn.size();
+  const size_t mats = 2 * (size_t)nci * nci * n2;
+  Arena a0{nullptr};
+  a0.take<double>(mats); a0.take<double>(mats); a0.take<double>(2 * n2);
+  a0.take<int>(nterm); a0.take<int>(nterm); a0.take<double>(2 * nterm);
+  int rc = ensure_ws(ctx, a0.off + 256);
+  if (rc) return rc;
+  Arena a1{ctx->ws};
+  double* dG = a1.take<double>(mats);
+  double* dR = a1.take<double>(mats);
+  double* dOut = a1.take<double>(2 * n2);
+  int* dmn = a1.take<int>(nterm);
+  int* dcj = a1.take<int>(nterm);
+  double* dcf = a1.take<double>(2 * nterm);
+  hipStream_t s = ctx->stream;
+  HIPCK(hipMemcpyAsync(dG, gradmat, mats * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCK(hipMemcpyAsync(dR, rdmmat, mats * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCK(hipMemcpyAsync(dmn, tmn.data(), nterm * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCK(hipMemcpyAsync(dcj, tconj.data(), nterm * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCK(hipMemcpyAsync(dcf, tcoef.data(), 2 * nterm * sizeof(double), hipMemcpyHostToDevice, s));
+  localg_kernel<<<(unsigned)((n2 + 255) / 256), 256, 0, s>>>(n2, nterm, dmn, dcj, dcf, dG, dR, e0[0], e0[1],
+                                                             ovlp.real(), ovlp.imag(), dOut);
+  HIPCK(hipGetLastError());
+  HIPCK(hipMemcpyAsync(g, dOut, 2 * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCK(hipStreamSynchronize(s));
+  return PHF_OK;
+}

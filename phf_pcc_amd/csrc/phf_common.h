@@ -1,0 +1,48 @@
+// Shared definitions for the spin-projection grid kernels (gfx950 / CDNA4).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+#define PHF_WAVE 64
+
+// Strided complex operand: element (i, j) of batch b has its real part at
+// p[b*bs + i*rs + j*cs] and its imaginary part im doubles further on
+// (im = 1 for interleaved complex128, im = plane size for split planes).
+struct ZMat {
+  const double* p;
+  long rs, cs, bs, im;
+  int conj;
+};
+
+struct ZOut {
+  double* p;
+  long rs, cs, bs, im;
+};
+
+// Host-side helpers for interleaved complex128 storage.
+inline ZMat zmat_colmajor(const double* p, long ld, long bs, int conj = 0) {
+  return ZMat{p, 2, 2 * ld, bs, 1, conj};
+}
+// op(X) = X^T of a column-major X
+inline ZMat zmat_colmajor_t(const double* p, long ld, long bs, int conj = 0) {
+  return ZMat{p, 2 * ld, 2, bs, 1, conj};
+}
+inline ZOut zout_colmajor(double* p, long ld, long bs) { return ZOut{p, 2, 2 * ld, bs, 1}; }
+
+// Launch batched C = alpha * op(A) op(B) + beta * C (complex FP64) on stream s.
+// m x k times k x n; `batch` problems (blockIdx.z).
+void zgemm_batched(hipStream_t s, int m, int n, int k, double ar, double ai, ZMat A, ZMat B,
+                   double br, double bi, ZOut C, int batch);
+
+// XCD-aware remap of a linear workgroup id: workgroups are dispatched
+// round-robin over the 8 XCDs; this gives each XCD a contiguous range of
+// tile ids so that tiles sharing an operand panel share that XCD's L2.
+__device__ inline int xcd_remap(int bid, int nwg) {
+  const int nx = 8;
+  int q = nwg / nx, r = nwg % nx;
+  int x = bid % nx, loc = bid / nx;
+  return (x < r) ? x * (q + 1) + loc : r * (q + 1) + (x - r) * q + loc;
+}

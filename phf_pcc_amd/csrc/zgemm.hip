@@ -1,0 +1,138 @@
+// Batched complex FP64 GEMM on the FP64 matrix cores (v_mfma_f64_16x16x4f64).
+//
+// C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b] for arbitrary element
+// strides, so transposes, conjugate transposes, row-major host layouts and
+// split re/im planes are all expressed through ZMat/ZOut without copies.
+// Used for the O(NSO^2 N) per-grid-point stages: the rotation chain
+// (Ref^H R1 R2 Rpg Ref C0), the N x N overlap C0^H T, the solve T M^{-1}, the
+// transition density rho^T = C0* X^T, the Fock contraction (h + Gamma) X and
+// the final Y C0^H.
+//
+// Tile: 64 x 64 per 256-thread workgroup, 2 x 2 waves of 32 x 32, k-slab 16.
+// Complex product = four real MFMAs into separate re/im accumulators.
+#include "phf_common.h"
+
+namespace {
+
+constexpr int ZBM = 64, ZBN = 64, ZBK = 16, ZLD = ZBK + 2;
+
+__global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double ar, double ai,
+                                                    ZMat A, ZMat B, double br, double bi, ZOut C,
+                                                    int a_kfast, int b_kfast) {
+  __shared__ double As_re[ZBM * ZLD], As_im[ZBM * ZLD];
+  __shared__ double Bs_re[ZBN * ZLD], Bs_im[ZBN * ZLD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const long b = blockIdx.z;
+  const int m0 = blockIdx.x * ZBM, n0 = blockIdx.y * ZBN;
+  const double* Ap = A.p + b * A.bs;
+  const double* Bp = B.p + b * B.bs;
+
+  d4_t cre[2][2], cim[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      cre[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+      cim[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+    }
+
+  for (int k0 = 0; k0 < k; k0 += ZBK) {
+#pragma unroll
+    for (int e = 0; e < (ZBM * ZBK) / 256; ++e) {
+      const int idx = tid + 256 * e;
+      int i, kk;
+      if (a_kfast) { kk = idx % ZBK; i = idx / ZBK; }
+      else { i = idx % ZBM; kk = idx / ZBM; }
+      const int gi = m0 + i, gk = k0 + kk;
+      double re = 0.0, im = 0.0;
+      if (gi < m && gk < k) {
+        const long off = (long)gi * A.rs + (long)gk * A.cs;
+        re = Ap[off];
+        im = Ap[off + A.im];
+        if (A.conj) im = -im;
+      }
+      As_re[i * ZLD + kk] = re;
+      As_im[i * ZLD + kk] = im;
+    }
+#pragma unroll
+    for (int e = 0; e < (ZBN * ZBK) / 256; ++e) {
+      const int idx = tid + 256 * e;
+      int j, kk;
+      if (b_kfast) { kk = idx % ZBK; j = idx / ZBK; }
+      else { j = idx % ZBN; kk = idx / ZBN; }
+      const int gj = n0 + j, gk = k0 + kk;
+      double re = 0.0, im = 0.0;
+      if (gj < n && gk < k) {
+        const long off = (long)gk * B.rs + (long)gj * B.cs;
+        re = Bp[off];
+        im = Bp[off + B.im];
+        if (B.conj) im = -im;
+      }
+      Bs_re[j * ZLD + kk] = re;
+      Bs_im[j * ZLD + kk] = im;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < ZBK; ks += 4) {
+      double are[2], aim[2], bre[2], bim[2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int row = wm * 32 + t * 16 + (lane & 15);
+        are[t] = As_re[row * ZLD + ks + (lane >> 4)];
+        aim[t] = As_im[row * ZLD + ks + (lane >> 4)];
+        const int col = wn * 32 + t * 16 + (lane & 15);
+        bre[t] = Bs_re[col * ZLD + ks + (lane >> 4)];
+        bim[t] = Bs_im[col * ZLD + ks + (lane >> 4)];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          cre[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bre[j], cre[i][j], 0, 0, 0);
+          cre[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-aim[i], bim[j], cre[i][j], 0, 0, 0);
+          cim[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bim[j], cim[i][j], 0, 0, 0);
+          cim[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bre[j], cim[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+
+  const bool use_beta = (br != 0.0 || bi != 0.0);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gi = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
+        const int gj = n0 + wn * 32 + j * 16 + (lane & 15);
+        if (gi < m && gj < n) {
+          const double xr = cre[i][j][r], xi = cim[i][j][r];
+          double vr = ar * xr - ai * xi, vi = ar * xi + ai * xr;
+          const long off = b * C.bs + (long)gi * C.rs + (long)gj * C.cs;
+          if (use_beta) {
+            const double orr = C.p[off], oi = C.p[off + C.im];
+            vr += br * orr - bi * oi;
+            vi += br * oi + bi * orr;
+          }
+          C.p[off] = vr;
+          C.p[off + C.im] = vi;
+        }
+      }
+}
+
+}  // namespace
+
+void zgemm_batched(hipStream_t s, int m, int n, int k, double ar, double ai, ZMat A, ZMat B,
+                   double br, double bi, ZOut C, int batch) {
+  if (m <= 0 || n <= 0 || batch <= 0) return;
+  if (k <= 0) {
+    // C = beta * C handled by a k=0 product of zeros
+    k = 0;
+  }
+  const int a_kfast = (labs(A.cs) < labs(A.rs)) ? 1 : 0;
+  const int b_kfast = (labs(B.rs) < labs(B.cs)) ? 1 : 0;
+  dim3 grid((m + ZBM - 1) / ZBM, (n + ZBN - 1) / ZBN, batch);
+  zgemm_kernel<<<grid, 256, 0, s>>>(m, n, k, ar, ai, A, B, br, bi, C, a_kfast, b_kfast);
+}

@@ -1,0 +1,173 @@
+"""GPU parity of phftools.buildhsg / localg (HIP path, through the C ABI)
+against the CPU oracle on the same inputs.
+
+Tolerance: FP64 path, energies to 1e-10 Eh (the plain-form target), every
+matrix element of Hmat/Smat/Gradmat/Rdmmat to 1e-10 relative to its largest
+entry.
+"""
+import numpy as np
+import pytest
+
+from oracle import phf_oracle as O
+from tests._cases import grid_args, hubbard_cfg1, hubbard_general_ref, random_unitary, synthetic_dense
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def phftools():
+    from phf_pcc_amd.PHFTools_F import phftools as t
+    t.precision = "fp64"
+    return t
+
+
+def _run_both(phftools, case, ga, cmplxconj=1):
+    args = (case["hone"], case["htwo"], case["ref"], case["z"], ga["ngrid"], ga["nci"], cmplxconj, ga["ncisp"],
+            case["nocc"], ga["r1"], ga["r2"], ga["rpg"], ga["weightsp"], ga["weightpg"], ga["roota"], ga["rootb"],
+            ga["rooty"], ga["qj"], ga["sp"], 0)
+    gpu = phftools.buildhsg(*args)
+    cpu = O.buildhsg(case["hone"], case["htwo"], case["ref"], case["z"], cmplxconj=cmplxconj, nocc=case["nocc"],
+                     **ga)
+    return gpu, cpu
+
+
+def _assert_close(gpu, cpu, tol=1e-10):
+    names = ["Hmat", "Smat", "Gradmat", "Rdmmat"]
+    for n, a, b in zip(names, gpu, cpu):
+        scale = max(np.abs(b).max(), 1e-300)
+        err = np.abs(a - b).max() / scale
+        assert err < tol, f"{n}: rel err {err:.3e}"
+
+
+def _energy_and_grad(phftools, gpu, cpu, nocc, cmplxconj=1):
+    out = []
+    for res, is_gpu in ((gpu, True), (cpu, False)):
+        h, s, gm, rm = res
+        ev, vec = O.eigen_solver(h, s)
+        fsp, fpg, fk = O.vec_decomp(vec[:, 0], h.shape[0], 1, 1)
+        if is_gpu:
+            g = phftools.localg(gm, rm, s, ev[0], fsp, fpg, fk, cmplxconj)
+        else:
+            g = O.localg(gm, rm, s, ev[0], fsp, fpg, fk, cmplxconj)
+        out.append((ev[0].real, g[nocc:, :nocc]))
+    return out
+
+
+def test_hubbard_ring_plain_fp64(phftools):
+    """Hubbard ring n=8, U=4, UHF + random Z, Lebedev 14 x 8 gamma (112 points)."""
+    case, nao = hubbard_cfg1()
+    ga = grid_args(nao, (14, 8), 0)
+    gpu, cpu = _run_both(phftools, case, ga)
+    _assert_close(gpu, cpu)
+    (eg, gg), (ec, gc) = _energy_and_grad(phftools, gpu, cpu, case["nocc"])
+    assert abs(eg - ec) < 1e-10
+    assert np.abs(gg - gc).max() < 1e-10
+
+
+@pytest.mark.parametrize("j,ngrid", [(1, (26, 8)), (2, (38, 8))])
+def test_wigner_blocks(phftools, j, ngrid):
+    case = hubbard_general_ref(4, 4.0, 100 + j, zscale=0.3)
+    ga = grid_args(4, ngrid, j)
+    gpu, cpu = _run_both(phftools, case, ga)
+    _assert_close(gpu, cpu)
+
+
+def test_complex_integrals(phftools):
+    """General complex Ref: complex A (four-way re/im contraction)."""
+    case = hubbard_general_ref(6, 3.0, 7, zscale=0.2, complex_ref=True)
+    assert np.abs(case["htwo"].imag).max() > 1e-3
+    ga = grid_args(6, (14, 8), 0)
+    gpu, cpu = _run_both(phftools, case, ga)
+    _assert_close(gpu, cpu)
+
+
+def test_real_z_cmplxconj0(phftools):
+    case = hubbard_general_ref(4, 4.0, 9, zscale=0.3)
+    ga = grid_args(4, (14, 8), 0)
+    gpu, cpu = _run_both(phftools, case, ga, cmplxconj=0)
+    _assert_close(gpu, cpu)
+
+
+def test_point_group_and_sp2(phftools):
+    """npg = 2 (identity + a site permutation), ncipg = 2, and the SP=2 grid."""
+    case = hubbard_general_ref(4, 4.0, 13, zscale=0.3)
+    nso = 8
+    perm = np.roll(np.arange(4), 1)
+    p4 = np.eye(4)[perm]
+    rpg = np.stack([np.eye(nso), np.kron(np.eye(2), p4)]).astype(complex)
+    wpg = np.array([[[1.0, 0.5], [0.5, 1.0]], [[1.0, -0.5], [-0.5, 1.0]]])
+    ga = grid_args(4, (14, 8), 0, rpg=rpg, weightpg=wpg)
+    gpu, cpu = _run_both(phftools, case, ga)
+    _assert_close(gpu, cpu)
+    ga2 = grid_args(4, (1, 6), 0, sp=2)
+    gpu, cpu = _run_both(phftools, case, ga2)
+    _assert_close(gpu, cpu)
+
+
+def test_odd_sizes_and_padding(phftools):
+    """n=9 spatial orbitals (NSO=18, NSO^2 not a multiple of 16), N=8."""
+    case = synthetic_dense(9, 8, seed=77, zscale=0.1)
+    ga = grid_args(9, (6, 5), 0)
+    gpu, cpu = _run_both(phftools, case, ga)
+    _assert_close(gpu, cpu)
+
+
+def test_large_grid_tile_path(phftools):
+    """2G > 768 rows takes the 128-row ERI tile."""
+    case = hubbard_general_ref(4, 4.0, 17, zscale=0.2)
+    ga = grid_args(4, (50, 16), 0)
+    gpu, cpu = _run_both(phftools, case, ga)
+    _assert_close(gpu, cpu)
+
+
+def test_synthetic_n32(phftools):
+    """Synthetic factored Hamiltonian n=32 (NSO=64), N=16, 14 x 8 grid."""
+    case = synthetic_dense(32, 16, seed=1234)
+    ga = grid_args(32, (14, 8), 0)
+    gpu, cpu = _run_both(phftools, case, ga)
+    _assert_close(gpu, cpu)
+    (eg, gg), (ec, gc) = _energy_and_grad(phftools, gpu, cpu, case["nocc"])
+    assert abs(eg - ec) < 1e-10
+    assert np.abs(gg - gc).max() < 1e-10
+
+
+def test_singular_overlap_raises(phftools):
+    """A rotation that maps the occupied space onto virtuals makes M = 0 exactly;
+    the reference STOPs in InvertC, the drop-in raises a LinAlgError."""
+    from phf_pcc_amd._lib import SingularOverlapError
+    case = hubbard_general_ref(4, 4.0, 19, zscale=0.0)
+    nso = 8
+    swap = np.zeros((nso, nso))
+    ref = case["ref"]
+    # R_MO swapping occupied and virtual MOs, expressed in the AO basis
+    r_mo = np.zeros((nso, nso))
+    r_mo[4:, :4] = np.eye(4)
+    r_mo[:4, 4:] = np.eye(4)
+    r_ao = ref @ r_mo @ ref.conj().T
+    ga = grid_args(4, (1, 1), 0, sp=0)
+    ga["r1"] = r_ao[None].astype(complex)
+    with pytest.raises(SingularOverlapError):
+        _run_both(phftools, case, ga)
+
+
+def test_not_hermitian_raises(phftools):
+    from phf_pcc_amd._lib import NotHermitianError
+    from phf_pcc_amd.spin_proj import build_rot_mat
+    case = hubbard_general_ref(4, 4.0, 41, zscale=0.3)
+    ga = grid_args(4, (1, 1), 0, sp=1)
+    ga["r2"] = build_rot_mat(0.0, 0.0, 0.7, 4)[None]
+    args = (case["hone"], case["htwo"], case["ref"], case["z"], ga["ngrid"], ga["nci"], 1, ga["ncisp"],
+            case["nocc"], ga["r1"], ga["r2"], ga["rpg"], ga["weightsp"], ga["weightpg"], ga["roota"], ga["rootb"],
+            ga["rooty"], ga["qj"], ga["sp"], 0)
+    with pytest.raises(NotHermitianError):
+        phftools.buildhsg(*args)
+
+
+def test_repeated_calls_and_hamiltonian_switch(phftools):
+    """The device Hamiltonian cache follows the htwo argument."""
+    c1, nao = hubbard_cfg1()
+    c2 = hubbard_general_ref(8, 2.0, 5, zscale=0.1)
+    ga = grid_args(8, (6, 4), 0)
+    for case in (c1, c2, c1):
+        gpu, cpu = _run_both(phftools, case, ga)
+        _assert_close(gpu, cpu)
