@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256, 2) void eri_gemm_f64_kernel(const double* __re
     for (int j = 0; j < FN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const long row = (long)mt * BM + wm * WM + i * 16 + 4 * (lane >> 4) + r;
+        const long row = (long)mt * BM + wm * WM + i * 16 + (lane >> 4) + 4 * r;   // f64 16x16x4 output row
         const long col = (long)nt * BN + wn * WN + j * 16 + (lane & 15);
         C[row * ldc + col] = acc[i][j][r];
       }

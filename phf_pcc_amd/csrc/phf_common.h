@@ -8,6 +8,13 @@ typedef float f4_t __attribute__((ext_vector_type(4)));
 
 #define PHF_WAVE 64
 
+// MFMA operand/result lane layouts on gfx950 (checked on the device):
+//   f64 16x16x4  : A[i][k] lane i + 16k;  B[k][j] lane j + 16k;
+//                  D[i][j] lane j + 16*(i % 4), register i / 4
+//   f32 16x16x4, bf16 16x16x32, i8 16x16x64 : A[i][k] lane i + 16*(k / kpl),
+//                  element k % kpl (kpl = 1, 8, 16 values per lane);
+//                  D[i][j] lane j + 16*(i / 4), register i % 4
+
 // Strided complex operand: element (i, j) of batch b has its real part at
 // p[b*bs + i*rs + j*cs] and its imaginary part im doubles further on
 // (im = 1 for interleaved complex128, im = plane size for split planes).

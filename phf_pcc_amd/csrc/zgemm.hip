@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double 
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gi = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
+        const int gi = m0 + wm * 32 + i * 16 + (lane >> 4) + 4 * r;   // f64 16x16x4: row = lane/16 + 4r
         const int gj = n0 + wn * 32 + j * 16 + (lane & 15);
         if (gi < m && gj < n) {
           const double xr = cre[i][j][r], xi = cim[i][j][r];

@@ -64,10 +64,10 @@ def test_hubbard_ring_plain_fp64(phftools):
     assert np.abs(gg - gc).max() < 1e-10
 
 
-@pytest.mark.parametrize("j,ngrid", [(1, (26, 8)), (2, (38, 8))])
-def test_wigner_blocks(phftools, j, ngrid):
-    case = hubbard_general_ref(4, 4.0, 100 + j, zscale=0.3)
-    ga = grid_args(4, ngrid, j)
+@pytest.mark.parametrize("nsite,seed,j,ngrid", [(4, 101, 1, (26, 8)), (6, 3, 2, (38, 8))])
+def test_wigner_blocks(phftools, nsite, seed, j, ngrid):
+    case = hubbard_general_ref(nsite, 4.0, seed, zscale=0.3)
+    ga = grid_args(nsite, ngrid, j)
     gpu, cpu = _run_both(phftools, case, ga)
     _assert_close(gpu, cpu)
 
@@ -89,18 +89,21 @@ def test_real_z_cmplxconj0(phftools):
 
 
 def test_point_group_and_sp2(phftools):
-    """npg = 2 (identity + a site permutation), ncipg = 2, and the SP=2 grid."""
+    """npg = 2 (identity + translation by two sites, weights 1/2), and the SP=2
+    (beta-only) grid with an S_z-conserving Z."""
     case = hubbard_general_ref(4, 4.0, 13, zscale=0.3)
     nso = 8
-    perm = np.roll(np.arange(4), 1)
-    p4 = np.eye(4)[perm]
+    p4 = np.eye(4)[np.roll(np.arange(4), 2)]
     rpg = np.stack([np.eye(nso), np.kron(np.eye(2), p4)]).astype(complex)
-    wpg = np.array([[[1.0, 0.5], [0.5, 1.0]], [[1.0, -0.5], [-0.5, 1.0]]])
-    ga = grid_args(4, (14, 8), 0, rpg=rpg, weightpg=wpg)
+    ga = grid_args(4, (14, 8), 0, rpg=rpg, weightpg=np.full((2, 1, 1), 0.5))
     gpu, cpu = _run_both(phftools, case, ga)
     _assert_close(gpu, cpu)
+    z = case["z"].copy()
+    z[:2, 2:] = 0
+    z[2:, :2] = 0
+    case_sz = dict(case, z=z)
     ga2 = grid_args(4, (1, 6), 0, sp=2)
-    gpu, cpu = _run_both(phftools, case, ga2)
+    gpu, cpu = _run_both(phftools, case_sz, ga2)
     _assert_close(gpu, cpu)
 
 
@@ -132,20 +135,20 @@ def test_synthetic_n32(phftools):
 
 
 def test_singular_overlap_raises(phftools):
-    """A rotation that maps the occupied space onto virtuals makes M = 0 exactly;
-    the reference STOPs in InvertC, the drop-in raises a LinAlgError."""
+    """A rotation that maps the occupied MOs onto virtual MOs makes M = 0 exactly
+    (permutation Ref, Z = 0); the reference STOPs in InvertC (Wrappers.f90:311),
+    the drop-in raises a LinAlgError."""
     from phf_pcc_amd._lib import SingularOverlapError
-    case = hubbard_general_ref(4, 4.0, 19, zscale=0.0)
+    case = synthetic_dense(4, 4, seed=3, zscale=0.0)
     nso = 8
-    swap = np.zeros((nso, nso))
-    ref = case["ref"]
-    # R_MO swapping occupied and virtual MOs, expressed in the AO basis
+    ref = case["ref"].real
     r_mo = np.zeros((nso, nso))
     r_mo[4:, :4] = np.eye(4)
     r_mo[:4, 4:] = np.eye(4)
-    r_ao = ref @ r_mo @ ref.conj().T
     ga = grid_args(4, (1, 1), 0, sp=0)
-    ga["r1"] = r_ao[None].astype(complex)
+    ga["r1"] = (ref @ r_mo @ ref.T)[None].astype(complex)
+    with pytest.raises(O.OracleError):
+        O.buildhsg(case["hone"], case["htwo"], case["ref"], case["z"], cmplxconj=1, nocc=4, **ga)
     with pytest.raises(SingularOverlapError):
         _run_both(phftools, case, ga)
 
@@ -167,7 +170,7 @@ def test_repeated_calls_and_hamiltonian_switch(phftools):
     """The device Hamiltonian cache follows the htwo argument."""
     c1, nao = hubbard_cfg1()
     c2 = hubbard_general_ref(8, 2.0, 5, zscale=0.1)
-    ga = grid_args(8, (6, 4), 0)
+    ga = grid_args(8, (14, 8), 0)
     for case in (c1, c2, c1):
         gpu, cpu = _run_both(phftools, case, ga)
         _assert_close(gpu, cpu)
