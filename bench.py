@@ -54,7 +54,19 @@ def main():
     ap.add_argument("--nocc", type=int, default=32)
     ap.add_argument("--cpu-points", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run one device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        nbytes = args.copy_calibration << 20
+        a = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": nbytes, "read_bytes": nbytes, "write_bytes": nbytes}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the grid-point sharding over GPUs "
               "is a design in DESIGN.md, not a measured path", file=sys.stderr)
