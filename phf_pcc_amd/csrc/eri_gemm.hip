@@ -42,31 +42,44 @@ __global__ __launch_bounds__(256, 2) void eri_gemm_f64_kernel(const double* __re
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
 
-  double2 rp[PL], ra[AL];
-  auto gload = [&](int k0) {
-#pragma unroll
-    for (int e = 0; e < PL; ++e)
-      rp[e] = *reinterpret_cast<const double2*>(Pg + (long)(lr + 32 * e) * K + k0 + lc);
-#pragma unroll
-    for (int e = 0; e < AL; ++e)
-      ra[e] = *reinterpret_cast<const double2*>(Ag + (long)(lr + 32 * e) * K + k0 + lc);
-  };
-  auto lstore = [&](int buf) {
-#pragma unroll
-    for (int e = 0; e < PL; ++e)
-      *reinterpret_cast<double2*>(&sP[buf][(lr + 32 * e) * LDSW + lc]) = rp[e];
-#pragma unroll
-    for (int e = 0; e < AL; ++e)
-      *reinterpret_cast<double2*>(&sA[buf][(lr + 32 * e) * LDSW + lc]) = ra[e];
-  };
+  static_assert(PL <= 4 && AL <= 4, "staging registers cover at most 4 rows of 32");
+  // global -> registers (next k-slab) and registers -> LDS through named
+  // registers: an array here is left in scratch memory by the compiler
+  double2 rp0, rp1, rp2, rp3, ra0, ra1, ra2, ra3;
+#define ERI_LD(dst, base, e, k0) \
+  dst = *reinterpret_cast<const double2*>((base) + (long)(lr + 32 * (e)) * K + (k0) + lc)
+#define ERI_ST(arr, buf, e, src) \
+  *reinterpret_cast<double2*>(&arr[buf][(lr + 32 * (e)) * LDSW + lc]) = src
+#define ERI_GLOAD(k0)                                   \
+  do {                                                  \
+    ERI_LD(rp0, Pg, 0, k0);                             \
+    if constexpr (PL > 1) ERI_LD(rp1, Pg, 1, k0);       \
+    if constexpr (PL > 2) ERI_LD(rp2, Pg, 2, k0);       \
+    if constexpr (PL > 3) ERI_LD(rp3, Pg, 3, k0);       \
+    ERI_LD(ra0, Ag, 0, k0);                             \
+    if constexpr (AL > 1) ERI_LD(ra1, Ag, 1, k0);       \
+    if constexpr (AL > 2) ERI_LD(ra2, Ag, 2, k0);       \
+    if constexpr (AL > 3) ERI_LD(ra3, Ag, 3, k0);       \
+  } while (0)
+#define ERI_LSTORE(buf)                                 \
+  do {                                                  \
+    ERI_ST(sP, buf, 0, rp0);                            \
+    if constexpr (PL > 1) ERI_ST(sP, buf, 1, rp1);      \
+    if constexpr (PL > 2) ERI_ST(sP, buf, 2, rp2);      \
+    if constexpr (PL > 3) ERI_ST(sP, buf, 3, rp3);      \
+    ERI_ST(sA, buf, 0, ra0);                            \
+    if constexpr (AL > 1) ERI_ST(sA, buf, 1, ra1);      \
+    if constexpr (AL > 2) ERI_ST(sA, buf, 2, ra2);      \
+    if constexpr (AL > 3) ERI_ST(sA, buf, 3, ra3);      \
+  } while (0)
 
-  gload(0);
-  lstore(0);
+  ERI_GLOAD(0);
+  ERI_LSTORE(0);
   __syncthreads();
   const int nk = K / BK;
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
-    if (kt + 1 < nk) gload((kt + 1) * BK);
+    if (kt + 1 < nk) ERI_GLOAD((kt + 1) * BK);
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
       double a[FM], b[FN];
@@ -80,9 +93,13 @@ __global__ __launch_bounds__(256, 2) void eri_gemm_f64_kernel(const double* __re
         for (int j = 0; j < FN; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
-    if (kt + 1 < nk) lstore(cur ^ 1);
+    if (kt + 1 < nk) ERI_LSTORE(cur ^ 1);
     __syncthreads();
   }
+#undef ERI_GLOAD
+#undef ERI_LSTORE
+#undef ERI_LD
+#undef ERI_ST
 
 #pragma unroll
   for (int i = 0; i < FM; ++i)
