@@ -33,10 +33,19 @@ enum {
   PHF_ERR_NOMEM = 7           /* device allocation failed */
 };
 
-/* Precision of the ERI contraction (the overlap LU, determinant, densities
- * and the grid sum are always FP64). */
+/* Precision of the ERI contraction (the overlap LU, determinant, densities,
+ * the Fock contraction and the grid sum are always FP64).  The reduced forms
+ * run on the FP32 / BF16 / FP16 matrix cores with FP32 accumulation folded
+ * into FP64:
+ *   PHF_PREC_FP32    one product of the FP32 roundings of A and rho
+ *   PHF_PREC_BF16X3  A, rho split into three BF16 terms; six products
+ *   PHF_PREC_F16X2   A, rho scaled by powers of two per row and split into
+ *                    two FP16 terms; three products (hi*hi + hi*lo + lo*hi) */
 enum {
-  PHF_PREC_FP64 = 0
+  PHF_PREC_FP64 = 0,
+  PHF_PREC_FP32 = 1,
+  PHF_PREC_BF16X3 = 2,
+  PHF_PREC_F16X2 = 3
 };
 
 /* Context: one GPU, one HIP stream, a cached device Hamiltonian and a

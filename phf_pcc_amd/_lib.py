@@ -22,7 +22,7 @@ PHF_ERR_NO_HAMILTONIAN = 5
 PHF_ERR_UNSUPPORTED = 6
 PHF_ERR_NOMEM = 7
 
-PREC = {"fp64": 0}
+PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2, "f16x2": 3}
 
 # every symbol declared in include/phf_hip.h
 EXPORTS = ["phf_ctx_create", "phf_ctx_destroy", "phf_last_error", "phf_set_hamiltonian",

@@ -35,6 +35,12 @@ int lu_inverse_max_n();
 void eri_gemm_f64(hipStream_t s, const double* P, const double* A, double* C, int Mp, int Nr, int K,
                   long ldc, int bm);
 int eri_gemm_f64_bm(int rows);
+int mixed_segments(int prec);
+void split_operand(hipStream_t s, int prec, bool is_a, const double* src, long lds, int rows, int kt, void* dst,
+                   double* inv);
+void convert_f32(hipStream_t s, const double* src, float* dst, long n);
+void eri_gemm_mixed(hipStream_t s, int prec, const void* P, const void* A, const double* invP, const double* invA,
+                    double* C, int Mp, int Nr, int Kseg, long ldc);
 
 struct phf_ctx_s {
   int device = 0;
@@ -47,6 +53,10 @@ struct phf_ctx_s {
   int kpad = 0, nrpad = 0, kt = 0;
   double* dA = nullptr;     // [nrpad][kt] FP64
   double* dH1 = nullptr;    // hone, row-major complex128
+  // reduced-precision copies of A, built on first use per mode:
+  // [nrpad][nseg*kt] of FP32 / BF16 / FP16 and per-row inverse scales
+  void* dAm[4] = {nullptr, nullptr, nullptr, nullptr};
+  double* dAinv[4] = {nullptr, nullptr, nullptr, nullptr};
   // workspace arena
   char* ws = nullptr;
   size_t ws_size = 0;
@@ -360,6 +370,10 @@ extern "C" int phf_ctx_destroy(phf_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   if (ctx->dA) (void)hipFree(ctx->dA);
   if (ctx->dH1) (void)hipFree(ctx->dH1);
+  for (int q = 0; q < 4; ++q) {
+    if (ctx->dAm[q]) (void)hipFree(ctx->dAm[q]);
+    if (ctx->dAinv[q]) (void)hipFree(ctx->dAinv[q]);
+  }
   if (ctx->ws) (void)hipFree(ctx->ws);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
@@ -391,6 +405,11 @@ extern "C" int phf_set_hamiltonian(phf_ctx* ctx, const double* hone, const doubl
     return fail(ctx, PHF_ERR_ARG, "phf_set_hamiltonian: bad arguments");
   HIPCK(hipSetDevice(ctx->device));
   ctx->has_ham = false;
+  HIPCK(hipStreamSynchronize(ctx->stream));
+  for (int q = 0; q < 4; ++q) {
+    if (ctx->dAm[q]) { HIPCK(hipFree(ctx->dAm[q])); ctx->dAm[q] = nullptr; }
+    if (ctx->dAinv[q]) { HIPCK(hipFree(ctx->dAinv[q])); ctx->dAinv[q] = nullptr; }
+  }
   if (ctx->dA) { HIPCK(hipFree(ctx->dA)); ctx->dA = nullptr; }
   if (ctx->dH1) { HIPCK(hipFree(ctx->dH1)); ctx->dH1 = nullptr; }
   const long n2 = (long)nso * nso;
@@ -410,7 +429,7 @@ extern "C" int phf_set_hamiltonian(phf_ctx* ctx, const double* hone, const doubl
   HIPCK(hipStreamSynchronize(ctx->stream));
   ctx->cplx = hflag != 0 || (flags & 1);
   ctx->nso = nso;
-  ctx->kpad = (int)round_up(n2, 16);
+  ctx->kpad = (int)round_up(n2, 64);
   ctx->nrpad = (int)round_up(n2, 128);
   ctx->kt = ctx->cplx ? 2 * ctx->kpad : ctx->kpad;
   const long asz = (long)ctx->nrpad * ctx->kt;
@@ -427,6 +446,23 @@ extern "C" int phf_set_hamiltonian(phf_ctx* ctx, const double* hone, const doubl
   HIPCK(hipFree(stage));
   HIPCK(hipFree(dflag));
   ctx->has_ham = true;
+  return PHF_OK;
+}
+
+// Reduced-precision copy of the cached A for mode prec (1 FP32, 2 BF16X3, 3 F16X2).
+static int ensure_mixed_a(phf_ctx* ctx, int prec) {
+  if (ctx->dAm[prec]) return PHF_OK;
+  const int nseg = mixed_segments(prec);
+  const long elems = (long)ctx->nrpad * ctx->kt * nseg;
+  const size_t esz = (prec == 1) ? sizeof(float) : 2;
+  HIPCK(hipMalloc(&ctx->dAm[prec], elems * esz));
+  if (prec == 1) {
+    convert_f32(ctx->stream, ctx->dA, (float*)ctx->dAm[prec], (long)ctx->nrpad * ctx->kt);
+  } else {
+    HIPCK(hipMalloc(&ctx->dAinv[prec], sizeof(double) * ctx->nrpad));
+    split_operand(ctx->stream, prec, true, ctx->dA, ctx->kt, ctx->nrpad, ctx->kt, ctx->dAm[prec], ctx->dAinv[prec]);
+  }
+  HIPCK(hipGetLastError());
   return PHF_OK;
 }
 
@@ -451,7 +487,8 @@ extern "C" int phf_buildhsg(phf_ctx* ctx, const double* ref, const double* z, co
   if (nl > nr1 || ny > nr2) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: ngrid exceeds R1/R2 extent");
   if (cmplxconj != 0 && cmplxconj != 1)
     return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: CmplxConj=2 is not implemented on the GPU path");
-  if (precision != PHF_PREC_FP64) return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: unknown precision");
+  if (precision < PHF_PREC_FP64 || precision > PHF_PREC_F16X2)
+    return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: unknown precision");
   if (qj + qj + (ncipg - 1) * ncisp >= nci)
     return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: nci too small for QJ/ncisp/ncipg");
   if (qj > 0) {
@@ -501,7 +538,7 @@ extern "C" int phf_buildhsg(phf_ctx* ctx, const double* ref, const double* z, co
       }
 
   // ---- workspace layout
-  const int bm = eri_gemm_f64_bm((int)(2 * G));
+  const int bm = (precision == PHF_PREC_FP64) ? eri_gemm_f64_bm((int)(2 * G)) : 128;
   const long Mp = round_up(2 * G, bm);
   const int kt = ctx->kt, nrpad = ctx->nrpad;
   const int gchunk = 32;
@@ -539,6 +576,10 @@ extern "C" int phf_buildhsg(phf_ctx* ctx, const double* ref, const double* z, co
     sink(a.template take<double>(4 * (size_t)nmn), "hs");
     sink(a.template take<double>(2 * (size_t)nmn * n2), "grad");
     sink(a.template take<double>(2 * (size_t)nmn * n2), "rdm");
+    // reduced-precision density rows and their inverse scales
+    const size_t pm = (precision == PHF_PREC_FP64) ? 0 : (size_t)Mp * kt * mixed_segments(precision);
+    sink(a.template take<double>(precision == PHF_PREC_FP32 ? (pm + 1) / 2 : (pm + 3) / 4), "pm");
+    sink(a.template take<double>(precision == PHF_PREC_FP64 ? 0 : (size_t)Mp), "pinv");
   };
   carve(A0, [](double*, const char*) {});
   {
@@ -556,6 +597,12 @@ extern "C" int phf_buildhsg(phf_ctx* ctx, const double* ref, const double* z, co
   double *dX = ptrs[pi++], *dPt = ptrs[pi++], *dGam = ptrs[pi++], *dFX = ptrs[pi++], *dQ = ptrs[pi++];
   double *dHc = ptrs[pi++], *dE = ptrs[pi++], *dY = ptrs[pi++], *dWd = ptrs[pi++], *dPart = ptrs[pi++];
   double *dYsum = ptrs[pi++], *dXsum = ptrs[pi++], *dHS = ptrs[pi++], *dGrad = ptrs[pi++], *dRdm = ptrs[pi++];
+  void* dPm = ptrs[pi++];
+  double* dPinv = ptrs[pi++];
+  if (precision != PHF_PREC_FP64) {
+    int rc = ensure_mixed_a(ctx, precision);
+    if (rc) return rc;
+  }
 
   // ---- uploads
   auto up = [&](double* d, const double* h, size_t nd) {
@@ -605,7 +652,16 @@ extern "C" int phf_buildhsg(phf_ctx* ctx, const double* ref, const double* z, co
   }
   // K3: Gamma for all grid points
   if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[1], s));
-  eri_gemm_f64(s, dPt, ctx->dA, dGam, (int)Mp, nrpad, kt, nrpad, bm);
+  if (precision == PHF_PREC_FP64) {
+    eri_gemm_f64(s, dPt, ctx->dA, dGam, (int)Mp, nrpad, kt, nrpad, bm);
+  } else if (precision == PHF_PREC_FP32) {
+    convert_f32(s, dPt, (float*)dPm, Mp * kt);
+    eri_gemm_mixed(s, precision, dPm, ctx->dAm[precision], nullptr, nullptr, dGam, (int)Mp, nrpad, kt, nrpad);
+  } else {
+    split_operand(s, precision, false, dPt, kt, (int)Mp, kt, dPm, dPinv);
+    eri_gemm_mixed(s, precision, dPm, ctx->dAm[precision], dPinv, ctx->dAinv[precision], dGam, (int)Mp, nrpad,
+                   kt * mixed_segments(precision), nrpad);
+  }
   if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[2], s));
   // K4: FX = h X + Gamma X
   zgemm_batched(s, nso, N, nso, 1, 0, ZMat{ctx->dH1, 2L * nso, 2, 0, 1, 0}, zmat_colmajor(dX, nso, sT), 0, 0,
