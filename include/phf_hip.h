@@ -85,6 +85,20 @@ int phf_buildhsg(phf_ctx* ctx, const double* ref, const double* z, const int* ng
                  const double* rootb, int nb, const double* rooty, int ny_, int qj, int sp,
                  int precision, double* hmat, double* smat, double* gradmat, double* rdmmat);
 
+/* The two halves of phf_buildhsg, for callers that evaluate many Z on one
+ * grid (the optimiser loop of PHF.py:17-53 calls buildhsg with the same R1,
+ * R2, Rpg, weights and Ref every time).  phf_set_grid uploads the grid and
+ * Ref, forms Ref^H R1[il], Rpg[ipg] Ref and the weight x Wigner x weightpg
+ * table on the device (arguments as phf_buildhsg); phf_buildhsg_z evaluates
+ * one Z on the current grid.  No counterpart in the reference: its Fortran
+ * recomputes everything per call. */
+int phf_set_grid(phf_ctx* ctx, const double* ref, const int* ngrid, int nci, int ncisp, const double* r1,
+                 int nr1, const double* r2, int nr2, const double* rpg, int npg, const double* weightsp,
+                 const double* weightpg, int ncipg, const double* roota, int na, const double* rootb, int nb,
+                 const double* rooty, int ny_, int qj, int sp);
+int phf_buildhsg_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int precision, double* hmat,
+                   double* smat, double* gradmat, double* rdmmat);
+
 /* phftools.localg (PHFTools.f90:203-247; call site PHF.py:76).
  *   gradmat, rdmmat complex [nci][nci][nso][nso]; smat complex [nci][nci];
  *   e0 complex scalar (2 doubles); fsp complex[ncisp]; fpg complex[ncipg];

@@ -63,15 +63,18 @@ class _PhfTools:
     def invalidate(self):
         if self._ctx is not None:
             self._ctx.ham_key = None
+            self._ctx.grid_key = None
 
     def set_hamiltonian(self, hone, htwo):
         key = (_fingerprint(np.asarray(hone)), _fingerprint(np.asarray(htwo)))
+        self.context().grid_key = None
         self.context().set_hamiltonian(hone, htwo, key=key)
 
     def _ensure_ham(self, hone, htwo):
         ctx = self.context()
         key = (_fingerprint(hone), _fingerprint(htwo))
         if ctx.ham_key != key:
+            ctx.grid_key = None
             ctx.set_hamiltonian(hone, htwo, key=key)
 
     # ------------------------------------------------------------------
@@ -119,18 +122,25 @@ class _PhfTools:
         self._ensure_ham(hone, htwo)
         ctx = self.context()
         nci = int(nci)
+        lib = _lib.load()
+        gkey = (ctx.ham_key, _fingerprint(ref), tuple(int(v) for v in ngrid), nci, int(ncisp), _fingerprint(r1),
+                _fingerprint(r2), _fingerprint(rpg), _fingerprint(weightsp), _fingerprint(weightpg),
+                _fingerprint(roota), _fingerprint(rootb), _fingerprint(rooty), int(qj), int(sp))
+        if getattr(ctx, "grid_key", None) != gkey:
+            ctx.grid_key = None
+            _lib.check(lib.phf_set_grid(ctx.handle, _lib.ptr(ref), ngrid.ctypes.data_as(_lib._ip), nci, int(ncisp),
+                                        _lib.ptr(r1), r1.shape[0], _lib.ptr(r2), r2.shape[0], _lib.ptr(rpg), npg,
+                                        _lib.ptr(weightsp), _lib.ptr(weightpg), ncipg, _lib.ptr(roota), roota.size,
+                                        _lib.ptr(rootb), rootb.size, _lib.ptr(rooty), rooty.size, int(qj),
+                                        int(sp)), ctx.handle)
+            ctx.grid_key = gkey
         hmat = np.zeros((nci, nci), dtype=np.complex128)
         smat = np.zeros((nci, nci), dtype=np.complex128)
         gradmat = np.zeros((nci, nci, nso, nso), dtype=np.complex128)
         rdmmat = np.zeros((nci, nci, nso, nso), dtype=np.complex128)
         prec = _lib.PREC[self.precision]
-        lib = _lib.load()
-        rc = lib.phf_buildhsg(ctx.handle, _lib.ptr(ref), _lib.ptr(z), ngrid.ctypes.data_as(_lib._ip), nci,
-                              int(cmplxconj), int(ncisp), nocc, _lib.ptr(r1), r1.shape[0], _lib.ptr(r2),
-                              r2.shape[0], _lib.ptr(rpg), npg, _lib.ptr(weightsp), _lib.ptr(weightpg), ncipg,
-                              _lib.ptr(roota), roota.size, _lib.ptr(rootb), rootb.size, _lib.ptr(rooty),
-                              rooty.size, int(qj), int(sp), prec, _lib.ptr(hmat), _lib.ptr(smat),
-                              _lib.ptr(gradmat), _lib.ptr(rdmmat))
+        rc = lib.phf_buildhsg_z(ctx.handle, _lib.ptr(z), int(cmplxconj), nocc, prec, _lib.ptr(hmat), _lib.ptr(smat),
+                                _lib.ptr(gradmat), _lib.ptr(rdmmat))
         _lib.check(rc, ctx.handle)
         return hmat, smat, gradmat, rdmmat
 

@@ -26,7 +26,8 @@ PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2, "f16x2": 3}
 
 # every symbol declared in include/phf_hip.h
 EXPORTS = ["phf_ctx_create", "phf_ctx_destroy", "phf_last_error", "phf_set_hamiltonian",
-           "phf_buildhsg", "phf_localg", "phf_set_profiling", "phf_stage_times"]
+           "phf_buildhsg", "phf_localg", "phf_set_profiling", "phf_stage_times", "phf_set_grid",
+           "phf_buildhsg_z"]
 
 
 class PhfError(RuntimeError):
@@ -67,6 +68,9 @@ def load() -> C.CDLL:
                                  _dp, _dp, _dp, _dp]
     lib.phf_localg.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int,
                                C.c_int, C.c_int, C.c_int, _dp]
+    lib.phf_set_grid.argtypes = [C.c_void_p, _dp, _ip, C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int,
+                                 _dp, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_int, C.c_int]
+    lib.phf_buildhsg_z.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
     lib.phf_set_profiling.argtypes = [C.c_void_p, C.c_int]
     lib.phf_stage_times.argtypes = [C.c_void_p, _dp, C.c_int]
     for name in EXPORTS:

@@ -31,87 +31,88 @@ template <> struct MfmaH<__bf16> {
   __device__ static f4_t mma(v8 a, v8 b, f4_t c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 };
 
-// 16-bit inputs, 16x16x32 MFMA, 128 x 128 workgroup tile (2 x 2 waves of 64 x 64),
-// k-slab of 64 elements double-buffered in LDS (row stride 72 elements = 144 B:
-// 16-byte fragment reads of 16 consecutive rows hit distinct banks).
-template <typename T, int FLUSH>
-__global__ __launch_bounds__(256, 1) void eri_gemm_h16_kernel(const T* __restrict__ P, const T* __restrict__ A,
-                                                              const double* __restrict__ invP,
-                                                              const double* __restrict__ invA,
-                                                              double* __restrict__ C, int K, long ldc,
-                                                              int tiles_m) {
-  constexpr int BM = 128, BN = 128, BK = 64, LDH = BK + 8;
+// 16-bit inputs, 16x16x32 MFMA.  Workgroup tile BM x 128 with BM/64 x 2 waves
+// of 64 x 64 (two waves per SIMD at BM = 256), k-slab of 128 elements in one
+// LDS stage (row stride 136 elements = 272 B) with the next slab prefetched
+// into registers while the current one is consumed.  SPLITK = 2 halves the
+// contraction per workgroup and adds the two FP64 partials with atomics (two
+// commutative additions onto zero: the result does not depend on their order).
+template <typename T, int BM, int SPLITK>
+__global__ __launch_bounds__(BM * 2) void eri_gemm_h16_kernel(const T* __restrict__ P, const T* __restrict__ A,
+                                                             const double* __restrict__ invP,
+                                                             const double* __restrict__ invA,
+                                                             double* __restrict__ C, int K, long ldc, int tiles_m) {
+  constexpr int BN = 128, BK = 128, LDH = BK + 8;
+  constexpr int NT = BM * 2;                  // threads
   constexpr int WM = 64, WN = 64, FM = 4, FN = 4;
+  constexpr int ROWS = BM + BN;               // rows staged per slab
+  constexpr int RPP = NT / 16;                // rows per pass: 16 threads x 16 B per 128-element row
+  constexpr int NLD = ROWS / RPP;             // 16-byte loads per thread per slab
+  static_assert(NLD <= 16 && NLD * RPP == ROWS, "staging registers");
   typedef typename MfmaH<T>::v8 v8;
-  __shared__ __attribute__((aligned(16))) T sP[2][BM * LDH];
-  __shared__ __attribute__((aligned(16))) T sA[2][BN * LDH];
+  __shared__ __attribute__((aligned(16))) T sm[ROWS * LDH];   // rows [0, BM): P tile, [BM, BM+BN): A tile
 
   const int t = xcd_remap(blockIdx.x, gridDim.x);
-  const int mt = t % tiles_m, nt = t / tiles_m;
+  const int ks_id = t % SPLITK;
+  const int tile = t / SPLITK;
+  const int mt = tile % tiles_m, nt = tile / tiles_m;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const T* Pg = P + (long)mt * BM * K;
-  const T* Ag = A + (long)nt * BN * K;
-  const int lr = tid >> 3, lc = (tid & 7) * 8;   // 8 threads x 16 B per 64-element row
+  const int kpart = K / SPLITK;
+  const int kbeg = ks_id * kpart;
+  const int lr = tid >> 4, lc = (tid & 15) * 8;
+
+  // global row pointer of staged row r: P rows of this tile, then A rows
+  auto grow = [&](int r) -> const T* {
+    return (r < BM) ? P + ((long)mt * BM + r) * K : A + ((long)nt * BN + (r - BM)) * K;
+  };
 
   f4_t acc[FM][FN];
-  d4_t facc[FM][FN];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
-      facc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
-    }
+    for (int j = 0; j < FN; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
 
-  uint4 rp0, rp1, rp2, rp3, ra0, ra1, ra2, ra3;
-#define H_LD(dst, base, e, k0) dst = *reinterpret_cast<const uint4*>((base) + (long)(lr + 32 * (e)) * K + (k0) + lc)
-#define H_ST(arr, buf, e, src) *reinterpret_cast<uint4*>(&arr[buf][(lr + 32 * (e)) * LDH + lc]) = src
-#define H_GLOAD(k0)                                                                          \
-  do {                                                                                       \
-    H_LD(rp0, Pg, 0, k0); H_LD(rp1, Pg, 1, k0); H_LD(rp2, Pg, 2, k0); H_LD(rp3, Pg, 3, k0); \
-    H_LD(ra0, Ag, 0, k0); H_LD(ra1, Ag, 1, k0); H_LD(ra2, Ag, 2, k0); H_LD(ra3, Ag, 3, k0); \
+  uint4 st0, st1, st2, st3, st4, st5, st6, st7, st8, st9, st10, st11, st12, st13, st14, st15;
+#define H_LD(dst, e, k0) \
+  if constexpr (NLD > (e)) dst = *reinterpret_cast<const uint4*>(grow(lr + RPP * (e)) + (k0) + lc)
+#define H_ST(e, src) \
+  if constexpr (NLD > (e)) *reinterpret_cast<uint4*>(&sm[(lr + RPP * (e)) * LDH + lc]) = src
+#define H_GLOAD(k0)                                                               \
+  do {                                                                            \
+    H_LD(st0, 0, k0); H_LD(st1, 1, k0); H_LD(st2, 2, k0); H_LD(st3, 3, k0);       \
+    H_LD(st4, 4, k0); H_LD(st5, 5, k0); H_LD(st6, 6, k0); H_LD(st7, 7, k0);       \
+    H_LD(st8, 8, k0); H_LD(st9, 9, k0); H_LD(st10, 10, k0); H_LD(st11, 11, k0);   \
+    H_LD(st12, 12, k0); H_LD(st13, 13, k0); H_LD(st14, 14, k0); H_LD(st15, 15, k0); \
   } while (0)
-#define H_LSTORE(buf)                                                                        \
-  do {                                                                                       \
-    H_ST(sP, buf, 0, rp0); H_ST(sP, buf, 1, rp1); H_ST(sP, buf, 2, rp2); H_ST(sP, buf, 3, rp3); \
-    H_ST(sA, buf, 0, ra0); H_ST(sA, buf, 1, ra1); H_ST(sA, buf, 2, ra2); H_ST(sA, buf, 3, ra3); \
+#define H_LSTORE()                                                                \
+  do {                                                                            \
+    H_ST(0, st0); H_ST(1, st1); H_ST(2, st2); H_ST(3, st3);                       \
+    H_ST(4, st4); H_ST(5, st5); H_ST(6, st6); H_ST(7, st7);                       \
+    H_ST(8, st8); H_ST(9, st9); H_ST(10, st10); H_ST(11, st11);                   \
+    H_ST(12, st12); H_ST(13, st13); H_ST(14, st14); H_ST(15, st15);               \
   } while (0)
 
-  H_GLOAD(0);
-  H_LSTORE(0);
-  __syncthreads();
-  const int nk = K / BK;
+  const int nk = kpart / BK;
+  H_GLOAD(kbeg);
   for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    if (kt + 1 < nk) H_GLOAD((kt + 1) * BK);
+    H_LSTORE();
+    __syncthreads();
+    if (kt + 1 < nk) H_GLOAD(kbeg + (kt + 1) * BK);
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 32) {
       v8 a[FM], b[FN];
 #pragma unroll
       for (int i = 0; i < FM; ++i)
-        a[i] = *reinterpret_cast<const v8*>(&sP[cur][(wm * WM + i * 16 + (lane & 15)) * LDH + ks + 8 * (lane >> 4)]);
+        a[i] = *reinterpret_cast<const v8*>(&sm[(wm * WM + i * 16 + (lane & 15)) * LDH + ks + 8 * (lane >> 4)]);
 #pragma unroll
       for (int j = 0; j < FN; ++j)
-        b[j] = *reinterpret_cast<const v8*>(&sA[cur][(wn * WN + j * 16 + (lane & 15)) * LDH + ks + 8 * (lane >> 4)]);
+        b[j] = *reinterpret_cast<const v8*>(&sm[(BM + wn * WN + j * 16 + (lane & 15)) * LDH + ks + 8 * (lane >> 4)]);
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = MfmaH<T>::mma(a[i], b[j], acc[i][j]);
     }
-    if constexpr (FLUSH > 0) {
-      if ((kt + 1) % FLUSH == 0 || kt + 1 == nk) {
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-          for (int j = 0; j < FN; ++j) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) facc[i][j][r] += (double)acc[i][j][r];
-            acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
-          }
-      }
-    }
-    if (kt + 1 < nk) H_LSTORE(cur ^ 1);
     __syncthreads();
   }
 #undef H_LD
@@ -127,8 +128,11 @@ __global__ __launch_bounds__(256, 1) void eri_gemm_h16_kernel(const T* __restric
       for (int r = 0; r < 4; ++r) {
         const long row = (long)mt * BM + wm * WM + i * 16 + 4 * (lane >> 4) + r;
         const long col = (long)nt * BN + wn * WN + j * 16 + (lane & 15);
-        const double v = (FLUSH > 0) ? facc[i][j][r] : (double)acc[i][j][r];
-        C[row * ldc + col] = v * invP[row] * invA[col];
+        const double v = (double)acc[i][j][r] * invP[row] * invA[col];
+        if constexpr (SPLITK == 1)
+          C[row * ldc + col] = v;
+        else
+          atomicAdd(&C[row * ldc + col], v);
       }
 }
 
@@ -301,16 +305,42 @@ void convert_f32(hipStream_t s, const double* src, float* dst, long n) {
   to_f32_kernel<<<blocks, 256, 0, s>>>(src, dst, n);
 }
 
-// Mp, Nr multiples of 128; Kseg = nseg * kt multiple of 64 (16-bit) or 32 (FP32).
+// Tile height (grid-point rows) and split-K of the 16-bit kernel for a problem
+// of Mp0 rows, Nr columns on ncu compute units: fewest padded MFMA cycles per
+// CU, counting whole rounds of workgroups (one 256-row or two 128-row
+// workgroups per CU).
+void mixed_tile_choice(long Mp0, int Nr, int ncu, int* bm, int* splitk) {
+  double best = 1e300;
+  for (int b : {128, 256})
+    for (int sk : {1, 2}) {
+      const long mp = (Mp0 + b - 1) / b * b;
+      const long units = (mp / b) * (Nr / 128) * sk;
+      const int per_round = (b == 256) ? ncu : 2 * ncu;
+      const long rounds = (units + per_round - 1) / per_round;
+      const double cost = (double)rounds * b / sk * (b == 256 ? 1.0 : 0.5) + (sk == 2 ? 0.02 * rounds : 0.0);
+      if (cost < best - 1e-9) { best = cost; *bm = b; *splitk = sk; }
+    }
+}
+
+// Mp multiple of bm, Nr multiple of 128; Kseg = nseg * kt multiple of 128 * splitk
+// (16-bit) or 32 (FP32).  With splitk = 2 the caller zero-fills C first.
 void eri_gemm_mixed(hipStream_t s, int prec, const void* P, const void* A, const double* invP, const double* invA,
-                    double* C, int Mp, int Nr, int Kseg, long ldc) {
-  const int tm = Mp / 128, tn = Nr / 128;
-  if (prec == 1)
+                    double* C, int Mp, int Nr, int Kseg, long ldc, int bm, int splitk) {
+  if (prec == 1) {
+    const int tm = Mp / 128, tn = Nr / 128;
     eri_gemm_f32_kernel<16><<<tm * tn, 256, 0, s>>>((const float*)P, (const float*)A, C, Kseg, ldc, tm);
-  else if (prec == 2)
-    eri_gemm_h16_kernel<__bf16, 8><<<tm * tn, 256, 0, s>>>((const __bf16*)P, (const __bf16*)A, invP, invA, C, Kseg,
-                                                           ldc, tm);
-  else
-    eri_gemm_h16_kernel<_Float16, 8><<<tm * tn, 256, 0, s>>>((const _Float16*)P, (const _Float16*)A, invP, invA, C,
-                                                             Kseg, ldc, tm);
+    return;
+  }
+  const int tm = Mp / bm, tn = Nr / 128;
+  const unsigned grid = (unsigned)(tm * tn * splitk);
+#define H_LAUNCH(TT, BMV, SK) \
+  eri_gemm_h16_kernel<TT, BMV, SK><<<grid, BMV * 2, 0, s>>>((const TT*)P, (const TT*)A, invP, invA, C, Kseg, ldc, tm)
+  if (prec == 2) {
+    if (bm == 256) { if (splitk == 2) H_LAUNCH(__bf16, 256, 2); else H_LAUNCH(__bf16, 256, 1); }
+    else { if (splitk == 2) H_LAUNCH(__bf16, 128, 2); else H_LAUNCH(__bf16, 128, 1); }
+  } else {
+    if (bm == 256) { if (splitk == 2) H_LAUNCH(_Float16, 256, 2); else H_LAUNCH(_Float16, 256, 1); }
+    else { if (splitk == 2) H_LAUNCH(_Float16, 128, 2); else H_LAUNCH(_Float16, 128, 1); }
+  }
+#undef H_LAUNCH
 }

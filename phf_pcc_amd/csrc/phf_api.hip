@@ -40,10 +40,12 @@ void split_operand(hipStream_t s, int prec, bool is_a, const double* src, long l
                    double* inv);
 void convert_f32(hipStream_t s, const double* src, float* dst, long n);
 void eri_gemm_mixed(hipStream_t s, int prec, const void* P, const void* A, const double* invP, const double* invA,
-                    double* C, int Mp, int Nr, int Kseg, long ldc);
+                    double* C, int Mp, int Nr, int Kseg, long ldc, int bm, int splitk);
+void mixed_tile_choice(long Mp0, int Nr, int ncu, int* bm, int* splitk);
 
 struct phf_ctx_s {
   int device = 0;
+  int ncu = 256;
   hipStream_t stream = nullptr;
   std::string err;
   // cached Hamiltonian
@@ -60,6 +62,14 @@ struct phf_ctx_s {
   // workspace arena
   char* ws = nullptr;
   size_t ws_size = 0;
+  // device-resident spin grid (phf_set_grid)
+  bool has_grid = false;
+  int g_nl = 0, g_ny = 0, g_npg = 0, g_nci = 0, g_ncisp = 0, g_ncipg = 0, g_qj = 0, g_sp = 0, g_nso = 0;
+  long g_G = 0;
+  double* dGr1p = nullptr;     // Ref^H R1[il]            [nl][nso x nso] column-major
+  double* dGr2 = nullptr;      // R2[iy]                  [ny] row-major (as given)
+  double* dGrpgref = nullptr;  // Rpg[ipg] Ref            [npg][nso x nso] column-major
+  double* dGw = nullptr;       // weight x Wigner x weightpg  [G][nci][nci]
   // per-stage timing with HIP events on `stream` (phf_set_profiling)
   bool profiling = false;
   hipEvent_t ev[8] = {};
@@ -68,6 +78,7 @@ struct phf_ctx_s {
 };
 
 static thread_local std::string g_create_err;
+static int free_grid(phf_ctx* ctx);
 
 namespace {
 
@@ -354,6 +365,11 @@ extern "C" int phf_ctx_create(int device, phf_ctx** out) {
   }
   phf_ctx* ctx = new phf_ctx_s();
   ctx->device = device;
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+      ctx->ncu = prop.multiProcessorCount;
+  }
   e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     g_create_err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
@@ -375,6 +391,8 @@ extern "C" int phf_ctx_destroy(phf_ctx* ctx) {
     if (ctx->dAinv[q]) (void)hipFree(ctx->dAinv[q]);
   }
   if (ctx->ws) (void)hipFree(ctx->ws);
+  for (double* q : {ctx->dGr1p, ctx->dGr2, ctx->dGrpgref, ctx->dGw})
+    if (q) (void)hipFree(q);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);
@@ -405,7 +423,10 @@ extern "C" int phf_set_hamiltonian(phf_ctx* ctx, const double* hone, const doubl
     return fail(ctx, PHF_ERR_ARG, "phf_set_hamiltonian: bad arguments");
   HIPCK(hipSetDevice(ctx->device));
   ctx->has_ham = false;
-  HIPCK(hipStreamSynchronize(ctx->stream));
+  {
+    int rc = free_grid(ctx);
+    if (rc) return rc;
+  }
   for (int q = 0; q < 4; ++q) {
     if (ctx->dAm[q]) { HIPCK(hipFree(ctx->dAm[q])); ctx->dAm[q] = nullptr; }
     if (ctx->dAinv[q]) { HIPCK(hipFree(ctx->dAinv[q])); ctx->dAinv[q] = nullptr; }
@@ -429,7 +450,7 @@ extern "C" int phf_set_hamiltonian(phf_ctx* ctx, const double* hone, const doubl
   HIPCK(hipStreamSynchronize(ctx->stream));
   ctx->cplx = hflag != 0 || (flags & 1);
   ctx->nso = nso;
-  ctx->kpad = (int)round_up(n2, 64);
+  ctx->kpad = (int)round_up(n2, 256);
   ctx->nrpad = (int)round_up(n2, 128);
   ctx->kt = ctx->cplx ? 2 * ctx->kpad : ctx->kpad;
   const long asz = (long)ctx->nrpad * ctx->kt;
@@ -466,53 +487,48 @@ static int ensure_mixed_a(phf_ctx* ctx, int prec) {
   return PHF_OK;
 }
 
-extern "C" int phf_buildhsg(phf_ctx* ctx, const double* ref, const double* z, const int* ngrid, int nci,
-                            int cmplxconj, int ncisp, int nocc, const double* r1, int nr1, const double* r2,
-                            int nr2, const double* rpg, int npg, const double* weightsp,
-                            const double* weightpg, int ncipg, const double* roota, int na,
-                            const double* rootb, int nb, const double* rooty, int ny_, int qj, int sp,
-                            int precision, double* hmat, double* smat, double* gradmat, double* rdmmat) {
+static int free_grid(phf_ctx* ctx) {
+  HIPCK(hipStreamSynchronize(ctx->stream));
+  for (double** q : {&ctx->dGr1p, &ctx->dGr2, &ctx->dGrpgref, &ctx->dGw})
+    if (*q) {
+      HIPCK(hipFree(*q));
+      *q = nullptr;
+    }
+  ctx->has_grid = false;
+  return PHF_OK;
+}
+
+extern "C" int phf_set_grid(phf_ctx* ctx, const double* ref, const int* ngrid, int nci, int ncisp,
+                            const double* r1, int nr1, const double* r2, int nr2, const double* rpg, int npg,
+                            const double* weightsp, const double* weightpg, int ncipg, const double* roota,
+                            int na, const double* rootb, int nb, const double* rooty, int ny_, int qj, int sp) {
   if (!ctx) return PHF_ERR_ARG;
-  if (!ctx->has_ham) return fail(ctx, PHF_ERR_NO_HAMILTONIAN, "phf_buildhsg: no Hamiltonian set");
+  if (!ctx->has_ham) return fail(ctx, PHF_ERR_NO_HAMILTONIAN, "phf_set_grid: no Hamiltonian set");
   const int nso = ctx->nso;
-  if (!ref || !ngrid || !r1 || !r2 || !rpg || !weightsp || !weightpg || !hmat || !smat || !gradmat || !rdmmat)
-    return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: NULL pointer argument");
+  if (!ref || !ngrid || !r1 || !r2 || !rpg || !weightsp || !weightpg)
+    return fail(ctx, PHF_ERR_ARG, "phf_set_grid: NULL pointer argument");
   const int nl = ngrid[0], ny = ngrid[1];
   if (nl <= 0 || ny <= 0 || npg <= 0 || nci <= 0 || ncisp <= 0 || ncipg <= 0 || qj < 0)
-    return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: non-positive grid or CI dimension");
-  if (nocc <= 0 || nocc > nso) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: nocc out of range");
-  if (nocc > lu_inverse_max_n())
-    return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: nocc above the LDS inverse limit");
-  if (nocc < nso && !z) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: z is NULL");
-  if (nl > nr1 || ny > nr2) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: ngrid exceeds R1/R2 extent");
-  if (cmplxconj != 0 && cmplxconj != 1)
-    return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: CmplxConj=2 is not implemented on the GPU path");
-  if (precision < PHF_PREC_FP64 || precision > PHF_PREC_F16X2)
-    return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: unknown precision");
+    return fail(ctx, PHF_ERR_ARG, "phf_set_grid: non-positive grid or CI dimension");
+  if (nl > nr1 || ny > nr2) return fail(ctx, PHF_ERR_ARG, "phf_set_grid: ngrid exceeds R1/R2 extent");
   if (qj + qj + (ncipg - 1) * ncisp >= nci)
-    return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: nci too small for QJ/ncisp/ncipg");
+    return fail(ctx, PHF_ERR_ARG, "phf_set_grid: nci too small for QJ/ncisp/ncipg");
   if (qj > 0) {
     const bool ok = (sp == 2) ? (nl <= na && ny <= nb && nl <= ny_) : (nl <= na && nl <= nb && ny <= ny_);
     if (!roota || !rootb || !rooty || !ok)
-      return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: Wigner angle arrays too short");
+      return fail(ctx, PHF_ERR_ARG, "phf_set_grid: Wigner angle arrays too short");
   }
+  const long G = (long)nl * ny * npg;
+  if (G > 65535) return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_set_grid: more than 65535 grid points");
   HIPCK(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
-
-  const int N = nocc;
-  const long G = (long)nl * ny * npg;
+  {
+    int rc = free_grid(ctx);
+    if (rc) return rc;
+  }
   const int nmn = nci * nci;
   const long n2 = (long)nso * nso;
-
-  // ---- host precompute: C0 = [I; Z] (column-major) and the grid weight table
-  std::vector<double> c0(2 * (size_t)nso * N, 0.0);
-  for (int a = 0; a < N; ++a) c0[2 * ((size_t)a + (size_t)a * nso)] = 1.0;
-  for (int i = N; i < nso; ++i)
-    for (int a = 0; a < N; ++a) {
-      const size_t src = 2 * ((size_t)(i - N) * N + a);
-      c0[2 * ((size_t)i + (size_t)a * nso)] = z[src];
-      c0[2 * ((size_t)i + (size_t)a * nso) + 1] = (cmplxconj == 0) ? 0.0 : z[src + 1];
-    }
+  // grid weight table W[g][m][n] = weightsp(il,iy) * Wignerfac * weightpg(ipg,p,q) (PHFTools.f90:120-152)
   std::vector<double> wtab(2 * (size_t)G * nmn, 0.0);
   for (int il = 0; il < nl; ++il)
     for (int iy = 0; iy < ny; ++iy)
@@ -536,26 +552,88 @@ extern "C" int phf_buildhsg(phf_ctx* ctx, const double* ref, const double* z, co
               }
           }
       }
+  HIPCK(hipMalloc(&ctx->dGr1p, sizeof(double) * 2 * nl * n2));
+  HIPCK(hipMalloc(&ctx->dGr2, sizeof(double) * 2 * ny * n2));
+  HIPCK(hipMalloc(&ctx->dGrpgref, sizeof(double) * 2 * npg * n2));
+  HIPCK(hipMalloc(&ctx->dGw, sizeof(double) * wtab.size()));
+  // staging for Ref, R1, Rpg in the workspace arena
+  {
+    int rc = ensure_ws(ctx, sizeof(double) * 2 * n2 * (1 + nl + npg) + 1024);
+    if (rc) return rc;
+  }
+  double* dRef = reinterpret_cast<double*>(ctx->ws);
+  double* dR1 = dRef + 2 * n2;
+  double* dRpg = dR1 + 2 * nl * n2;
+  auto up = [&](double* d, const double* h, size_t nd) {
+    return hipMemcpyAsync(d, h, nd * sizeof(double), hipMemcpyHostToDevice, s);
+  };
+  HIPCK(up(dRef, ref, 2 * n2));
+  HIPCK(up(dR1, r1, 2 * (size_t)nl * n2));
+  HIPCK(up(ctx->dGr2, r2, 2 * (size_t)ny * n2));
+  HIPCK(up(dRpg, rpg, 2 * (size_t)npg * n2));
+  HIPCK(up(ctx->dGw, wtab.data(), wtab.size()));
+  // R1p[il] = Ref^H R1[il]   (row-major host layouts in, column-major out)
+  zgemm_batched(s, nso, nso, nso, 1, 0, ZMat{dRef, 2, 2L * nso, 0, 1, 1}, ZMat{dR1, 2L * nso, 2, 2 * n2, 1, 0}, 0,
+                0, zout_colmajor(ctx->dGr1p, nso, 2 * n2), nl);
+  // RpgRef[ipg] = Rpg[ipg] Ref
+  zgemm_batched(s, nso, nso, nso, 1, 0, ZMat{dRpg, 2L * nso, 2, 2 * n2, 1, 0}, ZMat{dRef, 2L * nso, 2, 0, 1, 0}, 0,
+                0, zout_colmajor(ctx->dGrpgref, nso, 2 * n2), npg);
+  HIPCK(hipGetLastError());
+  HIPCK(hipStreamSynchronize(s));
+  ctx->g_nl = nl; ctx->g_ny = ny; ctx->g_npg = npg; ctx->g_nci = nci; ctx->g_ncisp = ncisp;
+  ctx->g_ncipg = ncipg; ctx->g_qj = qj; ctx->g_sp = sp; ctx->g_G = G; ctx->g_nso = nso;
+  ctx->has_grid = true;
+  return PHF_OK;
+}
+
+extern "C" int phf_buildhsg_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int precision, double* hmat,
+                              double* smat, double* gradmat, double* rdmmat) {
+  if (!ctx) return PHF_ERR_ARG;
+  if (!ctx->has_ham) return fail(ctx, PHF_ERR_NO_HAMILTONIAN, "phf_buildhsg: no Hamiltonian set");
+  if (!ctx->has_grid || ctx->g_nso != ctx->nso)
+    return fail(ctx, PHF_ERR_ARG, "phf_buildhsg_z: no spin grid set for this Hamiltonian (phf_set_grid)");
+  const int nso = ctx->nso;
+  if (!hmat || !smat || !gradmat || !rdmmat) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: NULL output pointer");
+  if (nocc <= 0 || nocc > nso) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: nocc out of range");
+  if (nocc > lu_inverse_max_n())
+    return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: nocc above the LDS inverse limit");
+  if (nocc < nso && !z) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: z is NULL");
+  if (cmplxconj != 0 && cmplxconj != 1)
+    return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: CmplxConj=2 is not implemented on the GPU path");
+  if (precision < PHF_PREC_FP64 || precision > PHF_PREC_F16X2)
+    return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: unknown precision");
+  HIPCK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int nl = ctx->g_nl, ny = ctx->g_ny, npg = ctx->g_npg, nci = ctx->g_nci;
+  const int N = nocc;
+  const long G = ctx->g_G;
+  const int nmn = nci * nci;
+  const long n2 = (long)nso * nso;
+
+  // ---- host: C0 = [I; Z] (column-major)
+  std::vector<double> c0(2 * (size_t)nso * N, 0.0);
+  for (int a = 0; a < N; ++a) c0[2 * ((size_t)a + (size_t)a * nso)] = 1.0;
+  for (int i = N; i < nso; ++i)
+    for (int a = 0; a < N; ++a) {
+      const size_t src = 2 * ((size_t)(i - N) * N + a);
+      c0[2 * ((size_t)i + (size_t)a * nso)] = z[src];
+      c0[2 * ((size_t)i + (size_t)a * nso) + 1] = (cmplxconj == 0) ? 0.0 : z[src + 1];
+    }
 
   // ---- workspace layout
-  const int bm = (precision == PHF_PREC_FP64) ? eri_gemm_f64_bm((int)(2 * G)) : 128;
-  const long Mp = round_up(2 * G, bm);
   const int kt = ctx->kt, nrpad = ctx->nrpad;
+  int bm = 128, splitk = 1;
+  if (precision == PHF_PREC_FP64) bm = eri_gemm_f64_bm((int)(2 * G));
+  else if (precision != PHF_PREC_FP32) mixed_tile_choice(2 * G, nrpad, ctx->ncu, &bm, &splitk);
+  const long Mp = round_up(2 * G, bm);
   const int gchunk = 32;
   const int nchunk = (int)((G + gchunk - 1) / gchunk);
   const long lenX = (long)nso * N;
   Arena A0{nullptr};
   auto carve = [&](Arena& a, auto&& sink) {
-    sink(a.template take<double>(2 * n2), "ref");
     sink(a.template take<double>(2 * lenX), "c0");
-    sink(a.template take<double>(2 * (size_t)nl * n2), "r1");
-    sink(a.template take<double>(2 * (size_t)ny * n2), "r2");
-    sink(a.template take<double>(2 * (size_t)npg * n2), "rpg");
-    sink(a.template take<double>(2 * (size_t)G * nmn), "w");
-    sink(a.template take<double>(2 * lenX), "w0");
     sink(a.template take<double>(2 * lenX * npg), "wpg");
     sink(a.template take<double>(2 * lenX * npg * ny), "w2");
-    sink(a.template take<double>(2 * (size_t)nl * n2), "r1p");
     sink(a.template take<double>(2 * lenX * G), "t");
     sink(a.template take<double>(2 * (size_t)N * N * G), "m");
     sink(a.template take<double>(2 * (size_t)N * N * G), "minv");
@@ -590,8 +668,7 @@ extern "C" int phf_buildhsg(phf_ctx* ctx, const double* ref, const double* z, co
   Arena A1{ctx->ws};
   carve(A1, [&](double* p, const char*) { ptrs.push_back(p); });
   int pi = 0;
-  double *dRef = ptrs[pi++], *dC0 = ptrs[pi++], *dR1 = ptrs[pi++], *dR2 = ptrs[pi++], *dRpg = ptrs[pi++];
-  double *dW = ptrs[pi++], *dW0 = ptrs[pi++], *dWpg = ptrs[pi++], *dW2 = ptrs[pi++], *dR1p = ptrs[pi++];
+  double *dC0 = ptrs[pi++], *dWpg = ptrs[pi++], *dW2 = ptrs[pi++];
   double *dT = ptrs[pi++], *dM = ptrs[pi++], *dMinv = ptrs[pi++], *dDet = ptrs[pi++];
   int* dInfo = reinterpret_cast<int*>(ptrs[pi++]);
   double *dX = ptrs[pi++], *dPt = ptrs[pi++], *dGam = ptrs[pi++], *dFX = ptrs[pi++], *dQ = ptrs[pi++];
@@ -599,41 +676,26 @@ extern "C" int phf_buildhsg(phf_ctx* ctx, const double* ref, const double* z, co
   double *dYsum = ptrs[pi++], *dXsum = ptrs[pi++], *dHS = ptrs[pi++], *dGrad = ptrs[pi++], *dRdm = ptrs[pi++];
   void* dPm = ptrs[pi++];
   double* dPinv = ptrs[pi++];
+  double* dW = ctx->dGw;
   if (precision != PHF_PREC_FP64) {
     int rc = ensure_mixed_a(ctx, precision);
     if (rc) return rc;
   }
-
-  // ---- uploads
-  auto up = [&](double* d, const double* h, size_t nd) {
-    return hipMemcpyAsync(d, h, nd * sizeof(double), hipMemcpyHostToDevice, s);
-  };
-  HIPCK(up(dRef, ref, 2 * n2));
-  HIPCK(up(dC0, c0.data(), c0.size()));
-  HIPCK(up(dR1, r1, 2 * (size_t)nl * n2));
-  HIPCK(up(dR2, r2, 2 * (size_t)ny * n2));
-  HIPCK(up(dRpg, rpg, 2 * (size_t)npg * n2));
-  HIPCK(up(dW, wtab.data(), wtab.size()));
+  HIPCK(hipMemcpyAsync(dC0, c0.data(), c0.size() * sizeof(double), hipMemcpyHostToDevice, s));
 
   if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[0], s));
   const long sT = 2 * lenX;          // batch stride of NSO x N column-major blocks
   const long sNN = 2L * N * N;
-  // K1: rotation chain, right to left
-  // W0 = Ref C0     (Ref row-major host layout)
-  zgemm_batched(s, nso, N, nso, 1, 0, ZMat{dRef, 2L * nso, 2, 0, 1, 0}, zmat_colmajor(dC0, nso, 0), 0, 0,
-                zout_colmajor(dW0, nso, 0), 1);
-  // Wpg[ipg] = Rpg[ipg] W0
-  zgemm_batched(s, nso, N, nso, 1, 0, ZMat{dRpg, 2L * nso, 2, 2 * n2, 1, 0}, zmat_colmajor(dW0, nso, 0), 0, 0,
+  // K1: rotation chain, right to left (Ref^H R1 and Rpg Ref are cached with the grid)
+  // Wpg[ipg] = (Rpg[ipg] Ref) C0
+  zgemm_batched(s, nso, N, nso, 1, 0, zmat_colmajor(ctx->dGrpgref, nso, 2 * n2), zmat_colmajor(dC0, nso, 0), 0, 0,
                 zout_colmajor(dWpg, nso, sT), npg);
   // W2[iy] = R2[iy] [Wpg_0 .. Wpg_{npg-1}]
-  zgemm_batched(s, nso, N * npg, nso, 1, 0, ZMat{dR2, 2L * nso, 2, 2 * n2, 1, 0},
+  zgemm_batched(s, nso, N * npg, nso, 1, 0, ZMat{ctx->dGr2, 2L * nso, 2, 2 * n2, 1, 0},
                 zmat_colmajor(dWpg, nso, 0), 0, 0, zout_colmajor(dW2, nso, sT * npg), ny);
-  // R1p[il] = Ref^H R1[il]
-  zgemm_batched(s, nso, nso, nso, 1, 0, ZMat{dRef, 2, 2L * nso, 0, 1, 1}, ZMat{dR1, 2L * nso, 2, 2 * n2, 1, 0}, 0,
-                0, zout_colmajor(dR1p, nso, 2 * n2), nl);
-  // T[il] = R1p[il] W2   -> T_g = columns g*N .. g*N+N-1
-  zgemm_batched(s, nso, N * npg * ny, nso, 1, 0, zmat_colmajor(dR1p, nso, 2 * n2), zmat_colmajor(dW2, nso, 0), 0, 0,
-                zout_colmajor(dT, nso, sT * npg * ny), nl);
+  // T[il] = (Ref^H R1[il]) W2   -> T_g = columns g*N .. g*N+N-1
+  zgemm_batched(s, nso, N * npg * ny, nso, 1, 0, zmat_colmajor(ctx->dGr1p, nso, 2 * n2), zmat_colmajor(dW2, nso, 0),
+                0, 0, zout_colmajor(dT, nso, sT * npg * ny), nl);
   // M_g = C0^H T_g
   zgemm_batched(s, N, N, nso, 1, 0, ZMat{dC0, 2L * nso, 2, 0, 1, 1}, zmat_colmajor(dT, nso, sT), 0, 0,
                 zout_colmajor(dM, N, sNN), (int)G);
@@ -656,11 +718,12 @@ extern "C" int phf_buildhsg(phf_ctx* ctx, const double* ref, const double* z, co
     eri_gemm_f64(s, dPt, ctx->dA, dGam, (int)Mp, nrpad, kt, nrpad, bm);
   } else if (precision == PHF_PREC_FP32) {
     convert_f32(s, dPt, (float*)dPm, Mp * kt);
-    eri_gemm_mixed(s, precision, dPm, ctx->dAm[precision], nullptr, nullptr, dGam, (int)Mp, nrpad, kt, nrpad);
+    eri_gemm_mixed(s, precision, dPm, ctx->dAm[precision], nullptr, nullptr, dGam, (int)Mp, nrpad, kt, nrpad, 128, 1);
   } else {
     split_operand(s, precision, false, dPt, kt, (int)Mp, kt, dPm, dPinv);
+    if (splitk > 1) fill_zero(s, dGam, Mp * nrpad);
     eri_gemm_mixed(s, precision, dPm, ctx->dAm[precision], dPinv, ctx->dAinv[precision], dGam, (int)Mp, nrpad,
-                   kt * mixed_segments(precision), nrpad);
+                   kt * mixed_segments(precision), nrpad, bm, splitk);
   }
   if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[2], s));
   // K4: FX = h X + Gamma X
@@ -739,6 +802,18 @@ extern "C" int phf_buildhsg(phf_ctx* ctx, const double* ref, const double* z, co
   const double tol = 1e-5;
   if (w1 > hmax * tol || w2 > smax * tol) return fail(ctx, PHF_ERR_NOT_HERMITIAN, "Not Hermitian");
   return PHF_OK;
+}
+
+extern "C" int phf_buildhsg(phf_ctx* ctx, const double* ref, const double* z, const int* ngrid, int nci,
+                            int cmplxconj, int ncisp, int nocc, const double* r1, int nr1, const double* r2,
+                            int nr2, const double* rpg, int npg, const double* weightsp,
+                            const double* weightpg, int ncipg, const double* roota, int na,
+                            const double* rootb, int nb, const double* rooty, int ny_, int qj, int sp,
+                            int precision, double* hmat, double* smat, double* gradmat, double* rdmmat) {
+  int rc = phf_set_grid(ctx, ref, ngrid, nci, ncisp, r1, nr1, r2, nr2, rpg, npg, weightsp, weightpg, ncipg, roota,
+                        na, rootb, nb, rooty, ny_, qj, sp);
+  if (rc) return rc;
+  return phf_buildhsg_z(ctx, z, cmplxconj, nocc, precision, hmat, smat, gradmat, rdmmat);
 }
 
 extern "C" int phf_set_profiling(phf_ctx* ctx, int on) {
