@@ -57,6 +57,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,f16x2",
                     help="comma list of ERI precisions to time (fp64 always runs: it is the reference form)")
+    ap.add_argument("--no-autotune", action="store_true", help="use the tile cost model instead of timing candidates")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -116,6 +117,8 @@ def main():
         return r
 
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    if args.no_autotune:
+        ctx.autotune(False)
     if "fp64" not in forms:
         forms = ["fp64"] + forms
     res = {f: run_form(f) for f in forms}
@@ -125,6 +128,8 @@ def main():
         r["dG"] = float(np.abs(r["G"] - base["G"]).max())
         log(f"[bench] {f}: |dE| vs fp64 = {r['dE']:.3e}  max|dG| = {r['dG']:.3e}  "
             f"speedup vs fp64 = {r['value'] / base['value']:.2f}x")
+    for line in ctx.tuning_report().splitlines():
+        log(f"[bench] tuning {line}")
     ok = [r for r in res.values() if r["dE"] <= 1e-6]
     best = max(ok, key=lambda r: r["value"])
     flops = 4.0 * nso ** 4 * G                       # algorithmic FLOPs of one ERI GEMM launch

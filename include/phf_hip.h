@@ -117,6 +117,15 @@ int phf_localg(phf_ctx* ctx, const double* gradmat, const double* rdmmat, const 
 int phf_set_profiling(phf_ctx* ctx, int on);
 int phf_stage_times(const phf_ctx* ctx, double* ms, int nmax);
 
+/* Tile autotuner of the reduced-precision ERI GEMM (no counterpart in the
+ * reference).  on = 1 (default): the first call for a problem shape times
+ * every tile/split-K candidate on the live operands and keeps the fastest;
+ * on = 0: a cost model of MFMA rounds per CU decides.  Either call clears the
+ * cache.  phf_tuning_report copies one JSON line per tuned shape (candidates,
+ * times, choice) into buf and returns the length written. */
+int phf_autotune(phf_ctx* ctx, int on);
+int phf_tuning_report(const phf_ctx* ctx, char* buf, int len);
+
 #ifdef __cplusplus
 }
 #endif

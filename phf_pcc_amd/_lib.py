@@ -27,7 +27,7 @@ PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2, "f16x2": 3}
 # every symbol declared in include/phf_hip.h
 EXPORTS = ["phf_ctx_create", "phf_ctx_destroy", "phf_last_error", "phf_set_hamiltonian",
            "phf_buildhsg", "phf_localg", "phf_set_profiling", "phf_stage_times", "phf_set_grid",
-           "phf_buildhsg_z"]
+           "phf_buildhsg_z", "phf_autotune", "phf_tuning_report"]
 
 
 class PhfError(RuntimeError):
@@ -71,6 +71,8 @@ def load() -> C.CDLL:
     lib.phf_set_grid.argtypes = [C.c_void_p, _dp, _ip, C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int,
                                  _dp, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_int, C.c_int]
     lib.phf_buildhsg_z.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+    lib.phf_autotune.argtypes = [C.c_void_p, C.c_int]
+    lib.phf_tuning_report.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
     lib.phf_set_profiling.argtypes = [C.c_void_p, C.c_int]
     lib.phf_stage_times.argtypes = [C.c_void_p, _dp, C.c_int]
     for name in EXPORTS:
@@ -130,6 +132,14 @@ class Context:
 
     def set_profiling(self, on: bool = True):
         check(load().phf_set_profiling(self._h, 1 if on else 0), self._h)
+
+    def autotune(self, on: bool = True):
+        check(load().phf_autotune(self._h, 1 if on else 0), self._h)
+
+    def tuning_report(self) -> str:
+        buf = C.create_string_buffer(1 << 16)
+        n = load().phf_tuning_report(self._h, buf, len(buf))
+        return buf.raw[:n].decode()
 
     def stage_times(self) -> np.ndarray:
         out = np.zeros(8)

@@ -14,6 +14,8 @@
 //
 // Layout contract as eri_gemm.hip: P' is [Mp][K'], A' is [Nr][K'], C is
 // [Mp][ldc]; Mp % BM == 0, Nr % BN == 0, K' % BK == 0.
+#include <type_traits>
+
 #include "phf_common.h"
 
 typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
@@ -31,94 +33,125 @@ template <> struct MfmaH<__bf16> {
   __device__ static f4_t mma(v8 a, v8 b, f4_t c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 };
 
-// 16-bit inputs, 16x16x32 MFMA.  Workgroup tile BM x 128 with BM/64 x 2 waves
-// of 64 x 64 (two waves per SIMD at BM = 256), k-slab of 128 elements in one
-// LDS stage (row stride 136 elements = 272 B) with the next slab prefetched
-// into registers while the current one is consumed.  SPLITK = 2 halves the
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) void glb_void_t;
+
+// s_waitcnt with only vmcnt <= v (expcnt / lgkmcnt left at their maxima), gfx9 encoding
+#define PHF_WAIT_VMCNT(v) __builtin_amdgcn_s_waitcnt(((v) & 0xF) | ((((v) >> 4) & 3) << 14) | (0x7 << 4) | (0xF << 8))
+
+// 16-bit inputs, 16x16x32 MFMA.  Workgroup tile BM x BN (rows of P' x rows of
+// A'), waves of WM x WN arranged (BM/WM) x (BN/WN).  Operands stream global ->
+// LDS with global_load_lds_dwordx4 (no staging registers) through a ring of
+// STAGES k-slabs of 64 elements; a slab row is 128 B = 8 chunks of 16 B, eight
+// rows form a 1 KB block written by one wave instruction, and chunk c of row r
+// sits at slot c ^ ((r >> 1) & 7) so the 16 rows of an MFMA fragment read hit
+// 16 distinct bank groups.  Every FOLD slabs the FP32 MFMA tile is added into
+// an FP64 tile and restarted, keeping the FP32 rounding of the long
+// contraction at the level of a FOLD*64 chunk.  SPLITK = 2 halves the
 // contraction per workgroup and adds the two FP64 partials with atomics (two
 // commutative additions onto zero: the result does not depend on their order).
-template <typename T, int BM, int SPLITK>
-__global__ __launch_bounds__(BM * 2) void eri_gemm_h16_kernel(const T* __restrict__ P, const T* __restrict__ A,
-                                                             const double* __restrict__ invP,
-                                                             const double* __restrict__ invA,
-                                                             double* __restrict__ C, int K, long ldc, int tiles_m) {
-  constexpr int BN = 128, BK = 128, LDH = BK + 8;
-  constexpr int NT = BM * 2;                  // threads
-  constexpr int WM = 64, WN = 64, FM = 4, FN = 4;
-  constexpr int ROWS = BM + BN;               // rows staged per slab
-  constexpr int RPP = NT / 16;                // rows per pass: 16 threads x 16 B per 128-element row
-  constexpr int NLD = ROWS / RPP;             // 16-byte loads per thread per slab
-  static_assert(NLD <= 16 && NLD * RPP == ROWS, "staging registers");
+// FOLD64: fold the FP32 MFMA tile into an FP64 tile (else into an FP32 master
+// tile: half the registers, used by the large per-wave tiles).
+template <int BM, int BN, int WM, int WN, int STAGES_, int MINB_, bool FOLD64_>
+struct H16Cfg {
+  static constexpr int NW = (BM / WM) * (BN / WN);
+  static constexpr int NT = NW * 64;
+  static constexpr int ROWS = BM + BN;
+  static constexpr int STAGE_BYTES = ROWS * 64 * 2;
+  static constexpr int STAGES = STAGES_;
+  static constexpr int LDS = STAGES * STAGE_BYTES;
+  static constexpr int MINB = MINB_;           // workgroups per CU the registers must allow
+  static constexpr bool FOLD64 = FOLD64_;
+  static_assert(LDS * MINB <= 160 * 1024, "LDS budget");
+};
+
+template <typename T, typename Cfg, int BM, int BN, int WM, int WN, int SPLITK>
+__global__ __launch_bounds__((Cfg::NT), (Cfg::MINB))
+void eri_gemm_h16_kernel(const T* __restrict__ P, const T* __restrict__ A, const double* __restrict__ invP,
+                         const double* __restrict__ invA, double* __restrict__ C, int K, long ldc, int tiles_m) {
+  constexpr int BK = 64, FOLD = 8;
+  constexpr int STAGES = Cfg::STAGES, NW = Cfg::NW, ROWS = Cfg::ROWS, STAGE_BYTES = Cfg::STAGE_BYTES;
+  constexpr int FM = WM / 16, FN = WN / 16, WGN = BN / WN;
+  constexpr int BLOCKS = ROWS / 8;            // 1 KB blocks per slab
+  constexpr int BPW = BLOCKS / NW;            // blocks (load instructions) per wave per slab
+  static_assert(BPW * NW == BLOCKS && BPW < 64, "blocks per wave");
   typedef typename MfmaH<T>::v8 v8;
-  __shared__ __attribute__((aligned(16))) T sm[ROWS * LDH];   // rows [0, BM): P tile, [BM, BM+BN): A tile
+  extern __shared__ __attribute__((aligned(1024))) unsigned char smem[];
 
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   const int ks_id = t % SPLITK;
   const int tile = t / SPLITK;
   const int mt = tile % tiles_m, nt = tile / tiles_m;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave / WGN, wn = wave % WGN;
   const int kpart = K / SPLITK;
   const int kbeg = ks_id * kpart;
-  const int lr = tid >> 4, lc = (tid & 15) * 8;
+  const int nk = kpart / BK;
 
-  // global row pointer of staged row r: P rows of this tile, then A rows
-  auto grow = [&](int r) -> const T* {
-    return (r < BM) ? P + ((long)mt * BM + r) * K : A + ((long)nt * BN + (r - BM)) * K;
+  const T* src[BPW];
+#pragma unroll
+  for (int q = 0; q < BPW; ++q) {
+    const int b = wave * BPW + q;
+    const int r = 8 * b + (lane >> 3);
+    const int c = (lane & 7) ^ ((r >> 1) & 7);
+    src[q] = ((r < BM) ? P + ((long)mt * BM + r) * K : A + ((long)nt * BN + (r - BM)) * K) + kbeg + c * 8;
+  }
+  auto issue = [&](int slab) {
+    unsigned char* st = smem + (slab % STAGES) * STAGE_BYTES;
+#pragma unroll
+    for (int q = 0; q < BPW; ++q)
+      __builtin_amdgcn_global_load_lds((glb_void_t*)(src[q] + slab * BK),
+                                       (lds_void_t*)(st + (wave * BPW + q) * 1024), 16, 0, 0);
+  };
+  auto frag = [&](const unsigned char* st, int r, int c) -> v8 {
+    return *reinterpret_cast<const v8*>(st + (r >> 3) * 1024 + ((r & 7) * 8 + (c ^ ((r >> 1) & 7))) * 16);
   };
 
+  typedef typename std::conditional<Cfg::FOLD64, d4_t, f4_t>::type facc_t;
   f4_t acc[FM][FN];
+  facc_t facc[FM][FN];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < FN; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
-
-  uint4 st0, st1, st2, st3, st4, st5, st6, st7, st8, st9, st10, st11, st12, st13, st14, st15;
-#define H_LD(dst, e, k0) \
-  if constexpr (NLD > (e)) dst = *reinterpret_cast<const uint4*>(grow(lr + RPP * (e)) + (k0) + lc)
-#define H_ST(e, src) \
-  if constexpr (NLD > (e)) *reinterpret_cast<uint4*>(&sm[(lr + RPP * (e)) * LDH + lc]) = src
-#define H_GLOAD(k0)                                                               \
-  do {                                                                            \
-    H_LD(st0, 0, k0); H_LD(st1, 1, k0); H_LD(st2, 2, k0); H_LD(st3, 3, k0);       \
-    H_LD(st4, 4, k0); H_LD(st5, 5, k0); H_LD(st6, 6, k0); H_LD(st7, 7, k0);       \
-    H_LD(st8, 8, k0); H_LD(st9, 9, k0); H_LD(st10, 10, k0); H_LD(st11, 11, k0);   \
-    H_LD(st12, 12, k0); H_LD(st13, 13, k0); H_LD(st14, 14, k0); H_LD(st15, 15, k0); \
-  } while (0)
-#define H_LSTORE()                                                                \
-  do {                                                                            \
-    H_ST(0, st0); H_ST(1, st1); H_ST(2, st2); H_ST(3, st3);                       \
-    H_ST(4, st4); H_ST(5, st5); H_ST(6, st6); H_ST(7, st7);                       \
-    H_ST(8, st8); H_ST(9, st9); H_ST(10, st10); H_ST(11, st11);                   \
-    H_ST(12, st12); H_ST(13, st13); H_ST(14, st14); H_ST(15, st15);               \
-  } while (0)
-
-  const int nk = kpart / BK;
-  H_GLOAD(kbeg);
-  for (int kt = 0; kt < nk; ++kt) {
-    H_LSTORE();
-    __syncthreads();
-    if (kt + 1 < nk) H_GLOAD(kbeg + (kt + 1) * BK);
-#pragma unroll
-    for (int ks = 0; ks < BK; ks += 32) {
-      v8 a[FM], b[FN];
-#pragma unroll
-      for (int i = 0; i < FM; ++i)
-        a[i] = *reinterpret_cast<const v8*>(&sm[(wm * WM + i * 16 + (lane & 15)) * LDH + ks + 8 * (lane >> 4)]);
-#pragma unroll
-      for (int j = 0; j < FN; ++j)
-        b[j] = *reinterpret_cast<const v8*>(&sm[(BM + wn * WN + j * 16 + (lane & 15)) * LDH + ks + 8 * (lane >> 4)]);
-#pragma unroll
-      for (int i = 0; i < FM; ++i)
-#pragma unroll
-        for (int j = 0; j < FN; ++j) acc[i][j] = MfmaH<T>::mma(a[i], b[j], acc[i][j]);
+    for (int j = 0; j < FN; ++j) {
+      acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+      facc[i][j] = facc_t{0, 0, 0, 0};
     }
-    __syncthreads();
+
+#pragma unroll
+  for (int sl = 0; sl < STAGES - 1; ++sl)
+    if (sl < nk) issue(sl);
+  for (int kt = 0; kt < nk; ++kt) {
+    // own slab-kt loads have landed; the barrier then publishes every wave's
+    // slab kt and retires every wave's reads of the stage about to be refilled
+    if (STAGES == 3 && kt + 1 < nk) PHF_WAIT_VMCNT(BPW);
+    else PHF_WAIT_VMCNT(0);
+    __builtin_amdgcn_s_barrier();
+    if (kt + STAGES - 1 < nk) issue(kt + STAGES - 1);
+    const unsigned char* st = smem + (kt % STAGES) * STAGE_BYTES;
+#pragma unroll
+    for (int ks = 0; ks < BK / 32; ++ks) {
+      v8 a[FM];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) a[i] = frag(st, wm * WM + i * 16 + (lane & 15), ks * 4 + (lane >> 4));
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const v8 b = frag(st, BM + wn * WN + j * 16 + (lane & 15), ks * 4 + (lane >> 4));
+#pragma unroll
+        for (int i = 0; i < FM; ++i) acc[i][j] = MfmaH<T>::mma(a[i], b, acc[i][j]);
+      }
+    }
+    if ((kt + 1) % FOLD == 0 || kt + 1 == nk) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) facc[i][j][r] += acc[i][j][r];
+          acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+        }
+    }
   }
-#undef H_LD
-#undef H_ST
-#undef H_GLOAD
-#undef H_LSTORE
 
 #pragma unroll
   for (int i = 0; i < FM; ++i)
@@ -128,7 +161,7 @@ __global__ __launch_bounds__(BM * 2) void eri_gemm_h16_kernel(const T* __restric
       for (int r = 0; r < 4; ++r) {
         const long row = (long)mt * BM + wm * WM + i * 16 + 4 * (lane >> 4) + r;
         const long col = (long)nt * BN + wn * WN + j * 16 + (lane & 15);
-        const double v = (double)acc[i][j][r] * invP[row] * invA[col];
+        const double v = (double)facc[i][j][r] * invP[row] * invA[col];
         if constexpr (SPLITK == 1)
           C[row * ldc + col] = v;
         else
@@ -305,42 +338,63 @@ void convert_f32(hipStream_t s, const double* src, float* dst, long n) {
   to_f32_kernel<<<blocks, 256, 0, s>>>(src, dst, n);
 }
 
-// Tile height (grid-point rows) and split-K of the 16-bit kernel for a problem
-// of Mp0 rows, Nr columns on ncu compute units: fewest padded MFMA cycles per
-// CU, counting whole rounds of workgroups (one 256-row or two 128-row
-// workgroups per CU).
-void mixed_tile_choice(long Mp0, int Nr, int ncu, int* bm, int* splitk) {
-  double best = 1e300;
-  for (int b : {128, 256})
-    for (int sk : {1, 2}) {
-      const long mp = (Mp0 + b - 1) / b * b;
-      const long units = (mp / b) * (Nr / 128) * sk;
-      const int per_round = (b == 256) ? ncu : 2 * ncu;
-      const long rounds = (units + per_round - 1) / per_round;
-      const double cost = (double)rounds * b / sk * (b == 256 ? 1.0 : 0.5) + (sk == 2 ? 0.02 * rounds : 0.0);
-      if (cost < best - 1e-9) { best = cost; *bm = b; *splitk = sk; }
-    }
+// Tile configurations of the 16-bit kernel: {BM, BN, WM, WN}.
+static const int kH16Cfgs[][4] = {{128, 128, 64, 64}, {256, 128, 64, 64}, {256, 128, 128, 64}, {128, 256, 64, 128}};
+int h16_num_configs() { return 4; }
+int h16_cfg_bm(int c) { return kH16Cfgs[c][0]; }
+int h16_cfg_bn(int c) { return kH16Cfgs[c][1]; }
+
+// Cost model used to order the autotuner's candidates (and to choose when it
+// is off): MFMA rounds over the CUs for cfg c and split-K sk, counting padded
+// rows, normalised to one CU's work per round.
+double h16_model_cost(int c, int sk, long Mp0, int Nr0, int ncu) {
+  const int bm = kH16Cfgs[c][0], bn = kH16Cfgs[c][1];
+  const long mp = (Mp0 + bm - 1) / bm * bm;
+  const long np = (Nr0 + bn - 1) / bn * bn;
+  const long units = (mp / bm) * (np / bn) * sk;
+  const int per_cu = (bm == 128 && bn == 128) ? 2 : 1;
+  const long rounds = (units + (long)per_cu * ncu - 1) / ((long)per_cu * ncu);
+  return (double)rounds * per_cu * bm * bn / sk;
 }
 
-// Mp multiple of bm, Nr multiple of 128; Kseg = nseg * kt multiple of 128 * splitk
-// (16-bit) or 32 (FP32).  With splitk = 2 the caller zero-fills C first.
+// Mp multiple of the config's BM, Nr multiple of its BN; Kseg = nseg * kt
+// multiple of 64 * splitk (16-bit) or 32 (FP32).  With splitk = 2 the caller
+// zero-fills C first.
 void eri_gemm_mixed(hipStream_t s, int prec, const void* P, const void* A, const double* invP, const double* invA,
-                    double* C, int Mp, int Nr, int Kseg, long ldc, int bm, int splitk) {
+                    double* C, int Mp, int Nr, int Kseg, long ldc, int cfg, int splitk) {
   if (prec == 1) {
     const int tm = Mp / 128, tn = Nr / 128;
     eri_gemm_f32_kernel<16><<<tm * tn, 256, 0, s>>>((const float*)P, (const float*)A, C, Kseg, ldc, tm);
     return;
   }
-  const int tm = Mp / bm, tn = Nr / 128;
+  const int bm = kH16Cfgs[cfg][0], bn = kH16Cfgs[cfg][1];
+  const int tm = Mp / bm, tn = Nr / bn;
   const unsigned grid = (unsigned)(tm * tn * splitk);
-#define H_LAUNCH(TT, BMV, SK) \
-  eri_gemm_h16_kernel<TT, BMV, SK><<<grid, BMV * 2, 0, s>>>((const TT*)P, (const TT*)A, invP, invA, C, Kseg, ldc, tm)
+#define H_LAUNCH(TT, B0, B1, W0, W1, ST, MB, F64, SK)                                                         \
+  do {                                                                                                       \
+    typedef H16Cfg<B0, B1, W0, W1, ST, MB, F64> CC;                                                          \
+    auto kern = eri_gemm_h16_kernel<TT, CC, B0, B1, W0, W1, SK>;                                             \
+    static bool attr_set = false;                                                                            \
+    if (!attr_set) {                                                                                         \
+      (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, CC::LDS);     \
+      attr_set = true;                                                                                       \
+    }                                                                                                        \
+    kern<<<grid, CC::NT, CC::LDS, s>>>((const TT*)P, (const TT*)A, invP, invA, C, Kseg, ldc, tm);            \
+  } while (0)
+#define H_CFGS(TT, SK)                                                   \
+  do {                                                                   \
+    switch (cfg) {                                                       \
+      case 0: H_LAUNCH(TT, 128, 128, 64, 64, 2, 2, true, SK); break;     \
+      case 1: H_LAUNCH(TT, 256, 128, 64, 64, 3, 1, true, SK); break;     \
+      case 2: H_LAUNCH(TT, 256, 128, 128, 64, 3, 1, false, SK); break;   \
+      default: H_LAUNCH(TT, 128, 256, 64, 128, 3, 1, false, SK); break;  \
+    }                                                                    \
+  } while (0)
   if (prec == 2) {
-    if (bm == 256) { if (splitk == 2) H_LAUNCH(__bf16, 256, 2); else H_LAUNCH(__bf16, 256, 1); }
-    else { if (splitk == 2) H_LAUNCH(__bf16, 128, 2); else H_LAUNCH(__bf16, 128, 1); }
+    if (splitk == 2) H_CFGS(__bf16, 2); else H_CFGS(__bf16, 1);
   } else {
-    if (bm == 256) { if (splitk == 2) H_LAUNCH(_Float16, 256, 2); else H_LAUNCH(_Float16, 256, 1); }
-    else { if (splitk == 2) H_LAUNCH(_Float16, 128, 2); else H_LAUNCH(_Float16, 128, 1); }
+    if (splitk == 2) H_CFGS(_Float16, 2); else H_CFGS(_Float16, 1);
   }
+#undef H_CFGS
 #undef H_LAUNCH
 }

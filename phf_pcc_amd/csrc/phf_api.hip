@@ -23,6 +23,8 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -40,8 +42,11 @@ void split_operand(hipStream_t s, int prec, bool is_a, const double* src, long l
                    double* inv);
 void convert_f32(hipStream_t s, const double* src, float* dst, long n);
 void eri_gemm_mixed(hipStream_t s, int prec, const void* P, const void* A, const double* invP, const double* invA,
-                    double* C, int Mp, int Nr, int Kseg, long ldc, int bm, int splitk);
-void mixed_tile_choice(long Mp0, int Nr, int ncu, int* bm, int* splitk);
+                    double* C, int Mp, int Nr, int Kseg, long ldc, int cfg, int splitk);
+int h16_num_configs();
+int h16_cfg_bm(int c);
+int h16_cfg_bn(int c);
+double h16_model_cost(int c, int sk, long Mp0, int Nr0, int ncu);
 
 struct phf_ctx_s {
   int device = 0;
@@ -70,6 +75,10 @@ struct phf_ctx_s {
   double* dGr2 = nullptr;      // R2[iy]                  [ny] row-major (as given)
   double* dGrpgref = nullptr;  // Rpg[ipg] Ref            [npg][nso x nso] column-major
   double* dGw = nullptr;       // weight x Wigner x weightpg  [G][nci][nci]
+  // autotuned ERI GEMM tiles: (precision, rows, cols, K) -> (config, split-K)
+  int autotune = 1;
+  std::map<std::tuple<int, long, int, int>, std::pair<int, int>> tuned;
+  std::string tune_log;
   // per-stage timing with HIP events on `stream` (phf_set_profiling)
   bool profiling = false;
   hipEvent_t ev[8] = {};
@@ -451,7 +460,7 @@ extern "C" int phf_set_hamiltonian(phf_ctx* ctx, const double* hone, const doubl
   ctx->cplx = hflag != 0 || (flags & 1);
   ctx->nso = nso;
   ctx->kpad = (int)round_up(n2, 256);
-  ctx->nrpad = (int)round_up(n2, 128);
+  ctx->nrpad = (int)round_up(n2, 256);
   ctx->kt = ctx->cplx ? 2 * ctx->kpad : ctx->kpad;
   const long asz = (long)ctx->nrpad * ctx->kt;
   hipError_t e = hipMalloc(&ctx->dA, sizeof(double) * asz);
@@ -586,6 +595,77 @@ extern "C" int phf_set_grid(phf_ctx* ctx, const double* ref, const int* ngrid, i
   return PHF_OK;
 }
 
+// Choose the 16-bit ERI GEMM tile configuration and split-K for one problem
+// shape: with autotuning on, every candidate runs on the live operands and the
+// fastest (HIP events, best of two after a warm-up) wins; otherwise the cost
+// model's choice.  The large-tile configurations fold into FP32 and are only
+// candidates when PHF_H16_ALLOW_F32FOLD=1.
+template <class Run>
+static int tune_h16(phf_ctx* ctx, const std::tuple<int, long, int, int>& key, long rows, int nrpad, Run&& run) {
+  const bool allow_f32fold = getenv("PHF_H16_ALLOW_F32FOLD") && atoi(getenv("PHF_H16_ALLOW_F32FOLD")) == 1;
+  std::vector<std::pair<int, int>> cands;
+  for (int c = 0; c < h16_num_configs(); ++c) {
+    if (c >= 2 && !allow_f32fold) continue;
+    if (nrpad % h16_cfg_bn(c) != 0) continue;
+    for (int sk : {1, 2}) cands.push_back({c, sk});
+  }
+  std::sort(cands.begin(), cands.end(), [&](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+    return h16_model_cost(a.first, a.second, rows, nrpad, ctx->ncu) <
+           h16_model_cost(b.first, b.second, rows, nrpad, ctx->ncu);
+  });
+  std::pair<int, int> best = cands.front();
+  char line[256];
+  snprintf(line, sizeof line, "{\"prec\": %d, \"rows\": %ld, \"cols\": %d, \"k\": %d, \"candidates\": [",
+           std::get<0>(key), rows, nrpad, std::get<3>(key));
+  std::string log = line;
+  if (ctx->autotune) {
+    hipEvent_t e0, e1;
+    HIPCK(hipEventCreate(&e0));
+    HIPCK(hipEventCreate(&e1));
+    float best_ms = 1e30f;
+    for (size_t q = 0; q < cands.size(); ++q) {
+      run(cands[q].first, cands[q].second);
+      float ms_min = 1e30f;
+      for (int rep = 0; rep < 2; ++rep) {
+        HIPCK(hipEventRecord(e0, ctx->stream));
+        run(cands[q].first, cands[q].second);
+        HIPCK(hipEventRecord(e1, ctx->stream));
+        HIPCK(hipEventSynchronize(e1));
+        float ms;
+        HIPCK(hipEventElapsedTime(&ms, e0, e1));
+        ms_min = std::min(ms_min, ms);
+      }
+      snprintf(line, sizeof line, "%s{\"cfg\": %d, \"splitk\": %d, \"ms\": %.4f}", q ? ", " : "",
+               cands[q].first, cands[q].second, ms_min);
+      log += line;
+      if (ms_min < best_ms) { best_ms = ms_min; best = cands[q]; }
+    }
+    HIPCK(hipEventDestroy(e0));
+    HIPCK(hipEventDestroy(e1));
+  }
+  snprintf(line, sizeof line, "], \"chosen\": {\"cfg\": %d, \"splitk\": %d}, \"timed\": %s}\n", best.first,
+           best.second, ctx->autotune ? "true" : "false");
+  log += line;
+  ctx->tune_log += log;
+  ctx->tuned[key] = best;
+  return PHF_OK;
+}
+
+extern "C" int phf_autotune(phf_ctx* ctx, int on) {
+  if (!ctx) return PHF_ERR_ARG;
+  ctx->autotune = on != 0;
+  ctx->tuned.clear();
+  return PHF_OK;
+}
+
+extern "C" int phf_tuning_report(const phf_ctx* ctx, char* buf, int len) {
+  if (!ctx || !buf || len <= 0) return 0;
+  const int n = std::min<int>(len - 1, (int)ctx->tune_log.size());
+  std::memcpy(buf, ctx->tune_log.data(), n);
+  buf[n] = 0;
+  return n;
+}
+
 extern "C" int phf_buildhsg_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int precision, double* hmat,
                               double* smat, double* gradmat, double* rdmmat) {
   if (!ctx) return PHF_ERR_ARG;
@@ -622,9 +702,8 @@ extern "C" int phf_buildhsg_z(phf_ctx* ctx, const double* z, int cmplxconj, int 
 
   // ---- workspace layout
   const int kt = ctx->kt, nrpad = ctx->nrpad;
-  int bm = 128, splitk = 1;
-  if (precision == PHF_PREC_FP64) bm = eri_gemm_f64_bm((int)(2 * G));
-  else if (precision != PHF_PREC_FP32) mixed_tile_choice(2 * G, nrpad, ctx->ncu, &bm, &splitk);
+  // rows are padded for the tallest tile any candidate uses
+  const int bm = (precision == PHF_PREC_FP64) ? eri_gemm_f64_bm((int)(2 * G)) : 256;
   const long Mp = round_up(2 * G, bm);
   const int gchunk = 32;
   const int nchunk = (int)((G + gchunk - 1) / gchunk);
@@ -721,9 +800,21 @@ extern "C" int phf_buildhsg_z(phf_ctx* ctx, const double* z, int cmplxconj, int 
     eri_gemm_mixed(s, precision, dPm, ctx->dAm[precision], nullptr, nullptr, dGam, (int)Mp, nrpad, kt, nrpad, 128, 1);
   } else {
     split_operand(s, precision, false, dPt, kt, (int)Mp, kt, dPm, dPinv);
-    if (splitk > 1) fill_zero(s, dGam, Mp * nrpad);
-    eri_gemm_mixed(s, precision, dPm, ctx->dAm[precision], dPinv, ctx->dAinv[precision], dGam, (int)Mp, nrpad,
-                   kt * mixed_segments(precision), nrpad, bm, splitk);
+    const int kseg = kt * mixed_segments(precision);
+    auto run = [&](int cfg, int sk) {
+      const int mp = (int)round_up(2 * G, h16_cfg_bm(cfg));
+      if (sk > 1) fill_zero(s, dGam, (long)mp * nrpad);
+      eri_gemm_mixed(s, precision, dPm, ctx->dAm[precision], dPinv, ctx->dAinv[precision], dGam, mp, nrpad, kseg,
+                     nrpad, cfg, sk);
+    };
+    const auto key = std::make_tuple(precision, 2 * G, nrpad, kseg);
+    auto it = ctx->tuned.find(key);
+    if (it == ctx->tuned.end()) {
+      int rc = tune_h16(ctx, key, 2 * G, nrpad, run);
+      if (rc) return rc;
+      it = ctx->tuned.find(key);
+    }
+    run(it->second.first, it->second.second);
   }
   if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[2], s));
   // K4: FX = h X + Gamma X

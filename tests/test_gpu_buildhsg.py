@@ -174,3 +174,36 @@ def test_repeated_calls_and_hamiltonian_switch(phftools):
     for case in (c1, c2, c1):
         gpu, cpu = _run_both(phftools, case, ga)
         _assert_close(gpu, cpu)
+
+
+def test_grid_cache_follows_grid_arguments(phftools):
+    """The device-resident grid is rebuilt when R1/R2/weights change."""
+    case = hubbard_general_ref(4, 4.0, 23, zscale=0.3)
+    for ngrid, j in [((14, 8), 0), ((26, 8), 1), ((14, 8), 0), ((50, 16), 0)]:
+        ga = grid_args(4, ngrid, j)
+        gpu, cpu = _run_both(phftools, case, ga)
+        _assert_close(gpu, cpu)
+
+
+def test_direct_c_abi_one_shot_matches_drop_in(phftools):
+    """phf_buildhsg (one call: grid upload + evaluation) through ctypes."""
+    from phf_pcc_amd import _lib
+    case = hubbard_general_ref(4, 4.0, 29, zscale=0.3)
+    ga = grid_args(4, (14, 8), 0)
+    gpu, cpu = _run_both(phftools, case, ga)
+    ctx = phftools.context()
+    lib = _lib.load()
+    C = lambda a, dt=np.complex128: np.ascontiguousarray(a, dtype=dt)
+    ngrid = np.ascontiguousarray(ga["ngrid"], dtype=np.int32)
+    out = [np.zeros((1, 1), complex), np.zeros((1, 1), complex), np.zeros((1, 1, 8, 8), complex),
+           np.zeros((1, 1, 8, 8), complex)]
+    rc = lib.phf_buildhsg(ctx.handle, _lib.ptr(C(case["ref"])), _lib.ptr(C(case["z"])), ngrid.ctypes.data_as(_lib._ip),
+                          1, 1, 1, 4, _lib.ptr(C(ga["r1"])), ga["r1"].shape[0], _lib.ptr(C(ga["r2"])), ga["r2"].shape[0],
+                          _lib.ptr(C(ga["rpg"])), 1, _lib.ptr(C(ga["weightsp"], np.float64)),
+                          _lib.ptr(C(ga["weightpg"], np.float64)), 1, _lib.ptr(C(ga["roota"], np.float64)),
+                          len(ga["roota"]), _lib.ptr(C(ga["rootb"], np.float64)), len(ga["rootb"]),
+                          _lib.ptr(C(ga["rooty"], np.float64)), len(ga["rooty"]), 0, 1, 0,
+                          *[_lib.ptr(o) for o in out])
+    _lib.check(rc, ctx.handle)
+    phftools.invalidate()
+    _assert_close(out, cpu)
