@@ -12,7 +12,9 @@
 // [P_re | -P_im] and [P_im | P_re].
 //
 // Layout contract: Pt is [Mp][K], A is [Nr][K], C is [Mp][ldc]; Mp % BM == 0,
-// Nr % BN == 0, K % 16 == 0 (the host pads with zeros).
+// Nr % BN == 0, K % 16 == 0 (the host pads with zeros).  An optional bias
+// [2][ldc] is added to even (real) and odd (imaginary) rows: the one-body h,
+// so that C holds h + Gamma for the Fock contraction that follows.
 #include "phf_common.h"
 
 namespace {
@@ -21,7 +23,7 @@ template <int BM, int BN>
 __global__ __launch_bounds__(256, 2) void eri_gemm_f64_kernel(const double* __restrict__ P,
                                                                const double* __restrict__ A,
                                                                double* __restrict__ C, int K, long ldc,
-                                                               int tiles_m) {
+                                                               int tiles_m, const double* __restrict__ bias) {
   constexpr int BK = 16, LDSW = BK + 2;
   constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
   constexpr int PL = BM / 32, AL = BN / 32;
@@ -109,7 +111,7 @@ __global__ __launch_bounds__(256, 2) void eri_gemm_f64_kernel(const double* __re
       for (int r = 0; r < 4; ++r) {
         const long row = (long)mt * BM + wm * WM + i * 16 + (lane >> 4) + 4 * r;   // f64 16x16x4 output row
         const long col = (long)nt * BN + wn * WN + j * 16 + (lane & 15);
-        C[row * ldc + col] = acc[i][j][r];
+        C[row * ldc + col] = acc[i][j][r] + (bias ? bias[(row & 1) * ldc + col] : 0.0);
       }
 }
 
@@ -119,12 +121,12 @@ __global__ __launch_bounds__(256, 2) void eri_gemm_f64_kernel(const double* __re
 int eri_gemm_f64_bm(int rows) { return rows <= 64 * 12 ? 64 : 128; }
 
 void eri_gemm_f64(hipStream_t s, const double* P, const double* A, double* C, int Mp, int Nr, int K,
-                  long ldc, int bm) {
+                  long ldc, int bm, const double* bias) {
   if (bm == 64) {
     const int tm = Mp / 64, tn = Nr / 128;
-    eri_gemm_f64_kernel<64, 128><<<tm * tn, 256, 0, s>>>(P, A, C, K, ldc, tm);
+    eri_gemm_f64_kernel<64, 128><<<tm * tn, 256, 0, s>>>(P, A, C, K, ldc, tm, bias);
   } else {
     const int tm = Mp / 128, tn = Nr / 128;
-    eri_gemm_f64_kernel<128, 128><<<tm * tn, 256, 0, s>>>(P, A, C, K, ldc, tm);
+    eri_gemm_f64_kernel<128, 128><<<tm * tn, 256, 0, s>>>(P, A, C, K, ldc, tm, bias);
   }
 }

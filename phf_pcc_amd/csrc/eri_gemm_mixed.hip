@@ -68,7 +68,8 @@ struct H16Cfg {
 template <typename T, typename Cfg, int BM, int BN, int WM, int WN, int SPLITK>
 __global__ __launch_bounds__((Cfg::NT), (Cfg::MINB))
 void eri_gemm_h16_kernel(const T* __restrict__ P, const T* __restrict__ A, const double* __restrict__ invP,
-                         const double* __restrict__ invA, double* __restrict__ C, int K, long ldc, int tiles_m) {
+                         const double* __restrict__ invA, double* __restrict__ C, int K, long ldc, int tiles_m,
+                         const double* __restrict__ bias) {
   constexpr int BK = 64, FOLD = 8;
   constexpr int STAGES = Cfg::STAGES, NW = Cfg::NW, ROWS = Cfg::ROWS, STAGE_BYTES = Cfg::STAGE_BYTES;
   constexpr int FM = WM / 16, FN = WN / 16, WGN = BN / WN;
@@ -161,7 +162,8 @@ void eri_gemm_h16_kernel(const T* __restrict__ P, const T* __restrict__ A, const
       for (int r = 0; r < 4; ++r) {
         const long row = (long)mt * BM + wm * WM + i * 16 + 4 * (lane >> 4) + r;
         const long col = (long)nt * BN + wn * WN + j * 16 + (lane & 15);
-        const double v = (double)facc[i][j][r] * invP[row] * invA[col];
+        double v = (double)facc[i][j][r] * invP[row] * invA[col];
+        if (bias && ks_id == 0) v += bias[(row & 1) * ldc + col];
         if constexpr (SPLITK == 1)
           C[row * ldc + col] = v;
         else
@@ -172,7 +174,8 @@ void eri_gemm_h16_kernel(const T* __restrict__ P, const T* __restrict__ A, const
 // FP32 inputs, v_mfma_f32_16x16x4f32, 128 x 128 tile, k-slab 32, FP64 fold every FLUSH slabs.
 template <int FLUSH>
 __global__ __launch_bounds__(256, 1) void eri_gemm_f32_kernel(const float* __restrict__ P, const float* __restrict__ A,
-                                                              double* __restrict__ C, int K, long ldc, int tiles_m) {
+                                                              double* __restrict__ C, int K, long ldc, int tiles_m,
+                                                              const double* __restrict__ bias) {
   constexpr int BM = 128, BN = 128, BK = 32, LDF = BK + 4;
   constexpr int WM = 64, WN = 64, FM = 4, FN = 4;
   __shared__ __attribute__((aligned(16))) float sP[2][BM * LDF];
@@ -253,7 +256,8 @@ __global__ __launch_bounds__(256, 1) void eri_gemm_f32_kernel(const float* __res
       for (int r = 0; r < 4; ++r) {
         const long row = (long)mt * BM + wm * WM + i * 16 + 4 * (lane >> 4) + r;
         const long col = (long)nt * BN + wn * WN + j * 16 + (lane & 15);
-        C[row * ldc + col] = (FLUSH > 0) ? facc[i][j][r] : (double)acc[i][j][r];
+        C[row * ldc + col] = ((FLUSH > 0) ? facc[i][j][r] : (double)acc[i][j][r]) +
+                             (bias ? bias[(row & 1) * ldc + col] : 0.0);
       }
 }
 
@@ -361,10 +365,10 @@ double h16_model_cost(int c, int sk, long Mp0, int Nr0, int ncu) {
 // multiple of 64 * splitk (16-bit) or 32 (FP32).  With splitk = 2 the caller
 // zero-fills C first.
 void eri_gemm_mixed(hipStream_t s, int prec, const void* P, const void* A, const double* invP, const double* invA,
-                    double* C, int Mp, int Nr, int Kseg, long ldc, int cfg, int splitk) {
+                    double* C, int Mp, int Nr, int Kseg, long ldc, int cfg, int splitk, const double* bias) {
   if (prec == 1) {
     const int tm = Mp / 128, tn = Nr / 128;
-    eri_gemm_f32_kernel<16><<<tm * tn, 256, 0, s>>>((const float*)P, (const float*)A, C, Kseg, ldc, tm);
+    eri_gemm_f32_kernel<16><<<tm * tn, 256, 0, s>>>((const float*)P, (const float*)A, C, Kseg, ldc, tm, bias);
     return;
   }
   const int bm = kH16Cfgs[cfg][0], bn = kH16Cfgs[cfg][1];
@@ -379,7 +383,7 @@ void eri_gemm_mixed(hipStream_t s, int prec, const void* P, const void* A, const
       (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, CC::LDS);     \
       attr_set = true;                                                                                       \
     }                                                                                                        \
-    kern<<<grid, CC::NT, CC::LDS, s>>>((const TT*)P, (const TT*)A, invP, invA, C, Kseg, ldc, tm);            \
+    kern<<<grid, CC::NT, CC::LDS, s>>>((const TT*)P, (const TT*)A, invP, invA, C, Kseg, ldc, tm, bias);            \
   } while (0)
 #define H_CFGS(TT, SK)                                                   \
   do {                                                                   \

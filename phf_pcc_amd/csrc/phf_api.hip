@@ -35,14 +35,14 @@ void lu_inverse_batched(hipStream_t s, int n, const double* M, long mbs, double*
                         double* det, int* info, int batch);
 int lu_inverse_max_n();
 void eri_gemm_f64(hipStream_t s, const double* P, const double* A, double* C, int Mp, int Nr, int K,
-                  long ldc, int bm);
+                  long ldc, int bm, const double* bias);
 int eri_gemm_f64_bm(int rows);
 int mixed_segments(int prec);
 void split_operand(hipStream_t s, int prec, bool is_a, const double* src, long lds, int rows, int kt, void* dst,
                    double* inv);
 void convert_f32(hipStream_t s, const double* src, float* dst, long n);
 void eri_gemm_mixed(hipStream_t s, int prec, const void* P, const void* A, const double* invP, const double* invA,
-                    double* C, int Mp, int Nr, int Kseg, long ldc, int cfg, int splitk);
+                    double* C, int Mp, int Nr, int Kseg, long ldc, int cfg, int splitk, const double* bias);
 int h16_num_configs();
 int h16_cfg_bm(int c);
 int h16_cfg_bn(int c);
@@ -60,6 +60,7 @@ struct phf_ctx_s {
   int kpad = 0, nrpad = 0, kt = 0;
   double* dA = nullptr;     // [nrpad][kt] FP64
   double* dH1 = nullptr;    // hone, row-major complex128
+  double* dHb = nullptr;    // hone as ERI-GEMM bias: [2][nrpad] (re plane, im plane), r = i*nso + k
   // reduced-precision copies of A, built on first use per mode:
   // [nrpad][nseg*kt] of FP32 / BF16 / FP16 and per-row inverse scales
   void* dAm[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -395,6 +396,7 @@ extern "C" int phf_ctx_destroy(phf_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   if (ctx->dA) (void)hipFree(ctx->dA);
   if (ctx->dH1) (void)hipFree(ctx->dH1);
+  if (ctx->dHb) (void)hipFree(ctx->dHb);
   for (int q = 0; q < 4; ++q) {
     if (ctx->dAm[q]) (void)hipFree(ctx->dAm[q]);
     if (ctx->dAinv[q]) (void)hipFree(ctx->dAinv[q]);
@@ -442,6 +444,7 @@ extern "C" int phf_set_hamiltonian(phf_ctx* ctx, const double* hone, const doubl
   }
   if (ctx->dA) { HIPCK(hipFree(ctx->dA)); ctx->dA = nullptr; }
   if (ctx->dH1) { HIPCK(hipFree(ctx->dH1)); ctx->dH1 = nullptr; }
+  if (ctx->dHb) { HIPCK(hipFree(ctx->dHb)); ctx->dHb = nullptr; }
   const long n2 = (long)nso * nso;
   const long n4 = n2 * n2;
   HIPCK(hipMalloc(&ctx->dH1, sizeof(double) * 2 * n2));
@@ -471,6 +474,16 @@ extern "C" int phf_set_hamiltonian(phf_ctx* ctx, const double* hone, const doubl
     return fail(ctx, PHF_ERR_NOMEM, std::string("phf_set_hamiltonian: ") + hipGetErrorString(e));
   }
   ham_permute_kernel<<<65536, 256, 0, ctx->stream>>>(stage, ctx->dA, nso, ctx->kpad, ctx->kt, ctx->nrpad);
+  {
+    std::vector<double> hb(2 * (size_t)ctx->nrpad, 0.0);
+    for (long r = 0; r < n2; ++r) {
+      hb[r] = hone[2 * r];
+      hb[ctx->nrpad + r] = hone[2 * r + 1];
+    }
+    HIPCK(hipMalloc(&ctx->dHb, sizeof(double) * hb.size()));
+    HIPCK(hipMemcpyAsync(ctx->dHb, hb.data(), sizeof(double) * hb.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCK(hipStreamSynchronize(ctx->stream));
+  }
   HIPCK(hipGetLastError());
   HIPCK(hipStreamSynchronize(ctx->stream));
   HIPCK(hipFree(stage));
@@ -794,10 +807,11 @@ extern "C" int phf_buildhsg_z(phf_ctx* ctx, const double* z, int cmplxconj, int 
   // K3: Gamma for all grid points
   if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[1], s));
   if (precision == PHF_PREC_FP64) {
-    eri_gemm_f64(s, dPt, ctx->dA, dGam, (int)Mp, nrpad, kt, nrpad, bm);
+    eri_gemm_f64(s, dPt, ctx->dA, dGam, (int)Mp, nrpad, kt, nrpad, bm, ctx->dHb);
   } else if (precision == PHF_PREC_FP32) {
     convert_f32(s, dPt, (float*)dPm, Mp * kt);
-    eri_gemm_mixed(s, precision, dPm, ctx->dAm[precision], nullptr, nullptr, dGam, (int)Mp, nrpad, kt, nrpad, 128, 1);
+    eri_gemm_mixed(s, precision, dPm, ctx->dAm[precision], nullptr, nullptr, dGam, (int)Mp, nrpad, kt, nrpad, 128, 1,
+                   ctx->dHb);
   } else {
     split_operand(s, precision, false, dPt, kt, (int)Mp, kt, dPm, dPinv);
     const int kseg = kt * mixed_segments(precision);
@@ -805,7 +819,7 @@ extern "C" int phf_buildhsg_z(phf_ctx* ctx, const double* z, int cmplxconj, int 
       const int mp = (int)round_up(2 * G, h16_cfg_bm(cfg));
       if (sk > 1) fill_zero(s, dGam, (long)mp * nrpad);
       eri_gemm_mixed(s, precision, dPm, ctx->dAm[precision], dPinv, ctx->dAinv[precision], dGam, mp, nrpad, kseg,
-                     nrpad, cfg, sk);
+                     nrpad, cfg, sk, ctx->dHb);
     };
     const auto key = std::make_tuple(precision, 2 * G, nrpad, kseg);
     auto it = ctx->tuned.find(key);
@@ -817,10 +831,8 @@ extern "C" int phf_buildhsg_z(phf_ctx* ctx, const double* z, int cmplxconj, int 
     run(it->second.first, it->second.second);
   }
   if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[2], s));
-  // K4: FX = h X + Gamma X
-  zgemm_batched(s, nso, N, nso, 1, 0, ZMat{ctx->dH1, 2L * nso, 2, 0, 1, 0}, zmat_colmajor(dX, nso, sT), 0, 0,
-                zout_colmajor(dFX, nso, sT), (int)G);
-  zgemm_batched(s, nso, N, nso, 1, 0, ZMat{dGam, nso, 1, 2L * nrpad, nrpad, 0}, zmat_colmajor(dX, nso, sT), 1, 0,
+  // K4: FX = (h + Gamma) X   (the ERI GEMM epilogue added h)
+  zgemm_batched(s, nso, N, nso, 1, 0, ZMat{dGam, nso, 1, 2L * nrpad, nrpad, 0}, zmat_colmajor(dX, nso, sT), 0, 0,
                 zout_colmajor(dFX, nso, sT), (int)G);
   // Q_g = C0^H FX_g ; Hc = C0^H h
   zgemm_batched(s, N, N, nso, 1, 0, ZMat{dC0, 2L * nso, 2, 0, 1, 1}, zmat_colmajor(dFX, nso, sT), 0, 0,

@@ -8,17 +8,21 @@
 // transition density rho^T = C0* X^T, the Fock contraction (h + Gamma) X and
 // the final Y C0^H.
 //
-// Tile: 64 x 64 per 256-thread workgroup, 2 x 2 waves of 32 x 32, k-slab 16.
+// Tile: (32 FMW) x (32 FNW) per 256-thread workgroup, 2 x 2 waves of
+// (16 FMW) x (16 FNW), k-slab 16; the launcher takes FMW/FNW = 1 for sides of
+// at most 32 (the N = nocc sides) so no MFMA work is spent on padding.
 // Complex product = four real MFMAs into separate re/im accumulators.
 #include "phf_common.h"
 
 namespace {
 
-constexpr int ZBM = 64, ZBN = 64, ZBK = 16, ZLD = ZBK + 2;
+constexpr int ZBK = 16, ZLD = ZBK + 2;
 
+template <int FMW, int FNW>
 __global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double ar, double ai,
                                                     ZMat A, ZMat B, double br, double bi, ZOut C,
                                                     int a_kfast, int b_kfast) {
+  constexpr int ZBM = 32 * FMW, ZBN = 32 * FNW;
   __shared__ double As_re[ZBM * ZLD], As_im[ZBM * ZLD];
   __shared__ double Bs_re[ZBN * ZLD], Bs_im[ZBN * ZLD];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -28,11 +32,11 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double 
   const double* Ap = A.p + b * A.bs;
   const double* Bp = B.p + b * B.bs;
 
-  d4_t cre[2][2], cim[2][2];
+  d4_t cre[FMW][FNW], cim[FMW][FNW];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < FMW; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < FNW; ++j) {
       cre[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
       cim[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
     }
@@ -75,20 +79,23 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double 
     __syncthreads();
 #pragma unroll
     for (int ks = 0; ks < ZBK; ks += 4) {
-      double are[2], aim[2], bre[2], bim[2];
+      double are[FMW], aim[FMW], bre[FNW], bim[FNW];
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int row = wm * 32 + t * 16 + (lane & 15);
+      for (int t = 0; t < FMW; ++t) {
+        const int row = wm * 16 * FMW + t * 16 + (lane & 15);
         are[t] = As_re[row * ZLD + ks + (lane >> 4)];
         aim[t] = As_im[row * ZLD + ks + (lane >> 4)];
-        const int col = wn * 32 + t * 16 + (lane & 15);
+      }
+#pragma unroll
+      for (int t = 0; t < FNW; ++t) {
+        const int col = wn * 16 * FNW + t * 16 + (lane & 15);
         bre[t] = Bs_re[col * ZLD + ks + (lane >> 4)];
         bim[t] = Bs_im[col * ZLD + ks + (lane >> 4)];
       }
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < FMW; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < FNW; ++j) {
           cre[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bre[j], cre[i][j], 0, 0, 0);
           cre[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-aim[i], bim[j], cre[i][j], 0, 0, 0);
           cim[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bim[j], cim[i][j], 0, 0, 0);
@@ -100,13 +107,13 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double 
 
   const bool use_beta = (br != 0.0 || bi != 0.0);
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < FMW; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < FNW; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gi = m0 + wm * 32 + i * 16 + (lane >> 4) + 4 * r;   // f64 16x16x4: row = lane/16 + 4r
-        const int gj = n0 + wn * 32 + j * 16 + (lane & 15);
+        const int gi = m0 + wm * 16 * FMW + i * 16 + (lane >> 4) + 4 * r;   // f64 16x16x4: row = lane/16 + 4r
+        const int gj = n0 + wn * 16 * FNW + j * 16 + (lane & 15);
         if (gi < m && gj < n) {
           const double xr = cre[i][j][r], xi = cim[i][j][r];
           double vr = ar * xr - ai * xi, vi = ar * xi + ai * xr;
@@ -133,6 +140,14 @@ void zgemm_batched(hipStream_t s, int m, int n, int k, double ar, double ai, ZMa
   }
   const int a_kfast = (labs(A.cs) < labs(A.rs)) ? 1 : 0;
   const int b_kfast = (labs(B.rs) < labs(B.cs)) ? 1 : 0;
-  dim3 grid((m + ZBM - 1) / ZBM, (n + ZBN - 1) / ZBN, batch);
-  zgemm_kernel<<<grid, 256, 0, s>>>(m, n, k, ar, ai, A, B, br, bi, C, a_kfast, b_kfast);
+  const int fmw = (m <= 32) ? 1 : 2, fnw = (n <= 32) ? 1 : 2;
+  dim3 grid((m + 32 * fmw - 1) / (32 * fmw), (n + 32 * fnw - 1) / (32 * fnw), batch);
+  if (fmw == 2 && fnw == 2)
+    zgemm_kernel<2, 2><<<grid, 256, 0, s>>>(m, n, k, ar, ai, A, B, br, bi, C, a_kfast, b_kfast);
+  else if (fmw == 2)
+    zgemm_kernel<2, 1><<<grid, 256, 0, s>>>(m, n, k, ar, ai, A, B, br, bi, C, a_kfast, b_kfast);
+  else if (fnw == 2)
+    zgemm_kernel<1, 2><<<grid, 256, 0, s>>>(m, n, k, ar, ai, A, B, br, bi, C, a_kfast, b_kfast);
+  else
+    zgemm_kernel<1, 1><<<grid, 256, 0, s>>>(m, n, k, ar, ai, A, B, br, bi, C, a_kfast, b_kfast);
 }
