@@ -64,6 +64,20 @@ const char* phf_last_error(const phf_ctx* ctx);
  * call.  flags: bit 0 forces the complex form. */
 int phf_set_hamiltonian(phf_ctx* ctx, const double* hone, const double* htwo, int nso, int flags);
 
+/* Device-side construction of the same A from a factored spatial Hamiltonian,
+ * for problems whose HTwo does not fit the host interface (complex128 HTwo
+ * is 16 NSO^4 bytes: 69 GB at NSO = 256).  It reproduces the reference's
+ * input pipeline (Main_PHF.py:22-33: spin-orbital blocks, Mulliken2Dirac
+ * makeH.py:63-67, ao2mo ao2mo.py:4-10) for (pq|rs) = sum_L b[L][p][q] b[L][r][s]:
+ *   hone = Ref^H (h (x) I2) Ref,  Bt_L = Ref^H (b_L (x) I2) Ref,
+ *   <ij||kl> = sum_L Bt_L(i,k) Bt_L(j,l) - Bt_L(i,l) Bt_L(j,k).
+ * h: double [n][n]; b: double [nl][n][n]; ref: complex [2n][2n] with zero
+ * imaginary part (a complex Ref returns PHF_ERR_UNSUPPORTED).  flags unused. */
+int phf_set_hamiltonian_factored(phf_ctx* ctx, const double* h, const double* b, int n, int nl,
+                                 const double* ref, int flags);
+/* Copy the device's MO-basis hone (complex [nso][nso]) to the host. */
+int phf_get_hone(phf_ctx* ctx, double* hone);
+
 /* phftools.buildhsg (PHFTools.f90:10-201; call site PHF.py:70-71).
  *   ref      complex [nso][nso]         Ref (MO coefficients, AO rows)
  *   z        complex [nso-nocc][nocc]   Thouless Z

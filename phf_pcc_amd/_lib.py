@@ -27,7 +27,8 @@ PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2, "f16x2": 3}
 # every symbol declared in include/phf_hip.h
 EXPORTS = ["phf_ctx_create", "phf_ctx_destroy", "phf_last_error", "phf_set_hamiltonian",
            "phf_buildhsg", "phf_localg", "phf_set_profiling", "phf_stage_times", "phf_set_grid",
-           "phf_buildhsg_z", "phf_autotune", "phf_tuning_report"]
+           "phf_buildhsg_z", "phf_autotune", "phf_tuning_report", "phf_set_hamiltonian_factored",
+           "phf_get_hone"]
 
 
 class PhfError(RuntimeError):
@@ -71,6 +72,8 @@ def load() -> C.CDLL:
     lib.phf_set_grid.argtypes = [C.c_void_p, _dp, _ip, C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int,
                                  _dp, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_int, C.c_int]
     lib.phf_buildhsg_z.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+    lib.phf_set_hamiltonian_factored.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_int, _dp, C.c_int]
+    lib.phf_get_hone.argtypes = [C.c_void_p, _dp]
     lib.phf_autotune.argtypes = [C.c_void_p, C.c_int]
     lib.phf_tuning_report.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
     lib.phf_set_profiling.argtypes = [C.c_void_p, C.c_int]
@@ -129,6 +132,23 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_hamiltonian_factored(self, h: np.ndarray, b: np.ndarray, ref: np.ndarray, key=None):
+        """Build A on the device from (pq|rs) = sum_L b[L,p,q] b[L,r,s] and a real Ref."""
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        ref = np.ascontiguousarray(ref, dtype=np.complex128)
+        n = h.shape[0]
+        if h.shape != (n, n) or b.ndim != 3 or b.shape[1:] != (n, n) or ref.shape != (2 * n, 2 * n):
+            raise ValueError("h (n, n), b (L, n, n) and ref (2n, 2n) expected")
+        check(load().phf_set_hamiltonian_factored(self._h, ptr(h), ptr(b), n, b.shape[0], ptr(ref), 0), self._h)
+        self.nso = 2 * n
+        self.ham_key = key
+
+    def hone(self) -> np.ndarray:
+        out = np.zeros((self.nso, self.nso), dtype=np.complex128)
+        check(load().phf_get_hone(self._h, ptr(out)), self._h)
+        return out
 
     def set_profiling(self, on: bool = True):
         check(load().phf_set_profiling(self._h, 1 if on else 0), self._h)

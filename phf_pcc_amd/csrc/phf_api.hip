@@ -89,6 +89,7 @@ struct phf_ctx_s {
 
 static thread_local std::string g_create_err;
 static int free_grid(phf_ctx* ctx);
+static int drop_hamiltonian(phf_ctx* ctx);
 
 namespace {
 
@@ -324,6 +325,67 @@ __global__ void localg_kernel(long n2, int nterm, const int* __restrict__ tmn, c
   out[2 * e + 1] = (ni * ovr - nr * ovi) / d;
 }
 
+// B_so[L] = B_L (x) I_2 in the [alpha..., beta...] spin-orbital order, complex column-major
+__global__ void spin_blocks_kernel(const double* __restrict__ b, int n, int nl, double* __restrict__ bso) {
+  const int nso = 2 * n;
+  const long total = (long)nl * nso * nso;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const int L = (int)(idx / ((long)nso * nso));
+    const int rem = (int)(idx % ((long)nso * nso));
+    const int col = rem / nso, row = rem % nso;     // column-major
+    double v = 0.0;
+    if ((row < n) == (col < n)) v = b[((long)L * n + row % n) * n + col % n];
+    bso[2 * idx] = v;
+    bso[2 * idx + 1] = 0.0;
+  }
+}
+
+// Bt[r][L] = Re Bt_L(i,k), r = i*nso + k (rows >= nso^2 and columns >= nl are zero);
+// flags imaginary parts above tol (a complex Ref is not supported on this path)
+__global__ void pack_bt_kernel(const double* __restrict__ bt, int nso, int nl, int lpad, long nrpad,
+                               double* __restrict__ out, int* __restrict__ cplx_flag) {
+  const long n2 = (long)nso * nso;
+  const long total = nrpad * lpad;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const long r = idx / lpad;
+    const int L = (int)(idx % lpad);
+    double v = 0.0;
+    if (r < n2 && L < nl) {
+      const long i = r / nso, k = r % nso;
+      const long src = (long)L * n2 + i + k * nso;   // column-major (i, k)
+      v = bt[2 * src];
+      if (fabs(bt[2 * src + 1]) > 1e-12 * (fabs(v) + 1.0)) atomicOr(cplx_flag, 1);
+    }
+    out[idx] = v;
+  }
+}
+
+// In place, for every (i, j): block M[k][l] = A[(i,k),(j,l)] becomes M - M^T,
+// i.e. <ij||kl> = (ik|jl) - (il|jk).  32 x 32 tiles of a block and of its
+// transpose are handled by one workgroup.
+__global__ __launch_bounds__(256) void antisym_kernel(double* __restrict__ A, int nso, long lda) {
+  __shared__ double t1[32][33], t2[32][33];
+  const int nkb = (nso + 31) / 32;
+  const int i = blockIdx.z, j = blockIdx.y / nkb;
+  const int kb = blockIdx.y % nkb, lb = blockIdx.x;
+  if (lb < kb) return;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+  auto at = [&](int k, int l) -> double& { return A[((long)i * nso + k) * lda + (long)j * nso + l]; };
+  for (int y = ty; y < 32; y += 8) {
+    const int k1 = kb * 32 + y, l1 = lb * 32 + tx;
+    t1[y][tx] = (k1 < nso && l1 < nso) ? at(k1, l1) : 0.0;
+    const int k2 = lb * 32 + y, l2 = kb * 32 + tx;
+    t2[y][tx] = (k2 < nso && l2 < nso) ? at(k2, l2) : 0.0;
+  }
+  __syncthreads();
+  for (int y = ty; y < 32; y += 8) {
+    const int k1 = kb * 32 + y, l1 = lb * 32 + tx;
+    if (k1 < nso && l1 < nso) at(k1, l1) = t1[y][tx] - t2[tx][y];
+    const int k2 = lb * 32 + y, l2 = kb * 32 + tx;
+    if (kb != lb && k2 < nso && l2 < nso) at(k2, l2) = t2[y][tx] - t1[tx][y];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
@@ -433,18 +495,10 @@ extern "C" int phf_set_hamiltonian(phf_ctx* ctx, const double* hone, const doubl
   if (!hone || !htwo || nso <= 0 || nso > 512)
     return fail(ctx, PHF_ERR_ARG, "phf_set_hamiltonian: bad arguments");
   HIPCK(hipSetDevice(ctx->device));
-  ctx->has_ham = false;
   {
-    int rc = free_grid(ctx);
+    int rc = drop_hamiltonian(ctx);
     if (rc) return rc;
   }
-  for (int q = 0; q < 4; ++q) {
-    if (ctx->dAm[q]) { HIPCK(hipFree(ctx->dAm[q])); ctx->dAm[q] = nullptr; }
-    if (ctx->dAinv[q]) { HIPCK(hipFree(ctx->dAinv[q])); ctx->dAinv[q] = nullptr; }
-  }
-  if (ctx->dA) { HIPCK(hipFree(ctx->dA)); ctx->dA = nullptr; }
-  if (ctx->dH1) { HIPCK(hipFree(ctx->dH1)); ctx->dH1 = nullptr; }
-  if (ctx->dHb) { HIPCK(hipFree(ctx->dHb)); ctx->dHb = nullptr; }
   const long n2 = (long)nso * nso;
   const long n4 = n2 * n2;
   HIPCK(hipMalloc(&ctx->dH1, sizeof(double) * 2 * n2));
@@ -506,6 +560,131 @@ static int ensure_mixed_a(phf_ctx* ctx, int prec) {
     split_operand(ctx->stream, prec, true, ctx->dA, ctx->kt, ctx->nrpad, ctx->kt, ctx->dAm[prec], ctx->dAinv[prec]);
   }
   HIPCK(hipGetLastError());
+  return PHF_OK;
+}
+
+// Release the cached Hamiltonian (and everything derived from it).
+static int drop_hamiltonian(phf_ctx* ctx) {
+  ctx->has_ham = false;
+  {
+    int rc = free_grid(ctx);
+    if (rc) return rc;
+  }
+  for (int q = 0; q < 4; ++q) {
+    if (ctx->dAm[q]) { HIPCK(hipFree(ctx->dAm[q])); ctx->dAm[q] = nullptr; }
+    if (ctx->dAinv[q]) { HIPCK(hipFree(ctx->dAinv[q])); ctx->dAinv[q] = nullptr; }
+  }
+  if (ctx->dA) { HIPCK(hipFree(ctx->dA)); ctx->dA = nullptr; }
+  if (ctx->dH1) { HIPCK(hipFree(ctx->dH1)); ctx->dH1 = nullptr; }
+  if (ctx->dHb) { HIPCK(hipFree(ctx->dHb)); ctx->dHb = nullptr; }
+  ctx->tuned.clear();
+  return PHF_OK;
+}
+
+extern "C" int phf_set_hamiltonian_factored(phf_ctx* ctx, const double* h, const double* b, int n, int nl,
+                                            const double* ref, int flags) {
+  if (!ctx) return PHF_ERR_ARG;
+  if (!h || !b || !ref || n <= 0 || nl <= 0 || 2 * n > 512)
+    return fail(ctx, PHF_ERR_ARG, "phf_set_hamiltonian_factored: bad arguments");
+  (void)flags;
+  const int nso = 2 * n;
+  const long n2 = (long)nso * nso;
+  for (long q = 0; q < n2; ++q)
+    if (ref[2 * q + 1] != 0.0)
+      return fail(ctx, PHF_ERR_UNSUPPORTED,
+                  "phf_set_hamiltonian_factored: complex Ref (use phf_set_hamiltonian with the dense HTwo)");
+  HIPCK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  {
+    int rc = drop_hamiltonian(ctx);
+    if (rc) return rc;
+  }
+  const long nrpad = round_up(n2, 256);
+  const int lpad = (int)round_up(nl, 16);
+  // one-body: hone = Ref^H (h (x) I2) Ref on the host (NSO^3)
+  std::vector<double> hso(2 * n2, 0.0), tmp(2 * n2, 0.0), hone(2 * n2, 0.0);
+  for (int p = 0; p < nso; ++p)
+    for (int q = 0; q < nso; ++q)
+      if ((p < n) == (q < n)) hso[2 * ((long)p * nso + q)] = h[(long)(p % n) * n + q % n];
+  for (int p = 0; p < nso; ++p)          // tmp = hso Ref  (real Ref)
+    for (int q = 0; q < nso; ++q) {
+      double acc = 0.0;
+      for (int r = 0; r < nso; ++r) acc += hso[2 * ((long)p * nso + r)] * ref[2 * ((long)r * nso + q)];
+      tmp[2 * ((long)p * nso + q)] = acc;
+    }
+  for (int p = 0; p < nso; ++p)          // hone = Ref^T tmp
+    for (int q = 0; q < nso; ++q) {
+      double acc = 0.0;
+      for (int r = 0; r < nso; ++r) acc += ref[2 * ((long)r * nso + p)] * tmp[2 * ((long)r * nso + q)];
+      hone[2 * ((long)p * nso + q)] = acc;
+    }
+  // device scratch: b, Ref, B_so, T1, Bt (complex), Bt packed real
+  const size_t bytes = sizeof(double) * ((size_t)nl * n * n + 2 * n2 + 3 * 2 * (size_t)nl * n2 + nrpad * lpad) + 4096;
+  {
+    int rc = ensure_ws(ctx, bytes);
+    if (rc) return rc;
+  }
+  Arena ar{ctx->ws};
+  double* db = ar.take<double>((size_t)nl * n * n);
+  double* dref = ar.take<double>(2 * n2);
+  double* dbso = ar.take<double>(2 * (size_t)nl * n2);
+  double* dt1 = ar.take<double>(2 * (size_t)nl * n2);
+  double* dbt = ar.take<double>(2 * (size_t)nl * n2);
+  double* dpk = ar.take<double>((size_t)nrpad * lpad);
+  int* dflag = reinterpret_cast<int*>(ar.take<double>(1));
+  HIPCK(hipMemcpyAsync(db, b, sizeof(double) * nl * n * n, hipMemcpyHostToDevice, s));
+  HIPCK(hipMemcpyAsync(dref, ref, sizeof(double) * 2 * n2, hipMemcpyHostToDevice, s));
+  fill_zero(s, reinterpret_cast<double*>(dflag), 1);
+  spin_blocks_kernel<<<4096, 256, 0, s>>>(db, n, nl, dbso);
+  // T1[L] = B_so[L] Ref ; Bt[L] = Ref^H T1[L]   (Ref row-major host layout)
+  zgemm_batched(s, nso, nso, nso, 1, 0, zmat_colmajor(dbso, nso, 2 * n2), ZMat{dref, 2L * nso, 2, 0, 1, 0}, 0, 0,
+                zout_colmajor(dt1, nso, 2 * n2), nl);
+  zgemm_batched(s, nso, nso, nso, 1, 0, ZMat{dref, 2, 2L * nso, 0, 1, 1}, zmat_colmajor(dt1, nso, 2 * n2), 0, 0,
+                zout_colmajor(dbt, nso, 2 * n2), nl);
+  pack_bt_kernel<<<8192, 256, 0, s>>>(dbt, nso, nl, lpad, nrpad, dpk, dflag);
+  ctx->nso = nso;
+  ctx->cplx = false;
+  ctx->kpad = (int)nrpad;
+  ctx->nrpad = (int)nrpad;
+  ctx->kt = (int)nrpad;
+  HIPCK(hipMalloc(&ctx->dA, sizeof(double) * nrpad * nrpad));
+  // A1[(i,k),(j,l)] = sum_L Bt_L(i,k) Bt_L(j,l) on the FP64 matrix cores, then the exchange term
+  eri_gemm_f64(s, dpk, dpk, ctx->dA, (int)nrpad, (int)nrpad, lpad, nrpad, 128, nullptr);
+  {
+    dim3 grid((nso + 31) / 32, (unsigned)(((nso + 31) / 32) * nso), (unsigned)nso);
+    antisym_kernel<<<grid, 256, 0, s>>>(ctx->dA, nso, nrpad);
+  }
+  int hflag = 0;
+  HIPCK(hipMemcpyAsync(&hflag, dflag, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCK(hipGetLastError());
+  HIPCK(hipStreamSynchronize(s));
+  if (hflag) {
+    (void)drop_hamiltonian(ctx);
+    return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_set_hamiltonian_factored: transformed factors are complex");
+  }
+  // one-body copies
+  HIPCK(hipMalloc(&ctx->dH1, sizeof(double) * 2 * n2));
+  HIPCK(hipMemcpyAsync(ctx->dH1, hone.data(), sizeof(double) * 2 * n2, hipMemcpyHostToDevice, s));
+  {
+    std::vector<double> hb(2 * (size_t)nrpad, 0.0);
+    for (long r = 0; r < n2; ++r) {
+      hb[r] = hone[2 * r];
+      hb[nrpad + r] = hone[2 * r + 1];
+    }
+    HIPCK(hipMalloc(&ctx->dHb, sizeof(double) * hb.size()));
+    HIPCK(hipMemcpyAsync(ctx->dHb, hb.data(), sizeof(double) * hb.size(), hipMemcpyHostToDevice, s));
+  }
+  HIPCK(hipStreamSynchronize(s));
+  ctx->has_ham = true;
+  return PHF_OK;
+}
+
+extern "C" int phf_get_hone(phf_ctx* ctx, double* hone) {
+  if (!ctx || !hone) return PHF_ERR_ARG;
+  if (!ctx->has_ham) return fail(ctx, PHF_ERR_NO_HAMILTONIAN, "phf_get_hone: no Hamiltonian set");
+  HIPCK(hipMemcpyAsync(hone, ctx->dH1, sizeof(double) * 2 * (size_t)ctx->nso * ctx->nso, hipMemcpyDeviceToHost,
+                       ctx->stream));
+  HIPCK(hipStreamSynchronize(ctx->stream));
   return PHF_OK;
 }
 
