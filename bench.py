@@ -46,6 +46,45 @@ def make_workload(n=64, nocc=32, ngrid=(50, 16), seed=1002):
     return prob, ga
 
 
+def cpu_baseline(prob, ga, ref, nocc, G, seconds):
+    """The oracle's C + OpenMP restatement of BuildHSG (oracle/buildhsg_cref.c)
+    on a bounded sample of the same grid and Hamiltonian, OpenMP over points."""
+    from oracle import cref
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    htwo = np.ascontiguousarray(prob.htwo, dtype=np.complex128)
+    allpts = [(il, iy, 0) for il in range(int(ga["ngrid"][0])) for iy in range(int(ga["ngrid"][1]))]
+    args = dict(ngrid=ga["ngrid"], nci=ga["nci"], cmplxconj=1, ncisp=ga["ncisp"], nocc=nocc, r1=ga["r1"],
+                r2=ga["r2"], rpg=ga["rpg"], weightsp=ga["weightsp"], weightpg=ga["weightpg"], roota=ga["roota"],
+                rootb=ga["rootb"], rooty=ga["rooty"], qj=0, sp=1)
+    npts = min(threads, G)
+    t0 = time.perf_counter()
+    cref.buildhsg(prob.hone, htwo, ref, prob.z, points=allpts[:npts], **args)
+    t1 = time.perf_counter() - t0
+    if t1 < seconds / 2 and npts < G:
+        npts = min(G, max(npts, int(npts * seconds / max(t1, 1e-3)) // threads * threads))
+        t0 = time.perf_counter()
+        cref.buildhsg(prob.hone, htwo, ref, prob.z, points=allpts[:npts], **args)
+        t1 = time.perf_counter() - t0
+    return {"value": npts / t1, "unit": "grid pts/s", "cores": threads, "kind": "port",
+            "sample": f"C/OpenMP restatement of BuildHSG (oracle/buildhsg_cref.c), {npts} of {G} grid points of "
+                      f"the same workload in {t1:.1f} s; evals/s = value / {G}"}
+
+
+def load_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` at this workload from the committed
+    rocprofv3 counter summary (profiles/traffic.json), if present."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        data = json.load(open(path))
+    except ValueError:
+        return None
+    rec = data.get(f"{kernel}@n{n}")
+    return None if rec is None else {"bytes_per_launch": rec["bytes_per_launch"], "source": rec["source"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,7 +92,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--nocc", type=int, default=32)
-    ap.add_argument("--cpu-points", type=int, default=24)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,f16x2",
                     help="comma list of ERI precisions to time (fp64 always runs: it is the reference form)")
@@ -147,19 +186,13 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        npts = min(args.cpu_points, G)
-        pts = [(il, iy, 0) for il in range(int(ga["ngrid"][0])) for iy in range(int(ga["ngrid"][1]))][:npts]
-        ga_cpu = dict(ga)
-        t0 = time.perf_counter()
-        O.buildhsg(prob.hone, prob.htwo, ref, prob.z, cmplxconj=1, nocc=nocc, points=pts, check_hermitian=False,
-                   **ga_cpu)
-        phftools.precision = "fp64"
-        tc = time.perf_counter() - t0
-        cpu = {"value": npts / tc, "unit": "grid pts/s", "cores": int(os.environ.get("OMP_NUM_THREADS", "1")),
-               "kind": "port",
-               "sample": f"numpy restatement of BuildHSG on {npts} of {G} grid points (same Hamiltonian), "
-                         f"{tc:.1f}s"}
-        log(f"[bench] cpu baseline {cpu['value']:.2f} grid pts/s")
+        cpu = cpu_baseline(prob, ga, ref, nocc, G, args.cpu_seconds)
+        log(f"[bench] cpu baseline {cpu['value']:.3f} grid pts/s on {cpu['cores']} threads ({cpu['sample']})")
+
+    traffic = load_traffic(roof["kernel"], args.n)
+    if traffic is not None:
+        roof["traffic"] = traffic["bytes_per_launch"]
+        roof["traffic_source"] = traffic["source"]
 
     forms_out = {f: {"grid_pts_per_s": r["value"], "evals_per_s": r["value"] / G, "ms_per_step": r["ms_step"],
                      "eri_gemm_ms": r["gemm_ms"], "device_ms": r["dev_ms"], "abs_dE_vs_fp64": r["dE"],

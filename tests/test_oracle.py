@@ -126,3 +126,15 @@ def test_not_hermitian_raises():
     ga["r2"] = build_rot_mat(0.0, 0.0, 0.7, 4)[None]
     with pytest.raises(O.OracleError):
         _oracle_energy(case, ga)
+
+
+def test_c_restatement_matches_numpy_oracle():
+    """oracle/buildhsg_cref.c (the CPU baseline) reproduces the numpy oracle."""
+    from oracle import cref
+    for j, ng in [(0, (14, 8)), (1, (26, 8))]:
+        case = hubbard_general_ref(4, 4.0, 61 + j, zscale=0.3, complex_ref=True)
+        ga = grid_args(4, ng, j)
+        a = O.buildhsg(case["hone"], case["htwo"], case["ref"], case["z"], cmplxconj=1, nocc=4, **ga)
+        b = cref.buildhsg(case["hone"], case["htwo"], case["ref"], case["z"], cmplxconj=1, nocc=4, **ga)
+        for x, y in zip(a, b):
+            assert np.abs(x - y).max() <= 1e-12 * np.abs(x).max()
