@@ -4,9 +4,13 @@ Workload (the plain-FP64 GPU configuration): synthetic factored Hamiltonian
 n=64 spatial orbitals (NSO=128), N=32 electrons, L=192, Lebedev degree 11
 (50 points) x 16 gamma = 800 grid points, complex128, seed 1002.
 
-One step = one projected energy + gradient evaluation (EandG, PHF.py:55-91):
-phftools.buildhsg -> EigenSolver -> phftools.localg, Hamiltonian resident on
-the device.  value = grid points / s over the timed steps.
+One step = one projected energy + gradient evaluation (EandG, PHF.py:55-91)
+on the device-resident objective: Z in; grid sums on the device; Hmat/Smat
+(nci x nci) out; host EigenSolver; the (NSO-N) x N gradient block out.  The
+Hamiltonian is built on the device from its factors; grid and Hamiltonian
+stay resident.  value = grid points / s over the timed steps (of the fastest
+form within 1e-6 Eh of FP64).  --dropin also times the PHFTools_F drop-in
+path (host arrays, full Gradmat/Rdmmat per call).
 
 roofline: the ERI GEMM (the dominant kernel), algorithmic FLOPs
 4*NSO^4 per grid point (real A, re/im density columns) over its average
@@ -96,6 +100,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,f16x2",
                     help="comma list of ERI precisions to time (fp64 always runs: it is the reference form)")
+    ap.add_argument("--dropin", action="store_true",
+                    help="also time the PHFTools_F drop-in (host arrays in, Hmat/Smat/Gradmat/Rdmmat out per call)")
     ap.add_argument("--no-autotune", action="store_true", help="use the tile cost model instead of timing candidates")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (HBM counter calibration)")
@@ -115,43 +121,39 @@ def main():
               "is a design in DESIGN.md, not a measured path", file=sys.stderr)
         sys.exit(2)
 
-    from oracle import phf_oracle as O
     from phf_pcc_amd.PHFTools_F import phftools
+    from phf_pcc_amd.model import ProjectedHF
+    from phf_pcc_amd.phf import eigen_solver, vec_decomp
 
     prob, ga = make_workload(args.n, args.nocc)
     nso, nocc = prob.nso, prob.nocc
     G = int(ga["ngrid"][0] * ga["ngrid"][1])
     ref = prob.ref.astype(complex)
     t0 = time.time()
-    phftools.set_hamiltonian(prob.hone, prob.htwo)
-    log(f"[bench] uploaded Hamiltonian in {time.time()-t0:.1f}s")
-    ctx = phftools.context()
+    model = ProjectedHF(nocc, ref, ga, factors=(prob.h_spatial, prob.b))
+    log(f"[bench] device Hamiltonian (factored builder) and grid in {time.time()-t0:.1f}s")
+    ctx = model.ctx
     ctx.set_profiling(True)
 
-    call = (prob.hone, prob.htwo, ref, prob.z, ga["ngrid"], ga["nci"], 1, ga["ncisp"], nocc, ga["r1"], ga["r2"],
-            ga["rpg"], ga["weightsp"], ga["weightpg"], ga["roota"], ga["rootb"], ga["rooty"], 0, 1, 0)
-
     def step():
-        h, s, gm, rm = phftools.buildhsg(*call)
-        ev, vec = O.eigen_solver(h, s)
-        g = phftools.localg(gm, rm, s, ev[0], vec[:, 0] / vec[0, 0], np.ones(1), np.ones(1), 1)
-        return ev[0].real, g
+        return model.energy_and_gradient(prob.z)
 
-    def run_form(prec):
+    def run_form(prec, stepfn=step, tag=""):
+        model.precision = prec
         phftools.precision = prec
         for i in range(args.warmup):
-            e, _ = step()
+            e, _ = stepfn()
         gemm_ms, dev_ms = [], []
         t0 = time.perf_counter()
         for i in range(args.steps):
-            e, g = step()
-            st = ctx.stage_times()
+            e, g = stepfn()
+            st = stepfn.ctx.stage_times() if hasattr(stepfn, "ctx") else ctx.stage_times()
             dev_ms.append(st[0])
             gemm_ms.append(st[2])
         dt = time.perf_counter() - t0
-        r = {"prec": prec, "E": e, "G": g[nocc:, :nocc], "ms_step": dt / args.steps * 1e3,
+        r = {"prec": prec, "E": e, "G": g, "ms_step": dt / args.steps * 1e3,
              "value": G * args.steps / dt, "gemm_ms": float(np.mean(gemm_ms)), "dev_ms": float(np.mean(dev_ms))}
-        log(f"[bench] {prec}: {args.steps} steps {dt:.3f}s  E={e:.12f}  step {r['ms_step']:.3f} ms  device "
+        log(f"[bench] {tag}{prec}: {args.steps} steps {dt:.3f}s  E={e:.12f}  step {r['ms_step']:.3f} ms  device "
             f"{r['dev_ms']:.3f} ms  ERI GEMM {r['gemm_ms']:.3f} ms")
         return r
 
@@ -167,6 +169,24 @@ def main():
         r["dG"] = float(np.abs(r["G"] - base["G"]).max())
         log(f"[bench] {f}: |dE| vs fp64 = {r['dE']:.3e}  max|dG| = {r['dG']:.3e}  "
             f"speedup vs fp64 = {r['value'] / base['value']:.2f}x")
+    dropin = {}
+    if args.dropin:
+        phftools.set_hamiltonian(prob.hone, prob.htwo)
+        dctx = phftools.context()
+        dctx.set_profiling(True)
+        call = (prob.hone, prob.htwo, ref, prob.z, ga["ngrid"], ga["nci"], 1, ga["ncisp"], nocc, ga["r1"], ga["r2"],
+                ga["rpg"], ga["weightsp"], ga["weightpg"], ga["roota"], ga["rootb"], ga["rooty"], 0, 1, 0)
+
+        def dstep():
+            h, s_, gm, rm = phftools.buildhsg(*call)
+            ev, vec = eigen_solver(h, s_)
+            fsp, fpg, fk = vec_decomp(vec[:, 0], int(ga["ncisp"]), 1, 1)
+            g = phftools.localg(gm, rm, s_, ev[0], fsp, fpg, fk, 1)
+            return ev[0].real, g[nocc:, :nocc]
+        dstep.ctx = dctx
+        for f in ("fp64", "f16x2"):
+            r = run_form(f, dstep, "drop-in ")
+            dropin[f] = {"grid_pts_per_s": r["value"], "ms_per_step": r["ms_step"], "device_ms": r["dev_ms"]}
     for line in ctx.tuning_report().splitlines():
         log(f"[bench] tuning {line}")
     ok = [r for r in res.values() if r["dE"] <= 1e-6]
@@ -207,7 +227,10 @@ def main():
                                   f"L={3*args.n}, grid 50x16={G}, complex128 host interface",
                       "form": best["prec"], "evals_per_s": value / G, "grid_points": G,
                       "speedup_vs_plain_fp64": value / base["value"], "abs_dE_vs_fp64": best["dE"],
-                      "plain_fp64": forms_out["fp64"], "forms": forms_out},
+                      "plain_fp64": forms_out["fp64"], "forms": forms_out,
+                      "path": "device-resident objective (phf_eval_resident + phf_localg_vo), factored Hamiltonian "
+                              "built on the device",
+                      "drop_in_phftools": dropin or None},
            "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 

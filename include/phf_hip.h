@@ -113,6 +113,18 @@ int phf_set_grid(phf_ctx* ctx, const double* ref, const int* ngrid, int nci, int
 int phf_buildhsg_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int precision, double* hmat,
                    double* smat, double* gradmat, double* rdmmat);
 
+/* Device-resident objective (EandG, PHF.py:55-91, without moving Gradmat and
+ * Rdmmat): phf_eval_resident evaluates Z like phf_buildhsg_z but returns only
+ * hmat/smat and keeps the grid sums on the device; after the host solves the
+ * nci x nci problem (EigenSolver, VecDecomp), phf_localg_vo returns the
+ * (NSO-N) x N block G[N:, :N] of LocalG (gvo: complex [nso-nocc][nocc]),
+ * which is the gradient dE/dZ*.  Valid until the next call that uses the
+ * context.  CmplxConj = 2 is not supported here. */
+int phf_eval_resident(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int precision, double* hmat,
+                      double* smat);
+int phf_localg_vo(phf_ctx* ctx, const double* smat, const double* e0, const double* fsp, int ncisp,
+                  const double* fpg, int ncipg, const double* fk, int nfk, int cmplxconj, double* gvo);
+
 /* phftools.localg (PHFTools.f90:203-247; call site PHF.py:76).
  *   gradmat, rdmmat complex [nci][nci][nso][nso]; smat complex [nci][nci];
  *   e0 complex scalar (2 doubles); fsp complex[ncisp]; fpg complex[ncipg];

@@ -28,7 +28,7 @@ PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2, "f16x2": 3}
 EXPORTS = ["phf_ctx_create", "phf_ctx_destroy", "phf_last_error", "phf_set_hamiltonian",
            "phf_buildhsg", "phf_localg", "phf_set_profiling", "phf_stage_times", "phf_set_grid",
            "phf_buildhsg_z", "phf_autotune", "phf_tuning_report", "phf_set_hamiltonian_factored",
-           "phf_get_hone"]
+           "phf_get_hone", "phf_eval_resident", "phf_localg_vo"]
 
 
 class PhfError(RuntimeError):
@@ -74,6 +74,8 @@ def load() -> C.CDLL:
     lib.phf_buildhsg_z.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
     lib.phf_set_hamiltonian_factored.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_int, _dp, C.c_int]
     lib.phf_get_hone.argtypes = [C.c_void_p, _dp]
+    lib.phf_eval_resident.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, _dp, _dp]
+    lib.phf_localg_vo.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_int, _dp]
     lib.phf_autotune.argtypes = [C.c_void_p, C.c_int]
     lib.phf_tuning_report.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
     lib.phf_set_profiling.argtypes = [C.c_void_p, C.c_int]

@@ -76,6 +76,14 @@ struct phf_ctx_s {
   double* dGr2 = nullptr;      // R2[iy]                  [ny] row-major (as given)
   double* dGrpgref = nullptr;  // Rpg[ipg] Ref            [npg][nso x nso] column-major
   double* dGw = nullptr;       // weight x Wigner x weightpg  [G][nci][nci]
+  // results of the last evaluation kept on the device (phf_eval_resident / phf_localg_vo)
+  double* r_ysum = nullptr;   // [nci*nci] x (NSO x N) column-major, sum_g Wd Y_g
+  double* r_xsum = nullptr;   // [nci*nci] x (NSO x N), sum_g Wd X_g
+  int r_nocc = 0;
+  bool r_valid = false;
+  // small persistent device buffer for per-call arguments/results outside the arena
+  char* small = nullptr;
+  size_t small_size = 0;
   // autotuned ERI GEMM tiles: (precision, rows, cols, K) -> (config, split-K)
   int autotune = 1;
   std::map<std::tuple<int, long, int, int>, std::pair<int, int>> tuned;
@@ -386,6 +394,34 @@ __global__ __launch_bounds__(256) void antisym_kernel(double* __restrict__ A, in
   }
 }
 
+// G_vo[p - N][a] = (sum_t c_t Ysum_t(p,a) - E0 sum_t c_t Xsum_t(p,a)) / ovlp, p >= N, a < N:
+// the (NSO-N, N) block of LocalG, since (Y C0^H)[p, a] = Y(p, a) for a < N.
+__global__ void localg_vo_kernel(int nso, int nocc, int nterm, const int* __restrict__ tmn,
+                                 const double* __restrict__ tcoef, const double* __restrict__ ysum,
+                                 const double* __restrict__ xsum, double e0r, double e0i, double ovr, double ovi,
+                                 double* __restrict__ out) {
+  const int nv = nso - nocc;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nv * nocc) return;
+  const int pv = e / nocc, a = e % nocc;
+  const long off = (long)(nocc + pv) + (long)a * nso;
+  const long blk = (long)nso * nocc;
+  double gr = 0, gi = 0, rr = 0, ri = 0;
+  for (int t = 0; t < nterm; ++t) {
+    const double cr = tcoef[2 * t], ci = tcoef[2 * t + 1];
+    const long o = (long)tmn[t] * blk + off;
+    const double yr = ysum[2 * o], yi = ysum[2 * o + 1], xr = xsum[2 * o], xi = xsum[2 * o + 1];
+    gr += cr * yr - ci * yi;
+    gi += cr * yi + ci * yr;
+    rr += cr * xr - ci * xi;
+    ri += cr * xi + ci * xr;
+  }
+  const double nr = gr - (e0r * rr - e0i * ri), ni = gi - (e0r * ri + e0i * rr);
+  const double d = ovr * ovr + ovi * ovi;
+  out[2 * e] = (nr * ovr + ni * ovi) / d;
+  out[2 * e + 1] = (ni * ovr - nr * ovi) / d;
+}
+
 // ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
@@ -464,6 +500,7 @@ extern "C" int phf_ctx_destroy(phf_ctx* ctx) {
     if (ctx->dAinv[q]) (void)hipFree(ctx->dAinv[q]);
   }
   if (ctx->ws) (void)hipFree(ctx->ws);
+  if (ctx->small) (void)hipFree(ctx->small);
   for (double* q : {ctx->dGr1p, ctx->dGr2, ctx->dGrpgref, ctx->dGw})
     if (q) (void)hipFree(q);
   for (auto& e : ctx->ev)
@@ -478,6 +515,7 @@ extern "C" const char* phf_last_error(const phf_ctx* ctx) {
 }
 
 static int ensure_ws(phf_ctx* ctx, size_t bytes) {
+  ctx->r_valid = false;   // every user of the arena may overwrite the resident results
   if (ctx->ws_size >= bytes) return PHF_OK;
   if (ctx->ws) {
     HIPCK(hipStreamSynchronize(ctx->stream));
@@ -858,14 +896,29 @@ extern "C" int phf_tuning_report(const phf_ctx* ctx, char* buf, int len) {
   return n;
 }
 
+static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int precision, double* hmat,
+                  double* smat, double* gradmat, double* rdmmat, bool resident);
+
 extern "C" int phf_buildhsg_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int precision, double* hmat,
                               double* smat, double* gradmat, double* rdmmat) {
+  if (!gradmat || !rdmmat) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: NULL output pointer");
+  return eval_z(ctx, z, cmplxconj, nocc, precision, hmat, smat, gradmat, rdmmat, false);
+}
+
+extern "C" int phf_eval_resident(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int precision,
+                                 double* hmat, double* smat) {
+  return eval_z(ctx, z, cmplxconj, nocc, precision, hmat, smat, nullptr, nullptr, true);
+}
+
+static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int precision, double* hmat,
+                  double* smat, double* gradmat, double* rdmmat, bool resident) {
   if (!ctx) return PHF_ERR_ARG;
+  ctx->r_valid = false;
   if (!ctx->has_ham) return fail(ctx, PHF_ERR_NO_HAMILTONIAN, "phf_buildhsg: no Hamiltonian set");
   if (!ctx->has_grid || ctx->g_nso != ctx->nso)
     return fail(ctx, PHF_ERR_ARG, "phf_buildhsg_z: no spin grid set for this Hamiltonian (phf_set_grid)");
   const int nso = ctx->nso;
-  if (!hmat || !smat || !gradmat || !rdmmat) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: NULL output pointer");
+  if (!hmat || !smat) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: NULL output pointer");
   if (nocc <= 0 || nocc > nso) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: nocc out of range");
   if (nocc > lu_inverse_max_n())
     return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: nocc above the LDS inverse limit");
@@ -1035,10 +1088,12 @@ extern "C" int phf_buildhsg_z(phf_ctx* ctx, const double* z, int cmplxconj, int 
     wreduce_final_kernel<<<fin, 256, 0, s>>>(nchunk, (int)lenX, nmn, dPart, dXsum);
   }
   // Gradmat[mn] = Ysum[mn] C0^H, Rdmmat[mn] = Xsum[mn] C0^H (C-order [nso][nso])
-  zgemm_batched(s, nso, nso, N, 1, 0, zmat_colmajor(dYsum, nso, sT), ZMat{dC0, 2L * nso, 2, 0, 1, 1}, 0, 0,
-                ZOut{dGrad, 2L * nso, 2, 2 * n2, 1}, nmn);
-  zgemm_batched(s, nso, nso, N, 1, 0, zmat_colmajor(dXsum, nso, sT), ZMat{dC0, 2L * nso, 2, 0, 1, 1}, 0, 0,
-                ZOut{dRdm, 2L * nso, 2, 2 * n2, 1}, nmn);
+  if (!resident) {
+    zgemm_batched(s, nso, nso, N, 1, 0, zmat_colmajor(dYsum, nso, sT), ZMat{dC0, 2L * nso, 2, 0, 1, 1}, 0, 0,
+                  ZOut{dGrad, 2L * nso, 2, 2 * n2, 1}, nmn);
+    zgemm_batched(s, nso, nso, N, 1, 0, zmat_colmajor(dXsum, nso, sT), ZMat{dC0, 2L * nso, 2, 0, 1, 1}, 0, 0,
+                  ZOut{dRdm, 2L * nso, 2, 2 * n2, 1}, nmn);
+  }
   HIPCK(hipGetLastError());
   if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[3], s));
 
@@ -1046,8 +1101,10 @@ extern "C" int phf_buildhsg_z(phf_ctx* ctx, const double* z, int cmplxconj, int 
   std::vector<int> info((size_t)G);
   HIPCK(hipMemcpyAsync(hs.data(), dHS, hs.size() * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCK(hipMemcpyAsync(info.data(), dInfo, info.size() * sizeof(int), hipMemcpyDeviceToHost, s));
-  HIPCK(hipMemcpyAsync(gradmat, dGrad, 2 * (size_t)nmn * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
-  HIPCK(hipMemcpyAsync(rdmmat, dRdm, 2 * (size_t)nmn * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (!resident) {
+    HIPCK(hipMemcpyAsync(gradmat, dGrad, 2 * (size_t)nmn * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCK(hipMemcpyAsync(rdmmat, dRdm, 2 * (size_t)nmn * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
   HIPCK(hipStreamSynchronize(s));
   if (ctx->profiling) {
     float t01, t12, t23, t03;
@@ -1083,6 +1140,10 @@ extern "C" int phf_buildhsg_z(phf_ctx* ctx, const double* z, int cmplxconj, int 
     }
   const double tol = 1e-5;
   if (w1 > hmax * tol || w2 > smax * tol) return fail(ctx, PHF_ERR_NOT_HERMITIAN, "Not Hermitian");
+  ctx->r_ysum = dYsum;
+  ctx->r_xsum = dXsum;
+  ctx->r_nocc = N;
+  ctx->r_valid = true;
   return PHF_OK;
 }
 
@@ -1096,6 +1157,61 @@ extern "C" int phf_buildhsg(phf_ctx* ctx, const double* ref, const double* z, co
                         na, rootb, nb, rooty, ny_, qj, sp);
   if (rc) return rc;
   return phf_buildhsg_z(ctx, z, cmplxconj, nocc, precision, hmat, smat, gradmat, rdmmat);
+}
+
+extern "C" int phf_localg_vo(phf_ctx* ctx, const double* smat, const double* e0, const double* fsp, int ncisp,
+                             const double* fpg, int ncipg, const double* fk, int nfk, int cmplxconj, double* gvo) {
+  if (!ctx) return PHF_ERR_ARG;
+  if (!ctx->r_valid) return fail(ctx, PHF_ERR_ARG, "phf_localg_vo: no resident evaluation (phf_eval_resident)");
+  if (!smat || !e0 || !fsp || !fpg || !fk || !gvo || ncisp <= 0 || ncipg <= 0 || nfk < 1)
+    return fail(ctx, PHF_ERR_ARG, "phf_localg_vo: bad arguments");
+  if (cmplxconj == 2) return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_localg_vo: CmplxConj=2");
+  const int nci = ctx->g_nci;
+  if (ncisp * ncipg > nci) return fail(ctx, PHF_ERR_ARG, "phf_localg_vo: ncisp*ncipg > nci");
+  HIPCK(hipSetDevice(ctx->device));
+  auto C = [](const double* p, int i) { return cplx(p[2 * i], p[2 * i + 1]); };
+  std::vector<int> tmn;
+  std::vector<double> tcoef;
+  cplx ovlp(0, 0);
+  const cplx fk0 = C(fk, 0);
+  for (int ni = 0; ni < ncisp; ++ni)
+    for (int nj = 0; nj < ncisp; ++nj)
+      for (int p = 0; p < ncipg; ++p)
+        for (int q = 0; q < ncipg; ++q) {
+          const int m = ni + p * ncisp, n = nj + q * ncisp;
+          const cplx f = std::conj(C(fsp, ni)) * C(fsp, nj) * std::conj(C(fpg, p)) * C(fpg, q) * std::conj(fk0) * fk0;
+          tmn.push_back(m * nci + n);
+          tcoef.push_back(f.real());
+          tcoef.push_back(f.imag());
+          ovlp += f * C(smat, m * nci + n);
+        }
+  const int nterm = (int)tmn.size();
+  const int nso = ctx->nso, N = ctx->r_nocc;
+  const size_t need = 256 + sizeof(int) * nterm + 256 + sizeof(double) * 2 * nterm + 256 +
+                      sizeof(double) * 2 * (size_t)(nso - N) * N;
+  if (ctx->small_size < need) {
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    if (ctx->small) HIPCK(hipFree(ctx->small));
+    ctx->small = nullptr;
+    ctx->small_size = 0;
+    HIPCK(hipMalloc(&ctx->small, need));
+    ctx->small_size = need;
+  }
+  Arena sa{ctx->small};
+  int* dmn = sa.take<int>(nterm);
+  double* dcf = sa.take<double>(2 * nterm);
+  double* dout = sa.take<double>(2 * (size_t)(nso - N) * N);
+  hipStream_t s = ctx->stream;
+  HIPCK(hipMemcpyAsync(dmn, tmn.data(), sizeof(int) * nterm, hipMemcpyHostToDevice, s));
+  HIPCK(hipMemcpyAsync(dcf, tcoef.data(), sizeof(double) * 2 * nterm, hipMemcpyHostToDevice, s));
+  const int nel = (nso - N) * N;
+  if (nel > 0)
+    localg_vo_kernel<<<(nel + 255) / 256, 256, 0, s>>>(nso, N, nterm, dmn, dcf, ctx->r_ysum, ctx->r_xsum, e0[0],
+                                                       e0[1], ovlp.real(), ovlp.imag(), dout);
+  HIPCK(hipGetLastError());
+  HIPCK(hipMemcpyAsync(gvo, dout, sizeof(double) * 2 * (size_t)nel, hipMemcpyDeviceToHost, s));
+  HIPCK(hipStreamSynchronize(s));
+  return PHF_OK;
 }
 
 extern "C" int phf_set_profiling(phf_ctx* ctx, int on) {

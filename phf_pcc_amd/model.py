@@ -90,12 +90,33 @@ class ProjectedHF:
         return g
 
     def energy_and_gradient(self, z: np.ndarray):
-        """Projected energy E0 and G = dE/dZ* (shape (NSO-N, N))."""
-        h, s, gm, rm = self.buildhsg(z)
+        """Projected energy E0 and G = dE/dZ* (shape (NSO-N, N)).  The grid sums
+        stay on the device; only Hmat/Smat (nci x nci) and G cross PCIe."""
+        z = np.ascontiguousarray(z, dtype=np.complex128)
+        if z.shape != (self.nso - self.nocc, self.nocc):
+            raise ValueError(f"z must be ({self.nso - self.nocc}, {self.nocc})")
+        if self.cmplxconj == 2:
+            h, s, gm, rm = self.buildhsg(z)
+            ev, vec = eigen_solver(h, s)
+            fsp, fpg, fk = vec_decomp(vec[:, 0], self.ncisp, self.ncipg, 2)
+            return float(ev[0].real), self.localg(gm, rm, s, ev[0], fsp, fpg, fk)[self.nocc:, :self.nocc]
+        lib = _lib.load()
+        nci = self.nci
+        h = np.zeros((nci, nci), complex)
+        s = np.zeros((nci, nci), complex)
+        _lib.check(lib.phf_eval_resident(self.ctx.handle, _lib.ptr(z), self.cmplxconj, self.nocc,
+                                         _lib.PREC[self.precision], _lib.ptr(h), _lib.ptr(s)), self.ctx.handle)
+        self.nevals += 1
         ev, vec = eigen_solver(h, s)
-        fsp, fpg, fk = vec_decomp(vec[:, 0], self.ncisp, self.ncipg, 2 if self.cmplxconj == 2 else 1)
-        g = self.localg(gm, rm, s, ev[0], fsp, fpg, fk)
-        return float(ev[0].real), g[self.nocc:, :self.nocc]
+        fsp, fpg, fk = vec_decomp(vec[:, 0], self.ncisp, self.ncipg, 1)
+        c = lambda a: np.ascontiguousarray(a, dtype=np.complex128).reshape(-1)
+        fsp, fpg, fk = c(fsp), c(fpg), c(fk)
+        e0a = np.array([complex(ev[0])])
+        g = np.zeros((self.nso - self.nocc, self.nocc), complex)
+        _lib.check(lib.phf_localg_vo(self.ctx.handle, _lib.ptr(s), _lib.ptr(e0a), _lib.ptr(fsp), fsp.size,
+                                     _lib.ptr(fpg), fpg.size, _lib.ptr(fk), fk.size, self.cmplxconj, _lib.ptr(g)),
+                   self.ctx.handle)
+        return float(ev[0].real), g
 
 
 def _torch():
