@@ -152,6 +152,46 @@ int phf_stage_times(const phf_ctx* ctx, double* ms, int nmax);
 int phf_autotune(phf_ctx* ctx, int on);
 int phf_tuning_report(const phf_ctx* ctx, char* buf, int len);
 
+/* Batched evaluation of many independent instances (no counterpart in the
+ * reference, which evaluates one Hamiltonian per process; each instance is
+ * one EandG of PHF.py:55-91 at J = 0).  Instances may differ in size, N and
+ * grid.  phf_batch_set_instance builds instance i's A on the device from its
+ * factors (as phf_set_hamiltonian_factored) and uploads its J = 0 spin grid
+ * (phf_set_grid arguments with nci = ncisp = ncipg = 1; weightpg may be NULL
+ * for ones); storage = PHF_PREC_FP64 keeps the FP64 A, a reduced-precision
+ * mode keeps only that mode's copy of A (the instance then evaluates in that
+ * mode only).  phf_batch_eval takes the Z of every instance concatenated in
+ * instance order (complex C order (2n - nocc) x nocc each) and returns per
+ * instance the projected energy E0 (energies[i]), optionally the 1 x 1 Hmat,
+ * Smat (hs[4 i .. 4 i + 3] = Re H, Im H, Re S, Im S) and the gradient block
+ * dE/dZ* = LocalG[N:, :N] (grads, laid out like z).  An instance with a
+ * singular overlap, |Re S| <= 1e-6 or a failed Hermiticity test
+ * (PHFTools.f90:195-200) fails: every instance's outputs are still written,
+ * status[i] (optional) receives each instance's code and the call returns
+ * the first failure's code.  One evaluation is one
+ * launch sequence: every instance's rotation/overlap/density stages, one
+ * persistent grouped ERI GEMM over all instances (16-bit modes; FP64 and
+ * FP32 launch per instance), every instance's epilogue and grid sums.  The
+ * first evaluation of a precision autotunes the grouped tiles and records
+ * the sequence in a hipGraph that later evaluations replay
+ * (phf_batch_set_graph(b, 0) turns that off).  phf_batch_stats: [0] device
+ * time of the last evaluation in ms, [1] its ERI GEMM time (-1 for a graph
+ * replay), [2] 1 if it was a graph replay. */
+typedef struct phf_batch_s phf_batch;
+int phf_batch_create(int device, int ninst, phf_batch** out);
+int phf_batch_destroy(phf_batch* b);
+const char* phf_batch_last_error(const phf_batch* b);
+int phf_batch_set_instance(phf_batch* b, int i, const double* h, const double* bfac, int n, int nl,
+                           const double* ref, int nocc, const int* ngrid, const double* r1, int nr1,
+                           const double* r2, int nr2, const double* rpg, int npg, const double* weightsp,
+                           const double* weightpg, int storage);
+int phf_batch_eval(phf_batch* b, const double* z, int cmplxconj, int precision, double* energies,
+                   double* grads, double* hs, int* status);
+int phf_batch_set_graph(phf_batch* b, int on);
+int phf_batch_autotune(phf_batch* b, int on);
+int phf_batch_stats(const phf_batch* b, double* ms, int nmax);
+int phf_batch_tuning_report(const phf_batch* b, char* buf, int len);
+
 #ifdef __cplusplus
 }
 #endif

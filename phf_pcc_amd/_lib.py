@@ -28,7 +28,9 @@ PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2, "f16x2": 3}
 EXPORTS = ["phf_ctx_create", "phf_ctx_destroy", "phf_last_error", "phf_set_hamiltonian",
            "phf_buildhsg", "phf_localg", "phf_set_profiling", "phf_stage_times", "phf_set_grid",
            "phf_buildhsg_z", "phf_autotune", "phf_tuning_report", "phf_set_hamiltonian_factored",
-           "phf_get_hone", "phf_eval_resident", "phf_localg_vo"]
+           "phf_get_hone", "phf_eval_resident", "phf_localg_vo", "phf_batch_create", "phf_batch_destroy",
+           "phf_batch_last_error", "phf_batch_set_instance", "phf_batch_eval", "phf_batch_set_graph",
+           "phf_batch_autotune", "phf_batch_stats", "phf_batch_tuning_report"]
 
 
 class PhfError(RuntimeError):
@@ -80,8 +82,19 @@ def load() -> C.CDLL:
     lib.phf_tuning_report.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
     lib.phf_set_profiling.argtypes = [C.c_void_p, C.c_int]
     lib.phf_stage_times.argtypes = [C.c_void_p, _dp, C.c_int]
+    lib.phf_batch_create.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+    lib.phf_batch_destroy.argtypes = [C.c_void_p]
+    lib.phf_batch_last_error.argtypes = [C.c_void_p]
+    lib.phf_batch_last_error.restype = C.c_char_p
+    lib.phf_batch_set_instance.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp, C.c_int, _ip,
+                                           _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp, C.c_int]
+    lib.phf_batch_eval.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, _dp, _dp, _dp, _ip]
+    lib.phf_batch_set_graph.argtypes = [C.c_void_p, C.c_int]
+    lib.phf_batch_autotune.argtypes = [C.c_void_p, C.c_int]
+    lib.phf_batch_stats.argtypes = [C.c_void_p, _dp, C.c_int]
+    lib.phf_batch_tuning_report.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
     for name in EXPORTS:
-        if name != "phf_last_error":
+        if name not in ("phf_last_error", "phf_batch_last_error"):
             getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib
@@ -93,10 +106,13 @@ def ptr(a: np.ndarray | None):
     return a.ctypes.data_as(_dp)
 
 
-def check(rc: int, ctx) -> None:
+def check(rc: int, ctx, batch: bool = False) -> None:
+    """Raise the Python exception of status rc (message from the context, or
+    from the batch handle when batch is True)."""
     if rc == PHF_OK:
         return
-    msg = load().phf_last_error(ctx).decode(errors="replace")
+    lib = load()
+    msg = (lib.phf_batch_last_error(ctx) if batch else lib.phf_last_error(ctx)).decode(errors="replace")
     if rc == PHF_ERR_SINGULAR:
         raise SingularOverlapError(rc, msg)
     if rc == PHF_ERR_NOT_HERMITIAN:

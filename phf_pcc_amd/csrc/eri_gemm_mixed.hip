@@ -2,18 +2,19 @@
 //
 // Compensated split products on the matrix cores:
 //   x = s^{-1} (x0 + x1 + x2),  x_i in FP16 or BF16,  s a power of two per row
-// of A and per (grid point, re/im) row of P.  The kept terms of A P are laid
-// out along the contraction index so one GEMM pass computes them all:
-//   F16X2  (3 products)  A' = [a0 | a0 | a1],           P' = [p0 | p1 | p0]
-//   BF16X3 (6 products)  A' = [a0 a0 a1 a0 a1 a2],       P' = [p0 p1 p0 p2 p1 p0]
-// The MFMA accumulates in FP32; every FLUSH k-slabs the FP32 tile is folded
+// of A and per (grid point, re/im) row of P.  The kept terms of A P form one
+// contraction over a sequence of segments, each a (P slice, A slice) pair:
+//   F16X2  (3 products)  A [a0 | a0 | a1],           P [p0 | p1 | p0]
+//   BF16X3 (6 products)  A [a0 a0 a1 a0 a1 a2],      P [p0 p1 p0 p2 p1 p0]
+// Each slice is stored once per operand row; the kernels walk the segment
+// sequence.  The MFMA accumulates in FP32; every FOLD k-slabs the FP32 tile is folded
 // into an FP64 accumulator, and the FP64 epilogue applies s_P^{-1} s_A^{-1}
 // (exact powers of two) and writes Gamma in FP64 for the FP64 epilogue of the
 // grid pipeline.  FP32 form: one v_mfma_f32_16x16x4f32 product of the FP32
 // roundings.
 //
-// Layout contract as eri_gemm.hip: P' is [Mp][K'], A' is [Nr][K'], C is
-// [Mp][ldc]; Mp % BM == 0, Nr % BN == 0, K' % BK == 0.
+// Layout: P is [Mp][nslice*K], A is [Nr][nslice*K] (slices of one row side by
+// side), C is [Mp][ldc]; Mp % BM == 0, Nr % BN == 0, K % 64 == 0.
 #include <type_traits>
 
 #include "phf_common.h"
@@ -39,19 +40,22 @@ typedef __attribute__((address_space(1))) void glb_void_t;
 // s_waitcnt with only vmcnt <= v (expcnt / lgkmcnt left at their maxima), gfx9 encoding
 #define PHF_WAIT_VMCNT(v) __builtin_amdgcn_s_waitcnt(((v) & 0xF) | ((((v) >> 4) & 3) << 14) | (0x7 << 4) | (0xF << 8))
 
-// 16-bit inputs, 16x16x32 MFMA.  Workgroup tile BM x BN (rows of P' x rows of
-// A'), waves of WM x WN arranged (BM/WM) x (BN/WN).  Operands stream global ->
-// LDS with global_load_lds_dwordx4 (no staging registers) through a ring of
-// STAGES k-slabs of 64 elements; a slab row is 128 B = 8 chunks of 16 B, eight
-// rows form a 1 KB block written by one wave instruction, and chunk c of row r
-// sits at slot c ^ ((r >> 1) & 7) so the 16 rows of an MFMA fragment read hit
-// 16 distinct bank groups.  Every FOLD slabs the FP32 MFMA tile is added into
-// an FP64 tile and restarted, keeping the FP32 rounding of the long
-// contraction at the level of a FOLD*64 chunk.  SPLITK = 2 halves the
-// contraction per workgroup and adds the two FP64 partials with atomics (two
-// commutative additions onto zero: the result does not depend on their order).
-// FOLD64: fold the FP32 MFMA tile into an FP64 tile (else into an FP32 master
-// tile: half the registers, used by the large per-wave tiles).
+// 16-bit inputs, 16x16x32 MFMA.  Workgroup tile BM x BN (rows of P x rows of
+// A), waves of WM x WN arranged (BM/WM) x (BN/WN).  The operands hold nslice
+// slices per row ([x0 | x1 | ...], each K long); the contraction runs over a
+// sequence of segments, segment sg pairing P slice (patP >> 3 sg) & 7 with A
+// slice (patA >> 3 sg) & 7, so duplicated slices are read twice but stored once.
+// Operands stream global -> LDS with global_load_lds_dwordx4 (no staging
+// registers) through a ring of STAGES k-slabs of 64 elements; a slab row is
+// 128 B = 8 chunks of 16 B, eight rows form a 1 KB block written by one wave
+// instruction, and chunk c of row r sits at slot c ^ ((r >> 1) & 7) so the 16
+// rows of an MFMA fragment read hit 16 distinct bank groups.  Every FOLD slabs
+// the FP32 MFMA tile is added into an FP64 tile and restarted, keeping the FP32
+// rounding of the long contraction at the level of a FOLD*64 chunk.  A split
+// contraction (slab range [sbeg, send) of the segment sequence) adds its FP64
+// partial into C with atomics; two partials onto zero give an order-independent
+// result.  FOLD64: fold the FP32 MFMA tile into an FP64 tile (else into an FP32
+// master tile: half the registers, used by the large per-wave tiles).
 template <int BM, int BN, int WM, int WN, int STAGES_, int MINB_, bool FOLD64_>
 struct H16Cfg {
   static constexpr int NW = (BM / WM) * (BN / WN);
@@ -65,11 +69,12 @@ struct H16Cfg {
   static_assert(LDS * MINB <= 160 * 1024, "LDS budget");
 };
 
-template <typename T, typename Cfg, int BM, int BN, int WM, int WN, int SPLITK>
-__global__ __launch_bounds__((Cfg::NT), (Cfg::MINB))
-void eri_gemm_h16_kernel(const T* __restrict__ P, const T* __restrict__ A, const double* __restrict__ invP,
-                         const double* __restrict__ invA, double* __restrict__ C, int K, long ldc, int tiles_m,
-                         const double* __restrict__ bias) {
+template <typename T, typename Cfg, int BM, int BN, int WM, int WN>
+__device__ __forceinline__ void h16_tile(const T* __restrict__ P, const T* __restrict__ A,
+                                         const double* __restrict__ invP, const double* __restrict__ invA,
+                                         double* __restrict__ C, const double* __restrict__ bias, long ldc, int K,
+                                         int nslice, unsigned patP, unsigned patA, int mt, int nt, int sbeg,
+                                         int send, bool atomic, unsigned char* smem) {
   constexpr int BK = 64, FOLD = 8;
   constexpr int STAGES = Cfg::STAGES, NW = Cfg::NW, ROWS = Cfg::ROWS, STAGE_BYTES = Cfg::STAGE_BYTES;
   constexpr int FM = WM / 16, FN = WN / 16, WGN = BN / WN;
@@ -77,17 +82,12 @@ void eri_gemm_h16_kernel(const T* __restrict__ P, const T* __restrict__ A, const
   constexpr int BPW = BLOCKS / NW;            // blocks (load instructions) per wave per slab
   static_assert(BPW * NW == BLOCKS && BPW < 64, "blocks per wave");
   typedef typename MfmaH<T>::v8 v8;
-  extern __shared__ __attribute__((aligned(1024))) unsigned char smem[];
 
-  const int t = xcd_remap(blockIdx.x, gridDim.x);
-  const int ks_id = t % SPLITK;
-  const int tile = t / SPLITK;
-  const int mt = tile % tiles_m, nt = tile / tiles_m;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WGN, wn = wave % WGN;
-  const int kpart = K / SPLITK;
-  const int kbeg = ks_id * kpart;
-  const int nk = kpart / BK;
+  const int nk = send - sbeg;
+  const int nsl = K / BK;                     // slabs per segment
+  const long rowlen = (long)nslice * K;
 
   const T* src[BPW];
 #pragma unroll
@@ -95,14 +95,20 @@ void eri_gemm_h16_kernel(const T* __restrict__ P, const T* __restrict__ A, const
     const int b = wave * BPW + q;
     const int r = 8 * b + (lane >> 3);
     const int c = (lane & 7) ^ ((r >> 1) & 7);
-    src[q] = ((r < BM) ? P + ((long)mt * BM + r) * K : A + ((long)nt * BN + (r - BM)) * K) + kbeg + c * 8;
+    src[q] = ((r < BM) ? P + ((long)mt * BM + r) * rowlen : A + ((long)nt * BN + (r - BM)) * rowlen) + c * 8;
   }
+  int iseg = sbeg / nsl, ioff = sbeg % nsl;   // segment / slab of the next slab to issue
   auto issue = [&](int slab) {
     unsigned char* st = smem + (slab % STAGES) * STAGE_BYTES;
+    const long oP = (long)((patP >> (3 * iseg)) & 7) * K + ioff * BK;
+    const long oA = (long)((patA >> (3 * iseg)) & 7) * K + ioff * BK;
 #pragma unroll
-    for (int q = 0; q < BPW; ++q)
-      __builtin_amdgcn_global_load_lds((glb_void_t*)(src[q] + slab * BK),
+    for (int q = 0; q < BPW; ++q) {
+      const bool isP = (wave * BPW + q) < BM / 8;     // wave-uniform
+      __builtin_amdgcn_global_load_lds((glb_void_t*)(src[q] + (isP ? oP : oA)),
                                        (lds_void_t*)(st + (wave * BPW + q) * 1024), 16, 0, 0);
+    }
+    if (++ioff == nsl) { ioff = 0; ++iseg; }
   };
   auto frag = [&](const unsigned char* st, int r, int c) -> v8 {
     return *reinterpret_cast<const v8*>(st + (r >> 3) * 1024 + ((r & 7) * 8 + (c ^ ((r >> 1) & 7))) * 16);
@@ -163,12 +169,44 @@ void eri_gemm_h16_kernel(const T* __restrict__ P, const T* __restrict__ A, const
         const long row = (long)mt * BM + wm * WM + i * 16 + 4 * (lane >> 4) + r;
         const long col = (long)nt * BN + wn * WN + j * 16 + (lane & 15);
         double v = (double)facc[i][j][r] * invP[row] * invA[col];
-        if (bias && ks_id == 0) v += bias[(row & 1) * ldc + col];
-        if constexpr (SPLITK == 1)
-          C[row * ldc + col] = v;
-        else
+        if (bias) v += bias[(row & 1) * ldc + col];
+        if (atomic)
           atomicAdd(&C[row * ldc + col], v);
+        else
+          C[row * ldc + col] = v;
       }
+}
+
+// One GEMM: workgroup -> (split part, row tile, column tile), XCD-aware.
+template <typename T, typename Cfg, int BM, int BN, int WM, int WN, int SPLITK>
+__global__ __launch_bounds__((Cfg::NT), (Cfg::MINB))
+void eri_gemm_h16_kernel(const T* __restrict__ P, const T* __restrict__ A, const double* __restrict__ invP,
+                         const double* __restrict__ invA, double* __restrict__ C, int K, int nslice, int nseg,
+                         unsigned patP, unsigned patA, long ldc, int tiles_m, const double* __restrict__ bias) {
+  extern __shared__ __attribute__((aligned(1024))) unsigned char smem[];
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int ks_id = t % SPLITK;
+  const int tile = t / SPLITK;
+  const int mt = tile % tiles_m, nt = tile / tiles_m;
+  const int part = nseg * (K / 64) / SPLITK;
+  h16_tile<T, Cfg, BM, BN, WM, WN>(P, A, invP, invA, C, ks_id == 0 ? bias : nullptr, ldc, K, nslice, patP, patA, mt,
+                                   nt, ks_id * part, (ks_id + 1) * part, SPLITK > 1, smem);
+}
+
+// Many GEMMs of one precision in one persistent launch: workgroup b takes
+// items b, b + gridDim.x, ... of a list the host ordered for load balance and
+// XCD locality.
+template <typename T, typename Cfg, int BM, int BN, int WM, int WN, bool ATOMIC>
+__global__ __launch_bounds__((Cfg::NT), (Cfg::MINB))
+void eri_gemm_h16_grouped_kernel(const H16Item* __restrict__ items, int nitems, int nslice, unsigned patP,
+                                 unsigned patA) {
+  extern __shared__ __attribute__((aligned(1024))) unsigned char smem[];
+  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    const H16Item it = items[w];
+    __syncthreads();   // every wave is done with the previous tile's LDS ring
+    h16_tile<T, Cfg, BM, BN, WM, WN>((const T*)it.P, (const T*)it.A, it.invP, it.invA, it.C, it.bias, it.ldc, it.K,
+                                     nslice, patP, patA, it.mt, it.nt, it.sbeg, it.send, ATOMIC, smem);
+  }
 }
 
 // FP32 inputs, v_mfma_f32_16x16x4f32, 128 x 128 tile, k-slab 32, FP64 fold every FLUSH slabs.
@@ -269,12 +307,13 @@ template <typename T> __device__ inline T to16(double x);
 template <> __device__ inline _Float16 to16<_Float16>(double x) { return (_Float16)(float)x; }
 template <> __device__ inline __bf16 to16<__bf16>(double x) { return (__bf16)(float)x; }
 
-// One workgroup per row: s = 2^(top - ceil(log2 max|x|)); slices x_i of s*x;
-// dst row = concatenation of the slices named by `pattern` (3 bits per entry).
+// One workgroup per row: s = 2^(top - ceil(log2 max|x|)); nslice slices x_i of
+// s*x (x_0 the rounding of s*x, x_1 of the remainder, ...), stored as
+// [x_0 | x_1 | ...] along the row (each kt long).
 template <typename T>
 __global__ __launch_bounds__(256) void split_rows_kernel(const double* __restrict__ src, long lds, int kt,
-                                                         T* __restrict__ dst, long ldd, int nseg,
-                                                         unsigned pattern, int top, double* __restrict__ inv) {
+                                                         T* __restrict__ dst, int nslice, int top,
+                                                         double* __restrict__ inv) {
   __shared__ double red[256];
   const long row = blockIdx.x;
   const double* x = src + row * lds;
@@ -293,16 +332,14 @@ __global__ __launch_bounds__(256) void split_rows_kernel(const double* __restric
   }
   const double sc = (m > 0.0) ? ldexp(1.0, top - e) : 1.0;
   if (threadIdx.x == 0) inv[row] = 1.0 / sc;
-  T* d = dst + row * ldd;
+  T* d = dst + row * (long)nslice * kt;
   for (int c = threadIdx.x; c < kt; c += 256) {
     double r = x[c] * sc;
-    T sl[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      sl[q] = to16<T>(r);
-      r -= (double)(float)sl[q];
+    for (int q = 0; q < nslice; ++q) {
+      const T v = to16<T>(r);
+      d[(long)q * kt + c] = v;
+      r -= (double)(float)v;
     }
-    for (int sg = 0; sg < nseg; ++sg) d[(long)sg * kt + c] = sl[(pattern >> (3 * sg)) & 7];
   }
 }
 
@@ -320,21 +357,30 @@ static unsigned pack_pattern(const int* p, int n) {
   return v;
 }
 
-// precision modes (values of PHF_PREC_*)
+// precision modes (values of PHF_PREC_*): stored slices per operand row and
+// product segments of the contraction
+int mixed_slices(int prec) { return prec == 3 ? 2 : (prec == 2 ? 3 : 1); }
 int mixed_segments(int prec) { return prec == 3 ? 3 : (prec == 2 ? 6 : 1); }
 
-void split_operand(hipStream_t s, int prec, bool is_a, const double* src, long lds, int rows, int kt, void* dst,
-                   double* inv) {
+//   F16X2  (3 products)  P slices [p0 p1 p0] against A slices [a0 a0 a1]
+//   BF16X3 (6 products)  P slices [p0 p1 p0 p2 p1 p0] against A [a0 a0 a1 a0 a1 a2]
+static void mixed_patterns(int prec, unsigned* patP, unsigned* patA) {
   static const int a_f16[3] = {0, 0, 1}, p_f16[3] = {0, 1, 0};
   static const int a_b16[6] = {0, 0, 1, 0, 1, 2}, p_b16[6] = {0, 1, 0, 2, 1, 0};
-  const int nseg = mixed_segments(prec);
-  const long ldd = (long)nseg * kt;
+  if (prec == 3) {
+    *patP = pack_pattern(p_f16, 3);
+    *patA = pack_pattern(a_f16, 3);
+  } else {
+    *patP = pack_pattern(p_b16, 6);
+    *patA = pack_pattern(a_b16, 6);
+  }
+}
+
+void split_operand(hipStream_t s, int prec, const double* src, long lds, int rows, int kt, void* dst, double* inv) {
   if (prec == 3)
-    split_rows_kernel<_Float16><<<rows, 256, 0, s>>>(src, lds, kt, (_Float16*)dst, ldd, nseg,
-                                                     pack_pattern(is_a ? a_f16 : p_f16, 3), 14, inv);
+    split_rows_kernel<_Float16><<<rows, 256, 0, s>>>(src, lds, kt, (_Float16*)dst, 2, 14, inv);
   else
-    split_rows_kernel<__bf16><<<rows, 256, 0, s>>>(src, lds, kt, (__bf16*)dst, ldd, nseg,
-                                                   pack_pattern(is_a ? a_b16 : p_b16, 6), 14, inv);
+    split_rows_kernel<__bf16><<<rows, 256, 0, s>>>(src, lds, kt, (__bf16*)dst, 3, 14, inv);
 }
 
 void convert_f32(hipStream_t s, const double* src, float* dst, long n) {
@@ -361,16 +407,19 @@ double h16_model_cost(int c, int sk, long Mp0, int Nr0, int ncu) {
   return (double)rounds * per_cu * bm * bn / sk;
 }
 
-// Mp multiple of the config's BM, Nr multiple of its BN; Kseg = nseg * kt
-// multiple of 64 * splitk (16-bit) or 32 (FP32).  With splitk = 2 the caller
-// zero-fills C first.
+// Mp multiple of the config's BM, Nr multiple of its BN; K (the slice
+// length) a multiple of 64 * splitk (16-bit) or 32 (FP32).  With splitk = 2 the
+// caller zero-fills C first.
 void eri_gemm_mixed(hipStream_t s, int prec, const void* P, const void* A, const double* invP, const double* invA,
-                    double* C, int Mp, int Nr, int Kseg, long ldc, int cfg, int splitk, const double* bias) {
+                    double* C, int Mp, int Nr, int K, long ldc, int cfg, int splitk, const double* bias) {
   if (prec == 1) {
     const int tm = Mp / 128, tn = Nr / 128;
-    eri_gemm_f32_kernel<16><<<tm * tn, 256, 0, s>>>((const float*)P, (const float*)A, C, Kseg, ldc, tm, bias);
+    eri_gemm_f32_kernel<16><<<tm * tn, 256, 0, s>>>((const float*)P, (const float*)A, C, K, ldc, tm, bias);
     return;
   }
+  unsigned patP, patA;
+  mixed_patterns(prec, &patP, &patA);
+  const int nslice = mixed_slices(prec), nseg = mixed_segments(prec);
   const int bm = kH16Cfgs[cfg][0], bn = kH16Cfgs[cfg][1];
   const int tm = Mp / bm, tn = Nr / bn;
   const unsigned grid = (unsigned)(tm * tn * splitk);
@@ -383,7 +432,8 @@ void eri_gemm_mixed(hipStream_t s, int prec, const void* P, const void* A, const
       (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, CC::LDS);     \
       attr_set = true;                                                                                       \
     }                                                                                                        \
-    kern<<<grid, CC::NT, CC::LDS, s>>>((const TT*)P, (const TT*)A, invP, invA, C, Kseg, ldc, tm, bias);            \
+    kern<<<grid, CC::NT, CC::LDS, s>>>((const TT*)P, (const TT*)A, invP, invA, C, K, nslice, nseg, patP, patA, \
+                                       ldc, tm, bias);                                                       \
   } while (0)
 #define H_CFGS(TT, SK)                                                   \
   do {                                                                   \
@@ -401,4 +451,46 @@ void eri_gemm_mixed(hipStream_t s, int prec, const void* P, const void* A, const
   }
 #undef H_CFGS
 #undef H_LAUNCH
+}
+
+// Workgroups of the persistent grouped launch for config cfg: one per CU
+// times the workgroups per CU the config's registers and LDS allow.
+int h16_grouped_grid(int cfg, int ncu) { return ncu * ((kH16Cfgs[cfg][0] == 128 && kH16Cfgs[cfg][1] == 128) ? 2 : 1); }
+
+// Items reference operands in the layout of eri_gemm_mixed (slices of length
+// it.K); every item's tile lies inside its operands.  atomic: every item adds
+// into a C the caller zero-filled (split contractions); else every item stores.
+void eri_gemm_mixed_grouped(hipStream_t s, int prec, const H16Item* items, int nitems, int cfg, int grid,
+                            bool atomic) {
+  unsigned patP, patA;
+  mixed_patterns(prec, &patP, &patA);
+  const int nslice = mixed_slices(prec);
+#define G_LAUNCH(TT, B0, B1, W0, W1, ST, MB, F64)                                                             \
+  do {                                                                                                       \
+    typedef H16Cfg<B0, B1, W0, W1, ST, MB, F64> CC;                                                          \
+    auto k_at = eri_gemm_h16_grouped_kernel<TT, CC, B0, B1, W0, W1, true>;                                   \
+    auto k_st = eri_gemm_h16_grouped_kernel<TT, CC, B0, B1, W0, W1, false>;                                  \
+    static bool attr_set = false;   /* once, before any capture of a launch */                               \
+    if (!attr_set) {                                                                                         \
+      (void)hipFuncSetAttribute((const void*)k_at, hipFuncAttributeMaxDynamicSharedMemorySize, CC::LDS);     \
+      (void)hipFuncSetAttribute((const void*)k_st, hipFuncAttributeMaxDynamicSharedMemorySize, CC::LDS);     \
+      attr_set = true;                                                                                       \
+    }                                                                                                        \
+    if (atomic)                                                                                              \
+      k_at<<<grid, CC::NT, CC::LDS, s>>>(items, nitems, nslice, patP, patA);                                 \
+    else                                                                                                     \
+      k_st<<<grid, CC::NT, CC::LDS, s>>>(items, nitems, nslice, patP, patA);                                 \
+  } while (0)
+#define G_CFGS(TT)                                                      \
+  do {                                                                  \
+    switch (cfg) {                                                      \
+      case 0: G_LAUNCH(TT, 128, 128, 64, 64, 2, 2, true); break;        \
+      case 1: G_LAUNCH(TT, 256, 128, 64, 64, 3, 1, true); break;        \
+      case 2: G_LAUNCH(TT, 256, 128, 128, 64, 3, 1, false); break;      \
+      default: G_LAUNCH(TT, 128, 256, 64, 128, 3, 1, false); break;     \
+    }                                                                   \
+  } while (0)
+  if (prec == 2) G_CFGS(__bf16); else G_CFGS(_Float16);
+#undef G_CFGS
+#undef G_LAUNCH
 }

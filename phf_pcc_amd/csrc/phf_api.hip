@@ -28,107 +28,22 @@
 #include <string>
 #include <vector>
 
-#include "phf_common.h"
-#include "../../include/phf_hip.h"
-
-void lu_inverse_batched(hipStream_t s, int n, const double* M, long mbs, double* Minv, long ibs,
-                        double* det, int* info, int batch);
-int lu_inverse_max_n();
-void eri_gemm_f64(hipStream_t s, const double* P, const double* A, double* C, int Mp, int Nr, int K,
-                  long ldc, int bm, const double* bias);
-int eri_gemm_f64_bm(int rows);
-int mixed_segments(int prec);
-void split_operand(hipStream_t s, int prec, bool is_a, const double* src, long lds, int rows, int kt, void* dst,
-                   double* inv);
-void convert_f32(hipStream_t s, const double* src, float* dst, long n);
-void eri_gemm_mixed(hipStream_t s, int prec, const void* P, const void* A, const double* invP, const double* invA,
-                    double* C, int Mp, int Nr, int Kseg, long ldc, int cfg, int splitk, const double* bias);
-int h16_num_configs();
-int h16_cfg_bm(int c);
-int h16_cfg_bn(int c);
-double h16_model_cost(int c, int sk, long Mp0, int Nr0, int ncu);
-
-struct phf_ctx_s {
-  int device = 0;
-  int ncu = 256;
-  hipStream_t stream = nullptr;
-  std::string err;
-  // cached Hamiltonian
-  int nso = 0;
-  bool has_ham = false;
-  bool cplx = false;
-  int kpad = 0, nrpad = 0, kt = 0;
-  double* dA = nullptr;     // [nrpad][kt] FP64
-  double* dH1 = nullptr;    // hone, row-major complex128
-  double* dHb = nullptr;    // hone as ERI-GEMM bias: [2][nrpad] (re plane, im plane), r = i*nso + k
-  // reduced-precision copies of A, built on first use per mode:
-  // [nrpad][nseg*kt] of FP32 / BF16 / FP16 and per-row inverse scales
-  void* dAm[4] = {nullptr, nullptr, nullptr, nullptr};
-  double* dAinv[4] = {nullptr, nullptr, nullptr, nullptr};
-  // workspace arena
-  char* ws = nullptr;
-  size_t ws_size = 0;
-  // device-resident spin grid (phf_set_grid)
-  bool has_grid = false;
-  int g_nl = 0, g_ny = 0, g_npg = 0, g_nci = 0, g_ncisp = 0, g_ncipg = 0, g_qj = 0, g_sp = 0, g_nso = 0;
-  long g_G = 0;
-  double* dGr1p = nullptr;     // Ref^H R1[il]            [nl][nso x nso] column-major
-  double* dGr2 = nullptr;      // R2[iy]                  [ny] row-major (as given)
-  double* dGrpgref = nullptr;  // Rpg[ipg] Ref            [npg][nso x nso] column-major
-  double* dGw = nullptr;       // weight x Wigner x weightpg  [G][nci][nci]
-  // results of the last evaluation kept on the device (phf_eval_resident / phf_localg_vo)
-  double* r_ysum = nullptr;   // [nci*nci] x (NSO x N) column-major, sum_g Wd Y_g
-  double* r_xsum = nullptr;   // [nci*nci] x (NSO x N), sum_g Wd X_g
-  int r_nocc = 0;
-  bool r_valid = false;
-  // small persistent device buffer for per-call arguments/results outside the arena
-  char* small = nullptr;
-  size_t small_size = 0;
-  // autotuned ERI GEMM tiles: (precision, rows, cols, K) -> (config, split-K)
-  int autotune = 1;
-  std::map<std::tuple<int, long, int, int>, std::pair<int, int>> tuned;
-  std::string tune_log;
-  // per-stage timing with HIP events on `stream` (phf_set_profiling)
-  bool profiling = false;
-  hipEvent_t ev[8] = {};
-  double stage_ms[8] = {};
-  int nstage = 0;
-};
+#include "phf_internal.h"
 
 static thread_local std::string g_create_err;
 static int free_grid(phf_ctx* ctx);
 static int drop_hamiltonian(phf_ctx* ctx);
 
-namespace {
-
-typedef std::complex<double> cplx;
-
-long round_up(long x, long m) { return (x + m - 1) / m * m; }
-
-int fail(phf_ctx* ctx, int code, const std::string& msg) {
+int phf_fail(phf_ctx* ctx, int code, const std::string& msg) {
   if (ctx) ctx->err = msg;
   return code;
 }
 
-#define HIPCK(expr)                                                                   \
-  do {                                                                                \
-    hipError_t e_ = (expr);                                                           \
-    if (e_ != hipSuccess)                                                             \
-      return fail(ctx, e_ == hipErrorOutOfMemory ? PHF_ERR_NOMEM : PHF_ERR_HIP,       \
-                  std::string(#expr) + ": " + hipGetErrorString(e_));                 \
-  } while (0)
+namespace {
 
-struct Arena {
-  char* base;
-  size_t off = 0;
-  template <class T>
-  T* take(size_t count) {
-    off = (off + 255) & ~size_t(255);
-    T* p = reinterpret_cast<T*>(base ? base + off : nullptr);
-    off += count * sizeof(T);
-    return p;
-  }
-};
+typedef std::complex<double> cplx;
+
+int fail(phf_ctx* ctx, int code, const std::string& msg) { return phf_fail(ctx, code, msg); }
 
 // ---------------------------------------------------------------------------
 // small kernels
@@ -139,11 +54,6 @@ __global__ void fill_zero_kernel(double* p, long n) {
     p[i] = 0.0;
 }
 
-void fill_zero(hipStream_t s, double* p, long n) {
-  if (n <= 0) return;
-  long blocks = std::min<long>((n + 255) / 256, 65536);
-  fill_zero_kernel<<<blocks, 256, 0, s>>>(p, n);
-}
 
 __global__ void imag_nonzero_kernel(const double* stage, long nelem, int* flag) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nelem; i += (long)gridDim.x * blockDim.x)
@@ -446,9 +356,40 @@ cplx wignerfac(int j, int n, int m, double alpha, double beta, double gama) {
 
 }  // namespace
 
+void fill_zero(hipStream_t s, double* p, long n) {
+  if (n <= 0) return;
+  long blocks = std::min<long>((n + 255) / 256, 65536);
+  fill_zero_kernel<<<blocks, 256, 0, s>>>(p, n);
+}
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
+
+// Device check and stream set-up of a new context: its own non-blocking
+// stream, or `shared_stream` (owned by the caller) when not null.
+int phf_ctx_init(phf_ctx* ctx, int device, hipStream_t shared_stream) {
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0)
+    return fail(ctx, PHF_ERR_HIP, std::string("no HIP device (") + hipGetErrorString(e) + ")");
+  if (device < 0 || device >= ndev) return fail(ctx, PHF_ERR_ARG, "device index out of range");
+  HIPCK(hipSetDevice(device));
+  ctx->device = device;
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+      ctx->ncu = prop.multiProcessorCount;
+  }
+  if (shared_stream) {
+    ctx->stream = shared_stream;
+    ctx->owns_stream = false;
+  } else {
+    HIPCK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    ctx->owns_stream = true;
+  }
+  return PHF_OK;
+}
 
 extern "C" int phf_ctx_create(int device, phf_ctx** out) {
   if (!out) {
@@ -456,33 +397,12 @@ extern "C" int phf_ctx_create(int device, phf_ctx** out) {
     return PHF_ERR_ARG;
   }
   *out = nullptr;
-  int ndev = 0;
-  hipError_t e = hipGetDeviceCount(&ndev);
-  if (e != hipSuccess || ndev == 0) {
-    g_create_err = std::string("phf_ctx_create: no HIP device (") + hipGetErrorString(e) + ")";
-    return PHF_ERR_HIP;
-  }
-  if (device < 0 || device >= ndev) {
-    g_create_err = "phf_ctx_create: device index out of range";
-    return PHF_ERR_ARG;
-  }
-  e = hipSetDevice(device);
-  if (e != hipSuccess) {
-    g_create_err = std::string("hipSetDevice: ") + hipGetErrorString(e);
-    return PHF_ERR_HIP;
-  }
   phf_ctx* ctx = new phf_ctx_s();
-  ctx->device = device;
-  {
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
-      ctx->ncu = prop.multiProcessorCount;
-  }
-  e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-  if (e != hipSuccess) {
-    g_create_err = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+  const int rc = phf_ctx_init(ctx, device, nullptr);
+  if (rc) {
+    g_create_err = "phf_ctx_create: " + ctx->err;
     delete ctx;
-    return PHF_ERR_HIP;
+    return rc;
   }
   *out = ctx;
   return PHF_OK;
@@ -505,13 +425,24 @@ extern "C" int phf_ctx_destroy(phf_ctx* ctx) {
     if (q) (void)hipFree(q);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
-  (void)hipStreamDestroy(ctx->stream);
+  if (ctx->owns_stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return PHF_OK;
 }
 
 extern "C" const char* phf_last_error(const phf_ctx* ctx) {
   return ctx ? ctx->err.c_str() : g_create_err.c_str();
+}
+
+// Free the workspace arena (the next user allocates it again at its size).
+int phf_release_ws(phf_ctx* ctx) {
+  ctx->r_valid = false;
+  if (!ctx->ws) return PHF_OK;
+  HIPCK(hipStreamSynchronize(ctx->stream));
+  HIPCK(hipFree(ctx->ws));
+  ctx->ws = nullptr;
+  ctx->ws_size = 0;
+  return PHF_OK;
 }
 
 static int ensure_ws(phf_ctx* ctx, size_t bytes) {
@@ -587,17 +518,37 @@ extern "C" int phf_set_hamiltonian(phf_ctx* ctx, const double* hone, const doubl
 // Reduced-precision copy of the cached A for mode prec (1 FP32, 2 BF16X3, 3 F16X2).
 static int ensure_mixed_a(phf_ctx* ctx, int prec) {
   if (ctx->dAm[prec]) return PHF_OK;
-  const int nseg = mixed_segments(prec);
-  const long elems = (long)ctx->nrpad * ctx->kt * nseg;
+  if (!ctx->dA) return fail(ctx, PHF_ERR_UNSUPPORTED, "the cached Hamiltonian is stored for another precision only");
+  const long elems = (long)ctx->nrpad * ctx->kt * mixed_slices(prec);
   const size_t esz = (prec == 1) ? sizeof(float) : 2;
   HIPCK(hipMalloc(&ctx->dAm[prec], elems * esz));
   if (prec == 1) {
     convert_f32(ctx->stream, ctx->dA, (float*)ctx->dAm[prec], (long)ctx->nrpad * ctx->kt);
   } else {
     HIPCK(hipMalloc(&ctx->dAinv[prec], sizeof(double) * ctx->nrpad));
-    split_operand(ctx->stream, prec, true, ctx->dA, ctx->kt, ctx->nrpad, ctx->kt, ctx->dAm[prec], ctx->dAinv[prec]);
+    split_operand(ctx->stream, prec, ctx->dA, ctx->kt, ctx->nrpad, ctx->kt, ctx->dAm[prec], ctx->dAinv[prec]);
   }
   HIPCK(hipGetLastError());
+  return PHF_OK;
+}
+
+// Keep only the reduced-precision copy of A for mode prec (1..3) and release
+// the FP64 A: the Hamiltonian then serves that mode only.
+int phf_keep_only_mixed(phf_ctx* ctx, int prec) {
+  if (!ctx->has_ham) return fail(ctx, PHF_ERR_NO_HAMILTONIAN, "no Hamiltonian set");
+  if (prec < PHF_PREC_FP32 || prec > PHF_PREC_F16X2) return fail(ctx, PHF_ERR_ARG, "not a reduced-precision mode");
+  if (ctx->a_only && ctx->a_only != prec) return fail(ctx, PHF_ERR_ARG, "A already reduced to another mode");
+  {
+    int rc = ensure_mixed_a(ctx, prec);
+    if (rc) return rc;
+  }
+  HIPCK(hipStreamSynchronize(ctx->stream));
+  if (ctx->dA) {
+    HIPCK(hipFree(ctx->dA));
+    ctx->dA = nullptr;
+  }
+  ctx->a_only = prec;
+  ctx->tuned.clear();
   return PHF_OK;
 }
 
@@ -613,6 +564,7 @@ static int drop_hamiltonian(phf_ctx* ctx) {
     if (ctx->dAinv[q]) { HIPCK(hipFree(ctx->dAinv[q])); ctx->dAinv[q] = nullptr; }
   }
   if (ctx->dA) { HIPCK(hipFree(ctx->dA)); ctx->dA = nullptr; }
+  ctx->a_only = 0;
   if (ctx->dH1) { HIPCK(hipFree(ctx->dH1)); ctx->dH1 = nullptr; }
   if (ctx->dHb) { HIPCK(hipFree(ctx->dHb)); ctx->dHb = nullptr; }
   ctx->tuned.clear();
@@ -896,47 +848,8 @@ extern "C" int phf_tuning_report(const phf_ctx* ctx, char* buf, int len) {
   return n;
 }
 
-static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int precision, double* hmat,
-                  double* smat, double* gradmat, double* rdmmat, bool resident);
-
-extern "C" int phf_buildhsg_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int precision, double* hmat,
-                              double* smat, double* gradmat, double* rdmmat) {
-  if (!gradmat || !rdmmat) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: NULL output pointer");
-  return eval_z(ctx, z, cmplxconj, nocc, precision, hmat, smat, gradmat, rdmmat, false);
-}
-
-extern "C" int phf_eval_resident(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int precision,
-                                 double* hmat, double* smat) {
-  return eval_z(ctx, z, cmplxconj, nocc, precision, hmat, smat, nullptr, nullptr, true);
-}
-
-static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int precision, double* hmat,
-                  double* smat, double* gradmat, double* rdmmat, bool resident) {
-  if (!ctx) return PHF_ERR_ARG;
-  ctx->r_valid = false;
-  if (!ctx->has_ham) return fail(ctx, PHF_ERR_NO_HAMILTONIAN, "phf_buildhsg: no Hamiltonian set");
-  if (!ctx->has_grid || ctx->g_nso != ctx->nso)
-    return fail(ctx, PHF_ERR_ARG, "phf_buildhsg_z: no spin grid set for this Hamiltonian (phf_set_grid)");
-  const int nso = ctx->nso;
-  if (!hmat || !smat) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: NULL output pointer");
-  if (nocc <= 0 || nocc > nso) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: nocc out of range");
-  if (nocc > lu_inverse_max_n())
-    return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: nocc above the LDS inverse limit");
-  if (nocc < nso && !z) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: z is NULL");
-  if (cmplxconj != 0 && cmplxconj != 1)
-    return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: CmplxConj=2 is not implemented on the GPU path");
-  if (precision < PHF_PREC_FP64 || precision > PHF_PREC_F16X2)
-    return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: unknown precision");
-  HIPCK(hipSetDevice(ctx->device));
-  hipStream_t s = ctx->stream;
-  const int nl = ctx->g_nl, ny = ctx->g_ny, npg = ctx->g_npg, nci = ctx->g_nci;
-  const int N = nocc;
-  const long G = ctx->g_G;
-  const int nmn = nci * nci;
-  const long n2 = (long)nso * nso;
-
-  // ---- host: C0 = [I; Z] (column-major)
-  std::vector<double> c0(2 * (size_t)nso * N, 0.0);
+void build_c0(const double* z, int nso, int N, int cmplxconj, double* c0) {
+  std::fill(c0, c0 + 2 * (size_t)nso * N, 0.0);
   for (int a = 0; a < N; ++a) c0[2 * ((size_t)a + (size_t)a * nso)] = 1.0;
   for (int i = N; i < nso; ++i)
     for (int a = 0; a < N; ++a) {
@@ -944,113 +857,136 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
       c0[2 * ((size_t)i + (size_t)a * nso)] = z[src];
       c0[2 * ((size_t)i + (size_t)a * nso) + 1] = (cmplxconj == 0) ? 0.0 : z[src + 1];
     }
+}
 
-  // ---- workspace layout
+int eval_plan(phf_ctx* ctx, int nocc, int precision, EvalBufs& b) {
+  if (!ctx->has_ham) return fail(ctx, PHF_ERR_NO_HAMILTONIAN, "phf_buildhsg: no Hamiltonian set");
+  if (!ctx->has_grid || ctx->g_nso != ctx->nso)
+    return fail(ctx, PHF_ERR_ARG, "phf_buildhsg_z: no spin grid set for this Hamiltonian (phf_set_grid)");
+  const int nso = ctx->nso;
+  if (nocc <= 0 || nocc > nso) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: nocc out of range");
+  if (nocc > lu_inverse_max_n())
+    return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: nocc above the LDS inverse limit");
+  if (precision < PHF_PREC_FP64 || precision > PHF_PREC_F16X2)
+    return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: unknown precision");
+  if (precision == PHF_PREC_FP64 && !ctx->dA)
+    return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: the Hamiltonian is stored for a reduced precision only");
+  const int npg = ctx->g_npg, ny = ctx->g_ny, nci = ctx->g_nci;
+  const int N = nocc;
+  const long G = ctx->g_G;
+  const int nmn = nci * nci;
+  const long n2 = (long)nso * nso;
   const int kt = ctx->kt, nrpad = ctx->nrpad;
+  b.N = N;
+  b.prec = precision;
+  b.G = G;
   // rows are padded for the tallest tile any candidate uses
-  const int bm = (precision == PHF_PREC_FP64) ? eri_gemm_f64_bm((int)(2 * G)) : 256;
-  const long Mp = round_up(2 * G, bm);
-  const int gchunk = 32;
-  const int nchunk = (int)((G + gchunk - 1) / gchunk);
+  b.bm = (precision == PHF_PREC_FP64) ? eri_gemm_f64_bm((int)(2 * G)) : 256;
+  b.Mp = round_up(2 * G, b.bm);
+  b.gchunk = 32;
+  b.nchunk = (int)((G + b.gchunk - 1) / b.gchunk);
+  const long Mp = b.Mp;
   const long lenX = (long)nso * N;
-  Arena A0{nullptr};
-  auto carve = [&](Arena& a, auto&& sink) {
-    sink(a.template take<double>(2 * lenX), "c0");
-    sink(a.template take<double>(2 * lenX * npg), "wpg");
-    sink(a.template take<double>(2 * lenX * npg * ny), "w2");
-    sink(a.template take<double>(2 * lenX * G), "t");
-    sink(a.template take<double>(2 * (size_t)N * N * G), "m");
-    sink(a.template take<double>(2 * (size_t)N * N * G), "minv");
-    sink(a.template take<double>(2 * (size_t)G), "det");
-    sink(a.template take<double>((size_t)G), "info");
-    sink(a.template take<double>(2 * lenX * G), "x");
-    sink(a.template take<double>((size_t)Mp * kt), "pt");
-    sink(a.template take<double>((size_t)Mp * nrpad), "gam");
-    sink(a.template take<double>(2 * lenX * G), "fx");
-    sink(a.template take<double>(2 * (size_t)N * N * G), "q");
-    sink(a.template take<double>(2 * lenX), "hc");
-    sink(a.template take<double>(2 * (size_t)G), "e");
-    sink(a.template take<double>(2 * lenX * G), "y");
-    sink(a.template take<double>(2 * (size_t)G * nmn), "wd");
-    sink(a.template take<double>(2 * (size_t)nchunk * nmn * lenX), "part");
-    sink(a.template take<double>(2 * (size_t)nmn * lenX), "ysum");
-    sink(a.template take<double>(2 * (size_t)nmn * lenX), "xsum");
-    sink(a.template take<double>(4 * (size_t)nmn), "hs");
-    sink(a.template take<double>(2 * (size_t)nmn * n2), "grad");
-    sink(a.template take<double>(2 * (size_t)nmn * n2), "rdm");
+  auto carve = [&](Arena& a, EvalBufs& o) {
+    o.C0 = a.take<double>(2 * lenX);
+    o.Wpg = a.take<double>(2 * lenX * npg);
+    o.W2 = a.take<double>(2 * lenX * npg * ny);
+    o.T = a.take<double>(2 * lenX * G);
+    o.M = a.take<double>(2 * (size_t)N * N * G);
+    o.Minv = a.take<double>(2 * (size_t)N * N * G);
+    o.Det = a.take<double>(2 * (size_t)G);
+    o.Info = reinterpret_cast<int*>(a.take<double>((size_t)G));
+    o.X = a.take<double>(2 * lenX * G);
+    o.Pt = a.take<double>((size_t)Mp * kt);
+    o.Gam = a.take<double>((size_t)Mp * nrpad);
+    o.FX = a.take<double>(2 * lenX * G);
+    o.Q = a.take<double>(2 * (size_t)N * N * G);
+    o.Hc = a.take<double>(2 * lenX);
+    o.E = a.take<double>(2 * (size_t)G);
+    o.Y = a.take<double>(2 * lenX * G);
+    o.Wd = a.take<double>(2 * (size_t)G * nmn);
+    o.Part = a.take<double>(2 * (size_t)b.nchunk * nmn * lenX);
+    o.Ysum = a.take<double>(2 * (size_t)nmn * lenX);
+    o.Xsum = a.take<double>(2 * (size_t)nmn * lenX);
+    o.HS = a.take<double>(4 * (size_t)nmn);
+    o.Grad = a.take<double>(2 * (size_t)nmn * n2);
+    o.Rdm = a.take<double>(2 * (size_t)nmn * n2);
     // reduced-precision density rows and their inverse scales
-    const size_t pm = (precision == PHF_PREC_FP64) ? 0 : (size_t)Mp * kt * mixed_segments(precision);
-    sink(a.template take<double>(precision == PHF_PREC_FP32 ? (pm + 1) / 2 : (pm + 3) / 4), "pm");
-    sink(a.template take<double>(precision == PHF_PREC_FP64 ? 0 : (size_t)Mp), "pinv");
+    const size_t pm = (precision == PHF_PREC_FP64) ? 0 : (size_t)Mp * kt * mixed_slices(precision);
+    o.Pm = a.take<double>(precision == PHF_PREC_FP32 ? (pm + 1) / 2 : (pm + 3) / 4);
+    o.Pinv = a.take<double>(precision == PHF_PREC_FP64 ? 0 : (size_t)Mp);
   };
-  carve(A0, [](double*, const char*) {});
+  Arena a0{nullptr};
+  EvalBufs dummy;
+  carve(a0, dummy);
   {
-    int rc = ensure_ws(ctx, A0.off + 256);
+    int rc = ensure_ws(ctx, a0.off + 256);
     if (rc) return rc;
   }
-  std::vector<double*> ptrs;
-  Arena A1{ctx->ws};
-  carve(A1, [&](double* p, const char*) { ptrs.push_back(p); });
-  int pi = 0;
-  double *dC0 = ptrs[pi++], *dWpg = ptrs[pi++], *dW2 = ptrs[pi++];
-  double *dT = ptrs[pi++], *dM = ptrs[pi++], *dMinv = ptrs[pi++], *dDet = ptrs[pi++];
-  int* dInfo = reinterpret_cast<int*>(ptrs[pi++]);
-  double *dX = ptrs[pi++], *dPt = ptrs[pi++], *dGam = ptrs[pi++], *dFX = ptrs[pi++], *dQ = ptrs[pi++];
-  double *dHc = ptrs[pi++], *dE = ptrs[pi++], *dY = ptrs[pi++], *dWd = ptrs[pi++], *dPart = ptrs[pi++];
-  double *dYsum = ptrs[pi++], *dXsum = ptrs[pi++], *dHS = ptrs[pi++], *dGrad = ptrs[pi++], *dRdm = ptrs[pi++];
-  void* dPm = ptrs[pi++];
-  double* dPinv = ptrs[pi++];
-  double* dW = ctx->dGw;
+  Arena a1{ctx->ws};
+  carve(a1, b);
   if (precision != PHF_PREC_FP64) {
     int rc = ensure_mixed_a(ctx, precision);
     if (rc) return rc;
   }
-  HIPCK(hipMemcpyAsync(dC0, c0.data(), c0.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  return PHF_OK;
+}
 
-  if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[0], s));
+void eval_pre(phf_ctx* ctx, const EvalBufs& b) {
+  hipStream_t s = ctx->stream;
+  const int nso = ctx->nso, N = b.N, npg = ctx->g_npg, ny = ctx->g_ny, nl = ctx->g_nl;
+  const long G = b.G, n2 = (long)nso * nso, lenX = (long)nso * N;
+  const int kt = ctx->kt;
   const long sT = 2 * lenX;          // batch stride of NSO x N column-major blocks
   const long sNN = 2L * N * N;
   // K1: rotation chain, right to left (Ref^H R1 and Rpg Ref are cached with the grid)
   // Wpg[ipg] = (Rpg[ipg] Ref) C0
-  zgemm_batched(s, nso, N, nso, 1, 0, zmat_colmajor(ctx->dGrpgref, nso, 2 * n2), zmat_colmajor(dC0, nso, 0), 0, 0,
-                zout_colmajor(dWpg, nso, sT), npg);
+  zgemm_batched(s, nso, N, nso, 1, 0, zmat_colmajor(ctx->dGrpgref, nso, 2 * n2), zmat_colmajor(b.C0, nso, 0), 0, 0,
+                zout_colmajor(b.Wpg, nso, sT), npg);
   // W2[iy] = R2[iy] [Wpg_0 .. Wpg_{npg-1}]
   zgemm_batched(s, nso, N * npg, nso, 1, 0, ZMat{ctx->dGr2, 2L * nso, 2, 2 * n2, 1, 0},
-                zmat_colmajor(dWpg, nso, 0), 0, 0, zout_colmajor(dW2, nso, sT * npg), ny);
+                zmat_colmajor(b.Wpg, nso, 0), 0, 0, zout_colmajor(b.W2, nso, sT * npg), ny);
   // T[il] = (Ref^H R1[il]) W2   -> T_g = columns g*N .. g*N+N-1
-  zgemm_batched(s, nso, N * npg * ny, nso, 1, 0, zmat_colmajor(ctx->dGr1p, nso, 2 * n2), zmat_colmajor(dW2, nso, 0),
-                0, 0, zout_colmajor(dT, nso, sT * npg * ny), nl);
+  zgemm_batched(s, nso, N * npg * ny, nso, 1, 0, zmat_colmajor(ctx->dGr1p, nso, 2 * n2), zmat_colmajor(b.W2, nso, 0),
+                0, 0, zout_colmajor(b.T, nso, sT * npg * ny), nl);
   // M_g = C0^H T_g
-  zgemm_batched(s, N, N, nso, 1, 0, ZMat{dC0, 2L * nso, 2, 0, 1, 1}, zmat_colmajor(dT, nso, sT), 0, 0,
-                zout_colmajor(dM, N, sNN), (int)G);
+  zgemm_batched(s, N, N, nso, 1, 0, ZMat{b.C0, 2L * nso, 2, 0, 1, 1}, zmat_colmajor(b.T, nso, sT), 0, 0,
+                zout_colmajor(b.M, N, sNN), (int)G);
   // K2: inverse and determinant
-  lu_inverse_batched(s, N, dM, sNN, dMinv, sNN, dDet, dInfo, (int)G);
+  lu_inverse_batched(s, N, b.M, sNN, b.Minv, sNN, b.Det, b.Info, (int)G);
   // X_g = T_g M_g^{-1}
-  zgemm_batched(s, nso, N, N, 1, 0, zmat_colmajor(dT, nso, sT), zmat_colmajor(dMinv, N, sNN), 0, 0,
-                zout_colmajor(dX, nso, sT), (int)G);
+  zgemm_batched(s, nso, N, N, 1, 0, zmat_colmajor(b.T, nso, sT), zmat_colmajor(b.Minv, N, sNN), 0, 0,
+                zout_colmajor(b.X, nso, sT), (int)G);
   // density pack: Pt row 2g = Re rho_g^T, row 2g+1 = Im rho_g^T (flattened j*nso + l)
-  fill_zero(s, dPt, Mp * kt);
-  zgemm_batched(s, nso, nso, N, 1, 0, ZMat{dC0, 2, 2L * nso, 0, 1, 1}, ZMat{dX, 2L * nso, 2, sT, 1, 0}, 0, 0,
-                ZOut{dPt, nso, 1, 2L * kt, kt}, (int)G);
+  fill_zero(s, b.Pt, b.Mp * kt);
+  zgemm_batched(s, nso, nso, N, 1, 0, ZMat{b.C0, 2, 2L * nso, 0, 1, 1}, ZMat{b.X, 2L * nso, 2, sT, 1, 0}, 0, 0,
+                ZOut{b.Pt, nso, 1, 2L * kt, kt}, (int)G);
   if (ctx->cplx) {
     dim3 grid((unsigned)std::min<long>((n2 + 255) / 256, 1024), (unsigned)G);
-    pack_cplx_kernel<<<grid, 256, 0, s>>>(dPt, (int)G, (int)n2, ctx->kpad, kt);
+    pack_cplx_kernel<<<grid, 256, 0, s>>>(b.Pt, (int)G, (int)n2, ctx->kpad, kt);
   }
-  // K3: Gamma for all grid points
-  if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[1], s));
+  // operand of the reduced-precision ERI GEMM
+  if (b.prec == PHF_PREC_FP32)
+    convert_f32(s, b.Pt, (float*)b.Pm, b.Mp * kt);
+  else if (b.prec != PHF_PREC_FP64)
+    split_operand(s, b.prec, b.Pt, kt, (int)b.Mp, kt, b.Pm, b.Pinv);
+}
+
+int eval_gemm(phf_ctx* ctx, const EvalBufs& b) {
+  hipStream_t s = ctx->stream;
+  const int kt = ctx->kt, nrpad = ctx->nrpad, precision = b.prec;
+  const long G = b.G;
   if (precision == PHF_PREC_FP64) {
-    eri_gemm_f64(s, dPt, ctx->dA, dGam, (int)Mp, nrpad, kt, nrpad, bm, ctx->dHb);
+    eri_gemm_f64(s, b.Pt, ctx->dA, b.Gam, (int)b.Mp, nrpad, kt, nrpad, b.bm, ctx->dHb);
   } else if (precision == PHF_PREC_FP32) {
-    convert_f32(s, dPt, (float*)dPm, Mp * kt);
-    eri_gemm_mixed(s, precision, dPm, ctx->dAm[precision], nullptr, nullptr, dGam, (int)Mp, nrpad, kt, nrpad, 128, 1,
-                   ctx->dHb);
+    eri_gemm_mixed(s, precision, b.Pm, ctx->dAm[precision], nullptr, nullptr, b.Gam, (int)b.Mp, nrpad, kt, nrpad,
+                   128, 1, ctx->dHb);
   } else {
-    split_operand(s, precision, false, dPt, kt, (int)Mp, kt, dPm, dPinv);
     const int kseg = kt * mixed_segments(precision);
     auto run = [&](int cfg, int sk) {
       const int mp = (int)round_up(2 * G, h16_cfg_bm(cfg));
-      if (sk > 1) fill_zero(s, dGam, (long)mp * nrpad);
-      eri_gemm_mixed(s, precision, dPm, ctx->dAm[precision], dPinv, ctx->dAinv[precision], dGam, mp, nrpad, kseg,
+      if (sk > 1) fill_zero(s, b.Gam, (long)mp * nrpad);
+      eri_gemm_mixed(s, precision, b.Pm, ctx->dAm[precision], b.Pinv, ctx->dAinv[precision], b.Gam, mp, nrpad, kt,
                      nrpad, cfg, sk, ctx->dHb);
     };
     const auto key = std::make_tuple(precision, 2 * G, nrpad, kseg);
@@ -1062,48 +998,89 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
     }
     run(it->second.first, it->second.second);
   }
-  if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[2], s));
+  return PHF_OK;
+}
+
+void eval_post(phf_ctx* ctx, const EvalBufs& b, bool resident) {
+  hipStream_t s = ctx->stream;
+  const int nso = ctx->nso, N = b.N, nci = ctx->g_nci, nrpad = ctx->nrpad;
+  const int nmn = nci * nci;
+  const long G = b.G, n2 = (long)nso * nso, lenX = (long)nso * N;
+  const long sT = 2 * lenX, sNN = 2L * N * N;
   // K4: FX = (h + Gamma) X   (the ERI GEMM epilogue added h)
-  zgemm_batched(s, nso, N, nso, 1, 0, ZMat{dGam, nso, 1, 2L * nrpad, nrpad, 0}, zmat_colmajor(dX, nso, sT), 0, 0,
-                zout_colmajor(dFX, nso, sT), (int)G);
+  zgemm_batched(s, nso, N, nso, 1, 0, ZMat{b.Gam, nso, 1, 2L * nrpad, nrpad, 0}, zmat_colmajor(b.X, nso, sT), 0, 0,
+                zout_colmajor(b.FX, nso, sT), (int)G);
   // Q_g = C0^H FX_g ; Hc = C0^H h
-  zgemm_batched(s, N, N, nso, 1, 0, ZMat{dC0, 2L * nso, 2, 0, 1, 1}, zmat_colmajor(dFX, nso, sT), 0, 0,
-                zout_colmajor(dQ, N, sNN), (int)G);
-  zgemm_batched(s, N, nso, nso, 1, 0, ZMat{dC0, 2L * nso, 2, 0, 1, 1}, ZMat{ctx->dH1, 2L * nso, 2, 0, 1, 0}, 0, 0,
-                zout_colmajor(dHc, N, 0), 1);
-  energy_kernel<<<(unsigned)G, 256, 0, s>>>(nso, N, dHc, dX, dQ, dE);
+  zgemm_batched(s, N, N, nso, 1, 0, ZMat{b.C0, 2L * nso, 2, 0, 1, 1}, zmat_colmajor(b.FX, nso, sT), 0, 0,
+                zout_colmajor(b.Q, N, sNN), (int)G);
+  zgemm_batched(s, N, nso, nso, 1, 0, ZMat{b.C0, 2L * nso, 2, 0, 1, 1}, ZMat{ctx->dH1, 2L * nso, 2, 0, 1, 0}, 0, 0,
+                zout_colmajor(b.Hc, N, 0), 1);
+  energy_kernel<<<(unsigned)G, 256, 0, s>>>(nso, N, b.Hc, b.X, b.Q, b.E);
   // Y_g = FX_g + e_g X_g - X_g Q_g
-  yinit_kernel<<<(unsigned)std::min<long>((lenX * G + 255) / 256, 65536), 256, 0, s>>>(lenX, (int)G, dFX, dX, dE, dY);
-  zgemm_batched(s, nso, N, N, -1, 0, zmat_colmajor(dX, nso, sT), zmat_colmajor(dQ, N, sNN), 1, 0,
-                zout_colmajor(dY, nso, sT), (int)G);
+  yinit_kernel<<<(unsigned)std::min<long>((lenX * G + 255) / 256, 65536), 256, 0, s>>>(lenX, (int)G, b.FX, b.X, b.E,
+                                                                                         b.Y);
+  zgemm_batched(s, nso, N, N, -1, 0, zmat_colmajor(b.X, nso, sT), zmat_colmajor(b.Q, N, sNN), 1, 0,
+                zout_colmajor(b.Y, nso, sT), (int)G);
   // K5: weighted grid sums
-  wd_kernel<<<(unsigned)((G * nmn + 255) / 256), 256, 0, s>>>((int)G, nmn, dW, dDet, dWd);
-  hs_reduce_kernel<<<1, 256, 0, s>>>((int)G, nmn, dWd, dE, dHS);
+  wd_kernel<<<(unsigned)((G * nmn + 255) / 256), 256, 0, s>>>((int)G, nmn, ctx->dGw, b.Det, b.Wd);
+  hs_reduce_kernel<<<1, 256, 0, s>>>((int)G, nmn, b.Wd, b.E, b.HS);
   {
-    dim3 grid((unsigned)((lenX + 255) / 256), (unsigned)nchunk);
+    dim3 grid((unsigned)((lenX + 255) / 256), (unsigned)b.nchunk);
     const unsigned fin = (unsigned)((lenX * nmn + 255) / 256);
-    wreduce_partial_kernel<<<grid, 256, 0, s>>>((int)G, gchunk, (int)lenX, nmn, dWd, dY, dPart);
-    wreduce_final_kernel<<<fin, 256, 0, s>>>(nchunk, (int)lenX, nmn, dPart, dYsum);
-    wreduce_partial_kernel<<<grid, 256, 0, s>>>((int)G, gchunk, (int)lenX, nmn, dWd, dX, dPart);
-    wreduce_final_kernel<<<fin, 256, 0, s>>>(nchunk, (int)lenX, nmn, dPart, dXsum);
+    wreduce_partial_kernel<<<grid, 256, 0, s>>>((int)G, b.gchunk, (int)lenX, nmn, b.Wd, b.Y, b.Part);
+    wreduce_final_kernel<<<fin, 256, 0, s>>>(b.nchunk, (int)lenX, nmn, b.Part, b.Ysum);
+    wreduce_partial_kernel<<<grid, 256, 0, s>>>((int)G, b.gchunk, (int)lenX, nmn, b.Wd, b.X, b.Part);
+    wreduce_final_kernel<<<fin, 256, 0, s>>>(b.nchunk, (int)lenX, nmn, b.Part, b.Xsum);
   }
   // Gradmat[mn] = Ysum[mn] C0^H, Rdmmat[mn] = Xsum[mn] C0^H (C-order [nso][nso])
   if (!resident) {
-    zgemm_batched(s, nso, nso, N, 1, 0, zmat_colmajor(dYsum, nso, sT), ZMat{dC0, 2L * nso, 2, 0, 1, 1}, 0, 0,
-                  ZOut{dGrad, 2L * nso, 2, 2 * n2, 1}, nmn);
-    zgemm_batched(s, nso, nso, N, 1, 0, zmat_colmajor(dXsum, nso, sT), ZMat{dC0, 2L * nso, 2, 0, 1, 1}, 0, 0,
-                  ZOut{dRdm, 2L * nso, 2, 2 * n2, 1}, nmn);
+    zgemm_batched(s, nso, nso, N, 1, 0, zmat_colmajor(b.Ysum, nso, sT), ZMat{b.C0, 2L * nso, 2, 0, 1, 1}, 0, 0,
+                  ZOut{b.Grad, 2L * nso, 2, 2 * n2, 1}, nmn);
+    zgemm_batched(s, nso, nso, N, 1, 0, zmat_colmajor(b.Xsum, nso, sT), ZMat{b.C0, 2L * nso, 2, 0, 1, 1}, 0, 0,
+                  ZOut{b.Rdm, 2L * nso, 2, 2 * n2, 1}, nmn);
   }
+}
+
+static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int precision, double* hmat,
+                  double* smat, double* gradmat, double* rdmmat, bool resident) {
+  if (!ctx) return PHF_ERR_ARG;
+  ctx->r_valid = false;
+  if (!hmat || !smat) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: NULL output pointer");
+  if (cmplxconj != 0 && cmplxconj != 1)
+    return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: CmplxConj=2 is not implemented on the GPU path");
+  if (ctx->has_ham && nocc > 0 && nocc < ctx->nso && !z) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: z is NULL");
+  HIPCK(hipSetDevice(ctx->device));
+  EvalBufs b;
+  {
+    int rc = eval_plan(ctx, nocc, precision, b);
+    if (rc) return rc;
+  }
+  hipStream_t s = ctx->stream;
+  const int nso = ctx->nso, N = nocc, nci = ctx->g_nci, ny = ctx->g_ny, npg = ctx->g_npg;
+  const int nmn = nci * nci;
+  const long G = b.G, n2 = (long)nso * nso;
+  std::vector<double> c0(2 * (size_t)nso * N);
+  build_c0(z, nso, N, cmplxconj, c0.data());
+  HIPCK(hipMemcpyAsync(b.C0, c0.data(), c0.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[0], s));
+  eval_pre(ctx, b);
+  if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[1], s));
+  {
+    int rc = eval_gemm(ctx, b);
+    if (rc) return rc;
+  }
+  if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[2], s));
+  eval_post(ctx, b, resident);
   HIPCK(hipGetLastError());
   if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[3], s));
 
   std::vector<double> hs(4 * (size_t)nmn);
   std::vector<int> info((size_t)G);
-  HIPCK(hipMemcpyAsync(hs.data(), dHS, hs.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-  HIPCK(hipMemcpyAsync(info.data(), dInfo, info.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCK(hipMemcpyAsync(hs.data(), b.HS, hs.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCK(hipMemcpyAsync(info.data(), b.Info, info.size() * sizeof(int), hipMemcpyDeviceToHost, s));
   if (!resident) {
-    HIPCK(hipMemcpyAsync(gradmat, dGrad, 2 * (size_t)nmn * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPCK(hipMemcpyAsync(rdmmat, dRdm, 2 * (size_t)nmn * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCK(hipMemcpyAsync(gradmat, b.Grad, 2 * (size_t)nmn * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCK(hipMemcpyAsync(rdmmat, b.Rdm, 2 * (size_t)nmn * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
   }
   HIPCK(hipStreamSynchronize(s));
   if (ctx->profiling) {
@@ -1132,19 +1109,30 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
   double w1 = 0, w2 = 0, hmax = 0, smax = 0;
   for (int m = 0; m < nci; ++m)
     for (int n = 0; n < nci; ++n) {
-      const size_t a = 2 * ((size_t)m * nci + n), b = 2 * ((size_t)n * nci + m);
-      w1 = std::max(w1, std::fabs(-hmat[b + 1] - hmat[a + 1]));
-      w2 = std::max(w2, std::fabs(-smat[b + 1] - smat[a + 1]));
+      const size_t a = 2 * ((size_t)m * nci + n), c = 2 * ((size_t)n * nci + m);
+      w1 = std::max(w1, std::fabs(-hmat[c + 1] - hmat[a + 1]));
+      w2 = std::max(w2, std::fabs(-smat[c + 1] - smat[a + 1]));
       hmax = std::max(hmax, std::hypot(hmat[a], hmat[a + 1]));
       smax = std::max(smax, std::hypot(smat[a], smat[a + 1]));
     }
   const double tol = 1e-5;
   if (w1 > hmax * tol || w2 > smax * tol) return fail(ctx, PHF_ERR_NOT_HERMITIAN, "Not Hermitian");
-  ctx->r_ysum = dYsum;
-  ctx->r_xsum = dXsum;
+  ctx->r_ysum = b.Ysum;
+  ctx->r_xsum = b.Xsum;
   ctx->r_nocc = N;
   ctx->r_valid = true;
   return PHF_OK;
+}
+
+extern "C" int phf_buildhsg_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int precision, double* hmat,
+                              double* smat, double* gradmat, double* rdmmat) {
+  if (!gradmat || !rdmmat) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: NULL output pointer");
+  return eval_z(ctx, z, cmplxconj, nocc, precision, hmat, smat, gradmat, rdmmat, false);
+}
+
+extern "C" int phf_eval_resident(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int precision,
+                                 double* hmat, double* smat) {
+  return eval_z(ctx, z, cmplxconj, nocc, precision, hmat, smat, nullptr, nullptr, true);
 }
 
 extern "C" int phf_buildhsg(phf_ctx* ctx, const double* ref, const double* z, const int* ngrid, int nci,

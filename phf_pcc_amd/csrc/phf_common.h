@@ -44,6 +44,24 @@ inline ZOut zout_colmajor(double* p, long ld, long bs) { return ZOut{p, 2, 2 * l
 void zgemm_batched(hipStream_t s, int m, int n, int k, double ar, double ai, ZMat A, ZMat B,
                    double br, double bi, ZOut C, int batch);
 
+// One output tile (or one part of a split contraction) of a 16-bit ERI GEMM in
+// a grouped launch: rows mt*BM.. of P against rows nt*BN.. of A over the slabs
+// [sbeg, send) (64 elements each) of the segment sequence; K is the slice
+// length of this GEMM's operands, ldc the row stride of C and of the bias planes.
+struct H16Item {
+  const void* P;
+  const void* A;
+  const double* invP;
+  const double* invA;
+  double* C;
+  const double* bias;   // null, or [2][ldc] added to row parity planes (first part only)
+  long ldc;
+  int K;
+  int mt, nt;
+  int sbeg, send;
+  int atomic;           // 1: add into C (zero-filled by the caller)
+};
+
 // XCD-aware remap of a linear workgroup id: workgroups are dispatched
 // round-robin over the 8 XCDs; this gives each XCD a contiguous range of
 // tile ids so that tiles sharing an operand panel share that XCD's L2.

@@ -1,0 +1,136 @@
+// Internal interface shared by the host files of the library (not installed).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "phf_common.h"
+#include "../../include/phf_hip.h"
+
+void lu_inverse_batched(hipStream_t s, int n, const double* M, long mbs, double* Minv, long ibs,
+                        double* det, int* info, int batch);
+int lu_inverse_max_n();
+void eri_gemm_f64(hipStream_t s, const double* P, const double* A, double* C, int Mp, int Nr, int K,
+                  long ldc, int bm, const double* bias);
+int eri_gemm_f64_bm(int rows);
+int mixed_segments(int prec);
+int mixed_slices(int prec);
+void split_operand(hipStream_t s, int prec, const double* src, long lds, int rows, int kt, void* dst, double* inv);
+void convert_f32(hipStream_t s, const double* src, float* dst, long n);
+void eri_gemm_mixed(hipStream_t s, int prec, const void* P, const void* A, const double* invP, const double* invA,
+                    double* C, int Mp, int Nr, int K, long ldc, int cfg, int splitk, const double* bias);
+void eri_gemm_mixed_grouped(hipStream_t s, int prec, const H16Item* items, int nitems, int cfg, int grid,
+                            bool atomic);
+int h16_grouped_grid(int cfg, int ncu);
+int h16_num_configs();
+int h16_cfg_bm(int c);
+int h16_cfg_bn(int c);
+double h16_model_cost(int c, int sk, long Mp0, int Nr0, int ncu);
+
+struct phf_ctx_s {
+  int device = 0;
+  int ncu = 256;
+  hipStream_t stream = nullptr;
+  bool owns_stream = true;    // false for the instances of a batch (they share the batch's stream)
+  std::string err;
+  // cached Hamiltonian
+  int nso = 0;
+  bool has_ham = false;
+  bool cplx = false;
+  int kpad = 0, nrpad = 0, kt = 0;
+  double* dA = nullptr;     // [nrpad][kt] FP64
+  double* dH1 = nullptr;    // hone, row-major complex128
+  double* dHb = nullptr;    // hone as ERI-GEMM bias: [2][nrpad] (re plane, im plane), r = i*nso + k
+  // reduced-precision copies of A, built on first use per mode:
+  // [nrpad][nseg*kt] of FP32 / BF16 / FP16 and per-row inverse scales
+  void* dAm[4] = {nullptr, nullptr, nullptr, nullptr};
+  double* dAinv[4] = {nullptr, nullptr, nullptr, nullptr};
+  int a_only = 0;            // > 0: only the reduced-precision copy of this mode is stored (dA released)
+  // workspace arena
+  char* ws = nullptr;
+  size_t ws_size = 0;
+  // device-resident spin grid (phf_set_grid)
+  bool has_grid = false;
+  int g_nl = 0, g_ny = 0, g_npg = 0, g_nci = 0, g_ncisp = 0, g_ncipg = 0, g_qj = 0, g_sp = 0, g_nso = 0;
+  long g_G = 0;
+  double* dGr1p = nullptr;     // Ref^H R1[il]            [nl][nso x nso] column-major
+  double* dGr2 = nullptr;      // R2[iy]                  [ny] row-major (as given)
+  double* dGrpgref = nullptr;  // Rpg[ipg] Ref            [npg][nso x nso] column-major
+  double* dGw = nullptr;       // weight x Wigner x weightpg  [G][nci][nci]
+  // results of the last evaluation kept on the device (phf_eval_resident / phf_localg_vo)
+  double* r_ysum = nullptr;   // [nci*nci] x (NSO x N) column-major, sum_g Wd Y_g
+  double* r_xsum = nullptr;   // [nci*nci] x (NSO x N), sum_g Wd X_g
+  int r_nocc = 0;
+  bool r_valid = false;
+  // small persistent device buffer for per-call arguments/results outside the arena
+  char* small = nullptr;
+  size_t small_size = 0;
+  // autotuned ERI GEMM tiles: (precision, rows, cols, K) -> (config, split-K)
+  int autotune = 1;
+  std::map<std::tuple<int, long, int, int>, std::pair<int, int>> tuned;
+  std::string tune_log;
+  // per-stage timing with HIP events on `stream` (phf_set_profiling)
+  bool profiling = false;
+  hipEvent_t ev[8] = {};
+  double stage_ms[8] = {};
+  int nstage = 0;
+};
+
+inline long round_up(long x, long m) { return (x + m - 1) / m * m; }
+
+// Bump allocator over a device block (base == nullptr measures).
+struct Arena {
+  char* base;
+  size_t off = 0;
+  template <class T>
+  T* take(size_t count) {
+    off = (off + 255) & ~size_t(255);
+    T* p = reinterpret_cast<T*>(base ? base + off : nullptr);
+    off += count * sizeof(T);
+    return p;
+  }
+};
+
+int phf_fail(phf_ctx* ctx, int code, const std::string& msg);
+
+#define HIPCK(expr)                                                                   \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return phf_fail(ctx, e_ == hipErrorOutOfMemory ? PHF_ERR_NOMEM : PHF_ERR_HIP,   \
+                      std::string(#expr) + ": " + hipGetErrorString(e_));             \
+  } while (0)
+
+void fill_zero(hipStream_t s, double* p, long n);
+int phf_ctx_init(phf_ctx* ctx, int device, hipStream_t shared_stream);
+int phf_release_ws(phf_ctx* ctx);
+int phf_keep_only_mixed(phf_ctx* ctx, int prec);
+
+// Device buffers of one evaluation (carved from the context's workspace).
+struct EvalBufs {
+  int N = 0, prec = 0, bm = 0, gchunk = 32, nchunk = 0;
+  long Mp = 0, G = 0;
+  double *C0 = nullptr, *Wpg = nullptr, *W2 = nullptr, *T = nullptr, *M = nullptr, *Minv = nullptr, *Det = nullptr;
+  int* Info = nullptr;
+  double *X = nullptr, *Pt = nullptr, *Gam = nullptr, *FX = nullptr, *Q = nullptr, *Hc = nullptr, *E = nullptr;
+  double *Y = nullptr, *Wd = nullptr, *Part = nullptr, *Ysum = nullptr, *Xsum = nullptr, *HS = nullptr;
+  double *Grad = nullptr, *Rdm = nullptr;
+  void* Pm = nullptr;
+  double* Pinv = nullptr;
+};
+
+// Stages of one evaluation on ctx->stream.  eval_plan checks the cached state,
+// sizes and carves the workspace (may allocate; not for graph capture).
+// eval_pre runs K1/K2, the density pack and the reduced-precision split of
+// the density rows once C0 = [I; Z] is in b.C0; eval_gemm the single ERI GEMM
+// (autotuning a new 16-bit shape first, not for capture); eval_post K4/K5 and,
+// unless resident, Gradmat/Rdmmat.  eval_pre/eval_post only launch kernels.
+int eval_plan(phf_ctx* ctx, int nocc, int precision, EvalBufs& b);
+void eval_pre(phf_ctx* ctx, const EvalBufs& b);
+int eval_gemm(phf_ctx* ctx, const EvalBufs& b);
+void eval_post(phf_ctx* ctx, const EvalBufs& b, bool resident);
+// C0 = [I; Z] column-major complex (NSO x N) from Z ((NSO-N) x N, C order)
+void build_c0(const double* z, int nso, int N, int cmplxconj, double* c0);
