@@ -171,6 +171,164 @@ def run_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def _form_dtype(prec):
+    return {"fp64": "f64", "fp32": "f32 (FP64 accumulate)", "bf16x3": "bf16x3 split (FP64 accumulate)",
+            "f16x2": "f16x2 scaled split (FP64 accumulate)"}[prec]
+
+
+def run_large(args):
+    """n=128 (NSO=256), N=64, L=384, 50 x 24 grid: every precision form of the
+    device-resident E + gradient on one factored Hamiltonian; errors against
+    the FP64 form per energy and gradient."""
+    from phf_pcc_amd.hamiltonian import synthetic_problem
+    from phf_pcc_amd.model import ProjectedHF
+    from tests._cases import grid_args
+    n, nocc, ngrid = 128, 64, (50, 24)
+    t0 = time.time()
+    prob = synthetic_problem(n, nocc, seed=1003, ngrid=ngrid)
+    ga = grid_args(n, ngrid, 0)
+    model = ProjectedHF(nocc, prob.ref, ga, factors=(prob.h_spatial, prob.b))
+    model.ctx.set_profiling(True)
+    G = ngrid[0] * ngrid[1]
+    nso = 2 * n
+    log(f"[bench] n={n} problem and device Hamiltonian in {time.time()-t0:.1f}s")
+    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    if "fp64" not in forms:
+        forms = ["fp64"] + forms
+    res = {}
+    for f in forms:
+        model.precision = f
+        for _ in range(args.warmup):
+            model.energy_and_gradient(prob.z)
+        st, t0 = [], time.perf_counter()
+        for _ in range(args.steps):
+            e, g = model.energy_and_gradient(prob.z)
+            st.append(model.ctx.stage_times())
+        dt = time.perf_counter() - t0
+        sm = np.mean(np.array(st), axis=0)
+        res[f] = dict(E=e, G=g, ms=dt / args.steps * 1e3, value=G * args.steps / dt, gemm_ms=sm[2], dev_ms=sm[0])
+        log(f"[bench] n=128 {f}: step {res[f]['ms']:.2f} ms  ERI GEMM {sm[2]:.2f} ms  E={e:.12f}")
+    base = res["fp64"]
+    flops = 4.0 * nso ** 4 * G
+    out_forms = {}
+    for f, r in res.items():
+        nprod = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3}[f]
+        out_forms[f] = {"grid_pts_per_s": r["value"], "ms_per_step": r["ms"], "eri_gemm_ms": r["gemm_ms"],
+                        "device_ms": r["dev_ms"], "E": r["E"], "abs_dE_vs_fp64": abs(r["E"] - base["E"]),
+                        "max_abs_dG_vs_fp64": float(np.abs(r["G"] - base["G"]).max()),
+                        "speedup_vs_fp64": r["value"] / base["value"],
+                        "mfma_frac": nprod * flops / (r["gemm_ms"] * 1e-3) / PEAK[f]}
+        log(f"[bench] n=128 {f}: |dE| = {out_forms[f]['abs_dE_vs_fp64']:.3e}  speedup {out_forms[f]['speedup_vs_fp64']:.2f}x"
+            f"  MFMA {100 * out_forms[f]['mfma_frac']:.1f}%")
+    ok = [f for f in res if out_forms[f]["abs_dE_vs_fp64"] <= 1e-6]
+    best = max(ok, key=lambda f: res[f]["value"])
+    nprod = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3}[best]
+    t_gemm = res[best]["gemm_ms"] * 1e-3
+    out = {"metric": "projected E+grad evals/s (grid pts/s) vs plain FP64 GPU at 1e-6 Eh; % MFMA peak",
+           "value": res[best]["value"], "unit": "grid pts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": res[best]["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": _form_dtype(best), "data": "synthetic",
+           "config": {"workload": f"projected E+grad, synthetic n={n} (NSO={nso}), N={nocc}, L={3 * n}, grid 50x24={G} "
+                                  "(seed 1003), device-resident objective", "form": best, "forms": out_forms},
+           "roofline": {"bound": "mfma", "achieved": nprod * flops / t_gemm / 1e12, "peak": PEAK[best] / 1e12,
+                        "unit": "TFLOP/s", "frac": nprod * flops / t_gemm / PEAK[best], "traffic": None},
+           "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+
+
+def run_adjoint(args):
+    """n=96 (NSO=192), N=48, L=288, 50 x 16 grid: the differentiable form.
+    Autograd gradient against central differences (h = 1e-5, 32 random
+    directions) and against the drop-in LocalG gradient; then 50 L-BFGS
+    iterations from the seeded Z, timed, in the chosen precision."""
+    import torch
+    from phf_pcc_amd.hamiltonian import synthetic_problem
+    from phf_pcc_amd.model import ProjectedEnergy, ProjectedHF, lbfgs_optimize
+    from phf_pcc_amd.phf import eigen_solver, vec_decomp
+    from tests._cases import grid_args
+    n, nocc, ngrid = 96, 48, (50, 16)
+    t0 = time.time()
+    prob = synthetic_problem(n, nocc, seed=1005, ngrid=ngrid)
+    ga = grid_args(n, ngrid, 0)
+    model = ProjectedHF(nocc, prob.ref, ga, factors=(prob.h_spatial, prob.b))
+    G = ngrid[0] * ngrid[1]
+    log(f"[bench] n={n} problem and device Hamiltonian in {time.time()-t0:.1f}s")
+    # gradient checks in FP64
+    model.precision = "fp64"
+    zr = torch.tensor(prob.z.real.copy(), requires_grad=True)
+    zi = torch.tensor(prob.z.imag.copy(), requires_grad=True)
+    e = ProjectedEnergy.apply(zr, zi, model)
+    e.backward()
+    grad = np.concatenate([zr.grad.numpy().ravel(), zi.grad.numpy().ravel()])
+    x0 = np.concatenate([prob.z.real.ravel(), prob.z.imag.ravel()])
+    nv = prob.z.size
+    f = lambda x: model.energy_and_gradient((x[:nv] + 1j * x[nv:]).reshape(prob.z.shape))[0]
+    rng = np.random.default_rng(1005)
+    h = 1e-5
+    fd_err = []
+    t0 = time.time()
+    for _ in range(32):
+        d = rng.normal(size=x0.shape)
+        d /= np.linalg.norm(d)
+        fd = (f(x0 + h * d) - f(x0 - h * d)) / (2 * h)
+        fd_err.append(abs(fd - grad @ d) / (1 + abs(fd)))
+    t_fd = time.time() - t0
+    h_, s_, gm, rm = model.buildhsg(prob.z)
+    ev, vec = eigen_solver(h_, s_)
+    fsp, fpg, fk = vec_decomp(vec[:, 0], model.ncisp, model.ncipg, 1)
+    g_loc = model.localg(gm, rm, s_, ev[0], fsp, fpg, fk)[nocc:, :nocc]
+    g_ana = np.concatenate([2 * g_loc.real.ravel(), 2 * g_loc.imag.ravel()])
+    ana_err = float(np.abs(g_ana - grad).max())
+    log(f"[bench] adjoint: max rel FD error over 32 directions {max(fd_err):.2e} ({t_fd:.1f}s); "
+        f"max |autograd - LocalG| {ana_err:.2e}")
+    res = {}
+    forms = [f_.strip() for f_ in args.forms.split(",") if f_.strip()]
+    for prec in forms:
+        model.precision = prec
+        model.nevals = 0
+        t0 = time.perf_counter()
+        z, energies = lbfgs_optimize(model, prob.z, steps=50)
+        dt = time.perf_counter() - t0
+        model.precision = "fp64"
+        e_final64 = model.energy_and_gradient(z)[0]
+        res[prec] = {"lbfgs_iterations": 50, "evaluations": len(energies), "seconds": dt,
+                     "evals_per_s": len(energies) / dt, "grid_pts_per_s": len(energies) * G / dt,
+                     "E_start": energies[0], "E_final": energies[-1], "E_final_fp64_check": e_final64}
+        log(f"[bench] L-BFGS {prec}: {len(energies)} evaluations in {dt:.2f}s ({res[prec]['evals_per_s']:.1f} evals/s), "
+            f"E {energies[0]:.8f} -> {energies[-1]:.8f} (FP64 re-evaluation {e_final64:.8f})")
+    best = max(res, key=lambda k: res[k]["grid_pts_per_s"])
+    out = {"metric": "projected E+grad evals/s (grid pts/s) vs plain FP64 GPU at 1e-6 Eh; % MFMA peak",
+           "value": res[best]["grid_pts_per_s"], "unit": "grid pts/s", "n_gpus": 1, "steps": res[best]["evaluations"],
+           "warmup": 0, "ms_per_step": res[best]["seconds"] / res[best]["evaluations"] * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": _form_dtype(best),
+           "data": "synthetic",
+           "config": {"workload": f"differentiable form: E + adjoint dE/dZ, synthetic n={n} (NSO={2 * n}), N={nocc}, "
+                                  f"L={3 * n}, grid 50x16={G} (seed 1005), 50 L-BFGS iterations (torch, strong Wolfe)",
+                      "form": best, "fd_check": {"directions": 32, "h": h, "max_rel_error": float(max(fd_err))},
+                      "max_abs_autograd_minus_localg": ana_err, "lbfgs": res},
+           "roofline": None, "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+
+
+def cpu_blas_gemm(prob, G):
+    """Strong CPU baseline for the dominant stage: Gamma for every grid point as
+    one BLAS dgemm A[(i,k),(j,l)] P (numpy's BLAS on all host cores; real A,
+    re/im density columns).  Times the GEMM only (>= 95% of the per-point work
+    at NSO = 128), so it bounds the CPU throughput from above."""
+    nso = prob.nso
+    n2 = nso * nso
+    a = np.ascontiguousarray(prob.htwo.real.transpose(0, 2, 1, 3).reshape(n2, n2))
+    rng = np.random.default_rng(0)
+    p = rng.normal(size=(n2, 2 * G))
+    a @ p[:, :16]
+    t0 = time.perf_counter()
+    a @ p
+    t1 = time.perf_counter() - t0
+    return {"value": G / t1, "unit": "grid pts/s", "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+            "kind": "blas", "sample": f"numpy BLAS dgemm {n2} x {n2} x {2 * G} (all {G} grid points) in {t1:.2f} s; "
+                                      "ERI GEMM only"}
+
+
 def load_traffic(kernel, n):
     """HBM bytes per launch of `kernel` at this workload from the committed
     rocprofv3 counter summary (profiles/traffic.json), if present."""
@@ -199,8 +357,9 @@ def main():
     ap.add_argument("--dropin", action="store_true",
                     help="also time the PHFTools_F drop-in (host arrays in, Hmat/Smat/Gradmat/Rdmmat out per call)")
     ap.add_argument("--no-autotune", action="store_true", help="use the tile cost model instead of timing candidates")
-    ap.add_argument("--workload", default="single", choices=["single", "batch64"],
-                    help="single: one n=64 instance (default); batch64: 64 uneven instances in one batch")
+    ap.add_argument("--workload", default="single", choices=["single", "batch64", "large", "adjoint"],
+                    help="single: one n=64 instance (default); batch64: 64 uneven instances in one batch; "
+                         "large: n=128, 50x24 grid, every precision; adjoint: n=96 gradient checks + 50 L-BFGS")
     ap.add_argument("--ninst", type=int, default=64, help="instances of the batch64 workload")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (HBM counter calibration)")
@@ -222,6 +381,10 @@ def main():
 
     if args.workload == "batch64":
         return run_batch(args)
+    if args.workload == "large":
+        return run_large(args)
+    if args.workload == "adjoint":
+        return run_adjoint(args)
     from phf_pcc_amd.PHFTools_F import phftools
     from phf_pcc_amd.model import ProjectedHF
     from phf_pcc_amd.phf import eigen_solver, vec_decomp
@@ -244,16 +407,18 @@ def main():
         phftools.precision = prec
         for i in range(args.warmup):
             e, _ = stepfn()
-        gemm_ms, dev_ms = [], []
+        gemm_ms, dev_ms, stages = [], [], []
         t0 = time.perf_counter()
         for i in range(args.steps):
             e, g = stepfn()
             st = stepfn.ctx.stage_times() if hasattr(stepfn, "ctx") else ctx.stage_times()
             dev_ms.append(st[0])
             gemm_ms.append(st[2])
+            stages.append(st)
         dt = time.perf_counter() - t0
         r = {"prec": prec, "E": e, "G": g, "ms_step": dt / args.steps * 1e3,
-             "value": G * args.steps / dt, "gemm_ms": float(np.mean(gemm_ms)), "dev_ms": float(np.mean(dev_ms))}
+             "value": G * args.steps / dt, "gemm_ms": float(np.mean(gemm_ms)), "dev_ms": float(np.mean(dev_ms)),
+             "stages": np.mean(np.array(stages), axis=0)}
         log(f"[bench] {tag}{prec}: {args.steps} steps {dt:.3f}s  E={e:.12f}  step {r['ms_step']:.3f} ms  device "
             f"{r['dev_ms']:.3f} ms  ERI GEMM {r['gemm_ms']:.3f} ms")
         return r
@@ -305,10 +470,28 @@ def main():
     value = best["value"]
     ms_step = best["ms_step"]
 
+    # HBM-bound stages of the plain FP64 form: algorithmic bytes per grid point
+    # rotation 32 NSO N (read C, write T); density 16 NSO N + 16 N^2 + 2 s NSO^2
+    # (read T, M; write the re/im rho planes, s = 8)
+    st64 = base["stages"]
+    b_rot = 32.0 * nso * nocc * G
+    b_den = (16.0 * nso * nocc + 16.0 * nocc ** 2 + 2 * 8 * nso ** 2) * G
+    t_rot, t_den = st64[4] * 1e-3, (st64[5] + st64[6]) * 1e-3
+    stage_out = {"rotation": {"ms": st64[4], "algorithmic_bytes": b_rot, "GBps": b_rot / t_rot / 1e9,
+                              "frac_hbm": b_rot / t_rot / 8e12},
+                 "density": {"ms": st64[5] + st64[6], "algorithmic_bytes": b_den, "GBps": b_den / t_den / 1e9,
+                             "frac_hbm": b_den / t_den / 8e12, "pack_ms": st64[6], "overlap_lu_solve_ms": st64[5]},
+                 "epilogue_ms": st64[3], "split_ms_f16x2": (res["f16x2"]["stages"][7] if "f16x2" in res else None)}
+    log(f"[bench] stages (fp64): rotation {st64[4]:.3f} ms = {stage_out['rotation']['GBps']:.0f} GB/s, density "
+        f"{st64[5] + st64[6]:.3f} ms = {stage_out['density']['GBps']:.0f} GB/s (algorithmic bytes)")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(prob, ga, ref, nocc, G, args.cpu_seconds)
         log(f"[bench] cpu baseline {cpu['value']:.3f} grid pts/s on {cpu['cores']} threads ({cpu['sample']})")
+        cpu["blas_gemm_variant"] = cpu_blas_gemm(prob, G)
+        log(f"[bench] cpu BLAS ERI GEMM variant {cpu['blas_gemm_variant']['value']:.1f} grid pts/s "
+            f"({cpu['blas_gemm_variant']['sample']})")
 
     traffic = load_traffic(roof["kernel"], args.n)
     if traffic is not None:
@@ -331,6 +514,7 @@ def main():
                       "plain_fp64": forms_out["fp64"], "forms": forms_out,
                       "path": "device-resident objective (phf_eval_resident + phf_localg_vo), factored Hamiltonian "
                               "built on the device",
+                      "hbm_stages_fp64": stage_out,
                       "drop_in_phftools": dropin or None},
            "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)

@@ -138,8 +138,9 @@ int phf_localg(phf_ctx* ctx, const double* gradmat, const double* rdmmat, const 
  * reference, which only prints wall-clock times, Main_PHF.py:15,40,44).
  * After a profiled phf_buildhsg, phf_stage_times fills up to nmax values in
  * milliseconds: [0] all device work of the call, [1] rotation + overlap +
- * inverse + density pack, [2] ERI GEMM, [3] epilogue + grid sums; it returns
- * how many it wrote. */
+ * inverse + density pack + reduced-precision split, [2] ERI GEMM, [3]
+ * epilogue + grid sums, [4] rotation chain, [5] overlap + LU inverse + X_g,
+ * [6] density pack, [7] reduced-precision split; it returns how many it wrote. */
 int phf_set_profiling(phf_ctx* ctx, int on);
 int phf_stage_times(const phf_ctx* ctx, double* ms, int nmax);
 

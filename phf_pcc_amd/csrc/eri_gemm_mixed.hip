@@ -74,8 +74,8 @@ __device__ __forceinline__ void h16_tile(const T* __restrict__ P, const T* __res
                                          const double* __restrict__ invP, const double* __restrict__ invA,
                                          double* __restrict__ C, const double* __restrict__ bias, long ldc, int K,
                                          int nslice, unsigned patP, unsigned patA, int mt, int nt, int sbeg,
-                                         int send, bool atomic, unsigned char* smem) {
-  constexpr int BK = 64, FOLD = 8;
+                                         int send, bool atomic, int fold, unsigned char* smem) {
+  constexpr int BK = 64;
   constexpr int STAGES = Cfg::STAGES, NW = Cfg::NW, ROWS = Cfg::ROWS, STAGE_BYTES = Cfg::STAGE_BYTES;
   constexpr int FM = WM / 16, FN = WN / 16, WGN = BN / WN;
   constexpr int BLOCKS = ROWS / 8;            // 1 KB blocks per slab
@@ -148,7 +148,7 @@ __device__ __forceinline__ void h16_tile(const T* __restrict__ P, const T* __res
         for (int i = 0; i < FM; ++i) acc[i][j] = MfmaH<T>::mma(a[i], b, acc[i][j]);
       }
     }
-    if ((kt + 1) % FOLD == 0 || kt + 1 == nk) {
+    if ((kt + 1) % fold == 0 || kt + 1 == nk) {
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -182,7 +182,8 @@ template <typename T, typename Cfg, int BM, int BN, int WM, int WN, int SPLITK>
 __global__ __launch_bounds__((Cfg::NT), (Cfg::MINB))
 void eri_gemm_h16_kernel(const T* __restrict__ P, const T* __restrict__ A, const double* __restrict__ invP,
                          const double* __restrict__ invA, double* __restrict__ C, int K, int nslice, int nseg,
-                         unsigned patP, unsigned patA, long ldc, int tiles_m, const double* __restrict__ bias) {
+                         unsigned patP, unsigned patA, long ldc, int tiles_m, const double* __restrict__ bias,
+                         int fold) {
   extern __shared__ __attribute__((aligned(1024))) unsigned char smem[];
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   const int ks_id = t % SPLITK;
@@ -190,7 +191,7 @@ void eri_gemm_h16_kernel(const T* __restrict__ P, const T* __restrict__ A, const
   const int mt = tile % tiles_m, nt = tile / tiles_m;
   const int part = nseg * (K / 64) / SPLITK;
   h16_tile<T, Cfg, BM, BN, WM, WN>(P, A, invP, invA, C, ks_id == 0 ? bias : nullptr, ldc, K, nslice, patP, patA, mt,
-                                   nt, ks_id * part, (ks_id + 1) * part, SPLITK > 1, smem);
+                                   nt, ks_id * part, (ks_id + 1) * part, SPLITK > 1, fold, smem);
 }
 
 // Many GEMMs of one precision in one persistent launch: workgroup b takes
@@ -199,13 +200,13 @@ void eri_gemm_h16_kernel(const T* __restrict__ P, const T* __restrict__ A, const
 template <typename T, typename Cfg, int BM, int BN, int WM, int WN, bool ATOMIC>
 __global__ __launch_bounds__((Cfg::NT), (Cfg::MINB))
 void eri_gemm_h16_grouped_kernel(const H16Item* __restrict__ items, int nitems, int nslice, unsigned patP,
-                                 unsigned patA) {
+                                 unsigned patA, int fold) {
   extern __shared__ __attribute__((aligned(1024))) unsigned char smem[];
   for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
     const H16Item it = items[w];
     __syncthreads();   // every wave is done with the previous tile's LDS ring
     h16_tile<T, Cfg, BM, BN, WM, WN>((const T*)it.P, (const T*)it.A, it.invP, it.invA, it.C, it.bias, it.ldc, it.K,
-                                     nslice, patP, patA, it.mt, it.nt, it.sbeg, it.send, ATOMIC, smem);
+                                     nslice, patP, patA, it.mt, it.nt, it.sbeg, it.send, ATOMIC, fold, smem);
   }
 }
 
@@ -388,6 +389,12 @@ void convert_f32(hipStream_t s, const double* src, float* dst, long n) {
   to_f32_kernel<<<blocks, 256, 0, s>>>(src, dst, n);
 }
 
+// Slabs (64 contraction elements) per FP32 chunk before the fold into FP64.
+// Folding more often does not lower the error of these forms: the 16-bit
+// MFMA sums its products with truncation inside the instruction, which no
+// fold interval removes (measured at 8, 4, 2 and 1 slabs on n = 128).
+static int h16_fold() { return 8; }
+
 // Tile configurations of the 16-bit kernel: {BM, BN, WM, WN}.
 static const int kH16Cfgs[][4] = {{128, 128, 64, 64}, {256, 128, 64, 64}, {256, 128, 128, 64}, {128, 256, 64, 128}};
 int h16_num_configs() { return 4; }
@@ -433,7 +440,7 @@ void eri_gemm_mixed(hipStream_t s, int prec, const void* P, const void* A, const
       attr_set = true;                                                                                       \
     }                                                                                                        \
     kern<<<grid, CC::NT, CC::LDS, s>>>((const TT*)P, (const TT*)A, invP, invA, C, K, nslice, nseg, patP, patA, \
-                                       ldc, tm, bias);                                                       \
+                                       ldc, tm, bias, h16_fold());                                           \
   } while (0)
 #define H_CFGS(TT, SK)                                                   \
   do {                                                                   \
@@ -477,9 +484,9 @@ void eri_gemm_mixed_grouped(hipStream_t s, int prec, const H16Item* items, int n
       attr_set = true;                                                                                       \
     }                                                                                                        \
     if (atomic)                                                                                              \
-      k_at<<<grid, CC::NT, CC::LDS, s>>>(items, nitems, nslice, patP, patA);                                 \
+      k_at<<<grid, CC::NT, CC::LDS, s>>>(items, nitems, nslice, patP, patA, h16_fold());                     \
     else                                                                                                     \
-      k_st<<<grid, CC::NT, CC::LDS, s>>>(items, nitems, nslice, patP, patA);                                 \
+      k_st<<<grid, CC::NT, CC::LDS, s>>>(items, nitems, nslice, patP, patA, h16_fold());                     \
   } while (0)
 #define G_CFGS(TT)                                                      \
   do {                                                                  \

@@ -932,7 +932,7 @@ int eval_plan(phf_ctx* ctx, int nocc, int precision, EvalBufs& b) {
   return PHF_OK;
 }
 
-void eval_pre(phf_ctx* ctx, const EvalBufs& b) {
+void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
   hipStream_t s = ctx->stream;
   const int nso = ctx->nso, N = b.N, npg = ctx->g_npg, ny = ctx->g_ny, nl = ctx->g_nl;
   const long G = b.G, n2 = (long)nso * nso, lenX = (long)nso * N;
@@ -949,6 +949,7 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b) {
   // T[il] = (Ref^H R1[il]) W2   -> T_g = columns g*N .. g*N+N-1
   zgemm_batched(s, nso, N * npg * ny, nso, 1, 0, zmat_colmajor(ctx->dGr1p, nso, 2 * n2), zmat_colmajor(b.W2, nso, 0),
                 0, 0, zout_colmajor(b.T, nso, sT * npg * ny), nl);
+  if (marks) (void)hipEventRecord(marks[0], s);
   // M_g = C0^H T_g
   zgemm_batched(s, N, N, nso, 1, 0, ZMat{b.C0, 2L * nso, 2, 0, 1, 1}, zmat_colmajor(b.T, nso, sT), 0, 0,
                 zout_colmajor(b.M, N, sNN), (int)G);
@@ -957,6 +958,7 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b) {
   // X_g = T_g M_g^{-1}
   zgemm_batched(s, nso, N, N, 1, 0, zmat_colmajor(b.T, nso, sT), zmat_colmajor(b.Minv, N, sNN), 0, 0,
                 zout_colmajor(b.X, nso, sT), (int)G);
+  if (marks) (void)hipEventRecord(marks[1], s);
   // density pack: Pt row 2g = Re rho_g^T, row 2g+1 = Im rho_g^T (flattened j*nso + l)
   fill_zero(s, b.Pt, b.Mp * kt);
   zgemm_batched(s, nso, nso, N, 1, 0, ZMat{b.C0, 2, 2L * nso, 0, 1, 1}, ZMat{b.X, 2L * nso, 2, sT, 1, 0}, 0, 0,
@@ -965,6 +967,7 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b) {
     dim3 grid((unsigned)std::min<long>((n2 + 255) / 256, 1024), (unsigned)G);
     pack_cplx_kernel<<<grid, 256, 0, s>>>(b.Pt, (int)G, (int)n2, ctx->kpad, kt);
   }
+  if (marks) (void)hipEventRecord(marks[2], s);
   // operand of the reduced-precision ERI GEMM
   if (b.prec == PHF_PREC_FP32)
     convert_f32(s, b.Pt, (float*)b.Pm, b.Mp * kt);
@@ -1063,7 +1066,7 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
   build_c0(z, nso, N, cmplxconj, c0.data());
   HIPCK(hipMemcpyAsync(b.C0, c0.data(), c0.size() * sizeof(double), hipMemcpyHostToDevice, s));
   if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[0], s));
-  eval_pre(ctx, b);
+  eval_pre(ctx, b, ctx->profiling ? ctx->ev + 4 : nullptr);
   if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[1], s));
   {
     int rc = eval_gemm(ctx, b);
@@ -1093,7 +1096,16 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
     ctx->stage_ms[1] = t01;   // rotation, overlap, inverse, density pack
     ctx->stage_ms[2] = t12;   // ERI GEMM
     ctx->stage_ms[3] = t23;   // epilogue and grid sums
-    ctx->nstage = 4;
+    float t04, t45, t56, t61;
+    HIPCK(hipEventElapsedTime(&t04, ctx->ev[0], ctx->ev[4]));
+    HIPCK(hipEventElapsedTime(&t45, ctx->ev[4], ctx->ev[5]));
+    HIPCK(hipEventElapsedTime(&t56, ctx->ev[5], ctx->ev[6]));
+    HIPCK(hipEventElapsedTime(&t61, ctx->ev[6], ctx->ev[1]));
+    ctx->stage_ms[4] = t04;   // rotation chain T_g
+    ctx->stage_ms[5] = t45;   // overlap, LU inverse + det, X_g
+    ctx->stage_ms[6] = t56;   // density pack rho_g^T (FP64 planes)
+    ctx->stage_ms[7] = t61;   // reduced-precision split / conversion of the densities
+    ctx->nstage = 8;
   }
   for (long g = 0; g < G; ++g)
     if (info[g] != 0) {

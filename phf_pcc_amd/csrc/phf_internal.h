@@ -127,9 +127,11 @@ struct EvalBufs {
 // eval_pre runs K1/K2, the density pack and the reduced-precision split of
 // the density rows once C0 = [I; Z] is in b.C0; eval_gemm the single ERI GEMM
 // (autotuning a new 16-bit shape first, not for capture); eval_post K4/K5 and,
-// unless resident, Gradmat/Rdmmat.  eval_pre/eval_post only launch kernels.
+// unless resident, Gradmat/Rdmmat.  eval_pre/eval_post only launch kernels
+// (eval_pre records marks[0..2] after the rotation chain, the X_g solve and the
+// density pack when marks is given).
 int eval_plan(phf_ctx* ctx, int nocc, int precision, EvalBufs& b);
-void eval_pre(phf_ctx* ctx, const EvalBufs& b);
+void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks = nullptr);
 int eval_gemm(phf_ctx* ctx, const EvalBufs& b);
 void eval_post(phf_ctx* ctx, const EvalBufs& b, bool resident);
 // C0 = [I; Z] column-major complex (NSO x N) from Z ((NSO-N) x N, C order)

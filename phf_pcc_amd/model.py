@@ -166,7 +166,7 @@ def lbfgs_optimize(model: ProjectedHF, z0: np.ndarray, steps: int = 50, lr: floa
         opt.zero_grad()
         e = ProjectedEnergy.apply(zr, zi, model)
         e.backward()
-        energies.append(float(e))
+        energies.append(float(e.detach()))
         if callback:
             callback(len(energies), float(e))
         return e
