@@ -119,7 +119,8 @@ int phf_buildhsg_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int p
  * nci x nci problem (EigenSolver, VecDecomp), phf_localg_vo returns the
  * (NSO-N) x N block G[N:, :N] of LocalG (gvo: complex [nso-nocc][nocc]),
  * which is the gradient dE/dZ*.  Valid until the next call that uses the
- * context.  CmplxConj = 2 is not supported here. */
+ * context.  phf_localg_vo does not take CmplxConj = 2 (use phf_buildhsg_z +
+ * phf_localg there). */
 int phf_eval_resident(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int precision, double* hmat,
                       double* smat);
 int phf_localg_vo(phf_ctx* ctx, const double* smat, const double* e0, const double* fsp, int ncisp,

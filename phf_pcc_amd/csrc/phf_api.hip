@@ -92,11 +92,26 @@ __global__ void pack_cplx_kernel(double* Pt, int G, int n2, int kpad, int kt) {
   }
 }
 
-__global__ void wd_kernel(int G, int nmn, const double* W, const double* det, double* Wd) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= G * nmn) return;
-  const int g = i / nmn;
-  const double wr = W[2 * i], wi = W[2 * i + 1], dr = det[2 * g], di = det[2 * g + 1];
+// Wd[v G + g][M][N] = W[g][M - d bm_v][N - d bn_v] det[v G + g] (zero outside
+// the shifted block): variant v of the V density variants of a grid point
+// lands in CI block (bm_v, bn_v) of the nci x nci matrices (V = 1: d = nci,
+// block (0, 0)).  W holds the d x d weights at the top left of nci x nci.
+__global__ void wd_kernel(int G, int V, int nci, int d, const double* W, const double* det, double* Wd) {
+  const int nmn = nci * nci;
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= (long)V * G * nmn) return;
+  const long gv = i / nmn;
+  const int mn = (int)(i % nmn), M = mn / nci, N = mn % nci;
+  const int v = (int)(gv / G), g = (int)(gv % G);
+  // variants: 0 (L C0, B C0) -> (0,0); 1 (L C0*, B C0) -> (0,1); 2 (C0*, C0*) -> (1,1); 3 (C0, C0*) -> (1,0)
+  const int bm = (v == 2 || v == 3) ? 1 : 0, bn = (v == 1 || v == 2) ? 1 : 0;
+  const int m = M - d * bm, n = N - d * bn;
+  double wr = 0.0, wi = 0.0;
+  if (m >= 0 && m < d && n >= 0 && n < d) {
+    wr = W[2 * ((long)g * nmn + m * nci + n)];
+    wi = W[2 * ((long)g * nmn + m * nci + n) + 1];
+  }
+  const double dr = det[2 * gv], di = det[2 * gv + 1];
   Wd[2 * i] = wr * dr - wi * di;
   Wd[2 * i + 1] = wr * di + wi * dr;
 }
@@ -859,7 +874,7 @@ void build_c0(const double* z, int nso, int N, int cmplxconj, double* c0) {
     }
 }
 
-int eval_plan(phf_ctx* ctx, int nocc, int precision, EvalBufs& b) {
+int eval_plan(phf_ctx* ctx, int nocc, int precision, int cmplxconj, EvalBufs& b) {
   if (!ctx->has_ham) return fail(ctx, PHF_ERR_NO_HAMILTONIAN, "phf_buildhsg: no Hamiltonian set");
   if (!ctx->has_grid || ctx->g_nso != ctx->nso)
     return fail(ctx, PHF_ERR_ARG, "phf_buildhsg_z: no spin grid set for this Hamiltonian (phf_set_grid)");
@@ -872,39 +887,45 @@ int eval_plan(phf_ctx* ctx, int nocc, int precision, EvalBufs& b) {
   if (precision == PHF_PREC_FP64 && !ctx->dA)
     return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: the Hamiltonian is stored for a reduced precision only");
   const int npg = ctx->g_npg, ny = ctx->g_ny, nci = ctx->g_nci;
+  if (cmplxconj == 2 && (nci % 2 != 0 || 2 * ctx->g_ncisp * ctx->g_ncipg > nci))
+    return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: CmplxConj=2 needs nci = 2 ncisp ncipg");
   const int N = nocc;
+  const int V = (cmplxconj == 2) ? 4 : 1;   // density variants per grid point
   const long G = ctx->g_G;
+  const long Gv = V * G;
   const int nmn = nci * nci;
   const long n2 = (long)nso * nso;
   const int kt = ctx->kt, nrpad = ctx->nrpad;
   b.N = N;
   b.prec = precision;
   b.G = G;
+  b.V = V;
   // rows are padded for the tallest tile any candidate uses
-  b.bm = (precision == PHF_PREC_FP64) ? eri_gemm_f64_bm((int)(2 * G)) : 256;
-  b.Mp = round_up(2 * G, b.bm);
+  b.bm = (precision == PHF_PREC_FP64) ? eri_gemm_f64_bm((int)(2 * Gv)) : 256;
+  b.Mp = round_up(2 * Gv, b.bm);
   b.gchunk = 32;
-  b.nchunk = (int)((G + b.gchunk - 1) / b.gchunk);
+  b.nchunk = (int)((Gv + b.gchunk - 1) / b.gchunk);
   const long Mp = b.Mp;
   const long lenX = (long)nso * N;
   auto carve = [&](Arena& a, EvalBufs& o) {
     o.C0 = a.take<double>(2 * lenX);
-    o.Wpg = a.take<double>(2 * lenX * npg);
-    o.W2 = a.take<double>(2 * lenX * npg * ny);
-    o.T = a.take<double>(2 * lenX * G);
-    o.M = a.take<double>(2 * (size_t)N * N * G);
-    o.Minv = a.take<double>(2 * (size_t)N * N * G);
-    o.Det = a.take<double>(2 * (size_t)G);
-    o.Info = reinterpret_cast<int*>(a.take<double>((size_t)G));
-    o.X = a.take<double>(2 * lenX * G);
+    o.Wpg = a.take<double>(2 * lenX * npg * ((V == 4) ? 2 : 1));
+    const int nl_ = (V == 4) ? 2 : 1;          // distinct left factors C0, C0*
+    o.W2 = a.take<double>(2 * lenX * npg * ny * nl_);
+    o.T = a.take<double>(2 * lenX * G * nl_);
+    o.M = a.take<double>(2 * (size_t)N * N * Gv);
+    o.Minv = a.take<double>(2 * (size_t)N * N * Gv);
+    o.Det = a.take<double>(2 * (size_t)Gv);
+    o.Info = reinterpret_cast<int*>(a.take<double>((size_t)Gv));
+    o.X = a.take<double>(2 * lenX * Gv);
     o.Pt = a.take<double>((size_t)Mp * kt);
     o.Gam = a.take<double>((size_t)Mp * nrpad);
-    o.FX = a.take<double>(2 * lenX * G);
-    o.Q = a.take<double>(2 * (size_t)N * N * G);
-    o.Hc = a.take<double>(2 * lenX);
-    o.E = a.take<double>(2 * (size_t)G);
-    o.Y = a.take<double>(2 * lenX * G);
-    o.Wd = a.take<double>(2 * (size_t)G * nmn);
+    o.FX = a.take<double>(2 * lenX * Gv);
+    o.Q = a.take<double>(2 * (size_t)N * N * Gv);
+    o.Hc = a.take<double>(2 * lenX * (V == 4 ? 2 : 1));
+    o.E = a.take<double>(2 * (size_t)Gv);
+    o.Y = a.take<double>(2 * lenX * Gv);
+    o.Wd = a.take<double>(2 * (size_t)Gv * nmn);
     o.Part = a.take<double>(2 * (size_t)b.nchunk * nmn * lenX);
     o.Ysum = a.take<double>(2 * (size_t)nmn * lenX);
     o.Xsum = a.take<double>(2 * (size_t)nmn * lenX);
@@ -932,40 +953,64 @@ int eval_plan(phf_ctx* ctx, int nocc, int precision, EvalBufs& b) {
   return PHF_OK;
 }
 
+// Density variants (V = 4 for CmplxConj = 2, PHFTools.f90:95-119): variant v
+// uses the left factor L_v (T = R L_v) and the right factor B_v (M = B_v^H T,
+// rho = T M^{-1} B_v^H):  v = 0: (C0, C0) RDM;  1: (C0*, C0) RDMConj;
+// 2: (C0*, C0*) RDM of the conjugate;  3: (C0, C0*) RDMConj of the conjugate.
+// Virtual point gv = v G + g; V = 1 is variant 0 alone.
+static inline int variant_left(int v) { return (v == 1 || v == 2) ? 1 : 0; }
+static inline int variant_right(int v) { return (v == 2 || v == 3) ? 1 : 0; }
+
 void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
   hipStream_t s = ctx->stream;
-  const int nso = ctx->nso, N = b.N, npg = ctx->g_npg, ny = ctx->g_ny, nl = ctx->g_nl;
+  const int nso = ctx->nso, N = b.N, npg = ctx->g_npg, ny = ctx->g_ny, nl = ctx->g_nl, V = b.V;
   const long G = b.G, n2 = (long)nso * nso, lenX = (long)nso * N;
   const int kt = ctx->kt;
   const long sT = 2 * lenX;          // batch stride of NSO x N column-major blocks
   const long sNN = 2L * N * N;
-  // K1: rotation chain, right to left (Ref^H R1 and Rpg Ref are cached with the grid)
-  // Wpg[ipg] = (Rpg[ipg] Ref) C0
-  zgemm_batched(s, nso, N, nso, 1, 0, zmat_colmajor(ctx->dGrpgref, nso, 2 * n2), zmat_colmajor(b.C0, nso, 0), 0, 0,
-                zout_colmajor(b.Wpg, nso, sT), npg);
-  // W2[iy] = R2[iy] [Wpg_0 .. Wpg_{npg-1}]
-  zgemm_batched(s, nso, N * npg, nso, 1, 0, ZMat{ctx->dGr2, 2L * nso, 2, 2 * n2, 1, 0},
-                zmat_colmajor(b.Wpg, nso, 0), 0, 0, zout_colmajor(b.W2, nso, sT * npg), ny);
-  // T[il] = (Ref^H R1[il]) W2   -> T_g = columns g*N .. g*N+N-1
-  zgemm_batched(s, nso, N * npg * ny, nso, 1, 0, zmat_colmajor(ctx->dGr1p, nso, 2 * n2), zmat_colmajor(b.W2, nso, 0),
-                0, 0, zout_colmajor(b.T, nso, sT * npg * ny), nl);
+  const int nL = (V == 4) ? 2 : 1;
+  // K1: rotation chain, right to left (Ref^H R1 and Rpg Ref are cached with the
+  // grid), once per left factor L = C0 (lc = 0) or C0* (lc = 1)
+  for (int lc = 0; lc < nL; ++lc) {
+    double* Wpg = b.Wpg + lc * sT * npg;
+    double* W2 = b.W2 + lc * sT * npg * ny;
+    double* T = b.T + lc * sT * G;
+    // Wpg[ipg] = (Rpg[ipg] Ref) L
+    zgemm_batched(s, nso, N, nso, 1, 0, zmat_colmajor(ctx->dGrpgref, nso, 2 * n2), zmat_colmajor(b.C0, nso, 0, lc),
+                  0, 0, zout_colmajor(Wpg, nso, sT), npg);
+    // W2[iy] = R2[iy] [Wpg_0 .. Wpg_{npg-1}]
+    zgemm_batched(s, nso, N * npg, nso, 1, 0, ZMat{ctx->dGr2, 2L * nso, 2, 2 * n2, 1, 0}, zmat_colmajor(Wpg, nso, 0), 0,
+                  0, zout_colmajor(W2, nso, sT * npg), ny);
+    // T[il] = (Ref^H R1[il]) W2   -> T_g = columns g*N .. g*N+N-1
+    zgemm_batched(s, nso, N * npg * ny, nso, 1, 0, zmat_colmajor(ctx->dGr1p, nso, 2 * n2), zmat_colmajor(W2, nso, 0),
+                  0, 0, zout_colmajor(T, nso, sT * npg * ny), nl);
+  }
   if (marks) (void)hipEventRecord(marks[0], s);
-  // M_g = C0^H T_g
-  zgemm_batched(s, N, N, nso, 1, 0, ZMat{b.C0, 2L * nso, 2, 0, 1, 1}, zmat_colmajor(b.T, nso, sT), 0, 0,
-                zout_colmajor(b.M, N, sNN), (int)G);
-  // K2: inverse and determinant
-  lu_inverse_batched(s, N, b.M, sNN, b.Minv, sNN, b.Det, b.Info, (int)G);
-  // X_g = T_g M_g^{-1}
-  zgemm_batched(s, nso, N, N, 1, 0, zmat_colmajor(b.T, nso, sT), zmat_colmajor(b.Minv, N, sNN), 0, 0,
-                zout_colmajor(b.X, nso, sT), (int)G);
+  // M_v = B_v^H T_{L_v}   (B = C0: C0^H; B = C0*: C0^T)
+  for (int v = 0; v < V; ++v)
+    zgemm_batched(s, N, N, nso, 1, 0, ZMat{b.C0, 2L * nso, 2, 0, 1, variant_right(v) ? 0 : 1},
+                  zmat_colmajor(b.T + variant_left(v) * sT * G, nso, sT), 0, 0,
+                  zout_colmajor(b.M + v * sNN * G, N, sNN), (int)G);
+  // K2: inverse and determinant of every variant
+  lu_inverse_batched(s, N, b.M, sNN, b.Minv, sNN, b.Det, b.Info, (int)(V * G));
+  // X_v = T_{L_v} M_v^{-1}
+  for (int v = 0; v < V; ++v)
+    zgemm_batched(s, nso, N, N, 1, 0, zmat_colmajor(b.T + variant_left(v) * sT * G, nso, sT),
+                  zmat_colmajor(b.Minv + v * sNN * G, N, sNN), 0, 0, zout_colmajor(b.X + v * sT * G, nso, sT), (int)G);
   if (marks) (void)hipEventRecord(marks[1], s);
-  // density pack: Pt row 2g = Re rho_g^T, row 2g+1 = Im rho_g^T (flattened j*nso + l)
+  // density pack: Pt row 2gv = Re rho_gv^T, row 2gv+1 = Im rho_gv^T (flattened
+  // j*nso + l); rho^T = conj(B) X^T, one launch per right factor (variants
+  // with B = C0 come first, then B = C0*)
   fill_zero(s, b.Pt, b.Mp * kt);
-  zgemm_batched(s, nso, nso, N, 1, 0, ZMat{b.C0, 2, 2L * nso, 0, 1, 1}, ZMat{b.X, 2L * nso, 2, sT, 1, 0}, 0, 0,
-                ZOut{b.Pt, nso, 1, 2L * kt, kt}, (int)G);
+  const int nB = (V == 4) ? 2 : 1;
+  const long per = (V == 4) ? 2 * G : G;
+  for (int bc = 0; bc < nB; ++bc)
+    zgemm_batched(s, nso, nso, N, 1, 0, ZMat{b.C0, 2, 2L * nso, 0, 1, bc ? 0 : 1},
+                  ZMat{b.X + bc * per * sT, 2L * nso, 2, sT, 1, 0}, 0, 0,
+                  ZOut{b.Pt + bc * per * 2L * kt, nso, 1, 2L * kt, kt}, (int)per);
   if (ctx->cplx) {
-    dim3 grid((unsigned)std::min<long>((n2 + 255) / 256, 1024), (unsigned)G);
-    pack_cplx_kernel<<<grid, 256, 0, s>>>(b.Pt, (int)G, (int)n2, ctx->kpad, kt);
+    dim3 grid((unsigned)std::min<long>((n2 + 255) / 256, 1024), (unsigned)(V * G));
+    pack_cplx_kernel<<<grid, 256, 0, s>>>(b.Pt, (int)(V * G), (int)n2, ctx->kpad, kt);
   }
   if (marks) (void)hipEventRecord(marks[2], s);
   // operand of the reduced-precision ERI GEMM
@@ -978,7 +1023,7 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
 int eval_gemm(phf_ctx* ctx, const EvalBufs& b) {
   hipStream_t s = ctx->stream;
   const int kt = ctx->kt, nrpad = ctx->nrpad, precision = b.prec;
-  const long G = b.G;
+  const long G = b.V * b.G;                  // rows of the GEMM: every variant of every point
   if (precision == PHF_PREC_FP64) {
     eri_gemm_f64(s, b.Pt, ctx->dA, b.Gam, (int)b.Mp, nrpad, kt, nrpad, b.bm, ctx->dHb);
   } else if (precision == PHF_PREC_FP32) {
@@ -1006,41 +1051,58 @@ int eval_gemm(phf_ctx* ctx, const EvalBufs& b) {
 
 void eval_post(phf_ctx* ctx, const EvalBufs& b, bool resident) {
   hipStream_t s = ctx->stream;
-  const int nso = ctx->nso, N = b.N, nci = ctx->g_nci, nrpad = ctx->nrpad;
+  const int nso = ctx->nso, N = b.N, nci = ctx->g_nci, nrpad = ctx->nrpad, V = b.V;
   const int nmn = nci * nci;
-  const long G = b.G, n2 = (long)nso * nso, lenX = (long)nso * N;
+  const long G = b.G, Gv = V * b.G, n2 = (long)nso * nso, lenX = (long)nso * N;
   const long sT = 2 * lenX, sNN = 2L * N * N;
+  const int nB = (V == 4) ? 2 : 1;
+  const long per = (V == 4) ? 2 * G : G;
   // K4: FX = (h + Gamma) X   (the ERI GEMM epilogue added h)
   zgemm_batched(s, nso, N, nso, 1, 0, ZMat{b.Gam, nso, 1, 2L * nrpad, nrpad, 0}, zmat_colmajor(b.X, nso, sT), 0, 0,
-                zout_colmajor(b.FX, nso, sT), (int)G);
-  // Q_g = C0^H FX_g ; Hc = C0^H h
-  zgemm_batched(s, N, N, nso, 1, 0, ZMat{b.C0, 2L * nso, 2, 0, 1, 1}, zmat_colmajor(b.FX, nso, sT), 0, 0,
-                zout_colmajor(b.Q, N, sNN), (int)G);
-  zgemm_batched(s, N, nso, nso, 1, 0, ZMat{b.C0, 2L * nso, 2, 0, 1, 1}, ZMat{ctx->dH1, 2L * nso, 2, 0, 1, 0}, 0, 0,
-                zout_colmajor(b.Hc, N, 0), 1);
-  energy_kernel<<<(unsigned)G, 256, 0, s>>>(nso, N, b.Hc, b.X, b.Q, b.E);
-  // Y_g = FX_g + e_g X_g - X_g Q_g
-  yinit_kernel<<<(unsigned)std::min<long>((lenX * G + 255) / 256, 65536), 256, 0, s>>>(lenX, (int)G, b.FX, b.X, b.E,
-                                                                                         b.Y);
+                zout_colmajor(b.FX, nso, sT), (int)Gv);
+  // per right factor B: Q_v = B^H FX_v, Hc = B^H h, e_v = Makeh
+  for (int bc = 0; bc < nB; ++bc) {
+    const ZMat bh{b.C0, 2L * nso, 2, 0, 1, bc ? 0 : 1};
+    double* Hc = b.Hc + bc * sT;
+    zgemm_batched(s, N, N, nso, 1, 0, bh, zmat_colmajor(b.FX + bc * per * sT, nso, sT), 0, 0,
+                  zout_colmajor(b.Q + bc * per * sNN, N, sNN), (int)per);
+    zgemm_batched(s, N, nso, nso, 1, 0, bh, ZMat{ctx->dH1, 2L * nso, 2, 0, 1, 0}, 0, 0, zout_colmajor(Hc, N, 0), 1);
+    energy_kernel<<<(unsigned)per, 256, 0, s>>>(nso, N, Hc, b.X + bc * per * sT, b.Q + bc * per * sNN,
+                                                b.E + 2 * bc * per);
+  }
+  // Y_v = FX_v + e_v X_v - X_v Q_v
+  yinit_kernel<<<(unsigned)std::min<long>((lenX * Gv + 255) / 256, 65536), 256, 0, s>>>(lenX, (int)Gv, b.FX, b.X, b.E,
+                                                                                          b.Y);
   zgemm_batched(s, nso, N, N, -1, 0, zmat_colmajor(b.X, nso, sT), zmat_colmajor(b.Q, N, sNN), 1, 0,
-                zout_colmajor(b.Y, nso, sT), (int)G);
-  // K5: weighted grid sums
-  wd_kernel<<<(unsigned)((G * nmn + 255) / 256), 256, 0, s>>>((int)G, nmn, ctx->dGw, b.Det, b.Wd);
-  hs_reduce_kernel<<<1, 256, 0, s>>>((int)G, nmn, b.Wd, b.E, b.HS);
+                zout_colmajor(b.Y, nso, sT), (int)Gv);
+  // K5: weighted grid sums; every variant lands in its own CI block, so one
+  // reduction over all virtual points assembles the nci x nci matrices
+  const int d = (V == 4) ? nci / 2 : nci;
+  wd_kernel<<<(unsigned)((Gv * nmn + 255) / 256), 256, 0, s>>>((int)G, V, nci, d, ctx->dGw, b.Det, b.Wd);
+  // Hmat/Smat from variants 0 and 1 only (blocks (0,0), (0,1); PHFTools.f90:153-179)
+  hs_reduce_kernel<<<1, 256, 0, s>>>((int)((V == 4) ? 2 * G : G), nmn, b.Wd, b.E, b.HS);
   {
     dim3 grid((unsigned)((lenX + 255) / 256), (unsigned)b.nchunk);
     const unsigned fin = (unsigned)((lenX * nmn + 255) / 256);
-    wreduce_partial_kernel<<<grid, 256, 0, s>>>((int)G, b.gchunk, (int)lenX, nmn, b.Wd, b.Y, b.Part);
+    wreduce_partial_kernel<<<grid, 256, 0, s>>>((int)Gv, b.gchunk, (int)lenX, nmn, b.Wd, b.Y, b.Part);
     wreduce_final_kernel<<<fin, 256, 0, s>>>(b.nchunk, (int)lenX, nmn, b.Part, b.Ysum);
-    wreduce_partial_kernel<<<grid, 256, 0, s>>>((int)G, b.gchunk, (int)lenX, nmn, b.Wd, b.X, b.Part);
+    wreduce_partial_kernel<<<grid, 256, 0, s>>>((int)Gv, b.gchunk, (int)lenX, nmn, b.Wd, b.X, b.Part);
     wreduce_final_kernel<<<fin, 256, 0, s>>>(b.nchunk, (int)lenX, nmn, b.Part, b.Xsum);
   }
-  // Gradmat[mn] = Ysum[mn] C0^H, Rdmmat[mn] = Xsum[mn] C0^H (C-order [nso][nso])
+  // Gradmat[mn] = Ysum[mn] B^H, Rdmmat[mn] = Xsum[mn] B^H (C-order [nso][nso]);
+  // CI block rows m < d come from variants with B = C0, rows m >= d (V = 4)
+  // from variants with B = C0*
   if (!resident) {
-    zgemm_batched(s, nso, nso, N, 1, 0, zmat_colmajor(b.Ysum, nso, sT), ZMat{b.C0, 2L * nso, 2, 0, 1, 1}, 0, 0,
-                  ZOut{b.Grad, 2L * nso, 2, 2 * n2, 1}, nmn);
-    zgemm_batched(s, nso, nso, N, 1, 0, zmat_colmajor(b.Xsum, nso, sT), ZMat{b.C0, 2L * nso, 2, 0, 1, 1}, 0, 0,
-                  ZOut{b.Rdm, 2L * nso, 2, 2 * n2, 1}, nmn);
+    const int split = (V == 4) ? d * nci : nmn;
+    for (int part = 0; part < 2; ++part) {
+      const int m0 = part ? split : 0, cnt = part ? nmn - split : split;
+      if (cnt <= 0) continue;
+      const ZMat bh{b.C0, 2L * nso, 2, 0, 1, part ? 0 : 1};
+      zgemm_batched(s, nso, nso, N, 1, 0, zmat_colmajor(b.Ysum + m0 * sT, nso, sT), bh, 0, 0,
+                    ZOut{b.Grad + m0 * 2 * n2, 2L * nso, 2, 2 * n2, 1}, cnt);
+      zgemm_batched(s, nso, nso, N, 1, 0, zmat_colmajor(b.Xsum + m0 * sT, nso, sT), bh, 0, 0,
+                    ZOut{b.Rdm + m0 * 2 * n2, 2L * nso, 2, 2 * n2, 1}, cnt);
+    }
   }
 }
 
@@ -1049,13 +1111,12 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
   if (!ctx) return PHF_ERR_ARG;
   ctx->r_valid = false;
   if (!hmat || !smat) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: NULL output pointer");
-  if (cmplxconj != 0 && cmplxconj != 1)
-    return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: CmplxConj=2 is not implemented on the GPU path");
+  if (cmplxconj < 0 || cmplxconj > 2) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: CmplxConj must be 0, 1 or 2");
   if (ctx->has_ham && nocc > 0 && nocc < ctx->nso && !z) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: z is NULL");
   HIPCK(hipSetDevice(ctx->device));
   EvalBufs b;
   {
-    int rc = eval_plan(ctx, nocc, precision, b);
+    int rc = eval_plan(ctx, nocc, precision, cmplxconj, b);
     if (rc) return rc;
   }
   hipStream_t s = ctx->stream;
@@ -1077,8 +1138,9 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
   HIPCK(hipGetLastError());
   if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[3], s));
 
+  const long Gv = b.V * G;
   std::vector<double> hs(4 * (size_t)nmn);
-  std::vector<int> info((size_t)G);
+  std::vector<int> info((size_t)Gv);
   HIPCK(hipMemcpyAsync(hs.data(), b.HS, hs.size() * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCK(hipMemcpyAsync(info.data(), b.Info, info.size() * sizeof(int), hipMemcpyDeviceToHost, s));
   if (!resident) {
@@ -1107,16 +1169,31 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
     ctx->stage_ms[7] = t61;   // reduced-precision split / conversion of the densities
     ctx->nstage = 8;
   }
-  for (long g = 0; g < G; ++g)
-    if (info[g] != 0) {
+  for (long gv = 0; gv < Gv; ++gv)
+    if (info[gv] != 0) {
+      const long g = gv % G;
       const long il = g / ((long)ny * npg), iy = (g / npg) % ny, ipg = g % npg;
-      char buf[160];
-      snprintf(buf, sizeof buf, "Error in ZGetRF: singular overlap at grid point (il=%ld, iy=%ld, ipg=%ld)",
-               il + 1, iy + 1, ipg + 1);
+      char buf[200];
+      snprintf(buf, sizeof buf, "Error in ZGetRF: singular overlap at grid point (il=%ld, iy=%ld, ipg=%ld)%s",
+               il + 1, iy + 1, ipg + 1, b.V > 1 ? (gv / G == 0 ? " (RDM)" : " (conjugate-pair density)") : "");
       return fail(ctx, PHF_ERR_SINGULAR, buf);
     }
   std::memcpy(hmat, hs.data(), 2 * (size_t)nmn * sizeof(double));
   std::memcpy(smat, hs.data() + 2 * nmn, 2 * (size_t)nmn * sizeof(double));
+  if (b.V == 4) {
+    // Hmat(d:, :d) = Hmat(:d, d:)^H, Hmat(d:, d:) = Hmat(:d, :d), same for Smat (PHFTools.f90:174-179)
+    const int d = nci / 2;
+    for (double* mat : {hmat, smat})
+      for (int m = 0; m < d; ++m)
+        for (int n = 0; n < d; ++n) {
+          const size_t lo = 2 * ((size_t)(m + d) * nci + n), up = 2 * ((size_t)n * nci + m + d);
+          mat[lo] = mat[up];
+          mat[lo + 1] = -mat[up + 1];
+          const size_t dd = 2 * ((size_t)(m + d) * nci + n + d), tl = 2 * ((size_t)m * nci + n);
+          mat[dd] = mat[tl];
+          mat[dd + 1] = mat[tl + 1];
+        }
+  }
   // Hermiticity test (PHFTools.f90:195-200)
   double w1 = 0, w2 = 0, hmax = 0, smax = 0;
   for (int m = 0; m < nci; ++m)

@@ -411,7 +411,7 @@ static int plan_batch(phf_batch* bt, int prec) {
   size_t nc0 = 0, nout = 0;
   for (size_t i = 0; i < ni; ++i) {
     phf_ctx* c = bt->inst[i];
-    int rc = eval_plan(c, bt->nocc[i], prec, bt->bufs[i]);
+    int rc = eval_plan(c, bt->nocc[i], prec, 1, bt->bufs[i]);
     if (rc) return bfail(bt, rc, "instance " + std::to_string(i) + ": " + c->err);
     bt->off_c0[i] = nc0;
     nc0 += 2 * (size_t)c->nso * bt->nocc[i];

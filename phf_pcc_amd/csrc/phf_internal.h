@@ -112,6 +112,7 @@ int phf_keep_only_mixed(phf_ctx* ctx, int prec);
 // Device buffers of one evaluation (carved from the context's workspace).
 struct EvalBufs {
   int N = 0, prec = 0, bm = 0, gchunk = 32, nchunk = 0;
+  int V = 1;                  // density variants per grid point (4 for CmplxConj = 2)
   long Mp = 0, G = 0;
   double *C0 = nullptr, *Wpg = nullptr, *W2 = nullptr, *T = nullptr, *M = nullptr, *Minv = nullptr, *Det = nullptr;
   int* Info = nullptr;
@@ -130,7 +131,7 @@ struct EvalBufs {
 // unless resident, Gradmat/Rdmmat.  eval_pre/eval_post only launch kernels
 // (eval_pre records marks[0..2] after the rotation chain, the X_g solve and the
 // density pack when marks is given).
-int eval_plan(phf_ctx* ctx, int nocc, int precision, EvalBufs& b);
+int eval_plan(phf_ctx* ctx, int nocc, int precision, int cmplxconj, EvalBufs& b);
 void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks = nullptr);
 int eval_gemm(phf_ctx* ctx, const EvalBufs& b);
 void eval_post(phf_ctx* ctx, const EvalBufs& b, bool resident);

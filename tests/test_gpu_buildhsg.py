@@ -207,3 +207,45 @@ def test_direct_c_abi_one_shot_matches_drop_in(phftools):
     _lib.check(rc, ctx.handle)
     phftools.invalidate()
     _assert_close(out, cpu)
+
+
+@pytest.mark.parametrize("nsite,seed,j,ngrid,cplx", [(4, 5, 0, (14, 8), False), (6, 7, 0, (14, 8), False),
+                                                     (4, 11, 1, (26, 8), False), (4, 5, 0, (14, 8), True)])
+def test_cmplxconj2_against_oracle(phftools, nsite, seed, j, ngrid, cplx):
+    """CmplxConj = 2: four density variants per grid point, nci = 2 ncisp ncipg,
+    block assembly of Hmat/Smat/Gradmat/Rdmmat and the LocalG cross terms."""
+    case = hubbard_general_ref(nsite, 4.0, seed, zscale=0.3, complex_ref=cplx)
+    ga = grid_args(nsite, ngrid, j)
+    ga = dict(ga, nci=2 * ga["nci"])
+    gpu, cpu = _run_both(phftools, case, ga, cmplxconj=2)
+    _assert_close(gpu, cpu)
+    out = []
+    for res, is_gpu in ((gpu, True), (cpu, False)):
+        h, s, gm, rm = res
+        ev, vec = O.eigen_solver(h, s)
+        fsp, fpg, fk = O.vec_decomp(vec[:, 0], ga["ncisp"], 1, 2)
+        fn = phftools.localg if is_gpu else O.localg
+        g = fn(gm, rm, s, ev[0], fsp, fpg, fk, 2)
+        out.append((ev[0].real, g[case["nocc"]:, :case["nocc"]]))
+    (eg, gg), (ec, gc) = out
+    assert abs(eg - ec) < 1e-10
+    assert np.abs(gg - gc).max() < 1e-10
+
+
+def test_cmplxconj2_model_and_mixed(phftools):
+    """The device-resident model falls back to buildhsg + localg for CmplxConj = 2;
+    its E and gradient equal the oracle's EandG, and f16x2 stays within 1e-6 Eh."""
+    from phf_pcc_amd.model import ProjectedHF
+    case = hubbard_general_ref(6, 4.0, 7, zscale=0.3)
+    ga = grid_args(6, (14, 8), 0)
+    ga = dict(ga, nci=2 * ga["nci"])
+    m = ProjectedHF(case["nocc"], case["ref"], ga, hone=case["hone"], htwo=case["htwo"], cmplxconj=2)
+    e, g = m.energy_and_gradient(case["z"])
+    x = np.concatenate([case["z"].real.ravel(), case["z"].imag.ravel()])
+    e_ref, g_ref = O.eandg(x, case["hone"], case["htwo"], case["ref"], case["nocc"], ga, cmplxconj=2)
+    assert abs(e - e_ref) < 1e-10
+    assert np.abs(np.concatenate([g.real.ravel(), g.imag.ravel()]) - g_ref).max() < 1e-10
+    m.precision = "f16x2"
+    e16, g16 = m.energy_and_gradient(case["z"])
+    assert abs(e16 - e) <= 1e-6
+    assert np.abs(g16 - g).max() <= 1e-5
