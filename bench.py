@@ -109,10 +109,12 @@ def run_batch(args):
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
     if "fp64" not in forms:
         forms = forms + ["fp64"]
+    runs = [(f, True) for f in forms] + [("fp64", False)]     # last: the plain dense FP64 batch
     res = {}
-    for form in forms:
+    for form, structure in runs:
+        key = form if structure else "fp64_dense"
         t0 = time.time()
-        bt = ProjectedHFBatch(inst, storage=form)
+        bt = ProjectedHFBatch(inst, storage=form, structure=structure)
         t_build = time.time() - t0
         e, g, _ = bt.energy_and_gradient(zs, form, strict=False)
         eager_total, eager_gemm, _ = bt.stats()
@@ -125,35 +127,40 @@ def run_batch(args):
             dev.append(bt.stats()[0])
         dt = time.perf_counter() - t0
         status = bt.status.copy()
-        res[form] = dict(E=e.copy(), G=g, status=status, ms_step=dt / args.steps * 1e3, value=G * args.steps / dt,
-                         dev_ms=float(np.mean(dev)), eager_gemm_ms=eager_gemm, eager_total_ms=eager_total,
-                         build_s=t_build, tuning=bt.tuning_report().strip())
-        log(f"[bench] batch {form}: build {t_build:.1f}s  step {res[form]['ms_step']:.2f} ms  device "
-            f"{res[form]['dev_ms']:.2f} ms  (eager GEMM {eager_gemm:.2f} ms)  instances failing the "
-            f"Hermiticity/overlap test: {int((status != 0).sum())}")
+        res[key] = dict(E=e.copy(), G=g, status=status, ms_step=dt / args.steps * 1e3, value=G * args.steps / dt,
+                        dev_ms=float(np.mean(dev)), eager_gemm_ms=eager_gemm, eager_total_ms=eager_total,
+                        build_s=t_build, tuning=bt.tuning_report().strip(), prec=form, structure=structure)
+        log(f"[bench] batch {key}{' (spin blocks)' if structure else ''}: build {t_build:.1f}s  step "
+            f"{res[key]['ms_step']:.2f} ms  device {res[key]['dev_ms']:.2f} ms  (eager GEMM {eager_gemm:.2f} ms)  "
+            f"instances failing the Hermiticity/overlap test: {int((status != 0).sum())}")
         bt.close()
         del bt
-    base = res["fp64"]
+    base = res["fp64_dense"]
     for f, r in res.items():
         ok = (r["status"] == 0) & (base["status"] == 0)
         r["dE"] = float(np.abs(r["E"] - base["E"])[ok].max()) if ok.any() else None
         r["dG"] = float(max(np.abs(a - b).max() for a, b, k in zip(r["G"], base["G"], ok) if k)) if ok.any() else None
-        log(f"[bench] batch {f}: max|dE| vs fp64 = {r['dE']:.3e}  max|dG| = {r['dG']:.3e}  speedup vs fp64 = "
-            f"{r['value'] / base['value']:.2f}x")
+        log(f"[bench] batch {f}: max|dE| vs plain fp64 = {r['dE']:.3e}  max|dG| = {r['dG']:.3e}  speedup vs plain "
+            f"fp64 = {r['value'] / base['value']:.2f}x  vs fp64 on the same blocks = "
+            f"{r['value'] / res['fp64']['value']:.2f}x")
     okf = [r for r in res.values() if r["dE"] is not None and r["dE"] <= 1e-6]
     best = max(okf, key=lambda r: r["value"])
-    bprec = [f for f, r in res.items() if r is best][0]
+    bprec = best["prec"]
+    ns_ = ns.astype(np.float64)
+    flops_blk = float(np.sum(4.0 * ((2 * ns_ ** 2) ** 2 + 2 * (ns_ ** 2) ** 2) * gs))
+    flops_best = flops_blk if best["structure"] else flops
     nprod = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3}[bprec]
     t_gemm = best["eager_gemm_ms"] * 1e-3
-    roof = {"bound": "mfma", "achieved": nprod * flops / t_gemm / 1e12, "peak": PEAK[bprec] / 1e12,
-            "unit": "TFLOP/s", "frac": nprod * flops / t_gemm / PEAK[bprec], "traffic": None,
+    roof = {"bound": "mfma", "achieved": nprod * flops_best / t_gemm / 1e12, "peak": PEAK[bprec] / 1e12,
+            "unit": "TFLOP/s", "frac": nprod * flops_best / t_gemm / PEAK[bprec], "traffic": None,
             "kernel": "eri_gemm_h16_grouped_kernel" if nprod > 1 else "eri_gemm_f64_kernel (per instance)",
-            "flops_per_launch": nprod * flops, "algorithmic_flops_per_launch": flops, "products": nprod,
-            "fp64_equiv_tflops": flops / t_gemm / 1e12,
+            "flops_per_launch": nprod * flops_best, "algorithmic_flops_per_launch": flops_best, "products": nprod,
+            "spin_blocks": best["structure"], "fp64_equiv_tflops": flops_best / t_gemm / 1e12,
             "time_source": "HIP events around the grouped launch in the eager first evaluation"}
     forms_out = {f: {"grid_pts_per_s": r["value"], "evals_per_s": len(inst) * 1e3 / r["ms_step"],
                      "ms_per_step": r["ms_step"], "device_ms": r["dev_ms"], "eager_eri_gemm_ms": r["eager_gemm_ms"],
-                     "build_s": r["build_s"], "max_abs_dE_vs_fp64": r["dE"], "max_abs_dG_vs_fp64": r["dG"],
+                     "build_s": r["build_s"], "max_abs_dE_vs_plain_fp64": r["dE"], "max_abs_dG_vs_plain_fp64": r["dG"],
+                     "spin_blocks": r["structure"], "speedup_vs_plain_fp64": r["value"] / base["value"],
                      "failing_instances": int((r["status"] != 0).sum()), "tuning": r["tuning"] or None}
                  for f, r in res.items()}
     out = {"metric": "projected E+grad evals/s (grid pts/s) vs plain FP64 GPU at 1e-6 Eh; % MFMA peak",
@@ -165,8 +172,10 @@ def run_batch(args):
            "config": {"workload": f"batched: {len(inst)} synthetic instances, n in {{16..96}}, N=n/2, L=3n, grids "
                                   f"Lebedev {{6..50}} x gamma {{4,8,16}} (seed 1004), {G} grid points per step, "
                                   "E + adjoint gradient per instance, hipGraph replay",
-                      "form": bprec, "evals_per_s": len(inst) * 1e3 / best["ms_step"],
-                      "speedup_vs_fp64_batch": best["value"] / base["value"], "forms": forms_out},
+                      "form": bprec + (" (spin blocks)" if best["structure"] else ""),
+                      "evals_per_s": len(inst) * 1e3 / best["ms_step"],
+                      "speedup_vs_plain_fp64_batch": best["value"] / base["value"],
+                      "speedup_vs_fp64_same_blocks": best["value"] / res["fp64"]["value"], "forms": forms_out},
            "roofline": roof, "cpu_baseline": None}
     print(json.dumps(out), flush=True)
 
@@ -196,7 +205,9 @@ def run_large(args):
     if "fp64" not in forms:
         forms = ["fp64"] + forms
     res = {}
-    for f in forms:
+    blocked = model.ctx.blocked()
+    for key, f, blk in [("fp64_dense", "fp64", False)] + [(f, f, True) for f in forms]:
+        model.ctx.set_blocked(blk)
         model.precision = f
         for _ in range(args.warmup):
             model.energy_and_gradient(prob.z)
@@ -206,32 +217,39 @@ def run_large(args):
             st.append(model.ctx.stage_times())
         dt = time.perf_counter() - t0
         sm = np.mean(np.array(st), axis=0)
-        res[f] = dict(E=e, G=g, ms=dt / args.steps * 1e3, value=G * args.steps / dt, gemm_ms=sm[2], dev_ms=sm[0])
-        log(f"[bench] n=128 {f}: step {res[f]['ms']:.2f} ms  ERI GEMM {sm[2]:.2f} ms  E={e:.12f}")
-    base = res["fp64"]
-    flops = 4.0 * nso ** 4 * G
+        res[key] = dict(E=e, G=g, ms=dt / args.steps * 1e3, value=G * args.steps / dt, gemm_ms=sm[2], dev_ms=sm[0],
+                        prec=f, blocked=blk and blocked)
+        log(f"[bench] n=128 {key}{' (spin blocks)' if blk and blocked else ''}: step {res[key]['ms']:.2f} ms  "
+            f"ERI GEMM {sm[2]:.2f} ms  E={e:.12f}")
+    base = res["fp64_dense"]
+    flops_dense = 4.0 * nso ** 4 * G
+    flops_blk = 4.0 * ((2 * n ** 2) ** 2 + 2 * (n ** 2) ** 2) * G
     out_forms = {}
     for f, r in res.items():
-        nprod = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3}[f]
+        flops = flops_blk if r["blocked"] else flops_dense
+        nprod = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3}[r["prec"]]
         out_forms[f] = {"grid_pts_per_s": r["value"], "ms_per_step": r["ms"], "eri_gemm_ms": r["gemm_ms"],
                         "device_ms": r["dev_ms"], "E": r["E"], "abs_dE_vs_fp64": abs(r["E"] - base["E"]),
                         "max_abs_dG_vs_fp64": float(np.abs(r["G"] - base["G"]).max()),
-                        "speedup_vs_fp64": r["value"] / base["value"],
-                        "mfma_frac": nprod * flops / (r["gemm_ms"] * 1e-3) / PEAK[f]}
-        log(f"[bench] n=128 {f}: |dE| = {out_forms[f]['abs_dE_vs_fp64']:.3e}  speedup {out_forms[f]['speedup_vs_fp64']:.2f}x"
-            f"  MFMA {100 * out_forms[f]['mfma_frac']:.1f}%")
+                        "speedup_vs_plain_fp64": r["value"] / base["value"], "spin_blocks": r["blocked"],
+                        "mfma_frac": nprod * flops / (r["gemm_ms"] * 1e-3) / PEAK[r["prec"]]}
+        log(f"[bench] n=128 {f}: |dE| = {out_forms[f]['abs_dE_vs_fp64']:.3e}  speedup vs plain fp64 "
+            f"{out_forms[f]['speedup_vs_plain_fp64']:.2f}x  MFMA {100 * out_forms[f]['mfma_frac']:.1f}%")
     ok = [f for f in res if out_forms[f]["abs_dE_vs_fp64"] <= 1e-6]
     best = max(ok, key=lambda f: res[f]["value"])
-    nprod = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3}[best]
+    bprec = res[best]["prec"]
+    nprod = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3}[bprec]
+    flops = flops_blk if res[best]["blocked"] else flops_dense
     t_gemm = res[best]["gemm_ms"] * 1e-3
     out = {"metric": "projected E+grad evals/s (grid pts/s) vs plain FP64 GPU at 1e-6 Eh; % MFMA peak",
            "value": res[best]["value"], "unit": "grid pts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": res[best]["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": _form_dtype(best), "data": "synthetic",
+           "dtype": _form_dtype(bprec), "data": "synthetic",
            "config": {"workload": f"projected E+grad, synthetic n={n} (NSO={nso}), N={nocc}, L={3 * n}, grid 50x24={G} "
                                   "(seed 1003), device-resident objective", "form": best, "forms": out_forms},
-           "roofline": {"bound": "mfma", "achieved": nprod * flops / t_gemm / 1e12, "peak": PEAK[best] / 1e12,
-                        "unit": "TFLOP/s", "frac": nprod * flops / t_gemm / PEAK[best], "traffic": None},
+           "roofline": {"bound": "mfma", "achieved": nprod * flops / t_gemm / 1e12, "peak": PEAK[bprec] / 1e12,
+                        "unit": "TFLOP/s", "frac": nprod * flops / t_gemm / PEAK[bprec], "traffic": None,
+                        "algorithmic_flops_per_launch": flops, "spin_blocks": res[best]["blocked"]},
            "cpu_baseline": None}
     print(json.dumps(out), flush=True)
 
@@ -428,13 +446,19 @@ def main():
         ctx.autotune(False)
     if "fp64" not in forms:
         forms = ["fp64"] + forms
-    res = {f: run_form(f) for f in forms}
-    base = res["fp64"]
+    # the plain FP64 GPU form: MakeGamma over the dense NSO^2 x NSO^2 A
+    ctx.set_blocked(False)
+    plain = run_form("fp64", tag="plain dense ")
+    # every form through the spin blocks of A (spin-pure Ref)
+    ctx.set_blocked(True)
+    blocked = ctx.blocked()
+    res = {f: run_form(f, tag="spin-block " if blocked else "") for f in forms}
+    base = plain
     for f, r in res.items():
         r["dE"] = abs(r["E"] - base["E"])
         r["dG"] = float(np.abs(r["G"] - base["G"]).max())
-        log(f"[bench] {f}: |dE| vs fp64 = {r['dE']:.3e}  max|dG| = {r['dG']:.3e}  "
-            f"speedup vs fp64 = {r['value'] / base['value']:.2f}x")
+        log(f"[bench] {f}: |dE| vs plain fp64 = {r['dE']:.3e}  max|dG| = {r['dG']:.3e}  speedup vs plain fp64 = "
+            f"{r['value'] / base['value']:.2f}x  vs fp64 on the same blocks = {r['value'] / res['fp64']['value']:.2f}x")
     dropin = {}
     if args.dropin:
         phftools.set_hamiltonian(prob.hone, prob.htwo)
@@ -457,7 +481,11 @@ def main():
         log(f"[bench] tuning {line}")
     ok = [r for r in res.values() if r["dE"] <= 1e-6]
     best = max(ok, key=lambda r: r["value"])
-    flops = 4.0 * nso ** 4 * G                       # algorithmic FLOPs of one ERI GEMM launch
+    # algorithmic FLOPs of the ERI GEMM per evaluation: dense 4 NSO^4 G; over the
+    # spin blocks 4 (nD^2 + 2 nX^2) G with nD = 2 n^2, nX = n^2 (n alpha, n beta MOs)
+    flops_dense = 4.0 * nso ** 4 * G
+    n_sp = nso // 2
+    flops = 4.0 * ((2 * n_sp ** 2) ** 2 + 2 * (n_sp ** 2) ** 2) * G if blocked else flops_dense
     nprod = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3}[best["prec"]]
     peak = PEAK[best["prec"]]
     t_gemm = best["gemm_ms"] * 1e-3
@@ -465,15 +493,16 @@ def main():
             "frac": nprod * flops / t_gemm / peak, "traffic": None,
             "kernel": {"fp64": "eri_gemm_f64_kernel", "fp32": "eri_gemm_f32_kernel",
                        "bf16x3": "eri_gemm_h16_kernel<bf16>", "f16x2": "eri_gemm_h16_kernel<f16>"}[best["prec"]],
-            "flops_per_launch": nprod * flops, "algorithmic_flops_per_launch": flops,
-            "products": nprod, "fp64_equiv_tflops": flops / t_gemm / 1e12}
+            "flops_per_launch": nprod * flops, "algorithmic_flops_per_launch": flops, "spin_blocks": blocked,
+            "products": nprod, "fp64_equiv_tflops": flops / t_gemm / 1e12,
+            "plain_fp64_dense_frac": flops_dense / (plain["gemm_ms"] * 1e-3) / PEAK["fp64"]}
     value = best["value"]
     ms_step = best["ms_step"]
 
     # HBM-bound stages of the plain FP64 form: algorithmic bytes per grid point
     # rotation 32 NSO N (read C, write T); density 16 NSO N + 16 N^2 + 2 s NSO^2
     # (read T, M; write the re/im rho planes, s = 8)
-    st64 = base["stages"]
+    st64 = plain["stages"]
     b_rot = 32.0 * nso * nocc * G
     b_den = (16.0 * nso * nocc + 16.0 * nocc ** 2 + 2 * 8 * nso ** 2) * G
     t_rot, t_den = st64[4] * 1e-3, (st64[5] + st64[6]) * 1e-3
@@ -498,9 +527,14 @@ def main():
         roof["traffic"] = traffic["bytes_per_launch"]
         roof["traffic_source"] = traffic["source"]
 
-    forms_out = {f: {"grid_pts_per_s": r["value"], "evals_per_s": r["value"] / G, "ms_per_step": r["ms_step"],
-                     "eri_gemm_ms": r["gemm_ms"], "device_ms": r["dev_ms"], "abs_dE_vs_fp64": r["dE"],
-                     "max_abs_dG_vs_fp64": r["dG"]} for f, r in res.items()}
+    def form_out(r, fl):
+        nprod_ = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3}[r["prec"]]
+        return {"grid_pts_per_s": r["value"], "evals_per_s": r["value"] / G, "ms_per_step": r["ms_step"],
+                "eri_gemm_ms": r["gemm_ms"], "device_ms": r["dev_ms"], "abs_dE_vs_plain_fp64": r.get("dE", 0.0),
+                "max_abs_dG_vs_plain_fp64": r.get("dG", 0.0), "speedup_vs_plain_fp64": r["value"] / base["value"],
+                "mfma_frac": nprod_ * fl / (r["gemm_ms"] * 1e-3) / PEAK[r["prec"]]}
+    forms_out = {f: form_out(r, flops) for f, r in res.items()}
+    plain_out = form_out(plain, flops_dense)
     out = {"metric": "projected E+grad evals/s (grid pts/s) vs plain FP64 GPU at 1e-6 Eh; % MFMA peak",
            "value": value, "unit": "grid pts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -509,9 +543,10 @@ def main():
            "data": "synthetic",
            "config": {"workload": f"projected E+grad (EandG), synthetic n={args.n} (NSO={nso}), N={nocc}, "
                                   f"L={3*args.n}, grid 50x16={G}, complex128 host interface",
-                      "form": best["prec"], "evals_per_s": value / G, "grid_points": G,
-                      "speedup_vs_plain_fp64": value / base["value"], "abs_dE_vs_fp64": best["dE"],
-                      "plain_fp64": forms_out["fp64"], "forms": forms_out,
+                      "form": best["prec"] + (" (spin blocks)" if blocked else ""), "evals_per_s": value / G,
+                      "grid_points": G, "speedup_vs_plain_fp64": value / base["value"],
+                      "speedup_vs_fp64_same_blocks": value / res["fp64"]["value"], "abs_dE_vs_fp64": best["dE"],
+                      "plain_fp64": plain_out, "forms": forms_out,
                       "path": "device-resident objective (phf_eval_resident + phf_localg_vo), factored Hamiltonian "
                               "built on the device",
                       "hbm_stages_fp64": stage_out,

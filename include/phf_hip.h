@@ -33,6 +33,26 @@ enum {
   PHF_ERR_NOMEM = 7           /* device allocation failed */
 };
 
+/* Flags of phf_set_hamiltonian / phf_set_hamiltonian_factored.
+ * PHF_HAM_COMPLEX      treat A as complex even if its imaginary part is zero
+ *                      (dense entry point only).
+ * PHF_HAM_DENSE        never use the spin-block structure: every evaluation
+ *                      contracts the dense NSO^2 x NSO^2 A (MakeGamma as the
+ *                      reference does it, PHFTools.f90:310-337).
+ * PHF_HAM_BLOCKS_ONLY  when the structure is found, keep only the blocks and
+ *                      release the dense A (factored entry point).
+ * Without PHF_HAM_DENSE the library checks, once Ref is known (the factored
+ * builder's Ref, or phf_set_grid's for the dense entry point), whether every
+ * MO is purely alpha or beta and every entry of A outside the spin blocks is
+ * exactly zero; then it stores A as three diagonal blocks (6 n^4 of the 16 n^4
+ * entries for n alpha and n beta MOs) and evaluates through them.  The result
+ * is the same contraction minus exact zeros. */
+enum {
+  PHF_HAM_COMPLEX = 1,
+  PHF_HAM_DENSE = 2,
+  PHF_HAM_BLOCKS_ONLY = 4
+};
+
 /* Precision of the ERI contraction (the overlap LU, determinant, densities,
  * the Fock contraction and the grid sum are always FP64).  The reduced forms
  * run on the FP32 / BF16 / FP16 matrix cores with FP32 accumulation folded
@@ -61,7 +81,7 @@ const char* phf_last_error(const phf_ctx* ctx);
  * (PHFTools.f90:17): hone complex [nso][nso], htwo complex [nso]^4 with
  * htwo[i][j][k][l] = <ij||kl>.  The device keeps A[(i,k),(j,l)] = V(i,j,k,l)
  * (real plane only when every imaginary part is exactly zero) until the next
- * call.  flags: bit 0 forces the complex form. */
+ * call.  flags: PHF_HAM_* (bit 0 forces the complex form). */
 int phf_set_hamiltonian(phf_ctx* ctx, const double* hone, const double* htwo, int nso, int flags);
 
 /* Device-side construction of the same A from a factored spatial Hamiltonian,
@@ -72,7 +92,7 @@ int phf_set_hamiltonian(phf_ctx* ctx, const double* hone, const double* htwo, in
  *   hone = Ref^H (h (x) I2) Ref,  Bt_L = Ref^H (b_L (x) I2) Ref,
  *   <ij||kl> = sum_L Bt_L(i,k) Bt_L(j,l) - Bt_L(i,l) Bt_L(j,k).
  * h: double [n][n]; b: double [nl][n][n]; ref: complex [2n][2n] with zero
- * imaginary part (a complex Ref returns PHF_ERR_UNSUPPORTED).  flags unused. */
+ * imaginary part (a complex Ref returns PHF_ERR_UNSUPPORTED).  flags: PHF_HAM_*. */
 int phf_set_hamiltonian_factored(phf_ctx* ctx, const double* h, const double* b, int n, int nl,
                                  const double* ref, int flags);
 /* Copy the device's MO-basis hone (complex [nso][nso]) to the host. */
@@ -145,6 +165,13 @@ int phf_localg(phf_ctx* ctx, const double* gradmat, const double* rdmmat, const 
 int phf_set_profiling(phf_ctx* ctx, int on);
 int phf_stage_times(const phf_ctx* ctx, double* ms, int nmax);
 
+/* Spin-block evaluation switch (no counterpart in the reference).
+ * phf_set_blocked(ctx, 0) evaluates through the dense A although blocks exist
+ * (fails if the dense A was released); phf_set_blocked(ctx, 1) returns to the
+ * blocks.  phf_blocked reports 1 when evaluations run over the blocks. */
+int phf_set_blocked(phf_ctx* ctx, int on);
+int phf_blocked(const phf_ctx* ctx);
+
 /* Tile autotuner of the reduced-precision ERI GEMM (no counterpart in the
  * reference).  on = 1 (default): the first call for a problem shape times
  * every tile/split-K candidate on the live operands and keeps the fastest;
@@ -162,7 +189,8 @@ int phf_tuning_report(const phf_ctx* ctx, char* buf, int len);
  * (phf_set_grid arguments with nci = ncisp = ncipg = 1; weightpg may be NULL
  * for ones); storage = PHF_PREC_FP64 keeps the FP64 A, a reduced-precision
  * mode keeps only that mode's copy of A (the instance then evaluates in that
- * mode only).  phf_batch_eval takes the Z of every instance concatenated in
+ * mode only); flags = PHF_HAM_DENSE keeps the dense A (plain contraction),
+ * otherwise a spin-block structured A is stored as its blocks only.  phf_batch_eval takes the Z of every instance concatenated in
  * instance order (complex C order (2n - nocc) x nocc each) and returns per
  * instance the projected energy E0 (energies[i]), optionally the 1 x 1 Hmat,
  * Smat (hs[4 i .. 4 i + 3] = Re H, Im H, Re S, Im S) and the gradient block
@@ -186,7 +214,7 @@ const char* phf_batch_last_error(const phf_batch* b);
 int phf_batch_set_instance(phf_batch* b, int i, const double* h, const double* bfac, int n, int nl,
                            const double* ref, int nocc, const int* ngrid, const double* r1, int nr1,
                            const double* r2, int nr2, const double* rpg, int npg, const double* weightsp,
-                           const double* weightpg, int storage);
+                           const double* weightpg, int storage, int flags);
 int phf_batch_eval(phf_batch* b, const double* z, int cmplxconj, int precision, double* energies,
                    double* grads, double* hs, int* status);
 int phf_batch_set_graph(phf_batch* b, int on);

@@ -23,6 +23,9 @@ PHF_ERR_UNSUPPORTED = 6
 PHF_ERR_NOMEM = 7
 
 PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2, "f16x2": 3}
+PHF_HAM_COMPLEX = 1
+PHF_HAM_DENSE = 2
+PHF_HAM_BLOCKS_ONLY = 4
 
 # every symbol declared in include/phf_hip.h
 EXPORTS = ["phf_ctx_create", "phf_ctx_destroy", "phf_last_error", "phf_set_hamiltonian",
@@ -30,7 +33,7 @@ EXPORTS = ["phf_ctx_create", "phf_ctx_destroy", "phf_last_error", "phf_set_hamil
            "phf_buildhsg_z", "phf_autotune", "phf_tuning_report", "phf_set_hamiltonian_factored",
            "phf_get_hone", "phf_eval_resident", "phf_localg_vo", "phf_batch_create", "phf_batch_destroy",
            "phf_batch_last_error", "phf_batch_set_instance", "phf_batch_eval", "phf_batch_set_graph",
-           "phf_batch_autotune", "phf_batch_stats", "phf_batch_tuning_report"]
+           "phf_batch_autotune", "phf_batch_stats", "phf_batch_tuning_report", "phf_set_blocked", "phf_blocked"]
 
 
 class PhfError(RuntimeError):
@@ -82,12 +85,14 @@ def load() -> C.CDLL:
     lib.phf_tuning_report.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
     lib.phf_set_profiling.argtypes = [C.c_void_p, C.c_int]
     lib.phf_stage_times.argtypes = [C.c_void_p, _dp, C.c_int]
+    lib.phf_set_blocked.argtypes = [C.c_void_p, C.c_int]
+    lib.phf_blocked.argtypes = [C.c_void_p]
     lib.phf_batch_create.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_void_p)]
     lib.phf_batch_destroy.argtypes = [C.c_void_p]
     lib.phf_batch_last_error.argtypes = [C.c_void_p]
     lib.phf_batch_last_error.restype = C.c_char_p
     lib.phf_batch_set_instance.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp, C.c_int, _ip,
-                                           _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp, C.c_int]
+                                           _dp, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp, C.c_int, C.c_int]
     lib.phf_batch_eval.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, _dp, _dp, _dp, _ip]
     lib.phf_batch_set_graph.argtypes = [C.c_void_p, C.c_int]
     lib.phf_batch_autotune.argtypes = [C.c_void_p, C.c_int]
@@ -151,15 +156,17 @@ class Context:
         except Exception:
             pass
 
-    def set_hamiltonian_factored(self, h: np.ndarray, b: np.ndarray, ref: np.ndarray, key=None):
-        """Build A on the device from (pq|rs) = sum_L b[L,p,q] b[L,r,s] and a real Ref."""
+    def set_hamiltonian_factored(self, h: np.ndarray, b: np.ndarray, ref: np.ndarray, key=None, flags: int = 0):
+        """Build A on the device from (pq|rs) = sum_L b[L,p,q] b[L,r,s] and a real Ref
+        (flags: PHF_HAM_DENSE, PHF_HAM_BLOCKS_ONLY)."""
         h = np.ascontiguousarray(h, dtype=np.float64)
         b = np.ascontiguousarray(b, dtype=np.float64)
         ref = np.ascontiguousarray(ref, dtype=np.complex128)
         n = h.shape[0]
         if h.shape != (n, n) or b.ndim != 3 or b.shape[1:] != (n, n) or ref.shape != (2 * n, 2 * n):
             raise ValueError("h (n, n), b (L, n, n) and ref (2n, 2n) expected")
-        check(load().phf_set_hamiltonian_factored(self._h, ptr(h), ptr(b), n, b.shape[0], ptr(ref), 0), self._h)
+        check(load().phf_set_hamiltonian_factored(self._h, ptr(h), ptr(b), n, b.shape[0], ptr(ref), int(flags)),
+              self._h)
         self.nso = 2 * n
         self.ham_key = key
 
@@ -179,17 +186,27 @@ class Context:
         n = load().phf_tuning_report(self._h, buf, len(buf))
         return buf.raw[:n].decode()
 
+    def set_blocked(self, on: bool):
+        """Evaluate through the spin blocks of A (True, the default when they
+        exist) or through the dense A (False: the plain contraction)."""
+        check(load().phf_set_blocked(self._h, int(bool(on))), self._h)
+
+    def blocked(self) -> bool:
+        return bool(load().phf_blocked(self._h))
+
     def stage_times(self) -> np.ndarray:
         out = np.zeros(8)
         n = load().phf_stage_times(self._h, ptr(out), 8)
         return out[:n].copy()
 
-    def set_hamiltonian(self, hone: np.ndarray, htwo: np.ndarray, force_complex: bool = False, key=None):
+    def set_hamiltonian(self, hone: np.ndarray, htwo: np.ndarray, force_complex: bool = False, key=None,
+                        dense: bool = False):
         hone = np.ascontiguousarray(hone, dtype=np.complex128)
         htwo = np.ascontiguousarray(htwo, dtype=np.complex128)
         nso = hone.shape[0]
         if hone.shape != (nso, nso) or htwo.shape != (nso,) * 4:
             raise ValueError("hone must be (nso, nso) and htwo (nso, nso, nso, nso)")
-        check(load().phf_set_hamiltonian(self._h, ptr(hone), ptr(htwo), nso, 1 if force_complex else 0), self._h)
+        flags = (PHF_HAM_COMPLEX if force_complex else 0) | (PHF_HAM_DENSE if dense else 0)
+        check(load().phf_set_hamiltonian(self._h, ptr(hone), ptr(htwo), nso, flags), self._h)
         self.nso = nso
         self.ham_key = key

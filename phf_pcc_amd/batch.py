@@ -24,12 +24,14 @@ from . import _lib
 
 
 class ProjectedHFBatch:
-    def __init__(self, instances, storage: str = "fp64", device: int = 0, graph: bool = True):
+    def __init__(self, instances, storage: str = "fp64", device: int = 0, graph: bool = True, structure: bool = True):
         """instances: sequence of dicts with keys h (n x n), b (L x n x n), ref
         (2n x 2n real), nocc, and the J = 0 grid arguments ngrid, r1, r2, rpg,
         weightsp and optionally weightpg (npg values).  storage: "fp64" keeps
         the FP64 A of every instance (all precisions available); a reduced
-        mode keeps only that copy of A (evaluations in that mode only)."""
+        mode keeps only that copy of A (evaluations in that mode only).
+        structure: store a spin-block structured A as its blocks (False: the
+        dense A, the plain contraction)."""
         lib = _lib.load()
         self.n_inst = len(instances)
         h = C.c_void_p()
@@ -56,7 +58,8 @@ class ProjectedHFBatch:
             rc = lib.phf_batch_set_instance(self._h, i, _lib.ptr(hh), _lib.ptr(bb), n, bb.shape[0], _lib.ptr(ref),
                                             int(inst["nocc"]), ngrid.ctypes.data_as(_lib._ip), _lib.ptr(r1),
                                             r1.shape[0], _lib.ptr(r2), r2.shape[0], _lib.ptr(rpg), rpg.shape[0],
-                                            _lib.ptr(wsp), _lib.ptr(wpg), _lib.PREC[storage])
+                                            _lib.ptr(wsp), _lib.ptr(wpg), _lib.PREC[storage],
+                                            0 if structure else _lib.PHF_HAM_DENSE)
             _lib.check(rc, self._h, batch=True)
             self.nso.append(2 * n)
             self.nocc.append(int(inst["nocc"]))

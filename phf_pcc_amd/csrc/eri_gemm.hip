@@ -11,19 +11,23 @@
 // complex A the contraction runs over [A_re | A_im] and the density rows are
 // [P_re | -P_im] and [P_im | P_re].
 //
-// Layout contract: Pt is [Mp][K], A is [Nr][K], C is [Mp][ldc]; Mp % BM == 0,
-// Nr % BN == 0, K % 16 == 0 (the host pads with zeros).  An optional bias
-// [2][ldc] is added to even (real) and odd (imaginary) rows: the one-body h,
-// so that C holds h + Gamma for the Fock contraction that follows.
+// Layout contract: Pt is [Mp][ldp] (the contraction uses columns 0..K-1), A is
+// [Nr][K], C is [Mp][ldc]; Mp % BM == 0, Nr % BN == 0, K % 16 == 0 (the host
+// pads with zeros).  An optional bias [2][ldc] is added to even (real) and odd
+// (imaginary) rows: the one-body h, so that C holds h + Gamma for the Fock
+// contraction that follows.  An optional column map sends A row r to column
+// colmap[r] of C (and of the bias), -1 for padding rows that are not written:
+// a diagonal block of a permuted A writes its part of Gamma in place.
 #include "phf_common.h"
 
 namespace {
 
 template <int BM, int BN>
-__global__ __launch_bounds__(256, 2) void eri_gemm_f64_kernel(const double* __restrict__ P,
+__global__ __launch_bounds__(256, 2) void eri_gemm_f64_kernel(const double* __restrict__ P, long ldp,
                                                                const double* __restrict__ A,
                                                                double* __restrict__ C, int K, long ldc,
-                                                               int tiles_m, const double* __restrict__ bias) {
+                                                               int tiles_m, const double* __restrict__ bias,
+                                                               const int* __restrict__ colmap) {
   constexpr int BK = 16, LDSW = BK + 2;
   constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
   constexpr int PL = BM / 32, AL = BN / 32;
@@ -34,7 +38,7 @@ __global__ __launch_bounds__(256, 2) void eri_gemm_f64_kernel(const double* __re
   const int mt = t % tiles_m, nt = t / tiles_m;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const double* Pg = P + (long)mt * BM * K;
+  const double* Pg = P + (long)mt * BM * ldp;
   const double* Ag = A + (long)nt * BN * K;
   const int lr = tid >> 3, lc = (tid & 7) * 2;
 
@@ -48,20 +52,20 @@ __global__ __launch_bounds__(256, 2) void eri_gemm_f64_kernel(const double* __re
   // global -> registers (next k-slab) and registers -> LDS through named
   // registers: an array here is left in scratch memory by the compiler
   double2 rp0, rp1, rp2, rp3, ra0, ra1, ra2, ra3;
-#define ERI_LD(dst, base, e, k0) \
-  dst = *reinterpret_cast<const double2*>((base) + (long)(lr + 32 * (e)) * K + (k0) + lc)
+#define ERI_LD(dst, base, ld, e, k0) \
+  dst = *reinterpret_cast<const double2*>((base) + (long)(lr + 32 * (e)) * (ld) + (k0) + lc)
 #define ERI_ST(arr, buf, e, src) \
   *reinterpret_cast<double2*>(&arr[buf][(lr + 32 * (e)) * LDSW + lc]) = src
 #define ERI_GLOAD(k0)                                   \
   do {                                                  \
-    ERI_LD(rp0, Pg, 0, k0);                             \
-    if constexpr (PL > 1) ERI_LD(rp1, Pg, 1, k0);       \
-    if constexpr (PL > 2) ERI_LD(rp2, Pg, 2, k0);       \
-    if constexpr (PL > 3) ERI_LD(rp3, Pg, 3, k0);       \
-    ERI_LD(ra0, Ag, 0, k0);                             \
-    if constexpr (AL > 1) ERI_LD(ra1, Ag, 1, k0);       \
-    if constexpr (AL > 2) ERI_LD(ra2, Ag, 2, k0);       \
-    if constexpr (AL > 3) ERI_LD(ra3, Ag, 3, k0);       \
+    ERI_LD(rp0, Pg, ldp, 0, k0);                        \
+    if constexpr (PL > 1) ERI_LD(rp1, Pg, ldp, 1, k0);  \
+    if constexpr (PL > 2) ERI_LD(rp2, Pg, ldp, 2, k0);  \
+    if constexpr (PL > 3) ERI_LD(rp3, Pg, ldp, 3, k0);  \
+    ERI_LD(ra0, Ag, K, 0, k0);                          \
+    if constexpr (AL > 1) ERI_LD(ra1, Ag, K, 1, k0);    \
+    if constexpr (AL > 2) ERI_LD(ra2, Ag, K, 2, k0);    \
+    if constexpr (AL > 3) ERI_LD(ra3, Ag, K, 3, k0);    \
   } while (0)
 #define ERI_LSTORE(buf)                                 \
   do {                                                  \
@@ -110,7 +114,11 @@ __global__ __launch_bounds__(256, 2) void eri_gemm_f64_kernel(const double* __re
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const long row = (long)mt * BM + wm * WM + i * 16 + (lane >> 4) + 4 * r;   // f64 16x16x4 output row
-        const long col = (long)nt * BN + wn * WN + j * 16 + (lane & 15);
+        long col = (long)nt * BN + wn * WN + j * 16 + (lane & 15);
+        if (colmap) {
+          col = colmap[col];
+          if (col < 0) continue;
+        }
         C[row * ldc + col] = acc[i][j][r] + (bias ? bias[(row & 1) * ldc + col] : 0.0);
       }
 }
@@ -120,13 +128,13 @@ __global__ __launch_bounds__(256, 2) void eri_gemm_f64_kernel(const double* __re
 // Tile height for the grid-point dimension: the host pads Mp to this.
 int eri_gemm_f64_bm(int rows) { return rows <= 64 * 12 ? 64 : 128; }
 
-void eri_gemm_f64(hipStream_t s, const double* P, const double* A, double* C, int Mp, int Nr, int K,
-                  long ldc, int bm, const double* bias) {
+void eri_gemm_f64(hipStream_t s, const double* P, long ldp, const double* A, double* C, int Mp, int Nr, int K,
+                  long ldc, int bm, const double* bias, const int* colmap) {
   if (bm == 64) {
     const int tm = Mp / 64, tn = Nr / 128;
-    eri_gemm_f64_kernel<64, 128><<<tm * tn, 256, 0, s>>>(P, A, C, K, ldc, tm, bias);
+    eri_gemm_f64_kernel<64, 128><<<tm * tn, 256, 0, s>>>(P, ldp, A, C, K, ldc, tm, bias, colmap);
   } else {
     const int tm = Mp / 128, tn = Nr / 128;
-    eri_gemm_f64_kernel<128, 128><<<tm * tn, 256, 0, s>>>(P, A, C, K, ldc, tm, bias);
+    eri_gemm_f64_kernel<128, 128><<<tm * tn, 256, 0, s>>>(P, ldp, A, C, K, ldc, tm, bias, colmap);
   }
 }

@@ -69,12 +69,17 @@ struct H16Cfg {
   static_assert(LDS * MINB <= 160 * 1024, "LDS budget");
 };
 
+// Operand strides (elements): P rows ldP apart with slices sP apart, A rows
+// ldA apart with slices sA apart; the contraction runs over the first K
+// elements of each slice.  colmap (optional) sends A row r to column
+// colmap[r] of C and of the bias (-1: padding, not written).
 template <typename T, typename Cfg, int BM, int BN, int WM, int WN>
-__device__ __forceinline__ void h16_tile(const T* __restrict__ P, const T* __restrict__ A,
-                                         const double* __restrict__ invP, const double* __restrict__ invA,
-                                         double* __restrict__ C, const double* __restrict__ bias, long ldc, int K,
-                                         int nslice, unsigned patP, unsigned patA, int mt, int nt, int sbeg,
-                                         int send, bool atomic, int fold, unsigned char* smem) {
+__device__ __forceinline__ void h16_tile(const T* __restrict__ P, long ldP, long sP, const T* __restrict__ A,
+                                         long ldA, long sA, const double* __restrict__ invP,
+                                         const double* __restrict__ invA, double* __restrict__ C,
+                                         const double* __restrict__ bias, const int* __restrict__ colmap,
+                                         long ldc, int K, unsigned patP, unsigned patA, int mt, int nt,
+                                         int sbeg, int send, bool atomic, int fold, unsigned char* smem) {
   constexpr int BK = 64;
   constexpr int STAGES = Cfg::STAGES, NW = Cfg::NW, ROWS = Cfg::ROWS, STAGE_BYTES = Cfg::STAGE_BYTES;
   constexpr int FM = WM / 16, FN = WN / 16, WGN = BN / WN;
@@ -87,7 +92,6 @@ __device__ __forceinline__ void h16_tile(const T* __restrict__ P, const T* __res
   const int wm = wave / WGN, wn = wave % WGN;
   const int nk = send - sbeg;
   const int nsl = K / BK;                     // slabs per segment
-  const long rowlen = (long)nslice * K;
 
   const T* src[BPW];
 #pragma unroll
@@ -95,13 +99,13 @@ __device__ __forceinline__ void h16_tile(const T* __restrict__ P, const T* __res
     const int b = wave * BPW + q;
     const int r = 8 * b + (lane >> 3);
     const int c = (lane & 7) ^ ((r >> 1) & 7);
-    src[q] = ((r < BM) ? P + ((long)mt * BM + r) * rowlen : A + ((long)nt * BN + (r - BM)) * rowlen) + c * 8;
+    src[q] = ((r < BM) ? P + ((long)mt * BM + r) * ldP : A + ((long)nt * BN + (r - BM)) * ldA) + c * 8;
   }
   int iseg = sbeg / nsl, ioff = sbeg % nsl;   // segment / slab of the next slab to issue
   auto issue = [&](int slab) {
     unsigned char* st = smem + (slab % STAGES) * STAGE_BYTES;
-    const long oP = (long)((patP >> (3 * iseg)) & 7) * K + ioff * BK;
-    const long oA = (long)((patA >> (3 * iseg)) & 7) * K + ioff * BK;
+    const long oP = (long)((patP >> (3 * iseg)) & 7) * sP + ioff * BK;
+    const long oA = (long)((patA >> (3 * iseg)) & 7) * sA + ioff * BK;
 #pragma unroll
     for (int q = 0; q < BPW; ++q) {
       const bool isP = (wave * BPW + q) < BM / 8;     // wave-uniform
@@ -167,8 +171,10 @@ __device__ __forceinline__ void h16_tile(const T* __restrict__ P, const T* __res
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const long row = (long)mt * BM + wm * WM + i * 16 + 4 * (lane >> 4) + r;
-        const long col = (long)nt * BN + wn * WN + j * 16 + (lane & 15);
-        double v = (double)facc[i][j][r] * invP[row] * invA[col];
+        const long ar = (long)nt * BN + wn * WN + j * 16 + (lane & 15);
+        const long col = colmap ? colmap[ar] : ar;
+        if (col < 0) continue;
+        double v = (double)facc[i][j][r] * invP[row] * invA[ar];
         if (bias) v += bias[(row & 1) * ldc + col];
         if (atomic)
           atomicAdd(&C[row * ldc + col], v);
@@ -180,18 +186,18 @@ __device__ __forceinline__ void h16_tile(const T* __restrict__ P, const T* __res
 // One GEMM: workgroup -> (split part, row tile, column tile), XCD-aware.
 template <typename T, typename Cfg, int BM, int BN, int WM, int WN, int SPLITK>
 __global__ __launch_bounds__((Cfg::NT), (Cfg::MINB))
-void eri_gemm_h16_kernel(const T* __restrict__ P, const T* __restrict__ A, const double* __restrict__ invP,
-                         const double* __restrict__ invA, double* __restrict__ C, int K, int nslice, int nseg,
-                         unsigned patP, unsigned patA, long ldc, int tiles_m, const double* __restrict__ bias,
-                         int fold) {
+void eri_gemm_h16_kernel(const T* __restrict__ P, long ldP, long sP, const T* __restrict__ A, long ldA, long sA,
+                         const double* __restrict__ invP, const double* __restrict__ invA, double* __restrict__ C,
+                         int K, int nseg, unsigned patP, unsigned patA, long ldc, int tiles_m,
+                         const double* __restrict__ bias, const int* __restrict__ colmap, int fold) {
   extern __shared__ __attribute__((aligned(1024))) unsigned char smem[];
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   const int ks_id = t % SPLITK;
   const int tile = t / SPLITK;
   const int mt = tile % tiles_m, nt = tile / tiles_m;
   const int part = nseg * (K / 64) / SPLITK;
-  h16_tile<T, Cfg, BM, BN, WM, WN>(P, A, invP, invA, C, ks_id == 0 ? bias : nullptr, ldc, K, nslice, patP, patA, mt,
-                                   nt, ks_id * part, (ks_id + 1) * part, SPLITK > 1, fold, smem);
+  h16_tile<T, Cfg, BM, BN, WM, WN>(P, ldP, sP, A, ldA, sA, invP, invA, C, ks_id == 0 ? bias : nullptr, colmap, ldc, K,
+                                   patP, patA, mt, nt, ks_id * part, (ks_id + 1) * part, SPLITK > 1, fold, smem);
 }
 
 // Many GEMMs of one precision in one persistent launch: workgroup b takes
@@ -199,22 +205,25 @@ void eri_gemm_h16_kernel(const T* __restrict__ P, const T* __restrict__ A, const
 // XCD locality.
 template <typename T, typename Cfg, int BM, int BN, int WM, int WN, bool ATOMIC>
 __global__ __launch_bounds__((Cfg::NT), (Cfg::MINB))
-void eri_gemm_h16_grouped_kernel(const H16Item* __restrict__ items, int nitems, int nslice, unsigned patP,
-                                 unsigned patA, int fold) {
+void eri_gemm_h16_grouped_kernel(const H16Item* __restrict__ items, int nitems, unsigned patP, unsigned patA,
+                                 int fold) {
   extern __shared__ __attribute__((aligned(1024))) unsigned char smem[];
   for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
     const H16Item it = items[w];
     __syncthreads();   // every wave is done with the previous tile's LDS ring
-    h16_tile<T, Cfg, BM, BN, WM, WN>((const T*)it.P, (const T*)it.A, it.invP, it.invA, it.C, it.bias, it.ldc, it.K,
-                                     nslice, patP, patA, it.mt, it.nt, it.sbeg, it.send, ATOMIC, fold, smem);
+    h16_tile<T, Cfg, BM, BN, WM, WN>((const T*)it.P, it.ldP, it.sP, (const T*)it.A, it.ldA, it.sA, it.invP, it.invA,
+                                     it.C, it.bias, it.colmap, it.ldc, it.K, patP, patA, it.mt, it.nt, it.sbeg,
+                                     it.send, ATOMIC, fold, smem);
   }
 }
 
 // FP32 inputs, v_mfma_f32_16x16x4f32, 128 x 128 tile, k-slab 32, FP64 fold every FLUSH slabs.
 template <int FLUSH>
-__global__ __launch_bounds__(256, 1) void eri_gemm_f32_kernel(const float* __restrict__ P, const float* __restrict__ A,
-                                                              double* __restrict__ C, int K, long ldc, int tiles_m,
-                                                              const double* __restrict__ bias) {
+__global__ __launch_bounds__(256, 1) void eri_gemm_f32_kernel(const float* __restrict__ P, long ldp,
+                                                              const float* __restrict__ A, double* __restrict__ C,
+                                                              int K, long ldc, int tiles_m,
+                                                              const double* __restrict__ bias,
+                                                              const int* __restrict__ colmap) {
   constexpr int BM = 128, BN = 128, BK = 32, LDF = BK + 4;
   constexpr int WM = 64, WN = 64, FM = 4, FN = 4;
   __shared__ __attribute__((aligned(16))) float sP[2][BM * LDF];
@@ -223,7 +232,7 @@ __global__ __launch_bounds__(256, 1) void eri_gemm_f32_kernel(const float* __res
   const int mt = t % tiles_m, nt = t / tiles_m;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const float* Pg = P + (long)mt * BM * K;
+  const float* Pg = P + (long)mt * BM * ldp;
   const float* Ag = A + (long)nt * BN * K;
   const int lr = tid >> 3, lc = (tid & 7) * 4;   // 8 threads x 16 B per 32-element row
 
@@ -237,12 +246,14 @@ __global__ __launch_bounds__(256, 1) void eri_gemm_f32_kernel(const float* __res
       facc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
     }
   float4 rp0, rp1, rp2, rp3, ra0, ra1, ra2, ra3;
-#define F_LD(dst, base, e, k0) dst = *reinterpret_cast<const float4*>((base) + (long)(lr + 32 * (e)) * K + (k0) + lc)
+#define F_LD(dst, base, ld, e, k0) \
+  dst = *reinterpret_cast<const float4*>((base) + (long)(lr + 32 * (e)) * (ld) + (k0) + lc)
 #define F_ST(arr, buf, e, src) *reinterpret_cast<float4*>(&arr[buf][(lr + 32 * (e)) * LDF + lc]) = src
 #define F_GLOAD(k0)                                                                          \
   do {                                                                                       \
-    F_LD(rp0, Pg, 0, k0); F_LD(rp1, Pg, 1, k0); F_LD(rp2, Pg, 2, k0); F_LD(rp3, Pg, 3, k0); \
-    F_LD(ra0, Ag, 0, k0); F_LD(ra1, Ag, 1, k0); F_LD(ra2, Ag, 2, k0); F_LD(ra3, Ag, 3, k0); \
+    F_LD(rp0, Pg, ldp, 0, k0); F_LD(rp1, Pg, ldp, 1, k0); F_LD(rp2, Pg, ldp, 2, k0);           \
+    F_LD(rp3, Pg, ldp, 3, k0); F_LD(ra0, Ag, K, 0, k0); F_LD(ra1, Ag, K, 1, k0);               \
+    F_LD(ra2, Ag, K, 2, k0); F_LD(ra3, Ag, K, 3, k0);                                         \
   } while (0)
 #define F_LSTORE(buf)                                                                        \
   do {                                                                                       \
@@ -294,7 +305,11 @@ __global__ __launch_bounds__(256, 1) void eri_gemm_f32_kernel(const float* __res
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const long row = (long)mt * BM + wm * WM + i * 16 + 4 * (lane >> 4) + r;
-        const long col = (long)nt * BN + wn * WN + j * 16 + (lane & 15);
+        long col = (long)nt * BN + wn * WN + j * 16 + (lane & 15);
+        if (colmap) {
+          col = colmap[col];
+          if (col < 0) continue;
+        }
         C[row * ldc + col] = ((FLUSH > 0) ? facc[i][j][r] : (double)acc[i][j][r]) +
                              (bias ? bias[(row & 1) * ldc + col] : 0.0);
       }
@@ -310,9 +325,11 @@ template <> __device__ inline __bf16 to16<__bf16>(double x) { return (__bf16)(fl
 
 // One workgroup per row: s = 2^(top - ceil(log2 max|x|)); nslice slices x_i of
 // s*x (x_0 the rounding of s*x, x_1 of the remainder, ...), stored as
-// [x_0 | x_1 | ...] along the row (each kt long).
+// [x_0 | x_1 | ...] along the row, each kout long.  With cmap, output column c
+// takes source column cmap[c] (0 for -1); without, kout = kt and c maps to c.
 template <typename T>
 __global__ __launch_bounds__(256) void split_rows_kernel(const double* __restrict__ src, long lds, int kt,
+                                                         const int* __restrict__ cmap, int kout,
                                                          T* __restrict__ dst, int nslice, int top,
                                                          double* __restrict__ inv) {
   __shared__ double red[256];
@@ -333,12 +350,13 @@ __global__ __launch_bounds__(256) void split_rows_kernel(const double* __restric
   }
   const double sc = (m > 0.0) ? ldexp(1.0, top - e) : 1.0;
   if (threadIdx.x == 0) inv[row] = 1.0 / sc;
-  T* d = dst + row * (long)nslice * kt;
-  for (int c = threadIdx.x; c < kt; c += 256) {
-    double r = x[c] * sc;
+  T* d = dst + row * (long)nslice * kout;
+  for (int c = threadIdx.x; c < kout; c += 256) {
+    const int sc_col = cmap ? cmap[c] : c;
+    double r = (sc_col >= 0 ? x[sc_col] : 0.0) * sc;
     for (int q = 0; q < nslice; ++q) {
       const T v = to16<T>(r);
-      d[(long)q * kt + c] = v;
+      d[(long)q * kout + c] = v;
       r -= (double)(float)v;
     }
   }
@@ -377,11 +395,16 @@ static void mixed_patterns(int prec, unsigned* patP, unsigned* patA) {
   }
 }
 
-void split_operand(hipStream_t s, int prec, const double* src, long lds, int rows, int kt, void* dst, double* inv) {
+void split_operand_map(hipStream_t s, int prec, const double* src, long lds, int rows, int kt, const int* cmap,
+                       int kout, void* dst, double* inv) {
   if (prec == 3)
-    split_rows_kernel<_Float16><<<rows, 256, 0, s>>>(src, lds, kt, (_Float16*)dst, 2, 14, inv);
+    split_rows_kernel<_Float16><<<rows, 256, 0, s>>>(src, lds, kt, cmap, kout, (_Float16*)dst, 2, 14, inv);
   else
-    split_rows_kernel<__bf16><<<rows, 256, 0, s>>>(src, lds, kt, (__bf16*)dst, 3, 14, inv);
+    split_rows_kernel<__bf16><<<rows, 256, 0, s>>>(src, lds, kt, cmap, kout, (__bf16*)dst, 3, 14, inv);
+}
+
+void split_operand(hipStream_t s, int prec, const double* src, long lds, int rows, int kt, void* dst, double* inv) {
+  split_operand_map(s, prec, src, lds, rows, kt, nullptr, kt, dst, inv);
 }
 
 void convert_f32(hipStream_t s, const double* src, float* dst, long n) {
@@ -414,19 +437,24 @@ double h16_model_cost(int c, int sk, long Mp0, int Nr0, int ncu) {
   return (double)rounds * per_cu * bm * bn / sk;
 }
 
-// Mp multiple of the config's BM, Nr multiple of its BN; K (the slice
-// length) a multiple of 64 * splitk (16-bit) or 32 (FP32).  With splitk = 2 the
-// caller zero-fills C first.
-void eri_gemm_mixed(hipStream_t s, int prec, const void* P, const void* A, const double* invP, const double* invA,
-                    double* C, int Mp, int Nr, int K, long ldc, int cfg, int splitk, const double* bias) {
+// Mp multiple of the config's BM, Nr multiple of its BN; K (the contraction
+// length) a multiple of 64 * splitk (16-bit) or 32 (FP32).  P rows hold their
+// slices kp elements apart (kp >= K; the contraction reads the first K of
+// each), A rows are [nslice][K].  With splitk = 2 the caller zero-fills C
+// first.  colmap: optional A row -> C column map (-1: not written).
+void eri_gemm_mixed(hipStream_t s, int prec, const void* P, long kp, const void* A, const double* invP,
+                    const double* invA, double* C, int Mp, int Nr, int K, long ldc, int cfg, int splitk,
+                    const double* bias, const int* colmap) {
   if (prec == 1) {
     const int tm = Mp / 128, tn = Nr / 128;
-    eri_gemm_f32_kernel<16><<<tm * tn, 256, 0, s>>>((const float*)P, (const float*)A, C, K, ldc, tm, bias);
+    eri_gemm_f32_kernel<16><<<tm * tn, 256, 0, s>>>((const float*)P, kp, (const float*)A, C, K, ldc, tm, bias,
+                                                    colmap);
     return;
   }
   unsigned patP, patA;
   mixed_patterns(prec, &patP, &patA);
   const int nslice = mixed_slices(prec), nseg = mixed_segments(prec);
+  const long ldP = nslice * kp, ldA = (long)nslice * K;
   const int bm = kH16Cfgs[cfg][0], bn = kH16Cfgs[cfg][1];
   const int tm = Mp / bm, tn = Nr / bn;
   const unsigned grid = (unsigned)(tm * tn * splitk);
@@ -439,8 +467,8 @@ void eri_gemm_mixed(hipStream_t s, int prec, const void* P, const void* A, const
       (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, CC::LDS);     \
       attr_set = true;                                                                                       \
     }                                                                                                        \
-    kern<<<grid, CC::NT, CC::LDS, s>>>((const TT*)P, (const TT*)A, invP, invA, C, K, nslice, nseg, patP, patA, \
-                                       ldc, tm, bias, h16_fold());                                           \
+    kern<<<grid, CC::NT, CC::LDS, s>>>((const TT*)P, ldP, kp, (const TT*)A, ldA, (long)K, invP, invA, C, K,   \
+                                       nseg, patP, patA, ldc, tm, bias, colmap, h16_fold());                 \
   } while (0)
 #define H_CFGS(TT, SK)                                                   \
   do {                                                                   \
@@ -471,7 +499,6 @@ void eri_gemm_mixed_grouped(hipStream_t s, int prec, const H16Item* items, int n
                             bool atomic) {
   unsigned patP, patA;
   mixed_patterns(prec, &patP, &patA);
-  const int nslice = mixed_slices(prec);
 #define G_LAUNCH(TT, B0, B1, W0, W1, ST, MB, F64)                                                             \
   do {                                                                                                       \
     typedef H16Cfg<B0, B1, W0, W1, ST, MB, F64> CC;                                                          \
@@ -484,9 +511,9 @@ void eri_gemm_mixed_grouped(hipStream_t s, int prec, const H16Item* items, int n
       attr_set = true;                                                                                       \
     }                                                                                                        \
     if (atomic)                                                                                              \
-      k_at<<<grid, CC::NT, CC::LDS, s>>>(items, nitems, nslice, patP, patA, h16_fold());                     \
+      k_at<<<grid, CC::NT, CC::LDS, s>>>(items, nitems, patP, patA, h16_fold());                             \
     else                                                                                                     \
-      k_st<<<grid, CC::NT, CC::LDS, s>>>(items, nitems, nslice, patP, patA, h16_fold());                     \
+      k_st<<<grid, CC::NT, CC::LDS, s>>>(items, nitems, patP, patA, h16_fold());                             \
   } while (0)
 #define G_CFGS(TT)                                                      \
   do {                                                                  \

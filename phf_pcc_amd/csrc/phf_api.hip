@@ -33,6 +33,7 @@
 static thread_local std::string g_create_err;
 static int free_grid(phf_ctx* ctx);
 static int drop_hamiltonian(phf_ctx* ctx);
+static void drop_blocks(phf_ctx* ctx);
 
 int phf_fail(phf_ctx* ctx, int code, const std::string& msg) {
   if (ctx) ctx->err = msg;
@@ -369,6 +370,41 @@ cplx wignerfac(int j, int n, int m, double alpha, double beta, double gama) {
   return std::exp(cplx(0.0, -alpha * n)) * d * std::exp(cplx(0.0, -gama * m));
 }
 
+// Any nonzero A[r][c] outside the spin blocks (row class != column class of
+// the permutation) sets *flag.
+__global__ void blk_check_kernel(const double* __restrict__ A, long n2, long lda, const signed char* __restrict__ rcls,
+                                 const signed char* __restrict__ ccls, int* __restrict__ flag) {
+  const long total = n2 * n2;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const long r = i / n2, c = i % n2;
+    if (rcls[r] != ccls[c] && A[r * lda + c] != 0.0) *flag = 1;
+  }
+}
+
+// dst[r'][c'] = A[rowmap[r']][colmap[c']] (0 where either map is -1): block
+// rows x k of the permuted A.
+__global__ void blk_gather_kernel(const double* __restrict__ A, long lda, const int* __restrict__ rowmap,
+                                  const int* __restrict__ colmap, int rows, int k, double* __restrict__ dst) {
+  const long total = (long)rows * k;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const int r = (int)(i / k), c = (int)(i % k);
+    const int nr = rowmap[r], nc = colmap[c];
+    dst[i] = (nr >= 0 && nc >= 0) ? A[(long)nr * lda + nc] : 0.0;
+  }
+}
+
+// dst[row][c'] = src[row][cmap[c']] (0 for cmap = -1): density rows in the
+// permuted contraction order.
+__global__ void gather_cols_kernel(const double* __restrict__ src, long lds, const int* __restrict__ cmap, int kout,
+                                   long rows, double* __restrict__ dst) {
+  const long total = rows * kout;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const long r = i / kout;
+    const int c = (int)(i % kout), nc = cmap[c];
+    dst[i] = nc >= 0 ? src[r * lds + nc] : 0.0;
+  }
+}
+
 }  // namespace
 
 void fill_zero(hipStream_t s, double* p, long n) {
@@ -434,6 +470,7 @@ extern "C" int phf_ctx_destroy(phf_ctx* ctx) {
     if (ctx->dAm[q]) (void)hipFree(ctx->dAm[q]);
     if (ctx->dAinv[q]) (void)hipFree(ctx->dAinv[q]);
   }
+  drop_blocks(ctx);
   if (ctx->ws) (void)hipFree(ctx->ws);
   if (ctx->small) (void)hipFree(ctx->small);
   for (double* q : {ctx->dGr1p, ctx->dGr2, ctx->dGrpgref, ctx->dGw})
@@ -499,6 +536,7 @@ extern "C" int phf_set_hamiltonian(phf_ctx* ctx, const double* hone, const doubl
   HIPCK(hipMemcpyAsync(&hflag, dflag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   HIPCK(hipStreamSynchronize(ctx->stream));
   ctx->cplx = hflag != 0 || (flags & 1);
+  ctx->ham_flags = flags;
   ctx->nso = nso;
   ctx->kpad = (int)round_up(n2, 256);
   ctx->nrpad = (int)round_up(n2, 256);
@@ -547,13 +585,175 @@ static int ensure_mixed_a(phf_ctx* ctx, int prec) {
   return PHF_OK;
 }
 
+// ---------------------------------------------------------------------------
+// spin-block structure of A
+// ---------------------------------------------------------------------------
+
+std::vector<int> ref_spins(const double* ref, int nso) {
+  const int n = nso / 2;
+  std::vector<int> sp(nso, -1);
+  if (nso % 2) return {};
+  for (int p = 0; p < nso; ++p) {
+    bool a_zero = true, b_zero = true;
+    for (int r = 0; r < nso; ++r) {
+      const double re = ref[2 * ((long)r * nso + p)], im = ref[2 * ((long)r * nso + p) + 1];
+      if (re != 0.0 || im != 0.0) (r < n ? a_zero : b_zero) = false;
+    }
+    if (b_zero && !a_zero) sp[p] = 0;
+    else if (a_zero && !b_zero) sp[p] = 1;
+    else return {};
+  }
+  return sp;
+}
+
+static void drop_blocks(phf_ctx* ctx) {
+  for (int q = 0; q < 4; ++q) {
+    if (ctx->dAbm[q]) (void)hipFree(ctx->dAbm[q]);
+    if (ctx->dAbinv[q]) (void)hipFree(ctx->dAbinv[q]);
+    ctx->dAbm[q] = nullptr;
+    ctx->dAbinv[q] = nullptr;
+  }
+  if (ctx->dAb) (void)hipFree(ctx->dAb);
+  if (ctx->d_rowmap) (void)hipFree(ctx->d_rowmap);
+  if (ctx->d_colperm) (void)hipFree(ctx->d_colperm);
+  ctx->dAb = nullptr;
+  ctx->d_rowmap = nullptr;
+  ctx->d_colperm = nullptr;
+  ctx->blocked = false;
+  ctx->blk_spin.clear();
+}
+
+int phf_build_blocks(phf_ctx* ctx, const std::vector<int>& spin, bool release_dense) {
+  if (!ctx->dA || ctx->cplx || (int)spin.size() != ctx->nso) return PHF_OK;
+  if (ctx->blocked && ctx->blk_spin == spin) return PHF_OK;
+  HIPCK(hipStreamSynchronize(ctx->stream));
+  drop_blocks(ctx);
+  const int nso = ctx->nso;
+  const long n2 = (long)nso * nso;
+  // pair classes: rows (i,k): 0 if s_i == s_k, 1 for (alpha, beta), 2 for (beta, alpha);
+  // columns (j,l): 0 if s_j == s_l, 1 for (beta, alpha), 2 for (alpha, beta)
+  std::vector<signed char> rcls(n2), ccls(n2);
+  std::vector<int> rows[3], cols[3];
+  for (int i = 0; i < nso; ++i)
+    for (int k = 0; k < nso; ++k) {
+      const int r = i * nso + k;
+      const int c = (spin[i] == spin[k]) ? 0 : (spin[i] == 0 ? 1 : 2);
+      rcls[r] = (signed char)c;
+      rows[c].push_back(r);
+      const int cc = (spin[i] == spin[k]) ? 0 : (spin[i] == 1 ? 1 : 2);
+      ccls[r] = (signed char)cc;
+      cols[cc].push_back(r);
+    }
+  for (int b = 0; b < 3; ++b)
+    if (rows[b].size() != cols[b].size()) return PHF_OK;
+  hipStream_t s = ctx->stream;
+  signed char *drc = nullptr, *dcc = nullptr;
+  int* dflag = nullptr;
+  HIPCK(hipMalloc(&drc, n2));
+  HIPCK(hipMalloc(&dcc, n2));
+  HIPCK(hipMalloc(&dflag, sizeof(int)));
+  HIPCK(hipMemcpyAsync(drc, rcls.data(), n2, hipMemcpyHostToDevice, s));
+  HIPCK(hipMemcpyAsync(dcc, ccls.data(), n2, hipMemcpyHostToDevice, s));
+  HIPCK(hipMemsetAsync(dflag, 0, sizeof(int), s));
+  blk_check_kernel<<<8192, 256, 0, s>>>(ctx->dA, n2, ctx->kt, drc, dcc, dflag);
+  int flag = 0;
+  HIPCK(hipMemcpyAsync(&flag, dflag, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCK(hipStreamSynchronize(s));
+  HIPCK(hipFree(drc));
+  HIPCK(hipFree(dcc));
+  HIPCK(hipFree(dflag));
+  if (flag) return PHF_OK;   // not spin-block structured: stay dense
+  std::vector<int> rowmap, colperm;
+  long aoff = 0;
+  int kbeg = 0;
+  for (int b = 0; b < 3; ++b) {
+    const int cnt = (int)rows[b].size();
+    const int pr = (int)round_up(std::max(cnt, 1), 256);
+    ctx->blk_rows[b] = pr;
+    ctx->blk_k[b] = pr;
+    ctx->blk_roff[b] = (int)rowmap.size();
+    ctx->blk_kbeg[b] = kbeg;
+    ctx->blk_aoff[b] = aoff;
+    for (int q = 0; q < pr; ++q) {
+      rowmap.push_back(q < cnt ? rows[b][q] : -1);
+      colperm.push_back(q < cnt ? cols[b][q] : -1);
+    }
+    aoff += (long)pr * pr;
+    kbeg += pr;
+  }
+  ctx->ktp = kbeg;
+  HIPCK(hipMalloc(&ctx->d_rowmap, sizeof(int) * rowmap.size()));
+  HIPCK(hipMalloc(&ctx->d_colperm, sizeof(int) * colperm.size()));
+  HIPCK(hipMemcpyAsync(ctx->d_rowmap, rowmap.data(), sizeof(int) * rowmap.size(), hipMemcpyHostToDevice, s));
+  HIPCK(hipMemcpyAsync(ctx->d_colperm, colperm.data(), sizeof(int) * colperm.size(), hipMemcpyHostToDevice, s));
+  HIPCK(hipMalloc(&ctx->dAb, sizeof(double) * aoff));
+  for (int b = 0; b < 3; ++b)
+    blk_gather_kernel<<<8192, 256, 0, s>>>(ctx->dA, ctx->kt, ctx->d_rowmap + ctx->blk_roff[b],
+                                           ctx->d_colperm + ctx->blk_kbeg[b], ctx->blk_rows[b], ctx->blk_k[b],
+                                           ctx->dAb + ctx->blk_aoff[b]);
+  HIPCK(hipGetLastError());
+  HIPCK(hipStreamSynchronize(s));
+  ctx->blocked = true;
+  ctx->blk_spin = spin;
+  ctx->tuned.clear();
+  if (release_dense) {
+    HIPCK(hipFree(ctx->dA));
+    ctx->dA = nullptr;
+    for (int q = 0; q < 4; ++q) {
+      if (ctx->dAm[q]) { HIPCK(hipFree(ctx->dAm[q])); ctx->dAm[q] = nullptr; }
+      if (ctx->dAinv[q]) { HIPCK(hipFree(ctx->dAinv[q])); ctx->dAinv[q] = nullptr; }
+    }
+  }
+  return PHF_OK;
+}
+
+// Reduced-precision copy of the spin blocks for mode prec.
+static int ensure_mixed_ab(phf_ctx* ctx, int prec) {
+  if (ctx->dAbm[prec]) return PHF_OK;
+  if (!ctx->dAb) return fail(ctx, PHF_ERR_UNSUPPORTED, "the spin blocks are stored for another precision only");
+  const long elems = ctx->blk_aoff[2] + (long)ctx->blk_rows[2] * ctx->blk_k[2];
+  const int nrows = ctx->blk_roff[2] + ctx->blk_rows[2];
+  const int ns = mixed_slices(prec);
+  HIPCK(hipMalloc(&ctx->dAbm[prec], elems * ns * (prec == 1 ? sizeof(float) : 2)));
+  if (prec == 1) {
+    convert_f32(ctx->stream, ctx->dAb, (float*)ctx->dAbm[prec], elems);
+  } else {
+    HIPCK(hipMalloc(&ctx->dAbinv[prec], sizeof(double) * nrows));
+    for (int b = 0; b < 3; ++b)
+      split_operand(ctx->stream, prec, ctx->dAb + ctx->blk_aoff[b], ctx->blk_k[b], ctx->blk_rows[b], ctx->blk_k[b],
+                    (char*)ctx->dAbm[prec] + 2 * ns * ctx->blk_aoff[b], ctx->dAbinv[prec] + ctx->blk_roff[b]);
+  }
+  HIPCK(hipGetLastError());
+  return PHF_OK;
+}
+
+extern "C" int phf_set_blocked(phf_ctx* ctx, int on) {
+  if (!ctx) return PHF_ERR_ARG;
+  if (!on && !ctx->dA && ctx->blocked)
+    return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_set_blocked: the dense A was released");
+  ctx->use_blocks = on != 0;
+  ctx->r_valid = false;
+  return PHF_OK;
+}
+
+extern "C" int phf_blocked(const phf_ctx* ctx) { return (ctx && ctx->blocked && ctx->use_blocks) ? 1 : 0; }
+
 // Keep only the reduced-precision copy of A for mode prec (1..3) and release
 // the FP64 A: the Hamiltonian then serves that mode only.
 int phf_keep_only_mixed(phf_ctx* ctx, int prec) {
   if (!ctx->has_ham) return fail(ctx, PHF_ERR_NO_HAMILTONIAN, "no Hamiltonian set");
   if (prec < PHF_PREC_FP32 || prec > PHF_PREC_F16X2) return fail(ctx, PHF_ERR_ARG, "not a reduced-precision mode");
   if (ctx->a_only && ctx->a_only != prec) return fail(ctx, PHF_ERR_ARG, "A already reduced to another mode");
-  {
+  if (ctx->blocked) {
+    int rc = ensure_mixed_ab(ctx, prec);
+    if (rc) return rc;
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    if (ctx->dAb) {
+      HIPCK(hipFree(ctx->dAb));
+      ctx->dAb = nullptr;
+    }
+    ctx->use_blocks = true;
+  } else {
     int rc = ensure_mixed_a(ctx, prec);
     if (rc) return rc;
   }
@@ -562,6 +762,11 @@ int phf_keep_only_mixed(phf_ctx* ctx, int prec) {
     HIPCK(hipFree(ctx->dA));
     ctx->dA = nullptr;
   }
+  for (int q = 0; q < 4; ++q)
+    if (q != prec) {
+      if (ctx->dAm[q]) { HIPCK(hipFree(ctx->dAm[q])); ctx->dAm[q] = nullptr; }
+      if (ctx->dAinv[q]) { HIPCK(hipFree(ctx->dAinv[q])); ctx->dAinv[q] = nullptr; }
+    }
   ctx->a_only = prec;
   ctx->tuned.clear();
   return PHF_OK;
@@ -579,6 +784,8 @@ static int drop_hamiltonian(phf_ctx* ctx) {
     if (ctx->dAinv[q]) { HIPCK(hipFree(ctx->dAinv[q])); ctx->dAinv[q] = nullptr; }
   }
   if (ctx->dA) { HIPCK(hipFree(ctx->dA)); ctx->dA = nullptr; }
+  drop_blocks(ctx);
+  ctx->use_blocks = true;
   ctx->a_only = 0;
   if (ctx->dH1) { HIPCK(hipFree(ctx->dH1)); ctx->dH1 = nullptr; }
   if (ctx->dHb) { HIPCK(hipFree(ctx->dHb)); ctx->dHb = nullptr; }
@@ -591,7 +798,6 @@ extern "C" int phf_set_hamiltonian_factored(phf_ctx* ctx, const double* h, const
   if (!ctx) return PHF_ERR_ARG;
   if (!h || !b || !ref || n <= 0 || nl <= 0 || 2 * n > 512)
     return fail(ctx, PHF_ERR_ARG, "phf_set_hamiltonian_factored: bad arguments");
-  (void)flags;
   const int nso = 2 * n;
   const long n2 = (long)nso * nso;
   for (long q = 0; q < n2; ++q)
@@ -654,7 +860,7 @@ extern "C" int phf_set_hamiltonian_factored(phf_ctx* ctx, const double* h, const
   ctx->kt = (int)nrpad;
   HIPCK(hipMalloc(&ctx->dA, sizeof(double) * nrpad * nrpad));
   // A1[(i,k),(j,l)] = sum_L Bt_L(i,k) Bt_L(j,l) on the FP64 matrix cores, then the exchange term
-  eri_gemm_f64(s, dpk, dpk, ctx->dA, (int)nrpad, (int)nrpad, lpad, nrpad, 128, nullptr);
+  eri_gemm_f64(s, dpk, lpad, dpk, ctx->dA, (int)nrpad, (int)nrpad, lpad, nrpad, 128, nullptr, nullptr);
   {
     dim3 grid((nso + 31) / 32, (unsigned)(((nso + 31) / 32) * nso), (unsigned)nso);
     antisym_kernel<<<grid, 256, 0, s>>>(ctx->dA, nso, nrpad);
@@ -666,6 +872,11 @@ extern "C" int phf_set_hamiltonian_factored(phf_ctx* ctx, const double* h, const
   if (hflag) {
     (void)drop_hamiltonian(ctx);
     return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_set_hamiltonian_factored: transformed factors are complex");
+  }
+  ctx->ham_flags = flags;
+  if (!(flags & PHF_HAM_DENSE)) {
+    int rc = phf_build_blocks(ctx, ref_spins(ref, nso), (flags & PHF_HAM_BLOCKS_ONLY) != 0);
+    if (rc) return rc;
   }
   // one-body copies
   HIPCK(hipMalloc(&ctx->dH1, sizeof(double) * 2 * n2));
@@ -786,6 +997,11 @@ extern "C" int phf_set_grid(phf_ctx* ctx, const double* ref, const int* ngrid, i
                 0, zout_colmajor(ctx->dGrpgref, nso, 2 * n2), npg);
   HIPCK(hipGetLastError());
   HIPCK(hipStreamSynchronize(s));
+  // spin-block structure of a dense Hamiltonian, now that Ref is known
+  if (ctx->dA && !ctx->cplx && !(ctx->ham_flags & PHF_HAM_DENSE)) {
+    int rc = phf_build_blocks(ctx, ref_spins(ref, nso), false);
+    if (rc) return rc;
+  }
   ctx->g_nl = nl; ctx->g_ny = ny; ctx->g_npg = npg; ctx->g_nci = nci; ctx->g_ncisp = ncisp;
   ctx->g_ncipg = ncipg; ctx->g_qj = qj; ctx->g_sp = sp; ctx->g_G = G; ctx->g_nso = nso;
   ctx->has_grid = true;
@@ -884,7 +1100,8 @@ int eval_plan(phf_ctx* ctx, int nocc, int precision, int cmplxconj, EvalBufs& b)
     return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: nocc above the LDS inverse limit");
   if (precision < PHF_PREC_FP64 || precision > PHF_PREC_F16X2)
     return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: unknown precision");
-  if (precision == PHF_PREC_FP64 && !ctx->dA)
+  const bool blk = ctx->blocked && ctx->use_blocks;
+  if (precision == PHF_PREC_FP64 && !(blk ? ctx->dAb != nullptr : ctx->dA != nullptr))
     return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: the Hamiltonian is stored for a reduced precision only");
   const int npg = ctx->g_npg, ny = ctx->g_ny, nci = ctx->g_nci;
   if (cmplxconj == 2 && (nci % 2 != 0 || 2 * ctx->g_ncisp * ctx->g_ncipg > nci))
@@ -900,6 +1117,9 @@ int eval_plan(phf_ctx* ctx, int nocc, int precision, int cmplxconj, EvalBufs& b)
   b.prec = precision;
   b.G = G;
   b.V = V;
+  b.blk = blk;
+  b.kP = blk ? ctx->ktp : kt;
+  const long kP = b.kP;
   // rows are padded for the tallest tile any candidate uses
   b.bm = (precision == PHF_PREC_FP64) ? eri_gemm_f64_bm((int)(2 * Gv)) : 256;
   b.Mp = round_up(2 * Gv, b.bm);
@@ -932,10 +1152,12 @@ int eval_plan(phf_ctx* ctx, int nocc, int precision, int cmplxconj, EvalBufs& b)
     o.HS = a.take<double>(4 * (size_t)nmn);
     o.Grad = a.take<double>(2 * (size_t)nmn * n2);
     o.Rdm = a.take<double>(2 * (size_t)nmn * n2);
-    // reduced-precision density rows and their inverse scales
-    const size_t pm = (precision == PHF_PREC_FP64) ? 0 : (size_t)Mp * kt * mixed_slices(precision);
+    // reduced-precision density rows and their inverse scales; FP64 rows in
+    // the permuted (spin-block) contraction order
+    const size_t pm = (precision == PHF_PREC_FP64) ? 0 : (size_t)Mp * kP * mixed_slices(precision);
     o.Pm = a.take<double>(precision == PHF_PREC_FP32 ? (pm + 1) / 2 : (pm + 3) / 4);
     o.Pinv = a.take<double>(precision == PHF_PREC_FP64 ? 0 : (size_t)Mp);
+    o.Pperm = a.take<double>((blk && precision <= PHF_PREC_FP32) ? (size_t)Mp * kP : 0);
   };
   Arena a0{nullptr};
   EvalBufs dummy;
@@ -947,7 +1169,7 @@ int eval_plan(phf_ctx* ctx, int nocc, int precision, int cmplxconj, EvalBufs& b)
   Arena a1{ctx->ws};
   carve(a1, b);
   if (precision != PHF_PREC_FP64) {
-    int rc = ensure_mixed_a(ctx, precision);
+    int rc = blk ? ensure_mixed_ab(ctx, precision) : ensure_mixed_a(ctx, precision);
     if (rc) return rc;
   }
   return PHF_OK;
@@ -1013,38 +1235,78 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
     pack_cplx_kernel<<<grid, 256, 0, s>>>(b.Pt, (int)(V * G), (int)n2, ctx->kpad, kt);
   }
   if (marks) (void)hipEventRecord(marks[2], s);
-  // operand of the reduced-precision ERI GEMM
-  if (b.prec == PHF_PREC_FP32)
-    convert_f32(s, b.Pt, (float*)b.Pm, b.Mp * kt);
-  else if (b.prec != PHF_PREC_FP64)
-    split_operand(s, b.prec, b.Pt, kt, (int)b.Mp, kt, b.Pm, b.Pinv);
+  // operand of the ERI GEMM: FP64 rows as packed, or in the permuted order of
+  // the spin blocks; FP32 roundings; 16-bit slices (gathered when permuted)
+  const int* cmap = b.blk ? ctx->d_colperm : nullptr;
+  if (b.prec == PHF_PREC_FP64) {
+    if (b.blk)
+      gather_cols_kernel<<<8192, 256, 0, s>>>(b.Pt, kt, ctx->d_colperm, (int)b.kP, b.Mp, b.Pperm);
+  } else if (b.prec == PHF_PREC_FP32) {
+    if (b.blk) {
+      gather_cols_kernel<<<8192, 256, 0, s>>>(b.Pt, kt, ctx->d_colperm, (int)b.kP, b.Mp, b.Pperm);
+      convert_f32(s, b.Pperm, (float*)b.Pm, b.Mp * b.kP);
+    } else {
+      convert_f32(s, b.Pt, (float*)b.Pm, b.Mp * kt);
+    }
+  } else {
+    split_operand_map(s, b.prec, b.Pt, kt, (int)b.Mp, kt, cmap, (int)b.kP, b.Pm, b.Pinv);
+  }
 }
 
 int eval_gemm(phf_ctx* ctx, const EvalBufs& b) {
   hipStream_t s = ctx->stream;
   const int kt = ctx->kt, nrpad = ctx->nrpad, precision = b.prec;
   const long G = b.V * b.G;                  // rows of the GEMM: every variant of every point
+  const int nb = b.blk ? 3 : 1;
+  // per block: A rows, contraction length, operand offsets, output column map
+  auto rows_of = [&](int q) { return b.blk ? ctx->blk_rows[q] : nrpad; };
+  auto k_of = [&](int q) { return b.blk ? ctx->blk_k[q] : kt; };
+  auto kbeg_of = [&](int q) { return b.blk ? (long)ctx->blk_kbeg[q] : 0L; };
+  auto aoff_of = [&](int q) { return b.blk ? ctx->blk_aoff[q] : 0L; };
+  auto map_of = [&](int q) { return b.blk ? ctx->d_rowmap + ctx->blk_roff[q] : (const int*)nullptr; };
+  auto roff_of = [&](int q) { return b.blk ? (long)ctx->blk_roff[q] : 0L; };
   if (precision == PHF_PREC_FP64) {
-    eri_gemm_f64(s, b.Pt, ctx->dA, b.Gam, (int)b.Mp, nrpad, kt, nrpad, b.bm, ctx->dHb);
+    const double* P = b.blk ? b.Pperm : b.Pt;
+    const double* A = b.blk ? ctx->dAb : ctx->dA;
+    for (int q = 0; q < nb; ++q)
+      eri_gemm_f64(s, P + kbeg_of(q), b.kP, A + aoff_of(q), b.Gam, (int)b.Mp, rows_of(q), k_of(q), nrpad, b.bm,
+                   ctx->dHb, map_of(q));
   } else if (precision == PHF_PREC_FP32) {
-    eri_gemm_mixed(s, precision, b.Pm, ctx->dAm[precision], nullptr, nullptr, b.Gam, (int)b.Mp, nrpad, kt, nrpad,
-                   128, 1, ctx->dHb);
+    const float* P = (const float*)b.Pm;
+    const float* A = (const float*)(b.blk ? ctx->dAbm[1] : ctx->dAm[1]);
+    for (int q = 0; q < nb; ++q)
+      eri_gemm_mixed(s, precision, P + kbeg_of(q), b.kP, A + aoff_of(q), nullptr, nullptr, b.Gam, (int)b.Mp,
+                     rows_of(q), k_of(q), nrpad, 128, 1, ctx->dHb, map_of(q));
   } else {
-    const int kseg = kt * mixed_segments(precision);
-    auto run = [&](int cfg, int sk) {
+    const int ns = mixed_slices(precision);
+    const char* P = (const char*)b.Pm;
+    const char* A = (const char*)(b.blk ? ctx->dAbm[precision] : ctx->dAm[precision]);
+    const double* invA = b.blk ? ctx->dAbinv[precision] : ctx->dAinv[precision];
+    auto run_block = [&](int q, int cfg, int sk) {
       const int mp = (int)round_up(2 * G, h16_cfg_bm(cfg));
-      if (sk > 1) fill_zero(s, b.Gam, (long)mp * nrpad);
-      eri_gemm_mixed(s, precision, b.Pm, ctx->dAm[precision], b.Pinv, ctx->dAinv[precision], b.Gam, mp, nrpad, kt,
-                     nrpad, cfg, sk, ctx->dHb);
+      eri_gemm_mixed(s, precision, P + 2 * kbeg_of(q), b.kP, A + 2 * ns * aoff_of(q), b.Pinv, invA + roff_of(q),
+                     b.Gam, mp, rows_of(q), k_of(q), nrpad, cfg, sk, ctx->dHb, map_of(q));
     };
-    const auto key = std::make_tuple(precision, 2 * G, nrpad, kseg);
-    auto it = ctx->tuned.find(key);
-    if (it == ctx->tuned.end()) {
-      int rc = tune_h16(ctx, key, 2 * G, nrpad, run);
-      if (rc) return rc;
-      it = ctx->tuned.find(key);
+    std::pair<int, int> choice[3];
+    bool any_split = false;
+    for (int q = 0; q < nb; ++q) {
+      const auto key = std::make_tuple(precision, 2 * G, rows_of(q), k_of(q) * mixed_segments(precision));
+      auto it = ctx->tuned.find(key);
+      if (it == ctx->tuned.end()) {
+        auto run = [&](int cfg, int sk) {
+          if (sk > 1) fill_zero(s, b.Gam, (long)round_up(2 * G, h16_cfg_bm(cfg)) * nrpad);
+          run_block(q, cfg, sk);
+        };
+        int rc = tune_h16(ctx, key, 2 * G, rows_of(q), run);
+        if (rc) return rc;
+        it = ctx->tuned.find(key);
+      }
+      choice[q] = it->second;
+      any_split = any_split || choice[q].second > 1;
     }
-    run(it->second.first, it->second.second);
+    // blocks write disjoint columns of Gamma: one zero fill before all of them
+    if (any_split) fill_zero(s, b.Gam, b.Mp * nrpad);
+    for (int q = 0; q < nb; ++q) run_block(q, choice[q].first, choice[q].second);
   }
   return PHF_OK;
 }

@@ -132,28 +132,48 @@ void drop_graph(phf_batch* bt) {
 // placed so that XCD x (workgroups b with b % 8 == x) takes a contiguous
 // eighth of each round.
 void build_items(phf_batch* bt, int prec, int cfg, int sk, int grid, std::vector<H16Item>& out) {
-  const int bm = h16_cfg_bm(cfg), bn = h16_cfg_bn(cfg), nseg = mixed_segments(prec);
-  std::vector<int> order(bt->inst.size());
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return bt->inst[a]->kt > bt->inst[b]->kt; });
-  std::vector<H16Item> logical;
-  for (int i : order) {
+  const int bm = h16_cfg_bm(cfg), bn = h16_cfg_bn(cfg), nseg = mixed_segments(prec), ns = mixed_slices(prec);
+  // work units: (instance, spin block); the dense A is one unit
+  struct Unit { int inst, blk; long cost; };
+  std::vector<Unit> units;
+  for (size_t i = 0; i < bt->inst.size(); ++i) {
     const phf_ctx* c = bt->inst[i];
-    const EvalBufs& b = bt->bufs[i];
-    const int tm = (int)(round_up(2 * b.G, bm) / bm), tn = c->nrpad / bn;
-    const int part = nseg * (c->kt / 64) / sk;
+    if (bt->bufs[i].blk)
+      for (int q = 0; q < 3; ++q) units.push_back({(int)i, q, (long)c->blk_k[q]});
+    else
+      units.push_back({(int)i, -1, (long)c->kt});
+  }
+  std::stable_sort(units.begin(), units.end(), [](const Unit& a, const Unit& b) { return a.cost > b.cost; });
+  std::vector<H16Item> logical;
+  for (const Unit& u : units) {
+    const phf_ctx* c = bt->inst[u.inst];
+    const EvalBufs& b = bt->bufs[u.inst];
+    const bool blk = u.blk >= 0;
+    const int rows = blk ? c->blk_rows[u.blk] : c->nrpad;
+    const int K = blk ? c->blk_k[u.blk] : c->kt;
+    const long kbeg = blk ? c->blk_kbeg[u.blk] : 0, aoff = blk ? c->blk_aoff[u.blk] : 0;
+    const long roff = blk ? c->blk_roff[u.blk] : 0;
+    const char* Abase = (const char*)(blk ? c->dAbm[prec] : c->dAm[prec]);
+    const double* invA = blk ? c->dAbinv[prec] : c->dAinv[prec];
+    const int tm = (int)(round_up(2 * b.V * b.G, bm) / bm), tn = rows / bn;
+    const int part = nseg * (K / 64) / sk;
     for (int nt = 0; nt < tn; ++nt)
       for (int mt = 0; mt < tm; ++mt)
         for (int ks = 0; ks < sk; ++ks) {
           H16Item it;
-          it.P = b.Pm;
-          it.A = c->dAm[prec];
+          it.P = (const char*)b.Pm + 2 * kbeg;
+          it.A = Abase + 2 * ns * aoff;
           it.invP = b.Pinv;
-          it.invA = c->dAinv[prec];
+          it.invA = invA + roff;
           it.C = b.Gam;
           it.bias = ks == 0 ? c->dHb : nullptr;
+          it.colmap = blk ? c->d_rowmap + roff : nullptr;
+          it.ldP = ns * b.kP;
+          it.sP = b.kP;
+          it.ldA = (long)ns * K;
+          it.sA = K;
           it.ldc = c->nrpad;
-          it.K = c->kt;
+          it.K = K;
           it.mt = mt;
           it.nt = nt;
           it.sbeg = ks * part;
@@ -199,7 +219,7 @@ void launch_grouped(phf_batch* bt, int prec) {
 int tune_grouped(phf_batch* bt, int prec) {
   std::vector<std::pair<int, int>> cands;
   for (int c = 0; c < 2; ++c) {
-    bool fits = true;
+    bool fits = true;   // A rows (dense or per block) are multiples of 256
     for (phf_ctx* x : bt->inst) fits = fits && (x->nrpad % h16_cfg_bn(c) == 0);
     if (!fits) continue;
     for (int sk : {1, 2}) cands.push_back({c, sk});
@@ -341,7 +361,7 @@ extern "C" const char* phf_batch_last_error(const phf_batch* bt) { return bt ? b
 extern "C" int phf_batch_set_instance(phf_batch* bt, int i, const double* h, const double* b, int n, int nl,
                                       const double* ref, int nocc, const int* ngrid, const double* r1, int nr1,
                                       const double* r2, int nr2, const double* rpg, int npg,
-                                      const double* weightsp, const double* weightpg, int storage) {
+                                      const double* weightsp, const double* weightpg, int storage, int flags) {
   if (!bt) return PHF_ERR_ARG;
   if (i < 0 || i >= (int)bt->inst.size()) return bfail(bt, PHF_ERR_ARG, "phf_batch_set_instance: index out of range");
   if (nocc <= 0 || nocc >= 2 * n) return bfail(bt, PHF_ERR_ARG, "phf_batch_set_instance: nocc out of range");
@@ -368,7 +388,7 @@ extern "C" int phf_batch_set_instance(phf_batch* bt, int i, const double* h, con
     return PHF_OK;
   };
   if (rc) return done(rc);
-  rc = phf_set_hamiltonian_factored(c, h, b, n, nl, ref, 0);
+  rc = phf_set_hamiltonian_factored(c, h, b, n, nl, ref, (flags & PHF_HAM_DENSE) ? PHF_HAM_DENSE : PHF_HAM_BLOCKS_ONLY);
   if (rc) return done(rc);
   // J = 0 spin grid: nci = ncisp = ncipg = 1, no Wigner factors
   const double one = 1.0;

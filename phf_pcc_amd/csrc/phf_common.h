@@ -46,8 +46,10 @@ void zgemm_batched(hipStream_t s, int m, int n, int k, double ar, double ai, ZMa
 
 // One output tile (or one part of a split contraction) of a 16-bit ERI GEMM in
 // a grouped launch: rows mt*BM.. of P against rows nt*BN.. of A over the slabs
-// [sbeg, send) (64 elements each) of the segment sequence; K is the slice
-// length of this GEMM's operands, ldc the row stride of C and of the bias planes.
+// [sbeg, send) (64 elements each) of the segment sequence.  P rows are ldP
+// elements apart with their slices sP apart, A rows ldA apart with slices sA
+// apart; K is the contraction length; C rows are ldc apart (also the stride
+// of the bias planes); colmap (optional) sends A row r to C column colmap[r].
 struct H16Item {
   const void* P;
   const void* A;
@@ -55,6 +57,8 @@ struct H16Item {
   const double* invA;
   double* C;
   const double* bias;   // null, or [2][ldc] added to row parity planes (first part only)
+  const int* colmap;
+  long ldP, sP, ldA, sA;
   long ldc;
   int K;
   int mt, nt;

@@ -13,15 +13,18 @@
 void lu_inverse_batched(hipStream_t s, int n, const double* M, long mbs, double* Minv, long ibs,
                         double* det, int* info, int batch);
 int lu_inverse_max_n();
-void eri_gemm_f64(hipStream_t s, const double* P, const double* A, double* C, int Mp, int Nr, int K,
-                  long ldc, int bm, const double* bias);
+void eri_gemm_f64(hipStream_t s, const double* P, long ldp, const double* A, double* C, int Mp, int Nr, int K,
+                  long ldc, int bm, const double* bias, const int* colmap);
 int eri_gemm_f64_bm(int rows);
 int mixed_segments(int prec);
 int mixed_slices(int prec);
 void split_operand(hipStream_t s, int prec, const double* src, long lds, int rows, int kt, void* dst, double* inv);
+void split_operand_map(hipStream_t s, int prec, const double* src, long lds, int rows, int kt, const int* cmap,
+                       int kout, void* dst, double* inv);
 void convert_f32(hipStream_t s, const double* src, float* dst, long n);
-void eri_gemm_mixed(hipStream_t s, int prec, const void* P, const void* A, const double* invP, const double* invA,
-                    double* C, int Mp, int Nr, int K, long ldc, int cfg, int splitk, const double* bias);
+void eri_gemm_mixed(hipStream_t s, int prec, const void* P, long kp, const void* A, const double* invP,
+                    const double* invA, double* C, int Mp, int Nr, int K, long ldc, int cfg, int splitk,
+                    const double* bias, const int* colmap);
 void eri_gemm_mixed_grouped(hipStream_t s, int prec, const H16Item* items, int nitems, int cfg, int grid,
                             bool atomic);
 int h16_grouped_grid(int cfg, int ncu);
@@ -49,6 +52,24 @@ struct phf_ctx_s {
   void* dAm[4] = {nullptr, nullptr, nullptr, nullptr};
   double* dAinv[4] = {nullptr, nullptr, nullptr, nullptr};
   int a_only = 0;            // > 0: only the reduced-precision copy of this mode is stored (dA released)
+  // Spin-block structure (every MO of Ref purely alpha or beta): with the pairs
+  // (i,k) and (j,l) ordered by spin class A is block diagonal, diag(D, X1, X2):
+  // D rows/cols s_i = s_k, X1 rows (alpha, beta) x cols (beta, alpha), X2 the
+  // reverse; all other entries of <ij||kl> vanish.  Blocks are stored in FP64
+  // (dAb, block b at element blk_aoff[b], rows blk_rows[b] x blk_k[b]) and in
+  // the reduced modes (dAbm/dAbinv, slices per block row).
+  int ham_flags = 0;         // flags of the phf_set_hamiltonian* call
+  bool blocked = false;      // blocks built for the current Hamiltonian
+  bool use_blocks = true;    // evaluate through the blocks when built
+  int blk_rows[3] = {0, 0, 0}, blk_k[3] = {0, 0, 0}, blk_kbeg[3] = {0, 0, 0}, blk_roff[3] = {0, 0, 0};
+  long blk_aoff[3] = {0, 0, 0};
+  int ktp = 0;               // permuted contraction length (sum of blk_k)
+  int* d_rowmap = nullptr;   // [sum blk_rows] permuted A row -> natural i*nso + k, -1 padding
+  int* d_colperm = nullptr;  // [ktp] permuted contraction column -> natural j*nso + l, -1 padding
+  double* dAb = nullptr;
+  void* dAbm[4] = {nullptr, nullptr, nullptr, nullptr};
+  double* dAbinv[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<int> blk_spin; // MO spins (0 alpha, 1 beta) the blocks were built for
   // workspace arena
   char* ws = nullptr;
   size_t ws_size = 0;
@@ -108,6 +129,15 @@ void fill_zero(hipStream_t s, double* p, long n);
 int phf_ctx_init(phf_ctx* ctx, int device, hipStream_t shared_stream);
 int phf_release_ws(phf_ctx* ctx);
 int phf_keep_only_mixed(phf_ctx* ctx, int prec);
+// Build the spin blocks of the cached real A for MO spins `spin` (size nso),
+// after checking on the device that every entry outside them is exactly zero;
+// returns PHF_OK without building when the check fails.  release_dense: free
+// the dense A afterwards.
+int phf_build_blocks(phf_ctx* ctx, const std::vector<int>& spin, bool release_dense);
+// MO spins of a real or complex Ref (nso x nso, row-major complex): 0 for a
+// column with exactly zero beta rows, 1 for exactly zero alpha rows; empty if
+// some column mixes spins.
+std::vector<int> ref_spins(const double* ref, int nso);
 
 // Device buffers of one evaluation (carved from the context's workspace).
 struct EvalBufs {
@@ -121,6 +151,9 @@ struct EvalBufs {
   double *Grad = nullptr, *Rdm = nullptr;
   void* Pm = nullptr;
   double* Pinv = nullptr;
+  bool blk = false;           // the GEMM runs over the spin blocks
+  long kP = 0;                // row length (per slice) of the GEMM's density operand
+  double* Pperm = nullptr;    // FP64 density rows in permuted column order (blk, FP64 form)
 };
 
 // Stages of one evaluation on ctx->stream.  eval_plan checks the cached state,

@@ -23,19 +23,23 @@ from .phf import eigen_solver, vec_decomp
 
 class ProjectedHF:
     def __init__(self, nocc: int, ref: np.ndarray, grid: dict, hone=None, htwo=None, factors=None,
-                 precision: str = "fp64", device: int = 0, cmplxconj: int = 1, ctx: _lib.Context | None = None):
+                 precision: str = "fp64", device: int = 0, cmplxconj: int = 1, ctx: _lib.Context | None = None,
+                 structure: bool = True):
         """grid: keys ngrid, nci, ncisp, r1, r2, rpg, weightsp, weightpg, roota, rootb, rooty, qj, sp
         (the buildhsg grid arguments).  Either hone/htwo (MO basis, as the
-        drop-in takes them) or factors=(h, b) with (pq|rs) = sum_L b[L,p,q] b[L,r,s]."""
+        drop-in takes them) or factors=(h, b) with (pq|rs) = sum_L b[L,p,q] b[L,r,s].
+        structure: use the spin-block structure of A when Ref has spin-pure MOs
+        (False: always contract the dense A, the plain form)."""
         self.ctx = ctx or _lib.Context(device)
         self.nocc = int(nocc)
         self.cmplxconj = int(cmplxconj)
         self.precision = precision
         ref = np.ascontiguousarray(ref, dtype=np.complex128)
         if factors is not None:
-            self.ctx.set_hamiltonian_factored(factors[0], factors[1], ref)
+            self.ctx.set_hamiltonian_factored(factors[0], factors[1], ref,
+                                              flags=0 if structure else _lib.PHF_HAM_DENSE)
         else:
-            self.ctx.set_hamiltonian(hone, htwo)
+            self.ctx.set_hamiltonian(hone, htwo, dense=not structure)
         self.nso = ref.shape[0]
         self.grid = grid
         self._set_grid(ref, grid)

@@ -149,3 +149,17 @@ def test_batch_autograd_central_differences(setup):
         fd = (ek(x0 + h * d) - ek(x0 - h * d)) / (2 * h)
         assert abs(w[k] * fd - grad @ d) <= 1e-6 * (1 + abs(w[k] * fd)), k
         off += 2 * sz
+
+
+def test_batch_dense_and_blocked_agree():
+    """The same instances stored dense (plain contraction) and as spin blocks."""
+    from phf_pcc_amd.batch import ProjectedHFBatch
+    inst, probs = _instances()
+    zs = [p.z for p, _ in probs]
+    e_b, g_b, _ = ProjectedHFBatch(inst, storage="fp64").energy_and_gradient(zs, "fp64")
+    e_d, g_d, _ = ProjectedHFBatch(inst, storage="fp64", structure=False).energy_and_gradient(zs, "fp64")
+    assert np.abs(e_b - e_d).max() < 1e-10
+    for a, b in zip(g_b, g_d):
+        assert np.abs(a - b).max() < 1e-10
+    e_h, _, _ = ProjectedHFBatch(inst, storage="f16x2", structure=False).energy_and_gradient(zs)
+    assert np.abs(e_h - e_d).max() <= 1e-6

@@ -128,3 +128,55 @@ def test_resident_gradient_projected_blocks(nsite, seed, j, ngrid, npg, cmplxcon
     if cmplxconj == 0:
         g2 = g2.real.astype(complex)
     assert np.abs(g2 - g).max() < 1e-11
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("fp32", 1e-6), ("bf16x3", 1e-6), ("f16x2", 1e-6)])
+def test_spin_blocks_match_dense(prob, prec, tol):
+    """Spin-pure Ref: A is stored as its three spin blocks; evaluating through
+    them equals the dense contraction (FP64: summation order only)."""
+    from phf_pcc_amd.model import ProjectedHF
+    p, ga = prob
+    m = ProjectedHF(p.nocc, p.ref, ga, factors=(p.h_spatial, p.b), precision=prec)
+    assert m.ctx.blocked()
+    e1, g1 = m.energy_and_gradient(p.z)
+    m.ctx.set_blocked(False)
+    assert not m.ctx.blocked()
+    e0, g0 = m.energy_and_gradient(p.z)
+    m.ctx.set_blocked(True)
+    assert abs(e1 - e0) <= tol
+    assert np.abs(g1 - g0).max() <= 10 * tol
+    if prec == "fp64":
+        x = np.concatenate([p.z.real.ravel(), p.z.imag.ravel()])
+        e_ref, _ = O.eandg(x, p.hone, p.htwo, p.ref.astype(complex), p.nocc, ga)
+        assert abs(e1 - e_ref) < 1e-10
+
+
+def test_structure_detection():
+    """Blocks for a spin-pure Ref through the dense drop-in (detected at
+    phf_set_grid from Ref and the zero pattern of A); none for a Ref that
+    mixes spins; PHF_HAM_BLOCKS_ONLY releases the dense A."""
+    from phf_pcc_amd import _lib
+    from phf_pcc_amd.model import ProjectedHF
+    from tests._cases import hubbard_general_ref
+    case = hubbard_general_ref(4, 4.0, 13, zscale=0.3)
+    ga = grid_args(4, (14, 8), 0)
+    m = ProjectedHF(case["nocc"], case["ref"], ga, hone=case["hone"], htwo=case["htwo"])
+    assert m.ctx.blocked()
+    e_b, _ = m.energy_and_gradient(case["z"])
+    x = np.concatenate([case["z"].real.ravel(), case["z"].imag.ravel()])
+    e_ref, _ = O.eandg(x, case["hone"], case["htwo"], case["ref"], case["nocc"], ga)
+    assert abs(e_b - e_ref) < 1e-10
+    mix = hubbard_general_ref(4, 4.0, 13, zscale=0.3, complex_ref=True)
+    m2 = ProjectedHF(mix["nocc"], mix["ref"], ga, hone=mix["hone"], htwo=mix["htwo"])
+    assert not m2.ctx.blocked()
+    p = synthetic_problem(8, 8, seed=3)
+    ga8 = grid_args(8, (14, 8), 0)
+    ctx = _lib.Context(0)
+    ctx.set_hamiltonian_factored(p.h_spatial, p.b, p.ref, flags=_lib.PHF_HAM_BLOCKS_ONLY)
+    assert ctx.blocked()
+    with pytest.raises(_lib.PhfError):
+        ctx.set_blocked(False)
+    m3 = ProjectedHF(p.nocc, p.ref, ga8, factors=(p.h_spatial, p.b))
+    dense = ProjectedHF(p.nocc, p.ref, ga8, factors=(p.h_spatial, p.b), structure=False)
+    assert m3.ctx.blocked() and not dense.ctx.blocked()
+    assert abs(m3.energy_and_gradient(p.z)[0] - dense.energy_and_gradient(p.z)[0]) < 1e-10
