@@ -349,7 +349,9 @@ def cpu_blas_gemm(prob, G):
 
 def load_traffic(kernel, n):
     """HBM bytes per launch of `kernel` at this workload from the committed
-    rocprofv3 counter summary (profiles/traffic.json), if present."""
+    rocprofv3 counter summary (profiles/traffic.json), if present.  The key
+    names the kernel and the workload (n and, for the spin-block GEMMs, the
+    block), e.g. 'eri_gemm_h16_kernel<f16>@n64-blocks'."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
@@ -522,7 +524,7 @@ def main():
         log(f"[bench] cpu BLAS ERI GEMM variant {cpu['blas_gemm_variant']['value']:.1f} grid pts/s "
             f"({cpu['blas_gemm_variant']['sample']})")
 
-    traffic = load_traffic(roof["kernel"], args.n)
+    traffic = load_traffic(roof["kernel"], f"{args.n}-blocks" if blocked else args.n)
     if traffic is not None:
         roof["traffic"] = traffic["bytes_per_launch"]
         roof["traffic_source"] = traffic["source"]
