@@ -32,8 +32,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# MI355X dense matrix-core peaks (AMD public figures)
-PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15, "f16x2": 2.5e15}
+# MI355X dense matrix-core peaks (AMD public figures; int8 in OP/s)
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15, "f16x2": 2.5e15, "i8x4": 5.0e15}
 
 
 def log(msg):
@@ -149,7 +149,7 @@ def run_batch(args):
     ns_ = ns.astype(np.float64)
     flops_blk = float(np.sum(4.0 * ((2 * ns_ ** 2) ** 2 + 2 * (ns_ ** 2) ** 2) * gs))
     flops_best = flops_blk if best["structure"] else flops
-    nprod = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3}[bprec]
+    nprod = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3, "i8x4": 10}[bprec]
     t_gemm = best["eager_gemm_ms"] * 1e-3
     roof = {"bound": "mfma", "achieved": nprod * flops_best / t_gemm / 1e12, "peak": PEAK[bprec] / 1e12,
             "unit": "TFLOP/s", "frac": nprod * flops_best / t_gemm / PEAK[bprec], "traffic": None,
@@ -182,7 +182,8 @@ def run_batch(args):
 
 def _form_dtype(prec):
     return {"fp64": "f64", "fp32": "f32 (FP64 accumulate)", "bf16x3": "bf16x3 split (FP64 accumulate)",
-            "f16x2": "f16x2 scaled split (FP64 accumulate)"}[prec]
+            "f16x2": "f16x2 scaled split (FP64 accumulate)",
+            "i8x4": "int8x4 scaled split (exact int32, FP64 accumulate)"}[prec]
 
 
 def run_large(args):
@@ -227,7 +228,7 @@ def run_large(args):
     out_forms = {}
     for f, r in res.items():
         flops = flops_blk if r["blocked"] else flops_dense
-        nprod = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3}[r["prec"]]
+        nprod = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3, "i8x4": 10}[r["prec"]]
         out_forms[f] = {"grid_pts_per_s": r["value"], "ms_per_step": r["ms"], "eri_gemm_ms": r["gemm_ms"],
                         "device_ms": r["dev_ms"], "E": r["E"], "abs_dE_vs_fp64": abs(r["E"] - base["E"]),
                         "max_abs_dG_vs_fp64": float(np.abs(r["G"] - base["G"]).max()),
@@ -238,7 +239,7 @@ def run_large(args):
     ok = [f for f in res if out_forms[f]["abs_dE_vs_fp64"] <= 1e-6]
     best = max(ok, key=lambda f: res[f]["value"])
     bprec = res[best]["prec"]
-    nprod = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3}[bprec]
+    nprod = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3, "i8x4": 10}[bprec]
     flops = flops_blk if res[best]["blocked"] else flops_dense
     t_gemm = res[best]["gemm_ms"] * 1e-3
     out = {"metric": "projected E+grad evals/s (grid pts/s) vs plain FP64 GPU at 1e-6 Eh; % MFMA peak",
@@ -488,13 +489,14 @@ def main():
     flops_dense = 4.0 * nso ** 4 * G
     n_sp = nso // 2
     flops = 4.0 * ((2 * n_sp ** 2) ** 2 + 2 * (n_sp ** 2) ** 2) * G if blocked else flops_dense
-    nprod = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3}[best["prec"]]
+    nprod = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3, "i8x4": 10}[best["prec"]]
     peak = PEAK[best["prec"]]
     t_gemm = best["gemm_ms"] * 1e-3
     roof = {"bound": "mfma", "achieved": nprod * flops / t_gemm / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
             "frac": nprod * flops / t_gemm / peak, "traffic": None,
             "kernel": {"fp64": "eri_gemm_f64_kernel", "fp32": "eri_gemm_f32_kernel",
-                       "bf16x3": "eri_gemm_h16_kernel<bf16>", "f16x2": "eri_gemm_h16_kernel<f16>"}[best["prec"]],
+                       "bf16x3": "eri_gemm_h16_kernel<bf16>", "f16x2": "eri_gemm_h16_kernel<f16>",
+                       "i8x4": "eri_gemm_h16_kernel<i8>"}[best["prec"]],
             "flops_per_launch": nprod * flops, "algorithmic_flops_per_launch": flops, "spin_blocks": blocked,
             "products": nprod, "fp64_equiv_tflops": flops / t_gemm / 1e12,
             "plain_fp64_dense_frac": flops_dense / (plain["gemm_ms"] * 1e-3) / PEAK["fp64"]}
@@ -530,7 +532,7 @@ def main():
         roof["traffic_source"] = traffic["source"]
 
     def form_out(r, fl):
-        nprod_ = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3}[r["prec"]]
+        nprod_ = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3, "i8x4": 10}[r["prec"]]
         return {"grid_pts_per_s": r["value"], "evals_per_s": r["value"] / G, "ms_per_step": r["ms_step"],
                 "eri_gemm_ms": r["gemm_ms"], "device_ms": r["dev_ms"], "abs_dE_vs_plain_fp64": r.get("dE", 0.0),
                 "max_abs_dG_vs_plain_fp64": r.get("dG", 0.0), "speedup_vs_plain_fp64": r["value"] / base["value"],
@@ -540,8 +542,7 @@ def main():
     out = {"metric": "projected E+grad evals/s (grid pts/s) vs plain FP64 GPU at 1e-6 Eh; % MFMA peak",
            "value": value, "unit": "grid pts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": {"fp64": "f64", "fp32": "f32 (FP64 accumulate)", "bf16x3": "bf16x3 split (FP64 accumulate)",
-                     "f16x2": "f16x2 scaled split (FP64 accumulate)"}[best["prec"]],
+           "dtype": _form_dtype(best["prec"]),
            "data": "synthetic",
            "config": {"workload": f"projected E+grad (EandG), synthetic n={args.n} (NSO={nso}), N={nocc}, "
                                   f"L={3*args.n}, grid 50x16={G}, complex128 host interface",

@@ -60,12 +60,20 @@ enum {
  *   PHF_PREC_FP32    one product of the FP32 roundings of A and rho
  *   PHF_PREC_BF16X3  A, rho split into three BF16 terms; six products
  *   PHF_PREC_F16X2   A, rho scaled by powers of two per row and split into
- *                    two FP16 terms; three products (hi*hi + hi*lo + lo*hi) */
+ *                    two FP16 terms; three products (hi*hi + hi*lo + lo*hi)
+ *   PHF_PREC_I8X4    A, rho scaled by powers of two per row and split into
+ *                    four int8 terms (28 bits); the ten products with
+ *                    i + j <= 3 on the int8 matrix cores with exact int32
+ *                    accumulation, folded into FP64 with weights 2^-7(i+j).
+ *                    No rounding inside the matrix cores: the 16-bit cores
+ *                    truncate their sums, which biases BF16X3/F16X2 at large
+ *                    NSO. */
 enum {
   PHF_PREC_FP64 = 0,
   PHF_PREC_FP32 = 1,
   PHF_PREC_BF16X3 = 2,
-  PHF_PREC_F16X2 = 3
+  PHF_PREC_F16X2 = 3,
+  PHF_PREC_I8X4 = 4
 };
 
 /* Context: one GPU, one HIP stream, a cached device Hamiltonian and a

@@ -22,7 +22,7 @@ PHF_ERR_NO_HAMILTONIAN = 5
 PHF_ERR_UNSUPPORTED = 6
 PHF_ERR_NOMEM = 7
 
-PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2, "f16x2": 3}
+PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2, "f16x2": 3, "i8x4": 4}
 PHF_HAM_COMPLEX = 1
 PHF_HAM_DENSE = 2
 PHF_HAM_BLOCKS_ONLY = 4

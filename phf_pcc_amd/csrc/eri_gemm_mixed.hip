@@ -21,17 +21,27 @@
 
 typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 b8_t __attribute__((ext_vector_type(8)));
+typedef int i4_t __attribute__((ext_vector_type(4)));
 
 namespace {
 
+// One 16-byte fragment per lane and one MFMA per 64-byte k-step: 16x16x32 for
+// the 16-bit types (FP32 accumulation), 16x16x64 for int8 (exact int32).
 template <typename T> struct MfmaH;
 template <> struct MfmaH<_Float16> {
   typedef h8_t v8;
-  __device__ static f4_t mma(v8 a, v8 b, f4_t c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+  typedef f4_t acc_t;
+  __device__ static acc_t mma(v8 a, v8 b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
 };
 template <> struct MfmaH<__bf16> {
   typedef b8_t v8;
-  __device__ static f4_t mma(v8 a, v8 b, f4_t c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+  typedef f4_t acc_t;
+  __device__ static acc_t mma(v8 a, v8 b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+};
+template <> struct MfmaH<signed char> {
+  typedef i4_t v8;
+  typedef i4_t acc_t;
+  __device__ static acc_t mma(v8 a, v8 b, acc_t c) { return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0); }
 };
 
 typedef __attribute__((address_space(3))) void lds_void_t;
@@ -78,15 +88,16 @@ __device__ __forceinline__ void h16_tile(const T* __restrict__ P, long ldP, long
                                          long ldA, long sA, const double* __restrict__ invP,
                                          const double* __restrict__ invA, double* __restrict__ C,
                                          const double* __restrict__ bias, const int* __restrict__ colmap,
-                                         long ldc, int K, unsigned patP, unsigned patA, int mt, int nt,
-                                         int sbeg, int send, bool atomic, int fold, unsigned char* smem) {
-  constexpr int BK = 64;
+                                         long ldc, int K, unsigned patP, unsigned patA, unsigned patW, int mt,
+                                         int nt, int sbeg, int send, bool atomic, int fold, unsigned char* smem) {
+  constexpr int BK = 128 / (int)sizeof(T);    // elements per slab: 128 bytes per slab row
   constexpr int STAGES = Cfg::STAGES, NW = Cfg::NW, ROWS = Cfg::ROWS, STAGE_BYTES = Cfg::STAGE_BYTES;
   constexpr int FM = WM / 16, FN = WN / 16, WGN = BN / WN;
   constexpr int BLOCKS = ROWS / 8;            // 1 KB blocks per slab
   constexpr int BPW = BLOCKS / NW;            // blocks (load instructions) per wave per slab
   static_assert(BPW * NW == BLOCKS && BPW < 64, "blocks per wave");
   typedef typename MfmaH<T>::v8 v8;
+  typedef typename MfmaH<T>::acc_t acc_t;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WGN, wn = wave % WGN;
@@ -99,7 +110,8 @@ __device__ __forceinline__ void h16_tile(const T* __restrict__ P, long ldP, long
     const int b = wave * BPW + q;
     const int r = 8 * b + (lane >> 3);
     const int c = (lane & 7) ^ ((r >> 1) & 7);
-    src[q] = ((r < BM) ? P + ((long)mt * BM + r) * ldP : A + ((long)nt * BN + (r - BM)) * ldA) + c * 8;
+    src[q] = ((r < BM) ? P + ((long)mt * BM + r) * ldP : A + ((long)nt * BN + (r - BM)) * ldA) +
+             c * (16 / (int)sizeof(T));   // chunk c = 16 bytes
   }
   int iseg = sbeg / nsl, ioff = sbeg % nsl;   // segment / slab of the next slab to issue
   auto issue = [&](int slab) {
@@ -119,15 +131,16 @@ __device__ __forceinline__ void h16_tile(const T* __restrict__ P, long ldP, long
   };
 
   typedef typename std::conditional<Cfg::FOLD64, d4_t, f4_t>::type facc_t;
-  f4_t acc[FM][FN];
+  acc_t acc[FM][FN];
   facc_t facc[FM][FN];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
-      acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+      acc[i][j] = acc_t{0, 0, 0, 0};
       facc[i][j] = facc_t{0, 0, 0, 0};
     }
+  int cseg = sbeg / nsl, coff = sbeg % nsl;   // segment / slab of the slab being multiplied
 
 #pragma unroll
   for (int sl = 0; sl < STAGES - 1; ++sl)
@@ -141,7 +154,7 @@ __device__ __forceinline__ void h16_tile(const T* __restrict__ P, long ldP, long
     if (kt + STAGES - 1 < nk) issue(kt + STAGES - 1);
     const unsigned char* st = smem + (kt % STAGES) * STAGE_BYTES;
 #pragma unroll
-    for (int ks = 0; ks < BK / 32; ++ks) {
+    for (int ks = 0; ks < 2; ++ks) {          // two 64-byte k-steps per slab
       v8 a[FM];
 #pragma unroll
       for (int i = 0; i < FM; ++i) a[i] = frag(st, wm * WM + i * 16 + (lane & 15), ks * 4 + (lane >> 4));
@@ -152,15 +165,23 @@ __device__ __forceinline__ void h16_tile(const T* __restrict__ P, long ldP, long
         for (int i = 0; i < FM; ++i) acc[i][j] = MfmaH<T>::mma(a[i], b, acc[i][j]);
       }
     }
-    if ((kt + 1) % fold == 0 || kt + 1 == nk) {
+    // fold at every fold-th slab, at the end and at every segment end; the
+    // segment's weight 2^-(7 class) scales the exact int8 partials (1 for 16-bit)
+    const bool seg_end = (++coff == nsl);
+    if ((kt + 1) % fold == 0 || kt + 1 == nk || seg_end) {
+      const double w = ldexp(1.0, -7 * (int)((patW >> (3 * cseg)) & 7));
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
 #pragma unroll
-          for (int r = 0; r < 4; ++r) facc[i][j][r] += acc[i][j][r];
-          acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+          for (int r = 0; r < 4; ++r) facc[i][j][r] += (double)acc[i][j][r] * w;
+          acc[i][j] = acc_t{0, 0, 0, 0};
         }
+    }
+    if (seg_end) {
+      coff = 0;
+      ++cseg;
     }
   }
 
@@ -188,16 +209,17 @@ template <typename T, typename Cfg, int BM, int BN, int WM, int WN, int SPLITK>
 __global__ __launch_bounds__((Cfg::NT), (Cfg::MINB))
 void eri_gemm_h16_kernel(const T* __restrict__ P, long ldP, long sP, const T* __restrict__ A, long ldA, long sA,
                          const double* __restrict__ invP, const double* __restrict__ invA, double* __restrict__ C,
-                         int K, int nseg, unsigned patP, unsigned patA, long ldc, int tiles_m,
+                         int K, int nseg, unsigned patP, unsigned patA, unsigned patW, long ldc, int tiles_m,
                          const double* __restrict__ bias, const int* __restrict__ colmap, int fold) {
   extern __shared__ __attribute__((aligned(1024))) unsigned char smem[];
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   const int ks_id = t % SPLITK;
   const int tile = t / SPLITK;
   const int mt = tile % tiles_m, nt = tile / tiles_m;
-  const int part = nseg * (K / 64) / SPLITK;
+  const int part = nseg * (K / (128 / (int)sizeof(T))) / SPLITK;
   h16_tile<T, Cfg, BM, BN, WM, WN>(P, ldP, sP, A, ldA, sA, invP, invA, C, ks_id == 0 ? bias : nullptr, colmap, ldc, K,
-                                   patP, patA, mt, nt, ks_id * part, (ks_id + 1) * part, SPLITK > 1, fold, smem);
+                                   patP, patA, patW, mt, nt, ks_id * part, (ks_id + 1) * part, SPLITK > 1, fold,
+                                   smem);
 }
 
 // Many GEMMs of one precision in one persistent launch: workgroup b takes
@@ -206,14 +228,14 @@ void eri_gemm_h16_kernel(const T* __restrict__ P, long ldP, long sP, const T* __
 template <typename T, typename Cfg, int BM, int BN, int WM, int WN, bool ATOMIC>
 __global__ __launch_bounds__((Cfg::NT), (Cfg::MINB))
 void eri_gemm_h16_grouped_kernel(const H16Item* __restrict__ items, int nitems, unsigned patP, unsigned patA,
-                                 int fold) {
+                                 unsigned patW, int fold) {
   extern __shared__ __attribute__((aligned(1024))) unsigned char smem[];
   for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
     const H16Item it = items[w];
     __syncthreads();   // every wave is done with the previous tile's LDS ring
     h16_tile<T, Cfg, BM, BN, WM, WN>((const T*)it.P, it.ldP, it.sP, (const T*)it.A, it.ldA, it.sA, it.invP, it.invA,
-                                     it.C, it.bias, it.colmap, it.ldc, it.K, patP, patA, it.mt, it.nt, it.sbeg,
-                                     it.send, ATOMIC, fold, smem);
+                                     it.C, it.bias, it.colmap, it.ldc, it.K, patP, patA, patW, it.mt, it.nt,
+                                     it.sbeg, it.send, ATOMIC, fold, smem);
   }
 }
 
@@ -322,6 +344,7 @@ __global__ __launch_bounds__(256, 1) void eri_gemm_f32_kernel(const float* __res
 template <typename T> __device__ inline T to16(double x);
 template <> __device__ inline _Float16 to16<_Float16>(double x) { return (_Float16)(float)x; }
 template <> __device__ inline __bf16 to16<__bf16>(double x) { return (__bf16)(float)x; }
+template <> __device__ inline signed char to16<signed char>(double x) { return (signed char)x; }
 
 // One workgroup per row: s = 2^(top - ceil(log2 max|x|)); nslice slices x_i of
 // s*x (x_0 the rounding of s*x, x_1 of the remainder, ...), stored as
@@ -355,9 +378,16 @@ __global__ __launch_bounds__(256) void split_rows_kernel(const double* __restric
     const int sc_col = cmap ? cmap[c] : c;
     double r = (sc_col >= 0 ? x[sc_col] : 0.0) * sc;
     for (int q = 0; q < nslice; ++q) {
-      const T v = to16<T>(r);
-      d[(long)q * kout + c] = v;
-      r -= (double)(float)v;
+      if constexpr (std::is_same<T, signed char>::value) {
+        // integer slices: x_q = rint(r) in [-127, 127], carry 128 (r - x_q)
+        const double v = fmin(127.0, fmax(-127.0, rint(r)));
+        d[(long)q * kout + c] = (signed char)v;
+        r = (r - v) * 128.0;
+      } else {
+        const T v = to16<T>(r);
+        d[(long)q * kout + c] = v;
+        r -= (double)(float)v;
+      }
     }
   }
 }
@@ -376,22 +406,33 @@ static unsigned pack_pattern(const int* p, int n) {
   return v;
 }
 
-// precision modes (values of PHF_PREC_*): stored slices per operand row and
-// product segments of the contraction
-int mixed_slices(int prec) { return prec == 3 ? 2 : (prec == 2 ? 3 : 1); }
-int mixed_segments(int prec) { return prec == 3 ? 3 : (prec == 2 ? 6 : 1); }
+// precision modes (values of PHF_PREC_*): stored slices per operand row,
+// product segments of the contraction, bytes per stored element
+int mixed_slices(int prec) { return prec == 4 ? 4 : (prec == 3 ? 2 : (prec == 2 ? 3 : 1)); }
+int mixed_segments(int prec) { return prec == 4 ? 10 : (prec == 3 ? 3 : (prec == 2 ? 6 : 1)); }
+int mixed_elem_bytes(int prec) { return prec == 1 ? 4 : (prec == 4 ? 1 : 2); }
 
 //   F16X2  (3 products)  P slices [p0 p1 p0] against A slices [a0 a0 a1]
 //   BF16X3 (6 products)  P slices [p0 p1 p0 p2 p1 p0] against A [a0 a0 a1 a0 a1 a2]
-static void mixed_patterns(int prec, unsigned* patP, unsigned* patA) {
+//   I8X4  (10 products)  every (p_i, a_j) with i + j <= 3, ordered by class i + j;
+//                        class c carries the weight 2^-7c (patW), applied at the
+//                        fold of the exact int32 partial sums
+static void mixed_patterns(int prec, unsigned* patP, unsigned* patA, unsigned* patW) {
   static const int a_f16[3] = {0, 0, 1}, p_f16[3] = {0, 1, 0};
   static const int a_b16[6] = {0, 0, 1, 0, 1, 2}, p_b16[6] = {0, 1, 0, 2, 1, 0};
+  static const int p_i8[10] = {0, 0, 1, 0, 1, 2, 0, 1, 2, 3}, a_i8[10] = {0, 1, 0, 2, 1, 0, 3, 2, 1, 0};
+  static const int w_i8[10] = {0, 1, 1, 2, 2, 2, 3, 3, 3, 3};
+  *patW = 0;
   if (prec == 3) {
     *patP = pack_pattern(p_f16, 3);
     *patA = pack_pattern(a_f16, 3);
-  } else {
+  } else if (prec == 2) {
     *patP = pack_pattern(p_b16, 6);
     *patA = pack_pattern(a_b16, 6);
+  } else {
+    *patP = pack_pattern(p_i8, 10);
+    *patA = pack_pattern(a_i8, 10);
+    *patW = pack_pattern(w_i8, 10);
   }
 }
 
@@ -399,8 +440,10 @@ void split_operand_map(hipStream_t s, int prec, const double* src, long lds, int
                        int kout, void* dst, double* inv) {
   if (prec == 3)
     split_rows_kernel<_Float16><<<rows, 256, 0, s>>>(src, lds, kt, cmap, kout, (_Float16*)dst, 2, 14, inv);
-  else
+  else if (prec == 2)
     split_rows_kernel<__bf16><<<rows, 256, 0, s>>>(src, lds, kt, cmap, kout, (__bf16*)dst, 3, 14, inv);
+  else
+    split_rows_kernel<signed char><<<rows, 256, 0, s>>>(src, lds, kt, cmap, kout, (signed char*)dst, 4, 7, inv);
 }
 
 void split_operand(hipStream_t s, int prec, const double* src, long lds, int rows, int kt, void* dst, double* inv) {
@@ -451,8 +494,8 @@ void eri_gemm_mixed(hipStream_t s, int prec, const void* P, long kp, const void*
                                                     colmap);
     return;
   }
-  unsigned patP, patA;
-  mixed_patterns(prec, &patP, &patA);
+  unsigned patP, patA, patW;
+  mixed_patterns(prec, &patP, &patA, &patW);
   const int nslice = mixed_slices(prec), nseg = mixed_segments(prec);
   const long ldP = nslice * kp, ldA = (long)nslice * K;
   const int bm = kH16Cfgs[cfg][0], bn = kH16Cfgs[cfg][1];
@@ -468,7 +511,7 @@ void eri_gemm_mixed(hipStream_t s, int prec, const void* P, long kp, const void*
       attr_set = true;                                                                                       \
     }                                                                                                        \
     kern<<<grid, CC::NT, CC::LDS, s>>>((const TT*)P, ldP, kp, (const TT*)A, ldA, (long)K, invP, invA, C, K,   \
-                                       nseg, patP, patA, ldc, tm, bias, colmap, h16_fold());                 \
+                                       nseg, patP, patA, patW, ldc, tm, bias, colmap, h16_fold());           \
   } while (0)
 #define H_CFGS(TT, SK)                                                   \
   do {                                                                   \
@@ -481,8 +524,10 @@ void eri_gemm_mixed(hipStream_t s, int prec, const void* P, long kp, const void*
   } while (0)
   if (prec == 2) {
     if (splitk == 2) H_CFGS(__bf16, 2); else H_CFGS(__bf16, 1);
-  } else {
+  } else if (prec == 3) {
     if (splitk == 2) H_CFGS(_Float16, 2); else H_CFGS(_Float16, 1);
+  } else {
+    if (splitk == 2) H_CFGS(signed char, 2); else H_CFGS(signed char, 1);
   }
 #undef H_CFGS
 #undef H_LAUNCH
@@ -497,8 +542,8 @@ int h16_grouped_grid(int cfg, int ncu) { return ncu * ((kH16Cfgs[cfg][0] == 128 
 // into a C the caller zero-filled (split contractions); else every item stores.
 void eri_gemm_mixed_grouped(hipStream_t s, int prec, const H16Item* items, int nitems, int cfg, int grid,
                             bool atomic) {
-  unsigned patP, patA;
-  mixed_patterns(prec, &patP, &patA);
+  unsigned patP, patA, patW;
+  mixed_patterns(prec, &patP, &patA, &patW);
 #define G_LAUNCH(TT, B0, B1, W0, W1, ST, MB, F64)                                                             \
   do {                                                                                                       \
     typedef H16Cfg<B0, B1, W0, W1, ST, MB, F64> CC;                                                          \
@@ -511,9 +556,9 @@ void eri_gemm_mixed_grouped(hipStream_t s, int prec, const H16Item* items, int n
       attr_set = true;                                                                                       \
     }                                                                                                        \
     if (atomic)                                                                                              \
-      k_at<<<grid, CC::NT, CC::LDS, s>>>(items, nitems, patP, patA, h16_fold());                             \
+      k_at<<<grid, CC::NT, CC::LDS, s>>>(items, nitems, patP, patA, patW, h16_fold());                       \
     else                                                                                                     \
-      k_st<<<grid, CC::NT, CC::LDS, s>>>(items, nitems, patP, patA, h16_fold());                             \
+      k_st<<<grid, CC::NT, CC::LDS, s>>>(items, nitems, patP, patA, patW, h16_fold());                       \
   } while (0)
 #define G_CFGS(TT)                                                      \
   do {                                                                  \
@@ -524,7 +569,7 @@ void eri_gemm_mixed_grouped(hipStream_t s, int prec, const H16Item* items, int n
       default: G_LAUNCH(TT, 128, 256, 64, 128, 3, 1, false); break;     \
     }                                                                   \
   } while (0)
-  if (prec == 2) G_CFGS(__bf16); else G_CFGS(_Float16);
+  if (prec == 2) G_CFGS(__bf16); else if (prec == 3) G_CFGS(_Float16); else G_CFGS(signed char);
 #undef G_CFGS
 #undef G_LAUNCH
 }

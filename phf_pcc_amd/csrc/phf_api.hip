@@ -466,7 +466,7 @@ extern "C" int phf_ctx_destroy(phf_ctx* ctx) {
   if (ctx->dA) (void)hipFree(ctx->dA);
   if (ctx->dH1) (void)hipFree(ctx->dH1);
   if (ctx->dHb) (void)hipFree(ctx->dHb);
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < 5; ++q) {
     if (ctx->dAm[q]) (void)hipFree(ctx->dAm[q]);
     if (ctx->dAinv[q]) (void)hipFree(ctx->dAinv[q]);
   }
@@ -573,8 +573,7 @@ static int ensure_mixed_a(phf_ctx* ctx, int prec) {
   if (ctx->dAm[prec]) return PHF_OK;
   if (!ctx->dA) return fail(ctx, PHF_ERR_UNSUPPORTED, "the cached Hamiltonian is stored for another precision only");
   const long elems = (long)ctx->nrpad * ctx->kt * mixed_slices(prec);
-  const size_t esz = (prec == 1) ? sizeof(float) : 2;
-  HIPCK(hipMalloc(&ctx->dAm[prec], elems * esz));
+  HIPCK(hipMalloc(&ctx->dAm[prec], elems * mixed_elem_bytes(prec)));
   if (prec == 1) {
     convert_f32(ctx->stream, ctx->dA, (float*)ctx->dAm[prec], (long)ctx->nrpad * ctx->kt);
   } else {
@@ -607,7 +606,7 @@ std::vector<int> ref_spins(const double* ref, int nso) {
 }
 
 static void drop_blocks(phf_ctx* ctx) {
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < 5; ++q) {
     if (ctx->dAbm[q]) (void)hipFree(ctx->dAbm[q]);
     if (ctx->dAbinv[q]) (void)hipFree(ctx->dAbinv[q]);
     ctx->dAbm[q] = nullptr;
@@ -699,7 +698,7 @@ int phf_build_blocks(phf_ctx* ctx, const std::vector<int>& spin, bool release_de
   if (release_dense) {
     HIPCK(hipFree(ctx->dA));
     ctx->dA = nullptr;
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < 5; ++q) {
       if (ctx->dAm[q]) { HIPCK(hipFree(ctx->dAm[q])); ctx->dAm[q] = nullptr; }
       if (ctx->dAinv[q]) { HIPCK(hipFree(ctx->dAinv[q])); ctx->dAinv[q] = nullptr; }
     }
@@ -714,14 +713,15 @@ static int ensure_mixed_ab(phf_ctx* ctx, int prec) {
   const long elems = ctx->blk_aoff[2] + (long)ctx->blk_rows[2] * ctx->blk_k[2];
   const int nrows = ctx->blk_roff[2] + ctx->blk_rows[2];
   const int ns = mixed_slices(prec);
-  HIPCK(hipMalloc(&ctx->dAbm[prec], elems * ns * (prec == 1 ? sizeof(float) : 2)));
+  const int eb = mixed_elem_bytes(prec);
+  HIPCK(hipMalloc(&ctx->dAbm[prec], elems * ns * eb));
   if (prec == 1) {
     convert_f32(ctx->stream, ctx->dAb, (float*)ctx->dAbm[prec], elems);
   } else {
     HIPCK(hipMalloc(&ctx->dAbinv[prec], sizeof(double) * nrows));
     for (int b = 0; b < 3; ++b)
       split_operand(ctx->stream, prec, ctx->dAb + ctx->blk_aoff[b], ctx->blk_k[b], ctx->blk_rows[b], ctx->blk_k[b],
-                    (char*)ctx->dAbm[prec] + 2 * ns * ctx->blk_aoff[b], ctx->dAbinv[prec] + ctx->blk_roff[b]);
+                    (char*)ctx->dAbm[prec] + (long)eb * ns * ctx->blk_aoff[b], ctx->dAbinv[prec] + ctx->blk_roff[b]);
   }
   HIPCK(hipGetLastError());
   return PHF_OK;
@@ -742,7 +742,7 @@ extern "C" int phf_blocked(const phf_ctx* ctx) { return (ctx && ctx->blocked && 
 // the FP64 A: the Hamiltonian then serves that mode only.
 int phf_keep_only_mixed(phf_ctx* ctx, int prec) {
   if (!ctx->has_ham) return fail(ctx, PHF_ERR_NO_HAMILTONIAN, "no Hamiltonian set");
-  if (prec < PHF_PREC_FP32 || prec > PHF_PREC_F16X2) return fail(ctx, PHF_ERR_ARG, "not a reduced-precision mode");
+  if (prec < PHF_PREC_FP32 || prec > PHF_PREC_I8X4) return fail(ctx, PHF_ERR_ARG, "not a reduced-precision mode");
   if (ctx->a_only && ctx->a_only != prec) return fail(ctx, PHF_ERR_ARG, "A already reduced to another mode");
   if (ctx->blocked) {
     int rc = ensure_mixed_ab(ctx, prec);
@@ -762,7 +762,7 @@ int phf_keep_only_mixed(phf_ctx* ctx, int prec) {
     HIPCK(hipFree(ctx->dA));
     ctx->dA = nullptr;
   }
-  for (int q = 0; q < 4; ++q)
+  for (int q = 0; q < 5; ++q)
     if (q != prec) {
       if (ctx->dAm[q]) { HIPCK(hipFree(ctx->dAm[q])); ctx->dAm[q] = nullptr; }
       if (ctx->dAinv[q]) { HIPCK(hipFree(ctx->dAinv[q])); ctx->dAinv[q] = nullptr; }
@@ -779,7 +779,7 @@ static int drop_hamiltonian(phf_ctx* ctx) {
     int rc = free_grid(ctx);
     if (rc) return rc;
   }
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < 5; ++q) {
     if (ctx->dAm[q]) { HIPCK(hipFree(ctx->dAm[q])); ctx->dAm[q] = nullptr; }
     if (ctx->dAinv[q]) { HIPCK(hipFree(ctx->dAinv[q])); ctx->dAinv[q] = nullptr; }
   }
@@ -1098,7 +1098,7 @@ int eval_plan(phf_ctx* ctx, int nocc, int precision, int cmplxconj, EvalBufs& b)
   if (nocc <= 0 || nocc > nso) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: nocc out of range");
   if (nocc > lu_inverse_max_n())
     return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: nocc above the LDS inverse limit");
-  if (precision < PHF_PREC_FP64 || precision > PHF_PREC_F16X2)
+  if (precision < PHF_PREC_FP64 || precision > PHF_PREC_I8X4)
     return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_buildhsg: unknown precision");
   const bool blk = ctx->blocked && ctx->use_blocks;
   if (precision == PHF_PREC_FP64 && !(blk ? ctx->dAb != nullptr : ctx->dA != nullptr))
@@ -1155,7 +1155,7 @@ int eval_plan(phf_ctx* ctx, int nocc, int precision, int cmplxconj, EvalBufs& b)
     // reduced-precision density rows and their inverse scales; FP64 rows in
     // the permuted (spin-block) contraction order
     const size_t pm = (precision == PHF_PREC_FP64) ? 0 : (size_t)Mp * kP * mixed_slices(precision);
-    o.Pm = a.take<double>(precision == PHF_PREC_FP32 ? (pm + 1) / 2 : (pm + 3) / 4);
+    o.Pm = a.take<double>((pm * mixed_elem_bytes(precision) + 7) / 8);
     o.Pinv = a.take<double>(precision == PHF_PREC_FP64 ? 0 : (size_t)Mp);
     o.Pperm = a.take<double>((blk && precision <= PHF_PREC_FP32) ? (size_t)Mp * kP : 0);
   };
@@ -1279,12 +1279,13 @@ int eval_gemm(phf_ctx* ctx, const EvalBufs& b) {
                      rows_of(q), k_of(q), nrpad, 128, 1, ctx->dHb, map_of(q));
   } else {
     const int ns = mixed_slices(precision);
+    const long eb = mixed_elem_bytes(precision);
     const char* P = (const char*)b.Pm;
     const char* A = (const char*)(b.blk ? ctx->dAbm[precision] : ctx->dAm[precision]);
     const double* invA = b.blk ? ctx->dAbinv[precision] : ctx->dAinv[precision];
     auto run_block = [&](int q, int cfg, int sk) {
       const int mp = (int)round_up(2 * G, h16_cfg_bm(cfg));
-      eri_gemm_mixed(s, precision, P + 2 * kbeg_of(q), b.kP, A + 2 * ns * aoff_of(q), b.Pinv, invA + roff_of(q),
+      eri_gemm_mixed(s, precision, P + eb * kbeg_of(q), b.kP, A + eb * ns * aoff_of(q), b.Pinv, invA + roff_of(q),
                      b.Gam, mp, rows_of(q), k_of(q), nrpad, cfg, sk, ctx->dHb, map_of(q));
     };
     std::pair<int, int> choice[3];

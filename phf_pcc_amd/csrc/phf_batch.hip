@@ -96,6 +96,12 @@ struct phf_batch_s {
   bool use_graph = true;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // side streams (default priority) that run the instances' per-point stages
+  // concurrently; fork/join events order them around the grouped GEMM
+  static constexpr int kSide = 8;
+  hipStream_t side[kSide] = {};
+  hipEvent_t fork_ev[2] = {};
+  hipEvent_t join_ev[2][kSide] = {};
   // timing of the last evaluation
   hipEvent_t ev[4] = {};
   double ms_total = 0, ms_gemm = 0;
@@ -133,6 +139,8 @@ void drop_graph(phf_batch* bt) {
 // eighth of each round.
 void build_items(phf_batch* bt, int prec, int cfg, int sk, int grid, std::vector<H16Item>& out) {
   const int bm = h16_cfg_bm(cfg), bn = h16_cfg_bn(cfg), nseg = mixed_segments(prec), ns = mixed_slices(prec);
+  const long eb = mixed_elem_bytes(prec);
+  const int bk = 128 / (int)eb;                // contraction elements per 128-byte slab
   // work units: (instance, spin block); the dense A is one unit
   struct Unit { int inst, blk; long cost; };
   std::vector<Unit> units;
@@ -156,13 +164,13 @@ void build_items(phf_batch* bt, int prec, int cfg, int sk, int grid, std::vector
     const char* Abase = (const char*)(blk ? c->dAbm[prec] : c->dAm[prec]);
     const double* invA = blk ? c->dAbinv[prec] : c->dAinv[prec];
     const int tm = (int)(round_up(2 * b.V * b.G, bm) / bm), tn = rows / bn;
-    const int part = nseg * (K / 64) / sk;
+    const int part = nseg * (K / bk) / sk;
     for (int nt = 0; nt < tn; ++nt)
       for (int mt = 0; mt < tm; ++mt)
         for (int ks = 0; ks < sk; ++ks) {
           H16Item it;
-          it.P = (const char*)b.Pm + 2 * kbeg;
-          it.A = Abase + 2 * ns * aoff;
+          it.P = (const char*)b.Pm + eb * kbeg;
+          it.A = Abase + eb * ns * aoff;
           it.invP = b.Pinv;
           it.invA = invA + roff;
           it.C = b.Gam;
@@ -272,38 +280,68 @@ int tune_grouped(phf_batch* bt, int prec) {
   return upload_items(bt, items);
 }
 
+// Instances run their per-point stages on the side streams: instance i (in
+// order of decreasing work) on side stream i % kSide, with its context's
+// stream pointed there for the duration of the enqueue.
+static void fork(phf_batch* bt, int phase) {
+  (void)hipEventRecord(bt->fork_ev[phase], bt->stream);
+  for (int k = 0; k < phf_batch_s::kSide; ++k) (void)hipStreamWaitEvent(bt->side[k], bt->fork_ev[phase], 0);
+}
+
+static void join(phf_batch* bt, int phase) {
+  for (int k = 0; k < phf_batch_s::kSide; ++k) {
+    (void)hipEventRecord(bt->join_ev[phase][k], bt->side[k]);
+    (void)hipStreamWaitEvent(bt->stream, bt->join_ev[phase][k], 0);
+  }
+}
+
 // The launch sequence of one evaluation after the plan (no allocation, no
 // host synchronisation: capturable).  Events only when `timed`.
 int enqueue_eval(phf_batch* bt, int prec, bool timed) {
   hipStream_t s = bt->stream;
   const size_t ni = bt->inst.size();
+  std::vector<int> order(ni);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return bt->bufs[a].G * bt->inst[a]->nso > bt->bufs[b].G * bt->inst[b]->nso;
+  });
+  auto on_side = [&](size_t rank, phf_ctx* c) { c->stream = bt->side[rank % phf_batch_s::kSide]; };
   if (timed) BHIPCK(hipEventRecord(bt->ev[0], s));
-  for (size_t i = 0; i < ni; ++i) {
+  fork(bt, 0);
+  for (size_t r = 0; r < ni; ++r) {
+    const int i = order[r];
     phf_ctx* c = bt->inst[i];
     const EvalBufs& b = bt->bufs[i];
+    on_side(r, c);
     BHIPCK(hipMemcpyAsync(b.C0, bt->h_c0 + bt->off_c0[i], sizeof(double) * 2 * (size_t)c->nso * b.N,
-                          hipMemcpyHostToDevice, s));
+                          hipMemcpyHostToDevice, c->stream));
     eval_pre(c, b);
-  }
-  if (timed) BHIPCK(hipEventRecord(bt->ev[1], s));
-  if (prec == PHF_PREC_FP64 || prec == PHF_PREC_FP32) {
-    for (size_t i = 0; i < ni; ++i) {
-      int rc = eval_gemm(bt->inst[i], bt->bufs[i]);
-      if (rc) return bfail(bt, rc, bt->inst[i]->err);
+    if (prec == PHF_PREC_FP64 || prec == PHF_PREC_FP32) {
+      int rc = eval_gemm(c, b);   // per-instance GEMMs overlap on the side streams
+      if (rc) {
+        for (phf_ctx* x : bt->inst) x->stream = bt->stream;
+        return bfail(bt, rc, c->err);
+      }
     }
-  } else {
-    launch_grouped(bt, prec);
   }
+  join(bt, 0);
+  if (timed) BHIPCK(hipEventRecord(bt->ev[1], s));
+  if (prec != PHF_PREC_FP64 && prec != PHF_PREC_FP32) launch_grouped(bt, prec);
   if (timed) BHIPCK(hipEventRecord(bt->ev[2], s));
-  for (size_t i = 0; i < ni; ++i) {
+  fork(bt, 1);
+  for (size_t r = 0; r < ni; ++r) {
+    const int i = order[r];
     phf_ctx* c = bt->inst[i];
     const EvalBufs& b = bt->bufs[i];
+    on_side(r, c);
     eval_post(c, b, true);
     const int nel = (c->nso - b.N) * b.N;
     const int blocks = std::max(1, std::min(1024, (nel + 255) / 256));
-    batch_finish_kernel<<<blocks, 256, 0, s>>>(c->nso, b.N, (int)b.G, b.Info, b.HS, b.Ysum, b.Xsum,
-                                               bt->d_out + bt->off_out[i]);
+    batch_finish_kernel<<<blocks, 256, 0, c->stream>>>(c->nso, b.N, (int)b.G, b.Info, b.HS, b.Ysum, b.Xsum,
+                                                       bt->d_out + bt->off_out[i]);
   }
+  join(bt, 1);
+  for (phf_ctx* c : bt->inst) c->stream = bt->stream;
   BHIPCK(hipGetLastError());
   BHIPCK(hipMemcpyAsync(bt->h_out, bt->d_out, sizeof(double) * bt->n_out, hipMemcpyDeviceToHost, s));
   if (timed) BHIPCK(hipEventRecord(bt->ev[3], s));
@@ -328,11 +366,15 @@ extern "C" int phf_batch_create(int device, int ninst, phf_batch** out) {
   bt->stream = probe.stream;   // the batch owns the stream created for the probe
   bt->inst.assign(ninst, nullptr);
   bt->nocc.assign(ninst, 0);
-  for (auto& e : bt->ev) {
-    if (hipEventCreate(&e) != hipSuccess) {
-      delete bt;
-      return PHF_ERR_HIP;
-    }
+  bool ok = true;
+  for (auto& e : bt->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+  for (auto& e : bt->fork_ev) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+  for (auto& row : bt->join_ev)
+    for (auto& e : row) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+  for (auto& st : bt->side) ok = ok && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
+  if (!ok) {
+    (void)phf_batch_destroy(bt);
+    return PHF_ERR_HIP;
   }
   *out = bt;
   return PHF_OK;
@@ -351,7 +393,17 @@ extern "C" int phf_batch_destroy(phf_batch* bt) {
   if (bt->h_out) (void)hipHostFree(bt->h_out);
   for (auto& e : bt->ev)
     if (e) (void)hipEventDestroy(e);
-  (void)hipStreamDestroy(bt->stream);
+  for (auto& e : bt->fork_ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& row : bt->join_ev)
+    for (auto& e : row)
+      if (e) (void)hipEventDestroy(e);
+  for (auto& st : bt->side)
+    if (st) {
+      (void)hipStreamSynchronize(st);
+      (void)hipStreamDestroy(st);
+    }
+  if (bt->stream) (void)hipStreamDestroy(bt->stream);
   delete bt;
   return PHF_OK;
 }
@@ -365,7 +417,7 @@ extern "C" int phf_batch_set_instance(phf_batch* bt, int i, const double* h, con
   if (!bt) return PHF_ERR_ARG;
   if (i < 0 || i >= (int)bt->inst.size()) return bfail(bt, PHF_ERR_ARG, "phf_batch_set_instance: index out of range");
   if (nocc <= 0 || nocc >= 2 * n) return bfail(bt, PHF_ERR_ARG, "phf_batch_set_instance: nocc out of range");
-  if (storage < PHF_PREC_FP64 || storage > PHF_PREC_F16X2)
+  if (storage < PHF_PREC_FP64 || storage > PHF_PREC_I8X4)
     return bfail(bt, PHF_ERR_ARG, "phf_batch_set_instance: unknown storage precision");
   BHIPCK(hipSetDevice(bt->device));
   BHIPCK(hipStreamSynchronize(bt->stream));
@@ -464,7 +516,7 @@ extern "C" int phf_batch_eval(phf_batch* bt, const double* z, int cmplxconj, int
   if (!z || !energies) return bfail(bt, PHF_ERR_ARG, "phf_batch_eval: NULL argument");
   if (cmplxconj != 0 && cmplxconj != 1)
     return bfail(bt, PHF_ERR_UNSUPPORTED, "phf_batch_eval: CmplxConj must be 0 or 1");
-  if (precision < PHF_PREC_FP64 || precision > PHF_PREC_F16X2)
+  if (precision < PHF_PREC_FP64 || precision > PHF_PREC_I8X4)
     return bfail(bt, PHF_ERR_ARG, "phf_batch_eval: unknown precision");
   for (size_t i = 0; i < bt->inst.size(); ++i) {
     if (!bt->inst[i]) return bfail(bt, PHF_ERR_ARG, "phf_batch_eval: instance " + std::to_string(i) + " not set");
@@ -487,7 +539,7 @@ extern "C" int phf_batch_eval(phf_batch* bt, const double* z, int cmplxconj, int
     zoff += 2 * (size_t)(nso - N) * N;
   }
   hipStream_t s = bt->stream;
-  const bool h16 = precision == PHF_PREC_BF16X3 || precision == PHF_PREC_F16X2;
+  const bool h16 = precision == PHF_PREC_BF16X3 || precision == PHF_PREC_F16X2 || precision == PHF_PREC_I8X4;
   if (fresh || !bt->exec) {
     if (fresh && h16) {
       // the grouped tiles are tuned on live operands: densities first

@@ -18,6 +18,7 @@ void eri_gemm_f64(hipStream_t s, const double* P, long ldp, const double* A, dou
 int eri_gemm_f64_bm(int rows);
 int mixed_segments(int prec);
 int mixed_slices(int prec);
+int mixed_elem_bytes(int prec);
 void split_operand(hipStream_t s, int prec, const double* src, long lds, int rows, int kt, void* dst, double* inv);
 void split_operand_map(hipStream_t s, int prec, const double* src, long lds, int rows, int kt, const int* cmap,
                        int kout, void* dst, double* inv);
@@ -49,8 +50,8 @@ struct phf_ctx_s {
   double* dHb = nullptr;    // hone as ERI-GEMM bias: [2][nrpad] (re plane, im plane), r = i*nso + k
   // reduced-precision copies of A, built on first use per mode:
   // [nrpad][nseg*kt] of FP32 / BF16 / FP16 and per-row inverse scales
-  void* dAm[4] = {nullptr, nullptr, nullptr, nullptr};
-  double* dAinv[4] = {nullptr, nullptr, nullptr, nullptr};
+  void* dAm[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  double* dAinv[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   int a_only = 0;            // > 0: only the reduced-precision copy of this mode is stored (dA released)
   // Spin-block structure (every MO of Ref purely alpha or beta): with the pairs
   // (i,k) and (j,l) ordered by spin class A is block diagonal, diag(D, X1, X2):
@@ -67,8 +68,8 @@ struct phf_ctx_s {
   int* d_rowmap = nullptr;   // [sum blk_rows] permuted A row -> natural i*nso + k, -1 padding
   int* d_colperm = nullptr;  // [ktp] permuted contraction column -> natural j*nso + l, -1 padding
   double* dAb = nullptr;
-  void* dAbm[4] = {nullptr, nullptr, nullptr, nullptr};
-  double* dAbinv[4] = {nullptr, nullptr, nullptr, nullptr};
+  void* dAbm[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  double* dAbinv[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   std::vector<int> blk_spin; // MO spins (0 alpha, 1 beta) the blocks were built for
   // workspace arena
   char* ws = nullptr;

@@ -1,4 +1,4 @@
-"""Reduced-precision ERI contraction (FP32, BF16x3, F16x2) against the FP64
+"""Reduced-precision ERI contraction (FP32, BF16x3, F16x2, I8x4) against the FP64
 oracle.  The overlap inverse, determinant, Fock contraction and grid sums stay
 FP64, so the error comes from the Gamma = A rho products only.
 
@@ -14,7 +14,7 @@ from tests._cases import grid_args, hubbard_cfg1, hubbard_general_ref, synthetic
 
 pytestmark = pytest.mark.gpu
 
-PRECS = ["fp32", "bf16x3", "f16x2"]
+PRECS = ["fp32", "bf16x3", "f16x2", "i8x4"]
 
 
 @pytest.fixture(scope="module")
@@ -61,7 +61,7 @@ def test_mixed_energy_within_1e6(tools, synth, prec):
         assert err < tol, f"{name} rel err {err:.3e}"
 
 
-@pytest.mark.parametrize("prec", ["f16x2", "bf16x3"])
+@pytest.mark.parametrize("prec", ["f16x2", "bf16x3", "i8x4"])
 def test_mixed_complex_integrals(tools, prec):
     case = hubbard_general_ref(6, 3.0, 7, zscale=0.2, complex_ref=True)
     ga = grid_args(6, (14, 8), 0)

@@ -130,7 +130,7 @@ def test_resident_gradient_projected_blocks(nsite, seed, j, ngrid, npg, cmplxcon
     assert np.abs(g2 - g).max() < 1e-11
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("fp32", 1e-6), ("bf16x3", 1e-6), ("f16x2", 1e-6)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("fp32", 1e-6), ("bf16x3", 1e-6), ("f16x2", 1e-6), ("i8x4", 1e-7)])
 def test_spin_blocks_match_dense(prob, prec, tol):
     """Spin-pure Ref: A is stored as its three spin blocks; evaluating through
     them equals the dense contraction (FP64: summation order only)."""
