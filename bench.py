@@ -166,8 +166,7 @@ def run_batch(args):
     out = {"metric": "projected E+grad evals/s (grid pts/s) vs plain FP64 GPU at 1e-6 Eh; % MFMA peak",
            "value": best["value"], "unit": "grid pts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": best["ms_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": {"fp64": "f64", "fp32": "f32 (FP64 accumulate)", "bf16x3": "bf16x3 split (FP64 accumulate)",
-                     "f16x2": "f16x2 scaled split (FP64 accumulate)"}[bprec],
+           "dtype": _form_dtype(bprec),
            "data": "synthetic",
            "config": {"workload": f"batched: {len(inst)} synthetic instances, n in {{16..96}}, N=n/2, L=3n, grids "
                                   f"Lebedev {{6..50}} x gamma {{4,8,16}} (seed 1004), {G} grid points per step, "
@@ -373,7 +372,7 @@ def main():
     ap.add_argument("--nocc", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="fp64,f16x2",
+    ap.add_argument("--forms", default="fp64,f16x2,i8x4",
                     help="comma list of ERI precisions to time (fp64 always runs: it is the reference form)")
     ap.add_argument("--dropin", action="store_true",
                     help="also time the PHFTools_F drop-in (host arrays in, Hmat/Smat/Gradmat/Rdmmat out per call)")
@@ -498,6 +497,9 @@ def main():
                        "bf16x3": "eri_gemm_h16_kernel<bf16>", "f16x2": "eri_gemm_h16_kernel<f16>",
                        "i8x4": "eri_gemm_h16_kernel<i8>"}[best["prec"]],
             "flops_per_launch": nprod * flops, "algorithmic_flops_per_launch": flops, "spin_blocks": blocked,
+            "launches_per_step": 3 if blocked else 1,
+            "time_source": "HIP events around the ERI GEMM stage on the library stream (the launches of one step: "
+                           "one per spin block when blocked)",
             "products": nprod, "fp64_equiv_tflops": flops / t_gemm / 1e12,
             "plain_fp64_dense_frac": flops_dense / (plain["gemm_ms"] * 1e-3) / PEAK["fp64"]}
     value = best["value"]

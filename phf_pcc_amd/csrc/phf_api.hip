@@ -1253,6 +1253,78 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
   }
 }
 
+// Work list of a grouped 16-bit ERI GEMM over (context, evaluation buffers)
+// units for tile config cfg and split-K sk.  Units are split into their spin
+// blocks (or the dense A) and ordered by decreasing contraction length (largest
+// first, for the static round-robin over persistent workgroups); within a
+// block, column tiles outermost, row tiles and contraction parts innermost, so
+// neighbouring items share A panels.  Full rounds of `grid` items are then
+// placed so that XCD x (workgroups b with b % 8 == x) takes a contiguous
+// eighth of each round.
+void build_h16_items(const std::vector<std::pair<const phf_ctx*, const EvalBufs*>>& units, int prec, int cfg,
+                     int sk, int grid, std::vector<H16Item>& out) {
+  const int bm = h16_cfg_bm(cfg), bn = h16_cfg_bn(cfg), nseg = mixed_segments(prec), ns = mixed_slices(prec);
+  const long eb = mixed_elem_bytes(prec);
+  const int bk = 128 / (int)eb;                // contraction elements per 128-byte slab
+  struct Unit { int u, blk; long cost; };
+  std::vector<Unit> work;
+  for (size_t i = 0; i < units.size(); ++i) {
+    const phf_ctx* c = units[i].first;
+    if (units[i].second->blk)
+      for (int q = 0; q < 3; ++q) work.push_back({(int)i, q, (long)c->blk_k[q]});
+    else
+      work.push_back({(int)i, -1, (long)c->kt});
+  }
+  std::stable_sort(work.begin(), work.end(), [](const Unit& a, const Unit& b) { return a.cost > b.cost; });
+  std::vector<H16Item> logical;
+  for (const Unit& u : work) {
+    const phf_ctx* c = units[u.u].first;
+    const EvalBufs& b = *units[u.u].second;
+    const bool blk = u.blk >= 0;
+    const int rows = blk ? c->blk_rows[u.blk] : c->nrpad;
+    const int K = blk ? c->blk_k[u.blk] : c->kt;
+    const long kbeg = blk ? c->blk_kbeg[u.blk] : 0, aoff = blk ? c->blk_aoff[u.blk] : 0;
+    const long roff = blk ? c->blk_roff[u.blk] : 0;
+    const char* Abase = (const char*)(blk ? c->dAbm[prec] : c->dAm[prec]);
+    const double* invA = blk ? c->dAbinv[prec] : c->dAinv[prec];
+    const int tm = (int)(round_up(2 * b.V * b.G, bm) / bm), tn = rows / bn;
+    const int part = nseg * (K / bk) / sk;
+    for (int nt = 0; nt < tn; ++nt)
+      for (int mt = 0; mt < tm; ++mt)
+        for (int ks = 0; ks < sk; ++ks) {
+          H16Item it;
+          it.P = (const char*)b.Pm + eb * kbeg;
+          it.A = Abase + eb * ns * aoff;
+          it.invP = b.Pinv;
+          it.invA = invA + roff;
+          it.C = b.Gam;
+          it.bias = ks == 0 ? c->dHb : nullptr;
+          it.colmap = blk ? c->d_rowmap + roff : nullptr;
+          it.ldP = ns * b.kP;
+          it.sP = b.kP;
+          it.ldA = (long)ns * K;
+          it.sA = K;
+          it.ldc = c->nrpad;
+          it.K = K;
+          it.mt = mt;
+          it.nt = nt;
+          it.sbeg = ks * part;
+          it.send = (ks + 1) * part;
+          it.atomic = sk > 1;
+          logical.push_back(it);
+        }
+  }
+  const size_t n = logical.size();
+  out.resize(n);
+  const int per = grid / 8;
+  for (size_t l = 0; l < n; ++l) {
+    const size_t r = l / grid, j = l % grid;
+    size_t p = l;
+    if (per > 0 && grid % 8 == 0 && (r + 1) * grid <= n) p = r * grid + (j % per) * 8 + j / per;
+    out[p] = logical[l];
+  }
+}
+
 int eval_gemm(phf_ctx* ctx, const EvalBufs& b) {
   hipStream_t s = ctx->stream;
   const int kt = ctx->kt, nrpad = ctx->nrpad, precision = b.prec;

@@ -130,75 +130,11 @@ void drop_graph(phf_batch* bt) {
   bt->graph = nullptr;
 }
 
-// Work list of the grouped launch for tile config cfg and split-K sk.
-// Instances in order of decreasing per-tile contraction length (largest
-// first, for the static round-robin over persistent workgroups); within an
-// instance column tiles outermost, row tiles and contraction parts innermost,
-// so neighbouring items share A panels.  Full rounds of `grid` items are then
-// placed so that XCD x (workgroups b with b % 8 == x) takes a contiguous
-// eighth of each round.
+// Work list of the grouped launch over every instance (shared builder).
 void build_items(phf_batch* bt, int prec, int cfg, int sk, int grid, std::vector<H16Item>& out) {
-  const int bm = h16_cfg_bm(cfg), bn = h16_cfg_bn(cfg), nseg = mixed_segments(prec), ns = mixed_slices(prec);
-  const long eb = mixed_elem_bytes(prec);
-  const int bk = 128 / (int)eb;                // contraction elements per 128-byte slab
-  // work units: (instance, spin block); the dense A is one unit
-  struct Unit { int inst, blk; long cost; };
-  std::vector<Unit> units;
-  for (size_t i = 0; i < bt->inst.size(); ++i) {
-    const phf_ctx* c = bt->inst[i];
-    if (bt->bufs[i].blk)
-      for (int q = 0; q < 3; ++q) units.push_back({(int)i, q, (long)c->blk_k[q]});
-    else
-      units.push_back({(int)i, -1, (long)c->kt});
-  }
-  std::stable_sort(units.begin(), units.end(), [](const Unit& a, const Unit& b) { return a.cost > b.cost; });
-  std::vector<H16Item> logical;
-  for (const Unit& u : units) {
-    const phf_ctx* c = bt->inst[u.inst];
-    const EvalBufs& b = bt->bufs[u.inst];
-    const bool blk = u.blk >= 0;
-    const int rows = blk ? c->blk_rows[u.blk] : c->nrpad;
-    const int K = blk ? c->blk_k[u.blk] : c->kt;
-    const long kbeg = blk ? c->blk_kbeg[u.blk] : 0, aoff = blk ? c->blk_aoff[u.blk] : 0;
-    const long roff = blk ? c->blk_roff[u.blk] : 0;
-    const char* Abase = (const char*)(blk ? c->dAbm[prec] : c->dAm[prec]);
-    const double* invA = blk ? c->dAbinv[prec] : c->dAinv[prec];
-    const int tm = (int)(round_up(2 * b.V * b.G, bm) / bm), tn = rows / bn;
-    const int part = nseg * (K / bk) / sk;
-    for (int nt = 0; nt < tn; ++nt)
-      for (int mt = 0; mt < tm; ++mt)
-        for (int ks = 0; ks < sk; ++ks) {
-          H16Item it;
-          it.P = (const char*)b.Pm + eb * kbeg;
-          it.A = Abase + eb * ns * aoff;
-          it.invP = b.Pinv;
-          it.invA = invA + roff;
-          it.C = b.Gam;
-          it.bias = ks == 0 ? c->dHb : nullptr;
-          it.colmap = blk ? c->d_rowmap + roff : nullptr;
-          it.ldP = ns * b.kP;
-          it.sP = b.kP;
-          it.ldA = (long)ns * K;
-          it.sA = K;
-          it.ldc = c->nrpad;
-          it.K = K;
-          it.mt = mt;
-          it.nt = nt;
-          it.sbeg = ks * part;
-          it.send = (ks + 1) * part;
-          it.atomic = sk > 1;
-          logical.push_back(it);
-        }
-  }
-  const size_t n = logical.size();
-  out.resize(n);
-  const int per = grid / 8;
-  for (size_t l = 0; l < n; ++l) {
-    const size_t r = l / grid, j = l % grid;
-    size_t p = l;
-    if (per > 0 && grid % 8 == 0 && (r + 1) * grid <= n) p = r * grid + (j % per) * 8 + j / per;
-    out[p] = logical[l];
-  }
+  std::vector<std::pair<const phf_ctx*, const EvalBufs*>> units;
+  for (size_t i = 0; i < bt->inst.size(); ++i) units.push_back({bt->inst[i], &bt->bufs[i]});
+  build_h16_items(units, prec, cfg, sk, grid, out);
 }
 
 int upload_items(phf_batch* bt, const std::vector<H16Item>& items) {

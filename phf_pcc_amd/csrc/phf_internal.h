@@ -168,6 +168,9 @@ struct EvalBufs {
 int eval_plan(phf_ctx* ctx, int nocc, int precision, int cmplxconj, EvalBufs& b);
 void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks = nullptr);
 int eval_gemm(phf_ctx* ctx, const EvalBufs& b);
+// Work list of a grouped 16-bit ERI GEMM over (context, buffers) units.
+void build_h16_items(const std::vector<std::pair<const phf_ctx*, const EvalBufs*>>& units, int prec, int cfg,
+                     int sk, int grid, std::vector<H16Item>& out);
 void eval_post(phf_ctx* ctx, const EvalBufs& b, bool resident);
 // C0 = [I; Z] column-major complex (NSO x N) from Z ((NSO-N) x N, C order)
 void build_c0(const double* z, int nso, int N, int cmplxconj, double* c0);
