@@ -15,6 +15,8 @@
 //
 // Layout: P is [Mp][nslice*K], A is [Nr][nslice*K] (slices of one row side by
 // side), C is [Mp][ldc]; Mp % BM == 0, Nr % BN == 0, K % 64 == 0.
+#include <cstdlib>
+#include <cstring>
 #include <type_traits>
 
 #include "phf_common.h"
@@ -66,6 +68,14 @@ typedef __attribute__((address_space(1))) void glb_void_t;
 // partial into C with atomics; two partials onto zero give an order-independent
 // result.  FOLD64: fold the FP32 MFMA tile into an FP64 tile (else into an FP32
 // master tile: half the registers, used by the large per-wave tiles).
+//
+// The segments are walked one after another rather than fused per k-range
+// (both slices of a range loaded once, three products per fragment pair):
+// fusing adds the small cross products into the accumulator that holds the
+// main product, and the truncating 16-bit MFMA then drops their low bits at
+// the main term's ulp (measured: |dE| 3.0e-7 -> 4.7e-7 at n = 64 for an 8%
+// faster contraction); a fresh accumulator per segment keeps them at their
+// own scale.
 template <int BM, int BN, int WM, int WN, int STAGES_, int MINB_, bool FOLD64_>
 struct H16Cfg {
   static constexpr int NW = (BM / WM) * (BN / WN);
@@ -110,8 +120,8 @@ __device__ __forceinline__ void h16_tile(const T* __restrict__ P, long ldP, long
     const int b = wave * BPW + q;
     const int r = 8 * b + (lane >> 3);
     const int c = (lane & 7) ^ ((r >> 1) & 7);
-    src[q] = ((r < BM) ? P + ((long)mt * BM + r) * ldP : A + ((long)nt * BN + (r - BM)) * ldA) +
-             c * (16 / (int)sizeof(T));   // chunk c = 16 bytes
+    const T* row = (r < BM) ? P + ((long)mt * BM + r) * ldP : A + ((long)nt * BN + (r - BM)) * ldA;
+    src[q] = row + c * (16 / (int)sizeof(T));   // chunk c = 16 bytes
   }
   int iseg = sbeg / nsl, ioff = sbeg % nsl;   // segment / slab of the next slab to issue
   auto issue = [&](int slab) {
@@ -220,6 +230,170 @@ void eri_gemm_h16_kernel(const T* __restrict__ P, long ldP, long sP, const T* __
   h16_tile<T, Cfg, BM, BN, WM, WN>(P, ldP, sP, A, ldA, sA, invP, invA, C, ks_id == 0 ? bias : nullptr, colmap, ldc, K,
                                    patP, patA, patW, mt, nt, ks_id * part, (ks_id + 1) * part, SPLITK > 1, fold,
                                    smem);
+}
+
+// Exact int8 form (I8X4) on a 256 x 256 workgroup tile: eight waves of 128 x
+// 64 (two per SIMD), 128 int32 accumulators each, 1 workgroup per CU.
+// Integer accumulation is exact, so no FP64 fold tile is needed, and the
+// larger tile halves the operand bytes per product of the 128 x 128 tile (the
+// operand stream, not the matrix cores, bounds the smaller tile).  The kernels
+// split every tile's slab range at each class boundary (i8_tile_parts); one
+// range sums the products of one class exactly in int32 and adds
+// S_c 2^(-7c) s_P^-1 s_A^-1 to C in FP64 (the first range of an unsplit tile
+// stores, with the bias).  One int32 accumulator cannot carry several classes:
+// rescaling between classes rounds below a class unit, and carrying the finer
+// class instead overflows.  Every slice lies in [-127, 127], so class 3 (four
+// products) sums at most 4 * 127^2 * K: K <= I8_TILE_MAX_K keeps every class
+// inside int32.  Operand layout, LDS ring, swizzle and split contractions as
+// in h16_tile; the loads address a wave-uniform base plus a 32-bit lane
+// offset.  add: C += the result (the same lanes wrote C before); atomic:
+// atomic adds onto a zero-filled C.
+template <int STAGES>
+__device__ __forceinline__ void i8_tile(const signed char* __restrict__ P, long ldP, long sP,
+                                        const signed char* __restrict__ A, long ldA, long sA,
+                                        const double* __restrict__ invP, const double* __restrict__ invA,
+                                        double* __restrict__ C, const double* __restrict__ bias,
+                                        const int* __restrict__ colmap, long ldc, int K, unsigned patP,
+                                        unsigned patA, unsigned patW, int mt, int nt, int sbeg, int send,
+                                        bool atomic, bool add, unsigned char* smem) {
+  constexpr int BM = 256, BN = 256, WM = 128, WN = 64, BK = 128;
+  constexpr int FM = WM / 16, FN = WN / 16;
+  constexpr int STAGE_BYTES = (BM + BN) * 128;
+  constexpr int BPW = 8;                      // 1 KB blocks (8 rows) per wave per slab
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave >> 2, wn = wave & 3;
+  const int nk = send - sbeg, nsl = K / BK;
+
+  // wave w loads slab rows 64 w .. 64 w + 63: waves 0-3 the P rows, 4-7 the A rows
+  const bool isP = wave < 4;
+  const long ld = isP ? ldP : ldA, sl = isP ? sP : sA;
+  const signed char* wbase = isP ? P + ((long)mt * BM + (wave & 3) * 64) * ldP
+                                 : A + ((long)nt * BN + (wave & 3) * 64) * ldA;
+  // LDS block b = 8 wave + q holds rows 8 b .. 8 b + 7; lane -> row rl = lane / 8
+  // of the block, whose chunk slot swizzle (r >> 1) & 7 = (4 (q & 1) + rl / 2) & 7
+  const int rl = lane >> 3;
+  const unsigned off_even = (unsigned)(rl * ld) + 16u * ((lane & 7) ^ ((rl >> 1) & 7));
+  const unsigned off_odd = (unsigned)(rl * ld) + 16u * ((lane & 7) ^ ((4 + (rl >> 1)) & 7));
+
+  int iseg = sbeg / nsl, ioff = sbeg % nsl;
+  auto issue = [&](int slab) {
+    unsigned char* st = smem + (slab % STAGES) * STAGE_BYTES + wave * BPW * 1024;
+    const signed char* b = wbase + (long)((isP ? (patP >> (3 * iseg)) : (patA >> (3 * iseg))) & 7) * sl + ioff * BK;
+#pragma unroll
+    for (int q = 0; q < BPW; ++q)
+      __builtin_amdgcn_global_load_lds((glb_void_t*)(b + (long)(8 * q) * ld + ((q & 1) ? off_odd : off_even)),
+                                       (lds_void_t*)(st + q * 1024), 16, 0, 0);
+    if (++ioff == nsl) { ioff = 0; ++iseg; }
+  };
+  auto frag = [&](const unsigned char* st, int r, int c) -> i4_t {
+    return *reinterpret_cast<const i4_t*>(st + (r >> 3) * 1024 + ((r & 7) * 8 + (c ^ ((r >> 1) & 7))) * 16);
+  };
+
+  i4_t acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = i4_t{0, 0, 0, 0};
+
+#pragma unroll
+  for (int sl0 = 0; sl0 < STAGES - 1; ++sl0)
+    if (sl0 < nk) issue(sl0);
+  for (int kt = 0; kt < nk; ++kt) {
+    if (STAGES == 3 && kt + 1 < nk) PHF_WAIT_VMCNT(BPW);
+    else PHF_WAIT_VMCNT(0);
+    __builtin_amdgcn_s_barrier();
+    if (kt + STAGES - 1 < nk) issue(kt + STAGES - 1);
+    const unsigned char* st = smem + (kt % STAGES) * STAGE_BYTES;
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      i4_t a[FM];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) a[i] = frag(st, wm * WM + i * 16 + (lane & 15), ks * 4 + (lane >> 4));
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const i4_t b = frag(st, BM + wn * WN + j * 16 + (lane & 15), ks * 4 + (lane >> 4));
+#pragma unroll
+        for (int i = 0; i < FM; ++i) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b, acc[i][j], 0, 0, 0);
+      }
+    }
+  }
+
+  const double w = ldexp(1.0, -7 * (int)((patW >> (3 * (sbeg / nsl))) & 7));
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long row = (long)mt * BM + wm * WM + i * 16 + 4 * (lane >> 4) + r;
+        const long ar = (long)nt * BN + wn * WN + j * 16 + (lane & 15);
+        const long col = colmap ? colmap[ar] : ar;
+        if (col < 0) continue;
+        double v = (double)acc[i][j][r] * w * invP[row] * invA[ar];
+        if (bias && !add) v += bias[(row & 1) * ldc + col];
+        if (atomic)
+          atomicAdd(&C[row * ldc + col], v);
+        else if (add)
+          C[row * ldc + col] += v;
+        else
+          C[row * ldc + col] = v;
+      }
+}
+
+constexpr int I8_TILE_STAGES = 2;
+constexpr int I8_TILE_LDS = I8_TILE_STAGES * 512 * 128;
+
+// Slab range [sbeg, send) of one tile, one single-class range after another.
+__device__ __forceinline__ void i8_tile_parts(const signed char* P, long ldP, long sP, const signed char* A, long ldA,
+                                              long sA, const double* invP, const double* invA, double* C,
+                                              const double* bias, const int* colmap, long ldc, int K,
+                                              unsigned patP, unsigned patA, unsigned patW, int mt, int nt,
+                                              int sbeg, int send, bool atomic, unsigned char* smem) {
+  const int nsl = K / 128;
+  bool add = false;
+  for (int b = sbeg; b < send;) {
+    // end of the class of slab b
+    int sg = b / nsl;
+    const unsigned cls = (patW >> (3 * sg)) & 7;
+    while (sg + 1 < 10 && ((patW >> (3 * (sg + 1))) & 7) == cls) ++sg;
+    const int e = min((sg + 1) * nsl, send);
+    if (add) __syncthreads();   // every wave is done with the LDS ring of the previous range
+    i8_tile<I8_TILE_STAGES>(P, ldP, sP, A, ldA, sA, invP, invA, C, bias, colmap, ldc, K, patP, patA, patW, mt, nt,
+                            b, e, atomic, add, smem);
+    add = true;
+    b = e;
+  }
+}
+
+template <int SPLITK>
+__global__ __launch_bounds__(512, 1)
+void eri_gemm_i8_kernel(const signed char* __restrict__ P, long ldP, long sP, const signed char* __restrict__ A,
+                        long ldA, long sA, const double* __restrict__ invP, const double* __restrict__ invA,
+                        double* __restrict__ C, int K, int nseg, unsigned patP, unsigned patA, unsigned patW,
+                        long ldc, int tiles_m, const double* __restrict__ bias, const int* __restrict__ colmap) {
+  extern __shared__ __attribute__((aligned(1024))) unsigned char smem[];
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int ks_id = t % SPLITK;
+  const int tile = t / SPLITK;
+  const int mt = tile % tiles_m, nt = tile / tiles_m;
+  const int part = nseg * (K / 128) / SPLITK;
+  i8_tile_parts(P, ldP, sP, A, ldA, sA, invP, invA, C, ks_id == 0 ? bias : nullptr, colmap, ldc, K, patP, patA,
+                patW, mt, nt, ks_id * part, (ks_id + 1) * part, SPLITK > 1, smem);
+}
+
+template <bool ATOMIC>
+__global__ __launch_bounds__(512, 1)
+void eri_gemm_i8_grouped_kernel(const H16Item* __restrict__ items, int nitems, unsigned patP, unsigned patA,
+                                unsigned patW) {
+  extern __shared__ __attribute__((aligned(1024))) unsigned char smem[];
+  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    const H16Item it = items[w];
+    __syncthreads();   // every wave is done with the previous tile's LDS ring
+    i8_tile_parts((const signed char*)it.P, it.ldP, it.sP, (const signed char*)it.A, it.ldA, it.sA, it.invP,
+                  it.invA, it.C, it.bias, it.colmap, it.ldc, it.K, patP, patA, patW, it.mt, it.nt, it.sbeg, it.send,
+                  ATOMIC, smem);
+  }
 }
 
 // Many GEMMs of one precision in one persistent launch: workgroup b takes
@@ -417,6 +591,16 @@ int mixed_elem_bytes(int prec) { return prec == 1 ? 4 : (prec == 4 ? 1 : 2); }
 //   I8X4  (10 products)  every (p_i, a_j) with i + j <= 3, ordered by class i + j;
 //                        class c carries the weight 2^-7c (patW), applied at the
 //                        fold of the exact int32 partial sums
+// The int8 tile walks the same ten products by decreasing class (3, 2, 1, 0),
+// patW holding each segment's class.
+static void mixed_patterns_desc(unsigned* patP, unsigned* patA, unsigned* patW) {
+  static const int p_d[10] = {0, 1, 2, 3, 0, 1, 2, 0, 1, 0}, a_d[10] = {3, 2, 1, 0, 2, 1, 0, 1, 0, 0};
+  static const int w_d[10] = {3, 3, 3, 3, 2, 2, 2, 1, 1, 0};
+  *patP = pack_pattern(p_d, 10);
+  *patA = pack_pattern(a_d, 10);
+  *patW = pack_pattern(w_d, 10);
+}
+
 static void mixed_patterns(int prec, unsigned* patP, unsigned* patA, unsigned* patW) {
   static const int a_f16[3] = {0, 0, 1}, p_f16[3] = {0, 1, 0};
   static const int a_b16[6] = {0, 0, 1, 0, 1, 2}, p_b16[6] = {0, 1, 0, 2, 1, 0};
@@ -459,23 +643,67 @@ void convert_f32(hipStream_t s, const double* src, float* dst, long n) {
 // Folding more often does not lower the error of these forms: the 16-bit
 // MFMA sums its products with truncation inside the instruction, which no
 // fold interval removes (measured at 8, 4, 2 and 1 slabs on n = 128).
-static int h16_fold() { return 8; }
+// PHF_H16_FOLD=<slabs> overrides it (a value above the slab count folds only
+// at segment ends).
+static int h16_fold() {
+  const char* v = getenv("PHF_H16_FOLD");
+  const int f = v ? atoi(v) : 0;
+  return f > 0 ? f : 8;
+}
 
-// Tile configurations of the 16-bit kernel: {BM, BN, WM, WN}.
-static const int kH16Cfgs[][4] = {{128, 128, 64, 64}, {256, 128, 64, 64}, {256, 128, 128, 64}, {128, 256, 64, 128}};
-int h16_num_configs() { return 4; }
-int h16_cfg_bm(int c) { return kH16Cfgs[c][0]; }
-int h16_cfg_bn(int c) { return kH16Cfgs[c][1]; }
+// Tile configurations of the reduced-precision kernels; the 256 x 256 int8
+// tile serves I8X4 only.  fold64: false for the ones that fold into an FP32
+// master tile (the int8 tile accumulates exactly).
+struct H16CfgDesc {
+  int bm, bn, wm, wn, stages, minb;
+  bool fold64, i8big;
+};
+static const H16CfgDesc kH16Cfgs[] = {
+    {128, 128, 64, 64, 2, 2, true, false},  {256, 128, 64, 64, 3, 1, true, false},
+    {256, 128, 128, 64, 3, 1, false, false}, {128, 256, 64, 128, 3, 1, false, false},
+    {256, 256, 128, 64, 2, 1, true, true},
+};
+// Longest contraction of the int8 tile: class 3 sums four products of slices
+// in [-127, 127] per element, 4 * 127^2 * K < 2^31.
+static const int I8_TILE_MAX_K = 33285;
+int h16_num_configs() { return (int)(sizeof kH16Cfgs / sizeof kH16Cfgs[0]); }
+int h16_cfg_bm(int c) { return kH16Cfgs[c].bm; }
+int h16_cfg_bn(int c) { return kH16Cfgs[c].bn; }
+bool h16_cfg_fold64(int c) { return kH16Cfgs[c].fold64; }
+// whether config c can run precision prec on contractions of length K
+bool h16_cfg_serves(int c, int prec, int K) {
+  if (kH16Cfgs[c].i8big) return prec == 4 && K <= I8_TILE_MAX_K;
+  return true;
+}
+// PHF_H16_CFG=<index>[,<splitk>]: the one configuration (and split-K) the
+// tuners may use; -1 / 0 when unset
+int h16_forced_config() {
+  const char* v = getenv("PHF_H16_CFG");
+  if (!v || !*v) return -1;
+  const int c = atoi(v);
+  return (c >= 0 && c < h16_num_configs()) ? c : -1;
+}
+int h16_forced_splitk() {
+  const char* v = getenv("PHF_H16_CFG");
+  const char* comma = v ? strchr(v, ',') : nullptr;
+  if (!comma) return 0;
+  const int sk = atoi(comma + 1);
+  return (sk == 1 || sk == 2) ? sk : 0;
+}
+// slabs of the whole contraction of length K (what split-K divides)
+int h16_slabs(int prec, int c, int K) {
+  (void)c;
+  return mixed_segments(prec) * (K / (128 / mixed_elem_bytes(prec)));
+}
 
 // Cost model used to order the autotuner's candidates (and to choose when it
 // is off): MFMA rounds over the CUs for cfg c and split-K sk, counting padded
 // rows, normalised to one CU's work per round.
 double h16_model_cost(int c, int sk, long Mp0, int Nr0, int ncu) {
-  const int bm = kH16Cfgs[c][0], bn = kH16Cfgs[c][1];
+  const int bm = kH16Cfgs[c].bm, bn = kH16Cfgs[c].bn, per_cu = kH16Cfgs[c].minb;
   const long mp = (Mp0 + bm - 1) / bm * bm;
   const long np = (Nr0 + bn - 1) / bn * bn;
   const long units = (mp / bm) * (np / bn) * sk;
-  const int per_cu = (bm == 128 && bn == 128) ? 2 : 1;
   const long rounds = (units + (long)per_cu * ncu - 1) / ((long)per_cu * ncu);
   return (double)rounds * per_cu * bm * bn / sk;
 }
@@ -498,10 +726,25 @@ void eri_gemm_mixed(hipStream_t s, int prec, const void* P, long kp, const void*
   mixed_patterns(prec, &patP, &patA, &patW);
   const int nslice = mixed_slices(prec), nseg = mixed_segments(prec);
   const long ldP = nslice * kp, ldA = (long)nslice * K;
-  const int bm = kH16Cfgs[cfg][0], bn = kH16Cfgs[cfg][1];
+  const int bm = kH16Cfgs[cfg].bm, bn = kH16Cfgs[cfg].bn;
   const int tm = Mp / bm, tn = Nr / bn;
   const unsigned grid = (unsigned)(tm * tn * splitk);
-#define H_LAUNCH(TT, B0, B1, W0, W1, ST, MB, F64, SK)                                                         \
+  if (kH16Cfgs[cfg].i8big) {
+    mixed_patterns_desc(&patP, &patA, &patW);
+    auto kern = splitk == 2 ? eri_gemm_i8_kernel<2> : eri_gemm_i8_kernel<1>;
+    static bool attr_set = false;
+    if (!attr_set) {
+      (void)hipFuncSetAttribute((const void*)eri_gemm_i8_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                I8_TILE_LDS);
+      (void)hipFuncSetAttribute((const void*)eri_gemm_i8_kernel<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                I8_TILE_LDS);
+      attr_set = true;
+    }
+    kern<<<grid, 512, I8_TILE_LDS, s>>>((const signed char*)P, ldP, kp, (const signed char*)A, ldA, (long)K, invP,
+                                        invA, C, K, nseg, patP, patA, patW, ldc, tm, bias, colmap);
+    return;
+  }
+#define H_LAUNCH(TT, B0, B1, W0, W1, ST, MB, F64, SK)                                                     \
   do {                                                                                                       \
     typedef H16Cfg<B0, B1, W0, W1, ST, MB, F64> CC;                                                          \
     auto kern = eri_gemm_h16_kernel<TT, CC, B0, B1, W0, W1, SK>;                                             \
@@ -513,14 +756,14 @@ void eri_gemm_mixed(hipStream_t s, int prec, const void* P, long kp, const void*
     kern<<<grid, CC::NT, CC::LDS, s>>>((const TT*)P, ldP, kp, (const TT*)A, ldA, (long)K, invP, invA, C, K,   \
                                        nseg, patP, patA, patW, ldc, tm, bias, colmap, h16_fold());           \
   } while (0)
-#define H_CFGS(TT, SK)                                                   \
-  do {                                                                   \
-    switch (cfg) {                                                       \
+#define H_CFGS(TT, SK)                                                          \
+  do {                                                                          \
+    switch (cfg) {                                                              \
       case 0: H_LAUNCH(TT, 128, 128, 64, 64, 2, 2, true, SK); break;     \
       case 1: H_LAUNCH(TT, 256, 128, 64, 64, 3, 1, true, SK); break;     \
       case 2: H_LAUNCH(TT, 256, 128, 128, 64, 3, 1, false, SK); break;   \
       default: H_LAUNCH(TT, 128, 256, 64, 128, 3, 1, false, SK); break;  \
-    }                                                                    \
+    }                                                                           \
   } while (0)
   if (prec == 2) {
     if (splitk == 2) H_CFGS(__bf16, 2); else H_CFGS(__bf16, 1);
@@ -535,7 +778,7 @@ void eri_gemm_mixed(hipStream_t s, int prec, const void* P, long kp, const void*
 
 // Workgroups of the persistent grouped launch for config cfg: one per CU
 // times the workgroups per CU the config's registers and LDS allow.
-int h16_grouped_grid(int cfg, int ncu) { return ncu * ((kH16Cfgs[cfg][0] == 128 && kH16Cfgs[cfg][1] == 128) ? 2 : 1); }
+int h16_grouped_grid(int cfg, int ncu) { return ncu * kH16Cfgs[cfg].minb; }
 
 // Items reference operands in the layout of eri_gemm_mixed (slices of length
 // it.K); every item's tile lies inside its operands.  atomic: every item adds
@@ -544,7 +787,23 @@ void eri_gemm_mixed_grouped(hipStream_t s, int prec, const H16Item* items, int n
                             bool atomic) {
   unsigned patP, patA, patW;
   mixed_patterns(prec, &patP, &patA, &patW);
-#define G_LAUNCH(TT, B0, B1, W0, W1, ST, MB, F64)                                                             \
+  if (kH16Cfgs[cfg].i8big) {
+    mixed_patterns_desc(&patP, &patA, &patW);
+    static bool attr_set = false;
+    if (!attr_set) {
+      (void)hipFuncSetAttribute((const void*)eri_gemm_i8_grouped_kernel<true>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, I8_TILE_LDS);
+      (void)hipFuncSetAttribute((const void*)eri_gemm_i8_grouped_kernel<false>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, I8_TILE_LDS);
+      attr_set = true;
+    }
+    if (atomic)
+      eri_gemm_i8_grouped_kernel<true><<<grid, 512, I8_TILE_LDS, s>>>(items, nitems, patP, patA, patW);
+    else
+      eri_gemm_i8_grouped_kernel<false><<<grid, 512, I8_TILE_LDS, s>>>(items, nitems, patP, patA, patW);
+    return;
+  }
+#define G_LAUNCH(TT, B0, B1, W0, W1, ST, MB, F64)                                                         \
   do {                                                                                                       \
     typedef H16Cfg<B0, B1, W0, W1, ST, MB, F64> CC;                                                          \
     auto k_at = eri_gemm_h16_grouped_kernel<TT, CC, B0, B1, W0, W1, true>;                                   \
@@ -560,16 +819,21 @@ void eri_gemm_mixed_grouped(hipStream_t s, int prec, const H16Item* items, int n
     else                                                                                                     \
       k_st<<<grid, CC::NT, CC::LDS, s>>>(items, nitems, patP, patA, patW, h16_fold());                       \
   } while (0)
-#define G_CFGS(TT)                                                      \
-  do {                                                                  \
-    switch (cfg) {                                                      \
+#define G_CFGS(TT)                                                             \
+  do {                                                                         \
+    switch (cfg) {                                                             \
       case 0: G_LAUNCH(TT, 128, 128, 64, 64, 2, 2, true); break;        \
       case 1: G_LAUNCH(TT, 256, 128, 64, 64, 3, 1, true); break;        \
       case 2: G_LAUNCH(TT, 256, 128, 128, 64, 3, 1, false); break;      \
       default: G_LAUNCH(TT, 128, 256, 64, 128, 3, 1, false); break;     \
-    }                                                                   \
+    }                                                                          \
   } while (0)
-  if (prec == 2) G_CFGS(__bf16); else if (prec == 3) G_CFGS(_Float16); else G_CFGS(signed char);
+  if (prec == 2)
+    G_CFGS(__bf16);
+  else if (prec == 3)
+    G_CFGS(_Float16);
+  else
+    G_CFGS(signed char);
 #undef G_CFGS
 #undef G_LAUNCH
 }

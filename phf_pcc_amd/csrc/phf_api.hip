@@ -438,6 +438,11 @@ int phf_ctx_init(phf_ctx* ctx, int device, hipStream_t shared_stream) {
   } else {
     HIPCK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     ctx->owns_stream = true;
+    for (int k = 0; k < 2; ++k) {
+      HIPCK(hipStreamCreateWithFlags(&ctx->side[k], hipStreamNonBlocking));
+      HIPCK(hipEventCreateWithFlags(&ctx->join_ev[k], hipEventDisableTiming));
+    }
+    HIPCK(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
   }
   return PHF_OK;
 }
@@ -477,6 +482,14 @@ extern "C" int phf_ctx_destroy(phf_ctx* ctx) {
     if (q) (void)hipFree(q);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
+  for (int k = 0; k < 2; ++k) {
+    if (ctx->side[k]) {
+      (void)hipStreamSynchronize(ctx->side[k]);
+      (void)hipStreamDestroy(ctx->side[k]);
+    }
+    if (ctx->join_ev[k]) (void)hipEventDestroy(ctx->join_ev[k]);
+  }
+  if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
   if (ctx->owns_stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return PHF_OK;
@@ -1011,17 +1024,23 @@ extern "C" int phf_set_grid(phf_ctx* ctx, const double* ref, const int* ngrid, i
 // Choose the 16-bit ERI GEMM tile configuration and split-K for one problem
 // shape: with autotuning on, every candidate runs on the live operands and the
 // fastest (HIP events, best of two after a warm-up) wins; otherwise the cost
-// model's choice.  The large-tile configurations fold into FP32 and are only
-// candidates when PHF_H16_ALLOW_F32FOLD=1.
+// model's choice.  The configurations that fold into FP32 are only candidates
+// when PHF_H16_ALLOW_F32FOLD=1; PHF_H16_CFG=<index>[,<splitk>] restricts the
+// candidates to that one configuration (and split-K).
 template <class Run>
 static int tune_h16(phf_ctx* ctx, const std::tuple<int, long, int, int>& key, long rows, int nrpad, Run&& run) {
+  const int forced = h16_forced_config();
   const bool allow_f32fold = getenv("PHF_H16_ALLOW_F32FOLD") && atoi(getenv("PHF_H16_ALLOW_F32FOLD")) == 1;
   std::vector<std::pair<int, int>> cands;
   for (int c = 0; c < h16_num_configs(); ++c) {
-    if (c >= 2 && !allow_f32fold) continue;
+    if (forced >= 0 ? c != forced : !h16_cfg_fold64(c) && !allow_f32fold) continue;
+    if (!h16_cfg_serves(c, std::get<0>(key), std::get<3>(key) / mixed_segments(std::get<0>(key)))) continue;
     if (nrpad % h16_cfg_bn(c) != 0) continue;
-    for (int sk : {1, 2}) cands.push_back({c, sk});
+    for (int sk : {1, 2})
+      if (!h16_forced_splitk() || sk == h16_forced_splitk()) cands.push_back({c, sk});
   }
+  if (cands.empty())
+    return phf_fail(ctx, PHF_ERR_UNSUPPORTED, "no 16-bit ERI GEMM configuration serves this precision and shape");
   std::sort(cands.begin(), cands.end(), [&](const std::pair<int, int>& a, const std::pair<int, int>& b) {
     return h16_model_cost(a.first, a.second, rows, nrpad, ctx->ncu) <
            h16_model_cost(b.first, b.second, rows, nrpad, ctx->ncu);
@@ -1263,9 +1282,8 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
 // eighth of each round.
 void build_h16_items(const std::vector<std::pair<const phf_ctx*, const EvalBufs*>>& units, int prec, int cfg,
                      int sk, int grid, std::vector<H16Item>& out) {
-  const int bm = h16_cfg_bm(cfg), bn = h16_cfg_bn(cfg), nseg = mixed_segments(prec), ns = mixed_slices(prec);
+  const int bm = h16_cfg_bm(cfg), bn = h16_cfg_bn(cfg), ns = mixed_slices(prec);
   const long eb = mixed_elem_bytes(prec);
-  const int bk = 128 / (int)eb;                // contraction elements per 128-byte slab
   struct Unit { int u, blk; long cost; };
   std::vector<Unit> work;
   for (size_t i = 0; i < units.size(); ++i) {
@@ -1288,7 +1306,7 @@ void build_h16_items(const std::vector<std::pair<const phf_ctx*, const EvalBufs*
     const char* Abase = (const char*)(blk ? c->dAbm[prec] : c->dAm[prec]);
     const double* invA = blk ? c->dAbinv[prec] : c->dAinv[prec];
     const int tm = (int)(round_up(2 * b.V * b.G, bm) / bm), tn = rows / bn;
-    const int part = nseg * (K / bk) / sk;
+    const int part = h16_slabs(prec, cfg, K) / sk;
     for (int nt = 0; nt < tn; ++nt)
       for (int mt = 0; mt < tm; ++mt)
         for (int ks = 0; ks < sk; ++ks) {
@@ -1337,27 +1355,51 @@ int eval_gemm(phf_ctx* ctx, const EvalBufs& b) {
   auto aoff_of = [&](int q) { return b.blk ? ctx->blk_aoff[q] : 0L; };
   auto map_of = [&](int q) { return b.blk ? ctx->d_rowmap + ctx->blk_roff[q] : (const int*)nullptr; };
   auto roff_of = [&](int q) { return b.blk ? (long)ctx->blk_roff[q] : 0L; };
+  // the blocks write disjoint columns of Gamma: for FP64, with side streams,
+  // they run concurrently (the exchange blocks fill the CUs the direct
+  // block's last round of tiles leaves idle; measured 6.41 -> 6.13 ms at
+  // n = 64), joined back before the epilogue.  The reduced forms run them one
+  // after another: concurrent blocks there compete for L2 over different
+  // operand ranges (measured slower: F16X2 1.34 -> 1.49 ms).
+  auto all_blocks = [&](auto&& run_q) -> int {
+    if (nb == 3 && ctx->side[0] && precision == PHF_PREC_FP64) {
+      HIPCK(hipEventRecord(ctx->fork_ev, s));
+      for (int k = 0; k < 2; ++k) HIPCK(hipStreamWaitEvent(ctx->side[k], ctx->fork_ev, 0));
+      run_q(0, s);
+      run_q(1, ctx->side[0]);
+      run_q(2, ctx->side[1]);
+      for (int k = 0; k < 2; ++k) {
+        HIPCK(hipEventRecord(ctx->join_ev[k], ctx->side[k]));
+        HIPCK(hipStreamWaitEvent(s, ctx->join_ev[k], 0));
+      }
+    } else {
+      for (int q = 0; q < nb; ++q) run_q(q, s);
+    }
+    return PHF_OK;
+  };
   if (precision == PHF_PREC_FP64) {
     const double* P = b.blk ? b.Pperm : b.Pt;
     const double* A = b.blk ? ctx->dAb : ctx->dA;
-    for (int q = 0; q < nb; ++q)
-      eri_gemm_f64(s, P + kbeg_of(q), b.kP, A + aoff_of(q), b.Gam, (int)b.Mp, rows_of(q), k_of(q), nrpad, b.bm,
+    return all_blocks([&](int q, hipStream_t sq) {
+      eri_gemm_f64(sq, P + kbeg_of(q), b.kP, A + aoff_of(q), b.Gam, (int)b.Mp, rows_of(q), k_of(q), nrpad, b.bm,
                    ctx->dHb, map_of(q));
+    });
   } else if (precision == PHF_PREC_FP32) {
     const float* P = (const float*)b.Pm;
     const float* A = (const float*)(b.blk ? ctx->dAbm[1] : ctx->dAm[1]);
-    for (int q = 0; q < nb; ++q)
-      eri_gemm_mixed(s, precision, P + kbeg_of(q), b.kP, A + aoff_of(q), nullptr, nullptr, b.Gam, (int)b.Mp,
+    return all_blocks([&](int q, hipStream_t sq) {
+      eri_gemm_mixed(sq, precision, P + kbeg_of(q), b.kP, A + aoff_of(q), nullptr, nullptr, b.Gam, (int)b.Mp,
                      rows_of(q), k_of(q), nrpad, 128, 1, ctx->dHb, map_of(q));
+    });
   } else {
     const int ns = mixed_slices(precision);
     const long eb = mixed_elem_bytes(precision);
     const char* P = (const char*)b.Pm;
     const char* A = (const char*)(b.blk ? ctx->dAbm[precision] : ctx->dAm[precision]);
     const double* invA = b.blk ? ctx->dAbinv[precision] : ctx->dAinv[precision];
-    auto run_block = [&](int q, int cfg, int sk) {
+    auto run_block = [&](int q, int cfg, int sk, hipStream_t sq) {
       const int mp = (int)round_up(2 * G, h16_cfg_bm(cfg));
-      eri_gemm_mixed(s, precision, P + eb * kbeg_of(q), b.kP, A + eb * ns * aoff_of(q), b.Pinv, invA + roff_of(q),
+      eri_gemm_mixed(sq, precision, P + eb * kbeg_of(q), b.kP, A + eb * ns * aoff_of(q), b.Pinv, invA + roff_of(q),
                      b.Gam, mp, rows_of(q), k_of(q), nrpad, cfg, sk, ctx->dHb, map_of(q));
     };
     std::pair<int, int> choice[3];
@@ -1368,7 +1410,7 @@ int eval_gemm(phf_ctx* ctx, const EvalBufs& b) {
       if (it == ctx->tuned.end()) {
         auto run = [&](int cfg, int sk) {
           if (sk > 1) fill_zero(s, b.Gam, (long)round_up(2 * G, h16_cfg_bm(cfg)) * nrpad);
-          run_block(q, cfg, sk);
+          run_block(q, cfg, sk, s);
         };
         int rc = tune_h16(ctx, key, 2 * G, rows_of(q), run);
         if (rc) return rc;
@@ -1379,9 +1421,8 @@ int eval_gemm(phf_ctx* ctx, const EvalBufs& b) {
     }
     // blocks write disjoint columns of Gamma: one zero fill before all of them
     if (any_split) fill_zero(s, b.Gam, b.Mp * nrpad);
-    for (int q = 0; q < nb; ++q) run_block(q, choice[q].first, choice[q].second);
+    return all_blocks([&](int q, hipStream_t sq) { run_block(q, choice[q].first, choice[q].second, sq); });
   }
-  return PHF_OK;
 }
 
 void eval_post(phf_ctx* ctx, const EvalBufs& b, bool resident) {

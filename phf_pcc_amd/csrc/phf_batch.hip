@@ -162,11 +162,18 @@ void launch_grouped(phf_batch* bt, int prec) {
 // list runs once to warm up and twice timed with HIP events, the fastest wins.
 int tune_grouped(phf_batch* bt, int prec) {
   std::vector<std::pair<int, int>> cands;
-  for (int c = 0; c < 2; ++c) {
+  const int forced = h16_forced_config();
+  for (int c = 0; c < h16_num_configs(); ++c) {
+    if (forced >= 0 ? c != forced : !h16_cfg_fold64(c)) continue;
     bool fits = true;   // A rows (dense or per block) are multiples of 256
-    for (phf_ctx* x : bt->inst) fits = fits && (x->nrpad % h16_cfg_bn(c) == 0);
+    for (phf_ctx* x : bt->inst) {
+      fits = fits && (x->nrpad % h16_cfg_bn(c) == 0);
+      for (int q = 0; q < 3; ++q)   // every contraction length of the instance (dense, or each block)
+        fits = fits && h16_cfg_serves(c, prec, (x->blocked && x->use_blocks) ? x->blk_k[q] : x->kt);
+    }
     if (!fits) continue;
-    for (int sk : {1, 2}) cands.push_back({c, sk});
+    for (int sk : {1, 2})
+      if (!h16_forced_splitk() || sk == h16_forced_splitk()) cands.push_back({c, sk});
   }
   if (cands.empty()) return bfail(bt, PHF_ERR_UNSUPPORTED, "phf_batch: no grouped tile config fits");
   std::pair<int, int> best = cands.front();

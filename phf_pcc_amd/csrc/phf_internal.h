@@ -32,6 +32,11 @@ int h16_grouped_grid(int cfg, int ncu);
 int h16_num_configs();
 int h16_cfg_bm(int c);
 int h16_cfg_bn(int c);
+bool h16_cfg_fold64(int c);
+bool h16_cfg_serves(int c, int prec, int K);
+int h16_slabs(int prec, int c, int K);
+int h16_forced_config();
+int h16_forced_splitk();
 double h16_model_cost(int c, int sk, long Mp0, int Nr0, int ncu);
 
 struct phf_ctx_s {
@@ -39,6 +44,10 @@ struct phf_ctx_s {
   int ncu = 256;
   hipStream_t stream = nullptr;
   bool owns_stream = true;    // false for the instances of a batch (they share the batch's stream)
+  // a context with its own stream runs the three spin-block GEMMs of an
+  // evaluation concurrently: two side streams, fork/join events
+  hipStream_t side[2] = {};
+  hipEvent_t fork_ev = nullptr, join_ev[2] = {};
   std::string err;
   // cached Hamiltonian
   int nso = 0;

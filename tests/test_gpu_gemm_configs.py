@@ -1,0 +1,117 @@
+"""Every tile configuration of the reduced-precision ERI GEMM, pinned one at a
+time (PHF_H16_CFG=<index>,<splitk>), against the FP64 contraction.
+
+Configurations 0-3 walk the product segments one after another (every 16-bit
+form and I8X4; 2 and 3 fold into FP32 and are candidates only when pinned or
+allowed); 4 is the 256 x 256 exact int8 tile (I8X4 only; one exact int32 sum
+per product class, added to Gamma in FP64).  Checks, on a spin blocked and a
+dense instance and on the grouped batch launch: |dE| <= 1e-6 Eh and max |dG|
+<= 1e-5 against FP64; the int8 tile against the FP64-fold int8 segment walk,
+|dE| <= 1e-9 and |dG| <= 1e-8 (both exact up to the dropped classes);
+configurations refuse the forms they do not serve.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from phf_pcc_amd.hamiltonian import synthetic_problem
+from tests._cases import grid_args
+
+pytestmark = pytest.mark.gpu
+
+SEGMENT_CFGS = [0, 1, 2, 3]
+I8_TILE = 4
+
+
+@pytest.fixture(scope="module")
+def prob():
+    p = synthetic_problem(16, 16, seed=77, zscale=0.05)
+    return p, grid_args(16, (14, 8), 0)
+
+
+class _pinned:
+    def __init__(self, cfg, sk):
+        self.v = f"{cfg},{sk}"
+
+    def __enter__(self):
+        os.environ["PHF_H16_CFG"] = self.v
+
+    def __exit__(self, *a):
+        os.environ.pop("PHF_H16_CFG", None)
+
+
+def _eval(prob, prec, structure=True, cfg=None, sk=1):
+    from phf_pcc_amd.model import ProjectedHF
+    p, ga = prob
+    m = ProjectedHF(p.nocc, p.ref, ga, factors=(p.h_spatial, p.b), structure=structure, precision=prec)
+    if cfg is None:
+        return m.energy_and_gradient(p.z)
+    with _pinned(cfg, sk):
+        out = m.energy_and_gradient(p.z)
+    assert f'"cfg": {cfg}, "splitk": {sk}' in m.ctx.tuning_report()
+    return out
+
+
+@pytest.fixture(scope="module")
+def fp64(prob):
+    return {st: _eval(prob, "fp64", structure=st) for st in (True, False)}
+
+
+@pytest.mark.parametrize("structure", [True, False])
+@pytest.mark.parametrize("sk", [1, 2])
+@pytest.mark.parametrize("cfg", SEGMENT_CFGS)
+def test_f16x2_config(prob, fp64, cfg, sk, structure):
+    e64, g64 = fp64[structure]
+    e, g = _eval(prob, "f16x2", structure, cfg, sk)
+    assert abs(e - e64) <= 1e-6, (cfg, sk, e - e64)
+    assert np.abs(g - g64).max() <= 1e-5
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "i8x4"])
+@pytest.mark.parametrize("cfg", SEGMENT_CFGS)
+def test_other_forms_config(prob, fp64, prec, cfg):
+    e64, g64 = fp64[True]
+    e, g = _eval(prob, prec, True, cfg, 1)
+    assert abs(e - e64) <= 1e-6, (prec, cfg, e - e64)
+    assert np.abs(g - g64).max() <= 1e-5
+
+
+def test_configs_refuse_other_forms(prob):
+    from phf_pcc_amd import _lib
+    for prec, cfg in [("f16x2", I8_TILE), ("bf16x3", I8_TILE)]:
+        with pytest.raises(_lib.PhfError):
+            _eval(prob, prec, True, cfg, 1)
+
+
+@pytest.mark.parametrize("structure", [True, False])
+@pytest.mark.parametrize("sk", [1, 2])
+def test_int8_tile(prob, fp64, sk, structure):
+    e64, g64 = fp64[structure]
+    e, g = _eval(prob, "i8x4", structure, I8_TILE, sk)
+    assert abs(e - e64) <= 1e-6, (sk, e - e64)
+    assert np.abs(g - g64).max() <= 1e-5
+    e0, g0 = _eval(prob, "i8x4", structure, 0, 1)
+    assert abs(e - e0) <= 1e-9
+    assert np.abs(g - g0).max() <= 1e-8
+
+
+@pytest.mark.parametrize("prec,cfg,sk", [("f16x2", 0, 1), ("f16x2", 0, 2), ("f16x2", 1, 1), ("bf16x3", 0, 1),
+                                         ("i8x4", I8_TILE, 1), ("i8x4", I8_TILE, 2)])
+def test_grouped_batch_config(prec, cfg, sk):
+    from phf_pcc_amd.batch import ProjectedHFBatch
+    inst, zs = [], []
+    for n, ngrid, seed in [(8, (6, 4), 21), (12, (14, 8), 22), (16, (26, 8), 23)]:
+        p = synthetic_problem(n, n // 2, seed=seed, zscale=0.05)
+        ga = grid_args(n, ngrid, 0)
+        inst.append(dict(h=p.h_spatial, b=p.b, ref=p.ref, nocc=p.nocc, ngrid=ga["ngrid"], r1=ga["r1"], r2=ga["r2"],
+                         rpg=ga["rpg"], weightsp=ga["weightsp"]))
+        zs.append(p.z)
+    bt = ProjectedHFBatch(inst, storage="fp64")
+    e64, g64, _ = bt.energy_and_gradient(zs, "fp64")
+    with _pinned(cfg, sk):
+        e, g, _ = bt.energy_and_gradient(zs, prec)
+    assert f'"chosen": {{"cfg": {cfg}, "splitk": {sk}' in bt.tuning_report()
+    assert np.abs(e - e64).max() <= 1e-6
+    for a, b in zip(g, g64):
+        assert np.abs(a - b).max() <= 1e-5
