@@ -443,6 +443,8 @@ int phf_ctx_init(phf_ctx* ctx, int device, hipStream_t shared_stream) {
       HIPCK(hipEventCreateWithFlags(&ctx->join_ev[k], hipEventDisableTiming));
     }
     HIPCK(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+    HIPCK(hipEventCreateWithFlags(&ctx->hc_fork, hipEventDisableTiming));
+    HIPCK(hipEventCreateWithFlags(&ctx->hc_join, hipEventDisableTiming));
   }
   return PHF_OK;
 }
@@ -489,7 +491,8 @@ extern "C" int phf_ctx_destroy(phf_ctx* ctx) {
     }
     if (ctx->join_ev[k]) (void)hipEventDestroy(ctx->join_ev[k]);
   }
-  if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+  for (hipEvent_t e : {ctx->fork_ev, ctx->hc_fork, ctx->hc_join})
+    if (e) (void)hipEventDestroy(e);
   if (ctx->owns_stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return PHF_OK;
@@ -1202,6 +1205,15 @@ int eval_plan(phf_ctx* ctx, int nocc, int precision, int cmplxconj, EvalBufs& b)
 static inline int variant_left(int v) { return (v == 1 || v == 2) ? 1 : 0; }
 static inline int variant_right(int v) { return (v == 2 || v == 3) ? 1 : 0; }
 
+// Hc_B = B^H h for each right factor B (C0, and C0* when V = 4), N x NSO
+static void hc_products(phf_ctx* ctx, const EvalBufs& b, hipStream_t s) {
+  const int nso = ctx->nso, N = b.N, nB = (b.V == 4) ? 2 : 1;
+  const long sT = 2L * nso * N;
+  for (int bc = 0; bc < nB; ++bc)
+    zgemm_batched(s, N, nso, nso, 1, 0, ZMat{b.C0, 2L * nso, 2, 0, 1, bc ? 0 : 1},
+                  ZMat{ctx->dH1, 2L * nso, 2, 0, 1, 0}, 0, 0, zout_colmajor(b.Hc + bc * sT, N, 0), 1);
+}
+
 void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
   hipStream_t s = ctx->stream;
   const int nso = ctx->nso, N = b.N, npg = ctx->g_npg, ny = ctx->g_ny, nl = ctx->g_nl, V = b.V;
@@ -1210,6 +1222,14 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
   const long sT = 2 * lenX;          // batch stride of NSO x N column-major blocks
   const long sNN = 2L * N * N;
   const int nL = (V == 4) ? 2 : 1;
+  // Hc = B^H h per right factor (used by the energy in eval_post): on a side
+  // stream, overlapping the pre-stages, when the context has one
+  if (ctx->side[0]) {
+    (void)hipEventRecord(ctx->hc_fork, s);
+    (void)hipStreamWaitEvent(ctx->side[0], ctx->hc_fork, 0);
+    hc_products(ctx, b, ctx->side[0]);
+    (void)hipEventRecord(ctx->hc_join, ctx->side[0]);
+  }
   // K1: rotation chain, right to left (Ref^H R1 and Rpg Ref are cached with the
   // grid), once per left factor L = C0 (lc = 0) or C0* (lc = 1)
   for (int lc = 0; lc < nL; ++lc) {
@@ -1242,7 +1262,12 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
   // density pack: Pt row 2gv = Re rho_gv^T, row 2gv+1 = Im rho_gv^T (flattened
   // j*nso + l); rho^T = conj(B) X^T, one launch per right factor (variants
   // with B = C0 come first, then B = C0*)
-  fill_zero(s, b.Pt, b.Mp * kt);
+  // the pack writes columns [0, NSO^2) of rows [0, 2 V G): clear the padding
+  // rows, or everything when the rows are padded too (or complex-packed)
+  if (ctx->cplx || kt != n2)
+    fill_zero(s, b.Pt, b.Mp * kt);
+  else
+    fill_zero(s, b.Pt + 2 * V * G * (long)kt, (b.Mp - 2 * V * G) * (long)kt);
   const int nB = (V == 4) ? 2 : 1;
   const long per = (V == 4) ? 2 * G : G;
   for (int bc = 0; bc < nB; ++bc)
@@ -1436,13 +1461,17 @@ void eval_post(phf_ctx* ctx, const EvalBufs& b, bool resident) {
   // K4: FX = (h + Gamma) X   (the ERI GEMM epilogue added h)
   zgemm_batched(s, nso, N, nso, 1, 0, ZMat{b.Gam, nso, 1, 2L * nrpad, nrpad, 0}, zmat_colmajor(b.X, nso, sT), 0, 0,
                 zout_colmajor(b.FX, nso, sT), (int)Gv);
-  // per right factor B: Q_v = B^H FX_v, Hc = B^H h, e_v = Makeh
+  // per right factor B: Q_v = B^H FX_v, Hc = B^H h (computed by eval_pre on a
+  // side stream when the context has one), e_v = Makeh
+  if (ctx->side[0])
+    (void)hipStreamWaitEvent(s, ctx->hc_join, 0);
+  else
+    hc_products(ctx, b, s);
   for (int bc = 0; bc < nB; ++bc) {
     const ZMat bh{b.C0, 2L * nso, 2, 0, 1, bc ? 0 : 1};
     double* Hc = b.Hc + bc * sT;
     zgemm_batched(s, N, N, nso, 1, 0, bh, zmat_colmajor(b.FX + bc * per * sT, nso, sT), 0, 0,
                   zout_colmajor(b.Q + bc * per * sNN, N, sNN), (int)per);
-    zgemm_batched(s, N, nso, nso, 1, 0, bh, ZMat{ctx->dH1, 2L * nso, 2, 0, 1, 0}, 0, 0, zout_colmajor(Hc, N, 0), 1);
     energy_kernel<<<(unsigned)per, 256, 0, s>>>(nso, N, Hc, b.X + bc * per * sT, b.Q + bc * per * sNN,
                                                 b.E + 2 * bc * per);
   }

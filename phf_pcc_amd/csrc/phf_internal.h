@@ -48,6 +48,8 @@ struct phf_ctx_s {
   // evaluation concurrently: two side streams, fork/join events
   hipStream_t side[2] = {};
   hipEvent_t fork_ev = nullptr, join_ev[2] = {};
+  // Hc = C0^H h runs on side[0] during the pre-stages (hc_fork, hc_join)
+  hipEvent_t hc_fork = nullptr, hc_join = nullptr;
   std::string err;
   // cached Hamiltonian
   int nso = 0;
