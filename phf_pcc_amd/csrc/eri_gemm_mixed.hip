@@ -524,34 +524,45 @@ template <> __device__ inline signed char to16<signed char>(double x) { return (
 // s*x (x_0 the rounding of s*x, x_1 of the remainder, ...), stored as
 // [x_0 | x_1 | ...] along the row, each kout long.  With cmap, output column c
 // takes source column cmap[c] (0 for -1); without, kout = kt and c maps to c.
-template <typename T>
+// amax (optional): the rows' largest magnitudes, known already (one pass over
+// the row instead of two); rows >= rows_data are zero and are not read.
+template <typename T, int NS>
 __global__ __launch_bounds__(256) void split_rows_kernel(const double* __restrict__ src, long lds, int kt,
                                                          const int* __restrict__ cmap, int kout,
-                                                         T* __restrict__ dst, int nslice, int top,
-                                                         double* __restrict__ inv) {
+                                                         T* __restrict__ dst, int top,
+                                                         double* __restrict__ inv, const double* __restrict__ amax,
+                                                         long rows_data) {
   __shared__ double red[256];
   const long row = blockIdx.x;
+  const bool data = row < rows_data;
   const double* x = src + row * lds;
   double m = 0.0;
-  for (int c = threadIdx.x; c < kt; c += 256) m = fmax(m, fabs(x[c]));
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+  if (amax) {
+    m = data ? amax[row] : 0.0;
+  } else {
+    for (int c = threadIdx.x; c < kt; c += 256) m = fmax(m, fabs(x[c]));
+    red[threadIdx.x] = m;
     __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+      __syncthreads();
+    }
+    m = red[0];
   }
-  m = red[0];
   int e = 0;
   if (m > 0.0) {
     frexp(m, &e);             // m = f * 2^e with f in [0.5, 1)
   }
   const double sc = (m > 0.0) ? ldexp(1.0, top - e) : 1.0;
   if (threadIdx.x == 0) inv[row] = 1.0 / sc;
-  T* d = dst + row * (long)nslice * kout;
+  // consecutive lanes take consecutive columns: the (gathered) reads and the
+  // slice stores stay coalesced
+  T* d = dst + row * (long)NS * kout;
   for (int c = threadIdx.x; c < kout; c += 256) {
     const int sc_col = cmap ? cmap[c] : c;
-    double r = (sc_col >= 0 ? x[sc_col] : 0.0) * sc;
-    for (int q = 0; q < nslice; ++q) {
+    double r = (data && sc_col >= 0 ? x[sc_col] : 0.0) * sc;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
       if constexpr (std::is_same<T, signed char>::value) {
         // integer slices: x_q = rint(r) in [-127, 127], carry 128 (r - x_q)
         const double v = fmin(127.0, fmax(-127.0, rint(r)));
@@ -621,17 +632,19 @@ static void mixed_patterns(int prec, unsigned* patP, unsigned* patA, unsigned* p
 }
 
 void split_operand_map(hipStream_t s, int prec, const double* src, long lds, int rows, int kt, const int* cmap,
-                       int kout, void* dst, double* inv) {
+                       int kout, void* dst, double* inv, const double* amax, int rows_data) {
+  const long rd = rows_data < 0 ? rows : rows_data;
   if (prec == 3)
-    split_rows_kernel<_Float16><<<rows, 256, 0, s>>>(src, lds, kt, cmap, kout, (_Float16*)dst, 2, 14, inv);
+    split_rows_kernel<_Float16, 2><<<rows, 256, 0, s>>>(src, lds, kt, cmap, kout, (_Float16*)dst, 14, inv, amax, rd);
   else if (prec == 2)
-    split_rows_kernel<__bf16><<<rows, 256, 0, s>>>(src, lds, kt, cmap, kout, (__bf16*)dst, 3, 14, inv);
+    split_rows_kernel<__bf16, 3><<<rows, 256, 0, s>>>(src, lds, kt, cmap, kout, (__bf16*)dst, 14, inv, amax, rd);
   else
-    split_rows_kernel<signed char><<<rows, 256, 0, s>>>(src, lds, kt, cmap, kout, (signed char*)dst, 4, 7, inv);
+    split_rows_kernel<signed char, 4><<<rows, 256, 0, s>>>(src, lds, kt, cmap, kout, (signed char*)dst, 7, inv,
+                                                           amax, rd);
 }
 
 void split_operand(hipStream_t s, int prec, const double* src, long lds, int rows, int kt, void* dst, double* inv) {
-  split_operand_map(s, prec, src, lds, rows, kt, nullptr, kt, dst, inv);
+  split_operand_map(s, prec, src, lds, rows, kt, nullptr, kt, dst, inv, nullptr, -1);
 }
 
 void convert_f32(hipStream_t s, const double* src, float* dst, long n) {

@@ -1179,6 +1179,7 @@ int eval_plan(phf_ctx* ctx, int nocc, int precision, int cmplxconj, EvalBufs& b)
     const size_t pm = (precision == PHF_PREC_FP64) ? 0 : (size_t)Mp * kP * mixed_slices(precision);
     o.Pm = a.take<double>((pm * mixed_elem_bytes(precision) + 7) / 8);
     o.Pinv = a.take<double>(precision == PHF_PREC_FP64 ? 0 : (size_t)Mp);
+    o.Amax = a.take<double>((precision >= PHF_PREC_BF16X3 && !ctx->cplx) ? (size_t)(2 * Gv) : 0);
     o.Pperm = a.take<double>((blk && precision <= PHF_PREC_FP32) ? (size_t)Mp * kP : 0);
   };
   Arena a0{nullptr};
@@ -1270,10 +1271,14 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
     fill_zero(s, b.Pt + 2 * V * G * (long)kt, (b.Mp - 2 * V * G) * (long)kt);
   const int nB = (V == 4) ? 2 : 1;
   const long per = (V == 4) ? 2 * G : G;
-  for (int bc = 0; bc < nB; ++bc)
+  // the reduced forms take the rows' largest magnitudes from the pack itself
+  if (b.Amax) fill_zero(s, b.Amax, 2 * V * G);
+  for (int bc = 0; bc < nB; ++bc) {
+    ZOut out{b.Pt + bc * per * 2L * kt, nso, 1, 2L * kt, kt};
+    if (b.Amax) out.amax = reinterpret_cast<unsigned long long*>(b.Amax + bc * per * 2);
     zgemm_batched(s, nso, nso, N, 1, 0, ZMat{b.C0, 2, 2L * nso, 0, 1, bc ? 0 : 1},
-                  ZMat{b.X + bc * per * sT, 2L * nso, 2, sT, 1, 0}, 0, 0,
-                  ZOut{b.Pt + bc * per * 2L * kt, nso, 1, 2L * kt, kt}, (int)per);
+                  ZMat{b.X + bc * per * sT, 2L * nso, 2, sT, 1, 0}, 0, 0, out, (int)per);
+  }
   if (ctx->cplx) {
     dim3 grid((unsigned)std::min<long>((n2 + 255) / 256, 1024), (unsigned)(V * G));
     pack_cplx_kernel<<<grid, 256, 0, s>>>(b.Pt, (int)(V * G), (int)n2, ctx->kpad, kt);
@@ -1293,7 +1298,7 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
       convert_f32(s, b.Pt, (float*)b.Pm, b.Mp * kt);
     }
   } else {
-    split_operand_map(s, b.prec, b.Pt, kt, (int)b.Mp, kt, cmap, (int)b.kP, b.Pm, b.Pinv);
+    split_operand_map(s, b.prec, b.Pt, kt, (int)b.Mp, kt, cmap, (int)b.kP, b.Pm, b.Pinv, b.Amax, (int)(2 * V * G));
   }
 }
 

@@ -27,6 +27,10 @@ struct ZMat {
 struct ZOut {
   double* p;
   long rs, cs, bs, im;
+  // optional: per batch entry b, amax[2 b] / amax[2 b + 1] receive (atomic max
+  // of the bit patterns) the largest |Re| / |Im| written, for the row scales
+  // of the reduced-precision split; zero-filled by the caller
+  unsigned long long* amax = nullptr;
 };
 
 // Host-side helpers for interleaved complex128 storage.

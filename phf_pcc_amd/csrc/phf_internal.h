@@ -21,7 +21,7 @@ int mixed_slices(int prec);
 int mixed_elem_bytes(int prec);
 void split_operand(hipStream_t s, int prec, const double* src, long lds, int rows, int kt, void* dst, double* inv);
 void split_operand_map(hipStream_t s, int prec, const double* src, long lds, int rows, int kt, const int* cmap,
-                       int kout, void* dst, double* inv);
+                       int kout, void* dst, double* inv, const double* amax = nullptr, int rows_data = -1);
 void convert_f32(hipStream_t s, const double* src, float* dst, long n);
 void eri_gemm_mixed(hipStream_t s, int prec, const void* P, long kp, const void* A, const double* invP,
                     const double* invA, double* C, int Mp, int Nr, int K, long ldc, int cfg, int splitk,
@@ -119,7 +119,8 @@ struct Arena {
   char* base;
   size_t off = 0;
   template <class T>
-  T* take(size_t count) {
+  T* take(size_t count) {   // nullptr for an empty buffer
+    if (count == 0) return nullptr;
     off = (off + 255) & ~size_t(255);
     T* p = reinterpret_cast<T*>(base ? base + off : nullptr);
     off += count * sizeof(T);
@@ -163,6 +164,7 @@ struct EvalBufs {
   double *Grad = nullptr, *Rdm = nullptr;
   void* Pm = nullptr;
   double* Pinv = nullptr;
+  double* Amax = nullptr;     // per density row, its largest magnitude (bit patterns; reduced forms)
   bool blk = false;           // the GEMM runs over the spin blocks
   long kP = 0;                // row length (per slice) of the GEMM's density operand
   double* Pperm = nullptr;    // FP64 density rows in permuted column order (blk, FP64 form)

@@ -106,6 +106,7 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double 
   }
 
   const bool use_beta = (br != 0.0 || bi != 0.0);
+  double mre = 0.0, mim = 0.0;
 #pragma unroll
   for (int i = 0; i < FMW; ++i)
 #pragma unroll
@@ -125,8 +126,21 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double 
           }
           C.p[off] = vr;
           C.p[off + C.im] = vi;
+          mre = fmax(mre, fabs(vr));
+          mim = fmax(mim, fabs(vi));
         }
       }
+  if (C.amax) {   // non-negative doubles order like their bit patterns
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      mre = fmax(mre, __shfl_xor(mre, off));
+      mim = fmax(mim, __shfl_xor(mim, off));
+    }
+    if (lane == 0) {
+      atomicMax(&C.amax[2 * b], (unsigned long long)__double_as_longlong(mre));
+      atomicMax(&C.amax[2 * b + 1], (unsigned long long)__double_as_longlong(mim));
+    }
+  }
 }
 
 }  // namespace
