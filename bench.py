@@ -238,6 +238,8 @@ def run_large(args):
     ok = [f for f in res if out_forms[f]["abs_dE_vs_fp64"] <= 1e-6]
     best = max(ok, key=lambda f: res[f]["value"])
     bprec = res[best]["prec"]
+    kernel = gemm_kernel_name(bprec, model.ctx.tuning_report())
+    traffic = load_traffic(kernel, f"{n}-blocks" if res[best]["blocked"] else n)
     nprod = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3, "i8x4": 10}[bprec]
     flops = flops_blk if res[best]["blocked"] else flops_dense
     t_gemm = res[best]["gemm_ms"] * 1e-3
@@ -249,8 +251,12 @@ def run_large(args):
                                   "(seed 1003), device-resident objective", "form": best, "forms": out_forms},
            "roofline": {"bound": "mfma", "achieved": nprod * flops / t_gemm / 1e12, "peak": PEAK[bprec] / 1e12,
                         "unit": "TFLOP/s", "frac": nprod * flops / t_gemm / PEAK[bprec], "traffic": None,
-                        "algorithmic_flops_per_launch": flops, "spin_blocks": res[best]["blocked"]},
+                        "kernel": kernel, "algorithmic_flops_per_launch": flops, "spin_blocks": res[best]["blocked"],
+                        "launches_per_step": 3 if res[best]["blocked"] else 1},
            "cpu_baseline": None}
+    if traffic is not None:
+        out["roofline"]["traffic"] = traffic["bytes_per_launch"]
+        out["roofline"]["traffic_source"] = traffic["source"]
     print(json.dumps(out), flush=True)
 
 
@@ -345,6 +351,25 @@ def cpu_blas_gemm(prob, G):
     return {"value": G / t1, "unit": "grid pts/s", "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
             "kind": "blas", "sample": f"numpy BLAS dgemm {n2} x {n2} x {2 * G} (all {G} grid points) in {t1:.2f} s; "
                                       "ERI GEMM only"}
+
+
+def gemm_kernel_name(prec, report):
+    """Kernel of the ERI GEMM of form `prec`: for I8X4 the tuner's choice on the
+    largest block (configuration 4 is the 256 x 256 int8 tile)."""
+    names = {"fp64": "eri_gemm_f64_kernel", "fp32": "eri_gemm_f32_kernel", "bf16x3": "eri_gemm_h16_kernel<bf16>",
+             "f16x2": "eri_gemm_h16_kernel<f16>", "i8x4": "eri_gemm_h16_kernel<i8>"}
+    if prec == "i8x4":
+        best_cols, cfg = -1, None
+        for line in (report or "").splitlines():
+            try:
+                rec = json.loads(line)
+            except ValueError:
+                continue
+            if rec.get("prec") == 4 and "cols" in rec and rec["cols"] > best_cols:
+                best_cols, cfg = rec["cols"], rec["chosen"]["cfg"]
+        if cfg == 4:
+            return "eri_gemm_i8_kernel"
+    return names[prec]
 
 
 def load_traffic(kernel, n):
@@ -493,9 +518,7 @@ def main():
     t_gemm = best["gemm_ms"] * 1e-3
     roof = {"bound": "mfma", "achieved": nprod * flops / t_gemm / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
             "frac": nprod * flops / t_gemm / peak, "traffic": None,
-            "kernel": {"fp64": "eri_gemm_f64_kernel", "fp32": "eri_gemm_f32_kernel",
-                       "bf16x3": "eri_gemm_h16_kernel<bf16>", "f16x2": "eri_gemm_h16_kernel<f16>",
-                       "i8x4": "eri_gemm_h16_kernel<i8>"}[best["prec"]],
+            "kernel": gemm_kernel_name(best["prec"], ctx.tuning_report()),
             "flops_per_launch": nprod * flops, "algorithmic_flops_per_launch": flops, "spin_blocks": blocked,
             "launches_per_step": 3 if blocked else 1,
             "time_source": "HIP events around the ERI GEMM stage on the library stream (the launches of one step: "
