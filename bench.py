@@ -354,21 +354,24 @@ def cpu_blas_gemm(prob, G):
 
 
 def gemm_kernel_name(prec, report):
-    """Kernel of the ERI GEMM of form `prec`: for I8X4 the tuner's choice on the
-    largest block (configuration 4 is the 256 x 256 int8 tile)."""
+    """Kernel of the ERI GEMM of form `prec` on its largest block, from the
+    tuner's choice: configuration 4 is the 256 x 256 int8 tile, 5 the 256 x 256
+    round-to-nearest 16-bit tile, the others the 128-row fold-tile kernels."""
     names = {"fp64": "eri_gemm_f64_kernel", "fp32": "eri_gemm_f32_kernel", "bf16x3": "eri_gemm_h16_kernel<bf16>",
              "f16x2": "eri_gemm_h16_kernel<f16>", "i8x4": "eri_gemm_h16_kernel<i8>"}
-    if prec == "i8x4":
-        best_cols, cfg = -1, None
-        for line in (report or "").splitlines():
-            try:
-                rec = json.loads(line)
-            except ValueError:
-                continue
-            if rec.get("prec") == 4 and "cols" in rec and rec["cols"] > best_cols:
-                best_cols, cfg = rec["cols"], rec["chosen"]["cfg"]
-        if cfg == 4:
-            return "eri_gemm_i8_kernel"
+    code = {"bf16x3": 2, "f16x2": 3, "i8x4": 4}.get(prec)
+    best_cols, cfg = -1, None
+    for line in (report or "").splitlines():
+        try:
+            rec = json.loads(line)
+        except ValueError:
+            continue
+        if code is not None and rec.get("prec") == code and "cols" in rec and rec["cols"] > best_cols:
+            best_cols, cfg = rec["cols"], rec["chosen"]["cfg"]
+    if prec == "i8x4" and cfg == 4:
+        return "eri_gemm_i8_kernel"
+    if prec in ("f16x2", "bf16x3") and cfg == 5:
+        return "eri_gemm_rn16_kernel<%s>" % ("f16" if prec == "f16x2" else "bf16")
     return names[prec]
 
 

@@ -396,6 +396,140 @@ void eri_gemm_i8_grouped_kernel(const H16Item* __restrict__ items, int nitems, u
   }
 }
 
+// 16-bit forms on the 256 x 256 tile (eight 128 x 64 waves, two per SIMD,
+// one workgroup per CU), without an FP64 fold tile.  Each pair of MFMAs (the
+// two 32-element k-steps of a slab) starts from C = 0, so the instruction's
+// truncating summation acts on a 64-product partial only; the partial is
+// added to an FP32 accumulator with a round-to-nearest VALU add.  The 16-bit
+// MFMA otherwise truncates against the running sum (a bias that grows with
+// the contraction); here the per-step error is unbiased and relative to the
+// partial.  Freeing the 128 registers of the FP64 fold tile is what lets a
+// wave hold 128 x 64, which halves the operand bytes per product of the
+// 128 x 128 tile.  Segments as in h16_tile (all weights 1); operand layout,
+// LDS ring and swizzle as in i8_tile.
+template <typename T, int STAGES>
+__device__ __forceinline__ void rn16_tile(const T* __restrict__ P, long ldP, long sP, const T* __restrict__ A,
+                                          long ldA, long sA, const double* __restrict__ invP,
+                                          const double* __restrict__ invA, double* __restrict__ C,
+                                          const double* __restrict__ bias, const int* __restrict__ colmap, long ldc,
+                                          int K, unsigned patP, unsigned patA, int mt, int nt, int sbeg, int send,
+                                          bool atomic, unsigned char* smem) {
+  constexpr int BM = 256, BN = 256, WM = 128, WN = 64, BK = 64;
+  constexpr int FM = WM / 16, FN = WN / 16;
+  constexpr int STAGE_BYTES = (BM + BN) * 128;
+  constexpr int BPW = 8;                      // 1 KB blocks (8 rows) per wave per slab
+  typedef typename MfmaH<T>::v8 v8;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave >> 2, wn = wave & 3;
+  const int nk = send - sbeg, nsl = K / BK;
+
+  // wave w loads slab rows 64 w .. 64 w + 63: waves 0-3 the P rows, 4-7 the A rows
+  const bool isP = wave < 4;
+  const long ld = isP ? ldP : ldA, sl = isP ? sP : sA;
+  const T* wbase = isP ? P + ((long)mt * BM + (wave & 3) * 64) * ldP : A + ((long)nt * BN + (wave & 3) * 64) * ldA;
+  const int rl = lane >> 3;
+  const unsigned off_even = (unsigned)(rl * ld) + 8u * ((lane & 7) ^ ((rl >> 1) & 7));
+  const unsigned off_odd = (unsigned)(rl * ld) + 8u * ((lane & 7) ^ ((4 + (rl >> 1)) & 7));
+
+  int iseg = sbeg / nsl, ioff = sbeg % nsl;
+  auto issue = [&](int slab) {
+    unsigned char* st = smem + (slab % STAGES) * STAGE_BYTES + wave * BPW * 1024;
+    const T* b = wbase + (long)((isP ? (patP >> (3 * iseg)) : (patA >> (3 * iseg))) & 7) * sl + ioff * BK;
+#pragma unroll
+    for (int q = 0; q < BPW; ++q)
+      __builtin_amdgcn_global_load_lds((glb_void_t*)(b + (long)(8 * q) * ld + ((q & 1) ? off_odd : off_even)),
+                                       (lds_void_t*)(st + q * 1024), 16, 0, 0);
+    if (++ioff == nsl) { ioff = 0; ++iseg; }
+  };
+  auto frag = [&](const unsigned char* st, int r, int c) -> v8 {
+    return *reinterpret_cast<const v8*>(st + (r >> 3) * 1024 + ((r & 7) * 8 + (c ^ ((r >> 1) & 7))) * 16);
+  };
+
+  f4_t acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+
+#pragma unroll
+  for (int sl0 = 0; sl0 < STAGES - 1; ++sl0)
+    if (sl0 < nk) issue(sl0);
+  for (int kt = 0; kt < nk; ++kt) {
+    if (STAGES == 3 && kt + 1 < nk) PHF_WAIT_VMCNT(BPW);
+    else PHF_WAIT_VMCNT(0);
+    __builtin_amdgcn_s_barrier();
+    if (kt + STAGES - 1 < nk) issue(kt + STAGES - 1);
+    const unsigned char* st = smem + (kt % STAGES) * STAGE_BYTES;
+    v8 a0[FM], a1[FM];
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+      a0[i] = frag(st, wm * WM + i * 16 + (lane & 15), lane >> 4);
+      a1[i] = frag(st, wm * WM + i * 16 + (lane & 15), 4 + (lane >> 4));
+    }
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const v8 b0 = frag(st, BM + wn * WN + j * 16 + (lane & 15), lane >> 4);
+      const v8 b1 = frag(st, BM + wn * WN + j * 16 + (lane & 15), 4 + (lane >> 4));
+      f4_t t[FM];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) t[i] = MfmaH<T>::mma(a0[i], b0, f4_t{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        t[i] = MfmaH<T>::mma(a1[i], b1, t[i]);
+        acc[i][j] += t[i];
+      }
+    }
+  }
+
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long row = (long)mt * BM + wm * WM + i * 16 + 4 * (lane >> 4) + r;
+        const long ar = (long)nt * BN + wn * WN + j * 16 + (lane & 15);
+        const long col = colmap ? colmap[ar] : ar;
+        if (col < 0) continue;
+        double v = (double)acc[i][j][r] * invP[row] * invA[ar];
+        if (bias) v += bias[(row & 1) * ldc + col];
+        if (atomic)
+          atomicAdd(&C[row * ldc + col], v);
+        else
+          C[row * ldc + col] = v;
+      }
+}
+
+template <typename T, int SPLITK>
+__global__ __launch_bounds__(512, 1)
+void eri_gemm_rn16_kernel(const T* __restrict__ P, long ldP, long sP, const T* __restrict__ A, long ldA, long sA,
+                          const double* __restrict__ invP, const double* __restrict__ invA, double* __restrict__ C,
+                          int K, int nseg, unsigned patP, unsigned patA, long ldc, int tiles_m,
+                          const double* __restrict__ bias, const int* __restrict__ colmap) {
+  extern __shared__ __attribute__((aligned(1024))) unsigned char smem[];
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int ks_id = t % SPLITK;
+  const int tile = t / SPLITK;
+  const int mt = tile % tiles_m, nt = tile / tiles_m;
+  const int part = nseg * (K / 64) / SPLITK;
+  rn16_tile<T, I8_TILE_STAGES>(P, ldP, sP, A, ldA, sA, invP, invA, C, ks_id == 0 ? bias : nullptr, colmap, ldc, K,
+                               patP, patA, mt, nt, ks_id * part, (ks_id + 1) * part, SPLITK > 1, smem);
+}
+
+template <typename T, bool ATOMIC>
+__global__ __launch_bounds__(512, 1)
+void eri_gemm_rn16_grouped_kernel(const H16Item* __restrict__ items, int nitems, unsigned patP, unsigned patA) {
+  extern __shared__ __attribute__((aligned(1024))) unsigned char smem[];
+  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    const H16Item it = items[w];
+    __syncthreads();   // every wave is done with the previous tile's LDS ring
+    rn16_tile<T, I8_TILE_STAGES>((const T*)it.P, it.ldP, it.sP, (const T*)it.A, it.ldA, it.sA, it.invP, it.invA,
+                                 it.C, it.bias, it.colmap, it.ldc, it.K, patP, patA, it.mt, it.nt, it.sbeg, it.send,
+                                 ATOMIC, smem);
+  }
+}
+
 // Many GEMMs of one precision in one persistent launch: workgroup b takes
 // items b, b + gridDim.x, ... of a list the host ordered for load balance and
 // XCD locality.
@@ -665,20 +799,26 @@ static int h16_fold() {
 }
 
 // Tile configurations of the reduced-precision kernels; the 256 x 256 int8
-// tile serves I8X4 only.  fold64: false for the ones that fold into an FP32
-// master tile (the int8 tile accumulates exactly).
+// tile serves I8X4 only, the 256 x 256 round-to-nearest tile (rn16) the
+// 16-bit forms.  fold64: false for the ones that fold into an FP32 master tile
+// by truncating MFMA accumulation (the int8 tile accumulates exactly, the
+// rn16 tile by round-to-nearest adds of short partials).
 struct H16CfgDesc {
   int bm, bn, wm, wn, stages, minb;
-  bool fold64, i8big;
+  bool fold64, i8big, rn16;
 };
 static const H16CfgDesc kH16Cfgs[] = {
-    {128, 128, 64, 64, 2, 2, true, false},  {256, 128, 64, 64, 3, 1, true, false},
-    {256, 128, 128, 64, 3, 1, false, false}, {128, 256, 64, 128, 3, 1, false, false},
-    {256, 256, 128, 64, 2, 1, true, true},
+    {128, 128, 64, 64, 2, 2, true, false, false},  {256, 128, 64, 64, 3, 1, true, false, false},
+    {256, 128, 128, 64, 3, 1, false, false, false}, {128, 256, 64, 128, 3, 1, false, false, false},
+    {256, 256, 128, 64, 2, 1, true, true, false},   {256, 256, 128, 64, 2, 1, true, false, true},
 };
 // Longest contraction of the int8 tile: class 3 sums four products of slices
 // in [-127, 127] per element, 4 * 127^2 * K < 2^31.
 static const int I8_TILE_MAX_K = 33285;
+// Longest contraction of the rn16 tile: its FP32 accumulation error grows with
+// the contraction (measured |dE| 3.6e-7 at K = 8192, n = 64; 2.8e-6 at
+// K = 32768, n = 128, where the FP64-fold tiles give 3.0e-7 and 1.3e-6).
+static const int RN16_TILE_MAX_K = 8192;
 int h16_num_configs() { return (int)(sizeof kH16Cfgs / sizeof kH16Cfgs[0]); }
 int h16_cfg_bm(int c) { return kH16Cfgs[c].bm; }
 int h16_cfg_bn(int c) { return kH16Cfgs[c].bn; }
@@ -686,6 +826,7 @@ bool h16_cfg_fold64(int c) { return kH16Cfgs[c].fold64; }
 // whether config c can run precision prec on contractions of length K
 bool h16_cfg_serves(int c, int prec, int K) {
   if (kH16Cfgs[c].i8big) return prec == 4 && K <= I8_TILE_MAX_K;
+  if (kH16Cfgs[c].rn16) return (prec == 2 || prec == 3) && K <= RN16_TILE_MAX_K;
   return true;
 }
 // PHF_H16_CFG=<index>[,<splitk>]: the one configuration (and split-K) the
@@ -742,6 +883,25 @@ void eri_gemm_mixed(hipStream_t s, int prec, const void* P, long kp, const void*
   const int bm = kH16Cfgs[cfg].bm, bn = kH16Cfgs[cfg].bn;
   const int tm = Mp / bm, tn = Nr / bn;
   const unsigned grid = (unsigned)(tm * tn * splitk);
+  if (kH16Cfgs[cfg].rn16) {
+#define RN_LAUNCH(TT)                                                                                           \
+  do {                                                                                                         \
+    auto kern = splitk == 2 ? eri_gemm_rn16_kernel<TT, 2> : eri_gemm_rn16_kernel<TT, 1>;                        \
+    static bool attr_set = false;                                                                              \
+    if (!attr_set) {                                                                                           \
+      (void)hipFuncSetAttribute((const void*)eri_gemm_rn16_kernel<TT, 1>,                                      \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, I8_TILE_LDS);                      \
+      (void)hipFuncSetAttribute((const void*)eri_gemm_rn16_kernel<TT, 2>,                                      \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, I8_TILE_LDS);                      \
+      attr_set = true;                                                                                         \
+    }                                                                                                          \
+    kern<<<grid, 512, I8_TILE_LDS, s>>>((const TT*)P, ldP, kp, (const TT*)A, ldA, (long)K, invP, invA, C, K, nseg, \
+                                        patP, patA, ldc, tm, bias, colmap);                                     \
+  } while (0)
+    if (prec == 3) RN_LAUNCH(_Float16); else RN_LAUNCH(__bf16);
+#undef RN_LAUNCH
+    return;
+  }
   if (kH16Cfgs[cfg].i8big) {
     mixed_patterns_desc(&patP, &patA, &patW);
     auto kern = splitk == 2 ? eri_gemm_i8_kernel<2> : eri_gemm_i8_kernel<1>;
@@ -800,6 +960,26 @@ void eri_gemm_mixed_grouped(hipStream_t s, int prec, const H16Item* items, int n
                             bool atomic) {
   unsigned patP, patA, patW;
   mixed_patterns(prec, &patP, &patA, &patW);
+  if (kH16Cfgs[cfg].rn16) {
+#define RNG_LAUNCH(TT)                                                                                          \
+  do {                                                                                                         \
+    static bool attr_set = false;                                                                              \
+    if (!attr_set) {                                                                                           \
+      (void)hipFuncSetAttribute((const void*)eri_gemm_rn16_grouped_kernel<TT, true>,                           \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, I8_TILE_LDS);                      \
+      (void)hipFuncSetAttribute((const void*)eri_gemm_rn16_grouped_kernel<TT, false>,                          \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, I8_TILE_LDS);                      \
+      attr_set = true;                                                                                         \
+    }                                                                                                          \
+    if (atomic)                                                                                                \
+      eri_gemm_rn16_grouped_kernel<TT, true><<<grid, 512, I8_TILE_LDS, s>>>(items, nitems, patP, patA);         \
+    else                                                                                                       \
+      eri_gemm_rn16_grouped_kernel<TT, false><<<grid, 512, I8_TILE_LDS, s>>>(items, nitems, patP, patA);        \
+  } while (0)
+    if (prec == 3) RNG_LAUNCH(_Float16); else RNG_LAUNCH(__bf16);
+#undef RNG_LAUNCH
+    return;
+  }
   if (kH16Cfgs[cfg].i8big) {
     mixed_patterns_desc(&patP, &patA, &patW);
     static bool attr_set = false;

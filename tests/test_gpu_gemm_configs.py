@@ -4,7 +4,8 @@ time (PHF_H16_CFG=<index>,<splitk>), against the FP64 contraction.
 Configurations 0-3 walk the product segments one after another (every 16-bit
 form and I8X4; 2 and 3 fold into FP32 and are candidates only when pinned or
 allowed); 4 is the 256 x 256 exact int8 tile (I8X4 only; one exact int32 sum
-per product class, added to Gamma in FP64).  Checks, on a spin blocked and a
+per product class, added to Gamma in FP64); 5 is the 256 x 256 16-bit tile
+with round-to-nearest accumulation of two-MFMA partials (no FP64 fold tile).  Checks, on a spin blocked and a
 dense instance and on the grouped batch launch: |dE| <= 1e-6 Eh and max |dG|
 <= 1e-5 against FP64; the int8 tile against the FP64-fold int8 segment walk,
 |dE| <= 1e-9 and |dG| <= 1e-8 (both exact up to the dropped classes);
@@ -22,6 +23,7 @@ pytestmark = pytest.mark.gpu
 
 SEGMENT_CFGS = [0, 1, 2, 3]
 I8_TILE = 4
+RN16_TILE = 5
 
 
 @pytest.fixture(scope="module")
@@ -79,9 +81,19 @@ def test_other_forms_config(prob, fp64, prec, cfg):
 
 def test_configs_refuse_other_forms(prob):
     from phf_pcc_amd import _lib
-    for prec, cfg in [("f16x2", I8_TILE), ("bf16x3", I8_TILE)]:
+    for prec, cfg in [("f16x2", I8_TILE), ("bf16x3", I8_TILE), ("i8x4", RN16_TILE)]:
         with pytest.raises(_lib.PhfError):
             _eval(prob, prec, True, cfg, 1)
+
+
+@pytest.mark.parametrize("structure", [True, False])
+@pytest.mark.parametrize("sk", [1, 2])
+@pytest.mark.parametrize("prec", ["f16x2", "bf16x3"])
+def test_rn16_tile(prob, fp64, prec, sk, structure):
+    e64, g64 = fp64[structure]
+    e, g = _eval(prob, prec, structure, RN16_TILE, sk)
+    assert abs(e - e64) <= 1e-6, (prec, sk, e - e64)
+    assert np.abs(g - g64).max() <= 1e-5
 
 
 @pytest.mark.parametrize("structure", [True, False])
@@ -97,7 +109,8 @@ def test_int8_tile(prob, fp64, sk, structure):
 
 
 @pytest.mark.parametrize("prec,cfg,sk", [("f16x2", 0, 1), ("f16x2", 0, 2), ("f16x2", 1, 1), ("bf16x3", 0, 1),
-                                         ("i8x4", I8_TILE, 1), ("i8x4", I8_TILE, 2)])
+                                         ("i8x4", I8_TILE, 1), ("i8x4", I8_TILE, 2), ("f16x2", RN16_TILE, 1),
+                                         ("f16x2", RN16_TILE, 2)])
 def test_grouped_batch_config(prec, cfg, sk):
     from phf_pcc_amd.batch import ProjectedHFBatch
     inst, zs = [], []
