@@ -189,6 +189,49 @@ int phf_blocked(const phf_ctx* ctx);
 int phf_autotune(phf_ctx* ctx, int on);
 int phf_tuning_report(const phf_ctx* ctx, char* buf, int len);
 
+/* Projected SD-space overlap times vector for LinDep.
+ *
+ * phf_buildsc replaces phftools.buildsc (BuildSC, PHFTools.f90:464-537, with
+ * BuildSC_Vec, BuildSC_dr.f90:8-1244; called at LinDep.py:73):
+ *   t = sum_g w_g BuildSC_Vec(c0, R c1 R^H, ao2mo4(c2, R^H), Y_g) / sum_g w_g,
+ *   R = R1[il] R2[iy] Rpg[ipg],  Y_g, s_g from RDM with the first nocc unit
+ *   vectors as newMO,  w_g = weightsp(il,iy) s_g sum_{m,n,p,q} w,
+ * where w is the reference's REAL weightpg(ipg,p,q) conj(fsp_m) fsp_n
+ * conj(fpg_p) fpg_q (each complex product keeps its real part, :518-520;
+ * the Wigner factor is one, :515).  Rk, fk, ncik, roota/b, rooty of the
+ * reference signature are not used by it and not taken here.
+ * Host arrays, numpy C order: c0 complex scalar; c1 complex[nso][nso];
+ * c2 complex[nso]^4; r1 complex[nr1][nso][nso], r2 complex[nr2][nso][nso],
+ * rpg complex[npg][nso][nso]; weightsp double[ngrid[0]][ngrid[1]]; weightpg
+ * double[npg][ncipg][ncipg]; fsp complex[ncisp] (ncisp >= 2 qj + 1), fpg
+ * complex[ncipg].  Outputs t0 (1), t1 (nso x nso), t2 (nso^4) complex,
+ * caller-allocated.  1 <= nocc <= nso <= 64.  PHF_ERR_SINGULAR when
+ * C0^H R C0 is exactly singular at a point (the reference STOPs in InvertC,
+ * Wrappers.f90:311).
+ *
+ * phf_sc_set_grid + phf_sc_apply split the same call so the grid factors
+ * stay on the device across the many products of a LinDep solve (no
+ * counterpart in the reference, which rebuilds them per call): phf_sc_apply
+ * takes nvec vectors at once (c0 complex[nvec], c1 complex[nvec][nso][nso],
+ * c2 complex[nvec][nso]^4, outputs alike).
+ * phf_sc_apply_packed takes nvec packed SD vectors (complex[nvec][dim],
+ * dim = phf_sd_dim(nso, nocc) = 1 + O V + O(O-1)/2 V(V-1)/2) in the order of
+ * Vec2Mat (LinDep.py:156-171) and returns Mat2Vec (LinDep.py:173-187) of the
+ * product: LinDep.py:71-76's BuildSC for a block of vectors in one call.
+ * phf_sc_apply_packed_dev is the same with device pointers (no copies). */
+int phf_buildsc(phf_ctx* ctx, const double* c0, const double* c1, const double* c2, int nocc, int nso,
+                const int* ngrid, int qj, const double* r1, int nr1, const double* r2, int nr2,
+                const double* rpg, int npg, const double* weightsp, const double* weightpg, int ncipg,
+                const double* fsp, int ncisp, const double* fpg, double* t0, double* t1, double* t2);
+int phf_sc_set_grid(phf_ctx* ctx, int nso, int nocc, const int* ngrid, int qj, const double* r1, int nr1,
+                    const double* r2, int nr2, const double* rpg, int npg, const double* weightsp,
+                    const double* weightpg, int ncipg, const double* fsp, int ncisp, const double* fpg);
+int phf_sc_apply(phf_ctx* ctx, int nvec, const double* c0, const double* c1, const double* c2, double* t0,
+                 double* t1, double* t2);
+int phf_sc_apply_packed(phf_ctx* ctx, int nvec, const double* vin, double* vout);
+int phf_sc_apply_packed_dev(phf_ctx* ctx, int nvec, const double* vin, double* vout);
+long phf_sd_dim(int nso, int nocc);
+
 /* Batched evaluation of many independent instances (no counterpart in the
  * reference, which evaluates one Hamiltonian per process; each instance is
  * one EandG of PHF.py:55-91 at J = 0).  Instances may differ in size, N and

@@ -9,6 +9,9 @@ the one f2py gives the Fortran module procedures (call sites PHF.py:70-71,76):
       weightsp, weightpg, roota, rootb, rooty, qj, sp, comm, [npg, ncipg, nso])
   g = phftools.localg(gradmat, rdmmat, smat, e0, fsp, fpg, fk, cmplxconj,
                       [nso, nci, ncisp, ncipg])
+  t0, t1, t2 = phftools.buildsc(c0, c1, c2, nocc, ngrid, qj, r1, r2, rpg, rk,
+      roota, rootb, rooty, weightsp, weightpg, fsp, fpg, fk,
+      [nso, npg, ncisp, ncipg, ncik])                      (LinDep.py:73)
 
 Inputs are cast to complex128 / float64 / int as f2py would; a shape that
 disagrees with the explicit dimension arguments raises ValueError.  Numerical
@@ -66,6 +69,7 @@ class _PhfTools:
         if self._ctx is not None:
             self._ctx.ham_key = None
             self._ctx.grid_key = None
+            self._ctx.sc_key = None
 
     def set_hamiltonian(self, hone, htwo):
         key = (_fingerprint(np.asarray(hone)), _fingerprint(np.asarray(htwo)))
@@ -171,6 +175,75 @@ class _PhfTools:
                                     fk.size, int(cmplxconj), nso, nci, _lib.ptr(g))
         _lib.check(rc, ctx.handle)
         return g
+
+    # ------------------------------------------------------------------
+    def buildsc(self, c0, c1, c2, nocc, ngrid, qj, *args, nso=None, npg=None, ncisp=None, ncipg=None, ncik=None):
+        """BuildSC (PHFTools.f90:464-537) on the GPU: (t0, t1, t2).
+
+        f2py order of the Fortran signature: c0, c1, c2, nocc, ngrid, qj, r1, r2,
+        rpg, rk, roota, rootb, rooty, weightsp, weightpg, fsp, fpg, fk,
+        [nso, npg, ncisp, ncipg, ncik].  The call at LinDep.py:73 passes one more
+        integer (nci) after qj; an integer in that place is accepted and, like
+        rk, fk and the roots (which the Fortran never reads), unused.
+        """
+        args = list(args)
+        if args and np.ndim(args[0]) == 0:
+            args.pop(0)
+        names = ["r1", "r2", "rpg", "rk", "roota", "rootb", "rooty", "weightsp", "weightpg", "fsp", "fpg", "fk"]
+        if len(args) < len(names):
+            raise TypeError(f"buildsc: missing required argument {names[len(args)]!r}")
+        opt = args[len(names):]
+        if len(opt) > 5:
+            raise TypeError("buildsc: too many arguments")
+        kw = dict(zip(["nso", "npg", "ncisp", "ncipg", "ncik"], opt))
+        nso = kw.get("nso", nso)
+        npg = kw.get("npg", npg)
+        ncisp = kw.get("ncisp", ncisp)
+        ncipg = kw.get("ncipg", ncipg)
+        r1, r2, rpg, _rk, _ra, _rb, _ry, weightsp, weightpg, fsp, fpg, _fk = args[:len(names)]
+        c1 = np.ascontiguousarray(c1, dtype=np.complex128)
+        c2 = np.ascontiguousarray(c2, dtype=np.complex128)
+        nso = c1.shape[0] if nso is None else int(nso)
+        nocc = int(nocc)
+        r1 = np.ascontiguousarray(r1, dtype=np.complex128)
+        r2 = np.ascontiguousarray(r2, dtype=np.complex128)
+        rpg = np.ascontiguousarray(rpg, dtype=np.complex128)
+        weightsp = np.ascontiguousarray(weightsp, dtype=np.float64)
+        weightpg = np.ascontiguousarray(weightpg, dtype=np.float64)
+        fsp = np.ascontiguousarray(fsp, dtype=np.complex128).reshape(-1)
+        fpg = np.ascontiguousarray(fpg, dtype=np.complex128).reshape(-1)
+        npg = rpg.shape[0] if npg is None else int(npg)
+        ncisp = fsp.size if ncisp is None else int(ncisp)
+        ncipg = fpg.size if ncipg is None else int(ncipg)
+        ngrid = np.ascontiguousarray(ngrid, dtype=np.int32).reshape(-1)
+        nl, ny = int(ngrid[0]), int(ngrid[1])
+        checks = [
+            (ngrid.size == 2, "ngrid must have 2 entries"),
+            (c1.shape == (nso, nso), "c1 must be (nso, nso)"),
+            (c2.shape == (nso,) * 4, "c2 must be (nso, nso, nso, nso)"),
+            (r1.ndim == 3 and r1.shape[1:] == (nso, nso) and r1.shape[0] >= nl, "r1 must be (>=ngrid[0], nso, nso)"),
+            (r2.ndim == 3 and r2.shape[1:] == (nso, nso) and r2.shape[0] >= ny, "r2 must be (>=ngrid[1], nso, nso)"),
+            (rpg.shape == (npg, nso, nso), "rpg must be (npg, nso, nso)"),
+            (weightpg.shape == (npg, ncipg, ncipg), "weightpg must be (npg, ncipg, ncipg)"),
+            (weightsp.ndim == 2 and weightsp.shape[0] >= nl and weightsp.shape[1] >= ny,
+             "weightsp must cover (ngrid[0], ngrid[1])"),
+            (fsp.size >= ncisp and ncisp >= 2 * int(qj) + 1, "fsp must hold 2 qj + 1 entries"),
+            (fpg.size >= ncipg, "fpg shorter than ncipg"),
+        ]
+        for ok, msg in checks:
+            if not ok:
+                raise ValueError(f"buildsc: {msg}")
+        ctx = self.context()
+        key = (nso, nocc, tuple(int(v) for v in ngrid), int(qj), _fingerprint(r1), _fingerprint(r2),
+               _fingerprint(rpg), _fingerprint(weightsp), _fingerprint(weightpg), _fingerprint(fsp),
+               _fingerprint(fpg))
+        if getattr(ctx, "sc_key", None) != key:
+            ctx.sc_key = None
+            ctx.sc_set_grid(nso, nocc, ngrid, int(qj), r1, r2, rpg, weightsp[:nl, :ny], weightpg, fsp[:ncisp],
+                            fpg[:ncipg])
+            ctx.sc_key = key
+        t0, t1, t2 = ctx.sc_apply(np.array([complex(np.asarray(c0).reshape(-1)[0])]), c1[None], c2[None])
+        return t0[0], np.asfortranarray(t1[0]), np.asfortranarray(t2[0])
 
 
 phftools = _PhfTools()

@@ -33,7 +33,9 @@ EXPORTS = ["phf_ctx_create", "phf_ctx_destroy", "phf_last_error", "phf_set_hamil
            "phf_buildhsg_z", "phf_autotune", "phf_tuning_report", "phf_set_hamiltonian_factored",
            "phf_get_hone", "phf_eval_resident", "phf_localg_vo", "phf_batch_create", "phf_batch_destroy",
            "phf_batch_last_error", "phf_batch_set_instance", "phf_batch_eval", "phf_batch_set_graph",
-           "phf_batch_autotune", "phf_batch_stats", "phf_batch_tuning_report", "phf_set_blocked", "phf_blocked"]
+           "phf_batch_autotune", "phf_batch_stats", "phf_batch_tuning_report", "phf_set_blocked", "phf_blocked",
+           "phf_buildsc", "phf_sc_set_grid", "phf_sc_apply", "phf_sc_apply_packed", "phf_sc_apply_packed_dev",
+           "phf_sd_dim"]
 
 
 class PhfError(RuntimeError):
@@ -98,9 +100,18 @@ def load() -> C.CDLL:
     lib.phf_batch_autotune.argtypes = [C.c_void_p, C.c_int]
     lib.phf_batch_stats.argtypes = [C.c_void_p, _dp, C.c_int]
     lib.phf_batch_tuning_report.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+    lib.phf_buildsc.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_int, C.c_int, _ip, C.c_int, _dp, C.c_int, _dp,
+                                C.c_int, _dp, C.c_int, _dp, _dp, C.c_int, _dp, C.c_int, _dp, _dp, _dp, _dp]
+    lib.phf_sc_set_grid.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp,
+                                    C.c_int, _dp, _dp, C.c_int, _dp, C.c_int, _dp]
+    lib.phf_sc_apply.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+    lib.phf_sc_apply_packed.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+    lib.phf_sc_apply_packed_dev.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    lib.phf_sd_dim.argtypes = [C.c_int, C.c_int]
     for name in EXPORTS:
         if name not in ("phf_last_error", "phf_batch_last_error"):
             getattr(lib, name).restype = C.c_int
+    lib.phf_sd_dim.restype = C.c_long
     _lib = lib
     return lib
 
@@ -198,6 +209,50 @@ class Context:
         out = np.zeros(8)
         n = load().phf_stage_times(self._h, ptr(out), 8)
         return out[:n].copy()
+
+    # ------------------------------------------------------------------
+    # projected SD-space overlap (BuildSC, PHFTools.f90:464-537)
+    def sc_set_grid(self, nso: int, nocc: int, ngrid, qj: int, r1, r2, rpg, weightsp, weightpg, fsp, fpg):
+        """Upload a projection grid and build its per-point factors on the device."""
+        ngrid = np.ascontiguousarray(np.asarray(ngrid, dtype=np.int32).reshape(-1)[:2])
+        nl, ny = int(ngrid[0]), int(ngrid[1])
+        r1 = np.ascontiguousarray(r1, dtype=np.complex128)
+        r2 = np.ascontiguousarray(r2, dtype=np.complex128)
+        rpg = np.ascontiguousarray(rpg, dtype=np.complex128)
+        weightsp = np.ascontiguousarray(np.asarray(weightsp, dtype=np.float64)[:nl, :ny])
+        weightpg = np.ascontiguousarray(weightpg, dtype=np.float64)
+        fsp = np.ascontiguousarray(np.asarray(fsp, dtype=np.complex128).reshape(-1))
+        fpg = np.ascontiguousarray(np.asarray(fpg, dtype=np.complex128).reshape(-1))
+        npg, ncipg = rpg.shape[0], weightpg.shape[1]
+        if (r1.shape[1:] != (nso, nso) or r2.shape[1:] != (nso, nso) or rpg.shape[1:] != (nso, nso)
+                or weightpg.shape != (npg, ncipg, ncipg) or fpg.size < ncipg or weightsp.shape != (nl, ny)):
+            raise ValueError("sc_set_grid: inconsistent grid shapes")
+        check(load().phf_sc_set_grid(self._h, nso, nocc, ngrid.ctypes.data_as(_ip), int(qj), ptr(r1), r1.shape[0], ptr(r2),
+                                     r2.shape[0], ptr(rpg), npg, ptr(weightsp), ptr(weightpg), ncipg, ptr(fsp),
+                                     fsp.size, ptr(fpg)), self._h)
+        self.sc_shape = (nso, nocc)
+
+    def sc_apply(self, c0, c1, c2):
+        """(t0, t1, t2) for stacked inputs c0 [nv], c1 [nv, nso, nso], c2 [nv, nso^4]."""
+        nso = self.sc_shape[0]
+        c0 = np.ascontiguousarray(np.asarray(c0, dtype=np.complex128).reshape(-1))
+        nv = c0.size
+        c1 = np.ascontiguousarray(np.asarray(c1, dtype=np.complex128).reshape(nv, nso, nso))
+        c2 = np.ascontiguousarray(np.asarray(c2, dtype=np.complex128).reshape((nv,) + (nso,) * 4))
+        t0 = np.empty(nv, np.complex128)
+        t1 = np.empty_like(c1)
+        t2 = np.empty_like(c2)
+        check(load().phf_sc_apply(self._h, nv, ptr(c0), ptr(c1), ptr(c2), ptr(t0), ptr(t1), ptr(t2)), self._h)
+        return t0, t1, t2
+
+    def sc_apply_packed(self, vecs: np.ndarray) -> np.ndarray:
+        """Packed SD vectors [nv, dim] (LinDep.py:156-187 order) -> their products."""
+        v = np.ascontiguousarray(vecs, dtype=np.complex128)
+        one = v.ndim == 1
+        v = v.reshape(1, -1) if one else v
+        out = np.empty_like(v)
+        check(load().phf_sc_apply_packed(self._h, v.shape[0], ptr(v), ptr(out)), self._h)
+        return out[0] if one else out
 
     def set_hamiltonian(self, hone: np.ndarray, htwo: np.ndarray, force_complex: bool = False, key=None,
                         dense: bool = False):

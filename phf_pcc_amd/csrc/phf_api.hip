@@ -478,6 +478,7 @@ extern "C" int phf_ctx_destroy(phf_ctx* ctx) {
     if (ctx->dAinv[q]) (void)hipFree(ctx->dAinv[q]);
   }
   drop_blocks(ctx);
+  phf_free_sc(ctx);
   if (ctx->ws) (void)hipFree(ctx->ws);
   if (ctx->small) (void)hipFree(ctx->small);
   for (double* q : {ctx->dGr1p, ctx->dGr2, ctx->dGrpgref, ctx->dGw})

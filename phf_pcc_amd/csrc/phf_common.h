@@ -16,12 +16,17 @@ typedef float f4_t __attribute__((ext_vector_type(4)));
 //                  D[i][j] lane j + 16*(i / 4), register i % 4
 
 // Strided complex operand: element (i, j) of batch b has its real part at
-// p[b*bs + i*rs + j*cs] and its imaginary part im doubles further on
-// (im = 1 for interleaved complex128, im = plane size for split planes).
+// p[(b / bdiv)*bs + (b % bdiv)*bs2 + i*rs + j*cs] and its imaginary part im
+// doubles further on (im = 1 for interleaved complex128, im = plane size for
+// split planes).  bdiv = 1, bs2 = 0 (the defaults) give p[b*bs + ...]; a
+// batch over pairs (outer, inner) with inner < bdiv strides each separately.
 struct ZMat {
   const double* p;
   long rs, cs, bs, im;
   int conj;
+  long bs2 = 0;
+  int bdiv = 1;
+  __host__ __device__ long boff(long b) const { return (b / bdiv) * bs + (b % bdiv) * bs2; }
 };
 
 struct ZOut {
@@ -31,6 +36,9 @@ struct ZOut {
   // of the bit patterns) the largest |Re| / |Im| written, for the row scales
   // of the reduced-precision split; zero-filled by the caller
   unsigned long long* amax = nullptr;
+  long bs2 = 0;
+  int bdiv = 1;
+  __host__ __device__ long boff(long b) const { return (b / bdiv) * bs + (b % bdiv) * bs2; }
 };
 
 // Host-side helpers for interleaved complex128 storage.

@@ -105,6 +105,14 @@ struct phf_ctx_s {
   int autotune = 1;
   std::map<std::tuple<int, long, int, int>, std::pair<int, int>> tuned;
   std::string tune_log;
+  // projected SD-space overlap grid (phf_sc_set_grid): per point g, column-major
+  // R (N x N), Y = R C0 M^-1 (N x NOcc), B^T = R^H Y (N x NOcc), K = R^H Y R,
+  // A = (I - Y) R; w_g and sum_g w_g; sc_ws is the workspace of phf_sc_apply*
+  int sc_G = 0, sc_nso = 0, sc_nocc = 0;
+  double *sc_R = nullptr, *sc_Y = nullptr, *sc_BT = nullptr, *sc_K = nullptr, *sc_A = nullptr;
+  double *sc_w = nullptr, *sc_ovlp = nullptr;
+  char* sc_ws = nullptr;
+  size_t sc_ws_size = 0;
   // per-stage timing with HIP events on `stream` (phf_set_profiling)
   bool profiling = false;
   hipEvent_t ev[8] = {};
@@ -129,6 +137,7 @@ struct Arena {
 };
 
 int phf_fail(phf_ctx* ctx, int code, const std::string& msg);
+void phf_free_sc(phf_ctx* ctx);   // the projected SD-space overlap grid and workspace
 
 #define HIPCK(expr)                                                                   \
   do {                                                                                \

@@ -29,8 +29,9 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double 
   const int wm = wave >> 1, wn = wave & 1;
   const long b = blockIdx.z;
   const int m0 = blockIdx.x * ZBM, n0 = blockIdx.y * ZBN;
-  const double* Ap = A.p + b * A.bs;
-  const double* Bp = B.p + b * B.bs;
+  const double* Ap = A.p + A.boff(b);
+  const double* Bp = B.p + B.boff(b);
+  const long cb = C.boff(b);
 
   d4_t cre[FMW][FNW], cim[FMW][FNW];
 #pragma unroll
@@ -118,7 +119,7 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double 
         if (gi < m && gj < n) {
           const double xr = cre[i][j][r], xi = cim[i][j][r];
           double vr = ar * xr - ai * xi, vi = ar * xi + ai * xr;
-          const long off = b * C.bs + (long)gi * C.rs + (long)gj * C.cs;
+          const long off = cb + (long)gi * C.rs + (long)gj * C.cs;
           if (use_beta) {
             const double orr = C.p[off], oi = C.p[off + C.im];
             vr += br * orr - bi * oi;
