@@ -334,6 +334,114 @@ def run_adjoint(args):
     print(json.dumps(out), flush=True)
 
 
+def run_buildsc(args):
+    """Projected SD-space overlap times vector (BuildSC, PHFTools.f90:464-537)
+    for LinDep: NSO=16 (8 spatial orbitals), N=8, Lebedev 14 x 8 gamma = 112
+    spin-grid points in a random MO basis (seed 1006), blocks of 280 packed
+    SD vectors (LinDep.py's Lanczos block; dim = 849) resident on the device.
+    One step = one block product.  value = SD vectors / s.  roofline: the
+    stage-2 FP64 complex GEMM (zgemm_kernel<2, 1>: the second pair transform
+    with the weighted grid sum in its contraction, the largest launch; 8 real
+    FLOPs per complex multiply-add as launched), its duration from HIP events
+    on the library's stream; all four GEMMs of a step are reported beside it.
+    cpu_baseline: the oracle's restatement of BuildSC (dense
+    per-point BuildSC_Vec + ao2mo4, numpy) on a bounded sample of grid points
+    for one vector, scaled to vectors / s over the full grid."""
+    import torch
+    from oracle import buildsc_oracle as B
+    from phf_pcc_amd.sc import ProjectedOverlap
+    from tests._cases import grid_args, random_unitary
+    nao, nocc, ngrid, nb = 8, 8, (14, 8), args.sc_block
+    nso = 2 * nao
+    rng = np.random.default_rng(1006)
+    u = random_unitary(nso, rng)
+    ga = grid_args(nao, ngrid, 0)
+    r1 = np.einsum('pa,gpq,qb->gab', u.conj(), ga["r1"], u)
+    r2 = np.einsum('pa,gpq,qb->gab', u.conj(), ga["r2"], u)
+    op = ProjectedOverlap(nso, nocc, ga["ngrid"], r1, r2, ga["rpg"], ga["weightsp"], ga["weightpg"])
+    G = int(np.prod(ga["ngrid"]))
+    dim = op.dim
+    x = rng.normal(size=(nb, dim)) + 1j * rng.normal(size=(nb, dim))
+    dx = torch.tensor(x, dtype=torch.complex128, device="cuda")
+    dy = torch.empty_like(dx)
+    from phf_pcc_amd import _lib
+    lib = _lib.load()
+    h = op.ctx.handle
+
+    def step():
+        _lib.check(lib.phf_sc_apply_packed_dev(h, nb, dx.data_ptr(), dy.data_ptr()), h)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    got = dy.cpu().numpy()
+    # parity of vector 0 against the oracle over the full grid
+    t = B.buildsc(*B.vec2mat(x[0], nso, nocc), nocc, ga["ngrid"], 0, r1, r2, ga["rpg"], ga["weightsp"],
+                  ga["weightpg"], np.ones(1, complex), np.ones(1, complex))
+    ref0 = B.mat2vec(*t, nocc)
+    parity = float(np.abs(got[0] - ref0).max() / np.abs(ref0).max())
+    log(f"[bench] buildsc: {args.steps} blocks of {nb} in {dt:.3f}s; parity vs oracle (vector 0) {parity:.2e}")
+    # stage times of one profiled step
+    op.ctx.set_profiling(True)
+    step()
+    st = op.ctx.stage_times()
+    op.ctx.set_profiling(False)
+    V = nso - nocc
+    p12, p34 = V * (V - 1) // 2, nocc * (nocc - 1) // 2      # pair spaces a<b, i<j
+    flops = 8.0 * (p12 * (nb * p34) * p12 * G             # stage 1: per point P12 x (nb P34) x P12
+                   + (p12 * nb) * p34 * (G * p34)         # stage 2: (P12 nb) x P34 x (G P34)
+                   + (V * nocc * nb) * G * (V * nocc)     # P
+                   + (V * nocc) * (V * nocc * nb) * G)    # rank term
+    gemm_ms = float(st[1] + st[2] + st[3] + st[4])
+    nlaunch = int(st[7])
+    all_gemm = flops / (gemm_ms * 1e-3)
+    # dominant launch: the stage-2 GEMM (zgemm_kernel<2, 1>, one launch per step)
+    flops2 = 8.0 * (p12 * nb) * p34 * (G * p34)
+    achieved = flops2 / (float(st[2]) * 1e-3)
+    log(f"[bench] buildsc stage ms: all {st[0]:.3f}, stage1 {st[1]:.3f}, stage2 {st[2]:.3f}, P {st[3]:.3f}, "
+        f"rank {st[4]:.3f}, point kernel {st[5]:.3f}, rest {st[6]:.3f}; {nlaunch} GEMM launches at "
+        f"{all_gemm / 1e12:.2f} TFLOP/s; stage-2 GEMM {achieved / 1e12:.2f} TFLOP/s")
+    cpu = None
+    if not args.no_cpu:
+        pts = [(il, iy, 0) for il in range(int(ga["ngrid"][0])) for iy in range(int(ga["ngrid"][1]))]
+        c0, c1, c2 = B.vec2mat(x[0], nso, nocc)
+        done, tc = 0, 0.0
+        t0c = time.perf_counter()
+        while done < len(pts) and tc < args.cpu_seconds:
+            B.buildsc(c0, c1, c2, nocc, ga["ngrid"], 0, r1, r2, ga["rpg"], ga["weightsp"], ga["weightpg"],
+                      np.ones(1, complex), np.ones(1, complex), points=pts[done:done + 4])
+            done += 4
+            tc = time.perf_counter() - t0c
+        per_vec = tc / done * G
+        cpu = {"value": 1.0 / per_vec, "unit": "SD vectors/s",
+               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",
+               "sample": f"oracle BuildSC (dense per-point BuildSC_Vec + ao2mo4, numpy einsum) on {done} of {G} "
+                         f"grid points for one vector in {tc:.1f} s, scaled to the full grid"}
+        log(f"[bench] buildsc CPU oracle: {cpu['value']:.3f} vectors/s")
+    out = {"metric": "projected SD-space overlap x vector (BuildSC for LinDep): SD vectors/s",
+           "value": nb * args.steps / dt, "unit": "SD vectors/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "c128", "data": "synthetic",
+           "config": {"workload": f"BuildSC packed: NSO={nso}, N={nocc}, SD dim={dim}, spin grid 14x8={G} points "
+                                  f"(seed 1006), block of {nb} vectors per step, device-resident",
+                      "parity_rel_err_vector0": parity,
+                      "stage_ms": {"all": st[0], "stage1_gemm": st[1], "stage2_gemm": st[2], "p_gemm": st[3],
+                                   "rank_gemm": st[4], "point_kernel": st[5], "rest": st[6]}},
+           "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
+                        "frac": achieved / PEAK["fp64"], "traffic": load_traffic("zgemm_kernel<2, 1>", "sc16"),
+                        "kernel": "zgemm_kernel<2, 1> (stage-2 pair transform with the grid sum)",
+                        "flops_per_launch": flops2, "launches_per_step": 1,
+                        "all_gemms": {"launches_per_step": nlaunch, "flops_per_step": flops,
+                                      "achieved_tflops": all_gemm / 1e12}},
+           "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
 def cpu_blas_gemm(prob, G):
     """Strong CPU baseline for the dominant stage: Gamma for every grid point as
     one BLAS dgemm A[(i,k),(j,l)] P (numpy's BLAS on all host cores; real A,
@@ -387,7 +495,7 @@ def load_traffic(kernel, n):
         data = json.load(open(path))
     except ValueError:
         return None
-    rec = data.get(f"{kernel}@n{n}")
+    rec = data.get(f"{kernel}@{n}" if isinstance(n, str) else f"{kernel}@n{n}")
     return None if rec is None else {"bytes_per_launch": rec["bytes_per_launch"], "source": rec["source"]}
 
 
@@ -405,7 +513,8 @@ def main():
     ap.add_argument("--dropin", action="store_true",
                     help="also time the PHFTools_F drop-in (host arrays in, Hmat/Smat/Gradmat/Rdmmat out per call)")
     ap.add_argument("--no-autotune", action="store_true", help="use the tile cost model instead of timing candidates")
-    ap.add_argument("--workload", default="single", choices=["single", "batch64", "large", "adjoint"],
+    ap.add_argument("--sc-block", type=int, default=280, help="vectors per block of the buildsc workload")
+    ap.add_argument("--workload", default="single", choices=["single", "batch64", "large", "adjoint", "buildsc"],
                     help="single: one n=64 instance (default); batch64: 64 uneven instances in one batch; "
                          "large: n=128, 50x24 grid, every precision; adjoint: n=96 gradient checks + 50 L-BFGS")
     ap.add_argument("--ninst", type=int, default=64, help="instances of the batch64 workload")
@@ -433,6 +542,8 @@ def main():
         return run_large(args)
     if args.workload == "adjoint":
         return run_adjoint(args)
+    if args.workload == "buildsc":
+        return run_buildsc(args)
     from phf_pcc_amd.PHFTools_F import phftools
     from phf_pcc_amd.model import ProjectedHF
     from phf_pcc_amd.phf import eigen_solver, vec_decomp

@@ -16,6 +16,12 @@
 // transform per index pair, and every entry of t2 with c or d >= NOcc is zero.
 // The result is sum_g w_g t_g / sum_g w_g, w_g = weightsp(il,iy) s_g W(ipg).
 //
+// c2a is antisymmetric in (p,q) and in (r,s), so the pair transforms act on
+// pair spaces: with p < q, r < s, a < b, i < j,
+//   t2(a,b,i,j) = sum KA(ab,pq) KB(ij,rs) c2a(pq,rs) + rank term,
+//   KA(ab,pq) = A(a,p) A(b,q) - A(a,q) A(b,p),  KB likewise from B,
+// 4 x fewer products per pair than the Kronecker A (x) A, and only i < j < NOcc.
+//
 // Index blocks: indices 1, 2 of c2a run over e12_in values from off12_in
 // (the output rows a, b over e12_out from off12_out), indices 3, 4 over e34_in
 // from off34_in, the output c, d over NOcc.  The full tensors use (N, 0, N, 0;
@@ -23,14 +29,15 @@
 // virtual x virtual x occupied x occupied, use (V, NOcc, V, NOcc; NOcc, 0).
 //
 // GEMM form over a chunk of Gc points and nv vectors (complex FP64, zgemm):
-//   stage 1, batched over (g, v):  Out1 = C2S_v KB_g^T,   KB_g = B_blk (x) B_blk
-//   stage 2, split over groups of points, contraction over (g, p, q):
-//                                   T2 += [w_g A_blk (x) A_blk]_g Out1
+//   stage 1, batched over g:  X_g = KA_g C2         (rows a<b, columns (v, r<s))
+//   stage 2, contraction over (g, r<s), split into groups of points:
+//                             T2 += X KBW^T,  KBW_g = w_g KB_g   (rows (a<b, v), columns i<j)
 //   P  = C2P Kvec  (all points at once);  RK += YW Qm  (sum_g w_g Y_g (x) Q_g)
 // plus one workgroup per (vector, group of points) for t0, t1 and Q.
 #include <algorithm>
 #include <complex>
 #include <cstdlib>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -60,8 +67,25 @@ __device__ inline void st2(double* p, long i, double2 v) {
 }
 
 struct ScBlk {
+  // element ranges: indices 1, 2 of c2a (in) and rows a, b of the result (out);
+  // indices 3, 4 of c2a (in); the result's c, d run over the first O
   int e12_in, off12_in, e12_out, off12_out, e34_in, off34_in, O;
+  // pair counts p < q of those ranges (the antisymmetric pair spaces)
+  __host__ __device__ long p12_in() const { return (long)e12_in * (e12_in - 1) / 2; }
+  __host__ __device__ long p12_out() const { return (long)e12_out * (e12_out - 1) / 2; }
+  __host__ __device__ long p34_in() const { return (long)e34_in * (e34_in - 1) / 2; }
+  __host__ __device__ long p34_out() const { return (long)O * (O - 1) / 2; }
 };
+
+// Pair index t = q (q - 1) / 2 + p of p < q (q-major, the order of LinDep.py:165-170).
+__device__ inline void pair_of(long t, int& p, int& q) {
+  int b = (int)((1.0 + sqrt(1.0 + 8.0 * (double)t)) * 0.5);
+  while ((long)b * (b - 1) / 2 > t) --b;
+  while ((long)(b + 1) * b / 2 <= t) ++b;
+  q = b;
+  p = (int)(t - (long)b * (b - 1) / 2);
+}
+__device__ inline long pair_index(int p, int q) { return (long)q * (q - 1) / 2 + p; }   // p < q
 
 // M_g = R_g[:O, :O] (column-major, contiguous) for the LU.
 __global__ void sc_overlap_kernel(int n, int o, const double* __restrict__ R, double* __restrict__ M) {
@@ -96,11 +120,14 @@ __global__ void sc_weight_kernel(int G, const double* __restrict__ wh, const dou
   if (threadIdx.x == 0) st2(ovlp, 0, make_double2(red[0][0], red[1][0]));
 }
 
-// Per point of a chunk (blockIdx.y): the Kronecker factors and point vectors.
+// Per point of a chunk (blockIdx.y): the pair factors
+//   KA_g((a,b),(p,q))   = A(a',p') A(b',q') - A(a',q') A(b',p')        a < b, p < q
+//   KBW_g((i,j),(r,s))  = w_g [B(i,r') B(j,s') - B(i,s') B(j,r')]      i < j, r < s
+// (B(i, x) = BT(x, i)), Kvec((p,s)) = K(s', p') and YW((a,c)) = w_g Y(a', c).
 __global__ void sc_factor_kernel(ScBlk k, int n, int g0, int G, const double* __restrict__ A,
                                  const double* __restrict__ BT, const double* __restrict__ Y,
                                  const double* __restrict__ K, const double* __restrict__ ws,
-                                 double* __restrict__ KB, double* __restrict__ KAW, double* __restrict__ Kvec,
+                                 double* __restrict__ KA, double* __restrict__ KBW, double* __restrict__ Kvec,
                                  double* __restrict__ YW) {
   const int gl = blockIdx.y;
   const long g = g0 + gl;
@@ -111,39 +138,44 @@ __global__ void sc_factor_kernel(ScBlk k, int n, int g0, int G, const double* __
   const long stride = (long)gridDim.x * blockDim.x;
   const long t0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const double2 z = make_double2(0.0, 0.0);
-  {  // KB_g((i,j),(r,s)) = B(i, r') B(j, s'),  B(i, x) = BT(x, i)
-    const long nkb = (long)o * o * k.e34_in * k.e34_in;
+  {
+    const long po = k.p34_out(), pi = k.p34_in(), nkb = po * pi;
     for (long idx = t0; idx < nkb; idx += stride) {
-      const long ij = idx % ((long)o * o), rs = idx / ((long)o * o);
-      const int i = ij % o, j = ij / o, r = rs % k.e34_in, s = rs / k.e34_in;
-      double2 v = z;
-      if (ok)
-        v = cmul(ld2(BT, g * no + k.off34_in + r + (long)n * i), ld2(BT, g * no + k.off34_in + s + (long)n * j));
-      st2(KB, gl * nkb + idx, v);
-    }
-  }
-  {  // KAW_g((a,b),(p,q)) = w_g A(a', p') A(b', q')
-    const long e1 = (long)k.e12_out * k.e12_out, e2 = (long)k.e12_in * k.e12_in, nka = e1 * e2;
-    for (long idx = t0; idx < nka; idx += stride) {
-      const long ab = idx % e1, pq = idx / e1;
-      const int a = ab % k.e12_out, b = ab / k.e12_out, p = pq % k.e12_in, q = pq / k.e12_in;
       double2 v = z;
       if (ok) {
-        const double2 x = ld2(A, g * n2 + k.off12_out + a + (long)n * (k.off12_in + p));
-        const double2 y = ld2(A, g * n2 + k.off12_out + b + (long)n * (k.off12_in + q));
-        v = cmul(w, cmul(x, y));
+        int i, j, r, s;
+        pair_of(idx % po, i, j);
+        pair_of(idx / po, r, s);
+        auto b = [&](int row, int x) { return ld2(BT, g * no + k.off34_in + x + (long)n * row); };
+        v = cmul(w, csub(cmul(b(i, r), b(j, s)), cmul(b(i, s), b(j, r))));
       }
-      st2(KAW, gl * nka + idx, v);
+      st2(KBW, gl * nkb + idx, v);
     }
   }
-  {  // Kvec((p,s)) = K(s', p')
+  {
+    const long po = k.p12_out(), pi = k.p12_in(), nka = po * pi;
+    for (long idx = t0; idx < nka; idx += stride) {
+      double2 v = z;
+      if (ok) {
+        int a, b, p, q;
+        pair_of(idx % po, a, b);
+        pair_of(idx / po, p, q);
+        auto m = [&](int row, int col) {
+          return ld2(A, g * n2 + k.off12_out + row + (long)n * (k.off12_in + col));
+        };
+        v = csub(cmul(m(a, p), m(b, q)), cmul(m(a, q), m(b, p)));
+      }
+      st2(KA, gl * nka + idx, v);
+    }
+  }
+  {
     const long nk = (long)k.e12_in * k.e34_in;
     for (long idx = t0; idx < nk; idx += stride) {
       const int p = idx % k.e12_in, s = idx / k.e12_in;
       st2(Kvec, gl * nk + idx, ok ? ld2(K, g * n2 + k.off34_in + s + (long)n * (k.off12_in + p)) : z);
     }
   }
-  {  // YW((a,c)) = w_g Y(a', c)
+  {
     const long ny = (long)k.e12_out * o;
     for (long idx = t0; idx < ny; idx += stride) {
       const int a = idx % k.e12_out, c = idx / k.e12_out;
@@ -152,12 +184,12 @@ __global__ void sc_factor_kernel(ScBlk k, int n, int g0, int G, const double* __
   }
 }
 
-// Full tensors in (host C order, c2[v][p][q][r][s], c1[v][q][r]) -> c2a in the
-// stage-1 layout C2S[(p,q) + N^2 v + N^2 nv (r,s)] and the P layout
-// C2P[(q,r) + N^2 v + N^2 nv (p,s)]; c1 -> column-major blocks.
+// Full tensors in (host C order, c2[v][p][q][r][s], c1[v][p][q]) -> c2a in
+// pair form C2[(p<q) + P12 (v + nv (r<s))] (stage 1) and in full-index form
+// C2P[(q,r) + N^2 v + N^2 nv (p,s)] (the P contraction); c1 -> column-major.
 __global__ void sc_prep_full_kernel(int n, int nv, const double* __restrict__ c2, const double* __restrict__ c1,
-                                    double* __restrict__ C2S, double* __restrict__ C2P, double* __restrict__ C1) {
-  const long n2 = (long)n * n, tot = n2 * n2 * nv;
+                                    double* __restrict__ C2, double* __restrict__ C2P, double* __restrict__ C1) {
+  const long n2 = (long)n * n, tot = n2 * n2 * nv, np = (long)n * (n - 1) / 2;
   const long stride = (long)gridDim.x * blockDim.x;
   for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += stride) {
     const long pq = idx % n2, rem = idx / n2;
@@ -166,19 +198,19 @@ __global__ void sc_prep_full_kernel(int n, int nv, const double* __restrict__ c2
     const double* cv = c2 + 2 * v * n2 * n2;
     auto x = [&](int i, int j, int kk, int l) { return ld2(cv, ((long)(i * n + j) * n + kk) * n + l); };
     const double2 val = cadd(csub(x(p, q, r, s), x(q, p, r, s)), csub(x(q, p, s, r), x(p, q, s, r)));
-    st2(C2S, idx, val);
     st2(C2P, (q + (long)n * r) + n2 * v + n2 * nv * (p + (long)n * s), val);
+    if (p < q && r < s) st2(C2, pair_index(p, q) + np * (v + nv * pair_index(r, s)), val);
     if (rs == 0) st2(C1, pq + n2 * v, ld2(c1, v * n2 + (long)p * n + q));   // C1[(p,q)] = c1[v][p][q]
   }
 }
 
 // Packed vectors (vin[v][dim], LinDep.py:156-171: c1[a,i] after c0, then
-// c2[a,b,i,j] for i<j, a<b, i-major) -> c2a blocks over (V, V, O, O).
+// c2[a,b,i,j] for i<j, a<b, (i,j)-major) -> the same layouts over (V, V, O, O).
 __global__ void sc_prep_packed_kernel(int o, int nvir, int nv, long dim, const double* __restrict__ vin,
-                                      double* __restrict__ C2S, double* __restrict__ C2P, double* __restrict__ C1,
+                                      double* __restrict__ C2, double* __restrict__ C2P, double* __restrict__ C1,
                                       double* __restrict__ c0v) {
   const long v2 = (long)nvir * nvir, vo = (long)nvir * o, tot = v2 * o * o * nv;
-  const long nab = (long)nvir * (nvir - 1) / 2;
+  const long nab = (long)nvir * (nvir - 1) / 2, nij = (long)o * (o - 1) / 2;
   const long stride = (long)gridDim.x * blockDim.x;
   const long t0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
   for (long idx = t0; idx < tot; idx += stride) {
@@ -187,13 +219,16 @@ __global__ void sc_prep_packed_kernel(int o, int nvir, int nv, long dim, const d
     const int p = pq % nvir, q = pq / nvir, r = rs % o, s = rs / o;
     double2 val = make_double2(0.0, 0.0);
     if (p != q && r != s) {
-      const int a = min(p, q), b = max(p, q), i = min(r, s), j = max(r, s);
-      const long kk = 1 + vo + ((long)j * (j - 1) / 2 + i) * nab + ((long)b * (b - 1) / 2 + a);
+      const long kk = 1 + vo + pair_index(min(r, s), max(r, s)) * nab + pair_index(min(p, q), max(p, q));
       val = ld2(vin, v * dim + kk);
       if ((p < q) != (r < s)) val = make_double2(-val.x, -val.y);
     }
-    st2(C2S, idx, val);
     st2(C2P, (q + (long)nvir * r) + vo * v + vo * nv * (p + (long)nvir * s), val);
+  }
+  for (long idx = t0; idx < nab * nv * nij; idx += stride) {   // C2[(a<b) + nab (v + nv (i<j))]
+    const long pab = idx % nab, rem = idx / nab;
+    const long v = rem % nv, pij = rem / nv;
+    st2(C2, idx, ld2(vin, v * dim + 1 + vo + pij * nab + pab));
   }
   for (long idx = t0; idx < vo * nv; idx += stride) {
     const long v = idx / vo, e = idx % vo;   // e = a + V i
@@ -203,11 +238,13 @@ __global__ void sc_prep_packed_kernel(int o, int nvir, int nv, long dim, const d
   }
 }
 
-constexpr int SC_T1_REG = 16;   // t1 elements owned per thread: e12_out * O <= 4096
+constexpr int SC_T1_REG = 16;   // t1 elements owned per thread: e12_out * O <= 16 NT
 
-// One workgroup per (group of points, vector): t0, t1 and Q of each point of
-// the group; partial sums of w_g t0 and w_g t1 into slot `slot0 + group`.
-__global__ __launch_bounds__(256) void sc_point_kernel(ScBlk k, int n, int nv, int g0, int gs, int G, int nslot,
+// One workgroup of NT threads per (group of points, vector): t0, t1 and Q of
+// each point of the group; partial sums of w_g t0 and w_g t1 into slot
+// `slot0 + group`.
+template <int NT>
+__global__ __launch_bounds__(NT) void sc_point_kernel(ScBlk k, int n, int nv, int g0, int gs, int G, int nslot,
                                                        int slot0, const double* __restrict__ c0v,
                                                        const double* __restrict__ C1, const double* __restrict__ Pm,
                                                        int gc, const double* __restrict__ A,
@@ -236,12 +273,12 @@ __global__ __launch_bounds__(256) void sc_point_kernel(ScBlk k, int n, int nv, i
     const long qbase = nt1 * v + nt1 * nv * gl;
     if (gl >= gc) break;
     if (g >= G) {
-      for (long e = tid; e < nt1; e += 256) st2(Qm, qbase + e, z);
+      for (long e = tid; e < nt1; e += NT) st2(Qm, qbase + e, z);
       continue;
     }
     // D = c1 - P and the traces tr(K c1), tr(K P)
     double2 s1 = z, s2 = z;
-    for (long e = tid; e < nd; e += 256) {
+    for (long e = tid; e < nd; e += NT) {
       const int q = e % e1, r = e / e1;
       const double2 c1v = ld2(C1, nd * v + e);
       const double2 pv = ld2(Pm, e + nd * v + nd * nv * gl);
@@ -258,9 +295,13 @@ __global__ __launch_bounds__(256) void sc_point_kernel(ScBlk k, int n, int nv, i
     }
     if ((tid & 63) == 0) red[tid >> 6] = s;
     __syncthreads();
-    if (tid == 0) s_t0 = cadd(c0, cadd(cadd(red[0], red[1]), cadd(red[2], red[3])));
+    if (tid == 0) {
+      double2 t = c0;
+      for (int w = 0; w < NT / 64; ++w) t = cadd(t, red[w]);
+      s_t0 = t;
+    }
     // T = D B^T:  T(p, i) = sum_s D(p, s) BT(s', i)
-    for (long e = tid; e < (long)e1 * o; e += 256) {
+    for (long e = tid; e < (long)e1 * o; e += NT) {
       const int p = e % e1, i = e / e1;
       double2 t = z;
       for (int s3 = 0; s3 < e3; ++s3) t = cadd(t, cmul(D[p + (long)e1 * s3], ld2(BT, g * no + k.off34_in + s3 + (long)n * i)));
@@ -271,7 +312,7 @@ __global__ __launch_bounds__(256) void sc_point_kernel(ScBlk k, int n, int nv, i
     // t1(a, i) = t0 Y(a', i) + sum_p A(a', p') T(p, i);  Q = t1 - t0 Y / 2
 #pragma unroll
     for (int r = 0; r < SC_T1_REG; ++r) {
-      const long e = tid + 256L * r;
+      const long e = tid + (long)NT * r;
       if (e < nt1) {
         const int a = e % eo, i = e / eo;
         const double2 yv = ld2(Y, g * no + k.off12_out + a + (long)n * i);
@@ -287,10 +328,27 @@ __global__ __launch_bounds__(256) void sc_point_kernel(ScBlk k, int n, int nv, i
   const long slot = (long)v * nslot + slot0 + grp;
 #pragma unroll
   for (int r = 0; r < SC_T1_REG; ++r) {
-    const long e = tid + 256L * r;
+    const long e = tid + (long)NT * r;
     if (e < nt1) st2(t1p, slot * nt1 + e, acc[r]);
   }
   if (tid == 0) st2(t0p, slot, acc0);
+}
+
+// sum of the per-slot partials x[(v nslot + sl) * len + e] over sl, eight
+// loads in flight
+__device__ inline double2 sum_slots(const double* x, long v, int nslot, long len, long e) {
+  double2 acc[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) acc[u] = make_double2(0.0, 0.0);
+  const double* base = x + 2 * (v * nslot * len + e);
+  int sl = 0;
+  for (; sl + 8 <= nslot; sl += 8)
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] = cadd(acc[u], ld2(base, (long)(sl + u) * len));
+  for (; sl < nslot; ++sl) acc[0] = cadd(acc[0], ld2(base, (long)sl * len));
+#pragma unroll
+  for (int u = 1; u < 8; ++u) acc[0] = cadd(acc[0], acc[u]);
+  return acc[0];
 }
 
 // sum over the split partials of stage 2 at T2p element `e` (stride `ps`)
@@ -300,13 +358,14 @@ __device__ inline double2 sum_parts(const double* T2p, int nsplit, long ps, long
   return t;
 }
 
-// Full outputs (host C order t2[v][a][b][c][d], t1[v][a][b], t0[v]).
+// Full outputs (host C order t2[v][a][b][c][d], t1[v][a][b], t0[v]).  T2 holds
+// the pair-space transform T2[(a<b) + P12 v + P12 nv (c<d)] over c, d < O.
 __global__ void sc_final_full_kernel(int n, int o, int nv, int nsplit, const double* __restrict__ T2p,
                                      const double* __restrict__ RK, const double* __restrict__ t1p,
                                      const double* __restrict__ t0p, int nslot, const double* __restrict__ ovlp,
                                      double* __restrict__ t0, double* __restrict__ t1, double* __restrict__ t2) {
-  const long n2 = (long)n * n, o2 = (long)o * o, no = (long)n * o;
-  const long ps = n2 * nv * o2;   // one split partial: (N^2) x (nv O^2)
+  const long n2 = (long)n * n, no = (long)n * o, np = (long)n * (n - 1) / 2, op = (long)o * (o - 1) / 2;
+  const long ps = np * nv * op;   // one split partial
   const double2 ov = ld2(ovlp, 0);
   const long stride = (long)gridDim.x * blockDim.x;
   const long t0i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -318,7 +377,10 @@ __global__ void sc_final_full_kernel(int n, int o, int nv, int nsplit, const dou
     const long v = idx / (n2 * n2);
     double2 val = make_double2(0.0, 0.0);
     if (c < o && d < o) {
-      val = sum_parts(T2p, nsplit, ps, (a + (long)n * b) + n2 * (v + nv * (c + (long)o * d)));
+      if (a != b && c != d) {
+        val = sum_parts(T2p, nsplit, ps, pair_index(min(a, b), max(a, b)) + np * (v + nv * pair_index(min(c, d), max(c, d))));
+        if ((a < b) != (c < d)) val = make_double2(-val.x, -val.y);
+      }
       val = cadd(val, csub(rk(a, c, b, d, v), rk(b, c, a, d, v)));
       val = cadd(val, csub(rk(b, d, a, c, v), rk(a, d, b, c, v)));
       val = cdiv(val, ov);
@@ -330,15 +392,12 @@ __global__ void sc_final_full_kernel(int n, int o, int nv, int nsplit, const dou
     const long v = idx / n2;
     double2 val = make_double2(0.0, 0.0);
     if (b < o) {
-      for (int sl = 0; sl < nslot; ++sl) val = cadd(val, ld2(t1p, ((long)v * nslot + sl) * no + a + (long)n * b));
-      val = cdiv(val, ov);
+      val = cdiv(sum_slots(t1p, v, nslot, no, a + (long)n * b), ov);
     }
     st2(t1, idx, val);
   }
   for (long v = t0i; v < nv; v += stride) {
-    double2 val = make_double2(0.0, 0.0);
-    for (int sl = 0; sl < nslot; ++sl) val = cadd(val, ld2(t0p, v * nslot + sl));
-    st2(t0, v, cdiv(val, ov));
+    st2(t0, v, cdiv(sum_slots(t0p, v, nslot, 1, 0), ov));
   }
 }
 
@@ -347,9 +406,9 @@ __global__ void sc_final_packed_kernel(int o, int nvir, int nv, long dim, int ns
                                        const double* __restrict__ RK, const double* __restrict__ t1p,
                                        const double* __restrict__ t0p, int nslot, const double* __restrict__ ovlp,
                                        double* __restrict__ vout) {
-  const long v2 = (long)nvir * nvir, o2 = (long)o * o, vo = (long)nvir * o;
-  const long ps = v2 * nv * o2;
-  const long nab = (long)nvir * (nvir - 1) / 2;
+  const long vo = (long)nvir * o;
+  const long nab = (long)nvir * (nvir - 1) / 2, nij = (long)o * (o - 1) / 2;
+  const long ps = nab * nv * nij;
   const double2 ov = ld2(ovlp, 0);
   const long stride = (long)gridDim.x * blockDim.x;
   auto rk = [&](int a, int c, int b, int d, long v) {
@@ -359,21 +418,18 @@ __global__ void sc_final_packed_kernel(int o, int nvir, int nv, long dim, int ns
     const long v = idx / dim, kk = idx % dim;
     double2 val = make_double2(0.0, 0.0);
     if (kk == 0) {
-      for (int sl = 0; sl < nslot; ++sl) val = cadd(val, ld2(t0p, v * nslot + sl));
+      val = sum_slots(t0p, v, nslot, 1, 0);
     } else if (kk <= vo) {
       const long e = kk - 1;   // = i V + a
       const int a = e % nvir, i = e / nvir;
-      for (int sl = 0; sl < nslot; ++sl) val = cadd(val, ld2(t1p, ((long)v * nslot + sl) * vo + a + (long)nvir * i));
+      val = sum_slots(t1p, v, nslot, vo, a + (long)nvir * i);
     } else {
       const long e = kk - 1 - vo;
       const long pij = e / nab, pab = e % nab;
-      int j = 1;
-      while ((long)(j + 1) * j / 2 <= pij) ++j;
-      const int i = (int)(pij - (long)j * (j - 1) / 2);
-      int b = 1;
-      while ((long)(b + 1) * b / 2 <= pab) ++b;
-      const int a = (int)(pab - (long)b * (b - 1) / 2);
-      val = sum_parts(T2p, nsplit, ps, (a + (long)nvir * b) + v2 * (v + nv * (i + (long)o * j)));
+      int a, b, i, j;
+      pair_of(pij, i, j);
+      pair_of(pab, a, b);
+      val = sum_parts(T2p, nsplit, ps, pab + nab * (v + nv * pij));
       val = cadd(val, csub(rk(a, i, b, j, v), rk(b, i, a, j, v)));
       val = cadd(val, csub(rk(b, j, a, i, v), rk(a, j, b, i, v)));
     }
@@ -406,13 +462,15 @@ int ensure_sc_ws(phf_ctx* ctx, size_t bytes) {
   return PHF_OK;
 }
 
-// Sizes of one pass over nv vectors with chunks of gc = nsplit * gs points.
+// Sizes (complex elements) of one pass over nv vectors with chunks of
+// gc = nsplit * gs points.
 struct ScSizes {
   int nv = 0, gs = 1, nsplit = 1, gc = 1, nchunk = 1, nslot = 1;
-  long c2 = 0, c1 = 0, kb = 0, kaw = 0, kvec = 0, yw = 0, pm = 0, qm = 0, out1 = 0, t2p = 0, rk = 0, t1p = 0,
-       io = 0;
+  int pg = 1, npgrp = 1;   // points per t0/t1 workgroup, its groups per chunk
+  long c2 = 0, c2p = 0, c1 = 0, kb = 0, kaw = 0, kvec = 0, yw = 0, pm = 0, qm = 0, out1 = 0, t2p = 0, rk = 0,
+       t1p = 0, io = 0;
   size_t bytes() const {
-    const long e = 2 * c2 + c1 + 2 * nv + kb + kaw + kvec + yw + pm + qm + out1 + t2p + rk + t1p + nslot * (long)nv +
+    const long e = c2 + c2p + c1 + nv + kb + kaw + kvec + yw + pm + qm + out1 + t2p + rk + t1p + nslot * (long)nv +
                    io + 8;
     return sizeof(double) * 2 * (size_t)e + 16 * 256;
   }
@@ -425,18 +483,23 @@ ScSizes sc_sizes(const ScBlk& k, int G, int nv, int gs, int nsplit, long io) {
   z.nsplit = nsplit;
   z.gc = gs * nsplit;
   z.nchunk = (G + z.gc - 1) / z.gc;
-  z.nslot = z.nchunk * nsplit;
+  // one point per one-wave workgroup when its t1 block fits (e12_out O <= 1024),
+  // else enough 256-thread (group, vector) workgroups to hide load latencies
+  z.pg = ((long)k.e12_out * k.O <= 16 * 64) ? 1 : (int)std::max<long>(1, std::min<long>(z.gc, ((long)z.gc * nv + 8191) / 8192));
+  z.npgrp = (z.gc + z.pg - 1) / z.pg;
+  z.nslot = z.nchunk * z.npgrp;
   const long e1 = k.e12_in, e3 = k.e34_in, eo = k.e12_out, o = k.O;
-  z.c2 = e1 * e1 * e3 * e3 * nv;
+  z.c2 = k.p12_in() * k.p34_in() * nv;
+  z.c2p = e1 * e1 * e3 * e3 * nv;
   z.c1 = e1 * e3 * nv;
-  z.kb = o * o * e3 * e3 * z.gc;
-  z.kaw = eo * eo * e1 * e1 * z.gc;
+  z.kb = k.p34_out() * k.p34_in() * z.gc;
+  z.kaw = k.p12_out() * k.p12_in() * z.gc;
   z.kvec = e1 * e3 * z.gc;
   z.yw = eo * o * z.gc;
   z.pm = e1 * e3 * nv * (long)z.gc;
   z.qm = eo * o * nv * (long)z.gc;
-  z.out1 = e1 * e1 * o * o * nv * (long)z.gc;
-  z.t2p = eo * eo * o * o * nv * (long)nsplit;
+  z.out1 = k.p12_out() * nv * k.p34_in() * z.gc;
+  z.t2p = k.p12_out() * nv * k.p34_out() * nsplit;
   z.rk = eo * o * eo * o * nv;
   z.t1p = eo * o * nv * (long)z.nslot;
   z.io = io;
@@ -444,20 +507,22 @@ ScSizes sc_sizes(const ScBlk& k, int G, int nv, int gs, int nsplit, long io) {
 }
 
 // Vectors per pass and the chunking: the largest pass that fits `budget`
-// bytes with batch counts inside the launch limits; stage 2 split so that
-// (tiles x splits) reaches about a thousand workgroups.
+// bytes with launch grids inside their limits; stage 2 split into groups of
+// points so that (tiles x groups) reaches about a thousand workgroups.
 bool sc_plan(const ScBlk& k, int G, int nv_total, long io_per_vec, size_t budget, ScSizes& out) {
   for (int nv = std::min(nv_total, 1024); nv >= 1; nv = (nv + 1) / 2) {
-    const long tm = (long)(k.e12_out * k.e12_out + 63) / 64, tn = ((long)nv * k.O * k.O + 63) / 64;
-    const int target = (int)std::max<long>(1, std::min<long>(256, (1024 + tm * tn - 1) / (tm * tn)));
-    int gcap = std::min(G, 65535 / nv);
-    while (gcap >= 1) {
+    const long m2 = k.p12_out() * nv, n2 = k.p34_out();
+    const long tm = (m2 + 63) / 64, tn = (n2 + 31) / 32;
+    const long tiles = std::max<long>(1, tm * tn);
+    const int target = (int)std::max<long>(1, std::min<long>(256, (1024 + tiles - 1) / tiles));
+    const long eoo = (long)k.e12_out * k.O;
+    const bool fits_grid = ((long)nv * k.p34_in() + 31) / 32 <= 65535 && (eoo * nv + 31) / 32 <= 65535;
+    int gcap = G;
+    while (fits_grid && gcap >= 1) {
       const int nsplit = std::min(target, gcap);
       const int gs = (gcap + nsplit - 1) / nsplit;
       ScSizes z = sc_sizes(k, G, nv, gs, nsplit, io_per_vec * nv);
-      const long eoo = (long)k.e12_out * k.O;
-      const bool fits_grid = ((long)nv * k.O * k.O + 63) / 64 <= 65535 && (eoo * nv + 63) / 64 <= 65535;
-      if (z.gc * (long)nv <= 65535 && fits_grid && z.bytes() <= budget && z.nslot * (long)nv <= (1L << 31)) {
+      if (z.bytes() <= budget && z.nslot * (long)nv <= (1L << 31)) {
         out = z;
         return true;
       }
@@ -474,60 +539,139 @@ size_t sc_budget() {
   return (size_t)std::max<long>(mb, 16) << 20;
 }
 
+// HIP-event timing of the launches of one phf_sc_apply* call (phf_set_profiling):
+// categories 0 stage-1 GEMM, 1 stage-2 GEMM, 2 P GEMM, 3 rank GEMM, 4 point kernel.
+struct ScProf {
+  hipStream_t s;
+  std::vector<hipEvent_t> ev[5];
+  hipEvent_t all0 = nullptr, all1 = nullptr;
+  template <class F>
+  void run(int cat, F&& f) {
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(a, s);
+    f();
+    (void)hipEventRecord(b, s);
+    ev[cat].push_back(a);
+    ev[cat].push_back(b);
+  }
+  // stage_ms: [0] all device work, [1..5] the categories, [6] the rest, [7] GEMM launches
+  void finish(phf_ctx* ctx) {
+    (void)hipEventSynchronize(all1);
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, all0, all1);
+    double rest = t;
+    int ng = 0;
+    for (int c = 0; c < 5; ++c) {
+      double sum = 0.0;
+      for (size_t i = 0; i + 1 < ev[c].size(); i += 2) {
+        float x = 0.f;
+        (void)hipEventElapsedTime(&x, ev[c][i], ev[c][i + 1]);
+        sum += x;
+      }
+      if (c < 4) ng += (int)ev[c].size() / 2;
+      ctx->stage_ms[1 + c] = sum;
+      rest -= sum;
+      for (hipEvent_t e : ev[c]) (void)hipEventDestroy(e);
+      ev[c].clear();
+    }
+    ctx->stage_ms[0] = t;
+    ctx->stage_ms[6] = rest;
+    ctx->stage_ms[7] = ng;
+    ctx->nstage = 8;
+    (void)hipEventDestroy(all0);
+    (void)hipEventDestroy(all1);
+  }
+};
+
+template <class F>
+inline void timed(ScProf* p, int cat, F&& f) {
+  if (p)
+    p->run(cat, f);
+  else
+    f();
+}
+
 // One pass over nv vectors whose c2a / c1 / c0 are already in the workspace.
-void sc_pass(phf_ctx* ctx, const ScBlk& k, const ScSizes& z, double* C2S, double* C2P, double* C1, double* c0v,
-             double* kb, double* kaw, double* kvec, double* yw, double* pm, double* qm, double* out1, double* t2p,
-             double* rk, double* t1p, double* t0p) {
+void sc_pass(phf_ctx* ctx, const ScBlk& k, const ScSizes& z, double* C2, double* C2P, double* C1, double* c0v,
+             double* kbw, double* ka, double* kvec, double* yw, double* pm, double* qm, double* X, double* t2p,
+             double* rk, double* t1p, double* t0p, ScProf* prof) {
   hipStream_t s = ctx->stream;
   const int n = ctx->sc_nso, G = ctx->sc_G, o = k.O, nv = z.nv, gc = z.gc;
-  const long e1 = k.e12_in, e3 = k.e34_in, eo = k.e12_out;
-  const long e12 = e1 * e1, e34 = e3 * e3, o2 = (long)o * o, eoo = eo * o;
+  const long e1 = k.e12_in, e3 = k.e34_in, eo = k.e12_out, eoo = eo * o;
+  const long p12i = k.p12_in(), p12o = k.p12_out(), p34i = k.p34_in(), p34o = k.p34_out();
   const size_t smem = sizeof(double2) * ((size_t)e1 * e3 + (size_t)e1 * o + 8);
+  const bool wave1 = eoo <= 16 * 64;
   if (smem > 65536)
-    (void)hipFuncSetAttribute((const void*)sc_point_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    (void)hipFuncSetAttribute(wave1 ? (const void*)sc_point_kernel<64> : (const void*)sc_point_kernel<256>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
   for (int c = 0; c < z.nchunk; ++c) {
     const int g0 = c * gc;
     const double beta = c ? 1.0 : 0.0;
-    sc_factor_kernel<<<dim3(grid_for(std::max({o2 * e34, eo * eo * e12, e1 * e3}), 256, 1024), gc), 256, 0, s>>>(
-        k, n, g0, G, ctx->sc_A, ctx->sc_BT, ctx->sc_Y, ctx->sc_K, ctx->sc_w, kb, kaw, kvec, yw);
+    sc_factor_kernel<<<dim3(grid_for(std::max({p34o * p34i, p12o * p12i, e1 * e3}), 256, 1024), gc), 256, 0, s>>>(
+        k, n, g0, G, ctx->sc_A, ctx->sc_BT, ctx->sc_Y, ctx->sc_K, ctx->sc_w, ka, kbw, kvec, yw);
     // P((q,r), v, g) = sum_{(p,s)} C2P((q,r), v; (p,s)) Kvec((p,s), g)
-    zgemm_batched(s, (int)(e1 * e3 * nv), gc, (int)(e1 * e3), 1, 0, ZMat{C2P, 2, 2 * e1 * e3 * nv, 0, 1, 0},
-                  ZMat{kvec, 2, 2 * e1 * e3, 0, 1, 0}, 0, 0, zout_colmajor(pm, e1 * e3 * nv, 0), 1);
-    sc_point_kernel<<<dim3(z.nsplit, nv), 256, smem, s>>>(k, n, nv, g0, z.gs, G, z.nslot, c * z.nsplit, c0v, C1, pm,
-                                                          gc, ctx->sc_A, ctx->sc_BT, ctx->sc_Y, ctx->sc_K, ctx->sc_w,
-                                                          qm, t0p, t1p);
-    // stage 1, batch (g, v):  Out1[(g,p,q), (v,i,j)] = sum_{(r,s)} C2S_v((p,q),(r,s)) KB_g((i,j),(r,s))
-    {
-      ZMat a{C2S, 2, 2 * e12 * nv, 0, 1, 0};
-      a.bs2 = 2 * e12;
-      a.bdiv = nv;
-      ZMat b{kb, 2 * o2, 2, 2 * o2 * e34, 1, 0};
-      b.bdiv = nv;
-      ZOut cc{out1, 2, 2 * e12 * gc * nv, 2 * e12, 1};
-      cc.bs2 = 2 * e12 * gc;
-      cc.bdiv = nv;
-      zgemm_batched(s, (int)e12, (int)o2, (int)e34, 1, 0, a, b, 0, 0, cc, gc * nv);
-    }
-    // stage 2, one batch entry per group of gs points: T2p[j] (+)= KAW[:, group j] Out1[group j, :]
-    zgemm_batched(s, (int)(eo * eo), (int)(nv * o2), (int)(z.gs * e12), 1, 0,
-                  ZMat{kaw, 2, 2 * eo * eo, 2 * eo * eo * e12 * z.gs, 1, 0},
-                  ZMat{out1, 2, 2 * e12 * gc, 2 * e12 * z.gs, 1, 0}, beta, 0,
-                  zout_colmajor(t2p, eo * eo, 2 * eo * eo * nv * o2), z.nsplit);
+    timed(prof, 2, [&] {
+      zgemm_batched(s, (int)(e1 * e3 * nv), gc, (int)(e1 * e3), 1, 0, ZMat{C2P, 2, 2 * e1 * e3 * nv, 0, 1, 0},
+                    ZMat{kvec, 2, 2 * e1 * e3, 0, 1, 0}, 0, 0, zout_colmajor(pm, e1 * e3 * nv, 0), 1);
+    });
+    timed(prof, 4, [&] {
+      if (wave1)
+        sc_point_kernel<64><<<dim3(z.npgrp, nv), 64, smem, s>>>(k, n, nv, g0, z.pg, G, z.nslot, c * z.npgrp, c0v, C1,
+                                                                pm, gc, ctx->sc_A, ctx->sc_BT, ctx->sc_Y, ctx->sc_K,
+                                                                ctx->sc_w, qm, t0p, t1p);
+      else
+        sc_point_kernel<256><<<dim3(z.npgrp, nv), 256, smem, s>>>(k, n, nv, g0, z.pg, G, z.nslot, c * z.npgrp, c0v,
+                                                                  C1, pm, gc, ctx->sc_A, ctx->sc_BT, ctx->sc_Y,
+                                                                  ctx->sc_K, ctx->sc_w, qm, t0p, t1p);
+    });
+    // stage 1, batched over the points: X_g[(a<b); (v, r<s)] = sum_{p<q} KA_g((a,b),(p,q)) C2[(p,q); (v, r<s)]
+    timed(prof, 0, [&] {
+      zgemm_batched(s, (int)p12o, (int)(nv * p34i), (int)p12i, 1, 0, ZMat{ka, 2, 2 * p12o, 2 * p12o * p12i, 1, 0},
+                    ZMat{C2, 2, 2 * p12i, 0, 1, 0}, 0, 0, zout_colmajor(X, p12o, 2 * p12o * nv * p34i), gc);
+    });
+    // stage 2, one batch entry per group of gs points, contraction over (point, r<s):
+    //   T2p[j][(a<b, v); (i<j)] (+)= sum_{g in group j, r<s} X_g[(a<b, v); (r<s)] KBW_g((i,j),(r,s))
+    timed(prof, 1, [&] {
+      zgemm_batched(s, (int)(p12o * nv), (int)p34o, (int)(z.gs * p34i), 1, 0,
+                    ZMat{X, 2, 2 * p12o * nv, 2 * p12o * nv * p34i * z.gs, 1, 0},
+                    ZMat{kbw, 2 * p34o, 2, 2 * p34o * p34i * z.gs, 1, 0}, beta, 0,
+                    zout_colmajor(t2p, p12o * nv, 2 * p12o * nv * p34o), z.nsplit);
+    });
     // rank term: RK((a,c),(b,d,v)) (+)= sum_g YW((a,c), g) Qm(g, (b,d,v))
-    zgemm_batched(s, (int)eoo, (int)(eoo * nv), gc, 1, 0, ZMat{yw, 2, 2 * eoo, 0, 1, 0},
-                  ZMat{qm, 2 * eoo * nv, 2, 0, 1, 0}, beta, 0, zout_colmajor(rk, eoo, 0), 1);
+    timed(prof, 3, [&] {
+      zgemm_batched(s, (int)eoo, (int)(eoo * nv), gc, 1, 0, ZMat{yw, 2, 2 * eoo, 0, 1, 0},
+                    ZMat{qm, 2 * eoo * nv, 2, 0, 1, 0}, beta, 0, zout_colmajor(rk, eoo, 0), 1);
+    });
   }
 }
 
+std::unique_ptr<ScProf> sc_prof_begin(phf_ctx* ctx) {
+  if (!ctx->profiling) return nullptr;
+  std::unique_ptr<ScProf> p(new ScProf);
+  p->s = ctx->stream;
+  (void)hipEventCreate(&p->all0);
+  (void)hipEventCreate(&p->all1);
+  (void)hipEventRecord(p->all0, ctx->stream);
+  return p;
+}
+
+void sc_prof_end(phf_ctx* ctx, ScProf* p) {
+  if (!p) return;
+  (void)hipEventRecord(p->all1, ctx->stream);
+  p->finish(ctx);
+}
+
 struct ScBufs {
-  double *C2S, *C2P, *C1, *c0v, *kb, *kaw, *kvec, *yw, *pm, *qm, *out1, *t2p, *rk, *t1p, *t0p, *io;
+  double *C2, *C2P, *C1, *c0v, *kb, *kaw, *kvec, *yw, *pm, *qm, *out1, *t2p, *rk, *t1p, *t0p, *io;
 };
 
 ScBufs carve(char* base, const ScSizes& z) {
   Arena ar{base};
   ScBufs b;
-  b.C2S = ar.take<double>(2 * z.c2);
-  b.C2P = ar.take<double>(2 * z.c2);
+  b.C2 = ar.take<double>(2 * z.c2);
+  b.C2P = ar.take<double>(2 * z.c2p);
   b.C1 = ar.take<double>(2 * z.c1);
   b.c0v = ar.take<double>(2 * z.nv);
   b.kb = ar.take<double>(2 * z.kb);
@@ -693,6 +837,7 @@ extern "C" int phf_sc_apply(phf_ctx* ctx, int nvec, const double* c0, const doub
     const int rc = ensure_sc_ws(ctx, z.bytes());
     if (rc) return rc;
   }
+  std::unique_ptr<ScProf> prof = sc_prof_begin(ctx);
   for (int v0 = 0; v0 < nvec; v0 += z.nv) {
     const int nv = std::min(z.nv, nvec - v0);
     ScSizes zz = sc_sizes(k, G, nv, z.gs, z.nsplit, 2 * (n4 + n2 + 1) * nv);
@@ -706,9 +851,9 @@ extern "C" int phf_sc_apply(phf_ctx* ctx, int nvec, const double* c0, const doub
     HIPCK(hipMemcpyAsync(in2, c2 + 2 * n4 * v0, sizeof(double) * 2 * n4 * nv, hipMemcpyHostToDevice, s));
     HIPCK(hipMemcpyAsync(in1, c1 + 2 * n2 * v0, sizeof(double) * 2 * n2 * nv, hipMemcpyHostToDevice, s));
     HIPCK(hipMemcpyAsync(b.c0v, c0 + 2 * (long)v0, sizeof(double) * 2 * nv, hipMemcpyHostToDevice, s));
-    sc_prep_full_kernel<<<grid_for(n4 * nv), 256, 0, s>>>(n, nv, in2, in1, b.C2S, b.C2P, b.C1);
-    sc_pass(ctx, k, zz, b.C2S, b.C2P, b.C1, b.c0v, b.kb, b.kaw, b.kvec, b.yw, b.pm, b.qm, b.out1, b.t2p, b.rk, b.t1p,
-            b.t0p);
+    sc_prep_full_kernel<<<grid_for(n4 * nv), 256, 0, s>>>(n, nv, in2, in1, b.C2, b.C2P, b.C1);
+    sc_pass(ctx, k, zz, b.C2, b.C2P, b.C1, b.c0v, b.kb, b.kaw, b.kvec, b.yw, b.pm, b.qm, b.out1, b.t2p, b.rk, b.t1p,
+            b.t0p, prof.get());
     sc_final_full_kernel<<<grid_for(n4 * nv), 256, 0, s>>>(n, o, nv, zz.nsplit, b.t2p, b.rk, b.t1p, b.t0p, zz.nslot,
                                                            ctx->sc_ovlp, o0, o1, o2);
     HIPCK(hipGetLastError());
@@ -717,6 +862,7 @@ extern "C" int phf_sc_apply(phf_ctx* ctx, int nvec, const double* c0, const doub
     HIPCK(hipMemcpyAsync(t0 + 2 * (long)v0, o0, sizeof(double) * 2 * nv, hipMemcpyDeviceToHost, s));
     HIPCK(hipStreamSynchronize(s));
   }
+  sc_prof_end(ctx, prof.get());
   return PHF_OK;
 }
 
@@ -743,19 +889,21 @@ extern "C" int phf_sc_apply_packed_dev(phf_ctx* ctx, int nvec, const double* vin
     const int rc = ensure_sc_ws(ctx, z.bytes());
     if (rc) return rc;
   }
+  std::unique_ptr<ScProf> prof = sc_prof_begin(ctx);
   for (int v0 = 0; v0 < nvec; v0 += z.nv) {
     const int nv = std::min(z.nv, nvec - v0);
     ScSizes zz = sc_sizes(k, G, nv, z.gs, z.nsplit, 0);
     ScBufs b = carve(ctx->sc_ws, zz);
     const double* vi = vin + 2 * dim * v0;
-    sc_prep_packed_kernel<<<grid_for((long)nvir * nvir * o * o * nv), 256, 0, s>>>(o, nvir, nv, dim, vi, b.C2S, b.C2P,
+    sc_prep_packed_kernel<<<grid_for((long)nvir * nvir * o * o * nv), 256, 0, s>>>(o, nvir, nv, dim, vi, b.C2, b.C2P,
                                                                                    b.C1, b.c0v);
-    sc_pass(ctx, k, zz, b.C2S, b.C2P, b.C1, b.c0v, b.kb, b.kaw, b.kvec, b.yw, b.pm, b.qm, b.out1, b.t2p, b.rk, b.t1p,
-            b.t0p);
+    sc_pass(ctx, k, zz, b.C2, b.C2P, b.C1, b.c0v, b.kb, b.kaw, b.kvec, b.yw, b.pm, b.qm, b.out1, b.t2p, b.rk, b.t1p,
+            b.t0p, prof.get());
     sc_final_packed_kernel<<<grid_for(dim * nv), 256, 0, s>>>(o, nvir, nv, dim, zz.nsplit, b.t2p, b.rk, b.t1p, b.t0p,
                                                               zz.nslot, ctx->sc_ovlp, vout + 2 * dim * v0);
     HIPCK(hipGetLastError());
   }
+  sc_prof_end(ctx, prof.get());
   return PHF_OK;
 }
 
