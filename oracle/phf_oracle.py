@@ -29,9 +29,8 @@ Pinning (tests/test_oracle_*.py): brute-force Fock-space spin projection on
 small lattices (energies for J=0 and J=1, exact once the grid is exact),
 central finite differences of E(Z) for the gradient (factor 2 on [Re, Im]),
 the R=I mean-field limit against ``make_ene``, numpy.linalg det/inv for the
-LU, closed-form Wigner d for j<=1.  ``buildsc`` is not restated (parity
-unpinned: the reference's BuildSC_Vec is sympy-generated and has no
-independent check available here).
+LU, closed-form Wigner d for j<=1.  BuildSC / BuildSC_Vec / ao2mo4 are in
+``buildsc_oracle.py``.
 """
 from __future__ import annotations
 
