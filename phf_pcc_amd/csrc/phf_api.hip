@@ -479,6 +479,7 @@ extern "C" int phf_ctx_destroy(phf_ctx* ctx) {
   }
   drop_blocks(ctx);
   phf_free_sc(ctx);
+  if (ctx->d_xitems) (void)hipFree(ctx->d_xitems);
   if (ctx->ws) (void)hipFree(ctx->ws);
   if (ctx->small) (void)hipFree(ctx->small);
   for (double* q : {ctx->dGr1p, ctx->dGr2, ctx->dGrpgref, ctx->dGw})
@@ -1091,6 +1092,7 @@ extern "C" int phf_autotune(phf_ctx* ctx, int on) {
   if (!ctx) return PHF_ERR_ARG;
   ctx->autotune = on != 0;
   ctx->tuned.clear();
+  ctx->xgroup.clear();
   return PHF_OK;
 }
 
@@ -1312,7 +1314,7 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
 // placed so that XCD x (workgroups b with b % 8 == x) takes a contiguous
 // eighth of each round.
 void build_h16_items(const std::vector<std::pair<const phf_ctx*, const EvalBufs*>>& units, int prec, int cfg,
-                     int sk, int grid, std::vector<H16Item>& out) {
+                     int sk, int grid, std::vector<H16Item>& out, unsigned blk_mask) {
   const int bm = h16_cfg_bm(cfg), bn = h16_cfg_bn(cfg), ns = mixed_slices(prec);
   const long eb = mixed_elem_bytes(prec);
   struct Unit { int u, blk; long cost; };
@@ -1320,7 +1322,9 @@ void build_h16_items(const std::vector<std::pair<const phf_ctx*, const EvalBufs*
   for (size_t i = 0; i < units.size(); ++i) {
     const phf_ctx* c = units[i].first;
     if (units[i].second->blk)
-      for (int q = 0; q < 3; ++q) work.push_back({(int)i, q, (long)c->blk_k[q]});
+      for (int q = 0; q < 3; ++q) {
+        if (blk_mask >> q & 1u) work.push_back({(int)i, q, (long)c->blk_k[q]});
+      }
     else
       work.push_back({(int)i, -1, (long)c->kt});
   }
@@ -1342,6 +1346,7 @@ void build_h16_items(const std::vector<std::pair<const phf_ctx*, const EvalBufs*
       for (int mt = 0; mt < tm; ++mt)
         for (int ks = 0; ks < sk; ++ks) {
           H16Item it;
+          std::memset(&it, 0, sizeof it);   // padding too: work lists are compared bytewise
           it.P = (const char*)b.Pm + eb * kbeg;
           it.A = Abase + eb * ns * aoff;
           it.invP = b.Pinv;
@@ -1450,8 +1455,79 @@ int eval_gemm(phf_ctx* ctx, const EvalBufs& b) {
       choice[q] = it->second;
       any_split = any_split || choice[q].second > 1;
     }
+    // X1 and X2 have one shape, hence one tuned (cfg, split); each alone
+    // leaves most CUs idle at n = 64 (112 of 256 tiles), so they can run as
+    // one grouped persistent launch.  The first call of a (precision, rows)
+    // times that against the two launches and keeps the faster.
+    const bool can_group = nb == 3 && choice[1] == choice[2];
+    int grouped = 0;
+    if (can_group) {
+      const int cfg = choice[1].first, sk = choice[1].second;
+      const int grid = h16_grouped_grid(cfg, ctx->ncu);
+      // the work list is rebuilt every call (a few hundred items) and
+      // uploaded only when it differs from the one on the device
+      std::vector<H16Item> items;
+      build_h16_items({{ctx, &b}}, precision, cfg, sk, grid, items, 6u);
+      if (items.size() != ctx->h_xitems.size() ||
+          std::memcmp(items.data(), ctx->h_xitems.data(), items.size() * sizeof(H16Item)) != 0) {
+        if (items.size() > ctx->xitems_cap) {
+          if (ctx->d_xitems) HIPCK(hipFree(ctx->d_xitems));
+          ctx->d_xitems = nullptr;
+          HIPCK(hipMalloc(&ctx->d_xitems, items.size() * sizeof(H16Item)));
+          ctx->xitems_cap = items.size();
+        }
+        HIPCK(hipMemcpy(ctx->d_xitems, items.data(), items.size() * sizeof(H16Item), hipMemcpyHostToDevice));
+        ctx->h_xitems.swap(items);
+      }
+      auto run_x = [&](int g) {
+        if (sk > 1) fill_zero(s, b.Gam, b.Mp * nrpad);
+        if (g)
+          eri_gemm_mixed_grouped(s, precision, ctx->d_xitems, (int)ctx->h_xitems.size(), cfg, grid, sk > 1);
+        else
+          for (int q = 1; q < 3; ++q) run_block(q, cfg, sk, s);
+      };
+      const auto gkey = std::make_tuple(precision, 2 * G, rows_of(1), k_of(1));
+      auto it = ctx->xgroup.find(gkey);
+      if (it == ctx->xgroup.end()) {
+        float t[2] = {1e30f, 1e30f};
+        if (ctx->autotune) {
+          hipEvent_t e0, e1;
+          HIPCK(hipEventCreate(&e0));
+          HIPCK(hipEventCreate(&e1));
+          for (int g = 0; g < 2; ++g) {
+            run_x(g);
+            for (int rep = 0; rep < 2; ++rep) {
+              HIPCK(hipEventRecord(e0, s));
+              run_x(g);
+              HIPCK(hipEventRecord(e1, s));
+              HIPCK(hipEventSynchronize(e1));
+              float ms;
+              HIPCK(hipEventElapsedTime(&ms, e0, e1));
+              t[g] = std::min(t[g], ms);
+            }
+          }
+          HIPCK(hipEventDestroy(e0));
+          HIPCK(hipEventDestroy(e1));
+        }
+        int pick = (ctx->autotune && t[1] < t[0]) ? 1 : 0;
+        if (const char* f = getenv("PHF_XGROUP")) pick = atoi(f) != 0;   // pin (tests, experiments)
+        char line[200];
+        snprintf(line, sizeof line, "{\"prec\": %d, \"rows\": %ld, \"exchange_blocks\": {\"two_launches_ms\": %.4f, "
+                 "\"grouped_ms\": %.4f, \"chosen\": \"%s\"}}\n", precision, 2 * G, t[0], t[1],
+                 pick ? "grouped" : "two_launches");
+        ctx->tune_log += line;
+        it = ctx->xgroup.emplace(gkey, pick).first;
+      }
+      grouped = it->second;
+    }
     // blocks write disjoint columns of Gamma: one zero fill before all of them
     if (any_split) fill_zero(s, b.Gam, b.Mp * nrpad);
+    if (grouped) {
+      run_block(0, choice[0].first, choice[0].second, s);
+      eri_gemm_mixed_grouped(s, precision, ctx->d_xitems, (int)ctx->h_xitems.size(), choice[1].first,
+                             h16_grouped_grid(choice[1].first, ctx->ncu), choice[1].second > 1);
+      return PHF_OK;
+    }
     return all_blocks([&](int q, hipStream_t sq) { run_block(q, choice[q].first, choice[q].second, sq); });
   }
 }

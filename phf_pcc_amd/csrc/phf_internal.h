@@ -101,6 +101,12 @@ struct phf_ctx_s {
   // small persistent device buffer for per-call arguments/results outside the arena
   char* small = nullptr;
   size_t small_size = 0;
+  // the two exchange blocks X1, X2 as one grouped launch (reduced forms):
+  // timed once per (precision, rows) against two launches, 1 = grouped
+  std::map<std::tuple<int, long, int, int>, int> xgroup;   // (precision, rows, block rows, block K)
+  H16Item* d_xitems = nullptr;
+  size_t xitems_cap = 0;
+  std::vector<H16Item> h_xitems;   // the work list on the device, as uploaded
   // autotuned ERI GEMM tiles: (precision, rows, cols, K) -> (config, split-K)
   int autotune = 1;
   std::map<std::tuple<int, long, int, int>, std::pair<int, int>> tuned;
@@ -191,8 +197,9 @@ int eval_plan(phf_ctx* ctx, int nocc, int precision, int cmplxconj, EvalBufs& b)
 void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks = nullptr);
 int eval_gemm(phf_ctx* ctx, const EvalBufs& b);
 // Work list of a grouped 16-bit ERI GEMM over (context, buffers) units.
+// blk_mask selects the spin blocks of blocked units (bit q: block q).
 void build_h16_items(const std::vector<std::pair<const phf_ctx*, const EvalBufs*>>& units, int prec, int cfg,
-                     int sk, int grid, std::vector<H16Item>& out);
+                     int sk, int grid, std::vector<H16Item>& out, unsigned blk_mask = 7u);
 void eval_post(phf_ctx* ctx, const EvalBufs& b, bool resident);
 // C0 = [I; Z] column-major complex (NSO x N) from Z ((NSO-N) x N, C order)
 void build_c0(const double* z, int nso, int N, int cmplxconj, double* c0);

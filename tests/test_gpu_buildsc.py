@@ -143,3 +143,24 @@ def test_singular_overlap_raises():
     with pytest.raises(_lib.SingularOverlapError):
         ProjectedOverlap(nso, nocc, [2, 1], np.stack([eye[0], perm]), eye, eye, np.ones((2, 1)),
                          np.ones((1, 1, 1)))
+
+
+def test_full_and_packed_paths_agree_nso48():
+    """NSO = 48, NOcc = 40 on 2 grid points: the full-tensor path (pair spaces
+    over all 48 orbitals; 256-thread t0/t1 workgroups with 68 KB of LDS) and
+    the packed path (V = 8) on the same vectors, a size-independent check of
+    both against each other (the oracle would take minutes here)."""
+    from phf_pcc_amd.sc import ProjectedOverlap
+    nso, nocc = 48, 40
+    g = _mo_grid(24, (1, 2), 15)
+    op = ProjectedOverlap(nso, nocc, g["ngrid"], g["r1"], g["r2"], g["rpg"], g["weightsp"], g["weightpg"])
+    rng = np.random.default_rng(16)
+    x = _cr(rng, op.dim, 2)
+    packed = op.matmat(x)
+    c0 = np.array([x[0, k] for k in range(2)])
+    mats = [B.vec2mat(x[:, k], nso, nocc) for k in range(2)]
+    t0, t1, t2 = op.apply_full(c0, np.stack([m[1] for m in mats]), np.stack([m[2] for m in mats]))
+    for k in range(2):
+        full = B.mat2vec(t0[k], t1[k], t2[k], nocc)
+        assert _rel(full, packed[:, k]) <= 1e-11
+    print(f"nso=48 dim={op.dim}", flush=True)

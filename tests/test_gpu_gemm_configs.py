@@ -128,3 +128,21 @@ def test_grouped_batch_config(prec, cfg, sk):
     assert np.abs(e - e64).max() <= 1e-6
     for a, b in zip(g, g64):
         assert np.abs(a - b).max() <= 1e-5
+
+
+@pytest.mark.parametrize("prec", ["f16x2", "i8x4"])
+def test_exchange_blocks_grouped_equal_two_launches(prob, fp64, prec):
+    """X1 and X2 as one grouped launch (PHF_XGROUP=1) against two launches
+    (PHF_XGROUP=0): the same tiles and arithmetic, so results agree to 1e-9."""
+    out = {}
+    for g in ("0", "1"):
+        os.environ["PHF_XGROUP"] = g
+        try:
+            out[g] = _eval(prob, prec, True)
+        finally:
+            os.environ.pop("PHF_XGROUP", None)
+    (e0, g0), (e1, g1) = out["0"], out["1"]
+    assert abs(e0 - e1) <= 1e-9
+    assert np.abs(g0 - g1).max() <= 1e-8
+    e64, _ = fp64[True]
+    assert abs(e1 - e64) <= 1e-6
