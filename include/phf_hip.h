@@ -235,6 +235,14 @@ int phf_sc_apply(phf_ctx* ctx, int nvec, const double* c0, const double* c1, con
 int phf_sc_apply_packed(phf_ctx* ctx, int nvec, const double* vin, double* vout);
 int phf_sc_apply_packed_dev(phf_ctx* ctx, int nvec, const double* vin, double* vout);
 long phf_sd_dim(int nso, int nocc);
+/* buildsc_dr.buildsc_vec (BuildSC_Vec, BuildSC_dr.f90:8-1244) and any other
+ * weighted sum over given points: replaces the grid of phf_sc_set_grid with
+ * npts points of rotation r (complex[npts][nso][nso], NULL for identities),
+ * transition density y (complex[npts][nso][nso], every column used) and
+ * weight w (complex[npts], NULL for ones); phf_sc_apply then returns
+ * sum_g w_g BuildSC_Vec(c0, R c1 R^H, ao2mo4(c2, R^H), Y_g) / sum_g w_g.
+ * With npts = 1, r = w = NULL it is BuildSC_Vec(c0, c1, c2, Y) itself. */
+int phf_sc_set_points(phf_ctx* ctx, int nso, int npts, const double* r, const double* y, const double* w);
 
 /* Batched evaluation of many independent instances (no counterpart in the
  * reference, which evaluates one Hamiltonian per process; each instance is

@@ -12,6 +12,7 @@ the one f2py gives the Fortran module procedures (call sites PHF.py:70-71,76):
   t0, t1, t2 = phftools.buildsc(c0, c1, c2, nocc, ngrid, qj, r1, r2, rpg, rk,
       roota, rootb, rooty, weightsp, weightpg, fsp, fpg, fk,
       [nso, npg, ncisp, ncipg, ncik])                      (LinDep.py:73)
+  t0, t1, t2 = buildsc_dr.buildsc_vec(c0, c1, c2, y, [nso])  (BuildSC_dr.f90:8)
 
 Inputs are cast to complex128 / float64 / int as f2py would; a shape that
 disagrees with the explicit dimension arguments raises ValueError.  Numerical
@@ -246,4 +247,26 @@ class _PhfTools:
         return t0[0], np.asfortranarray(t1[0]), np.asfortranarray(t2[0])
 
 
+class _BuildSCDr:
+    """The f2py module's ``buildsc_dr`` (BuildSC_dr.f90), imported at LinDep.py:6."""
+
+    def __init__(self, tools: _PhfTools):
+        self._tools = tools
+
+    def buildsc_vec(self, c0, c1, c2, y, nso=None):
+        """BuildSC_Vec (BuildSC_dr.f90:8-1244) on the GPU: (t0, t1, t2) for any Y."""
+        c1 = np.ascontiguousarray(c1, dtype=np.complex128)
+        c2 = np.ascontiguousarray(c2, dtype=np.complex128)
+        y = np.ascontiguousarray(y, dtype=np.complex128)
+        nso = c1.shape[0] if nso is None else int(nso)
+        if c1.shape != (nso, nso) or c2.shape != (nso,) * 4 or y.shape != (nso, nso):
+            raise ValueError("buildsc_vec: c1 and y must be (nso, nso), c2 (nso, nso, nso, nso)")
+        ctx = self._tools.context()
+        ctx.sc_key = None          # the grid of phftools.buildsc is replaced
+        ctx.sc_set_points(y)
+        t0, t1, t2 = ctx.sc_apply(np.array([complex(np.asarray(c0).reshape(-1)[0])]), c1[None], c2[None])
+        return t0[0], np.asfortranarray(t1[0]), np.asfortranarray(t2[0])
+
+
 phftools = _PhfTools()
+buildsc_dr = _BuildSCDr(phftools)

@@ -35,7 +35,7 @@ EXPORTS = ["phf_ctx_create", "phf_ctx_destroy", "phf_last_error", "phf_set_hamil
            "phf_batch_last_error", "phf_batch_set_instance", "phf_batch_eval", "phf_batch_set_graph",
            "phf_batch_autotune", "phf_batch_stats", "phf_batch_tuning_report", "phf_set_blocked", "phf_blocked",
            "phf_buildsc", "phf_sc_set_grid", "phf_sc_apply", "phf_sc_apply_packed", "phf_sc_apply_packed_dev",
-           "phf_sd_dim"]
+           "phf_sd_dim", "phf_sc_set_points"]
 
 
 class PhfError(RuntimeError):
@@ -108,6 +108,7 @@ def load() -> C.CDLL:
     lib.phf_sc_apply_packed.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
     lib.phf_sc_apply_packed_dev.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
     lib.phf_sd_dim.argtypes = [C.c_int, C.c_int]
+    lib.phf_sc_set_points.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
     for name in EXPORTS:
         if name not in ("phf_last_error", "phf_batch_last_error"):
             getattr(lib, name).restype = C.c_int
@@ -231,6 +232,22 @@ class Context:
                                      r2.shape[0], ptr(rpg), npg, ptr(weightsp), ptr(weightpg), ncipg, ptr(fsp),
                                      fsp.size, ptr(fpg)), self._h)
         self.sc_shape = (nso, nocc)
+
+    def sc_set_points(self, y, r=None, w=None):
+        """Points of given rotation r, transition density y and weight w
+        (stacked along a leading axis; r, w default to identities and ones)."""
+        y = np.ascontiguousarray(y, dtype=np.complex128)
+        if y.ndim == 2:
+            y = y[None]
+        npts, nso = y.shape[0], y.shape[1]
+        if y.shape != (npts, nso, nso):
+            raise ValueError("sc_set_points: y must be (npts, nso, nso)")
+        if r is not None:
+            r = np.ascontiguousarray(np.broadcast_to(r, y.shape), dtype=np.complex128)
+        if w is not None:
+            w = np.ascontiguousarray(np.asarray(w, dtype=np.complex128).reshape(npts))
+        check(load().phf_sc_set_points(self._h, nso, npts, ptr(r), ptr(y), ptr(w)), self._h)
+        self.sc_shape = (nso, nso)
 
     def sc_apply(self, c0, c1, c2):
         """(t0, t1, t2) for stacked inputs c0 [nv], c1 [nv, nso, nso], c2 [nv, nso^4]."""

@@ -818,6 +818,81 @@ extern "C" int phf_sc_set_grid(phf_ctx* ctx, int nso, int nocc, const int* ngrid
   return PHF_OK;
 }
 
+namespace {
+// row-major host matrices -> column-major (batch of n x n complex)
+__global__ void sc_colmajor_kernel(int n, long total, const double* __restrict__ in, double* __restrict__ out) {
+  const long n2 = (long)n * n;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const long g = idx / n2, e = idx % n2;
+    const int i = e % n, j = e / n;
+    st2(out, idx, ld2(in, g * n2 + (long)i * n + j));
+  }
+}
+}  // namespace
+
+extern "C" int phf_sc_set_points(phf_ctx* ctx, int nso, int npts, const double* r, const double* y, const double* w) {
+  if (!ctx) return PHF_ERR_ARG;
+  if (!y || nso < 2 || nso > 64 || npts < 1 || npts > 65535)
+    return phf_fail(ctx, PHF_ERR_ARG, "phf_sc_set_points: need Y, 2 <= nso <= 64 and 1 <= npts <= 65535");
+  HIPCK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  HIPCK(hipStreamSynchronize(s));
+  free_sc_grid(ctx);
+  const int n = nso;
+  const long n2 = (long)n * n, G = npts;
+  HIPCK(hipMalloc(&ctx->sc_R, sizeof(double) * 2 * G * n2));
+  HIPCK(hipMalloc(&ctx->sc_A, sizeof(double) * 2 * G * n2));
+  HIPCK(hipMalloc(&ctx->sc_K, sizeof(double) * 2 * G * n2));
+  HIPCK(hipMalloc(&ctx->sc_Y, sizeof(double) * 2 * G * n2));
+  HIPCK(hipMalloc(&ctx->sc_BT, sizeof(double) * 2 * G * n2));
+  HIPCK(hipMalloc(&ctx->sc_w, sizeof(double) * 2 * G));
+  HIPCK(hipMalloc(&ctx->sc_ovlp, sizeof(double) * 2));
+  {
+    const int rc = ensure_sc_ws(ctx, sizeof(double) * (2 * G * n2 + 3 * G) + 4 * 256);
+    if (rc) return rc;
+  }
+  Arena ar{ctx->sc_ws};
+  double* stage = ar.take<double>(2 * G * n2);
+  double* dW = ar.take<double>(2 * G);
+  double* dOne = ar.take<double>(G);
+  std::vector<double> ones(G, 1.0), wv(2 * G);
+  for (long g = 0; g < G; ++g) {
+    wv[2 * g] = w ? w[2 * g] : 1.0;
+    wv[2 * g + 1] = w ? w[2 * g + 1] : 0.0;
+  }
+  const int gr = (int)std::min<long>((G * n2 + 255) / 256, 16384);
+  if (r) {
+    HIPCK(hipMemcpyAsync(stage, r, sizeof(double) * 2 * G * n2, hipMemcpyHostToDevice, s));
+    sc_colmajor_kernel<<<gr, 256, 0, s>>>(n, G * n2, stage, ctx->sc_R);
+  } else {
+    std::vector<double> eye(2 * G * n2, 0.0);
+    for (long g = 0; g < G; ++g)
+      for (int i = 0; i < n; ++i) eye[2 * (g * n2 + i + (long)n * i)] = 1.0;
+    HIPCK(hipMemcpyAsync(ctx->sc_R, eye.data(), sizeof(double) * eye.size(), hipMemcpyHostToDevice, s));
+    HIPCK(hipStreamSynchronize(s));
+  }
+  HIPCK(hipStreamSynchronize(s));
+  HIPCK(hipMemcpyAsync(stage, y, sizeof(double) * 2 * G * n2, hipMemcpyHostToDevice, s));
+  sc_colmajor_kernel<<<gr, 256, 0, s>>>(n, G * n2, stage, ctx->sc_Y);
+  HIPCK(hipMemcpyAsync(dW, wv.data(), sizeof(double) * 2 * G, hipMemcpyHostToDevice, s));
+  HIPCK(hipMemcpyAsync(dOne, ones.data(), sizeof(double) * G, hipMemcpyHostToDevice, s));
+  // B^T = R^H Y;  K = B^T R;  A = R - Y R  (Y has all N columns here)
+  zgemm_batched(s, n, n, n, 1, 0, zmat_colmajor_t(ctx->sc_R, n, 2 * n2, 1), zmat_colmajor(ctx->sc_Y, n, 2 * n2), 0, 0,
+                zout_colmajor(ctx->sc_BT, n, 2 * n2), (int)G);
+  zgemm_batched(s, n, n, n, 1, 0, zmat_colmajor(ctx->sc_BT, n, 2 * n2), zmat_colmajor(ctx->sc_R, n, 2 * n2), 0, 0,
+                zout_colmajor(ctx->sc_K, n, 2 * n2), (int)G);
+  HIPCK(hipMemcpyAsync(ctx->sc_A, ctx->sc_R, sizeof(double) * 2 * G * n2, hipMemcpyDeviceToDevice, s));
+  zgemm_batched(s, n, n, n, -1, 0, zmat_colmajor(ctx->sc_Y, n, 2 * n2), zmat_colmajor(ctx->sc_R, n, 2 * n2), 1, 0,
+                zout_colmajor(ctx->sc_A, n, 2 * n2), (int)G);
+  sc_weight_kernel<<<1, 256, 0, s>>>((int)G, dOne, dW, ctx->sc_w, ctx->sc_ovlp);
+  HIPCK(hipGetLastError());
+  HIPCK(hipStreamSynchronize(s));
+  ctx->sc_G = (int)G;
+  ctx->sc_nso = n;
+  ctx->sc_nocc = n;
+  return PHF_OK;
+}
+
 extern "C" int phf_sc_apply(phf_ctx* ctx, int nvec, const double* c0, const double* c1, const double* c2, double* t0,
                             double* t1, double* t2) {
   if (!ctx) return PHF_ERR_ARG;

@@ -164,3 +164,17 @@ def test_full_and_packed_paths_agree_nso48():
         full = B.mat2vec(t0[k], t1[k], t2[k], nocc)
         assert _rel(full, packed[:, k]) <= 1e-11
     print(f"nso=48 dim={op.dim}", flush=True)
+
+
+@pytest.mark.parametrize("nso,seed", [(6, 21), (9, 22)])
+def test_buildsc_vec_any_density(nso, seed):
+    """buildsc_dr.buildsc_vec (BuildSC_Vec for an arbitrary Y, every column
+    used) against the oracle's statement-by-statement restatement."""
+    from phf_pcc_amd.PHFTools_F import buildsc_dr
+    rng = np.random.default_rng(seed)
+    c0, c1, c2, y = _cr(rng), _cr(rng, nso, nso), _cr(rng, nso, nso, nso, nso), 0.3 * _cr(rng, nso, nso)
+    want = B.buildsc_vec(c0, c1, c2, y)
+    got = buildsc_dr.buildsc_vec(c0, c1, c2, y)
+    assert abs(got[0] - want[0]) <= TOL * abs(want[0])
+    assert _rel(got[1], want[1]) <= TOL
+    assert _rel(got[2], want[2]) <= TOL
