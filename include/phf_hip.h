@@ -151,6 +151,17 @@ int phf_buildhsg_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int p
  * phf_localg there). */
 int phf_eval_resident(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int precision, double* hmat,
                       double* smat);
+/* One projected energy + gradient evaluation in one call when the grid has
+ * nci = 1 (J = 0, one point-group irrep) and CmplxConj is 0 or 1: the 1 x 1
+ * EigenSolver (PHFTools.py:8-18: E0 = Re H / |Re S|), VecDecomp and LocalG
+ * run on the device after the grid sums, and only E0 and the gradient block
+ * dE/dZ* = LocalG[N:, :N] (complex [nso-nocc][nocc]) cross PCIe.  The same
+ * errors as phf_eval_resident, plus PHF_ERR_SINGULAR when |Re S| <= 1e-6 (the
+ * EigenSolver would keep no state).  The sums stay resident as after
+ * phf_eval_resident.  No counterpart in the reference (EandG, PHF.py:55-91,
+ * does these steps on the host). */
+int phf_eval_objective(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int precision, double* e0,
+                       double* grad);
 int phf_localg_vo(phf_ctx* ctx, const double* smat, const double* e0, const double* fsp, int ncisp,
                   const double* fpg, int ncipg, const double* fk, int nfk, int cmplxconj, double* gvo);
 

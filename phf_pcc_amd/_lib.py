@@ -35,7 +35,7 @@ EXPORTS = ["phf_ctx_create", "phf_ctx_destroy", "phf_last_error", "phf_set_hamil
            "phf_batch_last_error", "phf_batch_set_instance", "phf_batch_eval", "phf_batch_set_graph",
            "phf_batch_autotune", "phf_batch_stats", "phf_batch_tuning_report", "phf_set_blocked", "phf_blocked",
            "phf_buildsc", "phf_sc_set_grid", "phf_sc_apply", "phf_sc_apply_packed", "phf_sc_apply_packed_dev",
-           "phf_sd_dim", "phf_sc_set_points"]
+           "phf_sd_dim", "phf_sc_set_points", "phf_eval_objective"]
 
 
 class PhfError(RuntimeError):
@@ -109,6 +109,7 @@ def load() -> C.CDLL:
     lib.phf_sc_apply_packed_dev.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
     lib.phf_sd_dim.argtypes = [C.c_int, C.c_int]
     lib.phf_sc_set_points.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
+    lib.phf_eval_objective.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, _dp, _dp]
     for name in EXPORTS:
         if name not in ("phf_last_error", "phf_batch_last_error"):
             getattr(lib, name).restype = C.c_int
@@ -194,7 +195,7 @@ class Context:
         check(load().phf_autotune(self._h, 1 if on else 0), self._h)
 
     def tuning_report(self) -> str:
-        buf = C.create_string_buffer(1 << 16)
+        buf = C.create_string_buffer(1 << 20)
         n = load().phf_tuning_report(self._h, buf, len(buf))
         return buf.raw[:n].decode()
 

@@ -348,6 +348,32 @@ __global__ void localg_vo_kernel(int nso, int nocc, int nterm, const int* __rest
   out[2 * e + 1] = (ni * ovr - nr * ovi) / d;
 }
 
+// nci = 1 (one CI state): EigenSolver, VecDecomp and LocalG on the device
+// (PHFTools.py:8-26, PHFTools.f90:203-247).  The 1 x 1 canonical
+// orthogonalisation gives E0 = Re H / |Re S| and fsp = |Re S|^{-1/2}, so the
+// gradient block is (Ysum - E0 Xsum)[N:, :] f / (f S) with f = 1 / |Re S|.
+// out[0..1] = E0, out[2..] = the (NSO - N) x N block, C order.
+__global__ void objective_kernel(int nso, int nocc, const double* __restrict__ hs, const double* __restrict__ ysum,
+                                 const double* __restrict__ xsum, double* __restrict__ out) {
+  const int nv = nso - nocc;
+  const double as = fabs(hs[2]);
+  const double e0 = hs[0] / as, f = 1.0 / as;
+  const double ovr = hs[2] * f, ovi = hs[3] * f;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e == 0) {
+    out[0] = e0;
+    out[1] = 0.0;
+  }
+  if (e >= nv * nocc) return;
+  const int pv = e / nocc, a = e % nocc;
+  const long o = (long)(nocc + pv) + (long)a * nso;
+  const double gr = f * ysum[2 * o], gi = f * ysum[2 * o + 1], rr = f * xsum[2 * o], ri = f * xsum[2 * o + 1];
+  const double nr = gr - e0 * rr, ni = gi - e0 * ri;
+  const double d = ovr * ovr + ovi * ovi;
+  out[2 + 2 * e] = (nr * ovr + ni * ovi) / d;
+  out[2 + 2 * e + 1] = (ni * ovr - nr * ovi) / d;
+}
+
 // ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
@@ -482,7 +508,7 @@ extern "C" int phf_ctx_destroy(phf_ctx* ctx) {
   if (ctx->d_xitems) (void)hipFree(ctx->d_xitems);
   if (ctx->ws) (void)hipFree(ctx->ws);
   if (ctx->small) (void)hipFree(ctx->small);
-  for (double* q : {ctx->dGr1p, ctx->dGr2, ctx->dGrpgref, ctx->dGw})
+  for (double* q : {ctx->dGr1p, ctx->dGr2, ctx->dGrpgref, ctx->dGw, ctx->dGu, ctx->dGref})
     if (q) (void)hipFree(q);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
@@ -924,11 +950,12 @@ extern "C" int phf_get_hone(phf_ctx* ctx, double* hone) {
 
 static int free_grid(phf_ctx* ctx) {
   HIPCK(hipStreamSynchronize(ctx->stream));
-  for (double** q : {&ctx->dGr1p, &ctx->dGr2, &ctx->dGrpgref, &ctx->dGw})
+  for (double** q : {&ctx->dGr1p, &ctx->dGr2, &ctx->dGrpgref, &ctx->dGw, &ctx->dGu, &ctx->dGref})
     if (*q) {
       HIPCK(hipFree(*q));
       *q = nullptr;
     }
+  ctx->g_spinrot = false;
   ctx->has_grid = false;
   return PHF_OK;
 }
@@ -1015,6 +1042,52 @@ extern "C" int phf_set_grid(phf_ctx* ctx, const double* ref, const int* ngrid, i
                 0, zout_colmajor(ctx->dGrpgref, nso, 2 * n2), npg);
   HIPCK(hipGetLastError());
   HIPCK(hipStreamSynchronize(s));
+  // Spin-rotation grid (every R1[il], R2[iy] exactly u (x) I_n): then
+  // Ref^H R1 R2 W = sum_{s,t} U_g[s][t] Ref_s^H W_t with U_g = u1[il] u2[iy]
+  // and Ref_s the rows of spin s, so the rotation chain needs four NSO x N x n
+  // products per evaluation instead of NSO^2 N per point.
+  {
+    std::vector<cplx> u1(4 * (size_t)nl), u2(4 * (size_t)ny);
+    auto su2 = [&](const double* R, cplx* u) {
+      if (nso % 2) return false;
+      const int n = nso / 2;
+      for (int st = 0; st < 4; ++st) {
+        const long e = (long)((st >> 1) * n) * nso + (st & 1) * n;
+        u[st] = cplx(R[2 * e], R[2 * e + 1]);
+      }
+      for (int i = 0; i < nso; ++i)
+        for (int j = 0; j < nso; ++j) {
+          const cplx want = (i % n == j % n) ? u[(i / n) * 2 + j / n] : cplx(0.0, 0.0);
+          const long e = (long)i * nso + j;
+          if (R[2 * e] != want.real() || R[2 * e + 1] != want.imag()) return false;
+        }
+      return true;
+    };
+    bool ok = true;
+    for (int il = 0; il < nl && ok; ++il) ok = su2(r1 + 2 * (long)il * n2, &u1[4 * il]);
+    for (int iy = 0; iy < ny && ok; ++iy) ok = su2(r2 + 2 * (long)iy * n2, &u2[4 * iy]);
+    if (ok) {
+      std::vector<double> ug(8 * (size_t)G);
+      for (int il = 0; il < nl; ++il)
+        for (int iy = 0; iy < ny; ++iy)
+          for (int ipg = 0; ipg < npg; ++ipg) {
+            const long g = ((long)il * ny + iy) * npg + ipg;
+            for (int a = 0; a < 2; ++a)
+              for (int c = 0; c < 2; ++c) {
+                const cplx v = u1[4 * il + 2 * a] * u2[4 * iy + c] + u1[4 * il + 2 * a + 1] * u2[4 * iy + 2 + c];
+                ug[8 * g + 2 * (2 * a + c)] = v.real();
+                ug[8 * g + 2 * (2 * a + c) + 1] = v.imag();
+              }
+          }
+      HIPCK(hipMalloc(&ctx->dGu, sizeof(double) * ug.size()));
+      HIPCK(hipMalloc(&ctx->dGref, sizeof(double) * 2 * n2));
+      HIPCK(hipMemcpy(ctx->dGu, ug.data(), sizeof(double) * ug.size(), hipMemcpyHostToDevice));
+      HIPCK(hipMemcpy(ctx->dGref, ref, sizeof(double) * 2 * n2, hipMemcpyHostToDevice));
+    }
+    ctx->g_spinrot = ok;
+    ctx->tune_log += std::string("{\"grid\": {\"points\": ") + std::to_string(G) + ", \"rotation\": \"" +
+                     (ok ? "su2 x identity" : "general") + "\"}}\n";
+  }
   // spin-block structure of a dense Hamiltonian, now that Ref is known
   if (ctx->dA && !ctx->cplx && !(ctx->ham_flags & PHF_HAM_DENSE)) {
     int rc = phf_build_blocks(ctx, ref_spins(ref, nso), false);
@@ -1156,7 +1229,7 @@ int eval_plan(phf_ctx* ctx, int nocc, int precision, int cmplxconj, EvalBufs& b)
     o.C0 = a.take<double>(2 * lenX);
     o.Wpg = a.take<double>(2 * lenX * npg * ((V == 4) ? 2 : 1));
     const int nl_ = (V == 4) ? 2 : 1;          // distinct left factors C0, C0*
-    o.W2 = a.take<double>(2 * lenX * npg * ny * nl_);
+    o.W2 = a.take<double>(2 * lenX * npg * std::max(ny, 4) * nl_);   // also the four spin products Q
     o.T = a.take<double>(2 * lenX * G * nl_);
     o.M = a.take<double>(2 * (size_t)N * N * Gv);
     o.Minv = a.take<double>(2 * (size_t)N * N * Gv);
@@ -1218,6 +1291,29 @@ static void hc_products(phf_ctx* ctx, const EvalBufs& b, hipStream_t s) {
                   ZMat{ctx->dH1, 2L * nso, 2, 0, 1, 0}, 0, 0, zout_colmajor(b.Hc + bc * sT, N, 0), 1);
 }
 
+namespace {
+// T_g = sum_{s,t} U_g[s][t] Q[s][t][g % npg] (NSO x N blocks, complex)
+__global__ void spinrot_kernel(long G, int npg, long lenX, const double* __restrict__ U, const double* __restrict__ Q,
+                               double* __restrict__ T) {
+  const long tot = G * lenX;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += (long)gridDim.x * blockDim.x) {
+    const long g = idx / lenX, e = idx % lenX;
+    const int ipg = (int)(g % npg);
+    double re = 0.0, im = 0.0;
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      const double ur = U[8 * g + 2 * st], ui = U[8 * g + 2 * st + 1];
+      const long q = 2 * ((st * (long)npg + ipg) * lenX + e);
+      const double qr = Q[q], qi = Q[q + 1];
+      re += ur * qr - ui * qi;
+      im += ur * qi + ui * qr;
+    }
+    T[2 * idx] = re;
+    T[2 * idx + 1] = im;
+  }
+}
+}  // namespace
+
 void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
   hipStream_t s = ctx->stream;
   const int nso = ctx->nso, N = b.N, npg = ctx->g_npg, ny = ctx->g_ny, nl = ctx->g_nl, V = b.V;
@@ -1243,6 +1339,29 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
     // Wpg[ipg] = (Rpg[ipg] Ref) L
     zgemm_batched(s, nso, N, nso, 1, 0, zmat_colmajor(ctx->dGrpgref, nso, 2 * n2), zmat_colmajor(b.C0, nso, 0, lc),
                   0, 0, zout_colmajor(Wpg, nso, sT), npg);
+    if (ctx->g_spinrot) {
+      // Q[s][t][ipg] = Ref_s^H (Wpg[ipg] rows of spin t), then T_g = sum U_g[s][t] Q[s][t][ipg]
+      const int n = nso / 2;
+      double* Q = W2;
+      if (npg == 1) {   // one launch: batch entry st = 2 s + t
+        ZMat a{ctx->dGref, 2, 2L * nso, 2L * n * nso, 1, 1};
+        a.bdiv = 2;     // Ref_s^H: s = st / 2
+        ZMat w{Wpg, 2, 2L * nso, 0, 1, 0};
+        w.bs2 = 2L * n;
+        w.bdiv = 2;     // rows of spin t = st % 2
+        zgemm_batched(s, nso, N, n, 1, 0, a, w, 0, 0, zout_colmajor(Q, nso, sT), 4);
+      } else {
+        for (int st = 0; st < 4; ++st) {
+          const int sa = st >> 1, tb = st & 1;
+          zgemm_batched(s, nso, N, n, 1, 0, ZMat{ctx->dGref + 2L * sa * n * nso, 2, 2L * nso, 0, 1, 1},
+                        ZMat{Wpg + 2L * tb * n, 2, 2L * nso, sT, 1, 0}, 0, 0,
+                        zout_colmajor(Q + st * sT * npg, nso, sT), npg);
+        }
+      }
+      spinrot_kernel<<<(unsigned)std::min<long>((G * lenX + 255) / 256, 65535), 256, 0, s>>>(G, npg, lenX, ctx->dGu,
+                                                                                            Q, T);
+      continue;
+    }
     // W2[iy] = R2[iy] [Wpg_0 .. Wpg_{npg-1}]
     zgemm_batched(s, nso, N * npg, nso, 1, 0, ZMat{ctx->dGr2, 2L * nso, 2, 2 * n2, 1, 0}, zmat_colmajor(Wpg, nso, 0), 0,
                   0, zout_colmajor(W2, nso, sT * npg), ny);
@@ -1594,7 +1713,7 @@ void eval_post(phf_ctx* ctx, const EvalBufs& b, bool resident) {
 }
 
 static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int precision, double* hmat,
-                  double* smat, double* gradmat, double* rdmmat, bool resident) {
+                  double* smat, double* gradmat, double* rdmmat, bool resident, double* objective = nullptr) {
   if (!ctx) return PHF_ERR_ARG;
   ctx->r_valid = false;
   if (!hmat || !smat) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: NULL output pointer");
@@ -1633,6 +1752,22 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
   if (!resident) {
     HIPCK(hipMemcpyAsync(gradmat, b.Grad, 2 * (size_t)nmn * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCK(hipMemcpyAsync(rdmmat, b.Rdm, 2 * (size_t)nmn * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  const size_t nobj = 2 + 2 * (size_t)(nso - N) * N;
+  if (objective) {
+    if (ctx->small_size < sizeof(double) * nobj) {
+      HIPCK(hipStreamSynchronize(s));
+      if (ctx->small) HIPCK(hipFree(ctx->small));
+      ctx->small = nullptr;
+      ctx->small_size = 0;
+      HIPCK(hipMalloc(&ctx->small, sizeof(double) * nobj));
+      ctx->small_size = sizeof(double) * nobj;
+    }
+    double* dob = reinterpret_cast<double*>(ctx->small);
+    const int nel = (nso - N) * N;
+    objective_kernel<<<(std::max(nel, 1) + 255) / 256, 256, 0, s>>>(nso, N, b.HS, b.Ysum, b.Xsum, dob);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(objective, dob, sizeof(double) * nobj, hipMemcpyDeviceToHost, s));
   }
   HIPCK(hipStreamSynchronize(s));
   if (ctx->profiling) {
@@ -1697,6 +1832,23 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
   ctx->r_xsum = b.Xsum;
   ctx->r_nocc = N;
   ctx->r_valid = true;
+  return PHF_OK;
+}
+
+extern "C" int phf_eval_objective(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int precision,
+                                  double* e0, double* grad) {
+  if (!ctx) return PHF_ERR_ARG;
+  if (!e0 || !grad) return fail(ctx, PHF_ERR_ARG, "phf_eval_objective: NULL output pointer");
+  if (!ctx->has_grid || ctx->g_nci != 1 || cmplxconj == 2)
+    return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_eval_objective: needs a grid with nci = 1 and CmplxConj 0 or 1");
+  double h[2], sm[2];
+  std::vector<double> obj(2 + 2 * (size_t)(ctx->nso - nocc) * std::max(nocc, 0));
+  int rc = eval_z(ctx, z, cmplxconj, nocc, precision, h, sm, nullptr, nullptr, true, obj.data());
+  if (rc) return rc;
+  if (!(std::fabs(sm[0]) > 1e-6))   // EigenSolver keeps no state (PHFTools.py:8-18, Tol 1e-6)
+    return fail(ctx, PHF_ERR_SINGULAR, "phf_eval_objective: |Re Smat| <= 1e-6, no projected state");
+  e0[0] = obj[0];
+  std::memcpy(grad, obj.data() + 2, sizeof(double) * (obj.size() - 2));
   return PHF_OK;
 }
 

@@ -93,6 +93,12 @@ struct phf_ctx_s {
   double* dGr2 = nullptr;      // R2[iy]                  [ny] row-major (as given)
   double* dGrpgref = nullptr;  // Rpg[ipg] Ref            [npg][nso x nso] column-major
   double* dGw = nullptr;       // weight x Wigner x weightpg  [G][nci][nci]
+  // spin-rotation grids: every R1[il] and R2[iy] is u (x) I_n for a 2 x 2 u
+  // (AO order [alpha | beta]); then R1 R2 = U_g (x) I_n and dGu holds U_g
+  // ([G][2][2] complex), dGref is Ref (row-major as given)
+  bool g_spinrot = false;
+  double* dGu = nullptr;
+  double* dGref = nullptr;
   // results of the last evaluation kept on the device (phf_eval_resident / phf_localg_vo)
   double* r_ysum = nullptr;   // [nci*nci] x (NSO x N) column-major, sum_g Wd Y_g
   double* r_xsum = nullptr;   // [nci*nci] x (NSO x N), sum_g Wd X_g

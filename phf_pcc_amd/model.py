@@ -106,6 +106,14 @@ class ProjectedHF:
             return float(ev[0].real), self.localg(gm, rm, s, ev[0], fsp, fpg, fk)[self.nocc:, :self.nocc]
         lib = _lib.load()
         nci = self.nci
+        if nci == 1:
+            # one call: grid sums, the 1 x 1 EigenSolver and LocalG on the device
+            e0 = np.zeros(1)
+            g = np.zeros((self.nso - self.nocc, self.nocc), complex)
+            _lib.check(lib.phf_eval_objective(self.ctx.handle, _lib.ptr(z), self.cmplxconj, self.nocc,
+                                              _lib.PREC[self.precision], _lib.ptr(e0), _lib.ptr(g)), self.ctx.handle)
+            self.nevals += 1
+            return float(e0[0]), g
         h = np.zeros((nci, nci), complex)
         s = np.zeros((nci, nci), complex)
         _lib.check(lib.phf_eval_resident(self.ctx.handle, _lib.ptr(z), self.cmplxconj, self.nocc,

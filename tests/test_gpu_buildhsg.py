@@ -249,3 +249,29 @@ def test_cmplxconj2_model_and_mixed(phftools):
     e16, g16 = m.energy_and_gradient(case["z"])
     assert abs(e16 - e) <= 1e-6
     assert np.abs(g16 - g).max() <= 1e-5
+
+
+def _last_grid_rotation(phftools):
+    lines = [ln for ln in phftools.context().tuning_report().splitlines() if '"grid"' in ln]
+    return lines[-1] if lines else ""
+
+
+def test_general_rotations_match_spin_rotation_path(phftools):
+    """Spin grids (every R1, R2 exactly u (x) I_n) take the four-product
+    rotation path; conjugating R1, R2, Rpg and Ref by a random unitary W of
+    the spin-orbital space leaves Ref^H R Ref, hence every output, unchanged
+    but breaks that structure, so the same problem runs the general chain.
+    Both equal the oracle to 1e-10."""
+    case = hubbard_general_ref(4, 4.0, 33, zscale=0.2)
+    ga = grid_args(4, (14, 8), 0)
+    gpu_su2, cpu = _run_both(phftools, case, ga)
+    assert '"rotation": "su2 x identity"' in _last_grid_rotation(phftools)
+    w = random_unitary(8, np.random.default_rng(34))
+    conj = lambda r: np.einsum('pq,gqr,sr->gps', w, r, w.conj())
+    ga2 = dict(ga, r1=conj(ga["r1"]), r2=conj(ga["r2"]), rpg=conj(ga["rpg"]))
+    case2 = dict(case, ref=w @ case["ref"])
+    gpu_gen, cpu2 = _run_both(phftools, case2, ga2)
+    assert '"rotation": "general"' in _last_grid_rotation(phftools)
+    _assert_close(gpu_su2, cpu)
+    _assert_close(gpu_gen, cpu2)
+    _assert_close(gpu_gen, gpu_su2)

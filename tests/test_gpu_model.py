@@ -180,3 +180,48 @@ def test_structure_detection():
     dense = ProjectedHF(p.nocc, p.ref, ga8, factors=(p.h_spatial, p.b), structure=False)
     assert m3.ctx.blocked() and not dense.ctx.blocked()
     assert abs(m3.energy_and_gradient(p.z)[0] - dense.energy_and_gradient(p.z)[0]) < 1e-10
+
+
+def test_device_objective_equals_host_eigensolver_path(prob, model):
+    """phf_eval_objective (nci = 1: E0 = Re H / |Re S| and LocalG on the
+    device, one copy back) against phf_eval_resident + the host EigenSolver /
+    VecDecomp (PHFTools.py:8-26) + phf_localg_vo on the same Z; a grid with
+    nci > 1 is refused."""
+    from phf_pcc_amd import _lib
+    from phf_pcc_amd.phf import eigen_solver, vec_decomp
+    p, ga = prob
+    lib = _lib.load()
+    h = model.ctx.handle
+    e0 = np.zeros(1)
+    g = np.zeros((p.nso - p.nocc, p.nocc), complex)
+    z = np.ascontiguousarray(p.z, dtype=np.complex128)
+    _lib.check(lib.phf_eval_objective(h, _lib.ptr(z), 1, p.nocc, 0, _lib.ptr(e0), _lib.ptr(g)), h)
+    hm = np.zeros((1, 1), complex)
+    sm = np.zeros((1, 1), complex)
+    _lib.check(lib.phf_eval_resident(h, _lib.ptr(z), 1, p.nocc, 0, _lib.ptr(hm), _lib.ptr(sm)), h)
+    ev, vec = eigen_solver(hm, sm)
+    fsp, fpg, fk = vec_decomp(vec[:, 0], 1, 1, 1)
+    c = lambda a: np.ascontiguousarray(a, dtype=np.complex128).reshape(-1)
+    fsp, fpg, fk = c(fsp), c(fpg), c(fk)
+    e0a = np.array([complex(ev[0])])
+    g2 = np.zeros_like(g)
+    _lib.check(lib.phf_localg_vo(h, _lib.ptr(sm), _lib.ptr(e0a), _lib.ptr(fsp), 1, _lib.ptr(fpg), 1, _lib.ptr(fk), 1,
+                                 1, _lib.ptr(g2)), h)
+    assert abs(e0[0] - ev[0].real) <= 1e-12 * abs(ev[0].real)
+    assert np.abs(g - g2).max() <= 1e-12 * np.abs(g2).max()
+
+
+def test_device_objective_refuses_nci_above_one():
+    from phf_pcc_amd import _lib
+    from phf_pcc_amd.model import ProjectedHF
+    from tests._cases import hubbard_general_ref
+    case = hubbard_general_ref(4, 4.0, 9, zscale=0.2)
+    ga = grid_args(4, (26, 8), 1)
+    m = ProjectedHF(case["nocc"], case["ref"], ga, hone=case["hone"], htwo=case["htwo"])
+    z = np.ascontiguousarray(case["z"], dtype=np.complex128)
+    e0 = np.zeros(1)
+    g = np.zeros_like(z)
+    rc = _lib.load().phf_eval_objective(m.ctx.handle, _lib.ptr(z), 1, case["nocc"], 0, _lib.ptr(e0), _lib.ptr(g))
+    assert rc == _lib.PHF_ERR_UNSUPPORTED
+    e, gg = m.energy_and_gradient(case["z"])     # the model takes the host EigenSolver path there
+    assert np.isfinite(e) and np.isfinite(gg).all()
