@@ -508,7 +508,7 @@ extern "C" int phf_ctx_destroy(phf_ctx* ctx) {
   if (ctx->d_xitems) (void)hipFree(ctx->d_xitems);
   if (ctx->ws) (void)hipFree(ctx->ws);
   if (ctx->small) (void)hipFree(ctx->small);
-  for (double* q : {ctx->dGr1p, ctx->dGr2, ctx->dGrpgref, ctx->dGw, ctx->dGu, ctx->dGref})
+  for (double* q : {ctx->dGr1p, ctx->dGr2, ctx->dGrpgref, ctx->dGw, ctx->dGu, ctx->dGref, ctx->dGP})
     if (q) (void)hipFree(q);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
@@ -950,7 +950,7 @@ extern "C" int phf_get_hone(phf_ctx* ctx, double* hone) {
 
 static int free_grid(phf_ctx* ctx) {
   HIPCK(hipStreamSynchronize(ctx->stream));
-  for (double** q : {&ctx->dGr1p, &ctx->dGr2, &ctx->dGrpgref, &ctx->dGw, &ctx->dGu, &ctx->dGref})
+  for (double** q : {&ctx->dGr1p, &ctx->dGr2, &ctx->dGrpgref, &ctx->dGw, &ctx->dGu, &ctx->dGref, &ctx->dGP})
     if (*q) {
       HIPCK(hipFree(*q));
       *q = nullptr;
@@ -1083,6 +1083,16 @@ extern "C" int phf_set_grid(phf_ctx* ctx, const double* ref, const int* ngrid, i
       HIPCK(hipMalloc(&ctx->dGref, sizeof(double) * 2 * n2));
       HIPCK(hipMemcpy(ctx->dGu, ug.data(), sizeof(double) * ug.size(), hipMemcpyHostToDevice));
       HIPCK(hipMemcpy(ctx->dGref, ref, sizeof(double) * 2 * n2, hipMemcpyHostToDevice));
+      // P[s][t][ipg] = Ref_s^H (Rpg[ipg] Ref)_t: each evaluation then needs
+      // only Q = P L (one batched product) before the combine
+      HIPCK(hipMalloc(&ctx->dGP, sizeof(double) * 2 * n2 * 4 * npg));
+      const int n = nso / 2;
+      for (int st = 0; st < 4; ++st)
+        zgemm_batched(s, nso, nso, n, 1, 0, ZMat{ctx->dGref + 2L * (st >> 1) * n * nso, 2, 2L * nso, 0, 1, 1},
+                      ZMat{ctx->dGrpgref + 2L * (st & 1) * n, 2, 2L * nso, 2 * n2, 1, 0}, 0, 0,
+                      zout_colmajor(ctx->dGP + 2 * n2 * npg * st, nso, 2 * n2), npg);
+      HIPCK(hipGetLastError());
+      HIPCK(hipStreamSynchronize(s));
     }
     ctx->g_spinrot = ok;
     ctx->tune_log += std::string("{\"grid\": {\"points\": ") + std::to_string(G) + ", \"rotation\": \"" +
@@ -1336,32 +1346,19 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
     double* Wpg = b.Wpg + lc * sT * npg;
     double* W2 = b.W2 + lc * sT * npg * ny;
     double* T = b.T + lc * sT * G;
-    // Wpg[ipg] = (Rpg[ipg] Ref) L
-    zgemm_batched(s, nso, N, nso, 1, 0, zmat_colmajor(ctx->dGrpgref, nso, 2 * n2), zmat_colmajor(b.C0, nso, 0, lc),
-                  0, 0, zout_colmajor(Wpg, nso, sT), npg);
     if (ctx->g_spinrot) {
-      // Q[s][t][ipg] = Ref_s^H (Wpg[ipg] rows of spin t), then T_g = sum U_g[s][t] Q[s][t][ipg]
-      const int n = nso / 2;
+      // Q[s][t][ipg] = Ref_s^H (Rpg[ipg] Ref L)_t = P[s][t][ipg] L in one batched
+      // product, then T_g = sum U_g[s][t] Q[s][t][ipg]
       double* Q = W2;
-      if (npg == 1) {   // one launch: batch entry st = 2 s + t
-        ZMat a{ctx->dGref, 2, 2L * nso, 2L * n * nso, 1, 1};
-        a.bdiv = 2;     // Ref_s^H: s = st / 2
-        ZMat w{Wpg, 2, 2L * nso, 0, 1, 0};
-        w.bs2 = 2L * n;
-        w.bdiv = 2;     // rows of spin t = st % 2
-        zgemm_batched(s, nso, N, n, 1, 0, a, w, 0, 0, zout_colmajor(Q, nso, sT), 4);
-      } else {
-        for (int st = 0; st < 4; ++st) {
-          const int sa = st >> 1, tb = st & 1;
-          zgemm_batched(s, nso, N, n, 1, 0, ZMat{ctx->dGref + 2L * sa * n * nso, 2, 2L * nso, 0, 1, 1},
-                        ZMat{Wpg + 2L * tb * n, 2, 2L * nso, sT, 1, 0}, 0, 0,
-                        zout_colmajor(Q + st * sT * npg, nso, sT), npg);
-        }
-      }
+      zgemm_batched(s, nso, N, nso, 1, 0, zmat_colmajor(ctx->dGP, nso, 2 * n2), zmat_colmajor(b.C0, nso, 0, lc), 0,
+                    0, zout_colmajor(Q, nso, sT), 4 * npg);
       spinrot_kernel<<<(unsigned)std::min<long>((G * lenX + 255) / 256, 65535), 256, 0, s>>>(G, npg, lenX, ctx->dGu,
                                                                                             Q, T);
       continue;
     }
+    // Wpg[ipg] = (Rpg[ipg] Ref) L
+    zgemm_batched(s, nso, N, nso, 1, 0, zmat_colmajor(ctx->dGrpgref, nso, 2 * n2), zmat_colmajor(b.C0, nso, 0, lc),
+                  0, 0, zout_colmajor(Wpg, nso, sT), npg);
     // W2[iy] = R2[iy] [Wpg_0 .. Wpg_{npg-1}]
     zgemm_batched(s, nso, N * npg, nso, 1, 0, ZMat{ctx->dGr2, 2L * nso, 2, 2 * n2, 1, 0}, zmat_colmajor(Wpg, nso, 0), 0,
                   0, zout_colmajor(W2, nso, sT * npg), ny);

@@ -99,6 +99,7 @@ struct phf_ctx_s {
   bool g_spinrot = false;
   double* dGu = nullptr;
   double* dGref = nullptr;
+  double* dGP = nullptr;       // [s][t][ipg]: Ref_s^H (Rpg[ipg] Ref)_t  (NSO x NSO, column-major)
   // results of the last evaluation kept on the device (phf_eval_resident / phf_localg_vo)
   double* r_ysum = nullptr;   // [nci*nci] x (NSO x N) column-major, sum_g Wd Y_g
   double* r_xsum = nullptr;   // [nci*nci] x (NSO x N), sum_g Wd X_g
