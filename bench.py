@@ -433,7 +433,7 @@ def run_buildsc(args):
                       "stage_ms": {"all": st[0], "stage1_gemm": st[1], "stage2_gemm": st[2], "p_gemm": st[3],
                                    "rank_gemm": st[4], "point_kernel": st[5], "rest": st[6]}},
            "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
-                        "frac": achieved / PEAK["fp64"], "traffic": load_traffic("zgemm_kernel<2, 1>", "sc16"),
+                        "frac": achieved / PEAK["fp64"], "traffic": load_traffic("zgemm_kernel<2, 1>", "so16-buildsc"),
                         "kernel": "zgemm_kernel<2, 1> (stage-2 pair transform with the grid sum)",
                         "flops_per_launch": flops2, "launches_per_step": 1,
                         "all_gemms": {"launches_per_step": nlaunch, "flops_per_step": flops,
@@ -495,7 +495,7 @@ def load_traffic(kernel, n):
         data = json.load(open(path))
     except ValueError:
         return None
-    rec = data.get(f"{kernel}@{n}" if isinstance(n, str) else f"{kernel}@n{n}")
+    rec = data.get(f"{kernel}@n{n}")
     return None if rec is None else {"bytes_per_launch": rec["bytes_per_launch"], "source": rec["source"]}
 
 
@@ -689,7 +689,8 @@ def main():
                       "grid_points": G, "speedup_vs_plain_fp64": value / base["value"],
                       "speedup_vs_fp64_same_blocks": value / res["fp64"]["value"], "abs_dE_vs_fp64": best["dE"],
                       "plain_fp64": plain_out, "forms": forms_out,
-                      "path": "device-resident objective (phf_eval_resident + phf_localg_vo), factored Hamiltonian "
+                      "path": "device-resident objective (phf_eval_objective: grid sums, the 1 x 1 EigenSolver and "
+                              "LocalG on the device), factored Hamiltonian "
                               "built on the device",
                       "hbm_stages_fp64": stage_out,
                       "drop_in_phftools": dropin or None},
