@@ -246,6 +246,16 @@ int phf_sc_apply(phf_ctx* ctx, int nvec, const double* c0, const double* c1, con
 int phf_sc_apply_packed(phf_ctx* ctx, int nvec, const double* vin, double* vout);
 int phf_sc_apply_packed_dev(phf_ctx* ctx, int nvec, const double* vin, double* vout);
 long phf_sd_dim(int nso, int nocc);
+
+/* Batched InvertC + determinantC (Wrappers.f90:295-335, 195-231) on the
+ * kernel that inverts the grid's overlaps M_g: m is complex[batch][n][n]
+ * (element (b,i,j) = Matrix_b(i,j)), n <= 96.  Partial pivoting on the
+ * largest |Re|+|Im| of the column, first index on ties (IZAMAX);
+ * det[b] = product of the pivots with a sign flip per row interchange;
+ * info[b] = the first exactly-zero pivot (1-based), 0 otherwise, in which
+ * case minv[b] and det[b] are zero.  A singular matrix is not an error here
+ * (the grid stages turn info != 0 into PHF_ERR_SINGULAR). */
+int phf_invert_batched(phf_ctx* ctx, int n, int batch, const double* m, double* minv, double* det, int* info);
 /* buildsc_dr.buildsc_vec (BuildSC_Vec, BuildSC_dr.f90:8-1244) and any other
  * weighted sum over given points: replaces the grid of phf_sc_set_grid with
  * npts points of rotation r (complex[npts][nso][nso], NULL for identities),

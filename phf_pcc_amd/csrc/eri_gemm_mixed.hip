@@ -692,6 +692,45 @@ __global__ __launch_bounds__(256) void split_rows_kernel(const double* __restric
   // consecutive lanes take consecutive columns: the (gathered) reads and the
   // slice stores stay coalesced
   T* d = dst + row * (long)NS * kout;
+  const bool vec4 = (kout & 3) == 0 && (reinterpret_cast<uintptr_t>(dst) % (4 * sizeof(T))) == 0 &&
+                    (reinterpret_cast<uintptr_t>(cmap) % 16) == 0;
+  if (vec4) {
+    // four columns per lane and step: one 16-byte map load, four independent
+    // gathers, one 4-element store per slice (kout % 4 == 0 and the aligned
+    // bases keep every vector access aligned)
+    typedef T T4 __attribute__((ext_vector_type(4)));
+    for (int c0 = 4 * threadIdx.x; c0 < kout; c0 += 1024) {
+      int cols[4];
+      if (cmap) {
+        const int4 cm = *reinterpret_cast<const int4*>(cmap + c0);
+        cols[0] = cm.x; cols[1] = cm.y; cols[2] = cm.z; cols[3] = cm.w;
+      } else {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) cols[u] = c0 + u;
+      }
+      double r[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) r[u] = (data && cols[u] >= 0 ? x[cols[u]] : 0.0) * sc;
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        T4 v;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if constexpr (std::is_same<T, signed char>::value) {
+            const double t = fmin(127.0, fmax(-127.0, rint(r[u])));
+            v[u] = (signed char)t;
+            r[u] = (r[u] - t) * 128.0;
+          } else {
+            const T t = to16<T>(r[u]);
+            v[u] = t;
+            r[u] -= (double)(float)t;
+          }
+        }
+        *reinterpret_cast<T4*>(d + (long)q * kout + c0) = v;
+      }
+    }
+    return;
+  }
   for (int c = threadIdx.x; c < kout; c += 256) {
     const int sc_col = cmap ? cmap[c] : c;
     double r = (data && sc_col >= 0 ? x[sc_col] : 0.0) * sc;

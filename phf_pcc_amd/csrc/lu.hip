@@ -4,9 +4,11 @@
 // (row) pivoting on the largest |Re|+|Im| in the column (IZAMAX's measure,
 // first index on ties), det = prod of pivots with a sign flip for every
 // PIV(k) != k, info = first exactly-zero pivot.  One workgroup per grid
-// point; the matrix lives in LDS and is inverted in place by Gauss-Jordan
-// elimination (its pivots are the U_kk of the LU factorization), then the
-// row interchanges are undone as column interchanges in reverse order.
+// point.  N <= 64 runs one wave per matrix with the rows in registers
+// (lu_inverse_wave_kernel); larger N keeps the matrix in LDS, inverted in
+// place by Gauss-Jordan elimination (its pivots are the U_kk of the LU
+// factorization), then the row interchanges are undone as column
+// interchanges in reverse order.
 #include "phf_common.h"
 
 namespace {
@@ -129,6 +131,99 @@ __global__ __launch_bounds__(256) void lu_inverse_kernel(int n, const double* __
   }
 }
 
+// One wave per matrix for n <= NMAX <= 64: row i of M lives in lane i's
+// registers.  Gauss-Jordan with the pivot column always in register 0: each
+// step writes every row's update shifted one place to the left and appends
+// the new inverse column in register NMAX-1, so every register index is a
+// compile-time constant; after n steps the inverse's column m sits in
+// register NMAX-n+m.  Rows never move: each lane tracks the position its row
+// holds in the row-interchanged matrix, so the pivot choice (largest
+// |Re|+|Im| among the rows at positions k..n-1, first position on ties), the
+// determinant's sign flips and info are those of the algorithm above.  The
+// pivot row reaches the other lanes through LDS.  With B = P M (P the row
+// interchanges), lane l holds row pos_l of B^{-1}, and M^{-1} = B^{-1} P, so
+// lane l's column m lands in column sigma(m) of M^{-1}, sigma(m) = the lane
+// whose row ended at position m.
+template <int NMAX>
+__global__ __launch_bounds__(64) void lu_inverse_wave_kernel(int n, const double* __restrict__ M, long mbs,
+                                                             double* __restrict__ Minv, long ibs,
+                                                             double* __restrict__ det, int* __restrict__ info) {
+  __shared__ double2 s_row[NMAX];
+  __shared__ int s_sig[64];
+  const long g = blockIdx.x;
+  const int lane = threadIdx.x;
+  const double* Mg = M + g * mbs;
+  const bool real_row = lane < n;
+  double2 a[NMAX];
+#pragma unroll
+  for (int j = 0; j < NMAX; ++j) {
+    a[j] = make_double2(0.0, 0.0);
+    if (real_row && j < n) {
+      const long off = 2 * ((long)j * n + lane);   // global is column-major
+      a[j] = make_double2(Mg[off], Mg[off + 1]);
+    }
+  }
+  int pos = lane;
+  bool done = !real_row;   // row already used as a pivot, or no row of M
+  double2 detv = make_double2(1.0, 0.0);
+  int sinfo = 0;
+  for (int k = 0; k < n; ++k) {
+    double c = done ? -1.0 : fabs(a[0].x) + fabs(a[0].y);
+    if (c != c) c = -1.0;   // a NaN row is never a pivot; keeps the reduction wave-uniform
+    int key = (pos << 6) | lane;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double oc = __shfl_xor(c, off);
+      const int ok = __shfl_xor(key, off);
+      if (oc > c || (oc == c && ok < key)) { c = oc; key = ok; }
+    }
+    if (!(c > 0.0)) { sinfo = k + 1; break; }
+    const int pl = key & 63, pp = key >> 6;
+    const double2 pv = make_double2(__shfl(a[0].x, pl), __shfl(a[0].y, pl));
+    detv = cmul(detv, pv);
+    if (pp != k) detv = make_double2(-detv.x, -detv.y);
+    const double2 pinv = crecip(pv);
+    const bool is_p = lane == pl;
+    if (is_p) {
+#pragma unroll
+      for (int j = 1; j < NMAX; ++j) s_row[j - 1] = a[j];
+    }
+    if (pos == k) pos = pp;
+    if (is_p) { pos = k; done = true; }
+    __syncthreads();
+    // pivot row: a <- pinv * row; other rows: a <- a - (f pinv) * row
+    const double2 gm = is_p ? make_double2(-pinv.x, -pinv.y) : cmul(a[0], pinv);
+    const double s0 = is_p ? 0.0 : 1.0;
+#pragma unroll
+    for (int j = 0; j < NMAX - 1; ++j) {
+      const double2 r = s_row[j];
+      a[j] = make_double2(fma(-gm.x, r.x, fma(gm.y, r.y, s0 * a[j + 1].x)),
+                          fma(-gm.x, r.y, fma(-gm.y, r.x, s0 * a[j + 1].y)));
+    }
+    a[NMAX - 1] = is_p ? pinv : make_double2(-gm.x, -gm.y);
+    __syncthreads();
+  }
+  s_sig[pos] = lane;
+  __syncthreads();
+  if (real_row) {
+    double* Ig = Minv + g * ibs;
+#pragma unroll
+    for (int q = 0; q < NMAX; ++q) {
+      const int m = q + n - NMAX;
+      if (m >= 0) {
+        const long off = 2 * ((long)s_sig[m] * n + pos);
+        Ig[off] = sinfo ? 0.0 : a[q].x;
+        Ig[off + 1] = sinfo ? 0.0 : a[q].y;
+      }
+    }
+  }
+  if (lane == 0) {
+    det[2 * g] = sinfo ? 0.0 : detv.x;
+    det[2 * g + 1] = sinfo ? 0.0 : detv.y;
+    info[g] = sinfo;
+  }
+}
+
 }  // namespace
 
 int lu_inverse_max_n() { return 96; }
@@ -136,10 +231,12 @@ int lu_inverse_max_n() { return 96; }
 // M: column-major N x N complex matrices with batch stride mbs (doubles).
 void lu_inverse_batched(hipStream_t s, int n, const double* M, long mbs, double* Minv, long ibs,
                         double* det, int* info, int batch) {
-  if (n <= 32)
-    lu_inverse_kernel<32><<<batch, 256, 0, s>>>(n, M, mbs, Minv, ibs, det, info);
+  if (n <= 16)
+    lu_inverse_wave_kernel<16><<<batch, 64, 0, s>>>(n, M, mbs, Minv, ibs, det, info);
+  else if (n <= 32)
+    lu_inverse_wave_kernel<32><<<batch, 64, 0, s>>>(n, M, mbs, Minv, ibs, det, info);
   else if (n <= 64)
-    lu_inverse_kernel<64><<<batch, 256, 0, s>>>(n, M, mbs, Minv, ibs, det, info);
+    lu_inverse_wave_kernel<64><<<batch, 64, 0, s>>>(n, M, mbs, Minv, ibs, det, info);
   else
     lu_inverse_kernel<96><<<batch, 256, 0, s>>>(n, M, mbs, Minv, ibs, det, info);
 }

@@ -939,6 +939,58 @@ extern "C" int phf_set_hamiltonian_factored(phf_ctx* ctx, const double* h, const
   return PHF_OK;
 }
 
+// Batched InvertC + determinantC (Wrappers.f90:195-231, 295-335) on the
+// kernel the grid stages use for the overlaps M_g: host arrays in numpy order,
+// complex[batch][n][n] (element (b,i,j) = Matrix_b(i,j)).
+extern "C" int phf_invert_batched(phf_ctx* ctx, int n, int batch, const double* m, double* minv, double* det,
+                                  int* info) {
+  if (!ctx) return PHF_ERR_ARG;
+  if (n <= 0 || batch <= 0 || !m || !minv || !det || !info)
+    return fail(ctx, PHF_ERR_ARG, "phf_invert_batched: bad arguments");
+  if (n > lu_inverse_max_n())
+    return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_invert_batched: n = " + std::to_string(n) + " exceeds " +
+                                              std::to_string(lu_inverse_max_n()));
+  HIPCK(hipSetDevice(ctx->device));
+  const size_t nn = (size_t)n * n, tot = nn * batch;
+  std::vector<double> h(2 * tot);
+  for (size_t b = 0; b < (size_t)batch; ++b)   // to column-major, as the kernel reads it
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        h[2 * (b * nn + (size_t)j * n + i)] = m[2 * (b * nn + (size_t)i * n + j)];
+        h[2 * (b * nn + (size_t)j * n + i) + 1] = m[2 * (b * nn + (size_t)i * n + j) + 1];
+      }
+  double *dM = nullptr, *dI = nullptr, *dD = nullptr;
+  int* dInfo = nullptr;
+  int rc = PHF_OK;
+  auto run = [&]() -> int {
+    HIPCK(hipMalloc(&dM, sizeof(double) * 2 * tot));
+    HIPCK(hipMalloc(&dI, sizeof(double) * 2 * tot));
+    HIPCK(hipMalloc(&dD, sizeof(double) * 2 * batch));
+    HIPCK(hipMalloc(&dInfo, sizeof(int) * batch));
+    HIPCK(hipMemcpyAsync(dM, h.data(), sizeof(double) * 2 * tot, hipMemcpyHostToDevice, ctx->stream));
+    lu_inverse_batched(ctx->stream, n, dM, 2L * (long)nn, dI, 2L * (long)nn, dD, dInfo, batch);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(h.data(), dI, sizeof(double) * 2 * tot, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipMemcpyAsync(det, dD, sizeof(double) * 2 * batch, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipMemcpyAsync(info, dInfo, sizeof(int) * batch, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    return PHF_OK;
+  };
+  rc = run();
+  hipFree(dM);
+  hipFree(dI);
+  hipFree(dD);
+  hipFree(dInfo);
+  if (rc) return rc;
+  for (size_t b = 0; b < (size_t)batch; ++b)
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        minv[2 * (b * nn + (size_t)i * n + j)] = h[2 * (b * nn + (size_t)j * n + i)];
+        minv[2 * (b * nn + (size_t)i * n + j) + 1] = h[2 * (b * nn + (size_t)j * n + i) + 1];
+      }
+  return PHF_OK;
+}
+
 extern "C" int phf_get_hone(phf_ctx* ctx, double* hone) {
   if (!ctx || !hone) return PHF_ERR_ARG;
   if (!ctx->has_ham) return fail(ctx, PHF_ERR_NO_HAMILTONIAN, "phf_get_hone: no Hamiltonian set");

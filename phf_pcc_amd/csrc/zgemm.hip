@@ -42,42 +42,61 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double 
       cim[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
     }
 
-  for (int k0 = 0; k0 < k; k0 += ZBK) {
+  // Global loads go to registers first: every load of a slab is issued
+  // before any is used (addresses clamped to element (0,0) and the value
+  // zeroed when out of range, so no branch separates them), and slab s+1 is
+  // loaded while the MFMAs of slab s run.
+  constexpr int EA = (ZBM * ZBK) / 256, EB = (ZBN * ZBK) / 256;
+  double lar[EA], lai[EA], lbr[EB], lbi[EB];
+  bool va[EA], vb[EB];
+  auto gload = [&](int k0) {
 #pragma unroll
-    for (int e = 0; e < (ZBM * ZBK) / 256; ++e) {
+    for (int e = 0; e < EA; ++e) {
       const int idx = tid + 256 * e;
       int i, kk;
       if (a_kfast) { kk = idx % ZBK; i = idx / ZBK; }
       else { i = idx % ZBM; kk = idx / ZBM; }
       const int gi = m0 + i, gk = k0 + kk;
-      double re = 0.0, im = 0.0;
-      if (gi < m && gk < k) {
-        const long off = (long)gi * A.rs + (long)gk * A.cs;
-        re = Ap[off];
-        im = Ap[off + A.im];
-        if (A.conj) im = -im;
-      }
-      As_re[i * ZLD + kk] = re;
-      As_im[i * ZLD + kk] = im;
+      va[e] = gi < m && gk < k;
+      const long off = va[e] ? (long)gi * A.rs + (long)gk * A.cs : 0;
+      lar[e] = Ap[off];
+      lai[e] = Ap[off + A.im];
     }
 #pragma unroll
-    for (int e = 0; e < (ZBN * ZBK) / 256; ++e) {
+    for (int e = 0; e < EB; ++e) {
       const int idx = tid + 256 * e;
       int j, kk;
       if (b_kfast) { kk = idx % ZBK; j = idx / ZBK; }
       else { j = idx % ZBN; kk = idx / ZBN; }
       const int gj = n0 + j, gk = k0 + kk;
-      double re = 0.0, im = 0.0;
-      if (gj < n && gk < k) {
-        const long off = (long)gk * B.rs + (long)gj * B.cs;
-        re = Bp[off];
-        im = Bp[off + B.im];
-        if (B.conj) im = -im;
-      }
-      Bs_re[j * ZLD + kk] = re;
-      Bs_im[j * ZLD + kk] = im;
+      vb[e] = gj < n && gk < k;
+      const long off = vb[e] ? (long)gk * B.rs + (long)gj * B.cs : 0;
+      lbr[e] = Bp[off];
+      lbi[e] = Bp[off + B.im];
+    }
+  };
+  if (k > 0) gload(0);
+  for (int k0 = 0; k0 < k; k0 += ZBK) {
+#pragma unroll
+    for (int e = 0; e < EA; ++e) {
+      const int idx = tid + 256 * e;
+      int i, kk;
+      if (a_kfast) { kk = idx % ZBK; i = idx / ZBK; }
+      else { i = idx % ZBM; kk = idx / ZBM; }
+      As_re[i * ZLD + kk] = va[e] ? lar[e] : 0.0;
+      As_im[i * ZLD + kk] = va[e] ? (A.conj ? -lai[e] : lai[e]) : 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < EB; ++e) {
+      const int idx = tid + 256 * e;
+      int j, kk;
+      if (b_kfast) { kk = idx % ZBK; j = idx / ZBK; }
+      else { j = idx % ZBN; kk = idx / ZBN; }
+      Bs_re[j * ZLD + kk] = vb[e] ? lbr[e] : 0.0;
+      Bs_im[j * ZLD + kk] = vb[e] ? (B.conj ? -lbi[e] : lbi[e]) : 0.0;
     }
     __syncthreads();
+    if (k0 + ZBK < k) gload(k0 + ZBK);
 #pragma unroll
     for (int ks = 0; ks < ZBK; ks += 4) {
       double are[FMW], aim[FMW], bre[FNW], bim[FNW];

@@ -49,6 +49,8 @@ def test_sc_entry_points_reject_null_context():
     assert lib.phf_sd_dim(8, 3) == 1 + 15 + 3 * 10
     assert lib.phf_sc_set_points(None, 4, 1, None, p, None) == _lib.PHF_ERR_ARG
     assert lib.phf_eval_objective(None, p, 1, 2, 0, p, p) == _lib.PHF_ERR_ARG
+    info = np.zeros(1, np.int32)
+    assert lib.phf_invert_batched(None, 4, 1, p, p, p, info.ctypes.data_as(_lib._ip)) == _lib.PHF_ERR_ARG
 
 
 def test_buildsc_shim_checks_shapes_before_the_device():
