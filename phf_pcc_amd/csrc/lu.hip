@@ -140,7 +140,10 @@ __global__ __launch_bounds__(256) void lu_inverse_kernel(int n, const double* __
 // holds in the row-interchanged matrix, so the pivot choice (largest
 // |Re|+|Im| among the rows at positions k..n-1, first position on ties), the
 // determinant's sign flips and info are those of the algorithm above.  The
-// pivot row reaches the other lanes through LDS.  With B = P M (P the row
+// pivot row (with the pivot) reaches the other lanes through LDS.  Each
+// step's pivot search (a butterfly over the wave) starts as soon as the new
+// column 0 is updated and overlaps the rest of the row update: with one wave
+// per matrix and fewer matrices than SIMDs, nothing else hides its latency.  With B = P M (P the row
 // interchanges), lane l holds row pos_l of B^{-1}, and M^{-1} = B^{-1} P, so
 // lane l's column m lands in column sigma(m) of M^{-1}, sigma(m) = the lane
 // whose row ended at position m.
@@ -167,36 +170,52 @@ __global__ __launch_bounds__(64) void lu_inverse_wave_kernel(int n, const double
   bool done = !real_row;   // row already used as a pivot, or no row of M
   double2 detv = make_double2(1.0, 0.0);
   int sinfo = 0;
+  // pivot search of step 0; each later search runs on the freshly updated
+  // column while the rest of the row update is still in flight
+  double c = done ? -1.0 : fabs(a[0].x) + fabs(a[0].y);
+  if (c != c) c = -1.0;   // a NaN row is never a pivot; keeps the reduction wave-uniform
+  int key = (pos << 6) | lane;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double oc = __shfl_xor(c, off);
+    const int ok = __shfl_xor(key, off);
+    if (oc > c || (oc == c && ok < key)) { c = oc; key = ok; }
+  }
   for (int k = 0; k < n; ++k) {
-    double c = done ? -1.0 : fabs(a[0].x) + fabs(a[0].y);
-    if (c != c) c = -1.0;   // a NaN row is never a pivot; keeps the reduction wave-uniform
-    int key = (pos << 6) | lane;
+    if (!(c > 0.0)) { sinfo = k + 1; break; }
+    const int pl = key & 63, pp = key >> 6;
+    const bool is_p = lane == pl;
+    if (is_p) {
+#pragma unroll
+      for (int j = 0; j < NMAX; ++j) s_row[j] = a[j];
+    }
+    if (pos == k) pos = pp;
+    if (is_p) { pos = k; done = true; }
+    __syncthreads();
+    const double2 pv = s_row[0];
+    detv = cmul(detv, pv);
+    if (pp != k) detv = make_double2(-detv.x, -detv.y);
+    const double2 pinv = crecip(pv);
+    // pivot row: a <- pinv * row; other rows: a <- a - (f pinv) * row
+    const double2 gm = is_p ? make_double2(-pinv.x, -pinv.y) : cmul(a[0], pinv);
+    const double s0 = is_p ? 0.0 : 1.0;
+    {
+      const double2 r = s_row[1];
+      a[0] = make_double2(fma(-gm.x, r.x, fma(gm.y, r.y, s0 * a[1].x)),
+                          fma(-gm.x, r.y, fma(-gm.y, r.x, s0 * a[1].y)));
+    }
+    c = done ? -1.0 : fabs(a[0].x) + fabs(a[0].y);
+    if (c != c) c = -1.0;
+    key = (pos << 6) | lane;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
       const double oc = __shfl_xor(c, off);
       const int ok = __shfl_xor(key, off);
       if (oc > c || (oc == c && ok < key)) { c = oc; key = ok; }
     }
-    if (!(c > 0.0)) { sinfo = k + 1; break; }
-    const int pl = key & 63, pp = key >> 6;
-    const double2 pv = make_double2(__shfl(a[0].x, pl), __shfl(a[0].y, pl));
-    detv = cmul(detv, pv);
-    if (pp != k) detv = make_double2(-detv.x, -detv.y);
-    const double2 pinv = crecip(pv);
-    const bool is_p = lane == pl;
-    if (is_p) {
 #pragma unroll
-      for (int j = 1; j < NMAX; ++j) s_row[j - 1] = a[j];
-    }
-    if (pos == k) pos = pp;
-    if (is_p) { pos = k; done = true; }
-    __syncthreads();
-    // pivot row: a <- pinv * row; other rows: a <- a - (f pinv) * row
-    const double2 gm = is_p ? make_double2(-pinv.x, -pinv.y) : cmul(a[0], pinv);
-    const double s0 = is_p ? 0.0 : 1.0;
-#pragma unroll
-    for (int j = 0; j < NMAX - 1; ++j) {
-      const double2 r = s_row[j];
+    for (int j = 1; j < NMAX - 1; ++j) {
+      const double2 r = s_row[j + 1];
       a[j] = make_double2(fma(-gm.x, r.x, fma(gm.y, r.y, s0 * a[j + 1].x)),
                           fma(-gm.x, r.y, fma(-gm.y, r.x, s0 * a[j + 1].y)));
     }

@@ -11,7 +11,10 @@
 // Tile: (32 FMW) x (32 FNW) per 256-thread workgroup, 2 x 2 waves of
 // (16 FMW) x (16 FNW), k-slab 16; the launcher takes FMW/FNW = 1 for sides of
 // at most 32 (the N = nocc sides) so no MFMA work is spent on padding.
-// Complex product = four real MFMAs into separate re/im accumulators.
+// Complex product = three real MFMAs (Gauss): P1 = sum ar br, P2 = sum ai bi,
+// P3 = sum (ar + ai)(br + bi), then re = P1 - P2, im = P3 - P1 - P2.  The
+// imaginary part's rounding error is bounded by eps (|ar| + |ai|)(|br| + |bi|)
+// per term instead of eps (|ar bi| + |ai br|): the same order for FP64.
 #include "phf_common.h"
 
 namespace {
@@ -33,13 +36,14 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double 
   const double* Bp = B.p + B.boff(b);
   const long cb = C.boff(b);
 
-  d4_t cre[FMW][FNW], cim[FMW][FNW];
+  d4_t p1[FMW][FNW], p2[FMW][FNW], p3[FMW][FNW];
 #pragma unroll
   for (int i = 0; i < FMW; ++i)
 #pragma unroll
     for (int j = 0; j < FNW; ++j) {
-      cre[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
-      cim[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+      p1[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+      p2[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+      p3[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
     }
 
   // Global loads go to registers first: every load of a slab is issued
@@ -116,10 +120,9 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double 
       for (int i = 0; i < FMW; ++i)
 #pragma unroll
         for (int j = 0; j < FNW; ++j) {
-          cre[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bre[j], cre[i][j], 0, 0, 0);
-          cre[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-aim[i], bim[j], cre[i][j], 0, 0, 0);
-          cim[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bim[j], cim[i][j], 0, 0, 0);
-          cim[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bre[j], cim[i][j], 0, 0, 0);
+          p1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bre[j], p1[i][j], 0, 0, 0);
+          p2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bim[j], p2[i][j], 0, 0, 0);
+          p3[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i] + aim[i], bre[j] + bim[j], p3[i][j], 0, 0, 0);
         }
     }
     __syncthreads();
@@ -136,7 +139,7 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double 
         const int gi = m0 + wm * 16 * FMW + i * 16 + (lane >> 4) + 4 * r;   // f64 16x16x4: row = lane/16 + 4r
         const int gj = n0 + wn * 16 * FNW + j * 16 + (lane & 15);
         if (gi < m && gj < n) {
-          const double xr = cre[i][j][r], xi = cim[i][j][r];
+          const double xr = p1[i][j][r] - p2[i][j][r], xi = p3[i][j][r] - p1[i][j][r] - p2[i][j][r];
           double vr = ar * xr - ai * xi, vi = ar * xi + ai * xr;
           const long off = cb + (long)gi * C.rs + (long)gj * C.cs;
           if (use_beta) {
