@@ -650,8 +650,21 @@ __global__ __launch_bounds__(256, 1) void eri_gemm_f32_kernel(const float* __res
 // ---------------------------------------------------------------------------
 
 template <typename T> __device__ inline T to16(double x);
-template <> __device__ inline _Float16 to16<_Float16>(double x) { return (_Float16)(float)x; }
-template <> __device__ inline __bf16 to16<__bf16>(double x) { return (__bf16)(float)x; }
+// Two hardware conversions (f64 -> f32 -> 16-bit).  Without the empty asm the
+// compiler folds them into one f64 -> 16-bit conversion, which has no
+// instruction and is expanded into ~40 integer operations.  The double
+// rounding can move the first slice by one unit in a tie; the next slice
+// takes the remainder either way.
+template <> __device__ inline _Float16 to16<_Float16>(double x) {
+  float f = (float)x;
+  asm volatile("" : "+v"(f));
+  return (_Float16)f;
+}
+template <> __device__ inline __bf16 to16<__bf16>(double x) {
+  float f = (float)x;
+  asm volatile("" : "+v"(f));
+  return (__bf16)f;
+}
 template <> __device__ inline signed char to16<signed char>(double x) { return (signed char)x; }
 
 // One workgroup per row: s = 2^(top - ceil(log2 max|x|)); nslice slices x_i of
@@ -750,6 +763,188 @@ __global__ __launch_bounds__(256) void split_rows_kernel(const double* __restric
   }
 }
 
+// Density pack fused with the split of its rows, for NSO <= 128: one
+// 1024-thread workgroup per grid point forms rho^T = conj(C0) X_g^T (NSO x NSO,
+// contraction N; C0 and X_g column-major NSO x N, complex interleaved) on the
+// FP64 matrix cores with the complex GEMM's exact MFMA sequence (three real
+// products per complex one, 16-deep k-chunks, 4-deep steps), takes the largest
+// |Re| and |Im| over the point, and writes the NS slices of both planes (rows
+// 2g and 2g + 1) straight into the ERI GEMM's operand, element (j, l) at
+// column ocol[j NSO + l] (j NSO + l without a map).  The slices are those
+// split_rows_kernel makes from the FP64 rows, bit for bit, without writing and
+// reading those rows.  Columns no element maps to are zeroed (padcols, or
+// [NSO^2, kout) without a map); workgroups past the last point clear the
+// padding rows up to rows_total (zero slices, inverse scale 1).
+template <typename T, int NS>
+__global__ __launch_bounds__(512) void dens_split_kernel(int nso, int nk, const double* __restrict__ C0,
+                                                          const double* __restrict__ X, long sX, int npts,
+                                                          const int* __restrict__ ocol,
+                                                          const int* __restrict__ padcols, int npad, int kout,
+                                                          long rows_total, T* __restrict__ dst, int top,
+                                                          double* __restrict__ inv) {
+  constexpr int TB = 128, KC = 16, LD = KC + 2, NT = 512, FM = 4, FN = 2;
+  __shared__ double Ar[TB * LD], Ai[TB * LD], Br[TB * LD], Bi[TB * LD];
+  __shared__ double red[2][NT / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int g = blockIdx.x;
+  const T zero = static_cast<T>(0.0f);
+  if (g >= npts) {
+    for (int pr = 0; pr < 2; ++pr) {
+      const long row = 2L * npts + 2L * (g - npts) + pr;
+      if (row >= rows_total) break;
+      T* d = dst + row * (long)NS * kout;
+      for (long c = tid; c < (long)NS * kout; c += NT) d[c] = zero;
+      if (tid == 0) inv[row] = 1.0;
+    }
+    return;
+  }
+  const int wm = wave >> 2, wn = wave & 3;   // 2 x 4 waves of 64 x 32
+  const double* Xg = X + (long)g * sX;
+  d4_t p1[FM][FN], p2[FM][FN], p3[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      p1[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+      p2[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+      p3[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+    }
+  for (int k0 = 0; k0 < nk; k0 += KC) {
+#pragma unroll
+    for (int e0 = 0; e0 < TB * KC; e0 += NT) {
+      const int e = e0 + tid, i = e % TB, kk = e / TB;
+      const int gk = k0 + kk;
+      const bool v = i < nso && gk < nk;
+      const long off = v ? 2L * (i + (long)gk * nso) : 0;
+      const double ar = C0[off], ai = C0[off + 1], br = Xg[off], bi = Xg[off + 1];
+      Ar[i * LD + kk] = v ? ar : 0.0;
+      Ai[i * LD + kk] = v ? -ai : 0.0;
+      Br[i * LD + kk] = v ? br : 0.0;
+      Bi[i * LD + kk] = v ? bi : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < KC; ks += 4) {
+      double are[FM], aim[FM], bre[FN], bim[FN];
+#pragma unroll
+      for (int t = 0; t < FM; ++t) {
+        const int row = wm * 16 * FM + t * 16 + (lane & 15);
+        are[t] = Ar[row * LD + ks + (lane >> 4)];
+        aim[t] = Ai[row * LD + ks + (lane >> 4)];
+      }
+#pragma unroll
+      for (int t = 0; t < FN; ++t) {
+        const int col = wn * 16 * FN + t * 16 + (lane & 15);
+        bre[t] = Br[col * LD + ks + (lane >> 4)];
+        bim[t] = Bi[col * LD + ks + (lane >> 4)];
+      }
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          p1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bre[j], p1[i][j], 0, 0, 0);
+          p2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bim[j], p2[i][j], 0, 0, 0);
+          p3[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i] + aim[i], bre[j] + bim[j], p3[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+  // complex results and the point's largest magnitudes
+  double mre = 0.0, mim = 0.0;
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double xr = p1[i][j][r] - p2[i][j][r], xi = p3[i][j][r] - p1[i][j][r] - p2[i][j][r];
+        p1[i][j][r] = xr;
+        p2[i][j][r] = xi;
+        const int gi = wm * 16 * FM + i * 16 + (lane >> 4) + 4 * r, gj = wn * 16 * FN + j * 16 + (lane & 15);
+        if (gi < nso && gj < nso) {
+          mre = fmax(mre, fabs(xr));
+          mim = fmax(mim, fabs(xi));
+        }
+      }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    mre = fmax(mre, __shfl_xor(mre, off));
+    mim = fmax(mim, __shfl_xor(mim, off));
+  }
+  if (lane == 0) {
+    red[0][wave] = mre;
+    red[1][wave] = mim;
+  }
+  __syncthreads();
+  double mx[2] = {0.0, 0.0};
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) {
+    mx[0] = fmax(mx[0], red[0][w]);
+    mx[1] = fmax(mx[1], red[1][w]);
+  }
+  double sc[2];
+#pragma unroll
+  for (int pl = 0; pl < 2; ++pl) {
+    int e = 0;
+    if (mx[pl] > 0.0) frexp(mx[pl], &e);
+    sc[pl] = (mx[pl] > 0.0) ? ldexp(1.0, top - e) : 1.0;
+  }
+  const long row0 = 2L * g;
+  if (tid == 0) {
+    inv[row0] = 1.0 / sc[0];
+    inv[row0 + 1] = 1.0 / sc[1];
+  }
+  T* d0 = dst + row0 * (long)NS * kout;
+  T* d1 = d0 + (long)NS * kout;
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gi = wm * 16 * FM + i * 16 + (lane >> 4) + 4 * r, gj = wn * 16 * FN + j * 16 + (lane & 15);
+        if (gi >= nso || gj >= nso) continue;
+        const int src = gi * nso + gj;
+        const int c = ocol ? ocol[src] : src;
+        if (c < 0) continue;
+        double rr = p1[i][j][r] * sc[0], ri = p2[i][j][r] * sc[1];
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+          if constexpr (std::is_same<T, signed char>::value) {
+            const double vr = fmin(127.0, fmax(-127.0, rint(rr))), vi = fmin(127.0, fmax(-127.0, rint(ri)));
+            d0[(long)q * kout + c] = (signed char)vr;
+            d1[(long)q * kout + c] = (signed char)vi;
+            rr = (rr - vr) * 128.0;
+            ri = (ri - vi) * 128.0;
+          } else {
+            const T vr = to16<T>(rr), vi = to16<T>(ri);
+            d0[(long)q * kout + c] = vr;
+            d1[(long)q * kout + c] = vi;
+            rr -= (double)(float)vr;
+            ri -= (double)(float)vi;
+          }
+        }
+      }
+  // output columns no element maps to
+  if (ocol) {
+    for (int p = tid; p < npad; p += NT) {
+      const int c = padcols[p];
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        d0[(long)q * kout + c] = zero;
+        d1[(long)q * kout + c] = zero;
+      }
+    }
+  } else {
+    for (int c = nso * nso + tid; c < kout; c += NT)
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        d0[(long)q * kout + c] = zero;
+        d1[(long)q * kout + c] = zero;
+      }
+  }
+}
+
 __global__ void to_f32_kernel(const double* __restrict__ src, float* __restrict__ dst, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     dst[i] = (float)src[i];
@@ -814,6 +1009,22 @@ void split_operand_map(hipStream_t s, int prec, const double* src, long lds, int
   else
     split_rows_kernel<signed char, 4><<<rows, 256, 0, s>>>(src, lds, kt, cmap, kout, (signed char*)dst, 7, inv,
                                                            amax, rd);
+}
+
+int dens_split_max_nso() { return 128; }
+
+void dens_split(hipStream_t s, int prec, int nso, int nk, const double* C0, const double* X, long sX, int npts,
+                const int* ocol, const int* padcols, int npad, int kout, long rows_total, void* dst, double* inv) {
+  const unsigned grid = (unsigned)(npts + (rows_total - 2L * npts + 1) / 2);
+  if (prec == 3)
+    dens_split_kernel<_Float16, 2><<<grid, 512, 0, s>>>(nso, nk, C0, X, sX, npts, ocol, padcols, npad, kout,
+                                                         rows_total, (_Float16*)dst, 14, inv);
+  else if (prec == 2)
+    dens_split_kernel<__bf16, 3><<<grid, 512, 0, s>>>(nso, nk, C0, X, sX, npts, ocol, padcols, npad, kout,
+                                                       rows_total, (__bf16*)dst, 14, inv);
+  else
+    dens_split_kernel<signed char, 4><<<grid, 512, 0, s>>>(nso, nk, C0, X, sX, npts, ocol, padcols, npad, kout,
+                                                            rows_total, (signed char*)dst, 7, inv);
 }
 
 void split_operand(hipStream_t s, int prec, const double* src, long lds, int rows, int kt, void* dst, double* inv) {

@@ -659,9 +659,14 @@ static void drop_blocks(phf_ctx* ctx) {
   if (ctx->dAb) (void)hipFree(ctx->dAb);
   if (ctx->d_rowmap) (void)hipFree(ctx->d_rowmap);
   if (ctx->d_colperm) (void)hipFree(ctx->d_colperm);
+  if (ctx->d_ocol) (void)hipFree(ctx->d_ocol);
+  if (ctx->d_padcols) (void)hipFree(ctx->d_padcols);
   ctx->dAb = nullptr;
   ctx->d_rowmap = nullptr;
   ctx->d_colperm = nullptr;
+  ctx->d_ocol = nullptr;
+  ctx->d_padcols = nullptr;
+  ctx->n_padcols = 0;
   ctx->blocked = false;
   ctx->blk_spin.clear();
 }
@@ -729,6 +734,20 @@ int phf_build_blocks(phf_ctx* ctx, const std::vector<int>& spin, bool release_de
   HIPCK(hipMalloc(&ctx->d_colperm, sizeof(int) * colperm.size()));
   HIPCK(hipMemcpyAsync(ctx->d_rowmap, rowmap.data(), sizeof(int) * rowmap.size(), hipMemcpyHostToDevice, s));
   HIPCK(hipMemcpyAsync(ctx->d_colperm, colperm.data(), sizeof(int) * colperm.size(), hipMemcpyHostToDevice, s));
+  // inverse map (natural element -> permuted column) and the padding columns,
+  // for the fused density pack + split
+  std::vector<int> ocol(n2, -1), padcols;
+  for (int c = 0; c < (int)colperm.size(); ++c) {
+    if (colperm[c] >= 0) ocol[colperm[c]] = c;
+    else padcols.push_back(c);
+  }
+  HIPCK(hipMalloc(&ctx->d_ocol, sizeof(int) * n2));
+  HIPCK(hipMemcpyAsync(ctx->d_ocol, ocol.data(), sizeof(int) * n2, hipMemcpyHostToDevice, s));
+  ctx->n_padcols = (int)padcols.size();
+  if (!padcols.empty()) {
+    HIPCK(hipMalloc(&ctx->d_padcols, sizeof(int) * padcols.size()));
+    HIPCK(hipMemcpyAsync(ctx->d_padcols, padcols.data(), sizeof(int) * padcols.size(), hipMemcpyHostToDevice, s));
+  }
   HIPCK(hipMalloc(&ctx->dAb, sizeof(double) * aoff));
   for (int b = 0; b < 3; ++b)
     blk_gather_kernel<<<8192, 256, 0, s>>>(ctx->dA, ctx->kt, ctx->d_rowmap + ctx->blk_roff[b],
@@ -1376,6 +1395,13 @@ __global__ void spinrot_kernel(long G, int npg, long lenX, const double* __restr
 }
 }  // namespace
 
+// PHF_FUSED_SPLIT=0 keeps the density pack and the split as two passes over
+// FP64 rows (the tests compare both).
+static bool no_fused_split() {
+  const char* f = getenv("PHF_FUSED_SPLIT");
+  return f && atoi(f) == 0;
+}
+
 void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
   hipStream_t s = ctx->stream;
   const int nso = ctx->nso, N = b.N, npg = ctx->g_npg, ny = ctx->g_ny, nl = ctx->g_nl, V = b.V;
@@ -1436,6 +1462,14 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
   // with B = C0 come first, then B = C0*)
   // the pack writes columns [0, NSO^2) of rows [0, 2 V G): clear the padding
   // rows, or everything when the rows are padded too (or complex-packed)
+  // 16-bit and int8 forms with one density per point (V = 1), real A and
+  // NSO <= 128: the pack writes the slices directly (no FP64 rows)
+  if (b.prec >= PHF_PREC_BF16X3 && V == 1 && !ctx->cplx && nso <= dens_split_max_nso() && !no_fused_split()) {
+    dens_split(s, b.prec, nso, N, b.C0, b.X, sT, (int)G, b.blk ? ctx->d_ocol : nullptr,
+               b.blk ? ctx->d_padcols : nullptr, b.blk ? ctx->n_padcols : 0, (int)b.kP, b.Mp, b.Pm, b.Pinv);
+    if (marks) (void)hipEventRecord(marks[2], s);
+    return;
+  }
   if (ctx->cplx || kt != n2)
     fill_zero(s, b.Pt, b.Mp * kt);
   else

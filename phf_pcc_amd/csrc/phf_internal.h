@@ -23,6 +23,11 @@ void split_operand(hipStream_t s, int prec, const double* src, long lds, int row
 void split_operand_map(hipStream_t s, int prec, const double* src, long lds, int rows, int kt, const int* cmap,
                        int kout, void* dst, double* inv, const double* amax = nullptr, int rows_data = -1);
 void convert_f32(hipStream_t s, const double* src, float* dst, long n);
+// Density pack + 16-bit/int8 split in one pass (prec 2, 3 or 4; NSO <= dens_split_max_nso()):
+// slices of rows 2g, 2g+1 for g < npts from C0 and X_g = X + g sX; padding rows up to rows_total.
+int dens_split_max_nso();
+void dens_split(hipStream_t s, int prec, int nso, int nk, const double* C0, const double* X, long sX, int npts,
+                const int* ocol, const int* padcols, int npad, int kout, long rows_total, void* dst, double* inv);
 void eri_gemm_mixed(hipStream_t s, int prec, const void* P, long kp, const void* A, const double* invP,
                     const double* invA, double* C, int Mp, int Nr, int K, long ldc, int cfg, int splitk,
                     const double* bias, const int* colmap);
@@ -78,6 +83,9 @@ struct phf_ctx_s {
   int ktp = 0;               // permuted contraction length (sum of blk_k)
   int* d_rowmap = nullptr;   // [sum blk_rows] permuted A row -> natural i*nso + k, -1 padding
   int* d_colperm = nullptr;  // [ktp] permuted contraction column -> natural j*nso + l, -1 padding
+  int* d_ocol = nullptr;     // [nso^2] natural j*nso + l -> permuted column (inverse of d_colperm)
+  int* d_padcols = nullptr;  // permuted columns with d_colperm == -1
+  int n_padcols = 0;
   double* dAb = nullptr;
   void* dAbm[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   double* dAbinv[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
