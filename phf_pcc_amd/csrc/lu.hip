@@ -131,81 +131,53 @@ __global__ __launch_bounds__(256) void lu_inverse_kernel(int n, const double* __
   }
 }
 
-// One wave per matrix for n <= NMAX <= 64: row i of M lives in lane i's
-// registers.  Gauss-Jordan with the pivot column always in register 0: each
-// step writes every row's update shifted one place to the left and appends
-// the new inverse column in register NMAX-1, so every register index is a
-// compile-time constant; after n steps the inverse's column m sits in
-// register NMAX-n+m.  Rows never move: each lane tracks the position its row
-// holds in the row-interchanged matrix, so the pivot choice (largest
-// |Re|+|Im| among the rows at positions k..n-1, first position on ties), the
-// determinant's sign flips and info are those of the algorithm above.  The
-// pivot row (with the pivot) reaches the other lanes through LDS.  Each
-// step's pivot search (a butterfly over the wave) starts as soon as the new
-// column 0 is updated and overlaps the rest of the row update: with one wave
-// per matrix and fewer matrices than SIMDs, nothing else hides its latency.  With B = P M (P the row
-// interchanges), lane l holds row pos_l of B^{-1}, and M^{-1} = B^{-1} P, so
-// lane l's column m lands in column sigma(m) of M^{-1}, sigma(m) = the lane
-// whose row ended at position m.
+// One wave per matrix for n <= NMAX <= 64: each row of M is held by
+// LPR = 64 / NMAX lanes, lane (h, i) = h NMAX + i holding the CPL = NMAX / LPR
+// storage columns [h CPL, (h+1) CPL) of row i in registers.  Gauss-Jordan
+// with the pivot column always in storage column 0: each step writes every
+// row's update shifted one place to the left (the last column of part h comes
+// from the first of part h+1, one shuffle) and appends the new inverse column
+// in storage column NMAX-1, so every register index is a compile-time
+// constant; after n steps the inverse's column m sits in storage column
+// NMAX-n+m.  Rows never move: each lane tracks the position its row holds in
+// the row-interchanged matrix, so the pivot choice (largest |Re|+|Im| among
+// the rows at positions k..n-1, first position on ties), the determinant's
+// sign flips and info are those of the algorithm above.  The pivot row reaches
+// the other lanes through LDS.  Each step's pivot search (a butterfly over the
+// wave) starts as soon as the new column 0 is updated and overlaps the rest of
+// the row update: with one wave per matrix and fewer matrices than SIMDs,
+// nothing else hides its latency.  With B = P M (P the row interchanges), row
+// i holds row pos_i of B^{-1}, and M^{-1} = B^{-1} P, so storage column m of
+// row i lands in column sigma(m) of M^{-1}, sigma(m) = the row that ended at
+// position m.
 template <int NMAX>
 __global__ __launch_bounds__(64) void lu_inverse_wave_kernel(int n, const double* __restrict__ M, long mbs,
                                                              double* __restrict__ Minv, long ibs,
                                                              double* __restrict__ det, int* __restrict__ info) {
+  constexpr int LPR = 64 / NMAX, CPL = NMAX / LPR;
   __shared__ double2 s_row[NMAX];
   __shared__ int s_sig[64];
   const long g = blockIdx.x;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x, row = lane % NMAX, part = lane / NMAX;
   const double* Mg = M + g * mbs;
-  const bool real_row = lane < n;
-  double2 a[NMAX];
+  const bool real_row = row < n;
+  double2 a[CPL];
 #pragma unroll
-  for (int j = 0; j < NMAX; ++j) {
-    a[j] = make_double2(0.0, 0.0);
+  for (int t = 0; t < CPL; ++t) {
+    const int j = part * CPL + t;
+    a[t] = make_double2(0.0, 0.0);
     if (real_row && j < n) {
-      const long off = 2 * ((long)j * n + lane);   // global is column-major
-      a[j] = make_double2(Mg[off], Mg[off + 1]);
+      const long off = 2 * ((long)j * n + row);   // global is column-major
+      a[t] = make_double2(Mg[off], Mg[off + 1]);
     }
   }
-  int pos = lane;
+  int pos = row;
   bool done = !real_row;   // row already used as a pivot, or no row of M
   double2 detv = make_double2(1.0, 0.0);
   int sinfo = 0;
-  // pivot search of step 0; each later search runs on the freshly updated
-  // column while the rest of the row update is still in flight
-  double c = done ? -1.0 : fabs(a[0].x) + fabs(a[0].y);
-  if (c != c) c = -1.0;   // a NaN row is never a pivot; keeps the reduction wave-uniform
-  int key = (pos << 6) | lane;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const double oc = __shfl_xor(c, off);
-    const int ok = __shfl_xor(key, off);
-    if (oc > c || (oc == c && ok < key)) { c = oc; key = ok; }
-  }
-  for (int k = 0; k < n; ++k) {
-    if (!(c > 0.0)) { sinfo = k + 1; break; }
-    const int pl = key & 63, pp = key >> 6;
-    const bool is_p = lane == pl;
-    if (is_p) {
-#pragma unroll
-      for (int j = 0; j < NMAX; ++j) s_row[j] = a[j];
-    }
-    if (pos == k) pos = pp;
-    if (is_p) { pos = k; done = true; }
-    __syncthreads();
-    const double2 pv = s_row[0];
-    detv = cmul(detv, pv);
-    if (pp != k) detv = make_double2(-detv.x, -detv.y);
-    const double2 pinv = crecip(pv);
-    // pivot row: a <- pinv * row; other rows: a <- a - (f pinv) * row
-    const double2 gm = is_p ? make_double2(-pinv.x, -pinv.y) : cmul(a[0], pinv);
-    const double s0 = is_p ? 0.0 : 1.0;
-    {
-      const double2 r = s_row[1];
-      a[0] = make_double2(fma(-gm.x, r.x, fma(gm.y, r.y, s0 * a[1].x)),
-                          fma(-gm.x, r.y, fma(-gm.y, r.x, s0 * a[1].y)));
-    }
-    c = done ? -1.0 : fabs(a[0].x) + fabs(a[0].y);
-    if (c != c) c = -1.0;
+  auto search = [&](double& c, int& key) {
+    c = (done || part > 0) ? -1.0 : fabs(a[0].x) + fabs(a[0].y);
+    if (c != c) c = -1.0;   // a NaN row is never a pivot; keeps the reduction wave-uniform
     key = (pos << 6) | lane;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -213,26 +185,63 @@ __global__ __launch_bounds__(64) void lu_inverse_wave_kernel(int n, const double
       const int ok = __shfl_xor(key, off);
       if (oc > c || (oc == c && ok < key)) { c = oc; key = ok; }
     }
+  };
+  double c;
+  int key;
+  search(c, key);
+  for (int k = 0; k < n; ++k) {
+    if (!(c > 0.0)) { sinfo = k + 1; break; }
+    const int pl = key & 63, pp = key >> 6;   // pl: a part-0 lane, i.e. the pivot row
+    const bool is_p = row == pl;
+    if (is_p) {
 #pragma unroll
-    for (int j = 1; j < NMAX - 1; ++j) {
-      const double2 r = s_row[j + 1];
-      a[j] = make_double2(fma(-gm.x, r.x, fma(gm.y, r.y, s0 * a[j + 1].x)),
-                          fma(-gm.x, r.y, fma(-gm.y, r.x, s0 * a[j + 1].y)));
+      for (int t = 0; t < CPL; ++t) s_row[part * CPL + t] = a[t];
     }
-    a[NMAX - 1] = is_p ? pinv : make_double2(-gm.x, -gm.y);
+    if (pos == k) pos = pp;
+    if (is_p) { pos = k; done = true; }
+    // column 0 of this row (part 0's register 0) and the next part's first column
+    double2 f = a[0], nxt = a[0];
+    if constexpr (LPR > 1) {
+      f = make_double2(__shfl(a[0].x, row), __shfl(a[0].y, row));
+      const int src = (part < LPR - 1) ? lane + NMAX : lane;
+      nxt = make_double2(__shfl(a[0].x, src), __shfl(a[0].y, src));
+    }
+    __syncthreads();
+    const double2 pv = s_row[0];
+    detv = cmul(detv, pv);
+    if (pp != k) detv = make_double2(-detv.x, -detv.y);
+    const double2 pinv = crecip(pv);
+    // pivot row: a <- pinv * row; other rows: a <- a - (f pinv) * row
+    const double2 gm = is_p ? make_double2(-pinv.x, -pinv.y) : cmul(f, pinv);
+    const double s0 = is_p ? 0.0 : 1.0;
+    // storage column j = part CPL + t takes column j + 1 (register t + 1, or
+    // the next part's register 0); column NMAX - 1 takes the new inverse column
+    const bool last = part == LPR - 1;
+#pragma unroll
+    for (int t = 0; t < CPL; ++t) {
+      if (t == CPL - 1 && (LPR == 1 || last)) {
+        a[t] = is_p ? pinv : make_double2(-gm.x, -gm.y);
+      } else {
+        const double2 old = (t < CPL - 1) ? a[t + 1 < CPL ? t + 1 : t] : nxt;
+        const double2 r = s_row[part * CPL + t + 1];
+        a[t] = make_double2(fma(-gm.x, r.x, fma(gm.y, r.y, s0 * old.x)),
+                            fma(-gm.x, r.y, fma(-gm.y, r.x, s0 * old.y)));
+      }
+      if (t == 0) search(c, key);
+    }
     __syncthreads();
   }
-  s_sig[pos] = lane;
+  if (part == 0) s_sig[pos] = row;
   __syncthreads();
   if (real_row) {
     double* Ig = Minv + g * ibs;
 #pragma unroll
-    for (int q = 0; q < NMAX; ++q) {
-      const int m = q + n - NMAX;
+    for (int t = 0; t < CPL; ++t) {
+      const int m = part * CPL + t + n - NMAX;
       if (m >= 0) {
         const long off = 2 * ((long)s_sig[m] * n + pos);
-        Ig[off] = sinfo ? 0.0 : a[q].x;
-        Ig[off + 1] = sinfo ? 0.0 : a[q].y;
+        Ig[off] = sinfo ? 0.0 : a[t].x;
+        Ig[off + 1] = sinfo ? 0.0 : a[t].y;
       }
     }
   }
