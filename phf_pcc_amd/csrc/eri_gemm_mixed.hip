@@ -775,14 +775,14 @@ __global__ __launch_bounds__(256) void split_rows_kernel(const double* __restric
 // reading those rows.  Columns no element maps to are zeroed (padcols, or
 // [NSO^2, kout) without a map); workgroups past the last point clear the
 // padding rows up to rows_total (zero slices, inverse scale 1).
-template <typename T, int NS>
-__global__ __launch_bounds__(512) void dens_split_kernel(int nso, int nk, const double* __restrict__ C0,
+template <typename T, int NS, int NT, int FM, int FN, bool G3>
+__global__ __launch_bounds__(NT) void dens_split_kernel(int nso, int nk, const double* __restrict__ C0,
                                                           const double* __restrict__ X, long sX, int npts,
                                                           const int* __restrict__ ocol,
                                                           const int* __restrict__ padcols, int npad, int kout,
                                                           long rows_total, T* __restrict__ dst, int top,
                                                           double* __restrict__ inv) {
-  constexpr int TB = 128, KC = 16, LD = KC + 2, NT = 512, FM = 4, FN = 2;
+  constexpr int TB = 128, KC = 16, LD = KC + 2, WN = TB / (16 * FN);
   __shared__ double Ar[TB * LD], Ai[TB * LD], Br[TB * LD], Bi[TB * LD];
   __shared__ double red[2][NT / 64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -798,7 +798,7 @@ __global__ __launch_bounds__(512) void dens_split_kernel(int nso, int nk, const 
     }
     return;
   }
-  const int wm = wave >> 2, wn = wave & 3;   // 2 x 4 waves of 64 x 32
+  const int wm = wave / WN, wn = wave % WN;   // waves of (16 FM) x (16 FN)
   const double* Xg = X + (long)g * sX;
   d4_t p1[FM][FN], p2[FM][FN], p3[FM][FN];
 #pragma unroll
@@ -842,9 +842,16 @@ __global__ __launch_bounds__(512) void dens_split_kernel(int nso, int nk, const 
       for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
-          p1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bre[j], p1[i][j], 0, 0, 0);
-          p2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bim[j], p2[i][j], 0, 0, 0);
-          p3[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i] + aim[i], bre[j] + bim[j], p3[i][j], 0, 0, 0);
+          if constexpr (G3) {
+            p1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bre[j], p1[i][j], 0, 0, 0);
+            p2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bim[j], p2[i][j], 0, 0, 0);
+            p3[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i] + aim[i], bre[j] + bim[j], p3[i][j], 0, 0, 0);
+          } else {   // p1 = re, p2 = im directly
+            p1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bre[j], p1[i][j], 0, 0, 0);
+            p1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-aim[i], bim[j], p1[i][j], 0, 0, 0);
+            p2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bim[j], p2[i][j], 0, 0, 0);
+            p2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bre[j], p2[i][j], 0, 0, 0);
+          }
         }
     }
     __syncthreads();
@@ -857,7 +864,8 @@ __global__ __launch_bounds__(512) void dens_split_kernel(int nso, int nk, const 
     for (int j = 0; j < FN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const double xr = p1[i][j][r] - p2[i][j][r], xi = p3[i][j][r] - p1[i][j][r] - p2[i][j][r];
+        const double xr = G3 ? p1[i][j][r] - p2[i][j][r] : p1[i][j][r];
+        const double xi = G3 ? p3[i][j][r] - p1[i][j][r] - p2[i][j][r] : p2[i][j][r];
         p1[i][j][r] = xr;
         p2[i][j][r] = xi;
         const int gi = wm * 16 * FM + i * 16 + (lane >> 4) + 4 * r, gj = wn * 16 * FN + j * 16 + (lane & 15);
@@ -1011,19 +1019,27 @@ void split_operand_map(hipStream_t s, int prec, const double* src, long lds, int
                                                            amax, rd);
 }
 
+// workgroup shape of the fused density pack + split: DS_NT threads, waves of
+// (16 DS_FM) x (16 DS_FN) over the 128 x 128 tile
+#ifndef DS_NT
+#define DS_NT 512
+#define DS_FM 4
+#define DS_FN 2
+#define DS_G3 true
+#endif
 int dens_split_max_nso() { return 128; }
 
 void dens_split(hipStream_t s, int prec, int nso, int nk, const double* C0, const double* X, long sX, int npts,
                 const int* ocol, const int* padcols, int npad, int kout, long rows_total, void* dst, double* inv) {
   const unsigned grid = (unsigned)(npts + (rows_total - 2L * npts + 1) / 2);
   if (prec == 3)
-    dens_split_kernel<_Float16, 2><<<grid, 512, 0, s>>>(nso, nk, C0, X, sX, npts, ocol, padcols, npad, kout,
+    dens_split_kernel<_Float16, 2, DS_NT, DS_FM, DS_FN, DS_G3><<<grid, DS_NT, 0, s>>>(nso, nk, C0, X, sX, npts, ocol, padcols, npad, kout,
                                                          rows_total, (_Float16*)dst, 14, inv);
   else if (prec == 2)
-    dens_split_kernel<__bf16, 3><<<grid, 512, 0, s>>>(nso, nk, C0, X, sX, npts, ocol, padcols, npad, kout,
+    dens_split_kernel<__bf16, 3, DS_NT, DS_FM, DS_FN, DS_G3><<<grid, DS_NT, 0, s>>>(nso, nk, C0, X, sX, npts, ocol, padcols, npad, kout,
                                                        rows_total, (__bf16*)dst, 14, inv);
   else
-    dens_split_kernel<signed char, 4><<<grid, 512, 0, s>>>(nso, nk, C0, X, sX, npts, ocol, padcols, npad, kout,
+    dens_split_kernel<signed char, 4, DS_NT, DS_FM, DS_FN, DS_G3><<<grid, DS_NT, 0, s>>>(nso, nk, C0, X, sX, npts, ocol, padcols, npad, kout,
                                                             rows_total, (signed char*)dst, 7, inv);
 }
 
