@@ -766,8 +766,8 @@ __global__ __launch_bounds__(256) void split_rows_kernel(const double* __restric
 // Density pack fused with the split of its rows, for NSO <= 128: one
 // 1024-thread workgroup per grid point forms rho^T = conj(C0) X_g^T (NSO x NSO,
 // contraction N; C0 and X_g column-major NSO x N, complex interleaved) on the
-// FP64 matrix cores with the complex GEMM's exact MFMA sequence (three real
-// products per complex one, 16-deep k-chunks, 4-deep steps), takes the largest
+// FP64 matrix cores with the complex GEMM's exact MFMA sequence per output
+// (three real products per complex one, 4-deep steps in k order), takes the largest
 // |Re| and |Im| over the point, and writes the NS slices of both planes (rows
 // 2g and 2g + 1) straight into the ERI GEMM's operand, element (j, l) at
 // column ocol[j NSO + l] (j NSO + l without a map).  The slices are those
@@ -801,28 +801,33 @@ __global__ __launch_bounds__(NT) void dens_split_kernel(int nso, int nk, const d
   const int wm = wave / WN, wn = wave % WN;   // waves of (16 FM) x (16 FN)
   const double* Xg = X + (long)g * sX;
   d4_t p1[FM][FN], p2[FM][FN], p3[FM][FN];
+  // every load of a 16-deep k-chunk is issued before the first LDS store;
+  // the first chunk is peeled so that its loads are consumed before the
+  // accumulators come alive
+  constexpr int EL = TB * KC / NT;
+  auto stage = [&](int k0) {
+    double lar[EL], lai[EL], lbr[EL], lbi[EL];
+    bool lv[EL];
 #pragma unroll
-  for (int i = 0; i < FM; ++i)
-#pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      p1[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
-      p2[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
-      p3[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+    for (int e = 0; e < EL; ++e) {
+      const int idx = e * NT + tid, i = idx % TB, gk = k0 + idx / TB;
+      lv[e] = i < nso && gk < nk;
+      const long off = lv[e] ? 2L * (i + (long)gk * nso) : 0;
+      lar[e] = C0[off];
+      lai[e] = C0[off + 1];
+      lbr[e] = Xg[off];
+      lbi[e] = Xg[off + 1];
     }
-  for (int k0 = 0; k0 < nk; k0 += KC) {
 #pragma unroll
-    for (int e0 = 0; e0 < TB * KC; e0 += NT) {
-      const int e = e0 + tid, i = e % TB, kk = e / TB;
-      const int gk = k0 + kk;
-      const bool v = i < nso && gk < nk;
-      const long off = v ? 2L * (i + (long)gk * nso) : 0;
-      const double ar = C0[off], ai = C0[off + 1], br = Xg[off], bi = Xg[off + 1];
-      Ar[i * LD + kk] = v ? ar : 0.0;
-      Ai[i * LD + kk] = v ? -ai : 0.0;
-      Br[i * LD + kk] = v ? br : 0.0;
-      Bi[i * LD + kk] = v ? bi : 0.0;
+    for (int e = 0; e < EL; ++e) {
+      const int idx = e * NT + tid, i = idx % TB, kk = idx / TB;
+      Ar[i * LD + kk] = lv[e] ? lar[e] : 0.0;
+      Ai[i * LD + kk] = lv[e] ? -lai[e] : 0.0;
+      Br[i * LD + kk] = lv[e] ? lbr[e] : 0.0;
+      Bi[i * LD + kk] = lv[e] ? lbi[e] : 0.0;
     }
-    __syncthreads();
+  };
+  auto mfma_chunk = [&]() {
 #pragma unroll
     for (int ks = 0; ks < KC; ks += 4) {
       double are[FM], aim[FM], bre[FN], bim[FN];
@@ -854,7 +859,23 @@ __global__ __launch_bounds__(NT) void dens_split_kernel(int nso, int nk, const d
           }
         }
     }
+  };
+  stage(0);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      p1[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+      p2[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+      p3[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+    }
+  mfma_chunk();
+  for (int k0 = KC; k0 < nk; k0 += KC) {
     __syncthreads();
+    stage(k0);
+    __syncthreads();
+    mfma_chunk();
   }
   // complex results and the point's largest magnitudes
   double mre = 0.0, mim = 0.0;
