@@ -626,10 +626,13 @@ void sc_pass(phf_ctx* ctx, const ScBlk& k, const ScSizes& z, double* C2, double*
                                                                   C1, pm, gc, ctx->sc_A, ctx->sc_BT, ctx->sc_Y,
                                                                   ctx->sc_K, ctx->sc_w, qm, t0p, t1p);
     });
-    // stage 1, batched over the points: X_g[(a<b); (v, r<s)] = sum_{p<q} KA_g((a,b),(p,q)) C2[(p,q); (v, r<s)]
+    // stage 1, batched over the points: X_g[(a<b); (v, r<s)] = sum_{p<q} KA_g((a,b),(p,q)) C2[(p,q); (v, r<s)],
+    // formed as X_g^T = C2^T KA_g^T (same storage, a<b fastest) so that the
+    // stores run along a<b; the products and their order are the same
     timed(prof, 0, [&] {
-      zgemm_batched(s, (int)p12o, (int)(nv * p34i), (int)p12i, 1, 0, ZMat{ka, 2, 2 * p12o, 2 * p12o * p12i, 1, 0},
-                    ZMat{C2, 2, 2 * p12i, 0, 1, 0}, 0, 0, zout_colmajor(X, p12o, 2 * p12o * nv * p34i), gc);
+      zgemm_batched(s, (int)(nv * p34i), (int)p12o, (int)p12i, 1, 0, ZMat{C2, 2 * p12i, 2, 0, 1, 0},
+                    ZMat{ka, 2 * p12o, 2, 2 * p12o * p12i, 1, 0}, 0, 0,
+                    ZOut{X, 2 * p12o, 2, 2 * p12o * nv * p34i, 1}, gc);
     });
     // stage 2, one batch entry per group of gs points, contraction over (point, r<s):
     //   T2p[j][(a<b, v); (i<j)] (+)= sum_{g in group j, r<s} X_g[(a<b, v); (r<s)] KBW_g((i,j),(r,s))
