@@ -1452,10 +1452,11 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
                   zout_colmajor(b.M + v * sNN * G, N, sNN), (int)G);
   // K2: inverse and determinant of every variant
   lu_inverse_batched(s, N, b.M, sNN, b.Minv, sNN, b.Det, b.Info, (int)(V * G));
-  // X_v = T_{L_v} M_v^{-1}
+  // X_v = T_{L_v} M_v^{-1}, formed as X^T = M^{-T} T^T (stores along the NSO index)
   for (int v = 0; v < V; ++v)
-    zgemm_batched(s, nso, N, N, 1, 0, zmat_colmajor(b.T + variant_left(v) * sT * G, nso, sT),
-                  zmat_colmajor(b.Minv + v * sNN * G, N, sNN), 0, 0, zout_colmajor(b.X + v * sT * G, nso, sT), (int)G);
+    zgemm_batched(s, N, nso, N, 1, 0, zmat_colmajor_t(b.Minv + v * sNN * G, N, sNN),
+                  zmat_colmajor_t(b.T + variant_left(v) * sT * G, nso, sT), 0, 0,
+                  zout_colmajor_t(b.X + v * sT * G, nso, sT), (int)G);
   if (marks) (void)hipEventRecord(marks[1], s);
   // density pack: Pt row 2gv = Re rho_gv^T, row 2gv+1 = Im rho_gv^T (flattened
   // j*nso + l); rho^T = conj(B) X^T, one launch per right factor (variants
@@ -1742,9 +1743,10 @@ void eval_post(phf_ctx* ctx, const EvalBufs& b, bool resident) {
   const long sT = 2 * lenX, sNN = 2L * N * N;
   const int nB = (V == 4) ? 2 : 1;
   const long per = (V == 4) ? 2 * G : G;
-  // K4: FX = (h + Gamma) X   (the ERI GEMM epilogue added h)
-  zgemm_batched(s, nso, N, nso, 1, 0, ZMat{b.Gam, nso, 1, 2L * nrpad, nrpad, 0}, zmat_colmajor(b.X, nso, sT), 0, 0,
-                zout_colmajor(b.FX, nso, sT), (int)Gv);
+  // K4: FX = (h + Gamma) X   (the ERI GEMM epilogue added h), formed as
+  // FX^T = X^T (h + Gamma)^T so that the stores run along the NSO index
+  zgemm_batched(s, N, nso, nso, 1, 0, zmat_colmajor_t(b.X, nso, sT), ZMat{b.Gam, 1, nso, 2L * nrpad, nrpad, 0}, 0, 0,
+                zout_colmajor_t(b.FX, nso, sT), (int)Gv);
   // per right factor B: Q_v = B^H FX_v, Hc = B^H h (computed by eval_pre on a
   // side stream when the context has one), e_v = Makeh
   if (ctx->side[0])
@@ -1762,8 +1764,8 @@ void eval_post(phf_ctx* ctx, const EvalBufs& b, bool resident) {
   // Y_v = FX_v + e_v X_v - X_v Q_v
   yinit_kernel<<<(unsigned)std::min<long>((lenX * Gv + 255) / 256, 65536), 256, 0, s>>>(lenX, (int)Gv, b.FX, b.X, b.E,
                                                                                           b.Y);
-  zgemm_batched(s, nso, N, N, -1, 0, zmat_colmajor(b.X, nso, sT), zmat_colmajor(b.Q, N, sNN), 1, 0,
-                zout_colmajor(b.Y, nso, sT), (int)Gv);
+  zgemm_batched(s, N, nso, N, -1, 0, zmat_colmajor_t(b.Q, N, sNN), zmat_colmajor_t(b.X, nso, sT), 1, 0,
+                zout_colmajor_t(b.Y, nso, sT), (int)Gv);
   // K5: weighted grid sums; every variant lands in its own CI block, so one
   // reduction over all virtual points assembles the nci x nci matrices
   const int d = (V == 4) ? nci / 2 : nci;

@@ -50,6 +50,10 @@ inline ZMat zmat_colmajor_t(const double* p, long ld, long bs, int conj = 0) {
   return ZMat{p, 2 * ld, 2, bs, 1, conj};
 }
 inline ZOut zout_colmajor(double* p, long ld, long bs) { return ZOut{p, 2, 2 * ld, bs, 1}; }
+// the transpose of a column-major output (element (i, j) at p[2 (j + ld i)]):
+// a product C^T = B^T A^T written through it lands where C would, with the
+// stores running along the contiguous index
+inline ZOut zout_colmajor_t(double* p, long ld, long bs) { return ZOut{p, 2 * ld, 2, bs, 1}; }
 
 // Launch batched C = alpha * op(A) op(B) + beta * C (complex FP64) on stream s.
 // m x k times k x n; `batch` problems (blockIdx.z).
