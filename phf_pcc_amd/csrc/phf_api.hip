@@ -1410,6 +1410,10 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
   const long sT = 2 * lenX;          // batch stride of NSO x N column-major blocks
   const long sNN = 2L * N * N;
   const int nL = (V == 4) ? 2 : 1;
+  // spin-rotation grids: the overlaps come from 4 npg small products and a
+  // combine; the products are staged in b.Q (G N^2 per variant, free until
+  // the epilogue), which holds them when G >= 4 npg
+  const bool m_from_spinrot = ctx->g_spinrot && G >= 4L * npg;
   // Hc = B^H h per right factor (used by the energy in eval_post): on a side
   // stream, overlapping the pre-stages, when the context has one
   if (ctx->side[0]) {
@@ -1432,6 +1436,18 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
                     0, zout_colmajor(Q, nso, sT), 4 * npg);
       spinrot_kernel<<<(unsigned)std::min<long>((G * lenX + 255) / 256, 65535), 256, 0, s>>>(G, npg, lenX, ctx->dGu,
                                                                                             Q, T);
+      // the overlaps of the variants with this left factor the same way:
+      // M_g = B^H T_g = sum U_g[s][t] (B^H Q[s][t][ipg]), four N x N products
+      // per point-group element (staged in the Q buffer of the epilogue)
+      if (m_from_spinrot)
+        for (int v = 0; v < V; ++v) {
+          if (variant_left(v) != lc) continue;
+          double* MQ = b.Q + v * 4L * npg * sNN;
+          zgemm_batched(s, N, N, nso, 1, 0, ZMat{b.C0, 2L * nso, 2, 0, 1, variant_right(v) ? 0 : 1},
+                        zmat_colmajor(Q, nso, sT), 0, 0, zout_colmajor(MQ, N, sNN), 4 * npg);
+          spinrot_kernel<<<(unsigned)std::min<long>((G * N * N + 255) / 256, 65535), 256, 0, s>>>(
+              G, npg, (long)N * N, ctx->dGu, MQ, b.M + v * sNN * G);
+        }
       continue;
     }
     // Wpg[ipg] = (Rpg[ipg] Ref) L
@@ -1445,11 +1461,13 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
                   0, 0, zout_colmajor(T, nso, sT * npg * ny), nl);
   }
   if (marks) (void)hipEventRecord(marks[0], s);
-  // M_v = B_v^H T_{L_v}   (B = C0: C0^H; B = C0*: C0^T)
-  for (int v = 0; v < V; ++v)
-    zgemm_batched(s, N, N, nso, 1, 0, ZMat{b.C0, 2L * nso, 2, 0, 1, variant_right(v) ? 0 : 1},
-                  zmat_colmajor(b.T + variant_left(v) * sT * G, nso, sT), 0, 0,
-                  zout_colmajor(b.M + v * sNN * G, N, sNN), (int)G);
+  // M_v = B_v^H T_{L_v}   (B = C0: C0^H; B = C0*: C0^T); formed above from
+  // the spin-rotation products when the grid has that structure
+  if (!m_from_spinrot)
+    for (int v = 0; v < V; ++v)
+      zgemm_batched(s, N, N, nso, 1, 0, ZMat{b.C0, 2L * nso, 2, 0, 1, variant_right(v) ? 0 : 1},
+                    zmat_colmajor(b.T + variant_left(v) * sT * G, nso, sT), 0, 0,
+                    zout_colmajor(b.M + v * sNN * G, N, sNN), (int)G);
   // K2: inverse and determinant of every variant
   lu_inverse_batched(s, N, b.M, sNN, b.Minv, sNN, b.Det, b.Info, (int)(V * G));
   // X_v = T_{L_v} M_v^{-1}, formed as X^T = M^{-T} T^T (stores along the NSO index)
