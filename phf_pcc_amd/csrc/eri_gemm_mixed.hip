@@ -775,7 +775,7 @@ __global__ __launch_bounds__(256) void split_rows_kernel(const double* __restric
 // reading those rows.  Columns no element maps to are zeroed (padcols, or
 // [NSO^2, kout) without a map); workgroups past the last point clear the
 // padding rows up to rows_total (zero slices, inverse scale 1).
-template <typename T, int NS, int NT, int FM, int FN, bool G3>
+template <typename T, int NS, int NT, int FM, int FN>
 __global__ __launch_bounds__(NT) void dens_split_kernel(int nso, int nk, const double* __restrict__ C0,
                                                           const double* __restrict__ X, long sX, int npts,
                                                           const int* __restrict__ ocol,
@@ -847,16 +847,9 @@ __global__ __launch_bounds__(NT) void dens_split_kernel(int nso, int nk, const d
       for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
-          if constexpr (G3) {
-            p1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bre[j], p1[i][j], 0, 0, 0);
-            p2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bim[j], p2[i][j], 0, 0, 0);
-            p3[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i] + aim[i], bre[j] + bim[j], p3[i][j], 0, 0, 0);
-          } else {   // p1 = re, p2 = im directly
-            p1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bre[j], p1[i][j], 0, 0, 0);
-            p1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-aim[i], bim[j], p1[i][j], 0, 0, 0);
-            p2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bim[j], p2[i][j], 0, 0, 0);
-            p2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bre[j], p2[i][j], 0, 0, 0);
-          }
+          p1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bre[j], p1[i][j], 0, 0, 0);
+          p2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bim[j], p2[i][j], 0, 0, 0);
+          p3[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i] + aim[i], bre[j] + bim[j], p3[i][j], 0, 0, 0);
         }
     }
   };
@@ -885,8 +878,7 @@ __global__ __launch_bounds__(NT) void dens_split_kernel(int nso, int nk, const d
     for (int j = 0; j < FN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const double xr = G3 ? p1[i][j][r] - p2[i][j][r] : p1[i][j][r];
-        const double xi = G3 ? p3[i][j][r] - p1[i][j][r] - p2[i][j][r] : p2[i][j][r];
+        const double xr = p1[i][j][r] - p2[i][j][r], xi = p3[i][j][r] - p1[i][j][r] - p2[i][j][r];
         p1[i][j][r] = xr;
         p2[i][j][r] = xi;
         const int gi = wm * 16 * FM + i * 16 + (lane >> 4) + 4 * r, gj = wn * 16 * FN + j * 16 + (lane & 15);
@@ -1040,28 +1032,24 @@ void split_operand_map(hipStream_t s, int prec, const double* src, long lds, int
                                                            amax, rd);
 }
 
-// workgroup shape of the fused density pack + split: DS_NT threads, waves of
-// (16 DS_FM) x (16 DS_FN) over the 128 x 128 tile
-#ifndef DS_NT
-#define DS_NT 512
-#define DS_FM 4
-#define DS_FN 2
-#define DS_G3 true
-#endif
+// workgroup shape of the fused density pack + split: 512 threads, waves of
+// 64 x 32 over the 128 x 128 tile (1024 threads spill the three accumulator
+// sets; four real products per complex one at 1024 threads measured within 10%)
+constexpr int DS_NT = 512, DS_FM = 4, DS_FN = 2;
 int dens_split_max_nso() { return 128; }
 
 void dens_split(hipStream_t s, int prec, int nso, int nk, const double* C0, const double* X, long sX, int npts,
                 const int* ocol, const int* padcols, int npad, int kout, long rows_total, void* dst, double* inv) {
   const unsigned grid = (unsigned)(npts + (rows_total - 2L * npts + 1) / 2);
   if (prec == 3)
-    dens_split_kernel<_Float16, 2, DS_NT, DS_FM, DS_FN, DS_G3><<<grid, DS_NT, 0, s>>>(nso, nk, C0, X, sX, npts, ocol, padcols, npad, kout,
-                                                         rows_total, (_Float16*)dst, 14, inv);
+    dens_split_kernel<_Float16, 2, DS_NT, DS_FM, DS_FN><<<grid, DS_NT, 0, s>>>(
+        nso, nk, C0, X, sX, npts, ocol, padcols, npad, kout, rows_total, (_Float16*)dst, 14, inv);
   else if (prec == 2)
-    dens_split_kernel<__bf16, 3, DS_NT, DS_FM, DS_FN, DS_G3><<<grid, DS_NT, 0, s>>>(nso, nk, C0, X, sX, npts, ocol, padcols, npad, kout,
-                                                       rows_total, (__bf16*)dst, 14, inv);
+    dens_split_kernel<__bf16, 3, DS_NT, DS_FM, DS_FN><<<grid, DS_NT, 0, s>>>(
+        nso, nk, C0, X, sX, npts, ocol, padcols, npad, kout, rows_total, (__bf16*)dst, 14, inv);
   else
-    dens_split_kernel<signed char, 4, DS_NT, DS_FM, DS_FN, DS_G3><<<grid, DS_NT, 0, s>>>(nso, nk, C0, X, sX, npts, ocol, padcols, npad, kout,
-                                                            rows_total, (signed char*)dst, 7, inv);
+    dens_split_kernel<signed char, 4, DS_NT, DS_FM, DS_FN><<<grid, DS_NT, 0, s>>>(
+        nso, nk, C0, X, sX, npts, ocol, padcols, npad, kout, rows_total, (signed char*)dst, 7, inv);
 }
 
 void split_operand(hipStream_t s, int prec, const double* src, long lds, int rows, int kt, void* dst, double* inv) {

@@ -119,12 +119,12 @@ __global__ void wd_kernel(int G, int V, int nci, int d, const double* W, const d
 
 // e_g = (sum_{a,p} Hc(a,p) X_g(p,a) + sum_a Q_g(a,a)) / 2; one workgroup per g.
 __global__ __launch_bounds__(256) void energy_kernel(int nso, int nocc, const double* __restrict__ Hc,
-                                                     const double* __restrict__ X, const double* __restrict__ Q,
+                                                     const double* __restrict__ X, double* __restrict__ Q,
                                                      double* __restrict__ E) {
   __shared__ double sr[256], si[256];
   const long g = blockIdx.x;
   const double* Xg = X + g * 2L * nso * nocc;
-  const double* Qg = Q + g * 2L * nocc * nocc;
+  double* Qg = Q + g * 2L * nocc * nocc;
   double ar = 0.0, ai = 0.0;
   for (int idx = threadIdx.x; idx < nso * nocc; idx += 256) {
     const int p = idx % nso, a = idx / nso;
@@ -147,22 +147,15 @@ __global__ __launch_bounds__(256) void energy_kernel(int nso, int nocc, const do
     }
     __syncthreads();
   }
+  const double er = 0.5 * sr[0], ei = 0.5 * si[0];
   if (threadIdx.x == 0) {
-    E[2 * g] = 0.5 * sr[0];
-    E[2 * g + 1] = 0.5 * si[0];
+    E[2 * g] = er;
+    E[2 * g + 1] = ei;
   }
-}
-
-// Y_g = FX_g + e_g X_g
-__global__ void yinit_kernel(long per_g, int G, const double* FX, const double* X, const double* E,
-                             double* Y) {
-  const long total = per_g * G;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
-    const long g = i / per_g;
-    const double er = E[2 * g], ei = E[2 * g + 1];
-    const double xr = X[2 * i], xi = X[2 * i + 1];
-    Y[2 * i] = FX[2 * i] + er * xr - ei * xi;
-    Y[2 * i + 1] = FX[2 * i + 1] + er * xi + ei * xr;
+  // Q_g <- Q_g - e_g I, so that the adjoint is one product: Y = FX - X (Q - e I)
+  for (int a = threadIdx.x; a < nocc; a += 256) {
+    Qg[2 * (a + a * nocc)] -= er;
+    Qg[2 * (a + a * nocc) + 1] -= ei;
   }
 }
 
@@ -1323,7 +1316,6 @@ int eval_plan(phf_ctx* ctx, int nocc, int precision, int cmplxconj, EvalBufs& b)
     o.Q = a.take<double>(2 * (size_t)N * N * Gv);
     o.Hc = a.take<double>(2 * lenX * (V == 4 ? 2 : 1));
     o.E = a.take<double>(2 * (size_t)Gv);
-    o.Y = a.take<double>(2 * lenX * Gv);
     o.Wd = a.take<double>(2 * (size_t)Gv * nmn);
     o.Part = a.take<double>(2 * (size_t)b.nchunk * nmn * lenX);
     o.Ysum = a.take<double>(2 * (size_t)nmn * lenX);
@@ -1779,11 +1771,11 @@ void eval_post(phf_ctx* ctx, const EvalBufs& b, bool resident) {
     energy_kernel<<<(unsigned)per, 256, 0, s>>>(nso, N, Hc, b.X + bc * per * sT, b.Q + bc * per * sNN,
                                                 b.E + 2 * bc * per);
   }
-  // Y_v = FX_v + e_v X_v - X_v Q_v
-  yinit_kernel<<<(unsigned)std::min<long>((lenX * Gv + 255) / 256, 65536), 256, 0, s>>>(lenX, (int)Gv, b.FX, b.X, b.E,
-                                                                                          b.Y);
+  // Y_v = FX_v + e_v X_v - X_v Q_v = FX_v - X_v (Q_v - e_v I), in place over FX
+  // (energy_kernel shifted the diagonal of Q); Y^T = FX^T - (Q - e I)^T X^T
+  double* Y = b.FX;
   zgemm_batched(s, N, nso, N, -1, 0, zmat_colmajor_t(b.Q, N, sNN), zmat_colmajor_t(b.X, nso, sT), 1, 0,
-                zout_colmajor_t(b.Y, nso, sT), (int)Gv);
+                zout_colmajor_t(Y, nso, sT), (int)Gv);
   // K5: weighted grid sums; every variant lands in its own CI block, so one
   // reduction over all virtual points assembles the nci x nci matrices
   const int d = (V == 4) ? nci / 2 : nci;
@@ -1793,7 +1785,7 @@ void eval_post(phf_ctx* ctx, const EvalBufs& b, bool resident) {
   {
     dim3 grid((unsigned)((lenX + 255) / 256), (unsigned)b.nchunk);
     const unsigned fin = (unsigned)((lenX * nmn + 255) / 256);
-    wreduce_partial_kernel<<<grid, 256, 0, s>>>((int)Gv, b.gchunk, (int)lenX, nmn, b.Wd, b.Y, b.Part);
+    wreduce_partial_kernel<<<grid, 256, 0, s>>>((int)Gv, b.gchunk, (int)lenX, nmn, b.Wd, Y, b.Part);
     wreduce_final_kernel<<<fin, 256, 0, s>>>(b.nchunk, (int)lenX, nmn, b.Part, b.Ysum);
     wreduce_partial_kernel<<<grid, 256, 0, s>>>((int)Gv, b.gchunk, (int)lenX, nmn, b.Wd, b.X, b.Part);
     wreduce_final_kernel<<<fin, 256, 0, s>>>(b.nchunk, (int)lenX, nmn, b.Part, b.Xsum);

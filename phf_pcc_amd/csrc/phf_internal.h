@@ -189,8 +189,9 @@ struct EvalBufs {
   long Mp = 0, G = 0;
   double *C0 = nullptr, *Wpg = nullptr, *W2 = nullptr, *T = nullptr, *M = nullptr, *Minv = nullptr, *Det = nullptr;
   int* Info = nullptr;
+  // FX holds (h + Gamma) X, then the adjoint Y = FX - X (Q - e I) in place
   double *X = nullptr, *Pt = nullptr, *Gam = nullptr, *FX = nullptr, *Q = nullptr, *Hc = nullptr, *E = nullptr;
-  double *Y = nullptr, *Wd = nullptr, *Part = nullptr, *Ysum = nullptr, *Xsum = nullptr, *HS = nullptr;
+  double *Wd = nullptr, *Part = nullptr, *Ysum = nullptr, *Xsum = nullptr, *HS = nullptr;
   double *Grad = nullptr, *Rdm = nullptr;
   void* Pm = nullptr;
   double* Pinv = nullptr;
