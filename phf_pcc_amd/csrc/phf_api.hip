@@ -159,11 +159,15 @@ __global__ __launch_bounds__(256) void energy_kernel(int nso, int nocc, const do
   }
 }
 
-// partial[c][mn][e] = sum_{g in chunk c} Wd[g][mn] * V[g][e]   (complex)
+// partial[c][mn][e] = sum_{g in chunk c} Wd[g][mn] * V[g][e]   (complex), for
+// two sources at once: blockIdx.z = 0 takes (V0, part0), 1 takes (V1, part1)
 __global__ void wreduce_partial_kernel(int G, int gchunk, int len, int nmn, const double* __restrict__ Wd,
-                                       const double* __restrict__ V, double* __restrict__ part) {
+                                       const double* __restrict__ V0, const double* __restrict__ V1,
+                                       double* __restrict__ part0, double* __restrict__ part1) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   const int c = blockIdx.y;
+  const double* V = blockIdx.z ? V1 : V0;
+  double* part = blockIdx.z ? part1 : part0;
   if (e >= len) return;
   const int g0 = c * gchunk, g1 = min(G, g0 + gchunk);
   for (int mn = 0; mn < nmn; ++mn) {
@@ -180,11 +184,17 @@ __global__ void wreduce_partial_kernel(int G, int gchunk, int len, int nmn, cons
   }
 }
 
-__global__ void wreduce_final_kernel(int nchunk, int len, int nmn, const double* __restrict__ part,
-                                     double* __restrict__ out) {
+// out_z[mn][e] = sum_c part_z[c][mn][e] in chunk order, for both sources
+// (blockIdx.y = z)
+__global__ void wreduce_final_kernel(int nchunk, int len, int nmn, const double* __restrict__ part0,
+                                     const double* __restrict__ part1, double* __restrict__ out0,
+                                     double* __restrict__ out1) {
   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const double* part = blockIdx.y ? part1 : part0;
+  double* out = blockIdx.y ? out1 : out0;
   if (i >= (long)len * nmn) return;
   double ar = 0.0, ai = 0.0;
+#pragma unroll 8
   for (int c = 0; c < nchunk; ++c) {
     ar += part[2 * ((long)c * nmn * len + i)];
     ai += part[2 * ((long)c * nmn * len + i) + 1];
@@ -1317,7 +1327,7 @@ int eval_plan(phf_ctx* ctx, int nocc, int precision, int cmplxconj, EvalBufs& b)
     o.Hc = a.take<double>(2 * lenX * (V == 4 ? 2 : 1));
     o.E = a.take<double>(2 * (size_t)Gv);
     o.Wd = a.take<double>(2 * (size_t)Gv * nmn);
-    o.Part = a.take<double>(2 * (size_t)b.nchunk * nmn * lenX);
+    o.Part = a.take<double>(4 * (size_t)b.nchunk * nmn * lenX);   // two partial blocks (Ysum, Xsum)
     o.Ysum = a.take<double>(2 * (size_t)nmn * lenX);
     o.Xsum = a.take<double>(2 * (size_t)nmn * lenX);
     o.HS = a.take<double>(4 * (size_t)nmn);
@@ -1783,12 +1793,12 @@ void eval_post(phf_ctx* ctx, const EvalBufs& b, bool resident) {
   // Hmat/Smat from variants 0 and 1 only (blocks (0,0), (0,1); PHFTools.f90:153-179)
   hs_reduce_kernel<<<1, 256, 0, s>>>((int)((V == 4) ? 2 * G : G), nmn, b.Wd, b.E, b.HS);
   {
-    dim3 grid((unsigned)((lenX + 255) / 256), (unsigned)b.nchunk);
-    const unsigned fin = (unsigned)((lenX * nmn + 255) / 256);
-    wreduce_partial_kernel<<<grid, 256, 0, s>>>((int)Gv, b.gchunk, (int)lenX, nmn, b.Wd, Y, b.Part);
-    wreduce_final_kernel<<<fin, 256, 0, s>>>(b.nchunk, (int)lenX, nmn, b.Part, b.Ysum);
-    wreduce_partial_kernel<<<grid, 256, 0, s>>>((int)Gv, b.gchunk, (int)lenX, nmn, b.Wd, b.X, b.Part);
-    wreduce_final_kernel<<<fin, 256, 0, s>>>(b.nchunk, (int)lenX, nmn, b.Part, b.Xsum);
+    // Ysum and Xsum in one pair of launches (two partial blocks)
+    double* part1 = b.Part + 2L * b.nchunk * nmn * lenX;
+    dim3 grid((unsigned)((lenX + 255) / 256), (unsigned)b.nchunk, 2);
+    dim3 fin((unsigned)((lenX * nmn + 255) / 256), 2);
+    wreduce_partial_kernel<<<grid, 256, 0, s>>>((int)Gv, b.gchunk, (int)lenX, nmn, b.Wd, Y, b.X, b.Part, part1);
+    wreduce_final_kernel<<<fin, 256, 0, s>>>(b.nchunk, (int)lenX, nmn, b.Part, part1, b.Ysum, b.Xsum);
   }
   // Gradmat[mn] = Ysum[mn] B^H, Rdmmat[mn] = Xsum[mn] B^H (C-order [nso][nso]);
   // CI block rows m < d come from variants with B = C0, rows m >= d (V = 4)
