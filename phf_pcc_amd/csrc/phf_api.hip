@@ -511,6 +511,7 @@ extern "C" int phf_ctx_destroy(phf_ctx* ctx) {
   if (ctx->d_xitems) (void)hipFree(ctx->d_xitems);
   if (ctx->ws) (void)hipFree(ctx->ws);
   if (ctx->small) (void)hipFree(ctx->small);
+  if (ctx->pin) (void)hipHostFree(ctx->pin);
   for (double* q : {ctx->dGr1p, ctx->dGr2, ctx->dGrpgref, ctx->dGw, ctx->dGu, ctx->dGref, ctx->dGP})
     if (q) (void)hipFree(q);
   for (auto& e : ctx->ev)
@@ -1834,9 +1835,31 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
   const int nso = ctx->nso, N = nocc, nci = ctx->g_nci, ny = ctx->g_ny, npg = ctx->g_npg;
   const int nmn = nci * nci;
   const long G = b.G, n2 = (long)nso * nso;
-  std::vector<double> c0(2 * (size_t)nso * N);
-  build_c0(z, nso, N, cmplxconj, c0.data());
-  HIPCK(hipMemcpyAsync(b.C0, c0.data(), c0.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  // pinned staging: [C0 | Hmat, Smat | objective | info]
+  const long Gv = b.V * G;
+  const size_t nobj = 2 + 2 * (size_t)(nso - N) * N;
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+  const size_t c0_bytes = al(sizeof(double) * 2 * (size_t)nso * N), hs_bytes = al(sizeof(double) * 4 * (size_t)nmn);
+  const size_t obj_bytes = objective ? al(sizeof(double) * nobj) : 0, info_bytes = al(sizeof(int) * (size_t)Gv);
+  const size_t pin_need = c0_bytes + hs_bytes + obj_bytes + info_bytes;
+  if (ctx->pin_busy) {
+    HIPCK(hipStreamSynchronize(s));
+    ctx->pin_busy = false;
+  }
+  if (ctx->pin_size < pin_need) {
+    if (ctx->pin) HIPCK(hipHostFree(ctx->pin));
+    ctx->pin = nullptr;
+    ctx->pin_size = 0;
+    HIPCK(hipHostMalloc((void**)&ctx->pin, pin_need, hipHostMallocDefault));
+    ctx->pin_size = pin_need;
+  }
+  double* c0 = reinterpret_cast<double*>(ctx->pin);
+  double* hs = reinterpret_cast<double*>(ctx->pin + c0_bytes);
+  double* pobj = reinterpret_cast<double*>(ctx->pin + c0_bytes + hs_bytes);
+  int* info = reinterpret_cast<int*>(ctx->pin + c0_bytes + hs_bytes + obj_bytes);
+  build_c0(z, nso, N, cmplxconj, c0);
+  ctx->pin_busy = true;
+  HIPCK(hipMemcpyAsync(b.C0, c0, sizeof(double) * 2 * (size_t)nso * N, hipMemcpyHostToDevice, s));
   if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[0], s));
   eval_pre(ctx, b, ctx->profiling ? ctx->ev + 4 : nullptr);
   if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[1], s));
@@ -1849,16 +1872,9 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
   HIPCK(hipGetLastError());
   if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[3], s));
 
-  const long Gv = b.V * G;
-  std::vector<double> hs(4 * (size_t)nmn);
-  std::vector<int> info((size_t)Gv);
-  HIPCK(hipMemcpyAsync(hs.data(), b.HS, hs.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-  HIPCK(hipMemcpyAsync(info.data(), b.Info, info.size() * sizeof(int), hipMemcpyDeviceToHost, s));
-  if (!resident) {
-    HIPCK(hipMemcpyAsync(gradmat, b.Grad, 2 * (size_t)nmn * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPCK(hipMemcpyAsync(rdmmat, b.Rdm, 2 * (size_t)nmn * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
-  }
-  const size_t nobj = 2 + 2 * (size_t)(nso - N) * N;
+  // the objective kernel goes in before any copy back, and the small results
+  // come back through pinned memory, so nothing waits on the host until the
+  // one synchronization below
   if (objective) {
     if (ctx->small_size < sizeof(double) * nobj) {
       HIPCK(hipStreamSynchronize(s));
@@ -1872,9 +1888,17 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
     const int nel = (nso - N) * N;
     objective_kernel<<<(std::max(nel, 1) + 255) / 256, 256, 0, s>>>(nso, N, b.HS, b.Ysum, b.Xsum, dob);
     HIPCK(hipGetLastError());
-    HIPCK(hipMemcpyAsync(objective, dob, sizeof(double) * nobj, hipMemcpyDeviceToHost, s));
+    HIPCK(hipMemcpyAsync(pobj, dob, sizeof(double) * nobj, hipMemcpyDeviceToHost, s));
+  }
+  HIPCK(hipMemcpyAsync(hs, b.HS, sizeof(double) * 4 * (size_t)nmn, hipMemcpyDeviceToHost, s));
+  HIPCK(hipMemcpyAsync(info, b.Info, sizeof(int) * (size_t)Gv, hipMemcpyDeviceToHost, s));
+  if (!resident) {
+    HIPCK(hipMemcpyAsync(gradmat, b.Grad, 2 * (size_t)nmn * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCK(hipMemcpyAsync(rdmmat, b.Rdm, 2 * (size_t)nmn * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
   }
   HIPCK(hipStreamSynchronize(s));
+  ctx->pin_busy = false;
+  if (objective) std::memcpy(objective, pobj, sizeof(double) * nobj);
   if (ctx->profiling) {
     float t01, t12, t23, t03;
     HIPCK(hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]));
@@ -1905,8 +1929,8 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
                il + 1, iy + 1, ipg + 1, b.V > 1 ? (gv / G == 0 ? " (RDM)" : " (conjugate-pair density)") : "");
       return fail(ctx, PHF_ERR_SINGULAR, buf);
     }
-  std::memcpy(hmat, hs.data(), 2 * (size_t)nmn * sizeof(double));
-  std::memcpy(smat, hs.data() + 2 * nmn, 2 * (size_t)nmn * sizeof(double));
+  std::memcpy(hmat, hs, 2 * (size_t)nmn * sizeof(double));
+  std::memcpy(smat, hs + 2 * nmn, 2 * (size_t)nmn * sizeof(double));
   if (b.V == 4) {
     // Hmat(d:, :d) = Hmat(:d, d:)^H, Hmat(d:, d:) = Hmat(:d, :d), same for Smat (PHFTools.f90:174-179)
     const int d = nci / 2;

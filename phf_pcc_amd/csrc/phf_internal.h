@@ -116,6 +116,11 @@ struct phf_ctx_s {
   // small persistent device buffer for per-call arguments/results outside the arena
   char* small = nullptr;
   size_t small_size = 0;
+  // pinned host staging of one evaluation's C0 upload and small results
+  // (asynchronous copies; reused after each call's final synchronization)
+  char* pin = nullptr;
+  size_t pin_size = 0;
+  bool pin_busy = false;   // copies from/to pin may be in flight (a call ended before its synchronization)
   // the two exchange blocks X1, X2 as one grouped launch (reduced forms):
   // timed once per (precision, rows) against two launches, 1 = grouped
   std::map<std::tuple<int, long, int, int>, int> xgroup;   // (precision, rows, block rows, block K)
