@@ -4,8 +4,9 @@
 // (row) pivoting on the largest |Re|+|Im| in the column (IZAMAX's measure,
 // first index on ties), det = prod of pivots with a sign flip for every
 // PIV(k) != k, info = first exactly-zero pivot.  One workgroup per grid
-// point.  N <= 64 runs one wave per matrix with the rows in registers
-// (lu_inverse_wave_kernel); larger N keeps the matrix in LDS, inverted in
+// point.  N <= 32 runs one wave per matrix with the rows in registers
+// (lu_inverse_wave_kernel), 32 < N <= 64 two waves (lu_inverse_wave2_kernel);
+// larger N keeps the matrix in LDS, inverted in
 // place by Gauss-Jordan elimination (its pivots are the U_kk of the LU
 // factorization), then the row interchanges are undone as column
 // interchanges in reverse order.
@@ -252,6 +253,117 @@ __global__ __launch_bounds__(64) void lu_inverse_wave_kernel(int n, const double
   }
 }
 
+// 32 < n <= 64: the same algorithm with two waves per matrix, row i held by
+// lane i of both (wave h the storage columns [32 h, 32 h + 32)).  What the
+// one-wave form moves by shuffles crosses between the waves through LDS:
+// column 0 of every row (wave 0's register 0), the first column of wave 1
+// (the shifted-in column of wave 0), and the pivot search, which wave 0 alone
+// runs over all rows.  Two barriers per step, as in the one-wave form.
+__global__ __launch_bounds__(128) void lu_inverse_wave2_kernel(int n, const double* __restrict__ M, long mbs,
+                                                               double* __restrict__ Minv, long ibs,
+                                                               double* __restrict__ det, int* __restrict__ info) {
+  constexpr int NMAX = 64, CPL = 32;
+  __shared__ double2 s_row[NMAX];
+  __shared__ double2 s_c0[NMAX], s_c1[NMAX];
+  __shared__ int s_sig[NMAX];
+  __shared__ double s_best;
+  __shared__ int s_key;
+  const long g = blockIdx.x;
+  const int tid = threadIdx.x, row = tid & 63, part = tid >> 6;
+  const double* Mg = M + g * mbs;
+  const bool real_row = row < n;
+  double2 a[CPL];
+#pragma unroll
+  for (int t = 0; t < CPL; ++t) {
+    const int j = part * CPL + t;
+    a[t] = make_double2(0.0, 0.0);
+    if (real_row && j < n) {
+      const long off = 2 * ((long)j * n + row);   // global is column-major
+      a[t] = make_double2(Mg[off], Mg[off + 1]);
+    }
+  }
+  int pos = row;
+  bool done = !real_row;
+  double2 detv = make_double2(1.0, 0.0);
+  int sinfo = 0;
+  // wave 0 searches column 0 over all rows and publishes the result
+  auto search = [&]() {
+    if (part == 0) {
+      double c = done ? -1.0 : fabs(a[0].x) + fabs(a[0].y);
+      if (c != c) c = -1.0;   // a NaN row is never a pivot; keeps the reduction wave-uniform
+      int key = (pos << 6) | row;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double oc = __shfl_xor(c, off);
+        const int ok = __shfl_xor(key, off);
+        if (oc > c || (oc == c && ok < key)) { c = oc; key = ok; }
+      }
+      if (row == 0) {
+        s_best = c;
+        s_key = key;
+      }
+    }
+  };
+  search();
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    const double c = s_best;
+    const int key = s_key;
+    if (!(c > 0.0)) { sinfo = k + 1; break; }
+    const int pl = key & 63, pp = key >> 6;
+    const bool is_p = row == pl;
+    if (is_p) {
+#pragma unroll
+      for (int t = 0; t < CPL; ++t) s_row[part * CPL + t] = a[t];
+    }
+    if (part == 0) s_c0[row] = a[0];
+    else s_c1[row] = a[0];
+    if (pos == k) pos = pp;
+    if (is_p) { pos = k; done = true; }
+    __syncthreads();
+    const double2 pv = s_row[0];
+    detv = cmul(detv, pv);
+    if (pp != k) detv = make_double2(-detv.x, -detv.y);
+    const double2 pinv = crecip(pv);
+    const double2 f = s_c0[row];
+    const double2 nxt = s_c1[row];   // used by wave 0 only
+    const double2 gm = is_p ? make_double2(-pinv.x, -pinv.y) : cmul(f, pinv);
+    const double s0 = is_p ? 0.0 : 1.0;
+#pragma unroll
+    for (int t = 0; t < CPL; ++t) {
+      if (t == CPL - 1 && part == 1) {
+        a[t] = is_p ? pinv : make_double2(-gm.x, -gm.y);
+      } else {
+        const double2 old = (t < CPL - 1) ? a[t + 1 < CPL ? t + 1 : t] : nxt;
+        const double2 r = s_row[part * CPL + t + 1];
+        a[t] = make_double2(fma(-gm.x, r.x, fma(gm.y, r.y, s0 * old.x)),
+                            fma(-gm.x, r.y, fma(-gm.y, r.x, s0 * old.y)));
+      }
+      if (t == 0) search();
+    }
+    __syncthreads();
+  }
+  if (part == 0) s_sig[pos] = row;
+  __syncthreads();
+  if (real_row) {
+    double* Ig = Minv + g * ibs;
+#pragma unroll
+    for (int t = 0; t < CPL; ++t) {
+      const int m = part * CPL + t + n - NMAX;
+      if (m >= 0) {
+        const long off = 2 * ((long)s_sig[m] * n + pos);
+        Ig[off] = sinfo ? 0.0 : a[t].x;
+        Ig[off + 1] = sinfo ? 0.0 : a[t].y;
+      }
+    }
+  }
+  if (tid == 0) {
+    det[2 * g] = sinfo ? 0.0 : detv.x;
+    det[2 * g + 1] = sinfo ? 0.0 : detv.y;
+    info[g] = sinfo;
+  }
+}
+
 }  // namespace
 
 int lu_inverse_max_n() { return 96; }
@@ -264,7 +376,7 @@ void lu_inverse_batched(hipStream_t s, int n, const double* M, long mbs, double*
   else if (n <= 32)
     lu_inverse_wave_kernel<32><<<batch, 64, 0, s>>>(n, M, mbs, Minv, ibs, det, info);
   else if (n <= 64)
-    lu_inverse_wave_kernel<64><<<batch, 64, 0, s>>>(n, M, mbs, Minv, ibs, det, info);
+    lu_inverse_wave2_kernel<<<batch, 128, 0, s>>>(n, M, mbs, Minv, ibs, det, info);
   else
     lu_inverse_kernel<96><<<batch, 256, 0, s>>>(n, M, mbs, Minv, ibs, det, info);
 }
