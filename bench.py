@@ -653,7 +653,7 @@ def main():
                               "frac_hbm": b_rot / t_rot / 8e12},
                  "density": {"ms": st64[5] + st64[6], "algorithmic_bytes": b_den, "GBps": b_den / t_den / 1e9,
                              "frac_hbm": b_den / t_den / 8e12, "pack_ms": st64[6], "overlap_lu_solve_ms": st64[5]},
-                 "epilogue_ms": st64[3], "split_ms_f16x2": (res["f16x2"]["stages"][7] if "f16x2" in res else None)}
+                 "epilogue_ms": st64[3], "pack_split_ms_f16x2": (res["f16x2"]["stages"][6] + res["f16x2"]["stages"][7] if "f16x2" in res else None)}
     log(f"[bench] stages (fp64): rotation {st64[4]:.3f} ms = {stage_out['rotation']['GBps']:.0f} GB/s, density "
         f"{st64[5] + st64[6]:.3f} ms = {stage_out['density']['GBps']:.0f} GB/s (algorithmic bytes)")
 

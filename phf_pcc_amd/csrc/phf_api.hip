@@ -1916,7 +1916,7 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
     HIPCK(hipEventElapsedTime(&t61, ctx->ev[6], ctx->ev[1]));
     ctx->stage_ms[4] = t04;   // rotation chain T_g
     ctx->stage_ms[5] = t45;   // overlap, LU inverse + det, X_g
-    ctx->stage_ms[6] = t56;   // density pack rho_g^T (FP64 planes)
+    ctx->stage_ms[6] = t56;   // density pack rho_g^T (FP64 planes, or the fused pack + split)
     ctx->stage_ms[7] = t61;   // reduced-precision split / conversion of the densities
     ctx->nstage = 8;
   }
