@@ -126,12 +126,17 @@ __global__ __launch_bounds__(256) void energy_kernel(int nso, int nocc, const do
   const double* Xg = X + g * 2L * nso * nocc;
   double* Qg = Q + g * 2L * nocc * nocc;
   double ar = 0.0, ai = 0.0;
+  // idx = a nso + p, stepped by 256 without dividing
+  int p = threadIdx.x % nso, a = threadIdx.x / nso;
+  const int dp = 256 % nso, da = 256 / nso;
   for (int idx = threadIdx.x; idx < nso * nocc; idx += 256) {
-    const int p = idx % nso, a = idx / nso;
     const double hr = Hc[2 * (a + (long)p * nocc)], hi = Hc[2 * (a + (long)p * nocc) + 1];
     const double xr = Xg[2 * idx], xi = Xg[2 * idx + 1];
     ar += hr * xr - hi * xi;
     ai += hr * xi + hi * xr;
+    p += dp;
+    a += da;
+    if (p >= nso) { p -= nso; ++a; }
   }
   for (int a = threadIdx.x; a < nocc; a += 256) {
     ar += Qg[2 * (a + a * nocc)];
@@ -1377,24 +1382,33 @@ static void hc_products(phf_ctx* ctx, const EvalBufs& b, hipStream_t s) {
 
 namespace {
 // T_g = sum_{s,t} U_g[s][t] Q[s][t][g % npg] (NSO x N blocks, complex)
+// grid: x over the lenX elements of a point, y over points (grid-stride), so
+// no element index is divided
 __global__ void spinrot_kernel(long G, int npg, long lenX, const double* __restrict__ U, const double* __restrict__ Q,
                                double* __restrict__ T) {
-  const long tot = G * lenX;
-  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < tot; idx += (long)gridDim.x * blockDim.x) {
-    const long g = idx / lenX, e = idx % lenX;
+  for (long g = blockIdx.y; g < G; g += gridDim.y) {
     const int ipg = (int)(g % npg);
-    double re = 0.0, im = 0.0;
+    double u[8];
 #pragma unroll
-    for (int st = 0; st < 4; ++st) {
-      const double ur = U[8 * g + 2 * st], ui = U[8 * g + 2 * st + 1];
-      const long q = 2 * ((st * (long)npg + ipg) * lenX + e);
-      const double qr = Q[q], qi = Q[q + 1];
-      re += ur * qr - ui * qi;
-      im += ur * qi + ui * qr;
+    for (int q = 0; q < 8; ++q) u[q] = U[8 * g + q];
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < lenX; e += (long)gridDim.x * blockDim.x) {
+      double re = 0.0, im = 0.0;
+#pragma unroll
+      for (int st = 0; st < 4; ++st) {
+        const double ur = u[2 * st], ui = u[2 * st + 1];
+        const long q = 2 * ((st * (long)npg + ipg) * lenX + e);
+        const double qr = Q[q], qi = Q[q + 1];
+        re += ur * qr - ui * qi;
+        im += ur * qi + ui * qr;
+      }
+      T[2 * (g * lenX + e)] = re;
+      T[2 * (g * lenX + e) + 1] = im;
     }
-    T[2 * idx] = re;
-    T[2 * idx + 1] = im;
   }
+}
+
+dim3 spinrot_grid(long G, long lenX) {
+  return dim3((unsigned)std::min<long>((lenX + 255) / 256, 1024), (unsigned)std::min<long>(G, 65535));
 }
 }  // namespace
 
@@ -1437,8 +1451,7 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
       double* Q = W2;
       zgemm_batched(s, nso, N, nso, 1, 0, zmat_colmajor(ctx->dGP, nso, 2 * n2), zmat_colmajor(b.C0, nso, 0, lc), 0,
                     0, zout_colmajor(Q, nso, sT), 4 * npg);
-      spinrot_kernel<<<(unsigned)std::min<long>((G * lenX + 255) / 256, 65535), 256, 0, s>>>(G, npg, lenX, ctx->dGu,
-                                                                                            Q, T);
+      spinrot_kernel<<<spinrot_grid(G, lenX), 256, 0, s>>>(G, npg, lenX, ctx->dGu, Q, T);
       // the overlaps of the variants with this left factor the same way:
       // M_g = B^H T_g = sum U_g[s][t] (B^H Q[s][t][ipg]), four N x N products
       // per point-group element (staged in the Q buffer of the epilogue)
@@ -1448,8 +1461,8 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
           double* MQ = b.Q + v * 4L * npg * sNN;
           zgemm_batched(s, N, N, nso, 1, 0, ZMat{b.C0, 2L * nso, 2, 0, 1, variant_right(v) ? 0 : 1},
                         zmat_colmajor(Q, nso, sT), 0, 0, zout_colmajor(MQ, N, sNN), 4 * npg);
-          spinrot_kernel<<<(unsigned)std::min<long>((G * N * N + 255) / 256, 65535), 256, 0, s>>>(
-              G, npg, (long)N * N, ctx->dGu, MQ, b.M + v * sNN * G);
+          spinrot_kernel<<<spinrot_grid(G, (long)N * N), 256, 0, s>>>(G, npg, (long)N * N, ctx->dGu, MQ,
+                                                                      b.M + v * sNN * G);
         }
       continue;
     }
