@@ -118,25 +118,21 @@ __global__ void wd_kernel(int G, int V, int nci, int d, const double* W, const d
 }
 
 // e_g = (sum_{a,p} Hc(a,p) X_g(p,a) + sum_a Q_g(a,a)) / 2; one workgroup per g.
+// Hc(a, p) is stored at a nso + p, where X_g(p, a) is, so both reads are
+// contiguous 16-byte loads.
 __global__ __launch_bounds__(256) void energy_kernel(int nso, int nocc, const double* __restrict__ Hc,
                                                      const double* __restrict__ X, double* __restrict__ Q,
                                                      double* __restrict__ E) {
   __shared__ double sr[256], si[256];
   const long g = blockIdx.x;
-  const double* Xg = X + g * 2L * nso * nocc;
+  const double2* Xg = reinterpret_cast<const double2*>(X + g * 2L * nso * nocc);
+  const double2* H2 = reinterpret_cast<const double2*>(Hc);
   double* Qg = Q + g * 2L * nocc * nocc;
   double ar = 0.0, ai = 0.0;
-  // idx = a nso + p, stepped by 256 without dividing
-  int p = threadIdx.x % nso, a = threadIdx.x / nso;
-  const int dp = 256 % nso, da = 256 / nso;
   for (int idx = threadIdx.x; idx < nso * nocc; idx += 256) {
-    const double hr = Hc[2 * (a + (long)p * nocc)], hi = Hc[2 * (a + (long)p * nocc) + 1];
-    const double xr = Xg[2 * idx], xi = Xg[2 * idx + 1];
-    ar += hr * xr - hi * xi;
-    ai += hr * xi + hi * xr;
-    p += dp;
-    a += da;
-    if (p >= nso) { p -= nso; ++a; }
+    const double2 h = H2[idx], x = Xg[idx];
+    ar += h.x * x.x - h.y * x.y;
+    ai += h.x * x.y + h.y * x.x;
   }
   for (int a = threadIdx.x; a < nocc; a += 256) {
     ar += Qg[2 * (a + a * nocc)];
@@ -1371,13 +1367,14 @@ int eval_plan(phf_ctx* ctx, int nocc, int precision, int cmplxconj, EvalBufs& b)
 static inline int variant_left(int v) { return (v == 1 || v == 2) ? 1 : 0; }
 static inline int variant_right(int v) { return (v == 2 || v == 3) ? 1 : 0; }
 
-// Hc_B = B^H h for each right factor B (C0, and C0* when V = 4), N x NSO
+// Hc_B = B^H h for each right factor B (C0, and C0* when V = 4), N x NSO,
+// element (a, p) stored at a NSO + p (the layout of X_g(p, a))
 static void hc_products(phf_ctx* ctx, const EvalBufs& b, hipStream_t s) {
   const int nso = ctx->nso, N = b.N, nB = (b.V == 4) ? 2 : 1;
   const long sT = 2L * nso * N;
   for (int bc = 0; bc < nB; ++bc)
     zgemm_batched(s, N, nso, nso, 1, 0, ZMat{b.C0, 2L * nso, 2, 0, 1, bc ? 0 : 1},
-                  ZMat{ctx->dH1, 2L * nso, 2, 0, 1, 0}, 0, 0, zout_colmajor(b.Hc + bc * sT, N, 0), 1);
+                  ZMat{ctx->dH1, 2L * nso, 2, 0, 1, 0}, 0, 0, zout_colmajor_t(b.Hc + bc * sT, nso, 0), 1);
 }
 
 namespace {
