@@ -1391,15 +1391,13 @@ __global__ void spinrot_kernel(long G, int npg, long lenX, const double* __restr
     for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < lenX; e += (long)gridDim.x * blockDim.x) {
       double re = 0.0, im = 0.0;
 #pragma unroll
-      for (int st = 0; st < 4; ++st) {
+      for (int st = 0; st < 4; ++st) {   // complex elements as 16-byte loads and stores
         const double ur = u[2 * st], ui = u[2 * st + 1];
-        const long q = 2 * ((st * (long)npg + ipg) * lenX + e);
-        const double qr = Q[q], qi = Q[q + 1];
-        re += ur * qr - ui * qi;
-        im += ur * qi + ui * qr;
+        const double2 q = reinterpret_cast<const double2*>(Q)[(st * (long)npg + ipg) * lenX + e];
+        re += ur * q.x - ui * q.y;
+        im += ur * q.y + ui * q.x;
       }
-      T[2 * (g * lenX + e)] = re;
-      T[2 * (g * lenX + e) + 1] = im;
+      reinterpret_cast<double2*>(T)[g * lenX + e] = make_double2(re, im);
     }
   }
 }
