@@ -42,7 +42,7 @@ def log(msg):
 
 def make_workload(n=64, nocc=32, ngrid=(50, 16), seed=1002):
     from phf_pcc_amd.hamiltonian import synthetic_problem
-    from tests._cases import grid_args
+    from phf_pcc_amd.spin_proj import grid_args
     t0 = time.time()
     prob = synthetic_problem(n, nocc, seed=seed, build_dense=True, ngrid=ngrid)
     log(f"[bench] built synthetic Hamiltonian n={n} NSO={2*n} in {time.time()-t0:.1f}s")
@@ -76,23 +76,8 @@ def cpu_baseline(prob, ga, ref, nocc, G, seconds):
 
 
 def make_batch_workload(ninst=64, seed=1004):
-    """Uneven instances: n uniform on {16, 24, ..., 96}, N = n/2, L = 3n, grid
-    Lebedev {6, 14, 26, 38, 50} x gamma {4, 8, 16} drawn per instance."""
-    from phf_pcc_amd.hamiltonian import synthetic_problem
-    from tests._cases import grid_args
-    rng = np.random.default_rng(seed)
-    ns = rng.choice(np.arange(16, 97, 8), size=ninst)
-    nls = rng.choice([6, 14, 26, 38, 50], size=ninst)
-    nys = rng.choice([4, 8, 16], size=ninst)
-    inst, zs = [], []
-    for i in range(ninst):
-        n, ngrid = int(ns[i]), (int(nls[i]), int(nys[i]))
-        p = synthetic_problem(n, n // 2, seed=seed * 1000 + i, ngrid=ngrid)
-        ga = grid_args(n, ngrid, 0)
-        inst.append(dict(h=p.h_spatial, b=p.b, ref=p.ref, nocc=n // 2, ngrid=ga["ngrid"], r1=ga["r1"], r2=ga["r2"],
-                         rpg=ga["rpg"], weightsp=ga["weightsp"]))
-        zs.append(p.z)
-    return inst, zs, ns, nls * nys
+    from phf_pcc_amd.hamiltonian import batch_instances
+    return batch_instances(ninst, seed)
 
 
 def run_batch(args):
@@ -191,7 +176,7 @@ def run_large(args):
     the FP64 form per energy and gradient."""
     from phf_pcc_amd.hamiltonian import synthetic_problem
     from phf_pcc_amd.model import ProjectedHF
-    from tests._cases import grid_args
+    from phf_pcc_amd.spin_proj import grid_args
     n, nocc, ngrid = 128, 64, (50, 24)
     t0 = time.time()
     prob = synthetic_problem(n, nocc, seed=1003, ngrid=ngrid)
@@ -269,7 +254,7 @@ def run_adjoint(args):
     from phf_pcc_amd.hamiltonian import synthetic_problem
     from phf_pcc_amd.model import ProjectedEnergy, ProjectedHF, lbfgs_optimize
     from phf_pcc_amd.phf import eigen_solver, vec_decomp
-    from tests._cases import grid_args
+    from phf_pcc_amd.spin_proj import grid_args
     n, nocc, ngrid = 96, 48, (50, 16)
     t0 = time.time()
     prob = synthetic_problem(n, nocc, seed=1005, ngrid=ngrid)
@@ -350,7 +335,8 @@ def run_buildsc(args):
     import torch
     from oracle import buildsc_oracle as B
     from phf_pcc_amd.sc import ProjectedOverlap
-    from tests._cases import grid_args, random_unitary
+    from phf_pcc_amd.hamiltonian import random_unitary
+    from phf_pcc_amd.spin_proj import grid_args
     nao, nocc, ngrid, nb = 8, 8, (14, 8), args.sc_block
     nso = 2 * nao
     rng = np.random.default_rng(1006)

@@ -182,22 +182,27 @@ def wignerfac(j: int, n: int, m: int, alpha: float, beta: float, gama: float) ->
 
 def buildhsg(hone, htwo, ref, z, ngrid, nci, cmplxconj, ncisp, nocc,
              r1, r2, rpg, weightsp, weightpg, roota, rootb, rooty, qj, sp,
-             check_hermitian=True, points=None):
+             check_hermitian=True, points=None, gamma=None):
     """BuildHSG (PHFTools.f90:10-201) for one rank.
 
     Returns (Hmat, Smat, Gradmat, Rdmmat) with shapes (nci,nci), (nci,nci),
     (nci,nci,NSO,NSO), (nci,nci,NSO,NSO).  ``points`` optionally restricts the
     grid to a list of (il, iy, ipg) triples (used for bounded CPU samples).
+    ``gamma`` optionally replaces MakeGamma over the dense htwo by a callable
+    rho -> Gamma (``oracle.factored.gamma_factored``; htwo may then be None).
     """
     hone = np.asarray(hone, dtype=complex)
     ref = np.asarray(ref, dtype=complex)
     nso = hone.shape[0]
     npg = rpg.shape[0]
     ncipg = weightpg.shape[1]
-    a_mat = np.asarray(htwo, dtype=complex).transpose(0, 2, 1, 3).reshape(nso * nso, nso * nso)
+    if gamma is not None:
+        gamma_of = gamma
+    else:
+        a_mat = np.asarray(htwo, dtype=complex).transpose(0, 2, 1, 3).reshape(nso * nso, nso * nso)
 
-    def gamma_of(rho):
-        return (a_mat @ rho.T.reshape(nso * nso)).reshape(nso, nso)
+        def gamma_of(rho):
+            return (a_mat @ rho.T.reshape(nso * nso)).reshape(nso, nso)
 
     ztmp = np.array(z, dtype=complex)
     if cmplxconj == 0:
@@ -344,7 +349,7 @@ def vec_decomp(vec, ncisp, ncipg, ncik):
     return fsp, fpg, fk
 
 
-def eandg(x, hone, htwo, ref, nocc, grid_args, cmplxconj=1):
+def eandg(x, hone, htwo, ref, nocc, grid_args, cmplxconj=1, gamma=None, points=None):
     """EandG (PHF.py:55-91) without the optional Z masks: x = [Re z, Im z] ->
     (E0.real, [Re G, Im G]) where G is the (NSO-N, N) block of LocalG."""
     nso = hone.shape[0]
@@ -353,7 +358,8 @@ def eandg(x, hone, htwo, ref, nocc, grid_args, cmplxconj=1):
     if cmplxconj == 0:
         z = z.real.astype(complex)
     ga = dict(grid_args)
-    hmat, smat, gradmat, rdmmat = buildhsg(hone, htwo, ref, z, cmplxconj=cmplxconj, nocc=nocc, **ga)
+    hmat, smat, gradmat, rdmmat = buildhsg(hone, htwo, ref, z, cmplxconj=cmplxconj, nocc=nocc, gamma=gamma,
+                                           points=points, **ga)
     evals, evecs = eigen_solver(hmat, smat)
     e0 = evals[0]
     ncik = 2 if cmplxconj == 2 else 1

@@ -21,10 +21,16 @@ failures that STOP the reference process raise instead:
 ``NotHermitianError`` (PHFTools.f90:195-200).  ``comm`` (an MPI handle) is
 accepted and ignored: one context drives one GPU.
 
-The device keeps the last Hamiltonian.  It is re-uploaded when ``htwo`` or
-``hone`` differs from the cached pair by identity, address, shape or a
-content fingerprint (a sample of elements); ``phftools.set_hamiltonian``
-uploads explicitly and ``phftools.invalidate()`` drops the cache.
+The device keeps the last Hamiltonian (the reference re-reads HTwo on every
+call; re-uploading a multi-GB array per call would cost more than the
+evaluation).  It is re-uploaded when ``hone`` or ``htwo`` differs from the
+cached pair in shape, dtype or content: arrays up to 256 MiB
+(``PHF_CACHE_FULL_HASH_MAX``) are compared by a hash of every byte; larger
+ones by address plus a hash of a sample of elements, unless
+``PHF_STRICT_CACHE=1`` asks for the full hash on every call.  A caller that
+edits a larger array in place without PHF_STRICT_CACHE must call
+``phftools.set_hamiltonian`` (uploads explicitly) or ``phftools.invalidate()``
+(drops the cache).
 
 ``phftools.precision`` selects the ERI-contraction precision (default
 "fp64", the reference's arithmetic).
@@ -39,18 +45,38 @@ import numpy as np
 from . import _lib
 
 
+try:
+    import xxhash as _xxhash
+except ImportError:  # pragma: no cover - the image ships xxhash
+    _xxhash = None
+
+# Arrays up to this many bytes are hashed in full on every call (xxh3-128
+# streams about 8 GB/s, so 256 MiB costs about 30 ms).  Larger arrays are
+# hashed in full only with PHF_STRICT_CACHE=1; otherwise their key is the
+# address plus a sample of elements, and a caller that edits such an array in
+# place must call phftools.set_hamiltonian or phftools.invalidate().
+FULL_HASH_MAX_BYTES = int(os.environ.get("PHF_CACHE_FULL_HASH_MAX", str(256 << 20)))
+
+
+def _digest(buf) -> str:
+    if _xxhash is not None:
+        return _xxhash.xxh3_128_hexdigest(buf)
+    return hashlib.blake2b(buf, digest_size=16).hexdigest()
+
+
 def _fingerprint(a: np.ndarray) -> tuple:
+    a = np.asarray(a)
     flat = a.reshape(-1)
+    strict = os.environ.get("PHF_STRICT_CACHE", "0") not in ("", "0")
+    if a.nbytes <= FULL_HASH_MAX_BYTES or strict:
+        # exact: every byte of the content (the address does not matter)
+        return (a.shape, a.dtype.str, _digest(memoryview(np.ascontiguousarray(flat)).cast("B")))
     n = flat.size
     h = hashlib.blake2b(digest_size=16)
-    if n <= 1 << 14:
-        h.update(np.ascontiguousarray(flat).tobytes())
-    else:
-        # 2048 evenly spaced elements plus both ends (an exhaustive hash of a
-        # multi-GB htwo per call would cost more than the evaluation)
-        h.update(np.ascontiguousarray(flat[:1024]).tobytes())
-        h.update(np.ascontiguousarray(flat[n - 1024:]).tobytes())
-        h.update(np.ascontiguousarray(flat[:: max(1, n // 2048)]).tobytes())
+    # sampled: 2048 evenly spaced elements plus both ends, keyed with the address
+    h.update(np.ascontiguousarray(flat[:1024]).tobytes())
+    h.update(np.ascontiguousarray(flat[n - 1024:]).tobytes())
+    h.update(np.ascontiguousarray(flat[:: max(1, n // 2048)]).tobytes())
     return (a.__array_interface__["data"][0], a.shape, a.dtype.str, h.hexdigest())
 
 

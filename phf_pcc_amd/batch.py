@@ -106,7 +106,10 @@ class ProjectedHFBatch:
             _lib.check(rc, self._h, batch=True)
         grads, off = [], 0
         for shp, sz in zip(self.shapes, self.sizes):
-            grads.append(g[off:off + sz].reshape(shp))
+            gi = g[off:off + sz].reshape(shp)
+            # CmplxConj = 0: the objective ignores Im Z (the kernel takes Re Z),
+            # and EandG returns a real gradient (PHF.py:80-81)
+            grads.append(gi.real.astype(np.complex128) if int(cmplxconj) == 0 else gi)
             off += sz
         return e, grads, hs
 

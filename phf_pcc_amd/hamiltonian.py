@@ -262,3 +262,32 @@ def synthetic_problem(n: int, nocc: int, nl: int | None = None, seed: int = 1002
         prob.eri = eri_from_factors(b)
         _, prob.htwo = dirac_mo(h, prob.eri, ref)
     return prob
+
+
+def random_unitary(n: int, rng, real: bool = False) -> np.ndarray:
+    """Haar-distributed unitary (orthogonal when real) from QR with the phase of R's diagonal removed."""
+    a = rng.normal(size=(n, n)) + (0 if real else 1j * rng.normal(size=(n, n)))
+    q, r = np.linalg.qr(a)
+    return q * (np.diag(r) / np.abs(np.diag(r)))
+
+
+def batch_instances(ninst: int = 64, seed: int = 1004):
+    """The uneven batch workload: n uniform on {16, 24, ..., 96}, N = n/2,
+    L = 3n, grids Lebedev {6, 14, 26, 38, 50} x gamma {4, 8, 16} drawn per
+    instance (instance i seeded with seed * 1000 + i).  Returns (instances,
+    zs, ns, grid points per instance) with instances in the form
+    ProjectedHFBatch takes (keys h, b, ref, nocc, ngrid, r1, r2, rpg, weightsp)."""
+    from .spin_proj import grid_args
+    rng = np.random.default_rng(seed)
+    ns = rng.choice(np.arange(16, 97, 8), size=ninst)
+    nls = rng.choice([6, 14, 26, 38, 50], size=ninst)
+    nys = rng.choice([4, 8, 16], size=ninst)
+    inst, zs = [], []
+    for i in range(ninst):
+        n, ngrid = int(ns[i]), (int(nls[i]), int(nys[i]))
+        p = synthetic_problem(n, n // 2, seed=seed * 1000 + i, ngrid=ngrid)
+        ga = grid_args(n, ngrid, 0)
+        inst.append(dict(h=p.h_spatial, b=p.b, ref=p.ref, nocc=n // 2, ngrid=ga["ngrid"], r1=ga["r1"], r2=ga["r2"],
+                         rpg=ga["rpg"], weightsp=ga["weightsp"]))
+        zs.append(p.z)
+    return inst, zs, ns, nls * nys

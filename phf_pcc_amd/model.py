@@ -18,18 +18,21 @@ from __future__ import annotations
 import numpy as np
 
 from . import _lib
-from .phf import eigen_solver, vec_decomp
+from .phf import ObjectiveConfig, eigen_solver, vec_decomp
 
 
 class ProjectedHF:
     def __init__(self, nocc: int, ref: np.ndarray, grid: dict, hone=None, htwo=None, factors=None,
                  precision: str = "fp64", device: int = 0, cmplxconj: int = 1, ctx: _lib.Context | None = None,
-                 structure: bool = True):
+                 structure: bool = True, is_rhf: bool = False, nfo: int = 0):
         """grid: keys ngrid, nci, ncisp, r1, r2, rpg, weightsp, weightpg, roota, rootb, rooty, qj, sp
         (the buildhsg grid arguments).  Either hone/htwo (MO basis, as the
         drop-in takes them) or factors=(h, b) with (pq|rs) = sum_L b[L,p,q] b[L,r,s].
         structure: use the spin-block structure of A when Ref has spin-pure MOs
-        (False: always contract the dense A, the plain form)."""
+        (False: always contract the dense A, the plain form).
+        is_rhf, nfo: the RHFProj and FrozenProj masks of EandG (PHF.py:62-68,
+        78-85); with the grid's sp == 2 SzProj and with cmplxconj == 0 the
+        real-Z mask apply too, to Z before the evaluation and to G after it."""
         self.ctx = ctx or _lib.Context(device)
         self.nocc = int(nocc)
         self.cmplxconj = int(cmplxconj)
@@ -42,6 +45,8 @@ class ProjectedHF:
             self.ctx.set_hamiltonian(hone, htwo, dense=not structure)
         self.nso = ref.shape[0]
         self.grid = grid
+        self.masks = ObjectiveConfig(nso=self.nso, nocc=self.nocc, cmplxconj=self.cmplxconj, sp=int(grid["sp"]),
+                                     is_rhf=bool(is_rhf), nfo=int(nfo)).masks
         self._set_grid(ref, grid)
         self.nevals = 0
 
@@ -94,11 +99,19 @@ class ProjectedHF:
         return g
 
     def energy_and_gradient(self, z: np.ndarray):
-        """Projected energy E0 and G = dE/dZ* (shape (NSO-N, N)).  The grid sums
-        stay on the device; only Hmat/Smat (nci x nci) and G cross PCIe."""
+        """Projected energy E0 and G = dE/dZ* (shape (NSO-N, N)) as EandG forms
+        them (PHF.py:55-91): the Z masks are applied to Z before the grid pass
+        and to G after it (with cmplxconj == 0, G's imaginary part is zero,
+        PHF.py:80-81).  The grid sums stay on the device; only Hmat/Smat
+        (nci x nci) and G cross PCIe."""
         z = np.ascontiguousarray(z, dtype=np.complex128)
         if z.shape != (self.nso - self.nocc, self.nocc):
             raise ValueError(f"z must be ({self.nso - self.nocc}, {self.nocc})")
+        z = np.ascontiguousarray(self.masks(z))
+        e0, g = self._energy_and_gradient(z)
+        return e0, np.ascontiguousarray(self.masks(g))
+
+    def _energy_and_gradient(self, z: np.ndarray):
         if self.cmplxconj == 2:
             h, s, gm, rm = self.buildhsg(z)
             ev, vec = eigen_solver(h, s)

@@ -20,10 +20,20 @@ TOL = 1e-6
 
 
 def eigen_solver(h: np.ndarray, s: np.ndarray, tol: float = TOL):
-    """Lowest states of H c = E S c in the span of S's eigenvectors with |eigenvalue| > tol."""
+    """Lowest states of H c = E S c by canonical orthogonalisation (PHFTools.py:8-18).
+
+    The column selection is the reference's: the inverse square roots of the
+    eigenvalues of S with |lambda| > tol, in eigh's ascending order, scale the
+    LAST dim eigenvectors, dim being how many eigenvalues pass.  When S has an
+    eigenvalue below -tol this pairs the columns with other eigenvalues than a
+    |lambda| > tol mask would (e.g. eigenvalues (-0.3, 1e-8, 2.0): the
+    reference keeps the eigenvectors of 1e-8 and 2.0, scaled by 0.3**-0.5 and
+    2.0**-0.5); it is kept exactly so that E0 and the CI vector agree with
+    the reference on indefinite S."""
     sval, svec = np.linalg.eigh(s)
-    keep = np.abs(sval) > tol
-    x = svec[:, keep] / np.sqrt(np.abs(sval[keep]))
+    inv_s = np.array([1.0 / np.sqrt(abs(v)) for v in sval if abs(v) > tol])
+    dim = inv_s.size
+    x = svec[:, sval.size - dim:] * inv_s[None, :]
     hh = x.conj().T @ h @ x
     ev, vec = np.linalg.eigh(hh)
     return ev, x @ vec

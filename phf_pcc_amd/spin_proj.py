@@ -263,3 +263,20 @@ def spin_proj(sp: int, ngrid, nao: int, j: int = 0) -> SpinGrid:
         r2 = np.stack([build_rot_mat(0.0, 0.0, rooty[i], nao) for i in range(ngrid[1])])
         return SpinGrid(3, ngrid, 1, roota, rootb, rooty, w, r1, r2)
     raise ValueError(f"unsupported SP={sp}")
+
+
+def grid_args(nao: int, ngrid=(14, 8), j: int = 0, sp: int = 1, rpg=None, weightpg=None) -> dict:
+    """The grid arguments of phftools.buildhsg for a spin grid (Spin_Proj.py:3-39),
+    as keywords: ngrid, nci, ncisp, r1, r2, rpg, weightsp, weightpg, roota,
+    rootb, rooty, qj, sp.  Without rpg/weightpg there is no point group (one
+    identity Rpg, weight 1); nci = ncisp * ncipg."""
+    g = spin_proj(sp, ngrid, nao, j)
+    nso = 2 * nao
+    if rpg is None:
+        rpg = np.eye(nso, dtype=complex)[None]
+    if weightpg is None:
+        weightpg = np.ones((rpg.shape[0], 1, 1))
+    ncipg = weightpg.shape[1]
+    return dict(ngrid=np.array(g.ngrid), nci=g.nci * ncipg, ncisp=g.nci, r1=g.r1, r2=g.r2, rpg=rpg,
+                weightsp=g.weightsp, weightpg=weightpg, roota=g.roota, rootb=g.rootb, rooty=g.rooty,
+                qj=j, sp=sp)

@@ -3,29 +3,9 @@ from __future__ import annotations
 
 import numpy as np
 
-from phf_pcc_amd.hamiltonian import (ao2mo, hubbard, hubbard_ring_problem, mulliken_to_dirac,
-                                     spin_orbital_blocks, synthetic_problem, sort_orb, uhf_hubbard)
-from phf_pcc_amd.spin_proj import spin_proj
-
-
-def grid_args(nao: int, ngrid=(14, 8), j: int = 0, sp: int = 1, rpg=None, weightpg=None):
-    """Keyword arguments of buildhsg for a spin grid (Spin_Proj.py:3-39)."""
-    g = spin_proj(sp, ngrid, nao, j)
-    nso = 2 * nao
-    if rpg is None:
-        rpg = np.eye(nso, dtype=complex)[None]
-    if weightpg is None:
-        weightpg = np.ones((rpg.shape[0], 1, 1))
-    ncipg = weightpg.shape[1]
-    return dict(ngrid=np.array(g.ngrid), nci=g.nci * ncipg, ncisp=g.nci, r1=g.r1, r2=g.r2, rpg=rpg,
-                weightsp=g.weightsp, weightpg=weightpg, roota=g.roota, rootb=g.rootb, rooty=g.rooty,
-                qj=j, sp=sp)
-
-
-def random_unitary(n: int, rng, real: bool = False) -> np.ndarray:
-    a = rng.normal(size=(n, n)) + (0 if real else 1j * rng.normal(size=(n, n)))
-    q, r = np.linalg.qr(a)
-    return q * (np.diag(r) / np.abs(np.diag(r)))
+from phf_pcc_amd.hamiltonian import (ao2mo, hubbard, hubbard_ring_problem, mulliken_to_dirac,  # noqa: F401
+                                     random_unitary, spin_orbital_blocks, synthetic_problem, sort_orb, uhf_hubbard)
+from phf_pcc_amd.spin_proj import grid_args  # noqa: F401
 
 
 def hubbard_general_ref(nsite: int, u: float, seed: int, zscale: float = 0.1, complex_ref: bool = False):

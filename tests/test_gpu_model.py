@@ -116,7 +116,8 @@ def test_resident_gradient_projected_blocks(nsite, seed, j, ngrid, npg, cmplxcon
     e_ref, g_ref = O.eandg(x, case["hone"], case["htwo"], case["ref"], case["nocc"], ga, cmplxconj=cmplxconj)
     assert abs(e - e_ref) < 1e-10
     if cmplxconj == 0:
-        g = g.real.astype(complex)
+        # EandG's mask (PHF.py:80-81) is applied by the model itself
+        assert np.all(g.imag == 0)
     gv = np.concatenate([g.real.ravel(), g.imag.ravel()])
     assert np.abs(gv - g_ref).max() < 1e-10
     # the same model through the full Gradmat/Rdmmat drop-in path
@@ -225,3 +226,49 @@ def test_device_objective_refuses_nci_above_one():
     assert rc == _lib.PHF_ERR_UNSUPPORTED
     e, gg = m.energy_and_gradient(case["z"])     # the model takes the host EigenSolver path there
     assert np.isfinite(e) and np.isfinite(gg).all()
+
+
+@pytest.mark.parametrize("cmplxconj,sp,is_rhf", [(0, 1, False), (1, 2, False), (1, 1, True)])
+def test_autograd_with_objective_masks(cmplxconj, sp, is_rhf):
+    """The Z masks of EandG (PHF.py:58-68,78-85) in the differentiable form:
+    CmplxConj = 0 (E ignores Im Z, so dE/dIm Z = 0), SP = 2 (SzProj: no
+    spin-flip blocks) and RHFProj.  Autograd equals central differences of
+    the masked objective along random directions of the full [Re Z, Im Z],
+    and equals the oracle's EandG with the same masks.  RHFProj copies the
+    alpha block of Z into the beta block (PHF.py:139-143) and of G likewise,
+    so there G is not the derivative of the masked objective (that would be
+    G_aa + G_bb): only the equality with EandG is checked."""
+    import torch
+    from phf_pcc_amd.model import ProjectedEnergy, ProjectedHF
+    from phf_pcc_amd.phf import ObjectiveConfig, make_eandg
+    from tests._cases import hubbard_general_ref
+    case = hubbard_general_ref(4, 4.0, 21, zscale=0.2)
+    ga = grid_args(4, (14, 8) if sp == 1 else (1, 8), 0, sp=sp)
+    m = ProjectedHF(case["nocc"], case["ref"], ga, hone=case["hone"], htwo=case["htwo"], cmplxconj=cmplxconj,
+                    is_rhf=is_rhf)
+    z = case["z"]
+    zr = torch.tensor(z.real, requires_grad=True)
+    zi = torch.tensor(z.imag, requires_grad=True)
+    e = ProjectedEnergy.apply(zr, zi, m)
+    e.backward()
+    grad = np.concatenate([zr.grad.numpy().ravel(), zi.grad.numpy().ravel()])
+    if cmplxconj == 0:
+        assert np.all(zi.grad.numpy() == 0)
+    cfg = ObjectiveConfig(nso=8, nocc=4, cmplxconj=cmplxconj, sp=sp, is_rhf=is_rhf, ncisp=ga["ncisp"])
+    bh = lambda zz: O.buildhsg(case["hone"], case["htwo"], case["ref"], zz, cmplxconj=cmplxconj, nocc=4, **ga)
+    f_ref = make_eandg(cfg, bh, lambda gm, rm, s, e0, fsp, fpg, fk, cc: O.localg(gm, rm, s, e0, fsp, fpg, fk, cc))
+    x0 = np.concatenate([z.real.ravel(), z.imag.ravel()])
+    e_ref, g_ref = f_ref(x0)
+    assert abs(float(e) - e_ref) < 1e-10
+    assert np.abs(grad - 2 * g_ref).max() < 1e-9
+    if is_rhf:
+        return
+    nv = z.size
+    f = lambda x: m.energy_and_gradient((x[:nv] + 1j * x[nv:]).reshape(z.shape))[0]
+    rng = np.random.default_rng(8)
+    h = 1e-5
+    for _ in range(6):
+        d = rng.normal(size=x0.shape)
+        d /= np.linalg.norm(d)
+        fd = (f(x0 + h * d) - f(x0 - h * d)) / (2 * h)
+        assert abs(fd - grad @ d) <= 1e-6 * (1 + abs(fd)), (fd, grad @ d)

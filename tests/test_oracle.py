@@ -138,3 +138,28 @@ def test_c_restatement_matches_numpy_oracle():
         b = cref.buildhsg(case["hone"], case["htwo"], case["ref"], case["z"], cmplxconj=1, nocc=4, **ga)
         for x, y in zip(a, b):
             assert np.abs(x - y).max() <= 1e-12 * np.abs(x).max()
+
+
+def test_factored_gamma_equals_dense_make_gamma():
+    """oracle.factored (MakeGamma from B factors, for the n = 96 / 128 sizes)
+    against the dense MakeGamma of the restatement on the Dirac integrals
+    built the reference's way (Main_PHF.py:22-33, makeH.py:63-67,
+    ao2mo.py:4-10), for random non-Hermitian rho and three Ref: a sorted
+    permutation, a random real orthogonal and a random complex unitary one."""
+    from oracle.factored import gamma_factored
+    from phf_pcc_amd.hamiltonian import (ao2mo, eri_from_factors, mulliken_to_dirac, random_unitary, sorted_ref,
+                                         spin_orbital_blocks, synthetic_factors)
+    n = 5
+    rng = np.random.default_rng(77)
+    h, b = synthetic_factors(n, 7, rng)
+    eri = eri_from_factors(b)
+    _, htwo_m = spin_orbital_blocks(h, eri)
+    vao = mulliken_to_dirac(htwo_m)
+    for ref in (sorted_ref(h, 4).astype(complex), random_unitary(2 * n, rng, real=True).astype(complex),
+                random_unitary(2 * n, rng)):
+        vmo = ao2mo(vao, ref, 4)
+        g = gamma_factored(b, ref)
+        for _ in range(3):
+            rho = rng.normal(size=(2 * n, 2 * n)) + 1j * rng.normal(size=(2 * n, 2 * n))
+            dense = O.make_gamma(vmo, rho)
+            assert np.abs(g(rho) - dense).max() < 1e-12 * max(1.0, np.abs(dense).max())
