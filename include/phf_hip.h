@@ -258,6 +258,20 @@ long phf_sd_dim(int nso, int nocc);
  * case minv[b] and det[b] are zero.  A singular matrix is not an error here
  * (the grid stages turn info != 0 into PHF_ERR_SINGULAR). */
 int phf_invert_batched(phf_ctx* ctx, int n, int batch, const double* m, double* minv, double* det, int* info);
+/* phf_invert_batched for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same arguments and results
+ * (bit for bit) as phf_invert_batched; it calls no hipMalloc/hipFree: the
+ * input, its column-major copy, the inverse, det and info are staged in a
+ * file-scope device array of 64 MiB per GPU (held until the process ends).
+ * Limit: 2 * 16 * n*n*batch + 20 * batch bytes (each of the four regions
+ * rounded up to 256) must fit 64 MiB, e.g. n = 32: 32,000 matrices, n = 96:
+ * 3,600; larger calls return PHF_ERR_UNSUPPORTED before any device work.
+ * Every call on one GPU shares that GPU's array: calls that overlap in time
+ * on one GPU (two contexts, two threads) race and must be serialised by the
+ * caller.  Each call synchronises the context's stream before it returns. */
+int phf_invert_batched_noalloc(phf_ctx* ctx, int n, int batch, const double* m, double* minv, double* det,
+                               int* info);
 /* buildsc_dr.buildsc_vec (BuildSC_Vec, BuildSC_dr.f90:8-1244) and any other
  * weighted sum over given points: replaces the grid of phf_sc_set_grid with
  * npts points of rotation r (complex[npts][nso][nso], NULL for identities),

@@ -35,7 +35,8 @@ EXPORTS = ["phf_ctx_create", "phf_ctx_destroy", "phf_last_error", "phf_set_hamil
            "phf_batch_last_error", "phf_batch_set_instance", "phf_batch_eval", "phf_batch_set_graph",
            "phf_batch_autotune", "phf_batch_stats", "phf_batch_tuning_report", "phf_set_blocked", "phf_blocked",
            "phf_buildsc", "phf_sc_set_grid", "phf_sc_apply", "phf_sc_apply_packed", "phf_sc_apply_packed_dev",
-           "phf_sd_dim", "phf_sc_set_points", "phf_eval_objective", "phf_invert_batched"]
+           "phf_sd_dim", "phf_sc_set_points", "phf_eval_objective", "phf_invert_batched",
+           "phf_invert_batched_noalloc"]
 
 
 class PhfError(RuntimeError):
@@ -111,6 +112,7 @@ def load() -> C.CDLL:
     lib.phf_sc_set_points.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp]
     lib.phf_eval_objective.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int, C.c_int, _dp, _dp]
     lib.phf_invert_batched.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, C.POINTER(C.c_int)]
+    lib.phf_invert_batched_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, C.POINTER(C.c_int)]
     for name in EXPORTS:
         if name not in ("phf_last_error", "phf_batch_last_error"):
             getattr(lib, name).restype = C.c_int
@@ -196,6 +198,20 @@ class Context:
         info = np.zeros(b, np.int32)
         check(load().phf_invert_batched(self._h, n, b, ptr(m), ptr(inv), ptr(det),
                                         info.ctypes.data_as(C.POINTER(C.c_int))), self._h)
+        return inv, det, info
+
+    def invert_batched_noalloc(self, m: np.ndarray):
+        """invert_batched through phf_invert_batched_noalloc (no device
+        allocation: staging in the library's 64 MiB scratch array)."""
+        m = np.ascontiguousarray(m, dtype=np.complex128)
+        if m.ndim != 3 or m.shape[1] != m.shape[2]:
+            raise ValueError("invert_batched_noalloc: m must be [batch][n][n]")
+        b, n = m.shape[0], m.shape[1]
+        inv = np.zeros_like(m)
+        det = np.zeros(b, np.complex128)
+        info = np.zeros(b, np.int32)
+        check(load().phf_invert_batched_noalloc(self._h, n, b, ptr(m), ptr(inv), ptr(det),
+                                                info.ctypes.data_as(C.POINTER(C.c_int))), self._h)
         return inv, det, info
 
     def hone(self) -> np.ndarray:

@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <string>
 #include <vector>
@@ -449,6 +450,30 @@ void fill_zero(hipStream_t s, double* p, long n) {
 
 // Device check and stream set-up of a new context: its own non-blocking
 // stream, or `shared_stream` (owned by the caller) when not null.
+// Staging of phf_invert_batched_noalloc: a file-scope device array, so that
+// entry point allocates nothing.  Each GPU the library runs on holds its own
+// copy for the life of the process.  In host code the array's name is only a
+// stand-in for the symbol: kernels and copies take the device address that
+// hipGetSymbolAddress resolves once per device (kept in g_scratch_ptr).
+static constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static constexpr int kMaxDevices = 64;
+static void* g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// With `device` current: resolve its copy of the scratch array (once).
+static int resolve_scratch(phf_ctx* ctx, int device) {
+  if (device < 0 || device >= kMaxDevices) return fail(ctx, PHF_ERR_UNSUPPORTED, "device index above 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return PHF_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(ctx, PHF_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = p;
+  return PHF_OK;
+}
+
 int phf_ctx_init(phf_ctx* ctx, int device, hipStream_t shared_stream) {
   int ndev = 0;
   hipError_t e = hipGetDeviceCount(&ndev);
@@ -457,6 +482,10 @@ int phf_ctx_init(phf_ctx* ctx, int device, hipStream_t shared_stream) {
   if (device < 0 || device >= ndev) return fail(ctx, PHF_ERR_ARG, "device index out of range");
   HIPCK(hipSetDevice(device));
   ctx->device = device;
+  {
+    const int rc = resolve_scratch(ctx, device);
+    if (rc) return rc;
+  }
   {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
@@ -1012,6 +1041,64 @@ extern "C" int phf_invert_batched(phf_ctx* ctx, int n, int batch, const double* 
         minv[2 * (b * nn + (size_t)i * n + j)] = h[2 * (b * nn + (size_t)j * n + i)];
         minv[2 * (b * nn + (size_t)i * n + j) + 1] = h[2 * (b * nn + (size_t)j * n + i) + 1];
       }
+  return PHF_OK;
+}
+
+// out[b][j][i] = in[b][i][j] for complex n x n matrices (row-major <-> column-major).
+__global__ void transpose_batched_kernel(const double2* __restrict__ in, double2* __restrict__ out, int n,
+                                         long total) {
+  const long nn = (long)n * n;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const long b = idx / nn, r = idx - b * nn;
+    const int i = (int)(r / n), j = (int)(r - (long)i * n);
+    out[b * nn + (long)j * n + i] = in[idx];
+  }
+}
+
+// phf_invert_batched without allocation: the row-major input, its
+// column-major copy, the inverse, det and info are staged in the scratch
+// array (largest call: 64 MiB of staging), transposed on the device, and the
+// same inverse kernels run.  Results equal phf_invert_batched's bit for bit.
+extern "C" int phf_invert_batched_noalloc(phf_ctx* ctx, int n, int batch, const double* m, double* minv,
+                                          double* det, int* info) {
+  if (!ctx) return PHF_ERR_ARG;
+  if (n <= 0 || batch <= 0 || !m || !minv || !det || !info)
+    return fail(ctx, PHF_ERR_ARG, "phf_invert_batched_noalloc: bad arguments");
+  if (n > lu_inverse_max_n())
+    return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_invert_batched_noalloc: n = " + std::to_string(n) + " exceeds " +
+                                              std::to_string(lu_inverse_max_n()));
+  const size_t tot = (size_t)n * n * batch;
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t mat_bytes = up(16 * tot), det_bytes = up(16 * (size_t)batch), info_bytes = up(4 * (size_t)batch);
+  const size_t need = 2 * mat_bytes + det_bytes + info_bytes;
+  if (need > kNoAllocScratchBytes)
+    return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_invert_batched_noalloc: " + std::to_string(batch) + " matrices of n = " +
+                                              std::to_string(n) + " need " + std::to_string(need) +
+                                              " bytes of staging, above the scratch limit of " +
+                                              std::to_string(kNoAllocScratchBytes) + " bytes");
+  HIPCK(hipSetDevice(ctx->device));
+  if (!g_scratch_ptr[ctx->device]) {
+    const int rc = resolve_scratch(ctx, ctx->device);
+    if (rc) return rc;
+  }
+  unsigned char* base = static_cast<unsigned char*>(g_scratch_ptr[ctx->device]);
+  double2* sA = reinterpret_cast<double2*>(base);                 // row-major input, then the inverse
+  double2* sB = reinterpret_cast<double2*>(base + mat_bytes);     // column-major input, then row-major inverse
+  double* sDet = reinterpret_cast<double*>(base + 2 * mat_bytes);
+  int* sInfo = reinterpret_cast<int*>(base + 2 * mat_bytes + det_bytes);
+  hipStream_t s = ctx->stream;
+  const int blocks = (int)std::min<long>(((long)tot + 255) / 256, 4096);
+  const long nn2 = 2L * n * n;
+  HIPCK(hipMemcpyAsync(sA, m, 16 * tot, hipMemcpyHostToDevice, s));
+  transpose_batched_kernel<<<blocks, 256, 0, s>>>(sA, sB, n, (long)tot);
+  lu_inverse_batched(s, n, reinterpret_cast<const double*>(sB), nn2, reinterpret_cast<double*>(sA), nn2, sDet,
+                     sInfo, batch);
+  transpose_batched_kernel<<<blocks, 256, 0, s>>>(sA, sB, n, (long)tot);
+  HIPCK(hipGetLastError());
+  HIPCK(hipMemcpyAsync(minv, sB, 16 * tot, hipMemcpyDeviceToHost, s));
+  HIPCK(hipMemcpyAsync(det, sDet, 16 * (size_t)batch, hipMemcpyDeviceToHost, s));
+  HIPCK(hipMemcpyAsync(info, sInfo, 4 * (size_t)batch, hipMemcpyDeviceToHost, s));
+  HIPCK(hipStreamSynchronize(s));
   return PHF_OK;
 }
 
