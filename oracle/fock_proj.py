@@ -143,15 +143,50 @@ def projected_energy(h1_ao, v_ao, c_ao, j: int = 0, m: int = 0, tol: float = 1e-
         num = np.vdot(phi, fs.apply_h(h1_ao, v_ao, pphi))
         den = np.vdot(phi, pphi)
         return float((num / den).real), float(den.real)
+    return _lowest_root(fs, h1_ao, v_ao, _projected_span(fs, [pphi], j, m), tol)
+
+
+def _projected_span(fs, pvecs, j, m):
+    """Columns L_{K->M} Pi_{Sz=K} v for K = -j..j and every spin-projected v."""
     sp, sm, _, _ = fs.spin_ops()
     szv = fs.sz_values()
     vecs = []
-    for k in range(-j, j + 1):
-        vk = np.where(np.abs(szv - k) < 1e-9, pphi, 0.0)
-        for _ in range(abs(m - k)):
-            vk = (sp @ vk) if m > k else (sm @ vk)
-        vecs.append(vk)
-    vmat = np.stack(vecs, 1)
+    for pv in pvecs:
+        for k in range(-j, j + 1):
+            vk = np.where(np.abs(szv - k) < 1e-9, pv, 0.0)
+            for _ in range(abs(m - k)):
+                vk = (sp @ vk) if m > k else (sm @ vk)
+            vecs.append(vk)
+    return np.stack(vecs, 1)
+
+
+def projected_energy_multi(h1_ao, v_ao, c_list, j: int = 0, m: int = 0, tol: float = 1e-8):
+    """Lowest energy on the span of the spin-projected determinants c_list
+    (AO orbital matrices): span{ L_{K->M} Pi_{Sz=K} P_J Phi_i }.  With
+    c_list = [C, conj(C)] this is the complex-conjugation-restored projection
+    (the space CmplxConj = 2 diagonalises over)."""
+    nso, nelec = c_list[0].shape
+    fs = FockSpace(nso, nelec)
+    pvecs = [fs.spin_projector_apply(float(j), fs.slater(c)) for c in c_list]
+    return _lowest_root(fs, h1_ao, v_ao, _projected_span(fs, pvecs, j, m), tol)
+
+
+def projected_matrices(h1_ao, v_ao, c_list, j: int = 0):
+    """<Phi_a| H P_J |Phi_b> and <Phi_a| P_J |Phi_b> for J = 0 (Loewdin P_J),
+    over the determinants of c_list: the matrices the grid kernel's J = 0
+    Hmat / Smat approximate up to one common normalisation."""
+    nso, nelec = c_list[0].shape
+    fs = FockSpace(nso, nelec)
+    phis = [fs.slater(c) for c in c_list]
+    pphis = [fs.spin_projector_apply(float(j), p) for p in phis]
+    hp = [fs.apply_h(h1_ao, v_ao, p) for p in pphis]
+    k = len(c_list)
+    hm = np.array([[np.vdot(phis[a], hp[b]) for b in range(k)] for a in range(k)])
+    sm = np.array([[np.vdot(phis[a], pphis[b]) for b in range(k)] for a in range(k)])
+    return hm, sm
+
+
+def _lowest_root(fs, h1_ao, v_ao, vmat, tol):
     hv = np.stack([fs.apply_h(h1_ao, v_ao, vmat[:, a]) for a in range(vmat.shape[1])], 1)
     hh = vmat.conj().T @ hv
     ss = vmat.conj().T @ vmat

@@ -28,6 +28,10 @@ reference repository):
 Pinning (tests/test_oracle_*.py): brute-force Fock-space spin projection on
 small lattices (energies for J=0 and J=1, exact once the grid is exact),
 central finite differences of E(Z) for the gradient (factor 2 on [Re, Im]),
+CmplxConj = 2 against the Fock-space matrices over {Phi, Phi*} at J = 0 and
+block by block as derivatives at J > 0 (where the reference's (1,1) copy,
+LocalG's exchange-block orientation and VecDecomp's product form are kept
+as the reference has them),
 the R=I mean-field limit against ``make_ene``, numpy.linalg det/inv for the
 LU, closed-form Wigner d for j<=1.  BuildSC / BuildSC_Vec / ao2mo4 are in
 ``buildsc_oracle.py``.

@@ -163,3 +163,140 @@ def test_factored_gamma_equals_dense_make_gamma():
             rho = rng.normal(size=(2 * n, 2 * n)) + 1j * rng.normal(size=(2 * n, 2 * n))
             dense = O.make_gamma(vmo, rho)
             assert np.abs(g(rho) - dense).max() < 1e-12 * max(1.0, np.abs(dense).max())
+
+
+# ----------------------------------------------------------------------------
+# CmplxConj = 2 (PHFTools.f90:95-119,153-179 blocks; LocalG :227-241)
+# ----------------------------------------------------------------------------
+
+def _cc2(nsite, j, ngrid, seed):
+    case = hubbard_general_ref(nsite, 4.0, seed, zscale=0.3, complex_ref=False)
+    ga = grid_args(nsite, ngrid, j)
+    ga["nci"] *= 2
+    return case, ga
+
+
+@pytest.mark.parametrize("nsite,ngrid,seed", [(4, (14, 8), 3), (6, (14, 8), 5)])
+def test_cmplxconj2_j0_matches_fock_space(nsite, ngrid, seed):
+    """J = 0, real Ref, complex Z, exact grid: every element of the 2 x 2
+    Hmat / Smat equals sum(weightsp) * <Phi_a| H P |Phi_b> / <Phi_a| P |Phi_b>
+    over {Phi, Phi*} (Phi* = the determinant of conj(C0), brute-force Fock
+    space); E0 is the lowest root on span{P Phi, P Phi*}; the LocalG gradient
+    (cross terms included) equals central differences of E (factor 2)."""
+    from oracle.fock_proj import projected_energy_multi, projected_matrices
+    case, ga = _cc2(nsite, 0, ngrid, seed)
+    nocc = case["nocc"]
+    h, s, gm, rm = O.buildhsg(case["hone"], case["htwo"], case["ref"], case["z"], cmplxconj=2, nocc=nocc, **ga)
+    c0 = np.vstack([np.eye(nocc), case["z"]])
+    cl = [case["ref"] @ c0, case["ref"] @ c0.conj()]
+    hm, sm = projected_matrices(case["hso"], case["vao"], cl, 0)
+    c = ga["weightsp"].sum()
+    assert abs(c - 2 * np.pi) < 1e-12
+    assert np.abs(h - c * hm).max() < 1e-11 * np.abs(h).max()
+    assert np.abs(s - c * sm).max() < 1e-11 * np.abs(s).max()
+    ev, _ = O.eigen_solver(h, s)
+    assert abs(ev[0] - projected_energy_multi(case["hso"], case["vao"], cl, 0)[0]) < 1e-11
+    x0 = np.concatenate([case["z"].real.ravel(), case["z"].imag.ravel()])
+    f = lambda x: O.eandg(x, case["hone"], case["htwo"], case["ref"], nocc, ga, cmplxconj=2)
+    e, g = f(x0)
+    rng = np.random.default_rng(seed)
+    for _ in range(4):
+        d = rng.normal(size=x0.shape)
+        d /= np.linalg.norm(d)
+        fd = (f(x0 + 1e-5 * d)[0] - f(x0 - 1e-5 * d)[0]) / 2e-5
+        assert abs(fd - 2 * g @ d) < 1e-8 * (1 + abs(fd))
+
+
+def _wirtinger(fun, z0, dz, h=1e-5):
+    """sum dF/dZ* conj(dz) of an array-valued F(Z) by central differences:
+    (F'(dz) + i F'(i dz)) / 2."""
+    d1 = (fun(z0 + h * dz) - fun(z0 - h * dz)) / (2 * h)
+    d2 = (fun(z0 + 1j * h * dz) - fun(z0 - 1j * h * dz)) / (2 * h)
+    return 0.5 * (d1 + 1j * d2)
+
+
+def test_cmplxconj2_gradmat_blocks_are_the_derivatives_of_hmat():
+    """J = 1, CmplxConj = 2: Gradmat / Rdmmat hold the Z* derivatives of the
+    Hmat / Smat blocks.  Measured block by block with d = 2J+1:
+      d H(m,n)/dZ*       = Gradmat(m,n)                      (bra)
+      d H(m+d,n+d)/dZ*   = conj Gradmat(m+d,n+d)             (ket of Phi*)
+      d H(m,n+d)/dZ*     = Gradmat(m,n+d) + conj Gradmat(n+d,m)
+      d H(m+d,n)/dZ*     = 0
+    and the same with Smat / Rdmmat.  The third line carries the CI indices
+    (n+d, m); LocalG (PHFTools.f90:236-237) uses conj Gradmat(m+d, n)."""
+    case, ga = _cc2(4, 1, (26, 8), 3)
+    nocc, d = case["nocc"], ga["nci"] // 2
+    run = lambda z: O.buildhsg(case["hone"], case["htwo"], case["ref"], z, cmplxconj=2, nocc=nocc, **ga)
+    h, s, gm, rm = run(case["z"])
+    rng = np.random.default_rng(9)
+    dz = rng.normal(size=case["z"].shape) + 1j * rng.normal(size=case["z"].shape)
+    proj = lambda m: np.sum(m[..., nocc:, :nocc] * np.conj(dz), axis=(-2, -1))
+    for idx, dmat in ((0, gm), (1, rm)):
+        b = _wirtinger(lambda z: run(z)[idx], case["z"], dz)
+        scale = np.abs(b).max()
+        g = proj(dmat)
+        gc = proj(np.conj(dmat))
+        assert np.abs(b[:d, :d] - g[:d, :d]).max() < 1e-7 * scale
+        assert np.abs(b[d:, d:] - gc[d:, d:]).max() < 1e-7 * scale
+        assert np.abs(b[:d, d:] - g[:d, d:] - gc[d:, :d].T).max() < 1e-7 * scale
+        assert np.abs(b[d:, :d]).max() < 1e-7 * scale
+        # LocalG's orientation differs wherever the exchange-block derivative is not symmetric
+        assert np.abs(b[:d, d:] - g[:d, d:] - gc[d:, :d]).max() > 1e-3 * scale
+
+
+def test_cmplxconj2_j1_reference_departures_from_the_projection():
+    """J = 1, CmplxConj = 2, exact grid: what the restatement keeps from the
+    reference, with the exact quantities beside it.
+    (1) Hmat/Smat block (1,1) is a copy of (0,0) (PHFTools.f90:177-178); the
+        projection on span{P_MK Phi, P_MK Phi*} has conj((0,0)) there (plain
+        complex conjugation commutes with P for a real Hamiltonian): with that
+        block, E0 equals the brute-force root; the copy gives a lower E0.
+    (2) The gradient of the reference's own E (its Hmat, its EigenSolver) is
+        f^H (dH - E dS) f / f^H S f with the block derivatives of the test
+        above and the whole CI vector f: it matches central differences,
+        while LocalG's (transposed exchange-block term, and f rebuilt as
+        fsp x fk by VecDecomp, PHFTools.py:20-26) does not."""
+    from oracle.fock_proj import projected_energy_multi
+    case, ga = _cc2(4, 1, (26, 8), 3)
+    nocc, d = case["nocc"], ga["nci"] // 2
+    run = lambda z: O.buildhsg(case["hone"], case["htwo"], case["ref"], z, cmplxconj=2, nocc=nocc, **ga)
+    h, s, gm, rm = run(case["z"])
+    assert np.array_equal(h[d:, d:], h[:d, :d]) and np.array_equal(s[d:, d:], s[:d, :d])
+    c0 = np.vstack([np.eye(nocc), case["z"]])
+    e_exact = projected_energy_multi(case["hso"], case["vao"], [case["ref"] @ c0, case["ref"] @ c0.conj()], 1)[0]
+    h2, s2 = h.copy(), s.copy()
+    h2[d:, d:], s2[d:, d:] = h[:d, :d].conj(), s[:d, :d].conj()
+    assert abs(O.eigen_solver(h2, s2)[0][0] - e_exact) < 1e-10
+    e_ref = O.eigen_solver(h, s)[0][0]
+    assert e_ref < e_exact - 1e-2
+
+    def energy_and_exact_gradient(z):
+        hh, ss, g_, r_ = run(z)
+        ev, vec = O.eigen_solver(hh, ss)
+        v = vec[:, 0]
+        dg, dr = np.zeros_like(g_), np.zeros_like(r_)
+        dg[:d, :d], dr[:d, :d] = g_[:d, :d], r_[:d, :d]
+        dg[d:, d:], dr[d:, d:] = np.conj(g_[d:, d:]), np.conj(r_[d:, d:])
+        dg[:d, d:] = g_[:d, d:] + np.conj(g_[d:, :d].transpose(1, 0, 2, 3))
+        dr[:d, d:] = r_[:d, d:] + np.conj(r_[d:, :d].transpose(1, 0, 2, 3))
+        num = np.einsum('m,n,mnpq->pq', v.conj(), v, dg - ev[0] * dr)
+        grad = (num / (v.conj() @ ss @ v))[nocc:, :nocc]
+        return ev[0].real, grad, (g_, r_, ss, ev, vec)
+
+    e, g_exact, (g_, r_, ss, ev, vec) = energy_and_exact_gradient(case["z"])
+    fsp, fpg, fk = O.vec_decomp(vec[:, 0], ga["ncisp"], 1, 2)
+    g_localg = O.localg(g_, r_, ss, ev[0], fsp, fpg, fk, 2)[nocc:, :nocc]
+    x0 = np.concatenate([case["z"].real.ravel(), case["z"].imag.ravel()])
+    nv = case["z"].size
+    f = lambda x: energy_and_exact_gradient((x[:nv] + 1j * x[nv:]).reshape(case["z"].shape))[0]
+    rng = np.random.default_rng(4)
+    worst_exact = worst_localg = 0.0
+    for _ in range(4):
+        dd = rng.normal(size=x0.shape)
+        dd /= np.linalg.norm(dd)
+        fd = (f(x0 + 1e-5 * dd) - f(x0 - 1e-5 * dd)) / 2e-5
+        dz = (dd[:nv] + 1j * dd[nv:]).reshape(case["z"].shape)
+        worst_exact = max(worst_exact, abs(fd - 2 * np.real(np.sum(np.conj(g_exact) * dz))))
+        worst_localg = max(worst_localg, abs(fd - 2 * np.real(np.sum(np.conj(g_localg) * dz))))
+    assert worst_exact < 1e-7
+    assert worst_localg > 1e-2
