@@ -181,8 +181,10 @@ int phf_localg(phf_ctx* ctx, const double* gradmat, const double* rdmmat, const 
  * inverse + density pack + reduced-precision split, [2] ERI GEMM, [3]
  * epilogue + grid sums, [4] rotation chain, [5] overlap + LU inverse + X_g,
  * [6] density pack, [7] reduced-precision split (when the pack writes the
- * 16-bit/int8 slices itself, NSO <= 128, [6] covers both and [7] is ~0);
- * it returns how many it wrote.
+ * 16-bit/int8 slices itself, NSO <= 128, [6] covers both and [7] is ~0),
+ * [8], [9], [10] the ERI GEMM launch of each spin block (D, X1, X2; [9]
+ * holds X1 + X2 when the exchange blocks run as one grouped launch, and
+ * [8] the whole GEMM for a dense A); it returns how many it wrote.
  * After a profiled phf_sc_apply / phf_sc_apply_packed(_dev): [0] all device
  * work, [1] stage-1 pair-transform GEMM, [2] stage-2 GEMM with the grid sum,
  * [3] P GEMM, [4] rank-term GEMM, [5] per-point t0/t1 kernel, [6] the rest

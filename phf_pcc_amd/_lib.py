@@ -239,8 +239,8 @@ class Context:
         return bool(load().phf_blocked(self._h))
 
     def stage_times(self) -> np.ndarray:
-        out = np.zeros(8)
-        n = load().phf_stage_times(self._h, ptr(out), 8)
+        out = np.zeros(11)
+        n = load().phf_stage_times(self._h, ptr(out), 11)
         return out[:n].copy()
 
     # ------------------------------------------------------------------

@@ -546,6 +546,9 @@ extern "C" int phf_ctx_destroy(phf_ctx* ctx) {
     if (q) (void)hipFree(q);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& pr : ctx->bev)
+    for (auto& e : pr)
+      if (e) (void)hipEventDestroy(e);
   for (int k = 0; k < 2; ++k) {
     if (ctx->side[k]) {
       (void)hipStreamSynchronize(ctx->side[k]);
@@ -1716,7 +1719,19 @@ int eval_gemm(phf_ctx* ctx, const EvalBufs& b) {
   // n = 64), joined back before the epilogue.  The reduced forms run them one
   // after another: concurrent blocks there compete for L2 over different
   // operand ranges (measured slower: F16X2 1.34 -> 1.49 ms).
-  auto all_blocks = [&](auto&& run_q) -> int {
+  ctx->nblk_timed = 0;
+  auto timed = [&](auto&& run_q) {
+    return [&, run_q](int q, hipStream_t sq) {
+      if (ctx->profiling && ctx->bev[q][0]) (void)hipEventRecord(ctx->bev[q][0], sq);
+      run_q(q, sq);
+      if (ctx->profiling && ctx->bev[q][0]) {
+        (void)hipEventRecord(ctx->bev[q][1], sq);
+        ctx->nblk_timed = std::max(ctx->nblk_timed, q + 1);
+      }
+    };
+  };
+  auto all_blocks = [&](auto&& run_q0) -> int {
+    auto run_q = timed(run_q0);
     if (nb == 3 && ctx->side[0] && precision == PHF_PREC_FP64) {
       HIPCK(hipEventRecord(ctx->fork_ev, s));
       for (int k = 0; k < 2; ++k) HIPCK(hipStreamWaitEvent(ctx->side[k], ctx->fork_ev, 0));
@@ -1842,9 +1857,15 @@ int eval_gemm(phf_ctx* ctx, const EvalBufs& b) {
     // blocks write disjoint columns of Gamma: one zero fill before all of them
     if (any_split) fill_zero(s, b.Gam, b.Mp * nrpad);
     if (grouped) {
-      run_block(0, choice[0].first, choice[0].second, s);
-      eri_gemm_mixed_grouped(s, precision, ctx->d_xitems, (int)ctx->h_xitems.size(), choice[1].first,
-                             h16_grouped_grid(choice[1].first, ctx->ncu), choice[1].second > 1);
+      auto run_q = timed([&](int q, hipStream_t sq) {
+        if (q == 0)
+          run_block(0, choice[0].first, choice[0].second, sq);
+        else
+          eri_gemm_mixed_grouped(sq, precision, ctx->d_xitems, (int)ctx->h_xitems.size(), choice[1].first,
+                                 h16_grouped_grid(choice[1].first, ctx->ncu), choice[1].second > 1);
+      });
+      run_q(0, s);
+      run_q(1, s);
       return PHF_OK;
     }
     return all_blocks([&](int q, hipStream_t sq) { run_block(q, choice[q].first, choice[q].second, sq); });
@@ -2014,6 +2035,12 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
     ctx->stage_ms[6] = t56;   // density pack rho_g^T (FP64 planes, or the fused pack + split)
     ctx->stage_ms[7] = t61;   // reduced-precision split / conversion of the densities
     ctx->nstage = 8;
+    for (int q = 0; q < ctx->nblk_timed; ++q) {   // [8..10] each spin block's GEMM launch
+      float tq;
+      HIPCK(hipEventElapsedTime(&tq, ctx->bev[q][0], ctx->bev[q][1]));
+      ctx->stage_ms[8 + q] = tq;
+      ctx->nstage = 9 + q;
+    }
   }
   for (long gv = 0; gv < Gv; ++gv)
     if (info[gv] != 0) {
@@ -2157,8 +2184,11 @@ extern "C" int phf_localg_vo(phf_ctx* ctx, const double* smat, const double* e0,
 extern "C" int phf_set_profiling(phf_ctx* ctx, int on) {
   if (!ctx) return PHF_ERR_ARG;
   HIPCK(hipSetDevice(ctx->device));
-  if (on && !ctx->ev[0])
+  if (on && !ctx->ev[0]) {
     for (auto& e : ctx->ev) HIPCK(hipEventCreate(&e));
+    for (auto& pr : ctx->bev)
+      for (auto& e : pr) HIPCK(hipEventCreate(&e));
+  }
   ctx->profiling = on != 0;
   ctx->nstage = 0;
   return PHF_OK;

@@ -142,8 +142,14 @@ struct phf_ctx_s {
   // per-stage timing with HIP events on `stream` (phf_set_profiling)
   bool profiling = false;
   hipEvent_t ev[8] = {};
-  double stage_ms[8] = {};
+  double stage_ms[11] = {};
   int nstage = 0;
+  // per spin block of the ERI GEMM (D, X1, X2; [q][0] before, [q][1] after
+  // its launch on the stream it runs on) when profiling; nblk_timed blocks
+  // were recorded by the last evaluation (X1 holds the grouped X1 + X2
+  // launch when the exchange blocks run as one)
+  hipEvent_t bev[3][2] = {};
+  int nblk_timed = 0;
 };
 
 inline long round_up(long x, long m) { return (x + m - 1) / m * m; }
