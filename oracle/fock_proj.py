@@ -195,3 +195,56 @@ def _lowest_root(fs, h1_ao, v_ao, vmat, tol):
     x = u[:, keep] / np.sqrt(ev[keep])
     e = np.linalg.eigvalsh(x.conj().T @ hh @ x)
     return float(e[0]), float(ev.max())
+
+
+# ----------------------------------------------------------------------------
+# helpers for the projected-CC checks (any orthonormal spin-orbital basis)
+# ----------------------------------------------------------------------------
+
+def one_body(fs: FockSpace, mat: np.ndarray):
+    """sum_pq mat[p, q] a+_p a_q as a sparse matrix."""
+    out = None
+    for p in range(fs.nso):
+        for q in range(fs.nso):
+            if mat[p, q] != 0:
+                term = fs.epq(p, q) * mat[p, q]
+                out = term if out is None else out + term
+    return out.tocsr() if out is not None else sps.csr_matrix((fs.dim, fs.dim))
+
+
+def rotation_operator(fs: FockSpace, r: np.ndarray) -> np.ndarray:
+    """The N-electron action of the orbital map a+_p -> sum_q r[q, p] a+_q:
+    |det(C)> -> |det(r C)>, so <D'| R |D> = det(r[D', D])."""
+    occ = [[i for i in range(fs.nso) if (m >> i) & 1] for m in fs.dets]
+    out = np.empty((fs.dim, fs.dim), dtype=complex)
+    for a, rows in enumerate(occ):
+        for b, cols in enumerate(occ):
+            out[a, b] = np.linalg.det(r[np.ix_(rows, cols)])
+    return out
+
+
+def cc_state(fs: FockSpace, t1: np.ndarray, t2: np.ndarray, nocc: int) -> np.ndarray:
+    """e^{T1 + T2} |Phi> with |Phi> the determinant of orbitals 0..nocc-1,
+    T1 = sum t1[a,i] a+_{nocc+a} a_i, T2 = 1/4 sum t2[a,b,i,j] a+_{nocc+a} a+_{nocc+b} a_j a_i
+    (the Taylor series terminates: T^k = 0 beyond k = nocc)."""
+    nv = fs.nso - nocc
+    t = None
+    for a in range(nv):
+        for i in range(nocc):
+            if t1[a, i] != 0:
+                term = fs.epq(nocc + a, i) * t1[a, i]
+                t = term if t is None else t + term
+    for a in range(nv):
+        for b in range(nv):
+            for i in range(nocc):
+                for j in range(nocc):
+                    if t2[a, b, i, j] != 0 and a != b and i != j:
+                        term = (fs.epq(nocc + a, i) @ fs.epq(nocc + b, j)) * (0.25 * t2[a, b, i, j])
+                        t = term if t is None else t + term
+    phi = np.zeros(fs.dim, dtype=complex)
+    phi[fs.index(np.array([(1 << nocc) - 1]))[0]] = 1.0
+    out, term = phi.copy(), phi.copy()
+    for k in range(1, nocc + 1):
+        term = (t @ term) / k
+        out += term
+    return out

@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libphf_hip.so")
 SOURCES = ["phf_api.hip", "phf_batch.hip", "zgemm.hip", "lu.hip", "eri_gemm.hip", "eri_gemm_mixed.hip",
-           "buildsc.hip"]
+           "buildsc.hip", "pcc.hip"]
 ARCH = os.environ.get("PHF_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -438,6 +438,11 @@ __global__ void gather_cols_kernel(const double* __restrict__ src, long lds, con
 
 }  // namespace
 
+// Wignerfac for the other host files of the library (the PCC grid weights).
+std::complex<double> phf_wignerfac(int j, int n, int m, double alpha, double beta, double gama) {
+  return wignerfac(j, n, m, alpha, beta, gama);
+}
+
 void fill_zero(hipStream_t s, double* p, long n) {
   if (n <= 0) return;
   long blocks = std::min<long>((n + 255) / 256, 65536);
@@ -540,6 +545,7 @@ extern "C" int phf_ctx_destroy(phf_ctx* ctx) {
   phf_free_sc(ctx);
   if (ctx->d_xitems) (void)hipFree(ctx->d_xitems);
   if (ctx->ws) (void)hipFree(ctx->ws);
+  if (ctx->pcc_ws) (void)hipFree(ctx->pcc_ws);
   if (ctx->small) (void)hipFree(ctx->small);
   if (ctx->pin) (void)hipHostFree(ctx->pin);
   for (double* q : {ctx->dGr1p, ctx->dGr2, ctx->dGrpgref, ctx->dGw, ctx->dGu, ctx->dGref, ctx->dGP})

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <complex>
 #include <map>
 #include <string>
 #include <tuple>
@@ -150,6 +151,9 @@ struct phf_ctx_s {
   // launch when the exchange blocks run as one)
   hipEvent_t bev[3][2] = {};
   int nblk_timed = 0;
+  // workspace of phf_pcc_olap0 (the projected-CC grid consumer)
+  char* pcc_ws = nullptr;
+  size_t pcc_ws_size = 0;
 };
 
 inline long round_up(long x, long m) { return (x + m - 1) / m * m; }
@@ -180,6 +184,8 @@ void phf_free_sc(phf_ctx* ctx);   // the projected SD-space overlap grid and wor
   } while (0)
 
 void fill_zero(hipStream_t s, double* p, long n);
+// Wigner D^j_{nm}(alpha, beta, gamma) by the factorial sum (PHFTools.f90:350-376)
+std::complex<double> phf_wignerfac(int j, int n, int m, double alpha, double beta, double gama);
 int phf_ctx_init(phf_ctx* ctx, int device, hipStream_t shared_stream);
 int phf_release_ws(phf_ctx* ctx);
 int phf_keep_only_mixed(phf_ctx* ctx, int prec);

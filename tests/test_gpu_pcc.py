@@ -1,0 +1,82 @@
+"""Projected-CC grid consumer, zeroth order (phf_pcc_olap0 through the C ABI)
+against the oracle's restatement of PbarHbarOlap / EvalKernel
+(oracle/pcc_oracle.py, pinned in brute-force Fock space): Olap0 and EOlap0
+to 1e-10 relative.
+
+Cases: the 4-site Hubbard ring (NSO = 8, 4 electrons) on the exact 14 x 8
+spin grid, J = 0 and J = 1 with complex fsp; a point-group pair with two
+irreps and complex fpg; SP = 2; the conjugate second pass (ncik = 2);
+a 6-site ring (NSO = 12, 6 electrons) with larger amplitudes; a 10-site
+ring (NSO = 20) on a coarse grid; a singular occupied block refused."""
+import numpy as np
+import pytest
+
+from oracle import pcc_oracle as P
+from phf_pcc_amd.hamiltonian import random_unitary
+from tests._cases import grid_args, hubbard_general_ref
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(nsite, seed, s1, s2, j=0, ngrid=(14, 8), sp=1, npg=1, ncik=1):
+    case = hubbard_general_ref(nsite, 4.0, seed, zscale=0.1)
+    nso, nocc = 2 * nsite, case["nocc"]
+    rng = np.random.default_rng(seed)
+    nv = nso - nocc
+    t1 = s1 * (rng.normal(size=(nv, nocc)) + 1j * rng.normal(size=(nv, nocc)))
+    t2 = s2 * (rng.normal(size=(nv, nv, nocc, nocc)) + 1j * rng.normal(size=(nv, nv, nocc, nocc)))
+    t2 = P.asymm2(t2) / 4
+    ga = grid_args(nsite, ngrid, j, sp=sp)
+    ref = case["ref"]
+    tomo = lambda r: np.einsum('pa,gpq,qb->gab', ref.conj(), r, ref)
+    if npg == 2:
+        perm = np.eye(nsite)[np.roll(np.arange(nsite), nsite // 2)]
+        rpg = tomo(np.stack([np.eye(nso), np.kron(np.eye(2), perm)]).astype(complex))
+        wpg = np.stack([np.diag([0.5, 0.5]), np.diag([0.5, -0.5])])
+        fpg = np.array([1.0, 0.3 - 0.2j])
+    else:
+        rpg = np.eye(nso, dtype=complex)[None]
+        wpg = np.ones((1, 1, 1))
+        fpg = np.ones(1, complex)
+    fsp = np.ones(2 * j + 1, complex) if j == 0 else np.array([0.4 + 0.1j, 1.0, -0.3 + 0.5j])
+    rk = np.stack([np.eye(nso), random_unitary(nso, rng)]) if ncik == 2 else None
+    fk = np.array([1.0, 0.6 - 0.3j]) if ncik == 2 else np.ones(1, complex)
+    args = (case["hone"], case["htwo"], t1, t2, nocc, 0.37, j, sp, ga["ngrid"], tomo(ga["r1"]), tomo(ga["r2"]), rpg, rk,
+            ga["roota"], ga["rootb"], ga["rooty"], ga["weightsp"], wpg, fsp, fpg, fk)
+    return args
+
+
+@pytest.mark.parametrize("kw", [
+    dict(nsite=4, seed=1, s1=0.2, s2=0.2),
+    dict(nsite=4, seed=2, s1=0.1, s2=0.3, j=1, ngrid=(26, 8)),
+    dict(nsite=4, seed=3, s1=0.2, s2=0.1, npg=2),
+    dict(nsite=4, seed=4, s1=0.2, s2=0.2, sp=2, ngrid=(1, 8)),
+    dict(nsite=4, seed=5, s1=0.15, s2=0.15, ncik=2, ngrid=(6, 4)),
+    dict(nsite=6, seed=6, s1=0.1, s2=0.4, ngrid=(6, 4)),
+    dict(nsite=10, seed=7, s1=0.05, s2=0.05, ngrid=(6, 4)),
+], ids=["ring4", "ring4-J1", "ring4-pg2", "ring4-sp2", "ring4-ncik2", "ring6", "ring10"])
+def test_olap0_eolap0_match_oracle(kw):
+    from phf_pcc_amd.pcc import pbar_hbar_olap0
+    args = _case(**kw)
+    o_gpu, e_gpu = pbar_hbar_olap0(*args)
+    o_ref, e_ref = P.pbar_hbar_olap0(*args)
+    print(f"[pcc] {kw}: Olap0 {o_gpu:.12f}  EOlap0 {e_gpu:.12f}", flush=True)
+    assert abs(o_gpu - o_ref) < 1e-10 * abs(o_ref)
+    assert abs(e_gpu - e_ref) < 1e-10 * abs(e_ref)
+
+
+def test_singular_occupied_block_is_an_error():
+    """T1 = 0 and a rotation that maps an occupied orbital onto a virtual one:
+    the occupied block L of R e^{T1} is singular (InvertC STOPs there)."""
+    from phf_pcc_amd import _lib
+    from phf_pcc_amd.pcc import pbar_hbar_olap0
+    args = list(_case(nsite=4, seed=8, s1=0.0, s2=0.1, ngrid=(1, 1)))
+    swap = np.eye(8, dtype=complex)
+    swap[[0, 4]] = swap[[4, 0]]
+    args[9] = swap[None]
+    args[10] = np.eye(8, dtype=complex)[None]
+    args[8] = np.array([1, 1])
+    args[13], args[14], args[15] = np.zeros(1), np.zeros(1), np.zeros(1)
+    args[16] = np.ones((1, 1))
+    with pytest.raises(_lib.SingularOverlapError):
+        pbar_hbar_olap0(*args)
