@@ -548,8 +548,8 @@ struct ScProf {
   template <class F>
   void run(int cat, F&& f) {
     hipEvent_t a, b;
-    (void)hipEventCreate(&a);
-    (void)hipEventCreate(&b);
+    (void)hipEventCreateWithFlags(&a, kTimingEventFlags);
+    (void)hipEventCreateWithFlags(&b, kTimingEventFlags);
     (void)hipEventRecord(a, s);
     f();
     (void)hipEventRecord(b, s);
@@ -654,8 +654,8 @@ std::unique_ptr<ScProf> sc_prof_begin(phf_ctx* ctx) {
   if (!ctx->profiling) return nullptr;
   std::unique_ptr<ScProf> p(new ScProf);
   p->s = ctx->stream;
-  (void)hipEventCreate(&p->all0);
-  (void)hipEventCreate(&p->all1);
+  (void)hipEventCreateWithFlags(&p->all0, kTimingEventFlags);
+  (void)hipEventCreateWithFlags(&p->all1, kTimingEventFlags);
   (void)hipEventRecord(p->all0, ctx->stream);
   return p;
 }

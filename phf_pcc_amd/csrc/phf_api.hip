@@ -504,11 +504,11 @@ int phf_ctx_init(phf_ctx* ctx, int device, hipStream_t shared_stream) {
     ctx->owns_stream = true;
     for (int k = 0; k < 2; ++k) {
       HIPCK(hipStreamCreateWithFlags(&ctx->side[k], hipStreamNonBlocking));
-      HIPCK(hipEventCreateWithFlags(&ctx->join_ev[k], hipEventDisableTiming));
+      HIPCK(hipEventCreateWithFlags(&ctx->join_ev[k], kSyncEventFlags));
     }
-    HIPCK(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
-    HIPCK(hipEventCreateWithFlags(&ctx->hc_fork, hipEventDisableTiming));
-    HIPCK(hipEventCreateWithFlags(&ctx->hc_join, hipEventDisableTiming));
+    HIPCK(hipEventCreateWithFlags(&ctx->fork_ev, kSyncEventFlags));
+    HIPCK(hipEventCreateWithFlags(&ctx->hc_fork, kSyncEventFlags));
+    HIPCK(hipEventCreateWithFlags(&ctx->hc_join, kSyncEventFlags));
   }
   return PHF_OK;
 }
@@ -2191,9 +2191,9 @@ extern "C" int phf_set_profiling(phf_ctx* ctx, int on) {
   if (!ctx) return PHF_ERR_ARG;
   HIPCK(hipSetDevice(ctx->device));
   if (on && !ctx->ev[0]) {
-    for (auto& e : ctx->ev) HIPCK(hipEventCreate(&e));
+    for (auto& e : ctx->ev) HIPCK(hipEventCreateWithFlags(&e, kTimingEventFlags));
     for (auto& pr : ctx->bev)
-      for (auto& e : pr) HIPCK(hipEventCreate(&e));
+      for (auto& e : pr) HIPCK(hipEventCreateWithFlags(&e, kTimingEventFlags));
   }
   ctx->profiling = on != 0;
   ctx->nstage = 0;

@@ -310,10 +310,10 @@ extern "C" int phf_batch_create(int device, int ninst, phf_batch** out) {
   bt->inst.assign(ninst, nullptr);
   bt->nocc.assign(ninst, 0);
   bool ok = true;
-  for (auto& e : bt->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
-  for (auto& e : bt->fork_ev) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+  for (auto& e : bt->ev) ok = ok && hipEventCreateWithFlags(&e, kTimingEventFlags) == hipSuccess;
+  for (auto& e : bt->fork_ev) ok = ok && hipEventCreateWithFlags(&e, kSyncEventFlags) == hipSuccess;
   for (auto& row : bt->join_ev)
-    for (auto& e : row) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+    for (auto& e : row) ok = ok && hipEventCreateWithFlags(&e, kSyncEventFlags) == hipSuccess;
   for (auto& st : bt->side) ok = ok && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
   if (!ok) {
     (void)phf_batch_destroy(bt);

@@ -158,6 +158,14 @@ struct phf_ctx_s {
 
 inline long round_up(long x, long m) { return (x + m - 1) / m * m; }
 
+// Events of the library are read only after a synchronization of their stream
+// (timing) or waited on by another stream of the same device (fork/join): a
+// device-scope release suffices.  The default system-scope fence writes back
+// and invalidates the caches at every record, which showed as ~10 us gaps
+// between kernels around each recorded event.
+constexpr unsigned kTimingEventFlags = hipEventDisableSystemFence;
+constexpr unsigned kSyncEventFlags = hipEventDisableTiming | hipEventDisableSystemFence;
+
 // Bump allocator over a device block (base == nullptr measures).
 struct Arena {
   char* base;
