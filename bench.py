@@ -155,7 +155,7 @@ def batch_forms(forms, steps, warmup, ninst=64, baselines=True):
     nprod = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3, "i8x4": 10}[bprec]
     flops_best = flops_blk if best["structure"] else flops_dense
     t_gemm = best["eager_gemm_ms"] * 1e-3
-    grouped = nprod > 1
+    grouped = bprec != "fp64"
     roof = {"bound": "mfma", "achieved": flops_best / t_gemm / 1e12, "peak": PEAK[bprec] / 1e12,
             "unit": "TFLOP/s", "frac": flops_best / t_gemm / PEAK[bprec],
             "frac_executed": nprod * flops_best / t_gemm / PEAK[bprec], "traffic": None,
@@ -176,6 +176,16 @@ def batch_forms(forms, steps, warmup, ninst=64, baselines=True):
                      "speedup_vs_fp64_same_blocks": r["value"] / res["fp64"]["value"],
                      "failing_instances": int((r["status"] != 0).sum()), "tuning": r["tuning"] or None}
                  for f, r in res.items()}
+    for f, r in res.items():   # the grouped launch of every reduced form against its own MFMA peak
+        if f != "fp64" and not f.startswith("fp64") and r["eager_gemm_ms"] > 0:
+            fl = flops_blk if r["structure"] else flops_dense
+            np_ = {"fp32": 1, "bf16x3": 6, "f16x2": 3, "i8x4": 10}[f]
+            forms_out[f]["grouped_gemm"] = {"kernel": "eri_gemm_h16_grouped_kernel<%s>" % _tname(f),
+                                            "ms_per_launch": r["eager_gemm_ms"],
+                                            "achieved_tflops": fl / (r["eager_gemm_ms"] * 1e-3) / 1e12,
+                                            "peak_tflops": PEAK[f] / 1e12,
+                                            "frac": fl / (r["eager_gemm_ms"] * 1e-3) / PEAK[f],
+                                            "frac_executed": np_ * fl / (r["eager_gemm_ms"] * 1e-3) / PEAK[f]}
     return {"value": best["value"], "ms_per_step": best["ms_step"], "prec": bprec, "structure": best["structure"],
             "grid_points": G, "instances": len(inst),
             "workload": f"batched: {len(inst)} synthetic instances, n in {{16..96}}, N=n/2, L=3n, grids "
@@ -206,6 +216,11 @@ def run_batch(args):
                                           "note", "forms")},
            "roofline": rec["roofline"], "cpu_baseline": None}
     print(json.dumps(out), flush=True)
+
+
+def _tname(prec):
+    """Element type of the grouped tile kernel's instantiation for a form."""
+    return {"fp32": "f32", "bf16x3": "bf16", "f16x2": "f16", "i8x4": "i8"}[prec]
 
 
 def _form_dtype(prec):
@@ -495,9 +510,9 @@ def gemm_kernel_name(prec, report):
     """Kernel of the ERI GEMM of form `prec` on its largest block, from the
     tuner's choice: configuration 4 is the 256 x 256 int8 tile, 5 the 256 x 256
     round-to-nearest 16-bit tile, the others the 128-row fold-tile kernels."""
-    names = {"fp64": "eri_gemm_f64_kernel", "fp32": "eri_gemm_f32_kernel", "bf16x3": "eri_gemm_h16_kernel<bf16>",
+    names = {"fp64": "eri_gemm_f64_kernel", "fp32": "eri_gemm_h16_kernel<f32>", "bf16x3": "eri_gemm_h16_kernel<bf16>",
              "f16x2": "eri_gemm_h16_kernel<f16>", "i8x4": "eri_gemm_h16_kernel<i8>"}
-    code = {"bf16x3": 2, "f16x2": 3, "i8x4": 4}.get(prec)
+    code = {"fp32": 1, "bf16x3": 2, "f16x2": 3, "i8x4": 4}.get(prec)
     best_cols, cfg = -1, None
     for line in (report or "").splitlines():
         try:

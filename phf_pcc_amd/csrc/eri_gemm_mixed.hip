@@ -28,8 +28,21 @@ typedef int i4_t __attribute__((ext_vector_type(4)));
 namespace {
 
 // One 16-byte fragment per lane and one MFMA per 64-byte k-step: 16x16x32 for
-// the 16-bit types (FP32 accumulation), 16x16x64 for int8 (exact int32).
+// the 16-bit types (FP32 accumulation), 16x16x64 for int8 (exact int32); for
+// FP32 the fragment is four k per lane and feeds four 16x16x4 MFMAs (lane l
+// holds k = 4 (l >> 4) + s of the 16-wide k-step for the s-th MFMA, the same
+// k for both operands, so the four products cover the step once).
 template <typename T> struct MfmaH;
+template <> struct MfmaH<float> {
+  typedef f4_t v8;
+  typedef f4_t acc_t;
+  __device__ static acc_t mma(v8 a, v8 b, acc_t c) {
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], b[0], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[1], b[1], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[2], b[2], c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a[3], b[3], c, 0, 0, 0);
+  }
+};
 template <> struct MfmaH<_Float16> {
   typedef h8_t v8;
   typedef f4_t acc_t;
@@ -52,7 +65,8 @@ typedef __attribute__((address_space(1))) void glb_void_t;
 // s_waitcnt with only vmcnt <= v (expcnt / lgkmcnt left at their maxima), gfx9 encoding
 #define PHF_WAIT_VMCNT(v) __builtin_amdgcn_s_waitcnt(((v) & 0xF) | ((((v) >> 4) & 3) << 14) | (0x7 << 4) | (0xF << 8))
 
-// 16-bit inputs, 16x16x32 MFMA.  Workgroup tile BM x BN (rows of P x rows of
+// 16-bit inputs, 16x16x32 MFMA (FP32 inputs: four 16x16x4 MFMAs per 16-byte
+// fragment, one slice, no row scales).  Workgroup tile BM x BN (rows of P x rows of
 // A), waves of WM x WN arranged (BM/WM) x (BN/WN).  The operands hold nslice
 // slices per row ([x0 | x1 | ...], each K long); the contraction runs over a
 // sequence of segments, segment sg pairing P slice (patP >> 3 sg) & 7 with A
@@ -205,7 +219,8 @@ __device__ __forceinline__ void h16_tile(const T* __restrict__ P, long ldP, long
         const long ar = (long)nt * BN + wn * WN + j * 16 + (lane & 15);
         const long col = colmap ? colmap[ar] : ar;
         if (col < 0) continue;
-        double v = (double)facc[i][j][r] * invP[row] * invA[ar];
+        double v = (double)facc[i][j][r];
+        if (invP) v *= invP[row] * invA[ar];   // row scales of the split forms (none for FP32)
         if (bias) v += bias[(row & 1) * ldc + col];
         if (atomic)
           atomicAdd(&C[row * ldc + col], v);
@@ -545,104 +560,6 @@ void eri_gemm_h16_grouped_kernel(const H16Item* __restrict__ items, int nitems, 
                                      it.C, it.bias, it.colmap, it.ldc, it.K, patP, patA, patW, it.mt, it.nt,
                                      it.sbeg, it.send, ATOMIC, fold, smem);
   }
-}
-
-// FP32 inputs, v_mfma_f32_16x16x4f32, 128 x 128 tile, k-slab 32, FP64 fold every FLUSH slabs.
-template <int FLUSH>
-__global__ __launch_bounds__(256, 1) void eri_gemm_f32_kernel(const float* __restrict__ P, long ldp,
-                                                              const float* __restrict__ A, double* __restrict__ C,
-                                                              int K, long ldc, int tiles_m,
-                                                              const double* __restrict__ bias,
-                                                              const int* __restrict__ colmap) {
-  constexpr int BM = 128, BN = 128, BK = 32, LDF = BK + 4;
-  constexpr int WM = 64, WN = 64, FM = 4, FN = 4;
-  __shared__ __attribute__((aligned(16))) float sP[2][BM * LDF];
-  __shared__ __attribute__((aligned(16))) float sA[2][BN * LDF];
-  const int t = xcd_remap(blockIdx.x, gridDim.x);
-  const int mt = t % tiles_m, nt = t / tiles_m;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  const float* Pg = P + (long)mt * BM * ldp;
-  const float* Ag = A + (long)nt * BN * K;
-  const int lr = tid >> 3, lc = (tid & 7) * 4;   // 8 threads x 16 B per 32-element row
-
-  f4_t acc[FM][FN];
-  d4_t facc[FM][FN];
-#pragma unroll
-  for (int i = 0; i < FM; ++i)
-#pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
-      facc[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
-    }
-  float4 rp0, rp1, rp2, rp3, ra0, ra1, ra2, ra3;
-#define F_LD(dst, base, ld, e, k0) \
-  dst = *reinterpret_cast<const float4*>((base) + (long)(lr + 32 * (e)) * (ld) + (k0) + lc)
-#define F_ST(arr, buf, e, src) *reinterpret_cast<float4*>(&arr[buf][(lr + 32 * (e)) * LDF + lc]) = src
-#define F_GLOAD(k0)                                                                          \
-  do {                                                                                       \
-    F_LD(rp0, Pg, ldp, 0, k0); F_LD(rp1, Pg, ldp, 1, k0); F_LD(rp2, Pg, ldp, 2, k0);           \
-    F_LD(rp3, Pg, ldp, 3, k0); F_LD(ra0, Ag, K, 0, k0); F_LD(ra1, Ag, K, 1, k0);               \
-    F_LD(ra2, Ag, K, 2, k0); F_LD(ra3, Ag, K, 3, k0);                                         \
-  } while (0)
-#define F_LSTORE(buf)                                                                        \
-  do {                                                                                       \
-    F_ST(sP, buf, 0, rp0); F_ST(sP, buf, 1, rp1); F_ST(sP, buf, 2, rp2); F_ST(sP, buf, 3, rp3); \
-    F_ST(sA, buf, 0, ra0); F_ST(sA, buf, 1, ra1); F_ST(sA, buf, 2, ra2); F_ST(sA, buf, 3, ra3); \
-  } while (0)
-  F_GLOAD(0);
-  F_LSTORE(0);
-  __syncthreads();
-  const int nk = K / BK;
-  for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    if (kt + 1 < nk) F_GLOAD((kt + 1) * BK);
-#pragma unroll
-    for (int ks = 0; ks < BK; ks += 4) {
-      float a[FM], b[FN];
-#pragma unroll
-      for (int i = 0; i < FM; ++i) a[i] = sP[cur][(wm * WM + i * 16 + (lane & 15)) * LDF + ks + (lane >> 4)];
-#pragma unroll
-      for (int j = 0; j < FN; ++j) b[j] = sA[cur][(wn * WN + j * 16 + (lane & 15)) * LDF + ks + (lane >> 4)];
-#pragma unroll
-      for (int i = 0; i < FM; ++i)
-#pragma unroll
-        for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
-    }
-    if constexpr (FLUSH > 0) {
-      if ((kt + 1) % FLUSH == 0 || kt + 1 == nk) {
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-          for (int j = 0; j < FN; ++j) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) facc[i][j][r] += (double)acc[i][j][r];
-            acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
-          }
-      }
-    }
-    if (kt + 1 < nk) F_LSTORE(cur ^ 1);
-    __syncthreads();
-  }
-#undef F_LD
-#undef F_ST
-#undef F_GLOAD
-#undef F_LSTORE
-#pragma unroll
-  for (int i = 0; i < FM; ++i)
-#pragma unroll
-    for (int j = 0; j < FN; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const long row = (long)mt * BM + wm * WM + i * 16 + 4 * (lane >> 4) + r;
-        long col = (long)nt * BN + wn * WN + j * 16 + (lane & 15);
-        if (colmap) {
-          col = colmap[col];
-          if (col < 0) continue;
-        }
-        C[row * ldc + col] = ((FLUSH > 0) ? facc[i][j][r] : (double)acc[i][j][r]) +
-                             (bias ? bias[(row & 1) * ldc + col] : 0.0);
-      }
 }
 
 // ---------------------------------------------------------------------------
@@ -1007,7 +924,10 @@ static void mixed_patterns(int prec, unsigned* patP, unsigned* patA, unsigned* p
   static const int p_i8[10] = {0, 0, 1, 0, 1, 2, 0, 1, 2, 3}, a_i8[10] = {0, 1, 0, 2, 1, 0, 3, 2, 1, 0};
   static const int w_i8[10] = {0, 1, 1, 2, 2, 2, 3, 3, 3, 3};
   *patW = 0;
-  if (prec == 3) {
+  if (prec == 1) {   // one segment: slice 0 against slice 0
+    *patP = 0;
+    *patA = 0;
+  } else if (prec == 3) {
     *patP = pack_pattern(p_f16, 3);
     *patA = pack_pattern(a_f16, 3);
   } else if (prec == 2) {
@@ -1138,19 +1058,14 @@ double h16_model_cost(int c, int sk, long Mp0, int Nr0, int ncu) {
 }
 
 // Mp multiple of the config's BM, Nr multiple of its BN; K (the contraction
-// length) a multiple of 64 * splitk (16-bit) or 32 (FP32).  P rows hold their
+// length) a multiple of one 128-byte slab times splitk (64 16-bit, 32 FP32 or
+// 128 int8 elements).  P rows hold their
 // slices kp elements apart (kp >= K; the contraction reads the first K of
 // each), A rows are [nslice][K].  With splitk = 2 the caller zero-fills C
 // first.  colmap: optional A row -> C column map (-1: not written).
 void eri_gemm_mixed(hipStream_t s, int prec, const void* P, long kp, const void* A, const double* invP,
                     const double* invA, double* C, int Mp, int Nr, int K, long ldc, int cfg, int splitk,
                     const double* bias, const int* colmap) {
-  if (prec == 1) {
-    const int tm = Mp / 128, tn = Nr / 128;
-    eri_gemm_f32_kernel<16><<<tm * tn, 256, 0, s>>>((const float*)P, kp, (const float*)A, C, K, ldc, tm, bias,
-                                                    colmap);
-    return;
-  }
   unsigned patP, patA, patW;
   mixed_patterns(prec, &patP, &patA, &patW);
   const int nslice = mixed_slices(prec), nseg = mixed_segments(prec);
@@ -1213,7 +1128,9 @@ void eri_gemm_mixed(hipStream_t s, int prec, const void* P, long kp, const void*
       default: H_LAUNCH(TT, 128, 256, 64, 128, 3, 1, false, SK); break;  \
     }                                                                           \
   } while (0)
-  if (prec == 2) {
+  if (prec == 1) {
+    if (splitk == 2) H_CFGS(float, 2); else H_CFGS(float, 1);
+  } else if (prec == 2) {
     if (splitk == 2) H_CFGS(__bf16, 2); else H_CFGS(__bf16, 1);
   } else if (prec == 3) {
     if (splitk == 2) H_CFGS(_Float16, 2); else H_CFGS(_Float16, 1);
@@ -1296,7 +1213,9 @@ void eri_gemm_mixed_grouped(hipStream_t s, int prec, const H16Item* items, int n
       default: G_LAUNCH(TT, 128, 256, 64, 128, 3, 1, false); break;     \
     }                                                                          \
   } while (0)
-  if (prec == 2)
+  if (prec == 1)
+    G_CFGS(float);
+  else if (prec == 2)
     G_CFGS(__bf16);
   else if (prec == 3)
     G_CFGS(_Float16);

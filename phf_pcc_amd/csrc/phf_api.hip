@@ -1667,7 +1667,8 @@ void build_h16_items(const std::vector<std::pair<const phf_ctx*, const EvalBufs*
     const long kbeg = blk ? c->blk_kbeg[u.blk] : 0, aoff = blk ? c->blk_aoff[u.blk] : 0;
     const long roff = blk ? c->blk_roff[u.blk] : 0;
     const char* Abase = (const char*)(blk ? c->dAbm[prec] : c->dAm[prec]);
-    const double* invA = blk ? c->dAbinv[prec] : c->dAinv[prec];
+    const bool f32 = prec == PHF_PREC_FP32;   // no row scales
+    const double* invA = f32 ? nullptr : (blk ? c->dAbinv[prec] : c->dAinv[prec]) + roff;
     const int tm = (int)(round_up(2 * b.V * b.G, bm) / bm), tn = rows / bn;
     const int part = h16_slabs(prec, cfg, K) / sk;
     for (int nt = 0; nt < tn; ++nt)
@@ -1677,8 +1678,8 @@ void build_h16_items(const std::vector<std::pair<const phf_ctx*, const EvalBufs*
           std::memset(&it, 0, sizeof it);   // padding too: work lists are compared bytewise
           it.P = (const char*)b.Pm + eb * kbeg;
           it.A = Abase + eb * ns * aoff;
-          it.invP = b.Pinv;
-          it.invA = invA + roff;
+          it.invP = f32 ? nullptr : b.Pinv;
+          it.invA = invA;
           it.C = b.Gam;
           it.bias = ks == 0 ? c->dHb : nullptr;
           it.colmap = blk ? c->d_rowmap + roff : nullptr;
@@ -1760,23 +1761,20 @@ int eval_gemm(phf_ctx* ctx, const EvalBufs& b) {
       eri_gemm_f64(sq, P + kbeg_of(q), b.kP, A + aoff_of(q), b.Gam, (int)b.Mp, rows_of(q), k_of(q), nrpad, b.bm,
                    ctx->dHb, map_of(q));
     });
-  } else if (precision == PHF_PREC_FP32) {
-    const float* P = (const float*)b.Pm;
-    const float* A = (const float*)(b.blk ? ctx->dAbm[1] : ctx->dAm[1]);
-    return all_blocks([&](int q, hipStream_t sq) {
-      eri_gemm_mixed(sq, precision, P + kbeg_of(q), b.kP, A + aoff_of(q), nullptr, nullptr, b.Gam, (int)b.Mp,
-                     rows_of(q), k_of(q), nrpad, 128, 1, ctx->dHb, map_of(q));
-    });
   } else {
+    // FP32 and the split forms share the tile routine; FP32 has one slice and no row scales
+    const bool f32 = precision == PHF_PREC_FP32;
     const int ns = mixed_slices(precision);
     const long eb = mixed_elem_bytes(precision);
     const char* P = (const char*)b.Pm;
     const char* A = (const char*)(b.blk ? ctx->dAbm[precision] : ctx->dAm[precision]);
-    const double* invA = b.blk ? ctx->dAbinv[precision] : ctx->dAinv[precision];
+    const double* invA = f32 ? nullptr : (b.blk ? ctx->dAbinv[precision] : ctx->dAinv[precision]);
+    const double* invP = f32 ? nullptr : b.Pinv;
     auto run_block = [&](int q, int cfg, int sk, hipStream_t sq) {
       const int mp = (int)round_up(2 * G, h16_cfg_bm(cfg));
-      eri_gemm_mixed(sq, precision, P + eb * kbeg_of(q), b.kP, A + eb * ns * aoff_of(q), b.Pinv, invA + roff_of(q),
-                     b.Gam, mp, rows_of(q), k_of(q), nrpad, cfg, sk, ctx->dHb, map_of(q));
+      eri_gemm_mixed(sq, precision, P + eb * kbeg_of(q), b.kP, A + eb * ns * aoff_of(q), invP,
+                     invA ? invA + roff_of(q) : nullptr, b.Gam, mp, rows_of(q), k_of(q), nrpad, cfg, sk, ctx->dHb,
+                     map_of(q));
     };
     std::pair<int, int> choice[3];
     bool any_split = false;

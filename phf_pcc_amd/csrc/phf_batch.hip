@@ -5,8 +5,9 @@
 // on that stream:
 //   for every instance  C0 upload, K1/K2, density pack and split     (eval_pre)
 //   one ERI GEMM launch over all instances: the 16-bit forms go through a
-//       persistent grouped kernel over a work list of (instance, row tile,
-//       column tile, contraction part) items; FP64/FP32 launch per instance
+//       reduced forms (FP32 included) go through a persistent grouped kernel
+//       over a work list of (instance, row tile, column tile, contraction
+//       part) items; FP64 launches per instance
 //   for every instance  K4/K5 grid sums (eval_post) and the J = 0 finish:
 //       E0 = Re H / |Re S| (EigenSolver of the 1 x 1 problem, PHFTools.py:8-18)
 //       and the (NSO-N) x N gradient block (Ysum - E0 Xsum)[N:, :] / S
@@ -259,7 +260,7 @@ int enqueue_eval(phf_batch* bt, int prec, bool timed) {
     BHIPCK(hipMemcpyAsync(b.C0, bt->h_c0 + bt->off_c0[i], sizeof(double) * 2 * (size_t)c->nso * b.N,
                           hipMemcpyHostToDevice, c->stream));
     eval_pre(c, b);
-    if (prec == PHF_PREC_FP64 || prec == PHF_PREC_FP32) {
+    if (prec == PHF_PREC_FP64) {
       int rc = eval_gemm(c, b);   // per-instance GEMMs overlap on the side streams
       if (rc) {
         for (phf_ctx* x : bt->inst) x->stream = bt->stream;
@@ -269,7 +270,7 @@ int enqueue_eval(phf_batch* bt, int prec, bool timed) {
   }
   join(bt, 0);
   if (timed) BHIPCK(hipEventRecord(bt->ev[1], s));
-  if (prec != PHF_PREC_FP64 && prec != PHF_PREC_FP32) launch_grouped(bt, prec);
+  if (prec != PHF_PREC_FP64) launch_grouped(bt, prec);
   if (timed) BHIPCK(hipEventRecord(bt->ev[2], s));
   fork(bt, 1);
   for (size_t r = 0; r < ni; ++r) {
@@ -482,7 +483,7 @@ extern "C" int phf_batch_eval(phf_batch* bt, const double* z, int cmplxconj, int
     zoff += 2 * (size_t)(nso - N) * N;
   }
   hipStream_t s = bt->stream;
-  const bool h16 = precision == PHF_PREC_BF16X3 || precision == PHF_PREC_F16X2 || precision == PHF_PREC_I8X4;
+  const bool h16 = precision != PHF_PREC_FP64;   // every reduced form runs one grouped launch
   if (fresh || !bt->exec) {
     if (fresh && h16) {
       // the grouped tiles are tuned on live operands: densities first

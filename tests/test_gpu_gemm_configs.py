@@ -2,8 +2,8 @@
 time (PHF_H16_CFG=<index>,<splitk>), against the FP64 contraction.
 
 Configurations 0-3 walk the product segments one after another (every 16-bit
-form and I8X4; 2 and 3 fold into FP32 and are candidates only when pinned or
-allowed); 4 is the 256 x 256 exact int8 tile (I8X4 only; one exact int32 sum
+form, I8X4, and FP32 with its one segment of four 16x16x4 MFMAs per fragment;
+2 and 3 fold into FP32 and are candidates only when pinned or allowed); 4 is the 256 x 256 exact int8 tile (I8X4 only; one exact int32 sum
 per product class, added to Gamma in FP64); 5 is the 256 x 256 16-bit tile
 with round-to-nearest accumulation of two-MFMA partials (no FP64 fold tile).  Checks, on a spin blocked and a
 dense instance and on the grouped batch launch: |dE| <= 1e-6 Eh and max |dG|
@@ -79,9 +79,20 @@ def test_other_forms_config(prob, fp64, prec, cfg):
     assert np.abs(g - g64).max() <= 1e-5
 
 
+@pytest.mark.parametrize("structure", [True, False])
+@pytest.mark.parametrize("sk", [1, 2])
+@pytest.mark.parametrize("cfg", SEGMENT_CFGS)
+def test_fp32_config(prob, fp64, cfg, sk, structure):
+    e64, g64 = fp64[structure]
+    e, g = _eval(prob, "fp32", structure, cfg, sk)
+    assert abs(e - e64) <= 1e-6, (cfg, sk, e - e64)
+    assert np.abs(g - g64).max() <= 1e-5
+
+
 def test_configs_refuse_other_forms(prob):
     from phf_pcc_amd import _lib
-    for prec, cfg in [("f16x2", I8_TILE), ("bf16x3", I8_TILE), ("i8x4", RN16_TILE)]:
+    for prec, cfg in [("f16x2", I8_TILE), ("bf16x3", I8_TILE), ("i8x4", RN16_TILE), ("fp32", I8_TILE),
+                      ("fp32", RN16_TILE)]:
         with pytest.raises(_lib.PhfError):
             _eval(prob, prec, True, cfg, 1)
 
@@ -110,7 +121,7 @@ def test_int8_tile(prob, fp64, sk, structure):
 
 @pytest.mark.parametrize("prec,cfg,sk", [("f16x2", 0, 1), ("f16x2", 0, 2), ("f16x2", 1, 1), ("bf16x3", 0, 1),
                                          ("i8x4", I8_TILE, 1), ("i8x4", I8_TILE, 2), ("f16x2", RN16_TILE, 1),
-                                         ("f16x2", RN16_TILE, 2)])
+                                         ("f16x2", RN16_TILE, 2), ("fp32", 0, 1), ("fp32", 0, 2), ("fp32", 1, 1)])
 def test_grouped_batch_config(prec, cfg, sk):
     from phf_pcc_amd.batch import ProjectedHFBatch
     inst, zs = [], []
