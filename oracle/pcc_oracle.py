@@ -17,17 +17,28 @@ statement by statement (paths relative to the reference repository):
   int_tran4_sim    SUCCSD_Boson_cmplx/IntTrans.f90:486-572
   wigner           SUCCSD_Boson_cmplx/Wrappers.f90:769-797 (the factorial sum of
                    phf_oracle.wignerfac)
+  cc_itms          SUCCSD_Boson_cmplx/CCRes.f90:355-536   (CCItms)
+  cc_res12         SUCCSD_Boson_cmplx/CCRes.f90:164-353   (CCRes12)
+  proj_sz1/2       SUCCSD_Boson_cmplx/FPUCC_Tools.f90:171-233
+  build_kern_sd    SUCCSD_Boson_cmplx/FPUCC_Tools.f90:291-436 (TrunODE 1 and 2)
+  eval_kernel      SUCCSD_Boson_cmplx/FPUCC.f90:529-602   (every output)
+  pbar_hbar_olap   SUCCSD_Boson_cmplx/FPUCC.f90:14-272    (every output)
 
 Arrays are 0-based: t1[a, i] is T1(NOcc+1+a, 1+i), t2[a, b, i, j] is
 T2(NOcc+1+a, NOcc+1+b, 1+i, 1+j); U and HTwo are NSO^4 with <pq||rs> at
-[p, q, r, s].  Only what Olap0 and EOlap0 need is restated: the singles and
-doubles kernels (CCRes12, the Ovl1/Ovl2 fills, ProjSz) are not.
+[p, q, r, s].
 
 Pinning (tests/test_pcc_oracle.py): with T2^3 |Phi> = 0 (at most five
 electrons or five holes) the truncation of TransT2 at second order is exact,
 so lambda0 = <Phi| R e^{T1+T2} |Phi> and eps0 = <Phi| H R e^{T1+T2} |Phi>
 for any orbital rotation R, checked in brute-force Fock space; the grid sum
-Olap0 / EOlap0 then equals the exact spin projection of the CC state.
+Olap0 / EOlap0 then equals the exact spin projection of the CC state.  The
+singles and doubles outputs are pinned the same way: CCRes12 against
+<Phi_i^a| e^{-T} H e^T |Phi> and <Phi_ij^ab| e^{-T} H e^T |Phi> for random
+amplitudes, and lambda1/2, eps1/2 of EvalKernel against <Phi_i^a| R e^T |Phi>,
+<Phi_ij^ab| R e^T |Phi> and the same with H (|Phi_i^a> = a+_a a_i |Phi>,
+|Phi_ij^ab> = a+_a a_i a+_b a_j |Phi>); ProjSz against the S_z rule of the
+reference's own commented-out form (FPUCC_Tools.f90:123-148).
 """
 from __future__ import annotations
 
@@ -217,3 +228,197 @@ def pbar_hbar_olap0(hone, htwo, t1, t2, nocc, enuc, qj, sp, npoints, r1, r2, rpg
                     eolap0 += w * eps0
                     olap0 += w * lam0
     return olap0, eolap0
+
+
+# ---------------------------------------------------------------------------
+# singles and doubles: CCRes12 / CCItms, the BuildKernSD fills, ProjSz
+# ---------------------------------------------------------------------------
+
+def _aperm1(m):
+    """APerm(M, 1) (CCRes.f90:120-156): M(I,J,:,:) - M(J,I,:,:)."""
+    return m - m.transpose(1, 0, 2, 3)
+
+
+def _aperm2(m):
+    """APerm(M, 2): M(:,:,I,J) - M(:,:,J,I)."""
+    return m - m.transpose(0, 1, 3, 2)
+
+
+def cc_itms(fock, h2, t1, t2, nocc):
+    """CCItms (CCRes.f90:355-536): tau, Fvv, Foo, Fov, Woooo, Wvvvv, Wovvo.
+    t1 [a, i], t2 [a, b, i, j]; every intermediate in the reference's index
+    order (Fov [i, a], Woooo [m, n, i, j], Wovvo [m, b, e, j])."""
+    nso = fock.shape[0]
+    o, v = slice(0, nocc), slice(nocc, nso)
+    hoovv, hooov, hoovo = h2[o, o, v, v], h2[o, o, o, v], h2[o, o, v, o]
+    hvovv, hovvv = h2[v, o, v, v], h2[o, v, v, v]
+    itm1 = np.einsum('ai,bj->abij', t1, t1)
+    itm4 = t2 / 2 + itm1                                            # before the APerms
+    itm1 = _aperm2(_aperm1(itm1))
+    tau = t2 + itm1 / 2
+    tb = t2 + itm1 / 4
+    # Fvv: Transpose2(tb/2) then At_dot_B over the last index (:407-424)
+    fvv = fock[v, v] - t1 @ fock[o, v] / 2
+    fvv = fvv + np.einsum('fm,ambf->ab', t1, hvovv)
+    fvv = fvv - 0.5 * np.einsum('ebmn,mnfb->ef', tb, hoovv)
+    # Foo (:426-441)
+    foo = fock[o, o] + fock[o, v] @ t1 / 2
+    foo = foo + np.einsum('em,imje->ij', t1, hooov)
+    foo = foo + 0.5 * np.einsum('ineb,ebjn->ij', hoovv, tb)
+    # Fov (:443-452)
+    fov = fock[o, v] + np.einsum('em,imae->ia', t1, hoovv)
+    # Woooo (:453-462)
+    woooo = h2[o, o, o, o] + np.einsum('mnef,efij->mnij', hoovv, tau / 4)
+    woooo = woooo + _aperm2(np.einsum('mnie,ej->mnij', hooov, t1))
+    # Wvvvv (:463-485)
+    wvvvv = h2[v, v, v, v] + np.einsum('efmn,mncb->efcb', tau / 4, hoovv)
+    wvvvv = wvvvv - _aperm1(np.einsum('amcd,bm->abcd', hvovv, t1))
+    # Wovvo (:487-519)
+    wovvo = h2[o, v, v, o] + np.einsum('mxcf,fj->mxcj', hovvv, t1)
+    wovvo = wovvo - np.einsum('mncj,bn->mbcj', hoovo, t1)
+    wovvo = wovvo - np.einsum('xbjn,incx->ibcj', itm4, hoovv)
+    return tau, fvv, foo, fov, woooo, wvvvv, wovvo
+
+
+def cc_res12(fock, h2, t1, t2, nocc):
+    """CCRes12 (CCRes.f90:164-353): the CCSD residuals res1 [a, i] and
+    res2 [a, b, i, j] of (t1, t2) with Fock and <pq||rs>.  r2tmp is kept in
+    the reference's (i, j, a, b) order and transposed at the end."""
+    nso = fock.shape[0]
+    o, v = slice(0, nocc), slice(nocc, nso)
+    tau, fvv, foo, fov, woooo, wvvvv, wovvo = cc_itms(fock, h2, t1, t2, nocc)
+    t2tmp = t2.transpose(2, 3, 0, 1)                                 # (i, j, a, b)
+    hvoov, hooov, hvovv = h2[v, o, o, v], h2[o, o, o, v], h2[v, o, v, v]
+    # res1 (:215-236)
+    r1 = fock[v, o] + fvv @ t1 - t1 @ foo
+    r1 = r1 + np.einsum('me,imae->ai', fov, t2tmp)
+    r1 = r1 + np.einsum('em,amie->ai', t1, hvoov)
+    r1 = r1 - 0.5 * np.einsum('mnab,mnib->ai', t2tmp, hooov)
+    r1 = r1 + 0.5 * np.einsum('ameb,imeb->ai', hvovv, t2tmp)
+    # res2 (:238-352), r2tmp (i, j, a, b)
+    r2tmp = np.zeros_like(t2tmp)
+    itm1 = fvv - t1 @ fov / 2
+    r2tmp = r2tmp + _aperm2(np.einsum('ijae,be->ijab', t2tmp, itm1))
+    itm3 = foo + fov @ t1 / 2
+    r2tmp = r2tmp - _aperm1(np.einsum('imab,mj->ijab', t2tmp, itm3))
+    hvoov2 = h2[o, v, v, o].transpose(2, 3, 0, 1)                     # Transpose2: (e, j, m, b) = H(m, b, e, j)
+    itm4 = np.einsum('ei,ejmb->ijmb', t1, hvoov2)
+    itm2 = np.einsum('ijmb,am->ijab', itm4, -t1)
+    itm2 = itm2 + np.einsum('imae,mbej->ijab', t2tmp, wovvo)
+    r2tmp = r2tmp + _aperm2(_aperm1(itm2))
+    wt = wvvvv.transpose(2, 3, 0, 1)                                  # Transpose1: (e, f, a, b) = W(a, b, e, f)
+    r2tmp = r2tmp + np.einsum('efij,efab->ijab', tau / 2, wt)
+    r2tmp = r2tmp + np.einsum('mnij,mnab->ijab', woooo, tau.transpose(2, 3, 0, 1) / 2)
+    hvovv2 = h2[v, v, v, o].transpose(2, 3, 0, 1)                     # (e, j, a, b) = H(a, b, e, j)
+    hooov2 = h2[o, v, o, o].transpose(2, 3, 0, 1)                     # (i, j, m, b) = H(m, b, i, j)
+    r2tmp = r2tmp + _aperm1(np.einsum('ei,ejab->ijab', t1, hvovv2))
+    r2tmp = r2tmp - _aperm2(np.einsum('ijmb,am->ijab', hooov2, t1))
+    res2 = r2tmp.transpose(2, 3, 0, 1) + h2[v, v, o, o]
+    return r1, res2
+
+
+def _spin_blocks(nocc, nso):
+    """Occupied alpha / beta and virtual alpha / beta ranges in the order the
+    ProjSz routines assume (NOA = NOcc/2 alpha occupied first; NVA = NAO - NOA)."""
+    noa = nocc // 2
+    nva = nso // 2 - noa
+    return noa, nva
+
+
+def proj_sz1(r1, nocc, nso):
+    """ProjSz1 on the vo block (FPUCC_Tools.f90:171-180): r1 [a, i] with the
+    spin-flip blocks (alpha virtual, beta occupied) and (beta virtual, alpha
+    occupied) zeroed."""
+    noa, nva = _spin_blocks(nocc, nso)
+    out = r1.copy()
+    out[:nva, noa:] = 0
+    out[nva:, :noa] = 0
+    return out
+
+
+def proj_sz2(r2, nocc, nso):
+    """ProjSz2VO (FPUCC_Tools.f90:217-233): keep the aaaa, bbbb, abab, abba,
+    baba and baab blocks of r2 [a, b, i, j] (virtual spins a, b; occupied
+    spins i, j), zero the rest."""
+    noa, nva = _spin_blocks(nocc, nso)
+    out = np.zeros_like(r2)
+    A, B = slice(0, nva), slice(nva, None)
+    a, b = slice(0, noa), slice(noa, None)
+    for va, vb, oi, oj in ((A, A, a, a), (B, B, b, b), (A, B, a, b), (A, B, b, a), (B, A, b, a), (B, A, a, b)):
+        out[va, vb, oi, oj] = r2[va, vb, oi, oj]
+    return out
+
+
+def build_kern_sd(fock, h2, escf, u1t, u2t, nocc, sp):
+    """BuildKernSD (FPUCC_Tools.f90:291-436) for TrunODE 1 and 2, the blocks
+    EvalKernel keeps: Ovl1[vo], Ovl2[vvoo], EOvl0, EOvl1[vo], EOvl2[vvoo]."""
+    nso = fock.shape[0]
+    ovl1 = u1t.copy()
+    if sp == 2:
+        ovl1 = proj_sz1(ovl1, nocc, nso)
+    ovl2 = u2t + np.einsum('ai,bj->abij', u1t, u1t) - np.einsum('bi,aj->abij', u1t, u1t)
+    if sp == 2:
+        ovl2 = proj_sz2(ovl2, nocc, nso)
+    eovl0 = cc_energy(u1t, u2t, fock, h2, nocc) + escf
+    ex1, ex2 = cc_res12(fock, h2, u1t, u2t, nocc)
+    eovl1 = ex1 + u1t * eovl0
+    if sp == 2:
+        eovl1 = proj_sz1(eovl1, nocc, nso)
+    eovl2 = (ex2 + ovl2 * eovl0
+             + np.einsum('ai,bj->abij', u1t, ex1) - np.einsum('bi,aj->abij', u1t, ex1)
+             + np.einsum('bj,ai->abij', u1t, ex1) - np.einsum('aj,bi->abij', u1t, ex1))
+    if sp == 2:
+        eovl2 = proj_sz2(eovl2, nocc, nso)
+    return ovl1, ovl2, eovl0, eovl1, eovl2
+
+
+def eval_kernel(hone, htwo, t1, t2, enuc, rotmat, nocc, sp=0):
+    """EvalKernel (FPUCC.f90:529-602), TrunODE 1: (lambda0, eps0, lambda1,
+    lambda2, eps1, eps2) of one grid point; the per-point ProjSz1/2 of SP = 2
+    (:583-588) repeat BuildKernSD's and change nothing."""
+    nso = hone.shape[0]
+    u0t, u1t, u2t = dstt2(rotmat, t1, t2, nocc)
+    fock, escf = mkfock_mo(hone, htwo, nocc)
+    escf = escf + enuc
+    ovl1, ovl2, eovl0, eovl1, eovl2 = build_kern_sd(fock, htwo, escf, u1t, u2t, nocc, sp)
+    if sp == 2:
+        ovl1, eovl1 = proj_sz1(ovl1, nocc, nso), proj_sz1(eovl1, nocc, nso)
+        ovl2, eovl2 = proj_sz2(ovl2, nocc, nso), proj_sz2(eovl2, nocc, nso)
+    lam0 = u0t
+    return lam0, eovl0 * lam0, ovl1 * lam0, ovl2 * lam0, eovl1 * lam0, eovl2 * lam0
+
+
+def pbar_hbar_olap(hone, htwo, t1, t2, nocc, enuc, qj, sp, npoints, r1, r2, rpg, rk, roota, rootb, rooty,
+                   weightsp, weightpg, fsp, fpg, fk):
+    """Every output of PbarHbarOlap (FPUCC.f90:14-272), one rank: Olap0,
+    OlapEx1 [a, i], OlapEx2 [a, b, i, j], EOlap0, EOlapEx1, EOlapEx2, with
+    the final ProjSz of SP = 2 (:234-237)."""
+    nso = hone.shape[0]
+    nv = nso - nocc
+    o0 = e0 = 0j
+    o1, e1 = np.zeros((nv, nocc), complex), np.zeros((nv, nocc), complex)
+    o2, e2 = np.zeros((nv, nv, nocc, nocc), complex), np.zeros((nv, nv, nocc, nocc), complex)
+    npg = rpg.shape[0]
+    passes = [(hone, htwo, t1, t2, None, 1.0)]
+    if len(fk) == 2:
+        passes.append((np.conj(hone), np.conj(htwo), np.conj(t1), np.conj(t2), rk[1], fk[1]))
+    for ho, ht, a1, a2, rkk, fac in passes:
+        for ia in range(int(npoints[0])):
+            for iy in range(int(npoints[1])):
+                for ipg in range(npg):
+                    rot = r1[ia] @ r2[iy]
+                    rot = rot @ rpg[ipg]
+                    if rkk is not None:
+                        rot = rot @ rkk
+                    lam0, eps0, lam1, lam2, eps1, eps2 = eval_kernel(ho, ht, a1, a2, enuc, rot, nocc, sp)
+                    w = point_weight(ia, iy, ipg, qj, sp, roota, rootb, rooty, weightsp, weightpg, fsp, fpg) * fac
+                    o0 += w * lam0
+                    e0 += w * eps0
+                    o1 += w * lam1
+                    e1 += w * eps1
+                    o2 += w * lam2
+                    e2 += w * eps2
+    if sp == 2:
+        o1, o2 = proj_sz1(o1, nocc, nso), proj_sz2(o2, nocc, nso)
+        e1, e2 = proj_sz1(e1, nocc, nso), proj_sz2(e2, nocc, nso)
+    return o0, o1, o2, e0, e1, e2
