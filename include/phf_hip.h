@@ -274,7 +274,7 @@ int phf_invert_batched(phf_ctx* ctx, int n, int batch, const double* m, double* 
  * caller.  Each call synchronises the context's stream before it returns. */
 int phf_invert_batched_noalloc(phf_ctx* ctx, int n, int batch, const double* m, double* minv, double* det,
                                int* info);
-/* Projected-CC grid consumer, zeroth order: Olap0 and EOlap0 of PbarHbarOlap
+/* Projected-CC grid consumer: Olap0 and EOlap0 of PbarHbarOlap
  * (SUCCSD_Boson_cmplx/FPUCC.f90:14-272, argument meaning as there) with the
  * lambda0 / eps0 part of EvalKernel (FPUCC.f90:529-602, TrunODE = 1):
  *   hone complex[nso][nso], htwo complex[nso]^4 (<pq||rs>, MO basis);
@@ -285,15 +285,30 @@ int phf_invert_batched_noalloc(phf_ctx* ctx, int n, int batch, const double* m, 
  *   rooty[nry]; weightsp[npoints0][npoints1]; weightpg[npg][ncipg][ncipg];
  *   fsp complex[ncisp] (ncisp = 2 qj + 1), fpg complex[ncipg], fk complex[ncik].
  * Outputs olap0, eolap0: one complex each.  nso <= 64, at most 32 occupied and
- * 32 virtual orbitals.  The singles/doubles outputs (OlapEx1/2, EOlapEx1/2,
- * which need the CCSD residual CCRes12) are not produced.  A singular
- * occupied block of R e^{T1} (InvertC's STOP) returns PHF_ERR_SINGULAR. */
+ * 32 virtual orbitals.  Only the zeroth-order outputs (phf_pcc_olap gives
+ * every output).  A singular occupied block of R e^{T1} (InvertC's STOP)
+ * returns PHF_ERR_SINGULAR. */
 int phf_pcc_olap0(phf_ctx* ctx, const double* hone, const double* htwo, const double* t1, const double* t2,
                   int nocc, int nso, double enuc, int qj, int sp, const int* npoints, int npg, int ncisp,
                   int ncipg, int ncik, const double* r1, int nr1, const double* r2, int nr2, const double* rpg,
                   const double* rk, const double* roota, int nra, const double* rootb, int nrb,
                   const double* rooty, int nry, const double* weightsp, const double* weightpg,
                   const double* fsp, const double* fpg, const double* fk, double* olap0, double* eolap0);
+/* Every output of PbarHbarOlap (SUCCSD_Boson_cmplx/FPUCC.f90:14-272; inputs as
+ * phf_pcc_olap0), TrunODE = 1: Olap0, OlapEx1, OlapEx2, EOlap0, EOlapEx1,
+ * EOlapEx2 in the Fortran argument order, with EvalKernel's singles and
+ * doubles (FPUCC.f90:529-602): the CCSD residual CCRes12 of every point's
+ * rotated amplitudes (CCRes.f90:164-536), BuildKernSD's fills
+ * (FPUCC_Tools.f90:291-436) and, for sp = 2, ProjSz (FPUCC_Tools.f90:150-233).
+ * olapex1 / eolapex1 complex[nso-nocc][nocc] (OlapEx1(NOcc+1+a, 1+i));
+ * olapex2 / eolapex2 complex[v][v][o][o] (OlapEx2(NOcc+1+a, NOcc+1+b, 1+i, 1+j)). */
+int phf_pcc_olap(phf_ctx* ctx, const double* hone, const double* htwo, const double* t1, const double* t2,
+                 int nocc, int nso, double enuc, int qj, int sp, const int* npoints, int npg, int ncisp,
+                 int ncipg, int ncik, const double* r1, int nr1, const double* r2, int nr2, const double* rpg,
+                 const double* rk, const double* roota, int nra, const double* rootb, int nrb,
+                 const double* rooty, int nry, const double* weightsp, const double* weightpg,
+                 const double* fsp, const double* fpg, const double* fk, double* olap0, double* olapex1,
+                 double* olapex2, double* eolap0, double* eolapex1, double* eolapex2);
 /* buildsc_dr.buildsc_vec (BuildSC_Vec, BuildSC_dr.f90:8-1244) and any other
  * weighted sum over given points: replaces the grid of phf_sc_set_grid with
  * npts points of rotation r (complex[npts][nso][nso], NULL for identities),

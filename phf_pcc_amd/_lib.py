@@ -36,7 +36,7 @@ EXPORTS = ["phf_ctx_create", "phf_ctx_destroy", "phf_last_error", "phf_set_hamil
            "phf_batch_autotune", "phf_batch_stats", "phf_batch_tuning_report", "phf_set_blocked", "phf_blocked",
            "phf_buildsc", "phf_sc_set_grid", "phf_sc_apply", "phf_sc_apply_packed", "phf_sc_apply_packed_dev",
            "phf_sd_dim", "phf_sc_set_points", "phf_eval_objective", "phf_invert_batched",
-           "phf_invert_batched_noalloc", "phf_pcc_olap0"]
+           "phf_invert_batched_noalloc", "phf_pcc_olap0", "phf_pcc_olap"]
 
 
 class PhfError(RuntimeError):
@@ -116,6 +116,7 @@ def load() -> C.CDLL:
     lib.phf_pcc_olap0.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, _ip,
                                   C.c_int, C.c_int, C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp, _dp,
                                   C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    lib.phf_pcc_olap.argtypes = lib.phf_pcc_olap0.argtypes[:-2] + [_dp] * 6
     for name in EXPORTS:
         if name not in ("phf_last_error", "phf_batch_last_error"):
             getattr(lib, name).restype = C.c_int

@@ -1,4 +1,4 @@
-"""Projected-CC grid consumer on the GPU, zeroth order.
+"""Projected-CC grid consumer on the GPU.
 
 ``pbar_hbar_olap0`` returns (Olap0, EOlap0) of PbarHbarOlap
 (SUCCSD_Boson_cmplx/FPUCC.f90:14-272) with the argument meaning of the
@@ -10,8 +10,17 @@ in the MO basis of the reference, as Main_PCC.py:30-42 prepares them):
 
 with the rotated amplitudes of DSTT2 (FPUCC_Tools.f90:697-722) and the CC
 energy of CCEnergy (CCRes.f90:9-29).  The per-point work runs in
-``phf_pcc_olap0`` (csrc/pcc.hip); the singles and doubles kernels
-(OlapEx1/2, EOlapEx1/2) are not produced.
+``phf_pcc_olap0`` (csrc/pcc.hip).  ``pbar_hbar_olap`` returns every output,
+(Olap0, OlapEx1, OlapEx2, EOlap0, EOlapEx1, EOlapEx2), adding EvalKernel's
+singles and doubles (``phf_pcc_olap``): per point the CCSD residual CCRes12
+(CCRes.f90:164-536) of the rotated amplitudes (U1t, U2t) and BuildKernSD's
+fills (FPUCC_Tools.f90:291-436),
+
+  OlapEx1  = sum_g w_g lambda0 U1t,  OlapEx2 = sum_g w_g lambda0 (U2t + U1t^U1t)
+  EOlapEx1 = sum_g w_g lambda0 (R1 + U1t EOvl0)
+  EOlapEx2 = sum_g w_g lambda0 (R2 + (U2t + U1t^U1t) EOvl0 + P(ij)P(ab) U1t R1)
+
+restricted by ProjSz (FPUCC_Tools.f90:150-233) when sp = 2.
 """
 from __future__ import annotations
 
@@ -26,6 +35,21 @@ def pbar_hbar_olap0(hone, htwo, t1, t2, nocc, enuc, qj, sp, npoints, r1, r2, rpg
     T1(NOcc+1:NSO, 1:NOcc); t2: (v, v, o, o); r1 (nl, nso, nso), r2 (ny, nso,
     nso), rpg (npg, nso, nso), rk (ncik, nso, nso) or None; weightsp (nl, ny);
     weightpg (npg, ncipg, ncipg); fsp (2 qj + 1); fpg (ncipg); fk (ncik)."""
+    return _run(False, hone, htwo, t1, t2, nocc, enuc, qj, sp, npoints, r1, r2, rpg, rk, roota, rootb, rooty,
+                weightsp, weightpg, fsp, fpg, fk, ctx)
+
+
+def pbar_hbar_olap(hone, htwo, t1, t2, nocc, enuc, qj, sp, npoints, r1, r2, rpg, rk, roota, rootb, rooty,
+                   weightsp, weightpg, fsp, fpg, fk, ctx: _lib.Context | None = None):
+    """(Olap0, OlapEx1, OlapEx2, EOlap0, EOlapEx1, EOlapEx2) of PbarHbarOlap,
+    arguments as pbar_hbar_olap0; OlapEx1 / EOlapEx1 (v, o), OlapEx2 /
+    EOlapEx2 (v, v, o, o)."""
+    return _run(True, hone, htwo, t1, t2, nocc, enuc, qj, sp, npoints, r1, r2, rpg, rk, roota, rootb, rooty,
+                weightsp, weightpg, fsp, fpg, fk, ctx)
+
+
+def _run(sd, hone, htwo, t1, t2, nocc, enuc, qj, sp, npoints, r1, r2, rpg, rk, roota, rootb, rooty,
+         weightsp, weightpg, fsp, fpg, fk, ctx):
     c128 = lambda a: np.ascontiguousarray(a, dtype=np.complex128)
     f64 = lambda a: np.ascontiguousarray(np.reshape(a, -1) if np.ndim(a) <= 1 else a, dtype=np.float64)
     hone, htwo, t1, t2 = c128(hone), c128(htwo), c128(t1), c128(t2)
@@ -62,15 +86,21 @@ def pbar_hbar_olap0(hone, htwo, t1, t2, nocc, enuc, qj, sp, npoints, r1, r2, rpg
     try:
         o = np.zeros(1, np.complex128)
         e = np.zeros(1, np.complex128)
-        rc = _lib.load().phf_pcc_olap0(ctx.handle, _lib.ptr(hone), _lib.ptr(htwo), _lib.ptr(t1), _lib.ptr(t2), nocc,
-                                       nso, float(enuc), int(qj), int(sp), npoints.ctypes.data_as(_lib._ip),
-                                       rpg.shape[0], fsp.size, fpg.size, ncik, _lib.ptr(r1), r1.shape[0],
-                                       _lib.ptr(r2), r2.shape[0], _lib.ptr(rpg), _lib.ptr(rk), _lib.ptr(roota),
-                                       roota.size, _lib.ptr(rootb), rootb.size, _lib.ptr(rooty), rooty.size,
-                                       _lib.ptr(weightsp), _lib.ptr(weightpg), _lib.ptr(fsp), _lib.ptr(fpg),
-                                       _lib.ptr(fk), _lib.ptr(o), _lib.ptr(e))
+        args = [ctx.handle, _lib.ptr(hone), _lib.ptr(htwo), _lib.ptr(t1), _lib.ptr(t2), nocc, nso, float(enuc),
+                int(qj), int(sp), npoints.ctypes.data_as(_lib._ip), rpg.shape[0], fsp.size, fpg.size, ncik,
+                _lib.ptr(r1), r1.shape[0], _lib.ptr(r2), r2.shape[0], _lib.ptr(rpg), _lib.ptr(rk), _lib.ptr(roota),
+                roota.size, _lib.ptr(rootb), rootb.size, _lib.ptr(rooty), rooty.size, _lib.ptr(weightsp),
+                _lib.ptr(weightpg), _lib.ptr(fsp), _lib.ptr(fpg), _lib.ptr(fk)]
+        if not sd:
+            rc = _lib.load().phf_pcc_olap0(*args, _lib.ptr(o), _lib.ptr(e))
+            _lib.check(rc, ctx.handle)
+            return complex(o[0]), complex(e[0])
+        o1, e1 = np.zeros((nv, nocc), np.complex128), np.zeros((nv, nocc), np.complex128)
+        o2, e2 = np.zeros((nv, nv, nocc, nocc), np.complex128), np.zeros((nv, nv, nocc, nocc), np.complex128)
+        rc = _lib.load().phf_pcc_olap(*args, _lib.ptr(o), _lib.ptr(o1), _lib.ptr(o2), _lib.ptr(e), _lib.ptr(e1),
+                                      _lib.ptr(e2))
         _lib.check(rc, ctx.handle)
-        return complex(o[0]), complex(e[0])
+        return complex(o[0]), o1, o2, complex(e[0]), e1, e2
     finally:
         if own:
             ctx.close()

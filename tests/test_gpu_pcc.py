@@ -1,7 +1,9 @@
-"""Projected-CC grid consumer, zeroth order (phf_pcc_olap0 through the C ABI)
-against the oracle's restatement of PbarHbarOlap / EvalKernel
+"""Projected-CC grid consumer (phf_pcc_olap0 and phf_pcc_olap through the C
+ABI) against the oracle's restatement of PbarHbarOlap / EvalKernel
 (oracle/pcc_oracle.py, pinned in brute-force Fock space): Olap0 and EOlap0
-to 1e-10 relative.
+to 1e-10 relative; OlapEx1/2 and EOlapEx1/2 (the CCSD residual of every
+point's rotated amplitudes, BuildKernSD's fills, ProjSz) to 1e-10 of each
+tensor's largest element.
 
 Cases: the 4-site Hubbard ring (NSO = 8, 4 electrons) on the exact 14 x 8
 spin grid, J = 0 and J = 1 with complex fsp; a point-group pair with two
@@ -63,6 +65,34 @@ def test_olap0_eolap0_match_oracle(kw):
     print(f"[pcc] {kw}: Olap0 {o_gpu:.12f}  EOlap0 {e_gpu:.12f}", flush=True)
     assert abs(o_gpu - o_ref) < 1e-10 * abs(o_ref)
     assert abs(e_gpu - e_ref) < 1e-10 * abs(e_ref)
+
+
+@pytest.mark.parametrize("kw", [
+    dict(nsite=4, seed=1, s1=0.2, s2=0.2),
+    dict(nsite=4, seed=2, s1=0.1, s2=0.3, j=1, ngrid=(26, 8)),
+    dict(nsite=4, seed=3, s1=0.2, s2=0.1, npg=2),
+    dict(nsite=4, seed=4, s1=0.2, s2=0.2, sp=2, ngrid=(1, 8)),
+    dict(nsite=4, seed=5, s1=0.15, s2=0.15, ncik=2, ngrid=(6, 4)),
+    dict(nsite=6, seed=6, s1=0.1, s2=0.4, ngrid=(6, 4)),
+    dict(nsite=5, seed=9, s1=0.2, s2=0.2, ngrid=(6, 4), ncik=2),
+    dict(nsite=10, seed=7, s1=0.05, s2=0.05, ngrid=(6, 4)),
+], ids=["ring4", "ring4-J1", "ring4-pg2", "ring4-sp2", "ring4-ncik2", "ring6", "ring5-ncik2", "ring10"])
+def test_singles_doubles_match_oracle(kw):
+    from phf_pcc_amd.pcc import pbar_hbar_olap, pbar_hbar_olap0
+    args = _case(**kw)
+    out = pbar_hbar_olap(*args)
+    ref = P.pbar_hbar_olap(*args)
+    names = ["Olap0", "OlapEx1", "OlapEx2", "EOlap0", "EOlapEx1", "EOlapEx2"]
+    for name, x, y in zip(names, out, ref):
+        err = np.abs(np.asarray(x) - np.asarray(y)).max() / np.abs(y).max()
+        print(f"[pcc-sd] {kw} {name}: max rel err {err:.2e}", flush=True)
+        assert err < 1e-10, name
+    # the zeroth-order outputs equal the zeroth-order entry point's
+    o0, e0 = pbar_hbar_olap0(*args)
+    assert abs(out[0] - o0) < 1e-12 * abs(o0) and abs(out[3] - e0) < 1e-12 * abs(e0)
+    if kw.get("sp") == 2:   # ProjSz: the spin-flip blocks are exactly zero
+        assert np.array_equal(out[1], P.proj_sz1(out[1], args[4], 2 * kw["nsite"]))
+        assert np.array_equal(out[5], P.proj_sz2(out[5], args[4], 2 * kw["nsite"]))
 
 
 def test_singular_occupied_block_is_an_error():
