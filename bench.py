@@ -487,6 +487,100 @@ def run_buildsc(args):
     print(json.dumps(out), flush=True)
 
 
+def run_pcc(args):
+    """Projected-CC grid consumer, every output of PbarHbarOlap
+    (FPUCC.f90:14-272, TrunODE = 1) through phf_pcc_olap: synthetic n=16
+    (NSO=32), N=16, L=48 Hamiltonian in its MO basis (seed 1007), random
+    complex T1, T2 ~ N(0, 0.05^2) (seed 1008), spin grid Lebedev 50 x 16
+    gamma = 800 points as rotations in the MO basis.  One step = one call
+    (host arrays in and out, as the Fortran routine takes them); value = grid
+    points per second of device time (HIP events from after the uploads to
+    the last grid sum).  roofline: the particle-particle ladder GEMM
+    (1/2 <ab||ef> tau_efij, V^2 x N^2 x V^2 per point, one batched launch per
+    chunk of points; 8 real FLOPs per complex multiply-add) on HIP events
+    around its launches.  cpu_baseline: the oracle's restatement
+    (numpy) of EvalKernel on a bounded sample of grid points, scaled."""
+    from oracle import pcc_oracle as P
+    from phf_pcc_amd import _lib
+    from phf_pcc_amd.hamiltonian import synthetic_problem
+    from phf_pcc_amd.pcc import pbar_hbar_olap
+    from phf_pcc_amd.spin_proj import grid_args
+    n, N, ngrid = 16, 16, (50, 16)
+    nso, V = 2 * n, 2 * n - N
+    p = synthetic_problem(n, N, seed=1007, build_dense=True)
+    rng = np.random.default_rng(1008)
+    t1 = 0.05 * (rng.normal(size=(V, N)) + 1j * rng.normal(size=(V, N)))
+    t2 = P.asymm2(0.05 * (rng.normal(size=(V, V, N, N)) + 1j * rng.normal(size=(V, V, N, N)))) / 4
+    ga = grid_args(n, ngrid, 0)
+    ref = p.ref.astype(complex)
+    tomo = lambda r: np.einsum('pa,gpq,qb->gab', ref.conj(), r, ref, optimize=True)
+    pargs = (p.hone, p.htwo, t1, t2, N, 0.0, 0, 1, ga["ngrid"], tomo(ga["r1"]), tomo(ga["r2"]),
+             np.eye(nso, dtype=complex)[None], None, ga["roota"], ga["rootb"], ga["rooty"], ga["weightsp"],
+             np.ones((1, 1, 1)), np.ones(1, complex), np.ones(1, complex), np.ones(1, complex))
+    G = int(np.prod(ga["ngrid"]))
+    ctx = _lib.Context(0)
+    ctx.set_profiling(True)
+    for _ in range(args.warmup):
+        out = pbar_hbar_olap(*pargs, ctx=ctx)
+    dev, wall, st = [], [], None
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        out = pbar_hbar_olap(*pargs, ctx=ctx)
+        wall.append(time.perf_counter() - t0)
+        st = ctx.stage_times()
+        dev.append(float(st[0]))
+    dev_ms, wall_s = float(np.mean(dev)), float(np.mean(wall))
+    log(f"[bench] pcc: {args.steps} calls, device {dev_ms:.2f} ms, wall {wall_s * 1e3:.2f} ms per call; stages "
+        f"zeroth {st[1]:.2f} ms, residual {st[2]:.2f} ms (ladder {st[3]:.2f} ms over {int(st[5])} launches), "
+        f"sums {st[4]:.2f} ms")
+    # parity on a sub-grid: the first 2 x 2 points against the oracle
+    sub = list(pargs)
+    sub[8] = np.array([2, 2])
+    sub[9], sub[10] = sub[9][:2], sub[10][:2]
+    sub[16] = ga["weightsp"][:2, :2]
+    got = pbar_hbar_olap(*sub, ctx=ctx)
+    want = P.pbar_hbar_olap(*sub)
+    parity = max(float(np.abs(np.asarray(x) - np.asarray(y)).max() / np.abs(y).max()) for x, y in zip(got, want))
+    log(f"[bench] pcc parity on a 2 x 2 sub-grid vs the oracle: max rel err {parity:.2e}")
+    nlad = max(1, int(st[5]))
+    chunk_pts = G / nlad
+    lad_flops = 8.0 * V ** 4 * N * N * chunk_pts
+    lad_ms = float(st[3]) / nlad
+    achieved = lad_flops / (lad_ms * 1e-3)
+    cpu = None
+    if not args.no_cpu:
+        rots = [(ia, iy) for ia in range(ngrid[0]) for iy in range(ngrid[1])]
+        done, tc = 0, 0.0
+        t0c = time.perf_counter()
+        while done < len(rots) and tc < args.cpu_seconds:
+            ia, iy = rots[done]
+            P.eval_kernel(p.hone, p.htwo, t1, t2, 0.0, pargs[9][ia] @ pargs[10][iy], N)
+            done += 1
+            tc = time.perf_counter() - t0c
+        cpu = {"value": done / tc, "unit": "grid pts/s",
+               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",
+               "sample": f"oracle EvalKernel (numpy einsum, BLAS-ordered) on {done} of {G} grid points in {tc:.1f} s"}
+        log(f"[bench] pcc CPU oracle: {cpu['value']:.2f} grid pts/s")
+    rec = {"metric": "projected-CC grid consumer (PbarHbarOlap, every output): grid pts/s",
+           "value": G / (dev_ms * 1e-3), "unit": "grid pts/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": dev_ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "c128", "data": "synthetic",
+           "config": {"workload": f"PbarHbarOlap TrunODE=1: synthetic n={n} (NSO={nso}), N={N}, L=48 (seed 1007), "
+                                  f"T1, T2 ~ N(0, 0.05^2) (seed 1008), spin grid 50x16={G} points, all six outputs",
+                      "wall_ms_per_call_with_host_transfers": wall_s * 1e3,
+                      "stage_ms": {"device": float(st[0]), "zeroth_order": float(st[1]), "residual": float(st[2]),
+                                   "ladder_gemm": float(st[3]), "sd_sums": float(st[4])},
+                      "ladder_launches": int(st[5]), "parity_rel_err_2x2": parity},
+           "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
+                        "frac": achieved / PEAK["fp64"], "traffic": load_traffic("zgemm_kernel<2, 2>", "so32-pcc"),
+                        "kernel": "zgemm_kernel<2, 2> (particle-particle ladder, 1/2 <ab||ef> tau_efij)",
+                        "flops_per_launch": lad_flops, "ms_per_launch": lad_ms,
+                        "time_source": "HIP events around the ladder launches on the library's stream"},
+           "cpu_baseline": cpu}
+    ctx.close()
+    print(json.dumps(rec), flush=True)
+
+
 def cpu_blas_gemm(prob, G):
     """Strong CPU baseline for the dominant stage: Gamma for every grid point as
     one BLAS dgemm A[(i,k),(j,l)] P (numpy's BLAS on all host cores; real A,
@@ -563,9 +657,11 @@ def main():
                     help="precisions of the batch block (fp64 and the plain dense FP64 batch always run)")
     ap.add_argument("--no-autotune", action="store_true", help="use the tile cost model instead of timing candidates")
     ap.add_argument("--sc-block", type=int, default=280, help="vectors per block of the buildsc workload")
-    ap.add_argument("--workload", default="single", choices=["single", "batch64", "large", "adjoint", "buildsc"],
+    ap.add_argument("--workload", default="single",
+                    choices=["single", "batch64", "large", "adjoint", "buildsc", "pcc"],
                     help="single: one n=64 instance (default); batch64: 64 uneven instances in one batch; "
-                         "large: n=128, 50x24 grid, every precision; adjoint: n=96 gradient checks + 50 L-BFGS")
+                         "large: n=128, 50x24 grid, every precision; adjoint: n=96 gradient checks + 50 L-BFGS; "
+                         "buildsc: BuildSC blocks; pcc: the projected-CC grid consumer (NSO=32, 800 points)")
     ap.add_argument("--ninst", type=int, default=64, help="instances of the batch64 workload")
     ap.add_argument("--no-baselines", action="store_true",
                     help="batch64: evaluate only the listed forms and print no line (for profiler runs)")
@@ -595,6 +691,8 @@ def main():
         return run_adjoint(args)
     if args.workload == "buildsc":
         return run_buildsc(args)
+    if args.workload == "pcc":
+        return run_pcc(args)
     from phf_pcc_amd.PHFTools_F import phftools
     from phf_pcc_amd.model import ProjectedHF
     from phf_pcc_amd.phf import eigen_solver, vec_decomp

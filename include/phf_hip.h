@@ -188,7 +188,12 @@ int phf_localg(phf_ctx* ctx, const double* gradmat, const double* rdmmat, const 
  * After a profiled phf_sc_apply / phf_sc_apply_packed(_dev): [0] all device
  * work, [1] stage-1 pair-transform GEMM, [2] stage-2 GEMM with the grid sum,
  * [3] P GEMM, [4] rank-term GEMM, [5] per-point t0/t1 kernel, [6] the rest
- * (factors, packing, outputs), [7] the number of GEMM launches. */
+ * (factors, packing, outputs), [7] the number of GEMM launches.
+ * After a profiled phf_pcc_olap0 / phf_pcc_olap: [0] all device work after
+ * the uploads, [1] the zeroth-order per-point stages (rotations through
+ * TransT2 and CCEnergy), [2] the CCSD residuals, [3] the particle-particle
+ * ladder GEMM launches (1/2 <ab||ef> tau_efij), [4] the singles/doubles grid
+ * sums, [5] the number of ladder launches (one per chunk of points). */
 int phf_set_profiling(phf_ctx* ctx, int on);
 int phf_stage_times(const phf_ctx* ctx, double* ms, int nmax);
 

@@ -234,6 +234,11 @@ def pbar_hbar_olap0(hone, htwo, t1, t2, nocc, enuc, qj, sp, npoints, r1, r2, rpg
 # singles and doubles: CCRes12 / CCItms, the BuildKernSD fills, ProjSz
 # ---------------------------------------------------------------------------
 
+def _es(*args):
+    """np.einsum with contraction-order optimisation (BLAS for the pair products)."""
+    return np.einsum(*args, optimize=True)
+
+
 def _aperm1(m):
     """APerm(M, 1) (CCRes.f90:120-156): M(I,J,:,:) - M(J,I,:,:)."""
     return m - m.transpose(1, 0, 2, 3)
@@ -252,31 +257,31 @@ def cc_itms(fock, h2, t1, t2, nocc):
     o, v = slice(0, nocc), slice(nocc, nso)
     hoovv, hooov, hoovo = h2[o, o, v, v], h2[o, o, o, v], h2[o, o, v, o]
     hvovv, hovvv = h2[v, o, v, v], h2[o, v, v, v]
-    itm1 = np.einsum('ai,bj->abij', t1, t1)
+    itm1 = _es('ai,bj->abij', t1, t1)
     itm4 = t2 / 2 + itm1                                            # before the APerms
     itm1 = _aperm2(_aperm1(itm1))
     tau = t2 + itm1 / 2
     tb = t2 + itm1 / 4
     # Fvv: Transpose2(tb/2) then At_dot_B over the last index (:407-424)
     fvv = fock[v, v] - t1 @ fock[o, v] / 2
-    fvv = fvv + np.einsum('fm,ambf->ab', t1, hvovv)
-    fvv = fvv - 0.5 * np.einsum('ebmn,mnfb->ef', tb, hoovv)
+    fvv = fvv + _es('fm,ambf->ab', t1, hvovv)
+    fvv = fvv - 0.5 * _es('ebmn,mnfb->ef', tb, hoovv)
     # Foo (:426-441)
     foo = fock[o, o] + fock[o, v] @ t1 / 2
-    foo = foo + np.einsum('em,imje->ij', t1, hooov)
-    foo = foo + 0.5 * np.einsum('ineb,ebjn->ij', hoovv, tb)
+    foo = foo + _es('em,imje->ij', t1, hooov)
+    foo = foo + 0.5 * _es('ineb,ebjn->ij', hoovv, tb)
     # Fov (:443-452)
-    fov = fock[o, v] + np.einsum('em,imae->ia', t1, hoovv)
+    fov = fock[o, v] + _es('em,imae->ia', t1, hoovv)
     # Woooo (:453-462)
-    woooo = h2[o, o, o, o] + np.einsum('mnef,efij->mnij', hoovv, tau / 4)
-    woooo = woooo + _aperm2(np.einsum('mnie,ej->mnij', hooov, t1))
+    woooo = h2[o, o, o, o] + _es('mnef,efij->mnij', hoovv, tau / 4)
+    woooo = woooo + _aperm2(_es('mnie,ej->mnij', hooov, t1))
     # Wvvvv (:463-485)
-    wvvvv = h2[v, v, v, v] + np.einsum('efmn,mncb->efcb', tau / 4, hoovv)
-    wvvvv = wvvvv - _aperm1(np.einsum('amcd,bm->abcd', hvovv, t1))
+    wvvvv = h2[v, v, v, v] + _es('efmn,mncb->efcb', tau / 4, hoovv)
+    wvvvv = wvvvv - _aperm1(_es('amcd,bm->abcd', hvovv, t1))
     # Wovvo (:487-519)
-    wovvo = h2[o, v, v, o] + np.einsum('mxcf,fj->mxcj', hovvv, t1)
-    wovvo = wovvo - np.einsum('mncj,bn->mbcj', hoovo, t1)
-    wovvo = wovvo - np.einsum('xbjn,incx->ibcj', itm4, hoovv)
+    wovvo = h2[o, v, v, o] + _es('mxcf,fj->mxcj', hovvv, t1)
+    wovvo = wovvo - _es('mncj,bn->mbcj', hoovo, t1)
+    wovvo = wovvo - _es('xbjn,incx->ibcj', itm4, hoovv)
     return tau, fvv, foo, fov, woooo, wvvvv, wovvo
 
 
@@ -291,28 +296,28 @@ def cc_res12(fock, h2, t1, t2, nocc):
     hvoov, hooov, hvovv = h2[v, o, o, v], h2[o, o, o, v], h2[v, o, v, v]
     # res1 (:215-236)
     r1 = fock[v, o] + fvv @ t1 - t1 @ foo
-    r1 = r1 + np.einsum('me,imae->ai', fov, t2tmp)
-    r1 = r1 + np.einsum('em,amie->ai', t1, hvoov)
-    r1 = r1 - 0.5 * np.einsum('mnab,mnib->ai', t2tmp, hooov)
-    r1 = r1 + 0.5 * np.einsum('ameb,imeb->ai', hvovv, t2tmp)
+    r1 = r1 + _es('me,imae->ai', fov, t2tmp)
+    r1 = r1 + _es('em,amie->ai', t1, hvoov)
+    r1 = r1 - 0.5 * _es('mnab,mnib->ai', t2tmp, hooov)
+    r1 = r1 + 0.5 * _es('ameb,imeb->ai', hvovv, t2tmp)
     # res2 (:238-352), r2tmp (i, j, a, b)
     r2tmp = np.zeros_like(t2tmp)
     itm1 = fvv - t1 @ fov / 2
-    r2tmp = r2tmp + _aperm2(np.einsum('ijae,be->ijab', t2tmp, itm1))
+    r2tmp = r2tmp + _aperm2(_es('ijae,be->ijab', t2tmp, itm1))
     itm3 = foo + fov @ t1 / 2
-    r2tmp = r2tmp - _aperm1(np.einsum('imab,mj->ijab', t2tmp, itm3))
+    r2tmp = r2tmp - _aperm1(_es('imab,mj->ijab', t2tmp, itm3))
     hvoov2 = h2[o, v, v, o].transpose(2, 3, 0, 1)                     # Transpose2: (e, j, m, b) = H(m, b, e, j)
-    itm4 = np.einsum('ei,ejmb->ijmb', t1, hvoov2)
-    itm2 = np.einsum('ijmb,am->ijab', itm4, -t1)
-    itm2 = itm2 + np.einsum('imae,mbej->ijab', t2tmp, wovvo)
+    itm4 = _es('ei,ejmb->ijmb', t1, hvoov2)
+    itm2 = _es('ijmb,am->ijab', itm4, -t1)
+    itm2 = itm2 + _es('imae,mbej->ijab', t2tmp, wovvo)
     r2tmp = r2tmp + _aperm2(_aperm1(itm2))
     wt = wvvvv.transpose(2, 3, 0, 1)                                  # Transpose1: (e, f, a, b) = W(a, b, e, f)
-    r2tmp = r2tmp + np.einsum('efij,efab->ijab', tau / 2, wt)
-    r2tmp = r2tmp + np.einsum('mnij,mnab->ijab', woooo, tau.transpose(2, 3, 0, 1) / 2)
+    r2tmp = r2tmp + _es('efij,efab->ijab', tau / 2, wt)
+    r2tmp = r2tmp + _es('mnij,mnab->ijab', woooo, tau.transpose(2, 3, 0, 1) / 2)
     hvovv2 = h2[v, v, v, o].transpose(2, 3, 0, 1)                     # (e, j, a, b) = H(a, b, e, j)
     hooov2 = h2[o, v, o, o].transpose(2, 3, 0, 1)                     # (i, j, m, b) = H(m, b, i, j)
-    r2tmp = r2tmp + _aperm1(np.einsum('ei,ejab->ijab', t1, hvovv2))
-    r2tmp = r2tmp - _aperm2(np.einsum('ijmb,am->ijab', hooov2, t1))
+    r2tmp = r2tmp + _aperm1(_es('ei,ejab->ijab', t1, hvovv2))
+    r2tmp = r2tmp - _aperm2(_es('ijmb,am->ijab', hooov2, t1))
     res2 = r2tmp.transpose(2, 3, 0, 1) + h2[v, v, o, o]
     return r1, res2
 
@@ -356,7 +361,7 @@ def build_kern_sd(fock, h2, escf, u1t, u2t, nocc, sp):
     ovl1 = u1t.copy()
     if sp == 2:
         ovl1 = proj_sz1(ovl1, nocc, nso)
-    ovl2 = u2t + np.einsum('ai,bj->abij', u1t, u1t) - np.einsum('bi,aj->abij', u1t, u1t)
+    ovl2 = u2t + _es('ai,bj->abij', u1t, u1t) - _es('bi,aj->abij', u1t, u1t)
     if sp == 2:
         ovl2 = proj_sz2(ovl2, nocc, nso)
     eovl0 = cc_energy(u1t, u2t, fock, h2, nocc) + escf
@@ -365,8 +370,8 @@ def build_kern_sd(fock, h2, escf, u1t, u2t, nocc, sp):
     if sp == 2:
         eovl1 = proj_sz1(eovl1, nocc, nso)
     eovl2 = (ex2 + ovl2 * eovl0
-             + np.einsum('ai,bj->abij', u1t, ex1) - np.einsum('bi,aj->abij', u1t, ex1)
-             + np.einsum('bj,ai->abij', u1t, ex1) - np.einsum('aj,bi->abij', u1t, ex1))
+             + _es('ai,bj->abij', u1t, ex1) - _es('bi,aj->abij', u1t, ex1)
+             + _es('bj,ai->abij', u1t, ex1) - _es('aj,bi->abij', u1t, ex1))
     if sp == 2:
         eovl2 = proj_sz2(eovl2, nocc, nso)
     return ovl1, ovl2, eovl0, eovl1, eovl2

@@ -137,26 +137,22 @@ __global__ void pcc_vmat_kernel(int nso, int N, const double* __restrict__ R, co
   }
 }
 
-// The second-order part of TransT2, CI2CC, DSTT2 and CCEnergy for one point.
-// U: the transformed T2, row-major [p][q][r][s]; scratch: F (nso^2) and
-// T (V^2 N^2) per point.
-__global__ void __launch_bounds__(kThreads) pcc_transt2_kernel(
-    int nso, int N, const double* __restrict__ Ubuf, long ustride, const double* __restrict__ U1all,
-    const double* __restrict__ U0all, const double* __restrict__ fovh, const double* __restrict__ escf,
-    const double* __restrict__ h2, int conj, double* __restrict__ Fbuf, double* __restrict__ Tbuf,
-    double* __restrict__ lam_out, double* __restrict__ eps_out, double* __restrict__ w1_out,
-    double* __restrict__ w2_out, double* __restrict__ e0_out) {
+// TransT2's second order (PCCCI.f90:98-245) runs as batched GEMMs over the
+// blocks of the transformed tensor U (below, host side); these kernels do the
+// rest.  Layouts per point: U row-major [p][q][r][s]; F = sum_i U(p,i,q,i)
+// row-major nso x nso; sc = (E0, C0); C1 [a][i]; C2 [a][b][i][j].
+
+// F, E0 = sum_ij U(i,j,i,j)/2 and C0 = 1 + E0 + E0^2/2 + sum fov fvo / 2 +
+// sum_ijab U(i,j,a,b) U(a,b,i,j) / 8.  One workgroup per point.
+__global__ void __launch_bounds__(kThreads) pcc_tt2_prep_kernel(int nso, int N, const double* __restrict__ Ubuf,
+                                                                long ustride, double* __restrict__ Fbuf,
+                                                                double* __restrict__ sc) {
   __shared__ double2 red[kThreads];
-  __shared__ double2 c1s[32 * 32];
-  __shared__ double2 w1s[32 * 32];
-  const int g = blockIdx.x, V = nso - N;
-  const int t = threadIdx.x;
+  const int g = blockIdx.x, V = nso - N, t = threadIdx.x;
   const long n1 = nso, n2 = n1 * nso, n3 = n2 * nso;
   const double* U = Ubuf + ustride * g;
   auto Ue = [&](int p, int q, int r, int s) { return ld2(U, p * n3 + q * n2 + (long)r * n1 + s); };
   double2* F = reinterpret_cast<double2*>(Fbuf) + n2 * g;
-  double2* T = reinterpret_cast<double2*>(Tbuf) + (long)V * V * N * N * g;
-  // F(p,q) = sum_i U(p,i,q,i)
   for (int idx = t; idx < nso * nso; idx += kThreads) {
     const int p = idx / nso, q = idx % nso;
     double2 acc = make_double2(0.0, 0.0);
@@ -166,90 +162,103 @@ __global__ void __launch_bounds__(kThreads) pcc_transt2_kernel(
   double2 part = make_double2(0.0, 0.0);
   for (int idx = t; idx < N * N; idx += kThreads) part = cadd(part, Ue(idx / N, idx % N, idx / N, idx % N));
   const double2 e0 = cscale(block_sum(part, red), 0.5);   // also orders the F writes before their reads
-  auto Fe = [&](int p, int q) { return F[p * nso + q]; };
-  auto vvoo = [&](int a, int b, int i, int j) { return Ue(N + a, N + b, i, j); };
-  // C0 = 1 + E0 + E0^2/2 + sum fov fvo / 2 + sum oovv vvoo / 8
   part = make_double2(0.0, 0.0);
   for (int idx = t; idx < N * V; idx += kThreads) {
     const int i = idx / V, a = idx % V;
-    part = cadd(part, cscale(cmul(Fe(i, N + a), Fe(N + a, i)), 0.5));
+    part = cadd(part, cscale(cmul(F[i * nso + N + a], F[(N + a) * nso + i]), 0.5));
   }
-  for (long idx = t; idx < (long)N * N * V * V; idx += kThreads) {
+  for (long idx = t; idx < (long)N * N * V * V; idx += kThreads) {   // (i, j, a, b), b fastest
     const int b = idx % V, a = (idx / V) % V, j = (idx / V / V) % N, i = idx / V / V / N;
-    part = cadd(part, cscale(cmul(Ue(i, j, N + a, N + b), vvoo(a, b, i, j)), 0.125));
+    part = cadd(part, cscale(cmul(Ue(i, j, N + a, N + b), Ue(N + a, N + b, i, j)), 0.125));
   }
-  const double2 c0 = cadd(cadd(make_double2(1.0 + e0.x, e0.y), cscale(cmul(e0, e0), 0.5)), block_sum(part, red));
-  // C1(a,i)
-  for (int idx = t; idx < V * N; idx += kThreads) {
+  const double2 q = block_sum(part, red);
+  if (t == 0) {
+    double2* o = reinterpret_cast<double2*>(sc) + 2L * g;
+    o[0] = e0;
+    o[1] = cadd(cadd(make_double2(1.0 + e0.x, e0.y), cscale(cmul(e0, e0), 0.5)), q);
+  }
+}
+
+// Elementwise parts: C2 = vvoo (1 + E0) / 4 - fvo(a,j) fvo(b,i) / 2 for the
+// threads of [0, V^2 N^2); for [0, V N) also
+// C1 = fvo (1 + E0) + (fvv fvo - fvo foo) / 2 + sum_bj vvoo(a,b,i,j) fov(j,b) / 2
+//      - sum_jb U(j,a,i,b) fvo(b,j) / 2.  grid (ceil(V^2 N^2 / 256), points).
+__global__ void __launch_bounds__(256) pcc_tt2_small_kernel(int nso, int N, const double* __restrict__ Ubuf,
+                                                            long ustride, const double* __restrict__ Fbuf,
+                                                            const double* __restrict__ sc, double* __restrict__ C1,
+                                                            double* __restrict__ C2, long c2stride) {
+  const int V = nso - N;
+  const long S = (long)V * V * N * N;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long g = blockIdx.y;
+  if (idx >= S) return;
+  const long n1 = nso, n2 = n1 * nso, n3 = n2 * nso;
+  const double* U = Ubuf + ustride * g;
+  auto Ue = [&](int p, int q, int r, int s) { return ld2(U, p * n3 + q * n2 + (long)r * n1 + s); };
+  const double2* F = reinterpret_cast<const double2*>(Fbuf) + n2 * g;
+  auto Fe = [&](int p, int q) { return F[p * nso + q]; };
+  const double2 e0 = ld2(sc, 2 * g), one_e0 = make_double2(1.0 + e0.x, e0.y);
+  {
+    const int j = idx % N, i = (idx / N) % N, b = (idx / N / N) % V, a = idx / ((long)N * N * V);
+    double2 v = cscale(cmul(Ue(N + a, N + b, i, j), one_e0), 0.25);
+    v = csub(v, cscale(cmul(Fe(N + a, j), Fe(N + b, i)), 0.5));
+    reinterpret_cast<double2*>(C2)[g * c2stride + idx] = v;
+  }
+  if (idx < (long)V * N) {
     const int a = idx / N, i = idx % N;
-    const double2 fvo_ai = Fe(N + a, i);
-    double2 acc = cadd(fvo_ai, cmul(e0, fvo_ai));
-    double2 s = make_double2(0.0, 0.0);
-    for (int k = 0; k < N; ++k) s = csub(s, cmul(Fe(N + a, k), Fe(k, i)));
-    for (int c = 0; c < V; ++c) s = cadd(s, cmul(Fe(N + a, N + c), Fe(N + c, i)));
-    acc = cadd(acc, cscale(s, 0.5));
-    double2 q = make_double2(0.0, 0.0);   // -1/4 terms
-    for (int j = 0; j < N; ++j)
-      for (int k = 0; k < N; ++k)
-        for (int b = 0; b < V; ++b) q = cadd(q, cmul(vvoo(a, b, j, k), Ue(j, k, i, N + b)));
-    for (int j = 0; j < N; ++j)
-      for (int c = 0; c < V; ++c)
-        for (int d = 0; d < V; ++d) q = cadd(q, cmul(Ue(j, N + a, N + c, N + d), vvoo(c, d, i, j)));
-    acc = csub(acc, cscale(q, 0.25));
-    double2 h = make_double2(0.0, 0.0);   // +1/2 and -1/2 terms
+    double2 acc = cmul(Fe(N + a, i), one_e0);
+    double2 h = make_double2(0.0, 0.0);
+    for (int c = 0; c < V; ++c) h = cadd(h, cmul(Fe(N + a, N + c), Fe(N + c, i)));
+    for (int k = 0; k < N; ++k) h = csub(h, cmul(Fe(N + a, k), Fe(k, i)));
     for (int b = 0; b < V; ++b)
       for (int j = 0; j < N; ++j) {
-        h = cadd(h, cmul(vvoo(a, b, i, j), Fe(j, N + b)));
+        h = cadd(h, cmul(Ue(N + a, N + b, i, j), Fe(j, N + b)));
         h = csub(h, cmul(Ue(j, N + a, i, N + b), Fe(N + b, j)));
       }
-    c1s[idx] = cadd(acc, cscale(h, 0.5));
+    reinterpret_cast<double2*>(C1)[g * V * N + idx] = cadd(acc, cscale(h, 0.5));
   }
-  __syncthreads();
-  // C2(a,b,i,j), then T = C2/C0 - [a != b, i != j] t1(a,i) t1(b,j) / 2 with t1 = C1/C0
-  for (long idx = t; idx < (long)V * V * N * N; idx += kThreads) {
-    const int j = idx % N, i = (idx / N) % N, b = (idx / N / N) % V, a = idx / N / N / V;
-    const double2 v = vvoo(a, b, i, j);
-    double2 acc = cadd(cscale(v, 0.25), cscale(cmul(e0, v), 0.25));
-    acc = csub(acc, cscale(cmul(Fe(N + a, j), Fe(N + b, i)), 0.5));
-    double2 s = make_double2(0.0, 0.0);
-    for (int k = 0; k < N; ++k)
-      for (int c = 0; c < V; ++c) s = cadd(s, cmul(vvoo(a, c, j, k), Ue(k, N + b, i, N + c)));
-    acc = cadd(acc, cscale(s, 0.5));
-    double2 s16 = make_double2(0.0, 0.0);
-    for (int c = 0; c < V; ++c)
-      for (int d = 0; d < V; ++d) s16 = cadd(s16, cmul(vvoo(c, d, i, j), Ue(N + a, N + b, N + c, N + d)));
-    for (int k = 0; k < N; ++k)
-      for (int l = 0; l < N; ++l) s16 = cadd(s16, cmul(Ue(k, l, i, j), vvoo(a, b, k, l)));
-    acc = cadd(acc, cscale(s16, 1.0 / 16.0));
-    double2 s4 = make_double2(0.0, 0.0);
-    for (int k = 0; k < N; ++k) {
-      s4 = csub(s4, cmul(vvoo(a, b, i, k), Fe(k, j)));
-      s4 = csub(s4, cmul(Ue(k, N + b, i, j), Fe(N + a, k)));
-    }
-    for (int c = 0; c < V; ++c) {
-      s4 = cadd(s4, cmul(vvoo(c, b, i, j), Fe(N + a, N + c)));
-      s4 = cadd(s4, cmul(Ue(N + a, N + b, i, N + c), Fe(N + c, j)));
-    }
-    acc = cadd(acc, cscale(s4, 0.25));
-    double2 tv = cdiv(acc, c0);
-    if (a != b && i != j) {
-      const double2 ta = cdiv(c1s[a * N + i], c0), tb = cdiv(c1s[b * N + j], c0);
-      tv = csub(tv, cscale(cmul(ta, tb), 0.5));
-    }
-    T[idx] = tv;
-  }
-  // W1 = C1/C0 + U1 (DSTT2)
+}
+
+// CI2CC + ASymm2 (PCCCI.f90:70-96, CCRes.f90:31-69), DSTT2 (W0 U0, W1 + U1)
+// and CCEnergy (CCRes.f90:9-29) for one point: T = C2/C0 - [a != b, i != j]
+// t1(a,i) t1(b,j) / 2 with t1 = C1/C0; W1 = t1 + U1; W2 = ASymm2(T);
+// lambda0 = C0 U0, eps0 = (ECorr(W1, W2) + ESCF + ENuc) lambda0.  Optional
+// outputs for the singles and doubles: W1, W2 and EOvl0 per point.
+__global__ void __launch_bounds__(kThreads) pcc_tt2_finish_kernel(
+    int nso, int N, const double* __restrict__ C1all, const double* __restrict__ C2all, long c2stride,
+    const double* __restrict__ sc, const double* __restrict__ U1all, const double* __restrict__ U0all,
+    const double* __restrict__ fovh, const double* __restrict__ escf, const double* __restrict__ h2, int conj,
+    double* __restrict__ Tbuf, double* __restrict__ lam_out, double* __restrict__ eps_out,
+    double* __restrict__ w1_out, double* __restrict__ w2_out, double* __restrict__ e0_out) {
+  __shared__ double2 red[kThreads];
+  __shared__ double2 t1s[32 * 32];
+  __shared__ double2 w1s[32 * 32];
+  const int g = blockIdx.x, V = nso - N;
+  const int t = threadIdx.x;
+  const long n1 = nso, n2 = n1 * nso, n3 = n2 * nso;
+  const double2 c0 = ld2(sc, 2L * g + 1);
+  const double2* C1 = reinterpret_cast<const double2*>(C1all) + (long)g * V * N;
+  const double2* C2 = reinterpret_cast<const double2*>(C2all) + c2stride * g;
+  double2* T = reinterpret_cast<double2*>(Tbuf) + (long)V * V * N * N * g;
   const double2* U1 = reinterpret_cast<const double2*>(U1all) + (long)g * V * N;
   for (int idx = t; idx < V * N; idx += kThreads) {
     const int a = idx / N, i = idx % N;
-    w1s[idx] = cadd(cdiv(c1s[idx], c0), U1[(long)i * V + a]);   // U1 is column-major V x N
+    t1s[idx] = cdiv(C1[idx], c0);
+    w1s[idx] = cadd(t1s[idx], U1[(long)i * V + a]);   // U1 is column-major V x N
     if (w1_out) reinterpret_cast<double2*>(w1_out)[(long)g * V * N + idx] = w1s[idx];
+  }
+  __syncthreads();
+  for (long idx = t; idx < (long)V * V * N * N; idx += kThreads) {
+    const int j = idx % N, i = (idx / N) % N, b = (idx / N / N) % V, a = idx / N / N / V;
+    double2 tv = cdiv(C2[idx], c0);
+    if (a != b && i != j) tv = csub(tv, cscale(cmul(t1s[a * N + i], t1s[b * N + j]), 0.5));
+    T[idx] = tv;
   }
   __syncthreads();
   // CCEnergy(W1, W2) with W2 = ASymm2(T): fov . W1 + 1/4 sum ERI(i,j,a,b) tau(a,b,i,j)
   const double sg = conj ? -1.0 : 1.0;
   const double2* fov = reinterpret_cast<const double2*>(fovh);
-  part = make_double2(0.0, 0.0);
+  double2 part = make_double2(0.0, 0.0);
   for (int idx = t; idx < N * V; idx += kThreads) part = cadd(part, cmul(fov[idx], w1s[(idx % V) * N + idx / V]));
   for (long idx = t; idx < (long)V * V * N * N; idx += kThreads) {
     const int j = idx % N, i = (idx / N) % N, b = (idx / N / N) % V, a = idx / N / N / V;
@@ -585,7 +594,7 @@ void build_shared(hipStream_t s, int nso, int N, const double* h1, const double*
 // 1/2 <ab||ef> tau_efij + 1/8 tau_abmn (<mn||ef> tau_efij)
 //   - 1/2 P(ab) t_bm (<am||ef> tau_efij)
 // (CCItms' Wvvvv contracted with tau, without forming Wvvvv per point).
-void pcc_residual(hipStream_t s, int V, int N, long cg, const PccShared& h, const PccSD& b) {
+void pcc_residual(hipStream_t s, int V, int N, long cg, const PccShared& h, const PccSD& b, hipEvent_t* lad) {
   const long O = N, S = (long)V * V * N * N, VN = (long)V * N, N2 = O * O, V2 = (long)V * V, N3 = N2 * O;
   const long N4 = N2 * N2;
   const ZMat W1all{b.W1, 2, 2 * VN, 0, 1, 0};   // [(e, m)][g]: every point's U1t as one k x cg operand
@@ -651,8 +660,10 @@ void pcc_residual(hipStream_t s, int V, int N, long cg, const PccShared& h, cons
          true, b.R2, S, cg);
   // (4) ladder and hole-hole terms: 1/2 <ab||ef> tau_efij; Weff = <mn||ij> + 1/2 <mn||ef> tau_efij
   //     + P(ij) <mn||ie> t_ej, then 1/2 tau_abmn Weff_mnij; -1/2 P(ab) t_bm (<am||ef> tau_efij)
+  if (lad) (void)hipEventRecord(lad[0], s);
   zgemm_batched(s, (int)V2, (int)N2, (int)V2, 0.5, 0, RM(h.Hvvvv, V2, 0), RM(b.tau, N2, S), 1, 0, RO(b.R2, N2, S),
                 (int)cg);
+  if (lad) (void)hipEventRecord(lad[1], s);
   zgemm_batched(s, (int)N2, (int)N2, (int)V2, 0.5, 0, RM(h.Hoovv, V2, 0), RM(b.tau, N2, S), 1, 0,
                 RO(b.Weff, N2, N4), (int)cg);
   zgemm_batched(s, (int)N3, N, V, 1, 0, RM(h.Hooov, V, 0), RM(b.W1, N, VN), 0, 0, RO(b.Wp, N, N4), (int)cg);
@@ -673,6 +684,72 @@ void pcc_residual(hipStream_t s, int V, int N, long cg, const PccShared& h, cons
   zgemm_batched(s, V, (int)(VN * O), N, 1, 0, RM(b.W1, N, VN), RM(h.Hovoo, VN * O, 0), 0, 0, RO(b.X, VN * O, S),
                 (int)cg);
   gather(s, V, V, O, O, {pterm(b.X, S, VN * O, N2, O, 1, -1.0), pterm(b.X, S, N2, VN * O, O, 1)}, true, b.R2, S, cg);
+}
+
+// TransT2's contractions (PCCCI.f90:98-245) for cg points as batched GEMMs
+// over compact blocks of U (per point: row-major nso^4 at U + 2 n4 g); the
+// blocks, C2 and a product scratch live in W (n4 complex elements per point,
+// offsets below; they fit since (V + N)^4 exceeds them by 2 V N (V^2 + N^2)).
+// F and sc come from pcc_tt2_prep_kernel; C1 (V N per point) and C2 start
+// from pcc_tt2_small_kernel's elementwise terms.  Returns C2's offset in W.
+long pcc_transt2_gemm(hipStream_t s, int nso, int N, long cg, const double* U, double* W, const double* F,
+                      const double* sc, double* C1) {
+  const long V = nso - N, O = N, S = V * V * O * O, VN = V * O;
+  const long n1 = nso, n2 = n1 * nso, n3 = n2 * nso, n4 = n2 * n2;
+  const long o_vvoo = 0, o_P1 = S, o_P2 = 2 * S, o_ovP = 3 * S, o_C2 = 4 * S, o_X = 5 * S, o_V4 = 6 * S;
+  const long o_O4 = o_V4 + V * V * V * V, o_ooov = o_O4 + O * O * O * O, o_ovvv = o_ooov + V * O * O * O;
+  const long o_ovoo = o_ovvv + V * V * V * O, o_vvov = o_ovoo + O * V * O * O;
+  auto w = [&](long off) { return W + 2 * off; };
+  // block of U: out dim k is occupied ('o') or virtual ('v') and reads U position pos[k]
+  const long hst[4] = {n3, n2, n1, 1};
+  auto ublk = [&](const char* ty, int p0, int p1, int p2, int p3, long off) {
+    const int pos[4] = {p0, p1, p2, p3};
+    int d[4];
+    long base = 0, st[4];
+    for (int k = 0; k < 4; ++k) {
+      d[k] = ty[k] == 'v' ? (int)V : (int)O;
+      st[k] = hst[pos[k]];
+      if (ty[k] == 'v') base += O * hst[pos[k]];
+    }
+    gather(s, d[0], d[1], d[2], d[3], {pterm(U + 2 * base, n4, st[0], st[1], st[2], st[3])}, false, w(off), n4, cg);
+  };
+  ublk("vvoo", 0, 1, 2, 3, o_vvoo);   // [a][b][i][j] = U(a,b,i,j)
+  ublk("vovo", 0, 2, 1, 3, o_P1);     // [a][j][c][k] = U(a,c,j,k)
+  ublk("vvoo", 0, 1, 3, 2, o_P2);     // [c][d][j][i] = U(c,d,i,j)
+  ublk("vovo", 3, 0, 1, 2, o_ovP);    // [c][k][b][i] = U(k,b,i,c)
+  ublk("vvvv", 0, 1, 2, 3, o_V4);
+  ublk("oooo", 0, 1, 2, 3, o_O4);
+  ublk("vooo", 3, 0, 1, 2, o_ooov);   // [b][i][j][k] = U(i,j,k,b)
+  ublk("vvvo", 1, 2, 3, 0, o_ovvv);   // [a][c][d][j] = U(j,a,c,d)
+  ublk("ovoo", 0, 1, 2, 3, o_ovoo);   // [k][b][i][j] = U(k,b,i,j)
+  ublk("vvov", 0, 1, 2, 3, o_vvov);   // [a][b][i][c] = U(a,b,i,c)
+  pcc_tt2_small_kernel<<<dim3((unsigned)((S + 255) / 256), (unsigned)cg), 256, 0, s>>>(nso, N, U, 2 * n4, F, sc, C1,
+                                                                                       w(o_C2), n4);
+  const ZMat Foo{F, 2 * n1, 2, 2 * n2, 1, 0};
+  const ZMat Fvv{F + 2 * (O * n1 + O), 2 * n1, 2, 2 * n2, 1, 0};
+  const ZMat Fvo{F + 2 * O * n1, 2 * n1, 2, 2 * n2, 1, 0};
+  // C1: -1/4 sum_bij vvoo(a,b,i,j) U(i,j,k,b) - 1/4 sum_cdj U(j,a,c,d) vvoo(c,d,i,j)
+  zgemm_batched(s, (int)V, N, (int)(VN * O), -0.25, 0, RM(w(o_vvoo), VN * O, n4), RM(w(o_ooov), O, n4), 1, 0,
+                RO(C1, O, VN), (int)cg);
+  zgemm_batched(s, (int)V, N, (int)(V * VN), -0.25, 0, RM(w(o_ovvv), V * VN, n4), RM(w(o_P2), O, n4), 1, 0,
+                RO(C1, O, VN), (int)cg);
+  // C2: 1/2 sum_kc vvoo(a,c,j,k) U(k,b,i,c) as X [a][j][b][i]
+  zgemm_batched(s, (int)VN, (int)VN, (int)VN, 1, 0, RM(w(o_P1), VN, n4), RM(w(o_ovP), VN, n4), 0, 0,
+                RO(w(o_X), VN, n4), (int)cg);
+  gather(s, (int)V, (int)V, N, N, {pterm(w(o_X), n4, O * VN, O, 1, VN, 0.5)}, true, w(o_C2), n4, cg);
+  // 1/16 (U_vvvv vvoo + vvoo U_oooo)
+  zgemm_batched(s, (int)(V * V), (int)(O * O), (int)(V * V), 1.0 / 16, 0, RM(w(o_V4), V * V, n4),
+                RM(w(o_vvoo), O * O, n4), 1, 0, RO(w(o_C2), O * O, n4), (int)cg);
+  zgemm_batched(s, (int)(V * V), (int)(O * O), (int)(O * O), 1.0 / 16, 0, RM(w(o_vvoo), O * O, n4),
+                RM(w(o_O4), O * O, n4), 1, 0, RO(w(o_C2), O * O, n4), (int)cg);
+  // -1/4 vvoo(a,b,i,k) foo(k,j) + 1/4 fvv(a,c) vvoo(c,b,i,j) - 1/4 fvo(a,k) U(k,b,i,j) + 1/4 U(a,b,i,c) fvo(c,j)
+  zgemm_batched(s, (int)(V * VN), N, N, -0.25, 0, RM(w(o_vvoo), O, n4), Foo, 1, 0, RO(w(o_C2), O, n4), (int)cg);
+  zgemm_batched(s, (int)V, (int)(VN * O), (int)V, 0.25, 0, Fvv, RM(w(o_vvoo), VN * O, n4), 1, 0,
+                RO(w(o_C2), VN * O, n4), (int)cg);
+  zgemm_batched(s, (int)V, (int)(VN * O), N, -0.25, 0, Fvo, RM(w(o_ovoo), VN * O, n4), 1, 0, RO(w(o_C2), VN * O, n4),
+                (int)cg);
+  zgemm_batched(s, (int)(V * VN), N, (int)V, 0.25, 0, RM(w(o_vvov), V, n4), Fvo, 1, 0, RO(w(o_C2), O, n4), (int)cg);
+  return o_C2;
 }
 
 }  // namespace
@@ -732,14 +809,15 @@ static int pcc_run(phf_ctx* ctx, const double* hone, const double* htwo, const d
   const long sd_point = sd ? 16 * (12 * S + (long)V * N * N * N + 2 * N4 + 2L * V * V + 2L * N * N + 3L * V * N + 1) +
                                  23 * 256
                            : 0;
-  const long per_point = 16 * (2 * n4 + 5 * n2 + (long)V * V * N * N + 2L * N * N + 2L * V * N) + 4 * 256 + sd_point;
+  const long per_point =
+      16 * (2 * n4 + 5 * n2 + (long)V * V * N * N + 2L * N * N + 3L * V * N + 2) + 6 * 256 + sd_point;
   // (the batched index transforms put chunk * V^2 problems on the grid's z dimension)
   const long chunk = std::max<long>(1, std::min<long>({G1, (4L << 30) / per_point, 65535L / ((long)V * V)}));
   Arena a0{nullptr};
   auto carve = [&](Arena& a) {
     struct B {
       double *h1, *h2, *t1, *t2, *r1, *r2, *rpg, *rk, *w, *lam, *eps, *out, *fov, *escf;
-      double *R, *L, *Y, *Linv, *detL, *U1, *Vm, *Vinv, *detV, *bufA, *bufB, *F, *T, *tmp;
+      double *R, *L, *Y, *Linv, *detL, *U1, *Vm, *Vinv, *detV, *bufA, *bufB, *F, *T, *tmp, *sc, *C1;
       int *infoL, *infoV;
     } b;
     b.h1 = a.take<double>(2 * n2);
@@ -769,6 +847,8 @@ static int pcc_run(phf_ctx* ctx, const double* hone, const double* htwo, const d
     b.bufA = a.take<double>(2 * n4 * chunk);
     b.bufB = a.take<double>(2 * n4 * chunk);
     b.F = a.take<double>(2 * n2 * chunk);
+    b.sc = a.take<double>(4 * chunk);
+    b.C1 = a.take<double>(2L * V * N * chunk);
     b.T = a.take<double>(2L * V * V * N * N * chunk);
     b.infoL = a.take<int>(chunk);
     b.infoV = a.take<int>(chunk);
@@ -796,6 +876,19 @@ static int pcc_run(phf_ctx* ctx, const double* hone, const double* htwo, const d
   }
   hipStream_t s = ctx->stream;
   if (sd) fill_zero(s, sdb.sum, 4 * ((long)V * N + S));
+  // profiling: [0] start / end of the device work, [1..2] zeroth-order
+  // stages of a chunk, [3..4] residual, [5..6] ladder GEMM, [7..8] grid sums
+  hipEvent_t ev[9] = {};
+  double tms[5] = {0, 0, 0, 0, 0};
+  long nladder = 0;
+  const bool prof = ctx->profiling;
+  if (prof)
+    for (auto& e : ev) HIPCK(hipEventCreateWithFlags(&e, kTimingEventFlags));
+  auto span = [&](int a, int b) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ev[a], ev[b]);
+    return (double)ms;
+  };
   const int noa = N / 2, nva = nso / 2 - noa;   // the ProjSz spin blocks (SP = 2)
   auto up = [&](double* d, const double* h, long nd) {
     return hipMemcpyAsync(d, h, sizeof(double) * nd, hipMemcpyHostToDevice, s);
@@ -811,6 +904,7 @@ static int pcc_run(phf_ctx* ctx, const double* hone, const double* htwo, const d
   HIPCK(up(b.w, w.data(), 2 * G));
   std::vector<int> hinfo(2 * chunk);
   const long VN = (long)V * N, V2 = (long)V * V;
+  if (prof) HIPCK(hipEventRecord(ev[0], s));
   for (int pass = 0; pass < ncik; ++pass) {
     const int cj = pass == 1;
     pcc_fock_kernel<<<1, kThreads, 0, s>>>(nso, N, b.h1, b.h2, enuc, cj, b.fov, b.escf);
@@ -819,6 +913,7 @@ static int pcc_run(phf_ctx* ctx, const double* hone, const double* htwo, const d
     const ZMat T2m{b.t2, 2L * N, 2, 0, 1, cj};
     for (long g0 = 0; g0 < G1; g0 += chunk) {
       const long cg = std::min<long>(chunk, G1 - g0);
+      if (prof) HIPCK(hipEventRecord(ev[1], s));
       // R_g = R1[ia] R2[iy] Rpg[ipg] (Rk(2)): row-major inputs, column-major products
       // tmp = R1 R2 over (ia, iy) pairs of this chunk's points, then tmp Rpg, then Rk
       for (long q = 0; q < cg; ++q) {
@@ -861,20 +956,35 @@ static int pcc_run(phf_ctx* ctx, const double* hone, const double* htwo, const d
                     ZMat{b.bufA, 2 * n2 * nso, 2, 2L * V * n2 * nso, 1, 0}, 0, 0,
                     ZOut{b.bufB, 2 * n2 * nso, 2, 2 * n4, 1}, (int)cg);
       const long goff = pass * G1 + g0;
-      pcc_transt2_kernel<<<(unsigned)cg, kThreads, 0, s>>>(nso, N, b.bufB, 2 * n4, b.U1, b.detL, b.fov, b.escf,
-                                                            b.h2, cj, b.F, b.T, b.lam + 2 * goff, b.eps + 2 * goff,
-                                                            sd ? sdb.W1 : nullptr, sd ? sdb.W2 : nullptr,
-                                                            sd ? sdb.E0 : nullptr);
+      // TransT2's second order (bufB holds U; bufA is free scratch), CI2CC, DSTT2, CCEnergy
+      pcc_tt2_prep_kernel<<<(unsigned)cg, kThreads, 0, s>>>(nso, N, b.bufB, 2 * n4, b.F, b.sc);
+      const long oc2 = pcc_transt2_gemm(s, nso, N, cg, b.bufB, b.bufA, b.F, b.sc, b.C1);
+      pcc_tt2_finish_kernel<<<(unsigned)cg, kThreads, 0, s>>>(
+          nso, N, b.C1, b.bufA + 2 * oc2, n4, b.sc, b.U1, b.detL, b.fov, b.escf, b.h2, cj, b.T, b.lam + 2 * goff,
+          b.eps + 2 * goff, sd ? sdb.W1 : nullptr, sd ? sdb.W2 : nullptr, sd ? sdb.E0 : nullptr);
+      if (prof) HIPCK(hipEventRecord(ev[2], s));
       if (sd) {
-        pcc_residual(s, V, N, cg, sh, sdb);
+        if (prof) HIPCK(hipEventRecord(ev[3], s));
+        pcc_residual(s, V, N, cg, sh, sdb, prof ? ev + 5 : nullptr);
+        if (prof) HIPCK(hipEventRecord(ev[4], s));
         pcc_sd_sum_kernel<<<(unsigned)((S + (long)V * N + 255) / 256), 256, 0, s>>>(
             V, N, cg, sp, noa, nva, b.w + 2 * goff, b.lam + 2 * goff, sdb.E0, sdb.W1, sdb.W2, sdb.r1, sdb.R2,
             sdb.sum);
+        if (prof) HIPCK(hipEventRecord(ev[7], s));
       }
       HIPCK(hipGetLastError());
       HIPCK(hipMemcpyAsync(hinfo.data(), b.infoL, sizeof(int) * cg, hipMemcpyDeviceToHost, s));
       HIPCK(hipMemcpyAsync(hinfo.data() + chunk, b.infoV, sizeof(int) * cg, hipMemcpyDeviceToHost, s));
       HIPCK(hipStreamSynchronize(s));
+      if (prof) {
+        tms[1] += span(1, 2);
+        if (sd) {
+          tms[2] += span(3, 4);
+          tms[3] += span(5, 6);
+          tms[4] += span(4, 7);
+          ++nladder;
+        }
+      }
       for (long q = 0; q < cg; ++q)
         if (hinfo[q] != 0 || hinfo[chunk + q] != 0)
           return phf_fail(ctx, PHF_ERR_SINGULAR,
@@ -884,6 +994,15 @@ static int pcc_run(phf_ctx* ctx, const double* hone, const double* htwo, const d
   }
   pcc_gridsum_kernel<<<1, kThreads, 0, s>>>(G, b.w, b.lam, b.eps, b.out);
   HIPCK(hipGetLastError());
+  if (prof) {
+    HIPCK(hipEventRecord(ev[8], s));
+    HIPCK(hipEventSynchronize(ev[8]));
+    tms[0] = span(0, 8);
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (int k = 0; k < 5; ++k) ctx->stage_ms[k] = tms[k];
+    ctx->stage_ms[5] = (double)nladder;
+    ctx->nstage = 6;
+  }
   double out[4];
   HIPCK(hipMemcpyAsync(out, b.out, sizeof out, hipMemcpyDeviceToHost, s));
   HIPCK(hipStreamSynchronize(s));
