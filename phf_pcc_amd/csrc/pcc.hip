@@ -180,13 +180,14 @@ __global__ void __launch_bounds__(kThreads) pcc_tt2_prep_kernel(int nso, int N, 
 }
 
 // Elementwise parts: C2 = vvoo (1 + E0) / 4 - fvo(a,j) fvo(b,i) / 2 for the
-// threads of [0, V^2 N^2); for [0, V N) also
-// C1 = fvo (1 + E0) + (fvv fvo - fvo foo) / 2 + sum_bj vvoo(a,b,i,j) fov(j,b) / 2
-//      - sum_jb U(j,a,i,b) fvo(b,j) / 2.  grid (ceil(V^2 N^2 / 256), points).
+// threads of [0, V^2 N^2); for [0, V N) also C1 = fvo (1 + E0) + (fvv fvo -
+// fvo foo) / 2 and the compact copies fovT [b][j] = fov(j,b), fvoC [b][j] =
+// fvo(b,j) for the C1 products.  grid (ceil(V^2 N^2 / 256), points).
 __global__ void __launch_bounds__(256) pcc_tt2_small_kernel(int nso, int N, const double* __restrict__ Ubuf,
                                                             long ustride, const double* __restrict__ Fbuf,
                                                             const double* __restrict__ sc, double* __restrict__ C1,
-                                                            double* __restrict__ C2, long c2stride) {
+                                                            double* __restrict__ C2, long c2stride,
+                                                            double* __restrict__ fovT, double* __restrict__ fvoC) {
   const int V = nso - N;
   const long S = (long)V * V * N * N;
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -210,12 +211,10 @@ __global__ void __launch_bounds__(256) pcc_tt2_small_kernel(int nso, int N, cons
     double2 h = make_double2(0.0, 0.0);
     for (int c = 0; c < V; ++c) h = cadd(h, cmul(Fe(N + a, N + c), Fe(N + c, i)));
     for (int k = 0; k < N; ++k) h = csub(h, cmul(Fe(N + a, k), Fe(k, i)));
-    for (int b = 0; b < V; ++b)
-      for (int j = 0; j < N; ++j) {
-        h = cadd(h, cmul(Ue(N + a, N + b, i, j), Fe(j, N + b)));
-        h = csub(h, cmul(Ue(j, N + a, i, N + b), Fe(N + b, j)));
-      }
     reinterpret_cast<double2*>(C1)[g * V * N + idx] = cadd(acc, cscale(h, 0.5));
+    // (a, i) read as (b, j)
+    reinterpret_cast<double2*>(fovT)[g * V * N + idx] = Fe(i, N + a);
+    reinterpret_cast<double2*>(fvoC)[g * V * N + idx] = Fe(N + a, i);
   }
 }
 
@@ -693,7 +692,7 @@ void pcc_residual(hipStream_t s, int V, int N, long cg, const PccShared& h, cons
 // F and sc come from pcc_tt2_prep_kernel; C1 (V N per point) and C2 start
 // from pcc_tt2_small_kernel's elementwise terms.  Returns C2's offset in W.
 long pcc_transt2_gemm(hipStream_t s, int nso, int N, long cg, const double* U, double* W, const double* F,
-                      const double* sc, double* C1) {
+                      const double* sc, double* C1, double* fovT, double* fvoC) {
   const long V = nso - N, O = N, S = V * V * O * O, VN = V * O;
   const long n1 = nso, n2 = n1 * nso, n3 = n2 * nso, n4 = n2 * n2;
   const long o_vvoo = 0, o_P1 = S, o_P2 = 2 * S, o_ovP = 3 * S, o_C2 = 4 * S, o_X = 5 * S, o_V4 = 6 * S;
@@ -724,11 +723,17 @@ long pcc_transt2_gemm(hipStream_t s, int nso, int N, long cg, const double* U, d
   ublk("ovoo", 0, 1, 2, 3, o_ovoo);   // [k][b][i][j] = U(k,b,i,j)
   ublk("vvov", 0, 1, 2, 3, o_vvov);   // [a][b][i][c] = U(a,b,i,c)
   pcc_tt2_small_kernel<<<dim3((unsigned)((S + 255) / 256), (unsigned)cg), 256, 0, s>>>(nso, N, U, 2 * n4, F, sc, C1,
-                                                                                       w(o_C2), n4);
+                                                                                       w(o_C2), n4, fovT, fvoC);
   const ZMat Foo{F, 2 * n1, 2, 2 * n2, 1, 0};
   const ZMat Fvv{F + 2 * (O * n1 + O), 2 * n1, 2, 2 * n2, 1, 0};
   const ZMat Fvo{F + 2 * O * n1, 2 * n1, 2, 2 * n2, 1, 0};
-  // C1: -1/4 sum_bij vvoo(a,b,i,j) U(i,j,k,b) - 1/4 sum_cdj U(j,a,c,d) vvoo(c,d,i,j)
+  // C1: 1/2 sum_bj vvoo(a,b,i,j) fov(j,b) - 1/2 sum_bj U(j,a,i,b) fvo(b,j) as matrix-vector products over
+  // (b, j) (P1 read as [a][i][b][j], ovP as [b][j][a][i]), then
+  // -1/4 sum_bij vvoo(a,b,i,j) U(i,j,k,b) - 1/4 sum_cdj U(j,a,c,d) vvoo(c,d,i,j)
+  zgemm_batched(s, (int)VN, 1, (int)VN, 0.5, 0, RM(w(o_P1), VN, n4), ZMat{fovT, 2, 0, 2 * VN, 1, 0}, 1, 0,
+                ZOut{C1, 2, 0, 2 * VN, 1}, (int)cg);
+  zgemm_batched(s, (int)VN, 1, (int)VN, -0.5, 0, ZMat{w(o_ovP), 2, 2 * VN, 2 * n4, 1, 0},
+                ZMat{fvoC, 2, 0, 2 * VN, 1, 0}, 1, 0, ZOut{C1, 2, 0, 2 * VN, 1}, (int)cg);
   zgemm_batched(s, (int)V, N, (int)(VN * O), -0.25, 0, RM(w(o_vvoo), VN * O, n4), RM(w(o_ooov), O, n4), 1, 0,
                 RO(C1, O, VN), (int)cg);
   zgemm_batched(s, (int)V, N, (int)(V * VN), -0.25, 0, RM(w(o_ovvv), V * VN, n4), RM(w(o_P2), O, n4), 1, 0,
@@ -810,14 +815,15 @@ static int pcc_run(phf_ctx* ctx, const double* hone, const double* htwo, const d
                                  23 * 256
                            : 0;
   const long per_point =
-      16 * (2 * n4 + 5 * n2 + (long)V * V * N * N + 2L * N * N + 3L * V * N + 2) + 6 * 256 + sd_point;
+      16 * (2 * n4 + 3 * n2 + (long)V * V * N * N + 2L * N * N + 5L * V * N + 2) + 8 * 256 + sd_point;
   // (the batched index transforms put chunk * V^2 problems on the grid's z dimension)
   const long chunk = std::max<long>(1, std::min<long>({G1, (4L << 30) / per_point, 65535L / ((long)V * V)}));
   Arena a0{nullptr};
   auto carve = [&](Arena& a) {
     struct B {
       double *h1, *h2, *t1, *t2, *r1, *r2, *rpg, *rk, *w, *lam, *eps, *out, *fov, *escf;
-      double *R, *L, *Y, *Linv, *detL, *U1, *Vm, *Vinv, *detV, *bufA, *bufB, *F, *T, *tmp, *sc, *C1;
+      double *L, *Y, *Linv, *detL, *U1, *Vm, *Vinv, *detV, *bufA, *bufB, *F, *T, *sc, *C1, *fovT, *fvoC;
+      double *P12, *Rpgk, *Rall;
       int *infoL, *infoV;
     } b;
     b.h1 = a.take<double>(2 * n2);
@@ -834,8 +840,6 @@ static int pcc_run(phf_ctx* ctx, const double* hone, const double* htwo, const d
     b.out = a.take<double>(4);
     b.fov = a.take<double>(2L * N * V);
     b.escf = a.take<double>(2);
-    b.R = a.take<double>(2 * n2 * chunk);
-    b.tmp = a.take<double>(2 * n2 * chunk);
     b.L = a.take<double>(2L * N * N * chunk);
     b.Linv = a.take<double>(2L * N * N * chunk);
     b.detL = a.take<double>(2 * chunk);
@@ -849,6 +853,11 @@ static int pcc_run(phf_ctx* ctx, const double* hone, const double* htwo, const d
     b.F = a.take<double>(2 * n2 * chunk);
     b.sc = a.take<double>(4 * chunk);
     b.C1 = a.take<double>(2L * V * N * chunk);
+    b.fovT = a.take<double>(2L * V * N * chunk);
+    b.fvoC = a.take<double>(2L * V * N * chunk);
+    b.P12 = a.take<double>(2 * n2 * na * ny);
+    b.Rpgk = a.take<double>(2 * n2 * npg);
+    b.Rall = a.take<double>(2 * n2 * G1);
     b.T = a.take<double>(2L * V * V * N * N * chunk);
     b.infoL = a.take<int>(chunk);
     b.infoV = a.take<int>(chunk);
@@ -911,33 +920,26 @@ static int pcc_run(phf_ctx* ctx, const double* hone, const double* htwo, const d
     if (sd) build_shared(s, nso, N, b.h1, b.h2, cj, sh);
     // T2 as a (V^2 N) x N row-major matrix, conjugated in pass 2
     const ZMat T2m{b.t2, 2L * N, 2, 0, 1, cj};
+    // R_g = R1[ia] R2[iy] Rpg[ipg] (Rk(2)) for every point g = (ia ny + iy) npg + ipg, column-major, in
+    // three batched launches: P12 over the (ia, iy) pairs, Rpg Rk (second pass), then P12 Rpg
+    zgemm_batched(s, nso, nso, nso, 1, 0, ZMat{b.r1, 2L * nso, 2, 2 * n2, 1, 0, 0, ny},
+                  ZMat{b.r2, 2L * nso, 2, 0, 1, 0, 2 * n2, ny}, 0, 0, zout_colmajor(b.P12, nso, 2 * n2), na * ny);
+    ZMat Bpg{b.rpg, 2L * nso, 2, 0, 1, 0, 2 * n2, npg};   // row-major Rpg[ipg]
+    if (cj) {
+      zgemm_batched(s, nso, nso, nso, 1, 0, ZMat{b.rpg, 2L * nso, 2, 2 * n2, 1, 0}, ZMat{b.rk, 2L * nso, 2, 0, 1, 0},
+                    0, 0, zout_colmajor(b.Rpgk, nso, 2 * n2), npg);
+      Bpg = ZMat{b.Rpgk, 2, 2L * nso, 0, 1, 0, 2 * n2, npg};   // column-major Rpg[ipg] Rk
+    }
+    zgemm_batched(s, nso, nso, nso, 1, 0, ZMat{b.P12, 2, 2L * nso, 2 * n2, 1, 0, 0, npg}, Bpg, 0, 0,
+                  zout_colmajor(b.Rall, nso, 2 * n2), (int)G1);
     for (long g0 = 0; g0 < G1; g0 += chunk) {
       const long cg = std::min<long>(chunk, G1 - g0);
       if (prof) HIPCK(hipEventRecord(ev[1], s));
-      // R_g = R1[ia] R2[iy] Rpg[ipg] (Rk(2)): row-major inputs, column-major products
-      // tmp = R1 R2 over (ia, iy) pairs of this chunk's points, then tmp Rpg, then Rk
-      for (long q = 0; q < cg; ++q) {
-        const long gg = g0 + q;
-        const long ia = gg / ((long)ny * npg), iy = (gg / npg) % ny, ipg = gg % npg;
-        const ZMat A1{b.r1 + 2 * n2 * ia, 2L * nso, 2, 0, 1, 0};
-        const ZMat B2{b.r2 + 2 * n2 * iy, 2L * nso, 2, 0, 1, 0};
-        zgemm_batched(s, nso, nso, nso, 1, 0, A1, B2, 0, 0, zout_colmajor(b.tmp + 2 * n2 * q, nso, 0), 1);
-        const ZMat Bp{b.rpg + 2 * n2 * ipg, 2L * nso, 2, 0, 1, 0};
-        if (cj) {
-          zgemm_batched(s, nso, nso, nso, 1, 0, zmat_colmajor(b.tmp + 2 * n2 * q, nso, 0), Bp, 0, 0,
-                        zout_colmajor(b.Vm + 2 * n2 * q, nso, 0), 1);
-          const ZMat Bk{b.rk, 2L * nso, 2, 0, 1, 0};
-          zgemm_batched(s, nso, nso, nso, 1, 0, zmat_colmajor(b.Vm + 2 * n2 * q, nso, 0), Bk, 0, 0,
-                        zout_colmajor(b.R + 2 * n2 * q, nso, 0), 1);
-        } else {
-          zgemm_batched(s, nso, nso, nso, 1, 0, zmat_colmajor(b.tmp + 2 * n2 * q, nso, 0), Bp, 0, 0,
-                        zout_colmajor(b.R + 2 * n2 * q, nso, 0), 1);
-        }
-      }
+      const double* R = b.Rall + 2 * n2 * g0;   // this chunk's rotations (column-major)
       // ProjT1: L, Y; L^-1 and det L = U0; U1 = Y L^-1; V = e^{-U1} R; V^-1
-      pcc_thouless_kernel<<<(unsigned)cg, kThreads, 0, s>>>(nso, N, b.R, b.t1, cj, b.L, b.Y);
+      pcc_thouless_kernel<<<(unsigned)cg, kThreads, 0, s>>>(nso, N, R, b.t1, cj, b.L, b.Y);
       lu_inverse_batched(s, N, b.L, 2L * N * N, b.Linv, 2L * N * N, b.detL, b.infoL, (int)cg);
-      pcc_vmat_kernel<<<(unsigned)cg, kThreads, 0, s>>>(nso, N, b.R, b.Y, b.Linv, b.U1, b.Vm);
+      pcc_vmat_kernel<<<(unsigned)cg, kThreads, 0, s>>>(nso, N, R, b.Y, b.Linv, b.U1, b.Vm);
       lu_inverse_batched(s, nso, b.Vm, 2 * n2, b.Vinv, 2 * n2, b.detV, b.infoV, (int)cg);
       // U = V T2 V^-1 (IntTran4Sim), index by index:
       // X1[(a,b,i)][s] = sum_j T2[(a,b,i)][j] Vinv(j,s)
@@ -958,7 +960,7 @@ static int pcc_run(phf_ctx* ctx, const double* hone, const double* htwo, const d
       const long goff = pass * G1 + g0;
       // TransT2's second order (bufB holds U; bufA is free scratch), CI2CC, DSTT2, CCEnergy
       pcc_tt2_prep_kernel<<<(unsigned)cg, kThreads, 0, s>>>(nso, N, b.bufB, 2 * n4, b.F, b.sc);
-      const long oc2 = pcc_transt2_gemm(s, nso, N, cg, b.bufB, b.bufA, b.F, b.sc, b.C1);
+      const long oc2 = pcc_transt2_gemm(s, nso, N, cg, b.bufB, b.bufA, b.F, b.sc, b.C1, b.fovT, b.fvoC);
       pcc_tt2_finish_kernel<<<(unsigned)cg, kThreads, 0, s>>>(
           nso, N, b.C1, b.bufA + 2 * oc2, n4, b.sc, b.U1, b.detL, b.fov, b.escf, b.h2, cj, b.T, b.lam + 2 * goff,
           b.eps + 2 * goff, sd ? sdb.W1 : nullptr, sd ? sdb.W2 : nullptr, sd ? sdb.E0 : nullptr);

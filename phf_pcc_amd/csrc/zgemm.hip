@@ -15,6 +15,8 @@
 // P3 = sum (ar + ai)(br + bi), then re = P1 - P2, im = P3 - P1 - P2.  The
 // imaginary part's rounding error is bounded by eps (|ar| + |ai|)(|br| + |bi|)
 // per term instead of eps (|ar bi| + |ai br|): the same order for FP64.
+#include <numeric>
+
 #include "phf_common.h"
 
 namespace {
@@ -171,6 +173,20 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double 
 void zgemm_batched(hipStream_t s, int m, int n, int k, double ar, double ai, ZMat A, ZMat B,
                    double br, double bi, ZOut C, int batch) {
   if (m <= 0 || n <= 0 || batch <= 0) return;
+  if (batch > 65535) {   // grid.z limit: slices whose starts are multiples of every operand's bdiv
+    const long l1 = std::lcm((long)A.bdiv, (long)B.bdiv), L = std::lcm(l1, (long)C.bdiv);
+    const long step = 65535 - 65535 % L;   // callers keep every bdiv (a grid dimension) below 65536
+    if (step <= 0) return;
+    for (long b0 = 0; b0 < batch; b0 += step) {
+      ZMat a = A, bb = B;
+      ZOut c = C;
+      a.p += A.boff(b0);
+      bb.p += B.boff(b0);
+      c.p += C.boff(b0);
+      zgemm_batched(s, m, n, k, ar, ai, a, bb, br, bi, c, (int)std::min<long>(step, batch - b0));
+    }
+    return;
+  }
   if (k <= 0) {
     // C = beta * C handled by a k=0 product of zeros
     k = 0;
