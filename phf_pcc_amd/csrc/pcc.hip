@@ -47,6 +47,7 @@ __device__ inline double2 cdiv(double2 a, double2 b) {
 __device__ inline double2 ld2(const double* p, long i) { return reinterpret_cast<const double2*>(p)[i]; }
 
 constexpr int kThreads = 256;
+constexpr long kKws = 4096L * 32 * 32;   // complex elements of the split-contraction workspace
 
 // Block-wide sum of one complex value per thread (fixed order: deterministic).
 __device__ double2 block_sum(double2 v, double2* red) {
@@ -593,7 +594,12 @@ void build_shared(hipStream_t s, int nso, int N, const double* h1, const double*
 // 1/2 <ab||ef> tau_efij + 1/8 tau_abmn (<mn||ef> tau_efij)
 //   - 1/2 P(ab) t_bm (<am||ef> tau_efij)
 // (CCItms' Wvvvv contracted with tau, without forming Wvvvv per point).
-void pcc_residual(hipStream_t s, int V, int N, long cg, const PccShared& h, const PccSD& b, hipEvent_t* lad) {
+void pcc_residual(hipStream_t s, int V, int N, long cg, const PccShared& h, const PccSD& b, hipEvent_t* lad,
+                  double* kws, long kws_cplx) {
+  auto zgb = [&](hipStream_t st, int m, int n, int k, double ar, double ai, ZMat A, ZMat B, double br, double bi,
+                 ZOut C, int batch) {
+    zgemm_batched_splitk(st, m, n, k, ar, ai, A, B, br, bi, C, batch, kws, kws_cplx);
+  };
   const long O = N, S = (long)V * V * N * N, VN = (long)V * N, N2 = O * O, V2 = (long)V * V, N3 = N2 * O;
   const long N4 = N2 * N2;
   const ZMat W1all{b.W1, 2, 2 * VN, 0, 1, 0};   // [(e, m)][g]: every point's U1t as one k x cg operand
@@ -608,50 +614,50 @@ void pcc_residual(hipStream_t s, int V, int N, long cg, const PccShared& h, cons
   gather(s, V, V, O, O, {pterm(h.Hvvoo, 0, V * N2, N2, O, 1)}, false, b.R2, S, cg);
   gather(s, O, O, O, O, {pterm(h.Hoooo, 0, N3, N2, O, 1)}, false, b.Weff, N4, cg);
   // Fvv (CCRes.f90:405-424)
-  zgemm_batched(s, V, V, N, -0.5, 0, RM(b.W1, N, VN), RM(h.fov, V, 0), 1, 0, RO(b.Fvv, V, V2), (int)cg);
-  zgemm_batched(s, (int)V2, (int)cg, (int)VN, 1, 0, RM(h.P[1], VN, 0), W1all, 1, 0, ZOut{b.Fvv, 2, 2 * V2, 0, 1}, 1);
-  zgemm_batched(s, V, V, (int)(VN * O), -0.5, 0, RM(b.tb, VN * O, S), RM(h.P[2], V, 0), 1, 0, RO(b.Fvv, V, V2),
+  zgb(s, V, V, N, -0.5, 0, RM(b.W1, N, VN), RM(h.fov, V, 0), 1, 0, RO(b.Fvv, V, V2), (int)cg);
+  zgb(s, (int)V2, (int)cg, (int)VN, 1, 0, RM(h.P[1], VN, 0), W1all, 1, 0, ZOut{b.Fvv, 2, 2 * V2, 0, 1}, 1);
+  zgb(s, V, V, (int)(VN * O), -0.5, 0, RM(b.tb, VN * O, S), RM(h.P[2], V, 0), 1, 0, RO(b.Fvv, V, V2),
                 (int)cg);
   // Foo (:426-441)
-  zgemm_batched(s, N, N, V, 0.5, 0, RM(h.fov, V, 0), RM(b.W1, N, VN), 1, 0, RO(b.Foo, N, N2), (int)cg);
-  zgemm_batched(s, (int)N2, (int)cg, (int)VN, 1, 0, RM(h.P[3], VN, 0), W1all, 1, 0, ZOut{b.Foo, 2, 2 * N2, 0, 1}, 1);
-  zgemm_batched(s, N, N, (int)(O * V2), 0.5, 0, RM(h.Hoovv, O * V2, 0), RM(b.tbP, N, S), 1, 0, RO(b.Foo, N, N2),
+  zgb(s, N, N, V, 0.5, 0, RM(h.fov, V, 0), RM(b.W1, N, VN), 1, 0, RO(b.Foo, N, N2), (int)cg);
+  zgb(s, (int)N2, (int)cg, (int)VN, 1, 0, RM(h.P[3], VN, 0), W1all, 1, 0, ZOut{b.Foo, 2, 2 * N2, 0, 1}, 1);
+  zgb(s, N, N, (int)(O * V2), 0.5, 0, RM(h.Hoovv, O * V2, 0), RM(b.tbP, N, S), 1, 0, RO(b.Foo, N, N2),
                 (int)cg);
   // Fov (:443-452)
-  zgemm_batched(s, (int)VN, (int)cg, (int)VN, 1, 0, RM(h.P[4], VN, 0), W1all, 1, 0, ZOut{b.Fov, 2, 2 * VN, 0, 1}, 1);
+  zgb(s, (int)VN, (int)cg, (int)VN, 1, 0, RM(h.P[4], VN, 0), W1all, 1, 0, ZOut{b.Fov, 2, 2 * VN, 0, 1}, 1);
   // Wovvo as WP [m][e][b][j] (:487-519)
-  zgemm_batched(s, (int)(O * V2), N, V, 1, 0, RM(h.P[6], V, 0), RM(b.W1, N, VN), 1, 0, RO(b.WP, N, S), (int)cg);
-  zgemm_batched(s, V, (int)(VN * O), N, 1, 0, RM(b.W1, N, VN), RM(h.P[7], VN * O, 0), 0, 0, RO(b.X, VN * O, S),
+  zgb(s, (int)(O * V2), N, V, 1, 0, RM(h.P[6], V, 0), RM(b.W1, N, VN), 1, 0, RO(b.WP, N, S), (int)cg);
+  zgb(s, V, (int)(VN * O), N, 1, 0, RM(b.W1, N, VN), RM(h.P[7], VN * O, 0), 0, 0, RO(b.X, VN * O, S),
                 (int)cg);   // X [b][m][e][j]
   gather(s, O, V, V, O, {pterm(b.X, S, V * O, O, O * V * O, 1, -1.0)}, true, b.WP, S, cg);
-  zgemm_batched(s, (int)VN, (int)VN, (int)VN, -1, 0, RM(h.P[8], VN, 0), RM(b.i4P, VN, S), 1, 0, RO(b.WP, VN, S),
+  zgb(s, (int)VN, (int)VN, (int)VN, -1, 0, RM(h.P[8], VN, 0), RM(b.i4P, VN, S), 1, 0, RO(b.WP, VN, S),
                 (int)cg);
   // res1 (:215-236)
-  zgemm_batched(s, V, N, V, 1, 0, RM(b.Fvv, V, V2), RM(b.W1, N, VN), 1, 0, RO(b.r1, N, VN), (int)cg);
-  zgemm_batched(s, V, N, N, -1, 0, RM(b.W1, N, VN), RM(b.Foo, N, N2), 1, 0, RO(b.r1, N, VN), (int)cg);
-  zgemm_batched(s, (int)VN, 1, (int)VN, 1, 0, RM(b.Y2, VN, S), ZMat{b.Fov, 2, 0, 2 * VN, 1, 0}, 1, 0,
+  zgb(s, V, N, V, 1, 0, RM(b.Fvv, V, V2), RM(b.W1, N, VN), 1, 0, RO(b.r1, N, VN), (int)cg);
+  zgb(s, V, N, N, -1, 0, RM(b.W1, N, VN), RM(b.Foo, N, N2), 1, 0, RO(b.r1, N, VN), (int)cg);
+  zgb(s, (int)VN, 1, (int)VN, 1, 0, RM(b.Y2, VN, S), ZMat{b.Fov, 2, 0, 2 * VN, 1, 0}, 1, 0,
                 ZOut{b.r1, 2, 0, 2 * VN, 1}, (int)cg);
-  zgemm_batched(s, (int)VN, (int)cg, (int)VN, 1, 0, RM(h.P[9], VN, 0), W1all, 1, 0, ZOut{b.r1, 2, 2 * VN, 0, 1}, 1);
-  zgemm_batched(s, V, N, (int)(VN * O), -0.5, 0, RM(b.W2, VN * O, S), RM(h.P[10], N, 0), 1, 0, RO(b.r1, N, VN),
+  zgb(s, (int)VN, (int)cg, (int)VN, 1, 0, RM(h.P[9], VN, 0), W1all, 1, 0, ZOut{b.r1, 2, 2 * VN, 0, 1}, 1);
+  zgb(s, V, N, (int)(VN * O), -0.5, 0, RM(b.W2, VN * O, S), RM(h.P[10], N, 0), 1, 0, RO(b.r1, N, VN),
                 (int)cg);
-  zgemm_batched(s, V, N, (int)(O * V2), 0.5, 0, RM(h.Hvovv, O * V2, 0), RM(b.t2P, N, S), 1, 0, RO(b.r1, N, VN),
+  zgb(s, V, N, (int)(O * V2), 0.5, 0, RM(h.Hvovv, O * V2, 0), RM(b.t2P, N, S), 1, 0, RO(b.r1, N, VN),
                 (int)cg);
   // res2 [a][b][i][j], starting from <ab||ij> (:238-352)
   // (1) P(ab) t2(a,e,i,j) itm1(b,e), itm1 = Fvv - t1 Fov / 2
   gather(s, V, V, 1, 1, {pterm(b.Fvv, V2, V, 1, 0, 0)}, false, b.itm1, V2, cg);
-  zgemm_batched(s, V, V, N, -0.5, 0, RM(b.W1, N, VN), RM(b.Fov, V, VN), 1, 0, RO(b.itm1, V, V2), (int)cg);
-  zgemm_batched(s, V, (int)N2, V, 1, 0, ZMat{b.itm1, 2L * V, 2, 2 * V2, 1, 0, 0, V},
+  zgb(s, V, V, N, -0.5, 0, RM(b.W1, N, VN), RM(b.Fov, V, VN), 1, 0, RO(b.itm1, V, V2), (int)cg);
+  zgb(s, V, (int)N2, V, 1, 0, ZMat{b.itm1, 2L * V, 2, 2 * V2, 1, 0, 0, V},
                 ZMat{b.W2, 2 * N2, 2, 2 * S, 1, 0, 2 * VN * O, V}, 0, 0,
                 ZOut{b.X, 2 * N2, 2, 2 * S, 1, nullptr, 2 * VN * O, V}, (int)(cg * V));   // X [a][b][i][j]
   gather(s, V, V, O, O, {pterm(b.X, S, VN * O, N2, O, 1), pterm(b.X, S, N2, VN * O, O, 1, -1.0)}, true, b.R2, S, cg);
   // (2) -P(ij) t2(a,b,i,m) itm3(m,j), itm3 = Foo + Fov t1 / 2
   gather(s, O, O, 1, 1, {pterm(b.Foo, N2, O, 1, 0, 0)}, false, b.itm3, N2, cg);
-  zgemm_batched(s, N, N, V, 0.5, 0, RM(b.Fov, V, VN), RM(b.W1, N, VN), 1, 0, RO(b.itm3, N, N2), (int)cg);
-  zgemm_batched(s, (int)(V2 * O), N, N, 1, 0, RM(b.W2, N, S), RM(b.itm3, N, N2), 0, 0, RO(b.X, N, S), (int)cg);
+  zgb(s, N, N, V, 0.5, 0, RM(b.Fov, V, VN), RM(b.W1, N, VN), 1, 0, RO(b.itm3, N, N2), (int)cg);
+  zgb(s, (int)(V2 * O), N, N, 1, 0, RM(b.W2, N, S), RM(b.itm3, N, N2), 0, 0, RO(b.X, N, S), (int)cg);
   gather(s, V, V, O, O, {pterm(b.X, S, VN * O, N2, O, 1, -1.0), pterm(b.X, S, VN * O, N2, 1, O)}, true, b.R2, S, cg);
   // (3) P(ij)P(ab) [t2(a,e,i,m) Wovvo(m,b,e,j) - t(a,m) t(e,i) <mb||ej>]: X [a][i][b][j]
-  zgemm_batched(s, (int)VN, (int)VN, (int)VN, 1, 0, RM(b.Y2, VN, S), RM(b.WP, VN, S), 0, 0, RO(b.X, VN, S), (int)cg);
-  zgemm_batched(s, (int)VN, (int)VN, (int)VN, -1, 0, RM(b.Y1, VN, S), RM(h.P[5], VN, 0), 1, 0, RO(b.X, VN, S),
+  zgb(s, (int)VN, (int)VN, (int)VN, 1, 0, RM(b.Y2, VN, S), RM(b.WP, VN, S), 0, 0, RO(b.X, VN, S), (int)cg);
+  zgb(s, (int)VN, (int)VN, (int)VN, -1, 0, RM(b.Y1, VN, S), RM(h.P[5], VN, 0), 1, 0, RO(b.X, VN, S),
                 (int)cg);
   gather(s, V, V, O, O,
          {pterm(b.X, S, O * VN, O, VN, 1), pterm(b.X, S, O * VN, O, 1, VN, -1.0), pterm(b.X, S, O, O * VN, VN, 1, -1.0),
@@ -660,27 +666,27 @@ void pcc_residual(hipStream_t s, int V, int N, long cg, const PccShared& h, cons
   // (4) ladder and hole-hole terms: 1/2 <ab||ef> tau_efij; Weff = <mn||ij> + 1/2 <mn||ef> tau_efij
   //     + P(ij) <mn||ie> t_ej, then 1/2 tau_abmn Weff_mnij; -1/2 P(ab) t_bm (<am||ef> tau_efij)
   if (lad) (void)hipEventRecord(lad[0], s);
-  zgemm_batched(s, (int)V2, (int)N2, (int)V2, 0.5, 0, RM(h.Hvvvv, V2, 0), RM(b.tau, N2, S), 1, 0, RO(b.R2, N2, S),
+  zgb(s, (int)V2, (int)N2, (int)V2, 0.5, 0, RM(h.Hvvvv, V2, 0), RM(b.tau, N2, S), 1, 0, RO(b.R2, N2, S),
                 (int)cg);
   if (lad) (void)hipEventRecord(lad[1], s);
-  zgemm_batched(s, (int)N2, (int)N2, (int)V2, 0.5, 0, RM(h.Hoovv, V2, 0), RM(b.tau, N2, S), 1, 0,
+  zgb(s, (int)N2, (int)N2, (int)V2, 0.5, 0, RM(h.Hoovv, V2, 0), RM(b.tau, N2, S), 1, 0,
                 RO(b.Weff, N2, N4), (int)cg);
-  zgemm_batched(s, (int)N3, N, V, 1, 0, RM(h.Hooov, V, 0), RM(b.W1, N, VN), 0, 0, RO(b.Wp, N, N4), (int)cg);
+  zgb(s, (int)N3, N, V, 1, 0, RM(h.Hooov, V, 0), RM(b.W1, N, VN), 0, 0, RO(b.Wp, N, N4), (int)cg);
   gather(s, O, O, O, O, {pterm(b.Wp, N4, N3, N2, O, 1), pterm(b.Wp, N4, N3, N2, 1, O, -1.0)}, true, b.Weff, N4, cg);
-  zgemm_batched(s, (int)V2, (int)N2, (int)N2, 0.5, 0, RM(b.tau, N2, S), RM(b.Weff, N2, N4), 1, 0, RO(b.R2, N2, S),
+  zgb(s, (int)V2, (int)N2, (int)N2, 0.5, 0, RM(b.tau, N2, S), RM(b.Weff, N2, N4), 1, 0, RO(b.R2, N2, S),
                 (int)cg);
-  zgemm_batched(s, (int)VN, (int)N2, (int)V2, 1, 0, RM(h.Hvovv, V2, 0), RM(b.tau, N2, S), 0, 0, RO(b.Z, N2, VN * N2),
+  zgb(s, (int)VN, (int)N2, (int)V2, 1, 0, RM(h.Hvovv, V2, 0), RM(b.tau, N2, S), 0, 0, RO(b.Z, N2, VN * N2),
                 (int)cg);   // Z [a][m][i][j]
-  zgemm_batched(s, V, (int)N2, N, 1, 0, ZMat{b.W1, 2L * N, 2, 2 * VN, 1, 0, 0, V},
+  zgb(s, V, (int)N2, N, 1, 0, ZMat{b.W1, 2L * N, 2, 2 * VN, 1, 0, 0, V},
                 ZMat{b.Z, 2 * N2, 2, 2 * VN * N2, 1, 0, 2 * N3, V}, 0, 0,
                 ZOut{b.X, 2 * N2, 2, 2 * S, 1, nullptr, 2 * VN * O, V}, (int)(cg * V));   // Q [a][b][i][j]
   gather(s, V, V, O, O, {pterm(b.X, S, VN * O, N2, O, 1, -0.5), pterm(b.X, S, N2, VN * O, O, 1, 0.5)}, true, b.R2, S,
          cg);
   // (6) P(ij) t(e,i) <ab||ej>: X [a][b][j][i] = sum_e P11[(a,b,j), e] t(e, i)
-  zgemm_batched(s, (int)(V2 * O), N, V, 1, 0, RM(h.P[11], V, 0), RM(b.W1, N, VN), 0, 0, RO(b.X, N, S), (int)cg);
+  zgb(s, (int)(V2 * O), N, V, 1, 0, RM(h.P[11], V, 0), RM(b.W1, N, VN), 0, 0, RO(b.X, N, S), (int)cg);
   gather(s, V, V, O, O, {pterm(b.X, S, VN * O, N2, 1, O), pterm(b.X, S, VN * O, N2, O, 1, -1.0)}, true, b.R2, S, cg);
   // (7) -P(ab) t(a,m) <mb||ij>
-  zgemm_batched(s, V, (int)(VN * O), N, 1, 0, RM(b.W1, N, VN), RM(h.Hovoo, VN * O, 0), 0, 0, RO(b.X, VN * O, S),
+  zgb(s, V, (int)(VN * O), N, 1, 0, RM(b.W1, N, VN), RM(h.Hovoo, VN * O, 0), 0, 0, RO(b.X, VN * O, S),
                 (int)cg);
   gather(s, V, V, O, O, {pterm(b.X, S, VN * O, N2, O, 1, -1.0), pterm(b.X, S, N2, VN * O, O, 1)}, true, b.R2, S, cg);
 }
@@ -692,7 +698,11 @@ void pcc_residual(hipStream_t s, int V, int N, long cg, const PccShared& h, cons
 // F and sc come from pcc_tt2_prep_kernel; C1 (V N per point) and C2 start
 // from pcc_tt2_small_kernel's elementwise terms.  Returns C2's offset in W.
 long pcc_transt2_gemm(hipStream_t s, int nso, int N, long cg, const double* U, double* W, const double* F,
-                      const double* sc, double* C1, double* fovT, double* fvoC) {
+                      const double* sc, double* C1, double* fovT, double* fvoC, double* kws, long kws_cplx) {
+  auto zgb = [&](hipStream_t st, int m, int n, int k, double ar, double ai, ZMat A, ZMat B, double br, double bi,
+                 ZOut C, int batch) {
+    zgemm_batched_splitk(st, m, n, k, ar, ai, A, B, br, bi, C, batch, kws, kws_cplx);
+  };
   const long V = nso - N, O = N, S = V * V * O * O, VN = V * O;
   const long n1 = nso, n2 = n1 * nso, n3 = n2 * nso, n4 = n2 * n2;
   const long o_vvoo = 0, o_P1 = S, o_P2 = 2 * S, o_ovP = 3 * S, o_C2 = 4 * S, o_X = 5 * S, o_V4 = 6 * S;
@@ -730,30 +740,30 @@ long pcc_transt2_gemm(hipStream_t s, int nso, int N, long cg, const double* U, d
   // C1: 1/2 sum_bj vvoo(a,b,i,j) fov(j,b) - 1/2 sum_bj U(j,a,i,b) fvo(b,j) as matrix-vector products over
   // (b, j) (P1 read as [a][i][b][j], ovP as [b][j][a][i]), then
   // -1/4 sum_bij vvoo(a,b,i,j) U(i,j,k,b) - 1/4 sum_cdj U(j,a,c,d) vvoo(c,d,i,j)
-  zgemm_batched(s, (int)VN, 1, (int)VN, 0.5, 0, RM(w(o_P1), VN, n4), ZMat{fovT, 2, 0, 2 * VN, 1, 0}, 1, 0,
+  zgb(s, (int)VN, 1, (int)VN, 0.5, 0, RM(w(o_P1), VN, n4), ZMat{fovT, 2, 0, 2 * VN, 1, 0}, 1, 0,
                 ZOut{C1, 2, 0, 2 * VN, 1}, (int)cg);
-  zgemm_batched(s, (int)VN, 1, (int)VN, -0.5, 0, ZMat{w(o_ovP), 2, 2 * VN, 2 * n4, 1, 0},
+  zgb(s, (int)VN, 1, (int)VN, -0.5, 0, ZMat{w(o_ovP), 2, 2 * VN, 2 * n4, 1, 0},
                 ZMat{fvoC, 2, 0, 2 * VN, 1, 0}, 1, 0, ZOut{C1, 2, 0, 2 * VN, 1}, (int)cg);
-  zgemm_batched(s, (int)V, N, (int)(VN * O), -0.25, 0, RM(w(o_vvoo), VN * O, n4), RM(w(o_ooov), O, n4), 1, 0,
+  zgb(s, (int)V, N, (int)(VN * O), -0.25, 0, RM(w(o_vvoo), VN * O, n4), RM(w(o_ooov), O, n4), 1, 0,
                 RO(C1, O, VN), (int)cg);
-  zgemm_batched(s, (int)V, N, (int)(V * VN), -0.25, 0, RM(w(o_ovvv), V * VN, n4), RM(w(o_P2), O, n4), 1, 0,
+  zgb(s, (int)V, N, (int)(V * VN), -0.25, 0, RM(w(o_ovvv), V * VN, n4), RM(w(o_P2), O, n4), 1, 0,
                 RO(C1, O, VN), (int)cg);
   // C2: 1/2 sum_kc vvoo(a,c,j,k) U(k,b,i,c) as X [a][j][b][i]
-  zgemm_batched(s, (int)VN, (int)VN, (int)VN, 1, 0, RM(w(o_P1), VN, n4), RM(w(o_ovP), VN, n4), 0, 0,
+  zgb(s, (int)VN, (int)VN, (int)VN, 1, 0, RM(w(o_P1), VN, n4), RM(w(o_ovP), VN, n4), 0, 0,
                 RO(w(o_X), VN, n4), (int)cg);
   gather(s, (int)V, (int)V, N, N, {pterm(w(o_X), n4, O * VN, O, 1, VN, 0.5)}, true, w(o_C2), n4, cg);
   // 1/16 (U_vvvv vvoo + vvoo U_oooo)
-  zgemm_batched(s, (int)(V * V), (int)(O * O), (int)(V * V), 1.0 / 16, 0, RM(w(o_V4), V * V, n4),
+  zgb(s, (int)(V * V), (int)(O * O), (int)(V * V), 1.0 / 16, 0, RM(w(o_V4), V * V, n4),
                 RM(w(o_vvoo), O * O, n4), 1, 0, RO(w(o_C2), O * O, n4), (int)cg);
-  zgemm_batched(s, (int)(V * V), (int)(O * O), (int)(O * O), 1.0 / 16, 0, RM(w(o_vvoo), O * O, n4),
+  zgb(s, (int)(V * V), (int)(O * O), (int)(O * O), 1.0 / 16, 0, RM(w(o_vvoo), O * O, n4),
                 RM(w(o_O4), O * O, n4), 1, 0, RO(w(o_C2), O * O, n4), (int)cg);
   // -1/4 vvoo(a,b,i,k) foo(k,j) + 1/4 fvv(a,c) vvoo(c,b,i,j) - 1/4 fvo(a,k) U(k,b,i,j) + 1/4 U(a,b,i,c) fvo(c,j)
-  zgemm_batched(s, (int)(V * VN), N, N, -0.25, 0, RM(w(o_vvoo), O, n4), Foo, 1, 0, RO(w(o_C2), O, n4), (int)cg);
-  zgemm_batched(s, (int)V, (int)(VN * O), (int)V, 0.25, 0, Fvv, RM(w(o_vvoo), VN * O, n4), 1, 0,
+  zgb(s, (int)(V * VN), N, N, -0.25, 0, RM(w(o_vvoo), O, n4), Foo, 1, 0, RO(w(o_C2), O, n4), (int)cg);
+  zgb(s, (int)V, (int)(VN * O), (int)V, 0.25, 0, Fvv, RM(w(o_vvoo), VN * O, n4), 1, 0,
                 RO(w(o_C2), VN * O, n4), (int)cg);
-  zgemm_batched(s, (int)V, (int)(VN * O), N, -0.25, 0, Fvo, RM(w(o_ovoo), VN * O, n4), 1, 0, RO(w(o_C2), VN * O, n4),
+  zgb(s, (int)V, (int)(VN * O), N, -0.25, 0, Fvo, RM(w(o_ovoo), VN * O, n4), 1, 0, RO(w(o_C2), VN * O, n4),
                 (int)cg);
-  zgemm_batched(s, (int)(V * VN), N, (int)V, 0.25, 0, RM(w(o_vvov), V, n4), Fvo, 1, 0, RO(w(o_C2), O, n4), (int)cg);
+  zgb(s, (int)(V * VN), N, (int)V, 0.25, 0, RM(w(o_vvov), V, n4), Fvo, 1, 0, RO(w(o_C2), O, n4), (int)cg);
   return o_C2;
 }
 
@@ -823,7 +833,7 @@ static int pcc_run(phf_ctx* ctx, const double* hone, const double* htwo, const d
     struct B {
       double *h1, *h2, *t1, *t2, *r1, *r2, *rpg, *rk, *w, *lam, *eps, *out, *fov, *escf;
       double *L, *Y, *Linv, *detL, *U1, *Vm, *Vinv, *detV, *bufA, *bufB, *F, *T, *sc, *C1, *fovT, *fvoC;
-      double *P12, *Rpgk, *Rall;
+      double *P12, *Rpgk, *Rall, *kws;
       int *infoL, *infoV;
     } b;
     b.h1 = a.take<double>(2 * n2);
@@ -858,6 +868,7 @@ static int pcc_run(phf_ctx* ctx, const double* hone, const double* htwo, const d
     b.P12 = a.take<double>(2 * n2 * na * ny);
     b.Rpgk = a.take<double>(2 * n2 * npg);
     b.Rall = a.take<double>(2 * n2 * G1);
+    b.kws = a.take<double>(2 * kKws);   // split-contraction partials
     b.T = a.take<double>(2L * V * V * N * N * chunk);
     b.infoL = a.take<int>(chunk);
     b.infoV = a.take<int>(chunk);
@@ -960,14 +971,14 @@ static int pcc_run(phf_ctx* ctx, const double* hone, const double* htwo, const d
       const long goff = pass * G1 + g0;
       // TransT2's second order (bufB holds U; bufA is free scratch), CI2CC, DSTT2, CCEnergy
       pcc_tt2_prep_kernel<<<(unsigned)cg, kThreads, 0, s>>>(nso, N, b.bufB, 2 * n4, b.F, b.sc);
-      const long oc2 = pcc_transt2_gemm(s, nso, N, cg, b.bufB, b.bufA, b.F, b.sc, b.C1, b.fovT, b.fvoC);
+      const long oc2 = pcc_transt2_gemm(s, nso, N, cg, b.bufB, b.bufA, b.F, b.sc, b.C1, b.fovT, b.fvoC, b.kws, kKws);
       pcc_tt2_finish_kernel<<<(unsigned)cg, kThreads, 0, s>>>(
           nso, N, b.C1, b.bufA + 2 * oc2, n4, b.sc, b.U1, b.detL, b.fov, b.escf, b.h2, cj, b.T, b.lam + 2 * goff,
           b.eps + 2 * goff, sd ? sdb.W1 : nullptr, sd ? sdb.W2 : nullptr, sd ? sdb.E0 : nullptr);
       if (prof) HIPCK(hipEventRecord(ev[2], s));
       if (sd) {
         if (prof) HIPCK(hipEventRecord(ev[3], s));
-        pcc_residual(s, V, N, cg, sh, sdb, prof ? ev + 5 : nullptr);
+        pcc_residual(s, V, N, cg, sh, sdb, prof ? ev + 5 : nullptr, b.kws, kKws);
         if (prof) HIPCK(hipEventRecord(ev[4], s));
         pcc_sd_sum_kernel<<<(unsigned)((S + (long)V * N + 255) / 256), 256, 0, s>>>(
             V, N, cg, sp, noa, nva, b.w + 2 * goff, b.lam + 2 * goff, sdb.E0, sdb.W1, sdb.W2, sdb.r1, sdb.R2,

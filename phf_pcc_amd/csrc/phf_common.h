@@ -59,6 +59,12 @@ inline ZOut zout_colmajor_t(double* p, long ld, long bs) { return ZOut{p, 2 * ld
 // m x k times k x n; `batch` problems (blockIdx.z).
 void zgemm_batched(hipStream_t s, int m, int n, int k, double ar, double ai, ZMat A, ZMat B,
                    double br, double bi, ZOut C, int batch);
+// The same product; for small outputs (m, n <= 32) with long contractions
+// (k >= 512) the contraction is split into d slices (d divides k, batch d <=
+// 4096, batch d m n <= ws_cplx) whose partials go to ws (complex elements)
+// and are summed in a fixed order; A and B must not use two-level batching.
+void zgemm_batched_splitk(hipStream_t s, int m, int n, int k, double ar, double ai, ZMat A, ZMat B, double br,
+                          double bi, ZOut C, int batch, double* ws, long ws_cplx);
 
 // One output tile (or one part of a split contraction) of a 16-bit ERI GEMM in
 // a grouped launch: rows mt*BM.. of P against rows nt*BN.. of A over the slabs

@@ -168,7 +168,56 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int m, int n, int k, double 
   }
 }
 
+// C[b] = alpha sum_s W[b d + s] + beta C[b] for m x n row-major partials W
+// (fixed summation order).  grid (ceil(m n / 256), batch).
+__global__ void __launch_bounds__(256) zsplit_reduce_kernel(int m, int n, int d, double ar, double ai, double br,
+                                                            double bi, const double* __restrict__ W, ZOut C) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= m * n) return;
+  const long b = blockIdx.y;
+  const int i = idx / n, j = idx % n;
+  double sr = 0.0, si = 0.0;
+  for (int q = 0; q < d; ++q) {
+    const double* w = W + 2 * ((b * d + q) * (long)m * n + idx);
+    sr += w[0];
+    si += w[1];
+  }
+  const long off = C.boff(b) + (long)i * C.rs + (long)j * C.cs;
+  double vr = ar * sr - ai * si, vi = ar * si + ai * sr;
+  if (br != 0.0 || bi != 0.0) {
+    const double orr = C.p[off], oi = C.p[off + C.im];
+    vr += br * orr - bi * oi;
+    vi += br * oi + bi * orr;
+  }
+  C.p[off] = vr;
+  C.p[off + C.im] = vi;
+}
+
 }  // namespace
+
+void zgemm_batched_splitk(hipStream_t s, int m, int n, int k, double ar, double ai, ZMat A, ZMat B, double br,
+                          double bi, ZOut C, int batch, double* ws, long ws_cplx) {
+  int d = 1;
+  if (ws && m <= 32 && n <= 32 && k >= 512 && A.bdiv == 1 && B.bdiv == 1)
+    for (int q = 32; q > 1; --q)
+      if (k % q == 0 && k / q >= 128 && (long)batch * q <= 4096 && (long)batch * q * m * n <= ws_cplx) {
+        d = q;
+        break;
+      }
+  if (d == 1) {
+    zgemm_batched(s, m, n, k, ar, ai, A, B, br, bi, C, batch);
+    return;
+  }
+  const int ks = k / d;
+  ZMat a = A, bb = B;
+  a.bs2 = (long)ks * A.cs;   // entry b d + q: batch b, contraction slice q
+  a.bdiv = d;
+  bb.bs2 = (long)ks * B.rs;
+  bb.bdiv = d;
+  zgemm_batched(s, m, n, ks, 1, 0, a, bb, 0, 0, ZOut{ws, 2L * n, 2, 2L * m * n, 1}, batch * d);
+  dim3 grid((unsigned)((m * n + 255) / 256), (unsigned)batch);
+  zsplit_reduce_kernel<<<grid, 256, 0, s>>>(m, n, d, ar, ai, br, bi, ws, C);
+}
 
 void zgemm_batched(hipStream_t s, int m, int n, int k, double ar, double ai, ZMat A, ZMat B,
                    double br, double bi, ZOut C, int batch) {
