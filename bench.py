@@ -622,6 +622,24 @@ def gemm_kernel_name(prec, report):
     return names[prec]
 
 
+def x_group_kernel_name(prec, report):
+    """Kernel of the exchange blocks' grouped launch, from the tuner's choice:
+    the 256 x 256 tile (round-to-nearest 16-bit, exact int8) or the blocks'
+    own tile in its grouped form."""
+    t = _tname(prec)
+    for line in (report or "").splitlines():
+        try:
+            rec = json.loads(line)
+        except ValueError:
+            continue
+        ch = rec.get("exchange_blocks", {}).get("chosen")
+        if ch == "grouped_256x256":
+            return "eri_gemm_i8_grouped_kernel" if prec == "i8x4" else f"eri_gemm_rn16_grouped_kernel<{t}>"
+        if ch == "grouped":
+            return f"eri_gemm_h16_grouped_kernel<{t}>"
+    return None
+
+
 def load_traffic(kernel, n):
     """HBM bytes per launch of `kernel` at this workload from the committed
     rocprofv3 counter summary (profiles/traffic.json), if present.  The key
@@ -797,6 +815,7 @@ def main():
         if ms is None:
             if q == 2 and kernels and kernels[1]["ms_per_launch"] is not None:
                 kernels[1]["launch"] = "X1 + X2 (one grouped launch)"
+                kernels[1]["kernel"] = x_group_kernel_name(best["prec"], ctx.tuning_report())
                 kernels[1]["algorithmic_flops_per_launch"] += alg
                 k1 = kernels[1]
                 k1["achieved_tflops"] = k1["algorithmic_flops_per_launch"] / (k1["ms_per_launch"] * 1e-3) / 1e12

@@ -1798,14 +1798,20 @@ int eval_gemm(phf_ctx* ctx, const EvalBufs& b) {
     // one grouped persistent launch.  The first call of a (precision, rows)
     // times that against the two launches and keeps the faster.
     const bool can_group = nb == 3 && choice[1] == choice[2];
+    // 0: two launches; 1: one grouped launch with the blocks' tuned tile; 2: one grouped launch with the
+    // 256 x 256 tile that serves this precision (the blocks' 2 x 112 tiles at n = 64 then nearly fill the CUs)
     int grouped = 0;
-    if (can_group) {
-      const int cfg = choice[1].first, sk = choice[1].second;
-      const int grid = h16_grouped_grid(cfg, ctx->ncu);
-      // the work list is rebuilt every call (a few hundred items) and
-      // uploaded only when it differs from the one on the device
+    int big = -1;
+    if (can_group && h16_forced_config() < 0)
+      for (int c = 0; c < h16_num_configs(); ++c)
+        if (h16_cfg_bm(c) == 256 && h16_cfg_bn(c) == 256 && h16_cfg_serves(c, precision, k_of(1)) &&
+            rows_of(1) % 256 == 0)
+          big = c;
+    auto sel = [&](int g) { return g == 2 ? std::make_pair(big, 1) : choice[1]; };
+    // the work list is rebuilt per call (a few hundred items) and uploaded only when it differs
+    auto upload = [&](int cfg, int sk) -> int {
       std::vector<H16Item> items;
-      build_h16_items({{ctx, &b}}, precision, cfg, sk, grid, items, 6u);
+      build_h16_items({{ctx, &b}}, precision, cfg, sk, h16_grouped_grid(cfg, ctx->ncu), items, 6u);
       if (items.size() != ctx->h_xitems.size() ||
           std::memcmp(items.data(), ctx->h_xitems.data(), items.size() * sizeof(H16Item)) != 0) {
         if (items.size() > ctx->xitems_cap) {
@@ -1817,26 +1823,39 @@ int eval_gemm(phf_ctx* ctx, const EvalBufs& b) {
         HIPCK(hipMemcpy(ctx->d_xitems, items.data(), items.size() * sizeof(H16Item), hipMemcpyHostToDevice));
         ctx->h_xitems.swap(items);
       }
-      auto run_x = [&](int g) {
-        if (sk > 1) fill_zero(s, b.Gam, b.Mp * nrpad);
-        if (g)
-          eri_gemm_mixed_grouped(s, precision, ctx->d_xitems, (int)ctx->h_xitems.size(), cfg, grid, sk > 1);
-        else
-          for (int q = 1; q < 3; ++q) run_block(q, cfg, sk, s);
-      };
+      return PHF_OK;
+    };
+    auto run_grouped = [&](int cfg, int sk, hipStream_t sq) {
+      eri_gemm_mixed_grouped(sq, precision, ctx->d_xitems, (int)ctx->h_xitems.size(), cfg,
+                             h16_grouped_grid(cfg, ctx->ncu), sk > 1);
+    };
+    if (can_group) {
       const auto gkey = std::make_tuple(precision, 2 * G, rows_of(1), k_of(1));
       auto it = ctx->xgroup.find(gkey);
       if (it == ctx->xgroup.end()) {
-        float t[2] = {1e30f, 1e30f};
+        const int ncand = big >= 0 ? 3 : 2;
+        float t[3] = {1e30f, 1e30f, 1e30f};
         if (ctx->autotune) {
           hipEvent_t e0, e1;
           HIPCK(hipEventCreate(&e0));
           HIPCK(hipEventCreate(&e1));
-          for (int g = 0; g < 2; ++g) {
-            run_x(g);
+          for (int g = 0; g < ncand; ++g) {
+            const auto cs = sel(g);
+            if (g) {
+              int rc = upload(cs.first, cs.second);
+              if (rc) return rc;
+            }
+            auto run_x = [&]() {
+              if (cs.second > 1) fill_zero(s, b.Gam, b.Mp * nrpad);
+              if (g)
+                run_grouped(cs.first, cs.second, s);
+              else
+                for (int q = 1; q < 3; ++q) run_block(q, cs.first, cs.second, s);
+            };
+            run_x();
             for (int rep = 0; rep < 2; ++rep) {
               HIPCK(hipEventRecord(e0, s));
-              run_x(g);
+              run_x();
               HIPCK(hipEventRecord(e1, s));
               HIPCK(hipEventSynchronize(e1));
               float ms;
@@ -1847,26 +1866,37 @@ int eval_gemm(phf_ctx* ctx, const EvalBufs& b) {
           HIPCK(hipEventDestroy(e0));
           HIPCK(hipEventDestroy(e1));
         }
-        int pick = (ctx->autotune && t[1] < t[0]) ? 1 : 0;
-        if (const char* f = getenv("PHF_XGROUP")) pick = atoi(f) != 0;   // pin (tests, experiments)
-        char line[200];
+        int pick = 0;
+        if (ctx->autotune)
+          for (int g = 1; g < ncand; ++g)
+            if (t[g] < t[pick]) pick = g;
+        if (const char* f = getenv("PHF_XGROUP")) {   // pin (tests, experiments): 0, 1, or 2 where it exists
+          pick = std::max(0, std::min(atoi(f), ncand - 1));
+        }
+        static const char* names[3] = {"two_launches", "grouped", "grouped_256x256"};
+        char line[256];
         snprintf(line, sizeof line, "{\"prec\": %d, \"rows\": %ld, \"exchange_blocks\": {\"two_launches_ms\": %.4f, "
-                 "\"grouped_ms\": %.4f, \"chosen\": \"%s\"}}\n", precision, 2 * G, t[0], t[1],
-                 pick ? "grouped" : "two_launches");
+                 "\"grouped_ms\": %.4f, \"grouped_256x256_ms\": %.4f, \"chosen\": \"%s\"}}\n", precision, 2 * G,
+                 t[0], t[1], ncand > 2 ? t[2] : -1.0f, names[pick]);
         ctx->tune_log += line;
         it = ctx->xgroup.emplace(gkey, pick).first;
       }
       grouped = it->second;
+      if (grouped == 2 && big < 0) grouped = 1;   // a pinned configuration since the tuning
+    }
+    const auto gsel = sel(grouped);
+    if (grouped) {
+      int rc = upload(gsel.first, gsel.second);
+      if (rc) return rc;
     }
     // blocks write disjoint columns of Gamma: one zero fill before all of them
-    if (any_split) fill_zero(s, b.Gam, b.Mp * nrpad);
+    if (any_split || (grouped && gsel.second > 1)) fill_zero(s, b.Gam, b.Mp * nrpad);
     if (grouped) {
       auto run_q = timed([&](int q, hipStream_t sq) {
         if (q == 0)
           run_block(0, choice[0].first, choice[0].second, sq);
         else
-          eri_gemm_mixed_grouped(sq, precision, ctx->d_xitems, (int)ctx->h_xitems.size(), choice[1].first,
-                                 h16_grouped_grid(choice[1].first, ctx->ncu), choice[1].second > 1);
+          run_grouped(gsel.first, gsel.second, sq);
       });
       run_q(0, s);
       run_q(1, s);

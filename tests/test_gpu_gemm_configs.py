@@ -157,3 +157,28 @@ def test_exchange_blocks_grouped_equal_two_launches(prob, fp64, prec):
     assert np.abs(g0 - g1).max() <= 1e-8
     e64, _ = fp64[True]
     assert abs(e1 - e64) <= 1e-6
+
+
+@pytest.mark.parametrize("prec", ["f16x2", "i8x4"])
+def test_exchange_blocks_grouped_256_tile(prec):
+    """X1 and X2 as one grouped launch of the 256 x 256 tile (PHF_XGROUP=2:
+    the round-to-nearest 16-bit tile, the exact int8 tile) on an n = 32
+    instance, whose exchange blocks have 256-row multiples: within the form's
+    budget of FP64 (|dE| <= 1e-6, |dG| <= 1e-5)."""
+    from phf_pcc_amd.model import ProjectedHF
+    p = synthetic_problem(32, 32, seed=78, zscale=0.05)
+    ga = grid_args(32, (50, 16), 0)
+    m64 = ProjectedHF(p.nocc, p.ref, ga, factors=(p.h_spatial, p.b), structure=True, precision="fp64")
+    e64, g64 = m64.energy_and_gradient(p.z)
+    m64.ctx.close()
+    os.environ["PHF_XGROUP"] = "2"
+    try:
+        m = ProjectedHF(p.nocc, p.ref, ga, factors=(p.h_spatial, p.b), structure=True, precision=prec)
+        e, g = m.energy_and_gradient(p.z)
+        rep = m.ctx.tuning_report()
+        m.ctx.close()
+    finally:
+        os.environ.pop("PHF_XGROUP", None)
+    assert '"chosen": "grouped_256x256"' in rep, rep
+    assert abs(e - e64) <= 1e-6
+    assert np.abs(g - g64).max() <= 1e-5
