@@ -543,6 +543,8 @@ extern "C" int phf_ctx_destroy(phf_ctx* ctx) {
   }
   drop_blocks(ctx);
   phf_free_sc(ctx);
+  if (ctx->og_exec) (void)hipGraphExecDestroy(ctx->og_exec);
+  if (ctx->og_graph) (void)hipGraphDestroy(ctx->og_graph);
   if (ctx->d_xitems) (void)hipFree(ctx->d_xitems);
   if (ctx->ws) (void)hipFree(ctx->ws);
   if (ctx->pcc_ws) (void)hipFree(ctx->pcc_ws);
@@ -600,6 +602,7 @@ static int ensure_ws(phf_ctx* ctx, size_t bytes) {
 
 extern "C" int phf_set_hamiltonian(phf_ctx* ctx, const double* hone, const double* htwo, int nso, int flags) {
   if (!ctx) return PHF_ERR_ARG;
+  ++ctx->gen;
   if (!hone || !htwo || nso <= 0 || nso > 512)
     return fail(ctx, PHF_ERR_ARG, "phf_set_hamiltonian: bad arguments");
   HIPCK(hipSetDevice(ctx->device));
@@ -715,6 +718,7 @@ static void drop_blocks(phf_ctx* ctx) {
 }
 
 int phf_build_blocks(phf_ctx* ctx, const std::vector<int>& spin, bool release_dense) {
+  ++ctx->gen;
   if (!ctx->dA || ctx->cplx || (int)spin.size() != ctx->nso) return PHF_OK;
   if (ctx->blocked && ctx->blk_spin == spin) return PHF_OK;
   HIPCK(hipStreamSynchronize(ctx->stream));
@@ -835,6 +839,7 @@ static int ensure_mixed_ab(phf_ctx* ctx, int prec) {
 
 extern "C" int phf_set_blocked(phf_ctx* ctx, int on) {
   if (!ctx) return PHF_ERR_ARG;
+  ++ctx->gen;
   if (!on && !ctx->dA && ctx->blocked)
     return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_set_blocked: the dense A was released");
   ctx->use_blocks = on != 0;
@@ -902,6 +907,7 @@ static int drop_hamiltonian(phf_ctx* ctx) {
 extern "C" int phf_set_hamiltonian_factored(phf_ctx* ctx, const double* h, const double* b, int n, int nl,
                                             const double* ref, int flags) {
   if (!ctx) return PHF_ERR_ARG;
+  ++ctx->gen;
   if (!h || !b || !ref || n <= 0 || nl <= 0 || 2 * n > 512)
     return fail(ctx, PHF_ERR_ARG, "phf_set_hamiltonian_factored: bad arguments");
   const int nso = 2 * n;
@@ -1136,6 +1142,7 @@ extern "C" int phf_set_grid(phf_ctx* ctx, const double* ref, const int* ngrid, i
                             const double* r1, int nr1, const double* r2, int nr2, const double* rpg, int npg,
                             const double* weightsp, const double* weightpg, int ncipg, const double* roota,
                             int na, const double* rootb, int nb, const double* rooty, int ny_, int qj, int sp) {
+  if (ctx) ++ctx->gen;
   if (!ctx) return PHF_ERR_ARG;
   if (!ctx->has_ham) return fail(ctx, PHF_ERR_NO_HAMILTONIAN, "phf_set_grid: no Hamiltonian set");
   const int nso = ctx->nso;
@@ -1345,6 +1352,7 @@ static int tune_h16(phf_ctx* ctx, const std::tuple<int, long, int, int>& key, lo
 
 extern "C" int phf_autotune(phf_ctx* ctx, int on) {
   if (!ctx) return PHF_ERR_ARG;
+  ++ctx->gen;
   ctx->autotune = on != 0;
   ctx->tuned.clear();
   ctx->xgroup.clear();
@@ -1968,8 +1976,17 @@ void eval_post(phf_ctx* ctx, const EvalBufs& b, bool resident) {
   }
 }
 
+static void drop_objective_graph(phf_ctx* ctx) {
+  if (ctx->og_exec) (void)hipGraphExecDestroy(ctx->og_exec);
+  if (ctx->og_graph) (void)hipGraphDestroy(ctx->og_graph);
+  ctx->og_exec = nullptr;
+  ctx->og_graph = nullptr;
+  ctx->og_key.clear();
+}
+
 static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int precision, double* hmat,
-                  double* smat, double* gradmat, double* rdmmat, bool resident, double* objective = nullptr) {
+                  double* smat, double* gradmat, double* rdmmat, bool resident, double* objective = nullptr,
+                  bool graph_ok = false) {
   if (!ctx) return PHF_ERR_ARG;
   ctx->r_valid = false;
   if (!hmat || !smat) return fail(ctx, PHF_ERR_ARG, "phf_buildhsg: NULL output pointer");
@@ -2009,42 +2026,85 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
   int* info = reinterpret_cast<int*>(ctx->pin + c0_bytes + hs_bytes + obj_bytes);
   build_c0(z, nso, N, cmplxconj, c0);
   ctx->pin_busy = true;
-  HIPCK(hipMemcpyAsync(b.C0, c0, sizeof(double) * 2 * (size_t)nso * N, hipMemcpyHostToDevice, s));
-  if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[0], s));
-  eval_pre(ctx, b, ctx->profiling ? ctx->ev + 4 : nullptr);
-  if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[1], s));
-  {
-    int rc = eval_gemm(ctx, b);
-    if (rc) return rc;
+  if (objective && ctx->small_size < sizeof(double) * nobj) {
+    HIPCK(hipStreamSynchronize(s));
+    if (ctx->small) HIPCK(hipFree(ctx->small));
+    ctx->small = nullptr;
+    ctx->small_size = 0;
+    HIPCK(hipMalloc(&ctx->small, sizeof(double) * nobj));
+    ctx->small_size = sizeof(double) * nobj;
   }
-  if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[2], s));
-  eval_post(ctx, b, resident);
-  HIPCK(hipGetLastError());
-  if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[3], s));
-
-  // the objective kernel goes in before any copy back, and the small results
-  // come back through pinned memory, so nothing waits on the host until the
-  // one synchronization below
-  if (objective) {
-    if (ctx->small_size < sizeof(double) * nobj) {
-      HIPCK(hipStreamSynchronize(s));
-      if (ctx->small) HIPCK(hipFree(ctx->small));
-      ctx->small = nullptr;
-      ctx->small_size = 0;
-      HIPCK(hipMalloc(&ctx->small, sizeof(double) * nobj));
-      ctx->small_size = sizeof(double) * nobj;
+  // one evaluation's launch sequence: eager, or captured for replay
+  auto enqueue = [&]() -> int {
+    HIPCK(hipMemcpyAsync(b.C0, c0, sizeof(double) * 2 * (size_t)nso * N, hipMemcpyHostToDevice, s));
+    if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[0], s));
+    eval_pre(ctx, b, ctx->profiling ? ctx->ev + 4 : nullptr);
+    if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[1], s));
+    {
+      int rc = eval_gemm(ctx, b);
+      if (rc) return rc;
     }
-    double* dob = reinterpret_cast<double*>(ctx->small);
-    const int nel = (nso - N) * N;
-    objective_kernel<<<(std::max(nel, 1) + 255) / 256, 256, 0, s>>>(nso, N, b.HS, b.Ysum, b.Xsum, dob);
+    if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[2], s));
+    eval_post(ctx, b, resident);
     HIPCK(hipGetLastError());
-    HIPCK(hipMemcpyAsync(pobj, dob, sizeof(double) * nobj, hipMemcpyDeviceToHost, s));
-  }
-  HIPCK(hipMemcpyAsync(hs, b.HS, sizeof(double) * 4 * (size_t)nmn, hipMemcpyDeviceToHost, s));
-  HIPCK(hipMemcpyAsync(info, b.Info, sizeof(int) * (size_t)Gv, hipMemcpyDeviceToHost, s));
-  if (!resident) {
-    HIPCK(hipMemcpyAsync(gradmat, b.Grad, 2 * (size_t)nmn * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPCK(hipMemcpyAsync(rdmmat, b.Rdm, 2 * (size_t)nmn * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (ctx->profiling) HIPCK(hipEventRecord(ctx->ev[3], s));
+
+    // the objective kernel goes in before any copy back, and the small results
+    // come back through pinned memory, so nothing waits on the host until the
+    // one synchronization below
+    if (objective) {
+      double* dob = reinterpret_cast<double*>(ctx->small);
+      const int nel = (nso - N) * N;
+      objective_kernel<<<(std::max(nel, 1) + 255) / 256, 256, 0, s>>>(nso, N, b.HS, b.Ysum, b.Xsum, dob);
+      HIPCK(hipGetLastError());
+      HIPCK(hipMemcpyAsync(pobj, dob, sizeof(double) * nobj, hipMemcpyDeviceToHost, s));
+    }
+    HIPCK(hipMemcpyAsync(hs, b.HS, sizeof(double) * 4 * (size_t)nmn, hipMemcpyDeviceToHost, s));
+    HIPCK(hipMemcpyAsync(info, b.Info, sizeof(int) * (size_t)Gv, hipMemcpyDeviceToHost, s));
+    if (!resident) {
+      HIPCK(hipMemcpyAsync(gradmat, b.Grad, 2 * (size_t)nmn * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
+      HIPCK(hipMemcpyAsync(rdmmat, b.Rdm, 2 * (size_t)nmn * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    return PHF_OK;
+  };
+  auto make_key = [&]() {
+    char kb[512];
+    auto env = [](const char* v) { const char* e = getenv(v); return e ? e : "-"; };
+    snprintf(kb, sizeof kb, "%d|%d|%d|%d|%d|%ld|%p|%p|%p|%p|%zu|%s|%s|%s|%s|%s", precision, nocc, cmplxconj,
+             (int)resident, (int)(objective != nullptr), ctx->gen, (void*)ctx->ws, (void*)ctx->pin,
+             (void*)ctx->small, (void*)ctx->d_xitems, ctx->ws_size, env("PHF_H16_CFG"), env("PHF_XGROUP"),
+             env("PHF_FUSED_SPLIT"), env("PHF_H16_FOLD"), env("PHF_H16_ALLOW_F32FOLD"));
+    return std::string(kb);
+  };
+  std::string key = graph_ok ? make_key() : std::string();
+  if (graph_ok && ctx->og_exec && key == ctx->og_key) {
+    HIPCK(hipGraphLaunch(ctx->og_exec, s));
+  } else {
+    int rc = enqueue();
+    if (rc) return rc;
+    if (graph_ok) key = make_key();   // the eager pass may have allocated (the exchange work list)
+    if (graph_ok && key != ctx->og_failed_key) {
+      // capture the same sequence (nothing runs during capture); the tuning and
+      // uploads it depends on were done by the eager pass above
+      drop_objective_graph(ctx);
+      HIPCK(hipStreamSynchronize(s));
+      hipGraph_t g = nullptr;
+      HIPCK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      const int rc2 = enqueue();
+      const hipError_t ec = hipStreamEndCapture(s, &g);
+      hipGraphExec_t ex = nullptr;
+      if (rc2 == PHF_OK && ec == hipSuccess && g && hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess) {
+        ctx->og_graph = g;
+        ctx->og_exec = ex;
+        ctx->og_key = key;
+        ctx->tune_log += "{\"objective_graph\": \"captured\"}\n";
+      } else {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        ctx->og_failed_key = key;   // stay eager for this key
+        ctx->tune_log += "{\"objective_graph\": \"capture failed, eager\"}\n";
+      }
+    }
   }
   HIPCK(hipStreamSynchronize(s));
   ctx->pin_busy = false;
@@ -2128,7 +2188,9 @@ extern "C" int phf_eval_objective(phf_ctx* ctx, const double* z, int cmplxconj, 
     return fail(ctx, PHF_ERR_UNSUPPORTED, "phf_eval_objective: needs a grid with nci = 1 and CmplxConj 0 or 1");
   double h[2], sm[2];
   std::vector<double> obj(2 + 2 * (size_t)(ctx->nso - nocc) * std::max(nocc, 0));
-  int rc = eval_z(ctx, z, cmplxconj, nocc, precision, h, sm, nullptr, nullptr, true, obj.data());
+  const char* gv = getenv("PHF_GRAPH");   // PHF_GRAPH=0: always eager
+  const bool graph_ok = !ctx->profiling && !(gv && atoi(gv) == 0);
+  int rc = eval_z(ctx, z, cmplxconj, nocc, precision, h, sm, nullptr, nullptr, true, obj.data(), graph_ok);
   if (rc) return rc;
   if (!(std::fabs(sm[0]) > 1e-6))   // EigenSolver keeps no state (PHFTools.py:8-18, Tol 1e-6)
     return fail(ctx, PHF_ERR_SINGULAR, "phf_eval_objective: |Re Smat| <= 1e-6, no projected state");

@@ -142,6 +142,15 @@ struct phf_ctx_s {
   size_t sc_ws_size = 0;
   // per-stage timing with HIP events on `stream` (phf_set_profiling)
   bool profiling = false;
+  // graph replay of phf_eval_objective: one evaluation's launch sequence
+  // (C0 upload from pinned memory through the copies back), captured after an
+  // eager evaluation and replayed while og_key (shapes, buffers, generation,
+  // the tile-pinning environment) is unchanged; gen is bumped by every setter
+  // that changes what an evaluation launches
+  hipGraph_t og_graph = nullptr;
+  hipGraphExec_t og_exec = nullptr;
+  std::string og_key, og_failed_key;
+  long gen = 0;
   hipEvent_t ev[8] = {};
   double stage_ms[11] = {};
   int nstage = 0;
