@@ -899,6 +899,13 @@ static int pcc_run(phf_ctx* ctx, const double* hone, const double* htwo, const d
   // profiling: [0] start / end of the device work, [1..2] zeroth-order
   // stages of a chunk, [3..4] residual, [5..6] ladder GEMM, [7..8] grid sums
   hipEvent_t ev[9] = {};
+  struct EvGuard {   // destroyed on every return path
+    hipEvent_t* e;
+    ~EvGuard() {
+      for (int k = 0; k < 9; ++k)
+        if (e[k]) (void)hipEventDestroy(e[k]);
+    }
+  } ev_guard{ev};
   double tms[5] = {0, 0, 0, 0, 0};
   long nladder = 0;
   const bool prof = ctx->profiling;
@@ -1011,7 +1018,6 @@ static int pcc_run(phf_ctx* ctx, const double* hone, const double* htwo, const d
     HIPCK(hipEventRecord(ev[8], s));
     HIPCK(hipEventSynchronize(ev[8]));
     tms[0] = span(0, 8);
-    for (auto& e : ev) (void)hipEventDestroy(e);
     for (int k = 0; k < 5; ++k) ctx->stage_ms[k] = tms[k];
     ctx->stage_ms[5] = (double)nladder;
     ctx->nstage = 6;

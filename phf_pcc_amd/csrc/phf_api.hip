@@ -2089,9 +2089,12 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
       drop_objective_graph(ctx);
       HIPCK(hipStreamSynchronize(s));
       hipGraph_t g = nullptr;
-      HIPCK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      const int rc2 = enqueue();
-      const hipError_t ec = hipStreamEndCapture(s, &g);
+      int rc2 = PHF_ERR_HIP;
+      hipError_t ec = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+      if (ec == hipSuccess) {
+        rc2 = enqueue();
+        ec = hipStreamEndCapture(s, &g);
+      }
       hipGraphExec_t ex = nullptr;
       if (rc2 == PHF_OK && ec == hipSuccess && g && hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess) {
         ctx->og_graph = g;
