@@ -1708,12 +1708,21 @@ void build_h16_items(const std::vector<std::pair<const phf_ctx*, const EvalBufs*
   const size_t n = logical.size();
   out.resize(n);
   const int per = grid / 8;
-  for (size_t l = 0; l < n; ++l) {
-    const size_t r = l / grid, j = l % grid;
-    size_t p = l;
-    if (per > 0 && grid % 8 == 0 && (r + 1) * grid <= n) p = r * grid + (j % per) * 8 + j / per;
-    out[p] = logical[l];
+  if (per == 0 || grid % 8 != 0) {
+    out = logical;
+    return;
   }
+  const size_t full = n / grid;   // full rounds
+  for (size_t l = 0; l < full * (size_t)grid; ++l) {
+    const size_t r = l / grid, j = l % grid;
+    out[r * grid + (j % per) * 8 + j / per] = logical[l];
+  }
+  // the last, partial round: workgroup b takes slot b (XCD b % 8); give each XCD
+  // a contiguous run of the remaining items as well
+  const size_t base = full * (size_t)grid, m = n - base;
+  size_t l = base;
+  for (size_t x = 0; x < 8 && l < n; ++x)
+    for (size_t slot = x; slot < m; slot += 8) out[base + slot] = logical[l++];
 }
 
 int eval_gemm(phf_ctx* ctx, const EvalBufs& b) {
