@@ -38,6 +38,16 @@ sys.path.insert(0, ROOT)
 
 # MI355X dense matrix-core peaks (AMD public figures; int8 in OP/s)
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15, "f16x2": 2.5e15, "i8x4": 5.0e15}
+# real products per complex-free product of each form (the split forms execute several)
+NPROD = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3, "i8x4": 10}
+# algorithmic ERI GEMM FLOPs per evaluation (n spatial orbitals, G grid points,
+# one density per point, real A; the re and im rows of each density count as
+# two GEMM rows): dense 2 (2G) NSO^4 = 4 (2n)^4 G; spin blocks D (2n^2 x 2n^2)
+# and X1, X2 (n^2 x n^2): 4 (4 + 2) n^4 G; spin-free (2J - K) on one set of
+# rows and K on three: 4 (1 + 3) n^4 G
+STRUCT_FLOPS = {"dense": lambda n, G: 4.0 * (2 * n) ** 4 * G,
+                "spin-blocks": lambda n, G: 24.0 * n ** 4 * G,
+                "spin-free": lambda n, G: 16.0 * n ** 4 * G}
 
 
 def log(msg):
@@ -92,13 +102,12 @@ def batch_forms(forms, steps, warmup, ninst=64, baselines=True):
     from phf_pcc_amd.batch import ProjectedHFBatch
     t0 = time.time()
     inst, zs, ns, gs = make_batch_workload(ninst)
-    nso = 2 * ns.astype(np.float64)
     G = int(gs.sum())
-    flops_dense = float(np.sum(4.0 * nso ** 4 * gs))
     ns_ = ns.astype(np.float64)
-    flops_blk = float(np.sum(4.0 * ((2 * ns_ ** 2) ** 2 + 2 * (ns_ ** 2) ** 2) * gs))
+    sflops = {m: float(np.sum(f(ns_, gs))) for m, f in STRUCT_FLOPS.items()}
     log(f"[bench] batch of {len(inst)} instances, n in [{ns.min()}, {ns.max()}], {G} grid points, "
-        f"{flops_blk:.3e} algorithmic ERI GEMM flops per step over the spin blocks (host set-up {time.time()-t0:.1f}s)")
+        f"{sflops['spin-free']:.3e} algorithmic ERI GEMM flops per step in the spin-free form, "
+        f"{sflops['spin-blocks']:.3e} over the spin blocks (host set-up {time.time()-t0:.1f}s)")
     if not baselines:
         # profiling runs: only the listed forms, no comparison
         for form in forms:
@@ -110,33 +119,36 @@ def batch_forms(forms, steps, warmup, ninst=64, baselines=True):
         return None
     if "fp64" not in forms:
         forms = forms + ["fp64"]
-    # first the plain dense FP64 batch (202 GB of A: it runs while device memory is unfragmented)
-    runs = [("fp64", False)] + [(f, True) for f in forms]
+    # first the plain dense FP64 batch (202 GB of A: it runs while device memory
+    # is unfragmented), then FP64 over the spin blocks, then every form in the
+    # spin-free form (the instances' Refs are spin-restricted)
+    runs = [("fp64_dense", "fp64", "dense"), ("fp64_spin_blocks", "fp64", "spin-blocks")]
+    runs += [(f, f, "spin-free") for f in forms]
+    # one child process per run: a batch whose launch sequence was captured in
+    # a hipGraph does not return all of its device memory to the process when
+    # it is destroyed (measured: about two thirds stay allocated), so runs in
+    # one process would accumulate it; each child returns it at exit
+    import subprocess
+    import tempfile
     res = {}
-    for form, structure in runs:
-        key = form if structure else "fp64_dense"
-        t0 = time.time()
-        bt = ProjectedHFBatch(inst, storage=form, structure=structure)
-        t_build = time.time() - t0
-        e, g, _ = bt.energy_and_gradient(zs, form, strict=False)
-        eager_total, eager_gemm, _ = bt.stats()
-        for _ in range(max(0, warmup - 1)):
-            e, g, _ = bt.energy_and_gradient(zs, form, strict=False)
-        dev = []
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            e, g, _ = bt.energy_and_gradient(zs, form, strict=False)
-            dev.append(bt.stats()[0])
-        dt = time.perf_counter() - t0
-        status = bt.status.copy()
-        res[key] = dict(E=e.copy(), G=g, status=status, ms_step=dt / steps * 1e3, value=G * steps / dt,
-                        dev_ms=float(np.mean(dev)), eager_gemm_ms=eager_gemm, eager_total_ms=eager_total,
-                        build_s=t_build, tuning=bt.tuning_report().strip(), prec=form, structure=structure)
-        log(f"[bench] batch {key}{' (spin blocks)' if structure else ''}: build {t_build:.1f}s  step "
-            f"{res[key]['ms_step']:.2f} ms  device {res[key]['dev_ms']:.2f} ms  (eager GEMM {eager_gemm:.2f} ms)  "
-            f"instances failing the Hermiticity test: {int((status != 0).sum())}")
-        bt.close()
-        del bt
+    with tempfile.TemporaryDirectory() as tmp:
+        for key, form, want in runs:
+            out = os.path.join(tmp, key + ".npz")
+            cmd = [sys.executable, os.path.join(ROOT, "bench.py"), "--workload", "batch64", "--ninst", str(ninst),
+                   "--batch-run", f"{form}:{want}:{out}", "--steps", str(steps), "--warmup", str(warmup)]
+            r = subprocess.run(cmd)
+            if r.returncode != 0:
+                raise RuntimeError(f"batch run {key} failed with status {r.returncode}")
+            d = np.load(out)
+            meta = json.loads(str(d["meta"]))
+            goff = np.cumsum([0] + [int(x) for x in d["gsizes"]])
+            gflat = d["G"]
+            res[key] = dict(E=d["E"], G=[gflat[goff[i]:goff[i + 1]] for i in range(len(goff) - 1)],
+                            status=d["status"], **meta)
+            log(f"[bench] batch {key} ({res[key]['mode']}): build {res[key]['build_s']:.1f}s  step "
+                f"{res[key]['ms_step']:.2f} ms  device {res[key]['dev_ms']:.2f} ms  (eager GEMM "
+                f"{res[key]['eager_gemm_ms']:.2f} ms)  instances failing the Hermiticity test: "
+                f"{int((res[key]['status'] != 0).sum())}")
     base = res["fp64_dense"]
     passing = base["status"] == 0
     for f, r in res.items():
@@ -147,23 +159,23 @@ def batch_forms(forms, steps, warmup, ninst=64, baselines=True):
         r["dE_passing"] = float(de[passing].max()) if passing.any() else None
         log(f"[bench] batch {f}: max|dE| vs plain fp64 over all {len(de)} instances = {r['dE']:.3e} (over the "
             f"{int(passing.sum())} passing the Hermiticity test {r['dE_passing']:.3e})  max|dG| = {r['dG']:.3e}  "
-            f"speedup vs plain fp64 = {r['value'] / base['value']:.2f}x  vs fp64 on the same blocks = "
-            f"{r['value'] / res['fp64']['value']:.2f}x")
-    okf = [r for r in res.values() if r["dE"] <= 1e-6]
+            f"speedup vs plain fp64 = {r['value'] / base['value']:.2f}x  vs fp64 in the same structure = "
+            f"{r['value'] / res['fp64']['value']:.2f}x  vs fp64 spin blocks = "
+            f"{r['value'] / res['fp64_spin_blocks']['value']:.2f}x")
+    okf = [r for k, r in res.items() if r["dE"] <= 1e-6 and k in forms]
     best = max(okf, key=lambda r: r["value"])
     bprec = best["prec"]
-    nprod = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3, "i8x4": 10}[bprec]
-    flops_best = flops_blk if best["structure"] else flops_dense
+    nprod = NPROD[bprec]
+    flops_best = sflops[best["mode"]]
     t_gemm = best["eager_gemm_ms"] * 1e-3
     grouped = bprec != "fp64"
     roof = {"bound": "mfma", "achieved": flops_best / t_gemm / 1e12, "peak": PEAK[bprec] / 1e12,
             "unit": "TFLOP/s", "frac": flops_best / t_gemm / PEAK[bprec],
             "frac_executed": nprod * flops_best / t_gemm / PEAK[bprec], "traffic": None,
-            "kernel": "eri_gemm_h16_grouped_kernel" if grouped else "per-instance ERI GEMM launches",
+            "kernel": _grouped_kernel(bprec, best["tuning"]) if grouped else "per-instance ERI GEMM launches",
             "algorithmic_flops_per_launch": flops_best, "executed_flops_per_launch": nprod * flops_best,
-            "products": nprod, "spin_blocks": best["structure"], "ms_per_launch": best["eager_gemm_ms"],
-            "time_source": "HIP events around the grouped launch (all instances, all spin blocks) in the eager "
-                           "first evaluation"}
+            "products": nprod, "structure": best["mode"], "ms_per_launch": best["eager_gemm_ms"],
+            "time_source": "HIP events around the grouped launch (all instances) in the eager first evaluation"}
     traffic = load_traffic(roof["kernel"], "batch64")
     if traffic is not None:
         roof["traffic"] = traffic["bytes_per_launch"]
@@ -172,37 +184,73 @@ def batch_forms(forms, steps, warmup, ninst=64, baselines=True):
                      "ms_per_step": r["ms_step"], "device_ms": r["dev_ms"], "eager_eri_gemm_ms": r["eager_gemm_ms"],
                      "build_s": r["build_s"], "max_abs_dE_vs_plain_fp64": r["dE"],
                      "max_abs_dE_vs_plain_fp64_passing": r["dE_passing"], "max_abs_dG_vs_plain_fp64": r["dG"],
-                     "spin_blocks": r["structure"], "speedup_vs_plain_fp64": r["value"] / base["value"],
-                     "speedup_vs_fp64_same_blocks": r["value"] / res["fp64"]["value"],
+                     "structure": r["mode"], "speedup_vs_plain_fp64": r["value"] / base["value"],
+                     "speedup_vs_fp64_same_structure": r["value"] / res["fp64"]["value"],
+                     "speedup_vs_fp64_spin_blocks": r["value"] / res["fp64_spin_blocks"]["value"],
                      "failing_instances": int((r["status"] != 0).sum()), "tuning": r["tuning"] or None}
                  for f, r in res.items()}
     for f, r in res.items():   # the grouped launch of every reduced form against its own MFMA peak
         if f != "fp64" and not f.startswith("fp64") and r["eager_gemm_ms"] > 0:
-            fl = flops_blk if r["structure"] else flops_dense
-            np_ = {"fp32": 1, "bf16x3": 6, "f16x2": 3, "i8x4": 10}[f]
-            forms_out[f]["grouped_gemm"] = {"kernel": "eri_gemm_h16_grouped_kernel<%s>" % _tname(f),
+            fl = sflops[r["mode"]]
+            np_ = NPROD[f]
+            forms_out[f]["grouped_gemm"] = {"kernel": _grouped_kernel(f, r["tuning"]),
                                             "ms_per_launch": r["eager_gemm_ms"],
                                             "achieved_tflops": fl / (r["eager_gemm_ms"] * 1e-3) / 1e12,
                                             "peak_tflops": PEAK[f] / 1e12,
                                             "frac": fl / (r["eager_gemm_ms"] * 1e-3) / PEAK[f],
                                             "frac_executed": np_ * fl / (r["eager_gemm_ms"] * 1e-3) / PEAK[f]}
-    return {"value": best["value"], "ms_per_step": best["ms_step"], "prec": bprec, "structure": best["structure"],
+    return {"value": best["value"], "ms_per_step": best["ms_step"], "prec": bprec, "structure": best["mode"],
             "grid_points": G, "instances": len(inst),
             "workload": f"batched: {len(inst)} synthetic instances, n in {{16..96}}, N=n/2, L=3n, grids "
                         f"Lebedev {{6..50}} x gamma {{4,8,16}} (seed 1004), {G} grid points per step, "
                         "E + adjoint gradient per instance, hipGraph replay",
-            "form": bprec + (" (spin blocks)" if best["structure"] else ""),
+            "form": f"{bprec} ({best['mode']})",
             "evals_per_s": len(inst) * 1e3 / best["ms_step"],
             "speedup_vs_plain_fp64_batch": best["value"] / base["value"],
-            "speedup_vs_fp64_same_blocks": best["value"] / res["fp64"]["value"],
+            "speedup_vs_fp64_same_structure": best["value"] / res["fp64"]["value"],
+            "speedup_vs_fp64_spin_blocks": best["value"] / res["fp64_spin_blocks"]["value"],
             "instances_failing_hermiticity_test": int((~passing).sum()),
             "note": "an instance that fails the Hermiticity test (PHFTools.f90:195-200; coarse grids with complex Z) "
                     "costs the same work; its E0 = Re H / |Re S| is formed all the same and is compared too",
             "forms": forms_out, "roofline": roof}
 
 
+def batch_run_child(spec, steps, warmup, ninst):
+    """One run of the batch workload in this process (spec = form:structure:out):
+    build, one eager evaluation (its ERI GEMM time), warm-up, timed graph
+    replays; E, G, status and the timings go to the .npz `out`."""
+    from phf_pcc_amd.batch import ProjectedHFBatch
+    form, want, out = spec.split(":", 2)
+    inst, zs, ns, gs = make_batch_workload(ninst)
+    G = int(gs.sum())
+    t0 = time.time()
+    os.environ["PHF_SPINFREE"] = "1" if want == "spin-free" else "0"   # read by the instances' contexts
+    bt = ProjectedHFBatch(inst, storage=form, structure=want != "dense")
+    t_build = time.time() - t0
+    e, g, _ = bt.energy_and_gradient(zs, form, strict=False)
+    eager_total, eager_gemm, _ = bt.stats()
+    for _ in range(max(0, warmup - 1)):
+        e, g, _ = bt.energy_and_gradient(zs, form, strict=False)
+    dev = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        e, g, _ = bt.energy_and_gradient(zs, form, strict=False)
+        dev.append(bt.stats()[0])
+    dt = time.perf_counter() - t0
+    nsf = bt.spinfree_instances()
+    mode = want if want != "spin-free" else ("spin-free" if nsf == len(inst) else "spin-blocks")
+    assert want != "spin-free" or nsf in (0, len(inst)), nsf   # one structure per run
+    meta = dict(ms_step=dt / steps * 1e3, value=G * steps / dt, dev_ms=float(np.mean(dev)), eager_gemm_ms=eager_gemm,
+                eager_total_ms=eager_total, build_s=t_build, tuning=bt.tuning_report().strip(), prec=form, mode=mode)
+    np.savez(out, E=np.asarray(e), G=np.concatenate([np.asarray(x).ravel() for x in g]),
+             gsizes=np.array([np.asarray(x).size for x in g]), status=bt.status.copy(), meta=json.dumps(meta))
+    bt.close()
+
+
 def run_batch(args):
     """The batch workload as its own JSON line."""
+    if args.batch_run:
+        return batch_run_child(args.batch_run, args.steps, args.warmup, args.ninst)
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
     rec = batch_forms(forms, args.steps, args.warmup, args.ninst, baselines=not args.no_baselines)
     if rec is None:
@@ -212,10 +260,20 @@ def run_batch(args):
            "ms_per_step": rec["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": _form_dtype(rec["prec"]), "data": "synthetic",
            "config": {k: rec[k] for k in ("workload", "form", "evals_per_s", "speedup_vs_plain_fp64_batch",
-                                          "speedup_vs_fp64_same_blocks", "instances_failing_hermiticity_test",
-                                          "note", "forms")},
+                                          "speedup_vs_fp64_same_structure", "speedup_vs_fp64_spin_blocks",
+                                          "instances_failing_hermiticity_test", "note", "forms")},
            "roofline": rec["roofline"], "cpu_baseline": None}
     print(json.dumps(out), flush=True)
+
+
+def _grouped_kernel(prec, tuning):
+    """Kernel of a batch's grouped launch from its tuning record (the chosen
+    tile configuration)."""
+    cfg = None
+    for rec in _report(tuning):
+        if rec.get("grouped") and "chosen" in rec:
+            cfg = rec["chosen"]["cfg"]
+    return _cfg_kernel(prec, cfg, grouped=True)
 
 
 def _tname(prec):
@@ -249,9 +307,17 @@ def run_large(args):
     if "fp64" not in forms:
         forms = ["fp64"] + forms
     res = {}
-    blocked = model.ctx.blocked()
-    for key, f, blk in [("fp64_dense", "fp64", False)] + [(f, f, True) for f in forms]:
-        model.ctx.set_blocked(blk)
+
+    def set_mode(m):
+        model.ctx.set_blocked(m != "dense")
+        model.ctx.set_spinfree(m == "spin-free")
+
+    set_mode("spin-free")
+    mode = "spin-free" if model.ctx.spinfree() else ("spin-blocks" if model.ctx.blocked() else "dense")
+    runs = [("fp64_dense", "fp64", "dense"), ("fp64_spin_blocks", "fp64", "spin-blocks")]
+    runs += [(f, f, mode) for f in forms]
+    for key, f, m in runs:
+        set_mode(m)
         model.precision = f
         for _ in range(args.warmup):
             model.energy_and_gradient(prob.z)
@@ -262,41 +328,46 @@ def run_large(args):
         dt = time.perf_counter() - t0
         sm = np.mean(np.array(st), axis=0)
         res[key] = dict(E=e, G=g, ms=dt / args.steps * 1e3, value=G * args.steps / dt, gemm_ms=sm[2], dev_ms=sm[0],
-                        prec=f, blocked=blk and blocked)
-        log(f"[bench] n=128 {key}{' (spin blocks)' if blk and blocked else ''}: step {res[key]['ms']:.2f} ms  "
-            f"ERI GEMM {sm[2]:.2f} ms  E={e:.12f}")
+                        prec=f, mode=m, stages=sm)
+        log(f"[bench] n=128 {key} ({m}): step {res[key]['ms']:.2f} ms  ERI GEMM {sm[2]:.2f} ms  E={e:.12f}")
+    set_mode("spin-free")
     base = res["fp64_dense"]
-    flops_dense = 4.0 * nso ** 4 * G
-    flops_blk = 4.0 * ((2 * n ** 2) ** 2 + 2 * (n ** 2) ** 2) * G
     out_forms = {}
     for f, r in res.items():
-        flops = flops_blk if r["blocked"] else flops_dense
-        nprod = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3, "i8x4": 10}[r["prec"]]
-        out_forms[f] = {"grid_pts_per_s": r["value"], "ms_per_step": r["ms"], "eri_gemm_ms": r["gemm_ms"],
-                        "device_ms": r["dev_ms"], "E": r["E"], "abs_dE_vs_fp64": abs(r["E"] - base["E"]),
+        flops = STRUCT_FLOPS[r["mode"]](n, G)
+        nprod = NPROD[r["prec"]]
+        out_forms[f] = {"structure": r["mode"], "grid_pts_per_s": r["value"], "ms_per_step": r["ms"],
+                        "eri_gemm_ms": r["gemm_ms"], "device_ms": r["dev_ms"], "E": r["E"],
+                        "abs_dE_vs_fp64": abs(r["E"] - base["E"]),
                         "max_abs_dG_vs_fp64": float(np.abs(r["G"] - base["G"]).max()),
-                        "speedup_vs_plain_fp64": r["value"] / base["value"], "spin_blocks": r["blocked"],
-                        "mfma_frac": nprod * flops / (r["gemm_ms"] * 1e-3) / PEAK[r["prec"]]}
+                        "speedup_vs_plain_fp64": r["value"] / base["value"],
+                        "eri_gemm_algorithmic_flops": flops,
+                        "mfma_frac": flops / (r["gemm_ms"] * 1e-3) / PEAK[r["prec"]],
+                        "mfma_frac_executed": nprod * flops / (r["gemm_ms"] * 1e-3) / PEAK[r["prec"]]}
         log(f"[bench] n=128 {f}: |dE| = {out_forms[f]['abs_dE_vs_fp64']:.3e}  speedup vs plain fp64 "
-            f"{out_forms[f]['speedup_vs_plain_fp64']:.2f}x  MFMA {100 * out_forms[f]['mfma_frac']:.1f}%")
-    ok = [f for f in res if out_forms[f]["abs_dE_vs_fp64"] <= 1e-6]
+            f"{out_forms[f]['speedup_vs_plain_fp64']:.2f}x  MFMA {100 * out_forms[f]['mfma_frac']:.1f}% "
+            f"({100 * out_forms[f]['mfma_frac_executed']:.1f}% executed)")
+    ok = [f for f in forms if out_forms[f]["abs_dE_vs_fp64"] <= 1e-6]
     best = max(ok, key=lambda f: res[f]["value"])
     bprec = res[best]["prec"]
-    kernel = gemm_kernel_name(bprec, model.ctx.tuning_report())
-    traffic = load_traffic(kernel, f"{n}-blocks" if res[best]["blocked"] else n)
-    nprod = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3, "i8x4": 10}[bprec]
-    flops = flops_blk if res[best]["blocked"] else flops_dense
-    t_gemm = res[best]["gemm_ms"] * 1e-3
+    kernels, unpack = gemm_launches(mode, res[best]["stages"], G, n, bprec, model.ctx.tuning_report())
+    dom = max(kernels, key=lambda k: k["ms_per_launch"])
+    traffic = load_traffic(dom["kernel"], f"{n}-{dom['launch']}")
+    nprod = NPROD[bprec]
     out = {"metric": "projected E+grad evals/s (grid pts/s) vs plain FP64 GPU at 1e-6 Eh; % MFMA peak",
            "value": res[best]["value"], "unit": "grid pts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": res[best]["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": _form_dtype(bprec), "data": "synthetic",
            "config": {"workload": f"projected E+grad, synthetic n={n} (NSO={nso}), N={nocc}, L={3 * n}, grid 50x24={G} "
-                                  "(seed 1003), device-resident objective", "form": best, "forms": out_forms},
-           "roofline": {"bound": "mfma", "achieved": nprod * flops / t_gemm / 1e12, "peak": PEAK[bprec] / 1e12,
-                        "unit": "TFLOP/s", "frac": nprod * flops / t_gemm / PEAK[bprec], "traffic": None,
-                        "kernel": kernel, "algorithmic_flops_per_launch": flops, "spin_blocks": res[best]["blocked"],
-                        "launches_per_step": 3 if res[best]["blocked"] else 1},
+                                  "(seed 1003), device-resident objective", "form": f"{best} ({mode})",
+                      "forms": out_forms},
+           "roofline": {"bound": "mfma", "achieved": dom["achieved_tflops"], "peak": PEAK[bprec] / 1e12,
+                        "unit": "TFLOP/s", "frac": dom["frac"], "frac_executed": dom["frac_executed"],
+                        "traffic": None, "kernel": dom["kernel"], "launch": dom["launch"], "structure": mode,
+                        "ms_per_launch": dom["ms_per_launch"],
+                        "algorithmic_flops_per_launch": dom["algorithmic_flops_per_launch"],
+                        "executed_flops_per_launch": nprod * dom["algorithmic_flops_per_launch"],
+                        "products": nprod, "kernels": kernels, "unpack": unpack},
            "cpu_baseline": None}
     if traffic is not None:
         out["roofline"]["traffic"] = traffic["bytes_per_launch"]
@@ -640,6 +711,81 @@ def x_group_kernel_name(prec, report):
     return None
 
 
+def _cfg_kernel(prec, cfg, grouped=False):
+    """Kernel of tile configuration cfg for a form (4: 256 x 256 int8 tile, 5:
+    256 x 256 round-to-nearest 16-bit tile, others: the 128-row tiles)."""
+    if prec == "fp64":
+        return "eri_gemm_f64_kernel"
+    t = _tname(prec)
+    if cfg == 4:
+        return "eri_gemm_i8_grouped_kernel" if grouped else "eri_gemm_i8_kernel"
+    if cfg == 5:
+        return f"eri_gemm_rn16_grouped_kernel<{t}>" if grouped else f"eri_gemm_rn16_kernel<{t}>"
+    return f"eri_gemm_h16_grouped_kernel<{t}>" if grouped else f"eri_gemm_h16_kernel<{t}>"
+
+
+def _report(report):
+    for line in (report or "").splitlines():
+        try:
+            yield json.loads(line)
+        except ValueError:
+            continue
+
+
+def gemm_launches(mode, st, G, n, prec, report):
+    """Every launch of the ERI GEMM stage of one evaluation from the profiled
+    stage times st[8:] (HIP events): name, kernel, ms, algorithmic FLOPs (one
+    product), achieved TFLOP/s and fractions of the form's peak; plus, for the
+    spin-free form, the unpack into Gamma + h with its algorithmic bytes
+    (read the four product sets, write Gamma: 2G rows x (4 + 4) n^2 doubles)."""
+    nprod, peak = NPROD[prec], PEAK[prec]
+    code = {"fp32": 1, "bf16x3": 2, "f16x2": 3, "i8x4": 4}.get(prec)
+
+    def row(name, kernel, ms, alg):
+        return {"launch": name, "kernel": kernel, "ms_per_launch": ms, "algorithmic_flops_per_launch": alg,
+                "achieved_tflops": alg / (ms * 1e-3) / 1e12, "frac": alg / (ms * 1e-3) / peak,
+                "frac_executed": nprod * alg / (ms * 1e-3) / peak}
+
+    def tuned_cfg(rows, cols):
+        for rec in _report(report):
+            if rec.get("prec") == code and rec.get("rows") == rows and rec.get("cols") == cols and "chosen" in rec:
+                return rec["chosen"]["cfg"]
+        return None
+
+    def ms(q):
+        return float(st[8 + q]) if len(st) > 8 + q and st[8 + q] > 0 else None
+
+    n2 = n * n
+    if mode == "spin-free":
+        a0, a1 = 2.0 * (2 * G) * n2 * n2, 2.0 * (6 * G) * n2 * n2
+        c1 = tuned_cfg(6 * G, n2)
+        chosen = None
+        for rec in _report(report):
+            if rec.get("prec") == code and "spin_free_units" in rec:
+                chosen = rec["spin_free_units"]["chosen"]
+        if ms(1) is None:   # one grouped launch of both products
+            cfg = (4 if prec == "i8x4" else 5) if chosen == "grouped_256x256" else c1
+            kernels = [row("2J-K + K (one grouped launch)", _cfg_kernel(prec, cfg, True), ms(0), a0 + a1)]
+        else:
+            kernels = [row("2J-K", _cfg_kernel(prec, tuned_cfg(2 * G, n2)), ms(0), a0),
+                       row("K", _cfg_kernel(prec, c1), ms(1), a1)]
+        ub = 8.0 * (2 * G) * (4 + 4) * n2
+        unpack = {"launch": "unpack into Gamma + h", "kernel": "rjk_unpack_kernel", "ms_per_launch": ms(2),
+                  "algorithmic_bytes": ub, "GBps": ub / (ms(2) * 1e-3) / 1e9 if ms(2) else None,
+                  "frac_hbm": ub / (ms(2) * 1e-3) / 8e12 if ms(2) else None}
+        return kernels, unpack
+    if mode == "spin-blocks":
+        kname = gemm_kernel_name(prec, report)
+        kernels = [row("D", kname, ms(0), 2.0 * (2 * G) * (2 * n2) ** 2)]
+        ax = 2.0 * (2 * G) * n2 * n2
+        if ms(2) is None:
+            kernels.append(row("X1 + X2 (one grouped launch)", x_group_kernel_name(prec, report), ms(1), 2 * ax))
+        else:
+            kernels += [row("X1", kname, ms(1), ax), row("X2", kname, ms(2), ax)]
+        return kernels, None
+    return [row("dense", gemm_kernel_name(prec, report), ms(0), 2.0 * (2 * G) * (4 * n2) ** 2)], None
+
+
 def load_traffic(kernel, n):
     """HBM bytes per launch of `kernel` at this workload from the committed
     rocprofv3 counter summary (profiles/traffic.json), if present.  The key
@@ -681,6 +827,8 @@ def main():
                          "large: n=128, 50x24 grid, every precision; adjoint: n=96 gradient checks + 50 L-BFGS; "
                          "buildsc: BuildSC blocks; pcc: the projected-CC grid consumer (NSO=32, 800 points)")
     ap.add_argument("--ninst", type=int, default=64, help="instances of the batch64 workload")
+    ap.add_argument("--batch-run", default=None, metavar="FORM:STRUCTURE:OUT",
+                    help="batch64: one run in this process, results to the .npz OUT (the workload's own children)")
     ap.add_argument("--no-baselines", action="store_true",
                     help="batch64: evaluate only the listed forms and print no line (for profiler runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -766,16 +914,24 @@ def main():
     # the plain FP64 GPU form: MakeGamma over the dense NSO^2 x NSO^2 A
     ctx.set_blocked(False)
     plain = run_form("fp64", tag="plain dense ")
-    # every form through the spin blocks of A (spin-pure Ref)
+    # FP64 through the spin blocks D, X1, X2 of A (spin-pure Ref)
     ctx.set_blocked(True)
-    blocked = ctx.blocked()
-    res = {f: run_form(f, tag="spin-block " if blocked else "") for f in forms}
+    ctx.set_spinfree(False)
+    blocks64 = run_form("fp64", tag="spin-block ") if ctx.blocked() else None
+    # every form through the spin-free form (spin-restricted Ref), else the blocks
+    ctx.set_spinfree(True)
+    mode = "spin-free" if ctx.spinfree() else ("spin-blocks" if ctx.blocked() else "dense")
+    res = {f: run_form(f, tag=mode + " ") for f in forms}
     base = plain
+    if blocks64 is not None:
+        blocks64["dE"] = abs(blocks64["E"] - base["E"])
+        blocks64["dG"] = float(np.abs(blocks64["G"] - base["G"]).max())
     for f, r in res.items():
         r["dE"] = abs(r["E"] - base["E"])
         r["dG"] = float(np.abs(r["G"] - base["G"]).max())
-        log(f"[bench] {f}: |dE| vs plain fp64 = {r['dE']:.3e}  max|dG| = {r['dG']:.3e}  speedup vs plain fp64 = "
-            f"{r['value'] / base['value']:.2f}x  vs fp64 on the same blocks = {r['value'] / res['fp64']['value']:.2f}x")
+        log(f"[bench] {mode} {f}: |dE| vs plain fp64 = {r['dE']:.3e}  max|dG| = {r['dG']:.3e}  speedup vs plain "
+            f"fp64 = {r['value'] / base['value']:.2f}x  vs fp64 {mode} = {r['value'] / res['fp64']['value']:.2f}x"
+            + (f"  vs fp64 spin blocks = {r['value'] / blocks64['value']:.2f}x" if blocks64 else ""))
     dropin = {}
     if not args.no_dropin:
         phftools.set_hamiltonian(prob.hone, prob.htwo)
@@ -804,49 +960,29 @@ def main():
         log(f"[bench] tuning {line}")
     ok = [r for r in res.values() if r["dE"] <= 1e-6]
     best = max(ok, key=lambda r: r["value"])
-    # algorithmic FLOPs of the ERI GEMM per evaluation: dense 4 NSO^4 G; over the
-    # spin blocks 4 (nD^2 + 2 nX^2) G with nD = 2 n^2, nX = n^2 (n alpha, n beta MOs)
-    flops_dense = 4.0 * nso ** 4 * G
     n_sp = nso // 2
-    flops = 4.0 * ((2 * n_sp ** 2) ** 2 + 2 * (n_sp ** 2) ** 2) * G if blocked else flops_dense
-    nprod = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3, "i8x4": 10}[best["prec"]]
+    flops_dense = STRUCT_FLOPS["dense"](n_sp, G)
+    flops = STRUCT_FLOPS[mode](n_sp, G)
+    nprod = NPROD[best["prec"]]
     peak = PEAK[best["prec"]]
     # each launch of the best form's ERI GEMM, timed with HIP events on the
-    # stream it runs on: algorithmic FLOPs per launch = 2 x (2G density rows)
-    # x (rows of the block) x (its contraction length), one product
-    st = best["stages"]
-    shapes = ([("D", 2 * n_sp ** 2), ("X1", n_sp ** 2), ("X2", n_sp ** 2)] if blocked else [("dense", nso ** 2)])
-    kname = gemm_kernel_name(best["prec"], ctx.tuning_report())
-    kernels = []
-    for q, (bname, dim) in enumerate(shapes):
-        ms = float(st[8 + q]) if len(st) > 8 + q and st[8 + q] > 0 else None
-        alg = 2.0 * (2 * G) * dim * dim
-        if ms is None:
-            if q == 2 and kernels and kernels[1]["ms_per_launch"] is not None:
-                kernels[1]["launch"] = "X1 + X2 (one grouped launch)"
-                kernels[1]["kernel"] = x_group_kernel_name(best["prec"], ctx.tuning_report())
-                kernels[1]["algorithmic_flops_per_launch"] += alg
-                k1 = kernels[1]
-                k1["achieved_tflops"] = k1["algorithmic_flops_per_launch"] / (k1["ms_per_launch"] * 1e-3) / 1e12
-                k1["frac"] = k1["achieved_tflops"] * 1e12 / peak
-                k1["frac_executed"] = nprod * k1["frac"]
-            continue
-        kernels.append({"launch": bname, "kernel": kname if q == 0 else None, "ms_per_launch": ms,
-                        "algorithmic_flops_per_launch": alg, "achieved_tflops": alg / (ms * 1e-3) / 1e12,
-                        "frac": alg / (ms * 1e-3) / peak, "frac_executed": nprod * alg / (ms * 1e-3) / peak})
+    # stream it runs on: algorithmic FLOPs per launch = 2 x (density rows) x
+    # (A rows) x (contraction length), one product
+    kernels, unpack = gemm_launches(mode, best["stages"], G, n_sp, best["prec"], ctx.tuning_report())
     dom = max(kernels, key=lambda k: k["ms_per_launch"])
     t_gemm = best["gemm_ms"] * 1e-3
     roof = {"bound": "mfma", "achieved": dom["achieved_tflops"], "peak": peak / 1e12, "unit": "TFLOP/s",
             "frac": dom["frac"], "frac_executed": dom["frac_executed"], "traffic": None,
-            "kernel": kname, "launch": dom["launch"] + (" spin block" if blocked else " A"),
+            "kernel": dom["kernel"], "launch": dom["launch"], "structure": mode,
             "ms_per_launch": dom["ms_per_launch"],
             "algorithmic_flops_per_launch": dom["algorithmic_flops_per_launch"],
             "executed_flops_per_launch": nprod * dom["algorithmic_flops_per_launch"], "products": nprod,
             "time_source": "HIP events around this launch on the stream it runs on, mean over the timed steps",
-            "kernels": kernels,
-            "gemm_stage": {"ms": best["gemm_ms"], "launches": len(kernels), "algorithmic_flops": flops,
-                           "achieved_tflops": flops / t_gemm / 1e12, "frac": flops / t_gemm / peak,
-                           "frac_executed": nprod * flops / t_gemm / peak},
+            "kernels": kernels, "unpack": unpack,
+            "gemm_stage": {"ms": best["gemm_ms"], "launches": len(kernels) + (1 if unpack else 0),
+                           "algorithmic_flops": flops, "achieved_tflops": flops / t_gemm / 1e12,
+                           "frac": flops / t_gemm / peak, "frac_executed": nprod * flops / t_gemm / peak,
+                           "note": "the stage includes the unpack of the spin-free products" if unpack else None},
             "plain_fp64_dense_frac": flops_dense / (plain["gemm_ms"] * 1e-3) / PEAK["fp64"]}
     value = best["value"]
     ms_step = best["ms_step"]
@@ -874,21 +1010,25 @@ def main():
         log(f"[bench] cpu BLAS ERI GEMM variant {cpu['blas_gemm_variant']['value']:.1f} grid pts/s "
             f"({cpu['blas_gemm_variant']['sample']})")
 
-    traffic = load_traffic(roof["kernel"], f"{args.n}-{dom['launch']}" if blocked else args.n)
+    traffic = load_traffic(roof["kernel"], f"{args.n}-{dom['launch']}" if mode != "dense" else args.n)
     if traffic is not None:
         roof["traffic"] = traffic["bytes_per_launch"]
         roof["traffic_source"] = traffic["source"]
 
-    def form_out(r, fl):
-        nprod_ = {"fp64": 1, "fp32": 1, "bf16x3": 6, "f16x2": 3, "i8x4": 10}[r["prec"]]
-        return {"grid_pts_per_s": r["value"], "evals_per_s": r["value"] / G, "ms_per_step": r["ms_step"],
-                "eri_gemm_ms": r["gemm_ms"], "device_ms": r["dev_ms"], "abs_dE_vs_plain_fp64": r.get("dE", 0.0),
-                "max_abs_dG_vs_plain_fp64": r.get("dG", 0.0), "speedup_vs_plain_fp64": r["value"] / base["value"],
+    def form_out(r, fl, structure):
+        nprod_ = NPROD[r["prec"]]
+        return {"structure": structure, "grid_pts_per_s": r["value"], "evals_per_s": r["value"] / G,
+                "ms_per_step": r["ms_step"], "eri_gemm_ms": r["gemm_ms"], "device_ms": r["dev_ms"],
+                "abs_dE_vs_plain_fp64": r.get("dE", 0.0), "max_abs_dG_vs_plain_fp64": r.get("dG", 0.0),
+                "speedup_vs_plain_fp64": r["value"] / base["value"],
+                "eri_gemm_algorithmic_flops": fl,
                 "eri_gemm_frac": fl / (r["gemm_ms"] * 1e-3) / PEAK[r["prec"]],
                 "eri_gemm_frac_executed": nprod_ * fl / (r["gemm_ms"] * 1e-3) / PEAK[r["prec"]],
-                "block_launch_ms": [float(x) for x in r["stages"][8:]]}
-    forms_out = {f: form_out(r, flops) for f, r in res.items()}
-    plain_out = form_out(plain, flops_dense)
+                "launch_ms": [float(x) for x in r["stages"][8:]]}
+    forms_out = {f: form_out(r, flops, mode) for f, r in res.items()}
+    plain_out = form_out(plain, flops_dense, "dense")
+    if blocks64 is not None:
+        forms_out["fp64_spin_blocks"] = form_out(blocks64, STRUCT_FLOPS["spin-blocks"](n_sp, G), "spin-blocks")
     batch = None
     if not args.no_batch:
         # the batch runs in a child process with a device of its own: its plain
@@ -911,9 +1051,11 @@ def main():
            "data": "synthetic",
            "config": {"workload": f"projected E+grad (EandG), synthetic n={args.n} (NSO={nso}), N={nocc}, "
                                   f"L={3*args.n}, grid 50x16={G}, device-resident objective",
-                      "form": best["prec"] + (" (spin blocks)" if blocked else ""), "evals_per_s": value / G,
+                      "form": f"{best['prec']} ({mode})", "evals_per_s": value / G,
                       "grid_points": G, "speedup_vs_plain_fp64": value / base["value"],
-                      "speedup_vs_fp64_same_blocks": value / res["fp64"]["value"], "abs_dE_vs_fp64": best["dE"],
+                      "speedup_vs_fp64_same_structure": value / res["fp64"]["value"],
+                      "speedup_vs_fp64_spin_blocks": value / blocks64["value"] if blocks64 else None,
+                      "abs_dE_vs_fp64": best["dE"],
                       "plain_fp64": plain_out, "forms": forms_out,
                       "path": "device-resident objective (phf_eval_objective: grid sums, the 1 x 1 EigenSolver and "
                               "LocalG on the device), factored Hamiltonian "

@@ -184,7 +184,9 @@ int phf_localg(phf_ctx* ctx, const double* gradmat, const double* rdmmat, const 
  * 16-bit/int8 slices itself, NSO <= 128, [6] covers both and [7] is ~0),
  * [8], [9], [10] the ERI GEMM launch of each spin block (D, X1, X2; [9]
  * holds X1 + X2 when the exchange blocks run as one grouped launch, and
- * [8] the whole GEMM for a dense A); it returns how many it wrote.
+ * [8] the whole GEMM for a dense A); in the spin-free form [8] the (2J - K)
+ * launch or the grouped launch of both products, [9] the K launch (0 when
+ * grouped), [10] the unpack into Gamma; it returns how many it wrote.
  * After a profiled phf_sc_apply / phf_sc_apply_packed(_dev): [0] all device
  * work, [1] stage-1 pair-transform GEMM, [2] stage-2 GEMM with the grid sum,
  * [3] P GEMM, [4] rank-term GEMM, [5] per-point t0/t1 kernel, [6] the rest
@@ -203,6 +205,19 @@ int phf_stage_times(const phf_ctx* ctx, double* ms, int nmax);
  * blocks.  phf_blocked reports 1 when evaluations run over the blocks. */
 int phf_set_blocked(phf_ctx* ctx, int on);
 int phf_blocked(const phf_ctx* ctx);
+
+/* Spin-free evaluation switch (no counterpart in the reference).  For a
+ * spin-restricted Ref (every alpha MO has a beta MO with the same spatial
+ * coefficients) whose A has the matching structure (checked on the device to
+ * 1e-12 of its largest entry), the blocks are D = [[J-K, J], [J, J-K]] and
+ * X1 = X2 = -K over spatial MO pairs, and Gamma comes from (2J - K) applied to
+ * rho_aa + rho_bb and K applied to rho_aa - rho_bb, rho_ab and rho_ba: four
+ * n^2 x n^2 products per density instead of six.  Built with the blocks;
+ * used by default (PHF_SPINFREE=0 at context creation, or
+ * phf_set_spinfree(ctx, 0), keeps the blocks).  phf_spinfree reports 1 when
+ * evaluations run through it. */
+int phf_set_spinfree(phf_ctx* ctx, int on);
+int phf_spinfree(const phf_ctx* ctx);
 
 /* Tile autotuner of the reduced-precision ERI GEMM (no counterpart in the
  * reference).  on = 1 (default): the first call for a problem shape times
@@ -348,7 +363,8 @@ int phf_sc_set_points(phf_ctx* ctx, int nso, int npts, const double* r, const do
  * the sequence in a hipGraph that later evaluations replay
  * (phf_batch_set_graph(b, 0) turns that off).  phf_batch_stats: [0] device
  * time of the last evaluation in ms, [1] its ERI GEMM time (-1 for a graph
- * replay), [2] 1 if it was a graph replay. */
+ * replay), [2] 1 if it was a graph replay, [3] the number of instances that
+ * evaluate through the spin-free form (phf_spinfree). */
 typedef struct phf_batch_s phf_batch;
 int phf_batch_create(int device, int ninst, phf_batch** out);
 int phf_batch_destroy(phf_batch* b);

@@ -34,6 +34,7 @@ EXPORTS = ["phf_ctx_create", "phf_ctx_destroy", "phf_last_error", "phf_set_hamil
            "phf_get_hone", "phf_eval_resident", "phf_localg_vo", "phf_batch_create", "phf_batch_destroy",
            "phf_batch_last_error", "phf_batch_set_instance", "phf_batch_eval", "phf_batch_set_graph",
            "phf_batch_autotune", "phf_batch_stats", "phf_batch_tuning_report", "phf_set_blocked", "phf_blocked",
+           "phf_set_spinfree", "phf_spinfree",
            "phf_buildsc", "phf_sc_set_grid", "phf_sc_apply", "phf_sc_apply_packed", "phf_sc_apply_packed_dev",
            "phf_sd_dim", "phf_sc_set_points", "phf_eval_objective", "phf_invert_batched",
            "phf_invert_batched_noalloc", "phf_pcc_olap0", "phf_pcc_olap"]
@@ -90,6 +91,8 @@ def load() -> C.CDLL:
     lib.phf_stage_times.argtypes = [C.c_void_p, _dp, C.c_int]
     lib.phf_set_blocked.argtypes = [C.c_void_p, C.c_int]
     lib.phf_blocked.argtypes = [C.c_void_p]
+    lib.phf_set_spinfree.argtypes = [C.c_void_p, C.c_int]
+    lib.phf_spinfree.argtypes = [C.c_void_p]
     lib.phf_batch_create.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_void_p)]
     lib.phf_batch_destroy.argtypes = [C.c_void_p]
     lib.phf_batch_last_error.argtypes = [C.c_void_p]
@@ -241,6 +244,14 @@ class Context:
 
     def blocked(self) -> bool:
         return bool(load().phf_blocked(self._h))
+
+    def set_spinfree(self, on: bool):
+        """Evaluate through the spin-free (2J - K, K) form when it is built
+        (True, the default) or through the spin blocks (False)."""
+        check(load().phf_set_spinfree(self._h, int(bool(on))), self._h)
+
+    def spinfree(self) -> bool:
+        return bool(load().phf_spinfree(self._h))
 
     def stage_times(self) -> np.ndarray:
         out = np.zeros(11)

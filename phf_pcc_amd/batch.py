@@ -116,9 +116,15 @@ class ProjectedHFBatch:
     def stats(self):
         """(device ms of the last evaluation, its ERI GEMM ms or -1 for a graph
         replay, 1.0 if it was a graph replay)."""
-        out = np.zeros(3)
-        _lib.load().phf_batch_stats(self._h, _lib.ptr(out), 3)
-        return tuple(out)
+        out = np.zeros(4)
+        _lib.load().phf_batch_stats(self._h, _lib.ptr(out), 4)
+        return tuple(out[:3])
+
+    def spinfree_instances(self) -> int:
+        """Instances evaluated through the spin-free (2J - K, K) form."""
+        out = np.zeros(4)
+        _lib.load().phf_batch_stats(self._h, _lib.ptr(out), 4)
+        return int(out[3])
 
     def tuning_report(self) -> str:
         buf = C.create_string_buffer(1 << 16)

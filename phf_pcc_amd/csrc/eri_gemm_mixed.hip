@@ -18,6 +18,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <type_traits>
+#include <vector>
 
 #include "phf_common.h"
 
@@ -680,68 +681,54 @@ __global__ __launch_bounds__(256) void split_rows_kernel(const double* __restric
   }
 }
 
-// Density pack fused with the split of its rows, for NSO <= 128: one
-// 1024-thread workgroup per grid point forms rho^T = conj(C0) X_g^T (NSO x NSO,
-// contraction N; C0 and X_g column-major NSO x N, complex interleaved) on the
-// FP64 matrix cores with the complex GEMM's exact MFMA sequence per output
-// (three real products per complex one, 4-deep steps in k order), takes the largest
-// |Re| and |Im| over the point, and writes the NS slices of both planes (rows
-// 2g and 2g + 1) straight into the ERI GEMM's operand, element (j, l) at
-// column ocol[j NSO + l] (j NSO + l without a map).  The slices are those
-// split_rows_kernel makes from the FP64 rows, bit for bit, without writing and
-// reading those rows.  Columns no element maps to are zeroed (padcols, or
-// [NSO^2, kout) without a map); workgroups past the last point clear the
-// padding rows up to rows_total (zero slices, inverse scale 1).
-template <typename T, int NS, int NT, int FM, int FN>
-__global__ __launch_bounds__(NT) void dens_split_kernel(int nso, int nk, const double* __restrict__ C0,
-                                                          const double* __restrict__ X, long sX, int npts,
-                                                          const int* __restrict__ ocol,
-                                                          const int* __restrict__ padcols, int npad, int kout,
-                                                          long rows_total, T* __restrict__ dst, int top,
-                                                          double* __restrict__ inv) {
+// The point's density tile rho^T = conj(C0) X_g^T (128 x 128, contraction nk;
+// C0 and X_g column-major NSO x N, complex interleaved) on the FP64 matrix
+// cores with the complex GEMM's exact MFMA sequence per output (three real
+// products per complex one, 4-deep steps in k order).  On return p1 holds the
+// real and p2 the imaginary parts: fragment (i, j), register r is tile row
+// wm 16 FM + 16 i + (lane >> 4) + 4 r, tile column wn 16 FN + 16 j + (lane & 15)
+// (waves of (16 FM) x (16 FN), WN = 128 / (16 FN) across).  Without maps tile
+// row / column q is MO q (q < nso); with MAPPED, rmap[q] / cmap[q] name the MO
+// (-1: an empty row or column, zero).
+template <int NT, int FM, int FN, bool MAPPED>
+__device__ __forceinline__ void dens_tile(int nso, int nk, const double* __restrict__ C0,
+                                          const double* __restrict__ Xg, const int* __restrict__ rmap,
+                                          const int* __restrict__ cmap, d4_t (&p1)[FM][FN], d4_t (&p2)[FM][FN]) {
   constexpr int TB = 128, KC = 16, LD = KC + 2, WN = TB / (16 * FN);
   __shared__ double Ar[TB * LD], Ai[TB * LD], Br[TB * LD], Bi[TB * LD];
-  __shared__ double red[2][NT / 64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int g = blockIdx.x;
-  const T zero = static_cast<T>(0.0f);
-  if (g >= npts) {
-    for (int pr = 0; pr < 2; ++pr) {
-      const long row = 2L * npts + 2L * (g - npts) + pr;
-      if (row >= rows_total) break;
-      T* d = dst + row * (long)NS * kout;
-      for (long c = tid; c < (long)NS * kout; c += NT) d[c] = zero;
-      if (tid == 0) inv[row] = 1.0;
-    }
-    return;
-  }
-  const int wm = wave / WN, wn = wave % WN;   // waves of (16 FM) x (16 FN)
-  const double* Xg = X + (long)g * sX;
-  d4_t p1[FM][FN], p2[FM][FN], p3[FM][FN];
+  const int wm = wave / WN, wn = wave % WN;
+  d4_t p3[FM][FN];
   // every load of a 16-deep k-chunk is issued before the first LDS store;
   // the first chunk is peeled so that its loads are consumed before the
   // accumulators come alive
   constexpr int EL = TB * KC / NT;
   auto stage = [&](int k0) {
     double lar[EL], lai[EL], lbr[EL], lbi[EL];
-    bool lv[EL];
+    bool la[EL], lb[EL];
 #pragma unroll
     for (int e = 0; e < EL; ++e) {
       const int idx = e * NT + tid, i = idx % TB, gk = k0 + idx / TB;
-      lv[e] = i < nso && gk < nk;
-      const long off = lv[e] ? 2L * (i + (long)gk * nso) : 0;
-      lar[e] = C0[off];
-      lai[e] = C0[off + 1];
-      lbr[e] = Xg[off];
-      lbi[e] = Xg[off + 1];
+      int ra = i, rb = i;
+      if constexpr (MAPPED) {
+        ra = rmap[i];
+        rb = cmap[i];
+      }
+      la[e] = (MAPPED ? ra >= 0 : i < nso) && gk < nk;
+      lb[e] = (MAPPED ? rb >= 0 : i < nso) && gk < nk;
+      const long offa = la[e] ? 2L * (ra + (long)gk * nso) : 0, offb = lb[e] ? 2L * (rb + (long)gk * nso) : 0;
+      lar[e] = C0[offa];
+      lai[e] = C0[offa + 1];
+      lbr[e] = Xg[offb];
+      lbi[e] = Xg[offb + 1];
     }
 #pragma unroll
     for (int e = 0; e < EL; ++e) {
       const int idx = e * NT + tid, i = idx % TB, kk = idx / TB;
-      Ar[i * LD + kk] = lv[e] ? lar[e] : 0.0;
-      Ai[i * LD + kk] = lv[e] ? -lai[e] : 0.0;
-      Br[i * LD + kk] = lv[e] ? lbr[e] : 0.0;
-      Bi[i * LD + kk] = lv[e] ? lbi[e] : 0.0;
+      Ar[i * LD + kk] = la[e] ? lar[e] : 0.0;
+      Ai[i * LD + kk] = la[e] ? -lai[e] : 0.0;
+      Br[i * LD + kk] = lb[e] ? lbr[e] : 0.0;
+      Bi[i * LD + kk] = lb[e] ? lbi[e] : 0.0;
     }
   };
   auto mfma_chunk = [&]() {
@@ -787,8 +774,6 @@ __global__ __launch_bounds__(NT) void dens_split_kernel(int nso, int nk, const d
     __syncthreads();
     mfma_chunk();
   }
-  // complex results and the point's largest magnitudes
-  double mre = 0.0, mim = 0.0;
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -798,10 +783,72 @@ __global__ __launch_bounds__(NT) void dens_split_kernel(int nso, int nk, const d
         const double xr = p1[i][j][r] - p2[i][j][r], xi = p3[i][j][r] - p1[i][j][r] - p2[i][j][r];
         p1[i][j][r] = xr;
         p2[i][j][r] = xi;
+      }
+}
+
+// NS slices of x (scaled) at element c of the row slices d (slice stride kout).
+template <typename T, int NS>
+__device__ __forceinline__ void put_slices(T* __restrict__ d, long kout, long c, double x) {
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    if constexpr (std::is_same<T, signed char>::value) {
+      const double v = fmin(127.0, fmax(-127.0, rint(x)));
+      d[(long)q * kout + c] = (signed char)v;
+      x = (x - v) * 128.0;
+    } else {
+      const T v = to16<T>(x);
+      d[(long)q * kout + c] = v;
+      x -= (double)(float)v;
+    }
+  }
+}
+
+// Density pack fused with the split of its rows, for NSO <= 128: one
+// 512-thread workgroup per grid point forms rho^T = conj(C0) X_g^T (dens_tile),
+// takes the largest |Re| and |Im| over the point, and writes the NS slices of
+// both planes (rows 2g and 2g + 1) straight into the ERI GEMM's operand,
+// element (j, l) at column ocol[j NSO + l] (j NSO + l without a map).  The
+// slices are those split_rows_kernel makes from the FP64 rows, bit for bit,
+// without writing and reading those rows.  Columns no element maps to are
+// zeroed (padcols, or [NSO^2, kout) without a map); workgroups past the last
+// point clear the padding rows up to rows_total (zero slices, inverse scale 1).
+template <typename T, int NS, int NT, int FM, int FN>
+__global__ __launch_bounds__(NT) void dens_split_kernel(int nso, int nk, const double* __restrict__ C0,
+                                                          const double* __restrict__ X, long sX, int npts,
+                                                          const int* __restrict__ ocol,
+                                                          const int* __restrict__ padcols, int npad, int kout,
+                                                          long rows_total, T* __restrict__ dst, int top,
+                                                          double* __restrict__ inv) {
+  constexpr int WN = 128 / (16 * FN);
+  __shared__ double red[2][NT / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int g = blockIdx.x;
+  const T zero = static_cast<T>(0.0f);
+  if (g >= npts) {
+    for (int pr = 0; pr < 2; ++pr) {
+      const long row = 2L * npts + 2L * (g - npts) + pr;
+      if (row >= rows_total) break;
+      T* d = dst + row * (long)NS * kout;
+      for (long c = tid; c < (long)NS * kout; c += NT) d[c] = zero;
+      if (tid == 0) inv[row] = 1.0;
+    }
+    return;
+  }
+  const int wm = wave / WN, wn = wave % WN;
+  d4_t p1[FM][FN], p2[FM][FN];
+  dens_tile<NT, FM, FN, false>(nso, nk, C0, X + (long)g * sX, nullptr, nullptr, p1, p2);
+  // complex results and the point's largest magnitudes
+  double mre = 0.0, mim = 0.0;
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
         const int gi = wm * 16 * FM + i * 16 + (lane >> 4) + 4 * r, gj = wn * 16 * FN + j * 16 + (lane & 15);
         if (gi < nso && gj < nso) {
-          mre = fmax(mre, fabs(xr));
-          mim = fmax(mim, fabs(xi));
+          mre = fmax(mre, fabs(p1[i][j][r]));
+          mim = fmax(mim, fabs(p2[i][j][r]));
         }
       }
 #pragma unroll
@@ -845,23 +892,8 @@ __global__ __launch_bounds__(NT) void dens_split_kernel(int nso, int nk, const d
         const int src = gi * nso + gj;
         const int c = ocol ? ocol[src] : src;
         if (c < 0) continue;
-        double rr = p1[i][j][r] * sc[0], ri = p2[i][j][r] * sc[1];
-#pragma unroll
-        for (int q = 0; q < NS; ++q) {
-          if constexpr (std::is_same<T, signed char>::value) {
-            const double vr = fmin(127.0, fmax(-127.0, rint(rr))), vi = fmin(127.0, fmax(-127.0, rint(ri)));
-            d0[(long)q * kout + c] = (signed char)vr;
-            d1[(long)q * kout + c] = (signed char)vi;
-            rr = (rr - vr) * 128.0;
-            ri = (ri - vi) * 128.0;
-          } else {
-            const T vr = to16<T>(rr), vi = to16<T>(ri);
-            d0[(long)q * kout + c] = vr;
-            d1[(long)q * kout + c] = vi;
-            rr -= (double)(float)vr;
-            ri -= (double)(float)vi;
-          }
-        }
+        put_slices<T, NS>(d0, kout, c, p1[i][j][r] * sc[0]);
+        put_slices<T, NS>(d1, kout, c, p2[i][j][r] * sc[1]);
       }
   // output columns no element maps to
   if (ocol) {
@@ -881,6 +913,113 @@ __global__ __launch_bounds__(NT) void dens_split_kernel(int nso, int nk, const d
         d1[(long)q * kout + c] = zero;
       }
   }
+}
+
+// Spin-free operand (labels s < n <= 64; smap from dens_split_rjk_maps): the
+// staging maps put label La's alpha and beta MOs on the tile rows of fragments
+// 2m and 2m + 1 and label Lc's on the columns of fragments 0 and 1, so one
+// lane holds P[(alpha b, alpha d)], P[(beta b, beta d)], P[(beta b, alpha d)] and
+// P[(alpha b, beta d)] of a label pair (b, d) and writes, at column b n + d,
+// set 0 = P_aa + P_bb (row 2g + plane), set 1 = P_aa - P_bb, set 2 = P_x1,
+// set 3 = P_x2 (rows Mp + (set - 1) 2 npts + 2g + plane), each (set, plane)
+// row with its own scale.  The values and slices equal those of the density
+// pack, rjk_pack_kernel and split_rows_kernel, bit for bit.  Columns
+// [n^2, kj) are zeroed; workgroups past the last point clear the padding rows
+// [2 npts, Mp) and [Mp + 6 npts, Mp + Mk).
+template <typename T, int NS, int NT, int FM, int FN>
+__global__ __launch_bounds__(NT) void dens_split_rjk_kernel(int n, int nso, int nk, const double* __restrict__ C0,
+                                                              const double* __restrict__ X, long sX, int npts,
+                                                              const int* __restrict__ smap, int kj, long Mp,
+                                                              long Mk, T* __restrict__ dst, int top,
+                                                              double* __restrict__ inv) {
+  static_assert(FM == 4 && FN == 2, "label layout assumes 4 x 2 fragments per wave");
+  constexpr int WN = 128 / (16 * FN);
+  __shared__ double red[8][NT / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int g = blockIdx.x;
+  const T zero = static_cast<T>(0.0f);
+  const long rs = (long)NS * kj;   // operand row stride
+  if (g >= npts) {
+    const long pad0 = Mp - 2L * npts;
+    for (int pr = 0; pr < 2; ++pr) {
+      const long p = 2L * (g - npts) + pr;
+      const long row = p < pad0 ? 2L * npts + p : Mp + 6L * npts + (p - pad0);
+      if (row >= Mp + Mk) break;
+      T* d = dst + row * rs;
+      for (long c = tid; c < rs; c += NT) d[c] = zero;
+      if (tid == 0) inv[row] = 1.0;
+    }
+    return;
+  }
+  const int wm = wave / WN, wn = wave % WN;
+  d4_t p1[FM][FN], p2[FM][FN];
+  dens_tile<NT, FM, FN, true>(nso, nk, C0, X + (long)g * sX, smap, smap + 128, p1, p2);
+  const int lc = wn * 16 + (lane & 15);
+  // the four sets of one (m, r): value q = set * 2 + plane
+  auto sets = [&](int m, int r, double (&v)[8]) {
+    const double ar = p1[2 * m][0][r], ai = p2[2 * m][0][r], br = p1[2 * m + 1][1][r], bi = p2[2 * m + 1][1][r];
+    v[0] = ar + br;
+    v[1] = ai + bi;
+    v[2] = ar - br;
+    v[3] = ai - bi;
+    v[4] = p1[2 * m + 1][0][r];
+    v[5] = p2[2 * m + 1][0][r];
+    v[6] = p1[2 * m][1][r];
+    v[7] = p2[2 * m][1][r];
+  };
+  double mx[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) mx[q] = 0.0;
+#pragma unroll
+  for (int m = 0; m < 2; ++m)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int la = wm * 32 + m * 16 + (lane >> 4) + 4 * r;
+      if (la >= n || lc >= n) continue;
+      double v[8];
+      sets(m, r, v);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) mx[q] = fmax(mx[q], fabs(v[q]));
+    }
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx[q] = fmax(mx[q], __shfl_xor(mx[q], off));
+    if (lane == 0) red[q][wave] = mx[q];
+  }
+  __syncthreads();
+  double sc[8];
+  T* drow[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    double m = 0.0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) m = fmax(m, red[q][w]);
+    int e = 0;
+    if (m > 0.0) frexp(m, &e);
+    sc[q] = (m > 0.0) ? ldexp(1.0, top - e) : 1.0;
+    const int set = q >> 1, pl = q & 1;
+    const long row = set == 0 ? 2L * g + pl : Mp + (long)(set - 1) * 2L * npts + 2L * g + pl;
+    drow[q] = dst + row * rs;
+    if (tid == 0) inv[row] = 1.0 / sc[q];
+  }
+#pragma unroll
+  for (int m = 0; m < 2; ++m)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int la = wm * 32 + m * 16 + (lane >> 4) + 4 * r;
+      if (la >= n || lc >= n) continue;
+      double v[8];
+      sets(m, r, v);
+      const long c = (long)la * n + lc;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) put_slices<T, NS>(drow[q], kj, c, v[q] * sc[q]);
+    }
+  for (long c = (long)n * n + tid; c < kj; c += NT)
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+#pragma unroll
+      for (int s = 0; s < NS; ++s) drow[q][(long)s * kj + c] = zero;
 }
 
 __global__ void to_f32_kernel(const double* __restrict__ src, float* __restrict__ dst, long n) {
@@ -970,6 +1109,34 @@ void dens_split(hipStream_t s, int prec, int nso, int nk, const double* C0, cons
   else
     dens_split_kernel<signed char, 4, DS_NT, DS_FM, DS_FN><<<grid, DS_NT, 0, s>>>(
         nso, nk, C0, X, sX, npts, ocol, padcols, npad, kout, rows_total, (signed char*)dst, 7, inv);
+}
+
+void dens_split_rjk_maps(const std::vector<int>& mo, int n, int* smap) {
+  // tile row q: wave row wm = q / 64, fragment f = (q % 64) / 16 (pair m = f / 2,
+  // spin f % 2), x = q % 16 -> label wm 32 + 16 m + x; tile column q: wave
+  // column q / 32, fragment (q % 32) / 16 = spin, x = q % 16 -> label (q / 32) 16 + x
+  for (int q = 0; q < 128; ++q) {
+    const int f = (q % 64) / 16, lr = (q / 64) * 32 + (f / 2) * 16 + q % 16;
+    smap[q] = lr < n ? mo[(f % 2) * n + lr] : -1;
+    const int lc = (q / 32) * 16 + q % 16;
+    smap[128 + q] = lc < n ? mo[((q % 32) / 16) * n + lc] : -1;
+  }
+}
+
+void dens_split_rjk(hipStream_t s, int prec, int n, int nk, const double* C0, const double* X, long sX, int npts,
+                    const int* smap, int kj, long Mp, long Mk, void* dst, double* inv) {
+  const long pad = (Mp - 2L * npts) + (Mk - 6L * npts);
+  const unsigned grid = (unsigned)(npts + (pad + 1) / 2);
+  const int nso = 2 * n;
+  if (prec == 3)
+    dens_split_rjk_kernel<_Float16, 2, DS_NT, DS_FM, DS_FN><<<grid, DS_NT, 0, s>>>(
+        n, nso, nk, C0, X, sX, npts, smap, kj, Mp, Mk, (_Float16*)dst, 14, inv);
+  else if (prec == 2)
+    dens_split_rjk_kernel<__bf16, 3, DS_NT, DS_FM, DS_FN><<<grid, DS_NT, 0, s>>>(
+        n, nso, nk, C0, X, sX, npts, smap, kj, Mp, Mk, (__bf16*)dst, 14, inv);
+  else
+    dens_split_rjk_kernel<signed char, 4, DS_NT, DS_FM, DS_FN><<<grid, DS_NT, 0, s>>>(
+        n, nso, nk, C0, X, sX, npts, smap, kj, Mp, Mk, (signed char*)dst, 7, inv);
 }
 
 void split_operand(hipStream_t s, int prec, const double* src, long lds, int rows, int kt, void* dst, double* inv) {

@@ -153,8 +153,13 @@ int upload_items(phf_batch* bt, const std::vector<H16Item>& items) {
 // Zero fills (split contractions) and the grouped launch of the chosen work list.
 void launch_grouped(phf_batch* bt, int prec) {
   if (bt->splitk > 1)
-    for (size_t i = 0; i < bt->inst.size(); ++i)
-      fill_zero(bt->stream, bt->bufs[i].Gam, bt->bufs[i].Mp * bt->inst[i]->nrpad);
+    for (size_t i = 0; i < bt->inst.size(); ++i) {
+      const EvalBufs& b = bt->bufs[i];
+      if (b.rjk)
+        fill_zero(bt->stream, b.Cjk, (b.Mp + b.Mk) * b.kP);
+      else
+        fill_zero(bt->stream, b.Gam, b.Mp * bt->inst[i]->nrpad);
+    }
   eri_gemm_mixed_grouped(bt->stream, prec, bt->d_items, bt->nitems, bt->cfg, bt->grid, bt->splitk > 1);
 }
 
@@ -169,8 +174,9 @@ int tune_grouped(phf_batch* bt, int prec) {
     bool fits = true;   // A rows (dense or per block) are multiples of 256
     for (phf_ctx* x : bt->inst) {
       fits = fits && (x->nrpad % h16_cfg_bn(c) == 0);
-      for (int q = 0; q < 3; ++q)   // every contraction length of the instance (dense, or each block)
-        fits = fits && h16_cfg_serves(c, prec, (x->blocked && x->use_blocks) ? x->blk_k[q] : x->kt);
+      for (int q = 0; q < 3; ++q)   // every contraction length of the instance (dense, each block, spin-free)
+        fits = fits && h16_cfg_serves(c, prec, phf_spinfree(x) ? x->rj_kj
+                                                               : (x->blocked && x->use_blocks) ? x->blk_k[q] : x->kt);
     }
     if (!fits) continue;
     for (int sk : {1, 2})
@@ -278,6 +284,7 @@ int enqueue_eval(phf_batch* bt, int prec, bool timed) {
     phf_ctx* c = bt->inst[i];
     const EvalBufs& b = bt->bufs[i];
     on_side(r, c);
+    if (prec != PHF_PREC_FP64 && b.rjk) rjk_unpack(c, b, c->stream);   // Gamma + h from the grouped products
     eval_post(c, b, true);
     const int nel = (c->nso - b.N) * b.N;
     const int blocks = std::max(1, std::min(1024, (nel + 255) / 256));
@@ -392,10 +399,8 @@ extern "C" int phf_batch_set_instance(phf_batch* bt, int i, const double* h, con
   rc = phf_set_grid(c, ref, ngrid, 1, 1, r1, nr1, r2, nr2, rpg, npg, weightsp, weightpg ? weightpg : wpg.data(), 1,
                     &one, 1, &one, 1, &one, 1, 0, 1);
   if (rc) return done(rc);
-  if (storage != PHF_PREC_FP64) {
-    rc = phf_keep_only_mixed(c, storage);
-    if (rc) return done(rc);
-  }
+  rc = storage != PHF_PREC_FP64 ? phf_keep_only_mixed(c, storage) : phf_keep_used_structure(c);
+  if (rc) return done(rc);
   rc = phf_release_ws(c);
   return done(rc);
 }
@@ -573,8 +578,10 @@ extern "C" int phf_batch_eval(phf_batch* bt, const double* z, int cmplxconj, int
 
 extern "C" int phf_batch_stats(const phf_batch* bt, double* ms, int nmax) {
   if (!bt || !ms || nmax <= 0) return 0;
-  const double v[3] = {bt->ms_total, bt->last_graph ? -1.0 : bt->ms_gemm, bt->last_graph ? 1.0 : 0.0};
-  const int n = std::min(nmax, 3);
+  int nsf = 0;
+  for (const phf_ctx* c : bt->inst) nsf += c ? phf_spinfree(c) : 0;
+  const double v[4] = {bt->ms_total, bt->last_graph ? -1.0 : bt->ms_gemm, bt->last_graph ? 1.0 : 0.0, (double)nsf};
+  const int n = std::min(nmax, 4);
   for (int i = 0; i < n; ++i) ms[i] = v[i];
   return n;
 }

@@ -29,6 +29,14 @@ void convert_f32(hipStream_t s, const double* src, float* dst, long n);
 int dens_split_max_nso();
 void dens_split(hipStream_t s, int prec, int nso, int nk, const double* C0, const double* X, long sX, int npts,
                 const int* ocol, const int* padcols, int npad, int kout, long rows_total, void* dst, double* inv);
+// The same pass writing the spin-free operand (n <= 64 labels): smap = tile
+// row -> MO [128] then tile column -> MO [128]; rows [0, Mp) P_aa + P_bb,
+// [Mp, Mp + Mk) P_aa - P_bb, P_x1, P_x2 (2 npts rows each); kj columns.
+void dens_split_rjk(hipStream_t s, int prec, int n, int nk, const double* C0, const double* X, long sX, int npts,
+                    const int* smap, int kj, long Mp, long Mk, void* dst, double* inv);
+// Staging maps of dens_split_rjk for the pairing mo (label s: alpha MO mo[s],
+// beta MO mo[n + s]): smap[0..128) tile row -> MO, smap[128..256) tile column -> MO.
+void dens_split_rjk_maps(const std::vector<int>& mo, int n, int* smap);
 void eri_gemm_mixed(hipStream_t s, int prec, const void* P, long kp, const void* A, const double* invP,
                     const double* invA, double* C, int Mp, int Nr, int K, long ldc, int cfg, int splitk,
                     const double* bias, const int* colmap);
@@ -91,6 +99,27 @@ struct phf_ctx_s {
   void* dAbm[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   double* dAbinv[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   std::vector<int> blk_spin; // MO spins (0 alpha, 1 beta) the blocks were built for
+  // Spin-free form (spin-restricted Ref: every alpha MO has a beta partner
+  // with the same spatial coefficients, label s = 0..n-1).  With the spatial
+  // MO integrals J[(a,c),(b,d)] = (ac|bd) and K[(a,c),(b,d)] = (ad|bc) the
+  // blocks are D = [[J-K, J], [J, J-K]], X1 = X2 = -K, so
+  //   Gamma_aa + Gamma_bb = (2J - K)(P_aa + P_bb),  Gamma_aa - Gamma_bb = -K (P_aa - P_bb),
+  //   Gamma_ab = -K P_x1,  Gamma_ba = -K P_x2
+  // (P_aa(b,d) = P[(alpha b, alpha d)], P_bb likewise, P_x1(b,d) = P[(beta b, alpha d)],
+  // P_x2(b,d) = P[(alpha b, beta d)]): four n^2 x n^2 products per density
+  // instead of the blocks' six.  dAj holds [2J - K; K] in FP64 ([2][rj_kj][rj_kj]),
+  // dAjm/dAjinv the reduced copies; d_rjmo the MOs of each label
+  // ([0..n) alpha, [n..2n) beta); d_rjsmap the fused density kernel's staging
+  // maps (tile row -> MO [128], tile column -> MO [128], -1 empty).
+  bool rjk = false;          // built for the current Hamiltonian and Ref
+  bool use_rjk = true;       // evaluate through it when built (and the blocks are on)
+  int rj_n = 0, rj_kj = 0;
+  double* dAj = nullptr;
+  void* dAjm[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  double* dAjinv[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  int* d_rjmo = nullptr;
+  int* d_rjsmap = nullptr;
+  std::vector<int> rj_mo;    // host copy of d_rjmo (the pairing it was built for)
   // workspace arena
   char* ws = nullptr;
   size_t ws_size = 0;
@@ -160,6 +189,7 @@ struct phf_ctx_s {
   // launch when the exchange blocks run as one)
   hipEvent_t bev[3][2] = {};
   int nblk_timed = 0;
+  bool rjk_grouped_last = false;   // the spin-free products ran as one grouped launch (bev[1] is empty)
   // workspace of phf_pcc_olap0 (the projected-CC grid consumer)
   char* pcc_ws = nullptr;
   size_t pcc_ws_size = 0;
@@ -206,11 +236,16 @@ std::complex<double> phf_wignerfac(int j, int n, int m, double alpha, double bet
 int phf_ctx_init(phf_ctx* ctx, int device, hipStream_t shared_stream);
 int phf_release_ws(phf_ctx* ctx);
 int phf_keep_only_mixed(phf_ctx* ctx, int prec);
+// Keep only the FP64 copy of the structure evaluations use (the spin-free form
+// when built and on, else the spin blocks).
+int phf_keep_used_structure(phf_ctx* ctx);
 // Build the spin blocks of the cached real A for MO spins `spin` (size nso),
 // after checking on the device that every entry outside them is exactly zero;
 // returns PHF_OK without building when the check fails.  release_dense: free
-// the dense A afterwards.
-int phf_build_blocks(phf_ctx* ctx, const std::vector<int>& spin, bool release_dense);
+// the dense A afterwards.  With ref (row-major complex, the Ref the
+// Hamiltonian is expressed in) the spin-free form is built too when Ref is
+// spin-restricted and A has that structure.
+int phf_build_blocks(phf_ctx* ctx, const std::vector<int>& spin, bool release_dense, const double* ref = nullptr);
 // MO spins of a real or complex Ref (nso x nso, row-major complex): 0 for a
 // column with exactly zero beta rows, 1 for exactly zero alpha rows; empty if
 // some column mixes spins.
@@ -233,6 +268,14 @@ struct EvalBufs {
   bool blk = false;           // the GEMM runs over the spin blocks
   long kP = 0;                // row length (per slice) of the GEMM's density operand
   double* Pperm = nullptr;    // FP64 density rows in permuted column order (blk, FP64 form)
+  // spin-free form: operand rows [0, Mp) P_aa + P_bb of every density (2 Gv
+  // data rows); rows [Mp, Mp + Mk) P_aa - P_bb, P_x1, P_x2 (2 Gv rows each,
+  // in that order), kP = rj_kj columns; Cjk the products in the same row
+  // order (FP64), unpacked into Gam by rjk_unpack
+  bool rjk = false;
+  long Mk = 0;
+  double* Cjk = nullptr;      // shares its memory with Pt (dead once the operand is packed)
+  bool fused = false;         // the density pack writes the reduced operand directly (no Pt)
 };
 
 // Stages of one evaluation on ctx->stream.  eval_plan checks the cached state,
@@ -251,5 +294,8 @@ int eval_gemm(phf_ctx* ctx, const EvalBufs& b);
 void build_h16_items(const std::vector<std::pair<const phf_ctx*, const EvalBufs*>>& units, int prec, int cfg,
                      int sk, int grid, std::vector<H16Item>& out, unsigned blk_mask = 7u);
 void eval_post(phf_ctx* ctx, const EvalBufs& b, bool resident);
+// Spin-free form: Gamma + h (b.Gam, natural column order) from the products in
+// b.Cjk; eval_gemm runs it after its launches (the batch after its grouped one).
+void rjk_unpack(phf_ctx* ctx, const EvalBufs& b, hipStream_t s);
 // C0 = [I; Z] column-major complex (NSO x N) from Z ((NSO-N) x N, C order)
 void build_c0(const double* z, int nso, int N, int cmplxconj, double* c0);

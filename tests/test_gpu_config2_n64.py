@@ -5,6 +5,7 @@ is measured on.
 Checks against the oracle's C restatement of BuildHSG (oracle/buildhsg_cref.c,
 PHFTools.f90:10-201) on the whole grid, every element of Hmat, Smat, Gradmat
 and Rdmmat to 1e-10 of the largest entry of each:
+  * FP64 through the spin-free form (2J - K and K over spatial pairs, K = 4096),
   * FP64 through the spin blocks (D: K = 8192, X1/X2: K = 4096),
   * FP64 through the dense A (the plain form, K = 16384),
   * the PHFTools_F drop-in with the dense host htwo (4.3 GB complex128).
@@ -44,16 +45,22 @@ def cfg2():
     model.ctx.close()
 
 
-@pytest.mark.parametrize("blocked", [True, False], ids=["spin-blocks", "dense"])
-def test_fp64_matches_c_oracle_full_grid(cfg2, blocked):
+def _structure(m, mode):
+    """mode: "spin-free", "spin-blocks" or "dense"."""
+    m.ctx.set_blocked(mode != "dense")
+    m.ctx.set_spinfree(mode == "spin-free")
+    assert m.ctx.blocked() == (mode != "dense") and m.ctx.spinfree() == (mode == "spin-free")
+
+
+@pytest.mark.parametrize("mode", ["spin-free", "spin-blocks", "dense"])
+def test_fp64_matches_c_oracle_full_grid(cfg2, mode):
     m = cfg2["model"]
-    m.ctx.set_blocked(blocked)
+    _structure(m, mode)
     m.precision = "fp64"
     try:
         gpu = m.buildhsg(cfg2["prob"].z)
     finally:
-        m.ctx.set_blocked(True)
-    assert m.ctx.blocked() or not blocked
+        _structure(m, "spin-free")
     for name, a, b in zip(("Hmat", "Smat", "Gradmat", "Rdmmat"), gpu, cfg2["ref"]):
         assert _rel(a, b) < 1e-10, name
 
@@ -77,15 +84,18 @@ def test_objective_matches_oracle_eandg(cfg2):
     assert np.abs(g - cfg2["g_ref"]).max() < 1e-10
 
 
+@pytest.mark.parametrize("mode", ["spin-free", "spin-blocks"])
 @pytest.mark.parametrize("prec", ["f16x2", "i8x4"])
-def test_reduced_forms_within_budget(cfg2, prec):
+def test_reduced_forms_within_budget(cfg2, prec, mode):
     m = cfg2["model"]
+    _structure(m, mode)
     m.precision = prec
     try:
         e, g = m.energy_and_gradient(cfg2["prob"].z)
     finally:
         m.precision = "fp64"
-    print(f"[cfg2] {prec}: |dE| = {abs(e - cfg2['e_ref']):.3e}", flush=True)
+        _structure(m, "spin-free")
+    print(f"[cfg2] {prec} {mode}: |dE| = {abs(e - cfg2['e_ref']):.3e}", flush=True)
     assert abs(e - cfg2["e_ref"]) <= 1e-6
     assert np.abs(g - cfg2["g_ref"]).max() <= 1e-5
 
@@ -97,6 +107,7 @@ def test_rn16_tile_at_its_contraction_limit(cfg2, monkeypatch):
     m = cfg2["model"]
     monkeypatch.setenv("PHF_H16_CFG", "5")
     m.ctx.autotune(True)               # drops the tuned choices, so the forced one is taken
+    _structure(m, "spin-blocks")       # the D block; the spin-free products contract over K = 4096
     m.precision = "f16x2"
     try:
         rep_before = len(m.ctx.tuning_report())
@@ -106,5 +117,6 @@ def test_rn16_tile_at_its_contraction_limit(cfg2, monkeypatch):
         m.precision = "fp64"
         monkeypatch.delenv("PHF_H16_CFG")
         m.ctx.autotune(True)
+        _structure(m, "spin-free")
     assert '"k": 24576' in rep and '"chosen": {"cfg": 5' in rep   # D block: K = 8192 x three segments
     assert abs(e - cfg2["e_ref"]) <= 1e-6

@@ -11,7 +11,9 @@ Checks:
     against the oracle in the other GPU tests.
   * F16X2, I8X4 and FP32 batches within 1e-6 Eh and 1e-5 (gradient) of the
     FP64 batch on every instance (a failed Hermiticity test does not change
-    the arithmetic: E0 = Re H / |Re S| is formed all the same).
+    the arithmetic: E0 = Re H / |Re S| is formed all the same).  The
+    instances' Refs are spin-restricted, so these run the spin-free form; the
+    F16X2 batch over the spin blocks (PHF_SPINFREE=0) is held to the same bar.
   * graph replays: a replay of the first Z after another Z reproduces it bit
     for bit.
 """
@@ -56,9 +58,10 @@ def test_fp64_batch_equals_single_instances(cfg4):
             m.ctx.close()
 
 
-@pytest.mark.parametrize("prec", ["f16x2", "i8x4", "fp32"])
-def test_reduced_batch_forms(cfg4, prec):
+@pytest.mark.parametrize("prec,spinfree", [("f16x2", "1"), ("i8x4", "1"), ("fp32", "1"), ("f16x2", "0")])
+def test_reduced_batch_forms(cfg4, prec, spinfree, monkeypatch):
     from phf_pcc_amd.batch import ProjectedHFBatch
+    monkeypatch.setenv("PHF_SPINFREE", spinfree)   # read when the instances' contexts are created
     bt = ProjectedHFBatch(cfg4["inst"], storage=prec)
     try:
         e, g, _ = bt.energy_and_gradient(cfg4["zs"], prec, strict=False)

@@ -39,7 +39,8 @@ def test_fused_equals_two_pass(n, ngrid, blocked, prec):
     ga = grid_args(n, ngrid, 0)
     model = ProjectedHF(p.nocc, p.ref, ga, factors=(p.h_spatial, p.b), precision=prec)
     model.ctx.set_blocked(blocked)
-    assert model.ctx.blocked() == blocked
+    model.ctx.set_spinfree(False)   # the spin blocks' column map (tests/test_gpu_spinfree.py covers the other)
+    assert model.ctx.blocked() == blocked and not model.ctx.spinfree()
     print(f"fused split n={n} G={int(np.prod(ngrid))} blocked={blocked} {prec}", flush=True)
     a = _run(model, p.z, True)
     b = _run(model, p.z, False)

@@ -44,9 +44,13 @@ class _pinned:
 
 
 def _eval(prob, prec, structure=True, cfg=None, sk=1):
+    """structure: True / "spinfree" (the spin-free form of the restricted Ref),
+    "blocks" (the spin blocks D, X1, X2), False (the dense A)."""
     from phf_pcc_amd.model import ProjectedHF
     p, ga = prob
-    m = ProjectedHF(p.nocc, p.ref, ga, factors=(p.h_spatial, p.b), structure=structure, precision=prec)
+    m = ProjectedHF(p.nocc, p.ref, ga, factors=(p.h_spatial, p.b), structure=bool(structure), precision=prec)
+    m.ctx.set_spinfree(structure != "blocks")
+    assert m.ctx.spinfree() == (structure in (True, "spinfree"))
     if cfg is None:
         return m.energy_and_gradient(p.z)
     with _pinned(cfg, sk):
@@ -57,10 +61,10 @@ def _eval(prob, prec, structure=True, cfg=None, sk=1):
 
 @pytest.fixture(scope="module")
 def fp64(prob):
-    return {st: _eval(prob, "fp64", structure=st) for st in (True, False)}
+    return {st: _eval(prob, "fp64", structure=st) for st in (True, "blocks", False)}
 
 
-@pytest.mark.parametrize("structure", [True, False])
+@pytest.mark.parametrize("structure", [True, "blocks", False])
 @pytest.mark.parametrize("sk", [1, 2])
 @pytest.mark.parametrize("cfg", SEGMENT_CFGS)
 def test_f16x2_config(prob, fp64, cfg, sk, structure):
@@ -141,15 +145,17 @@ def test_grouped_batch_config(prec, cfg, sk):
         assert np.abs(a - b).max() <= 1e-5
 
 
+@pytest.mark.parametrize("structure", ["blocks", "spinfree"])
 @pytest.mark.parametrize("prec", ["f16x2", "i8x4"])
-def test_exchange_blocks_grouped_equal_two_launches(prob, fp64, prec):
-    """X1 and X2 as one grouped launch (PHF_XGROUP=1) against two launches
-    (PHF_XGROUP=0): the same tiles and arithmetic, so results agree to 1e-9."""
+def test_exchange_blocks_grouped_equal_two_launches(prob, fp64, prec, structure):
+    """X1 and X2 (or the spin-free form's two products) as one grouped launch
+    (PHF_XGROUP=1) against two launches (PHF_XGROUP=0): the same tiles and
+    arithmetic, so results agree to 1e-9."""
     out = {}
     for g in ("0", "1"):
         os.environ["PHF_XGROUP"] = g
         try:
-            out[g] = _eval(prob, prec, True)
+            out[g] = _eval(prob, prec, structure)
         finally:
             os.environ.pop("PHF_XGROUP", None)
     (e0, g0), (e1, g1) = out["0"], out["1"]
@@ -159,21 +165,26 @@ def test_exchange_blocks_grouped_equal_two_launches(prob, fp64, prec):
     assert abs(e1 - e64) <= 1e-6
 
 
+@pytest.mark.parametrize("spinfree", [False, True])
 @pytest.mark.parametrize("prec", ["f16x2", "i8x4"])
-def test_exchange_blocks_grouped_256_tile(prec):
-    """X1 and X2 as one grouped launch of the 256 x 256 tile (PHF_XGROUP=2:
-    the round-to-nearest 16-bit tile, the exact int8 tile) on an n = 32
-    instance, whose exchange blocks have 256-row multiples: within the form's
-    budget of FP64 (|dE| <= 1e-6, |dG| <= 1e-5)."""
+def test_exchange_blocks_grouped_256_tile(prec, spinfree):
+    """X1 and X2 (or the spin-free form's two products) as one grouped launch
+    of the 256 x 256 tile (PHF_XGROUP=2: the round-to-nearest 16-bit tile,
+    the exact int8 tile) on an n = 32 instance, whose exchange blocks have
+    256-row multiples: within the form's budget of FP64 (|dE| <= 1e-6,
+    |dG| <= 1e-5)."""
     from phf_pcc_amd.model import ProjectedHF
     p = synthetic_problem(32, 32, seed=78, zscale=0.05)
     ga = grid_args(32, (50, 16), 0)
     m64 = ProjectedHF(p.nocc, p.ref, ga, factors=(p.h_spatial, p.b), structure=True, precision="fp64")
+    m64.ctx.set_spinfree(False)
     e64, g64 = m64.energy_and_gradient(p.z)
     m64.ctx.close()
     os.environ["PHF_XGROUP"] = "2"
     try:
         m = ProjectedHF(p.nocc, p.ref, ga, factors=(p.h_spatial, p.b), structure=True, precision=prec)
+        m.ctx.set_spinfree(spinfree)
+        assert m.ctx.spinfree() == spinfree
         e, g = m.energy_and_gradient(p.z)
         rep = m.ctx.tuning_report()
         m.ctx.close()
