@@ -352,7 +352,7 @@ def run_large(args):
     bprec = res[best]["prec"]
     kernels, unpack = gemm_launches(mode, res[best]["stages"], G, n, bprec, model.ctx.tuning_report())
     dom = max(kernels, key=lambda k: k["ms_per_launch"])
-    traffic = load_traffic(dom["kernel"], f"{n}-{dom['launch']}")
+    traffic = load_traffic(dom["kernel"], f"{n}-{dom['id']}")
     nprod = NPROD[bprec]
     out = {"metric": "projected E+grad evals/s (grid pts/s) vs plain FP64 GPU at 1e-6 Eh; % MFMA peak",
            "value": res[best]["value"], "unit": "grid pts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -741,8 +741,9 @@ def gemm_launches(mode, st, G, n, prec, report):
     nprod, peak = NPROD[prec], PEAK[prec]
     code = {"fp32": 1, "bf16x3": 2, "f16x2": 3, "i8x4": 4}.get(prec)
 
-    def row(name, kernel, ms, alg):
-        return {"launch": name, "kernel": kernel, "ms_per_launch": ms, "algorithmic_flops_per_launch": alg,
+    def row(ident, name, kernel, ms, alg):
+        return {"id": ident, "launch": name, "kernel": kernel, "ms_per_launch": ms,
+                "algorithmic_flops_per_launch": alg,
                 "achieved_tflops": alg / (ms * 1e-3) / 1e12, "frac": alg / (ms * 1e-3) / peak,
                 "frac_executed": nprod * alg / (ms * 1e-3) / peak}
 
@@ -765,10 +766,11 @@ def gemm_launches(mode, st, G, n, prec, report):
                 chosen = rec["spin_free_units"]["chosen"]
         if ms(1) is None:   # one grouped launch of both products
             cfg = (4 if prec == "i8x4" else 5) if chosen == "grouped_256x256" else c1
-            kernels = [row("2J-K + K (one grouped launch)", _cfg_kernel(prec, cfg, True), ms(0), a0 + a1)]
+            kernels = [row("spinfree", "2J-K + K (one grouped launch)", _cfg_kernel(prec, cfg, True), ms(0),
+                           a0 + a1)]
         else:
-            kernels = [row("2J-K", _cfg_kernel(prec, tuned_cfg(2 * G, n2)), ms(0), a0),
-                       row("K", _cfg_kernel(prec, c1), ms(1), a1)]
+            kernels = [row("2J-K", "2J-K", _cfg_kernel(prec, tuned_cfg(2 * G, n2)), ms(0), a0),
+                       row("K", "K", _cfg_kernel(prec, c1), ms(1), a1)]
         ub = 8.0 * (2 * G) * (4 + 4) * n2
         unpack = {"launch": "unpack into Gamma + h", "kernel": "rjk_unpack_kernel", "ms_per_launch": ms(2),
                   "algorithmic_bytes": ub, "GBps": ub / (ms(2) * 1e-3) / 1e9 if ms(2) else None,
@@ -776,14 +778,15 @@ def gemm_launches(mode, st, G, n, prec, report):
         return kernels, unpack
     if mode == "spin-blocks":
         kname = gemm_kernel_name(prec, report)
-        kernels = [row("D", kname, ms(0), 2.0 * (2 * G) * (2 * n2) ** 2)]
+        kernels = [row("D", "D", kname, ms(0), 2.0 * (2 * G) * (2 * n2) ** 2)]
         ax = 2.0 * (2 * G) * n2 * n2
         if ms(2) is None:
-            kernels.append(row("X1 + X2 (one grouped launch)", x_group_kernel_name(prec, report), ms(1), 2 * ax))
+            kernels.append(row("X1+X2", "X1 + X2 (one grouped launch)", x_group_kernel_name(prec, report), ms(1),
+                               2 * ax))
         else:
-            kernels += [row("X1", kname, ms(1), ax), row("X2", kname, ms(2), ax)]
+            kernels += [row("X1", "X1", kname, ms(1), ax), row("X2", "X2", kname, ms(2), ax)]
         return kernels, None
-    return [row("dense", gemm_kernel_name(prec, report), ms(0), 2.0 * (2 * G) * (4 * n2) ** 2)], None
+    return [row("dense", "dense", gemm_kernel_name(prec, report), ms(0), 2.0 * (2 * G) * (4 * n2) ** 2)], None
 
 
 def load_traffic(kernel, n):
@@ -1010,7 +1013,7 @@ def main():
         log(f"[bench] cpu BLAS ERI GEMM variant {cpu['blas_gemm_variant']['value']:.1f} grid pts/s "
             f"({cpu['blas_gemm_variant']['sample']})")
 
-    traffic = load_traffic(roof["kernel"], f"{args.n}-{dom['launch']}" if mode != "dense" else args.n)
+    traffic = load_traffic(roof["kernel"], f"{args.n}-{dom['id']}" if mode != "dense" else args.n)
     if traffic is not None:
         roof["traffic"] = traffic["bytes_per_launch"]
         roof["traffic_source"] = traffic["source"]
