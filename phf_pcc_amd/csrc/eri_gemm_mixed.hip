@@ -104,6 +104,41 @@ struct H16Cfg {
   static_assert(LDS * MINB <= 160 * 1024, "LDS budget");
 };
 
+// The spin-free epilogue of one tile (see H16Item) for the fragments of one
+// lane: rows row0 + 16 i + r, A rows ar0 + 16 j, values val(i, j, r, row, ar);
+// C[row][colmap[ar]] = s1 v + bias.  Per group of eight elements the bias
+// values are loaded first and the stores follow, so the group's loads are in
+// flight together (a store may alias a later load as far as the compiler can
+// tell, which would serialize them).
+template <int FM, int FN, class Val>
+__device__ __forceinline__ void epi_tile(double* __restrict__ C, long ldc, const double* __restrict__ bias,
+                                         const int* __restrict__ colmap, const Epi& e, long row0, long ar0,
+                                         Val val) {
+  constexpr int JC = FN >= 2 ? 2 : 1;   // fragments per group
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j0 = 0; j0 < FN; j0 += JC) {
+      int c1[JC][4];
+      double b1[JC][4];
+#pragma unroll
+      for (int jj = 0; jj < JC; ++jj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const long row = row0 + i * 16 + r, ar = ar0 + (j0 + jj) * 16;
+          c1[jj][r] = colmap[ar];
+          b1[jj][r] = (bias && c1[jj][r] >= 0) ? bias[(row & 1) * ldc + c1[jj][r]] : 0.0;
+        }
+#pragma unroll
+      for (int jj = 0; jj < JC; ++jj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const long row = row0 + i * 16 + r, ar = ar0 + (j0 + jj) * 16;
+          if (c1[jj][r] >= 0) C[row * ldc + c1[jj][r]] = b1[jj][r] + e.s1 * val(i, j0 + jj, r, row, ar);
+        }
+    }
+}
+
 // Operand strides (elements): P rows ldP apart with slices sP apart, A rows
 // ldA apart with slices sA apart; the contraction runs over the first K
 // elements of each slice.  colmap (optional) sends A row r to column
@@ -114,7 +149,8 @@ __device__ __forceinline__ void h16_tile(const T* __restrict__ P, long ldP, long
                                          const double* __restrict__ invA, double* __restrict__ C,
                                          const double* __restrict__ bias, const int* __restrict__ colmap,
                                          long ldc, int K, unsigned patP, unsigned patA, unsigned patW, int mt,
-                                         int nt, int sbeg, int send, bool atomic, int fold, unsigned char* smem) {
+                                         int nt, int sbeg, int send, bool atomic, int fold, unsigned char* smem,
+                                         const Epi& epi = Epi()) {
   constexpr int BK = 128 / (int)sizeof(T);    // elements per slab: 128 bytes per slab row
   constexpr int STAGES = Cfg::STAGES, NW = Cfg::NW, ROWS = Cfg::ROWS, STAGE_BYTES = Cfg::STAGE_BYTES;
   constexpr int FM = WM / 16, FN = WN / 16, WGN = BN / WN;
@@ -210,6 +246,15 @@ __device__ __forceinline__ void h16_tile(const T* __restrict__ P, long ldP, long
     }
   }
 
+  if (epi.on) {   // the spin-free epilogue (grouped launches)
+    epi_tile<FM, FN>(C, ldc, bias, colmap, epi, (long)mt * BM + wm * WM + 4 * (lane >> 4),
+                     (long)nt * BN + wn * WN + (lane & 15), [&](int i, int j, int r, long row, long ar) {
+      double v = (double)facc[i][j][r];
+      if (invP) v *= invP[row] * invA[ar];   // row scales of the split forms (none for FP32)
+      return v;
+    });
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -271,7 +316,7 @@ __device__ __forceinline__ void i8_tile(const signed char* __restrict__ P, long 
                                         double* __restrict__ C, const double* __restrict__ bias,
                                         const int* __restrict__ colmap, long ldc, int K, unsigned patP,
                                         unsigned patA, unsigned patW, int mt, int nt, int sbeg, int send,
-                                        bool atomic, bool add, unsigned char* smem) {
+                                        bool atomic, bool add, unsigned char* smem, const Epi& epi = Epi()) {
   constexpr int BM = 256, BN = 256, WM = 128, WN = 64, BK = 128;
   constexpr int FM = WM / 16, FN = WN / 16;
   constexpr int STAGE_BYTES = (BM + BN) * 128;
@@ -336,6 +381,15 @@ __device__ __forceinline__ void i8_tile(const signed char* __restrict__ P, long 
   }
 
   const double w = ldexp(1.0, -7 * (int)((patW >> (3 * (sbeg / nsl))) & 7));
+  if (epi.on && !add) {   // the spin-free epilogue (grouped launches), first class range
+    epi_tile<FM, FN>(C, ldc, bias, colmap, epi, (long)mt * BM + wm * WM + 4 * (lane >> 4),
+                     (long)nt * BN + wn * WN + (lane & 15), [&](int i, int j, int r, long row, long ar) {
+      double v = (double)acc[i][j][r] * w * invP[row] * invA[ar];
+      return v;
+    });
+    return;
+  }
+  const double sc = epi.on ? epi.s1 : 1.0;   // later class ranges of a spin-free item add s1 v
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -346,7 +400,7 @@ __device__ __forceinline__ void i8_tile(const signed char* __restrict__ P, long 
         const long ar = (long)nt * BN + wn * WN + j * 16 + (lane & 15);
         const long col = colmap ? colmap[ar] : ar;
         if (col < 0) continue;
-        double v = (double)acc[i][j][r] * w * invP[row] * invA[ar];
+        double v = (double)acc[i][j][r] * w * invP[row] * invA[ar] * sc;
         if (bias && !add) v += bias[(row & 1) * ldc + col];
         if (atomic)
           atomicAdd(&C[row * ldc + col], v);
@@ -365,7 +419,8 @@ __device__ __forceinline__ void i8_tile_parts(const signed char* P, long ldP, lo
                                               long sA, const double* invP, const double* invA, double* C,
                                               const double* bias, const int* colmap, long ldc, int K,
                                               unsigned patP, unsigned patA, unsigned patW, int mt, int nt,
-                                              int sbeg, int send, bool atomic, unsigned char* smem) {
+                                              int sbeg, int send, bool atomic, unsigned char* smem,
+                                              const Epi& epi = Epi()) {
   const int nsl = K / 128;
   bool add = false;
   for (int b = sbeg; b < send;) {
@@ -376,7 +431,7 @@ __device__ __forceinline__ void i8_tile_parts(const signed char* P, long ldP, lo
     const int e = min((sg + 1) * nsl, send);
     if (add) __syncthreads();   // every wave is done with the LDS ring of the previous range
     i8_tile<I8_TILE_STAGES>(P, ldP, sP, A, ldA, sA, invP, invA, C, bias, colmap, ldc, K, patP, patA, patW, mt, nt,
-                            b, e, atomic, add, smem);
+                            b, e, atomic, add, smem, epi);
     add = true;
     b = e;
   }
@@ -405,10 +460,11 @@ void eri_gemm_i8_grouped_kernel(const H16Item* __restrict__ items, int nitems, u
   extern __shared__ __attribute__((aligned(1024))) unsigned char smem[];
   for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
     const H16Item it = items[w];
+    if (it.K == 0) continue;   // empty slot
     __syncthreads();   // every wave is done with the previous tile's LDS ring
     i8_tile_parts((const signed char*)it.P, it.ldP, it.sP, (const signed char*)it.A, it.ldA, it.sA, it.invP,
                   it.invA, it.C, it.bias, it.colmap, it.ldc, it.K, patP, patA, patW, it.mt, it.nt, it.sbeg, it.send,
-                  ATOMIC, smem);
+                  ATOMIC, smem, item_epi(it));
   }
 }
 
@@ -429,7 +485,7 @@ __device__ __forceinline__ void rn16_tile(const T* __restrict__ P, long ldP, lon
                                           const double* __restrict__ invA, double* __restrict__ C,
                                           const double* __restrict__ bias, const int* __restrict__ colmap, long ldc,
                                           int K, unsigned patP, unsigned patA, int mt, int nt, int sbeg, int send,
-                                          bool atomic, unsigned char* smem) {
+                                          bool atomic, unsigned char* smem, const Epi& epi = Epi()) {
   constexpr int BM = 256, BN = 256, WM = 128, WN = 64, BK = 64;
   constexpr int FM = WM / 16, FN = WN / 16;
   constexpr int STAGE_BYTES = (BM + BN) * 128;
@@ -498,6 +554,14 @@ __device__ __forceinline__ void rn16_tile(const T* __restrict__ P, long ldP, lon
     }
   }
 
+  if (epi.on) {   // the spin-free epilogue (grouped launches)
+    epi_tile<FM, FN>(C, ldc, bias, colmap, epi, (long)mt * BM + wm * WM + 4 * (lane >> 4),
+                     (long)nt * BN + wn * WN + (lane & 15), [&](int i, int j, int r, long row, long ar) {
+      double v = (double)acc[i][j][r] * invP[row] * invA[ar];
+      return v;
+    });
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -539,10 +603,11 @@ void eri_gemm_rn16_grouped_kernel(const H16Item* __restrict__ items, int nitems,
   extern __shared__ __attribute__((aligned(1024))) unsigned char smem[];
   for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
     const H16Item it = items[w];
+    if (it.K == 0) continue;   // empty slot
     __syncthreads();   // every wave is done with the previous tile's LDS ring
     rn16_tile<T, I8_TILE_STAGES>((const T*)it.P, it.ldP, it.sP, (const T*)it.A, it.ldA, it.sA, it.invP, it.invA,
                                  it.C, it.bias, it.colmap, it.ldc, it.K, patP, patA, it.mt, it.nt, it.sbeg, it.send,
-                                 ATOMIC, smem);
+                                 ATOMIC, smem, item_epi(it));
   }
 }
 
@@ -556,10 +621,11 @@ void eri_gemm_h16_grouped_kernel(const H16Item* __restrict__ items, int nitems, 
   extern __shared__ __attribute__((aligned(1024))) unsigned char smem[];
   for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
     const H16Item it = items[w];
+    if (it.K == 0) continue;   // empty slot
     __syncthreads();   // every wave is done with the previous tile's LDS ring
     h16_tile<T, Cfg, BM, BN, WM, WN>((const T*)it.P, it.ldP, it.sP, (const T*)it.A, it.ldA, it.sA, it.invP, it.invA,
                                      it.C, it.bias, it.colmap, it.ldc, it.K, patP, patA, patW, it.mt, it.nt,
-                                     it.sbeg, it.send, ATOMIC, fold, smem);
+                                     it.sbeg, it.send, ATOMIC, fold, smem, item_epi(it));
   }
 }
 
@@ -920,12 +986,12 @@ __global__ __launch_bounds__(NT) void dens_split_kernel(int nso, int nk, const d
 // 2m and 2m + 1 and label Lc's on the columns of fragments 0 and 1, so one
 // lane holds P[(alpha b, alpha d)], P[(beta b, beta d)], P[(beta b, alpha d)] and
 // P[(alpha b, beta d)] of a label pair (b, d) and writes, at column b n + d,
-// set 0 = P_aa + P_bb (row 2g + plane), set 1 = P_aa - P_bb, set 2 = P_x1,
-// set 3 = P_x2 (rows Mp + (set - 1) 2 npts + 2g + plane), each (set, plane)
-// row with its own scale.  The values and slices equal those of the density
-// pack, rjk_pack_kernel and split_rows_kernel, bit for bit.  Columns
-// [n^2, kj) are zeroed; workgroups past the last point clear the padding rows
-// [2 npts, Mp) and [Mp + 6 npts, Mp + Mk).
+// set 0 = P_aa + P_bb, set 1 = P_aa - P_bb, set 2 = P_x1, set 3 = P_x2 at
+// rows set Mp + 2g + plane (Mk = 3 Mp), each (set, plane) row with its own
+// scale.  The values and slices equal those of the density pack,
+// rjk_pack_kernel and split_rows_kernel, bit for bit.  Columns [n^2, kj) are
+// zeroed; workgroups past the last point clear each set's padding rows
+// [2 npts, Mp).
 template <typename T, int NS, int NT, int FM, int FN>
 __global__ __launch_bounds__(NT) void dens_split_rjk_kernel(int n, int nso, int nk, const double* __restrict__ C0,
                                                               const double* __restrict__ X, long sX, int npts,
@@ -943,8 +1009,8 @@ __global__ __launch_bounds__(NT) void dens_split_rjk_kernel(int n, int nso, int 
     const long pad0 = Mp - 2L * npts;
     for (int pr = 0; pr < 2; ++pr) {
       const long p = 2L * (g - npts) + pr;
-      const long row = p < pad0 ? 2L * npts + p : Mp + 6L * npts + (p - pad0);
-      if (row >= Mp + Mk) break;
+      const long set = p / pad0, row = set * Mp + 2L * npts + p % pad0;   // each set's padding rows
+      if (set >= 4) break;
       T* d = dst + row * rs;
       for (long c = tid; c < rs; c += NT) d[c] = zero;
       if (tid == 0) inv[row] = 1.0;
@@ -999,7 +1065,7 @@ __global__ __launch_bounds__(NT) void dens_split_rjk_kernel(int n, int nso, int 
     if (m > 0.0) frexp(m, &e);
     sc[q] = (m > 0.0) ? ldexp(1.0, top - e) : 1.0;
     const int set = q >> 1, pl = q & 1;
-    const long row = set == 0 ? 2L * g + pl : Mp + (long)(set - 1) * 2L * npts + 2L * g + pl;
+    const long row = (long)set * Mp + 2L * g + pl;
     drow[q] = dst + row * rs;
     if (tid == 0) inv[row] = 1.0 / sc[q];
   }
@@ -1125,7 +1191,7 @@ void dens_split_rjk_maps(const std::vector<int>& mo, int n, int* smap) {
 
 void dens_split_rjk(hipStream_t s, int prec, int n, int nk, const double* C0, const double* X, long sX, int npts,
                     const int* smap, int kj, long Mp, long Mk, void* dst, double* inv) {
-  const long pad = (Mp - 2L * npts) + (Mk - 6L * npts);
+  const long pad = 4 * (Mp - 2L * npts);   // Mk = 3 Mp: four sets of Mp rows
   const unsigned grid = (unsigned)(npts + (pad + 1) / 2);
   const int nso = 2 * n;
   if (prec == 3)

@@ -491,9 +491,9 @@ __global__ void rjk_check_kernel(const double* __restrict__ A, long lda, const i
 }
 
 // Spin-free density operand from the natural density rows (src, lds columns,
-// rows r < rows of data): dst[r] = P_aa + P_bb, dst[Mp + r] = P_aa - P_bb,
-// dst[Mp + rows + r] = P_x1, dst[Mp + 2 rows + r] = P_x2, kj columns (zero
-// beyond n^2).  Padding rows are cleared by the caller.
+// rows r < rows of data): set s at rows s Mp: dst[r] = P_aa + P_bb,
+// dst[Mp + r] = P_aa - P_bb, dst[2 Mp + r] = P_x1, dst[3 Mp + r] = P_x2, kj
+// columns (zero beyond n^2).  Padding rows are cleared by the caller.
 __global__ void rjk_pack_kernel(const double* __restrict__ src, long lds, long rows, const int* __restrict__ mo,
                                 int n, int nso, long kj, long Mp, double* __restrict__ dst) {
   const long n2 = (long)n * n, total = rows * kj;
@@ -511,31 +511,34 @@ __global__ void rjk_pack_kernel(const double* __restrict__ src, long lds, long r
     }
     dst[r * kj + c] = s;
     dst[(Mp + r) * kj + c] = df;
-    dst[(Mp + rows + r) * kj + c] = x1;
-    dst[(Mp + 2 * rows + r) * kj + c] = x2;
+    dst[(2 * Mp + r) * kj + c] = x1;
+    dst[(3 * Mp + r) * kj + c] = x2;
   }
 }
 
 // Gamma + h in the natural column order from the spin-free products (row
 // order of rjk_pack_kernel): Gamma_aa = (Cs - Cd)/2, Gamma_bb = (Cs + Cd)/2,
-// Gamma_ab = -C_x1, Gamma_ba = -C_x2; h from the bias planes (row parity).
+// and with all, Gamma_ab = -C_x1, Gamma_ba = -C_x2 (else the GEMM wrote
+// those itself); h from the bias planes (row parity).
 __global__ void rjk_unpack_kernel(const double* __restrict__ C, long kj, long rows, long Mp,
                                   const int* __restrict__ mo, int n, int nso, const double* __restrict__ hb,
-                                  long nrpad, double* __restrict__ gam) {
+                                  long nrpad, double* __restrict__ gam, int all) {
   const long n2 = (long)n * n, total = rows * n2;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
     const long r = i / n2, c = i % n2;
     const int a = (int)(c / n), cc = (int)(c % n);
     const double cs = C[r * kj + c], cd = C[(Mp + r) * kj + c];
-    const double c1 = C[(Mp + rows + r) * kj + c], c2 = C[(Mp + 2 * rows + r) * kj + c];
     const double* h = hb + (r & 1) * nrpad;
     double* g = gam + r * nrpad;
     const long aa = (long)mo[a] * nso + mo[cc], bb = (long)mo[n + a] * nso + mo[n + cc];
-    const long ab = (long)mo[a] * nso + mo[n + cc], ba = (long)mo[n + a] * nso + mo[cc];
     g[aa] = h[aa] + 0.5 * (cs - cd);
     g[bb] = h[bb] + 0.5 * (cs + cd);
-    g[ab] = h[ab] - c1;
-    g[ba] = h[ba] - c2;
+    if (all) {
+      const double c1 = C[(2 * Mp + r) * kj + c], c2 = C[(3 * Mp + r) * kj + c];
+      const long ab = (long)mo[a] * nso + mo[n + cc], ba = (long)mo[n + a] * nso + mo[cc];
+      g[ab] = h[ab] - c1;
+      g[ba] = h[ba] - c2;
+    }
   }
 }
 
@@ -841,9 +844,11 @@ static void drop_rjk(phf_ctx* ctx) {
   if (ctx->dAj) (void)hipFree(ctx->dAj);
   if (ctx->d_rjmo) (void)hipFree(ctx->d_rjmo);
   if (ctx->d_rjsmap) (void)hipFree(ctx->d_rjsmap);
+  if (ctx->d_rjcol) (void)hipFree(ctx->d_rjcol);
   ctx->dAj = nullptr;
   ctx->d_rjmo = nullptr;
   ctx->d_rjsmap = nullptr;
+  ctx->d_rjcol = nullptr;
   ctx->rjk = false;
   ctx->rj_n = ctx->rj_kj = 0;
   ctx->rj_mo.clear();
@@ -882,6 +887,22 @@ static int build_rjk(phf_ctx* ctx, const double* ref) {
   HIPCK(hipMalloc(&ctx->dAj, sizeof(double) * 2 * kj * kj));
   rjk_gather_kernel<<<8192, 256, 0, s>>>(ctx->dA, ctx->kt, ctx->d_rjmo, n, nso, kj, ctx->dAj);
   HIPCK(hipGetLastError());
+  {
+    // output columns of the direct epilogue: label pair a n + c -> natural
+    // (i, k) = i nso + k of Gamma_aa, Gamma_bb, Gamma_ab, Gamma_ba
+    std::vector<int> col(4 * kj, -1);
+    for (int a = 0; a < n; ++a)
+      for (int c = 0; c < n; ++c) {
+        const long r = (long)a * n + c;
+        col[r] = mo[a] * nso + mo[c];
+        col[kj + r] = mo[n + a] * nso + mo[n + c];
+        col[2 * kj + r] = mo[a] * nso + mo[n + c];
+        col[3 * kj + r] = mo[n + a] * nso + mo[c];
+      }
+    HIPCK(hipMalloc(&ctx->d_rjcol, sizeof(int) * col.size()));
+    HIPCK(hipMemcpyAsync(ctx->d_rjcol, col.data(), sizeof(int) * col.size(), hipMemcpyHostToDevice, s));
+    HIPCK(hipStreamSynchronize(s));
+  }
   if (nso <= dens_split_max_nso()) {
     std::vector<int> smap(256);
     dens_split_rjk_maps(mo, n, smap.data());
@@ -1692,7 +1713,7 @@ int eval_plan(phf_ctx* ctx, int nocc, int precision, int cmplxconj, EvalBufs& b)
   // rows are padded for the tallest tile any candidate uses
   b.bm = (precision == PHF_PREC_FP64) ? eri_gemm_f64_bm((int)(2 * Gv)) : 256;
   b.Mp = round_up(2 * Gv, b.bm);
-  b.Mk = b.rjk ? round_up(6 * Gv, b.bm) : 0;
+  b.Mk = b.rjk ? 3 * b.Mp : 0;   // the spin-free form's three K row sets, each padded like the first
   const long rows_op = b.Mp + b.Mk;   // rows of the GEMM's density operand
   const bool fused = precision >= PHF_PREC_BF16X3 && V == 1 && !ctx->cplx && nso <= dens_split_max_nso() &&
                      !no_fused_split();
@@ -1915,8 +1936,7 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
   if (b.rjk) {
     // spin-free operand rows in FP64, then the form's rounding or split
     const long kj = b.kP, rows = 2 * V * G, rows_op = b.Mp + b.Mk;
-    fill_zero(s, b.Pperm + rows * kj, (b.Mp - rows) * kj);
-    fill_zero(s, b.Pperm + (b.Mp + 3 * rows) * kj, (b.Mk - 3 * rows) * kj);
+    for (int set = 0; set < 4; ++set) fill_zero(s, b.Pperm + (set * b.Mp + rows) * kj, (b.Mp - rows) * kj);
     rjk_pack_kernel<<<8192, 256, 0, s>>>(b.Pt, kt, rows, ctx->d_rjmo, ctx->rj_n, nso, kj, b.Mp, b.Pperm);
     if (b.prec == PHF_PREC_FP32)
       convert_f32(s, b.Pperm, (float*)b.Pm, rows_op * kj);
@@ -1951,15 +1971,33 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
 // neighbouring items share A panels.  Full rounds of `grid` items are then
 // placed so that XCD x (workgroups b with b % 8 == x) takes a contiguous
 // eighth of each round.
+//
+// With split-K 1 the spin-free form's K items over the P_x1 and P_x2 rows
+// write -C + h straight into the (alpha, beta) and (beta, alpha) elements of
+// Gamma (their own column maps, scale -1, the bias); the (2J - K) and
+// K (P_aa - P_bb) items store their products, and the unpack then forms only
+// Gamma_aa and Gamma_bb from those two sets.
+static bool rjk_direct(const std::vector<std::pair<const phf_ctx*, const EvalBufs*>>& units, int sk) {
+  if (sk != 1) return false;
+  for (const auto& u : units)
+    if (u.second->rjk) return true;
+  return false;
+}
+
+bool rjk_direct_items(const std::vector<std::pair<const phf_ctx*, const EvalBufs*>>& units, int sk) {
+  return rjk_direct(units, sk);
+}
+
 void build_h16_items(const std::vector<std::pair<const phf_ctx*, const EvalBufs*>>& units, int prec, int cfg,
                      int sk, int grid, std::vector<H16Item>& out, unsigned blk_mask) {
   const int bm = h16_cfg_bm(cfg), bn = h16_cfg_bn(cfg), ns = mixed_slices(prec);
   const long eb = mixed_elem_bytes(prec);
+  const bool direct = rjk_direct(units, sk);
   struct Unit { int u, blk; long cost; };
   std::vector<Unit> work;
   for (size_t i = 0; i < units.size(); ++i) {
     const phf_ctx* c = units[i].first;
-    if (units[i].second->rjk)   // spin-free form: 2J - K over rows [0, Mp), K over [Mp, Mp + Mk)
+    if (units[i].second->rjk)   // spin-free form: 2J - K over rows [0, Mp), K over [Mp, 4 Mp)
       for (int q = 0; q < 2; ++q) {
         if (blk_mask >> q & 1u) work.push_back({(int)i, 8 + q, (long)c->rj_kj});
       }
@@ -1975,11 +2013,52 @@ void build_h16_items(const std::vector<std::pair<const phf_ctx*, const EvalBufs*
   for (const Unit& u : work) {
     const phf_ctx* c = units[u.u].first;
     const EvalBufs& b = *units[u.u].second;
+    if (u.blk >= 8 && direct) {
+      if (u.blk == 9) continue;   // both units' items come with unit 0
+      const long kj = c->rj_kj, Mp = b.Mp;
+      const bool f32 = prec == PHF_PREC_FP32;
+      const int tm = (int)(Mp / bm), tn = (int)(kj / bn);
+      const int nsl = h16_slabs(prec, cfg, (int)kj);
+      // set: 0 = P_aa + P_bb against 2J - K, 1..3 = P_aa - P_bb, P_x1, P_x2 against K
+      for (int set = 0; set < 4; ++set)
+        for (int nt = 0; nt < tn; ++nt)
+          for (int mt = 0; mt < tm; ++mt) {
+            H16Item it;
+            std::memset(&it, 0, sizeof it);
+            const int q = set ? 1 : 0;
+            it.P = (const char*)b.Pm + eb * ns * kj * (set * Mp);
+            it.A = (const char*)c->dAjm[prec] + eb * ns * kj * kj * q;
+            it.invP = f32 ? nullptr : b.Pinv + set * Mp;
+            it.invA = f32 ? nullptr : c->dAjinv[prec] + q * kj;
+            it.ldP = ns * kj;
+            it.sP = kj;
+            it.ldA = ns * kj;
+            it.sA = kj;
+            it.K = (int)kj;
+            it.mt = mt;
+            it.nt = nt;
+            it.sbeg = 0;
+            it.send = nsl;
+            if (set < 2) {   // products for the unpack of Gamma_aa, Gamma_bb
+              it.C = b.Cjk + set * Mp * kj;
+              it.ldc = kj;
+            } else {         // -C + h into Gamma_ab (set 2) or Gamma_ba (set 3)
+              it.C = b.Gam;
+              it.ldc = c->nrpad;
+              it.bias = c->dHb;
+              it.colmap = c->d_rjcol + set * kj;
+              it.epi = 1;
+              it.s1 = -1.0;
+            }
+            logical.push_back(it);
+          }
+      continue;
+    }
     if (u.blk >= 8) {
       const int q = u.blk - 8;
       const long kj = c->rj_kj, r0 = q ? b.Mp : 0;
       const bool f32 = prec == PHF_PREC_FP32;
-      const int tm = (int)(round_up((q ? 6 : 2) * b.V * b.G, bm) / bm), tn = (int)(kj / bn);
+      const int tm = (int)((q ? 3 * b.Mp : round_up(2 * b.V * b.G, bm)) / bm), tn = (int)(kj / bn);
       const int part = h16_slabs(prec, cfg, (int)kj) / sk;
       for (int nt = 0; nt < tn; ++nt)
         for (int mt = 0; mt < tm; ++mt)
@@ -2064,9 +2143,9 @@ void build_h16_items(const std::vector<std::pair<const phf_ctx*, const EvalBufs*
     for (size_t slot = x; slot < m; slot += 8) out[base + slot] = logical[l++];
 }
 
-void rjk_unpack(phf_ctx* ctx, const EvalBufs& b, hipStream_t s) {
+void rjk_unpack(phf_ctx* ctx, const EvalBufs& b, hipStream_t s, bool all) {
   rjk_unpack_kernel<<<8192, 256, 0, s>>>(b.Cjk, b.kP, 2 * b.V * b.G, b.Mp, ctx->d_rjmo, ctx->rj_n, ctx->nso,
-                                         ctx->dHb, ctx->nrpad, b.Gam);
+                                         ctx->dHb, ctx->nrpad, b.Gam, all ? 1 : 0);
 }
 
 // The ERI GEMM of the spin-free form: unit 0 = (2J - K) against the P_aa + P_bb
@@ -2074,8 +2153,11 @@ void rjk_unpack(phf_ctx* ctx, const EvalBufs& b, hipStream_t s) {
 // the unpack into Gamma + h.  FP64: two launches.  Reduced forms: each unit's
 // tile is tuned like a block's; the first call of a (precision, rows) times
 // the two launches against one grouped launch of both units with unit 1's
-// tile and with the 256 x 256 tile, and keeps the fastest.  Profiling events:
-// bev[0] around unit 0 (or the grouped launch), bev[1] unit 1, bev[2] the unpack.
+// tile and with the 256 x 256 tile, and keeps the fastest; a grouped launch
+// with split-K 1 writes Gamma_ab and Gamma_ba + h directly (build_h16_items),
+// so its unpack forms Gamma_aa and Gamma_bb only.  Profiling events: bev[0]
+// around unit 0 (or the grouped launch), bev[1] unit 1 (0 when grouped),
+// bev[2] the unpack.
 static int rjk_gemm(phf_ctx* ctx, const EvalBufs& b) {
   hipStream_t s = ctx->stream;
   const int prec = b.prec;
@@ -2087,9 +2169,9 @@ static int rjk_gemm(phf_ctx* ctx, const EvalBufs& b) {
     (void)hipEventRecord(ctx->bev[q][k], s);
     if (k) ctx->nblk_timed = std::max(ctx->nblk_timed, q + 1);
   };
-  auto unpack = [&]() -> int {
+  auto unpack = [&](bool all) -> int {
     mark(2, 0);
-    rjk_unpack(ctx, b, s);
+    rjk_unpack(ctx, b, s, all);
     mark(2, 1);
     HIPCK(hipGetLastError());
     return PHF_OK;
@@ -2102,12 +2184,12 @@ static int rjk_gemm(phf_ctx* ctx, const EvalBufs& b) {
     eri_gemm_f64(s, b.Pperm + Mp * kj, kj, ctx->dAj + kj * kj, b.Cjk + Mp * kj, (int)b.Mk, (int)kj, (int)kj, kj,
                  b.bm, nullptr, nullptr);
     mark(1, 1);
-    return unpack();
+    return unpack(true);
   }
   const bool f32 = prec == PHF_PREC_FP32;
   const int ns = mixed_slices(prec);
   const long eb = mixed_elem_bytes(prec);
-  auto unit_rows = [&](int q) { return (q ? 6 : 2) * Gv; };
+  auto unit_rows = [&](int q) { return q ? 3 * Mp : 2 * Gv; };   // K: the three padded row sets
   auto run_unit = [&](int q, int cfg, int sk) {
     const long r0 = q ? Mp : 0;
     const int mp = (int)round_up(unit_rows(q), h16_cfg_bm(cfg));
@@ -2171,11 +2253,14 @@ static int rjk_gemm(phf_ctx* ctx, const EvalBufs& b) {
           int rc = upload(cs.first, cs.second);
           if (rc) return rc;
         }
-        auto run_x = [&]() {
-          if (g)
+        auto run_x = [&]() {   // each candidate with its unpack, if it needs one
+          if (g) {
             run_grouped(cs.first, cs.second);
-          else
+            rjk_unpack(ctx, b, s, cs.second > 1);
+          } else {
             for (int q = 0; q < 2; ++q) run_unit(q, choice[q].first, choice[q].second);
+            rjk_unpack(ctx, b, s, true);
+          }
         };
         run_x();
         for (int rep = 0; rep < 2; ++rep) {
@@ -2218,6 +2303,7 @@ static int rjk_gemm(phf_ctx* ctx, const EvalBufs& b) {
     mark(1, 0);   // unit 1 ran inside the grouped launch (reported as 0)
     mark(1, 1);
     ctx->rjk_grouped_last = true;
+    return unpack(cs.second > 1);   // split-K 1: Gamma_ab and Gamma_ba are written already
   } else {
     for (int q = 0; q < 2; ++q) {
       mark(q, 0);
@@ -2225,7 +2311,7 @@ static int rjk_gemm(phf_ctx* ctx, const EvalBufs& b) {
       mark(q, 1);
     }
   }
-  return unpack();
+  return unpack(true);
 }
 
 int eval_gemm(phf_ctx* ctx, const EvalBufs& b) {

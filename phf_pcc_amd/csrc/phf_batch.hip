@@ -284,7 +284,8 @@ int enqueue_eval(phf_batch* bt, int prec, bool timed) {
     phf_ctx* c = bt->inst[i];
     const EvalBufs& b = bt->bufs[i];
     on_side(r, c);
-    if (prec != PHF_PREC_FP64 && b.rjk) rjk_unpack(c, b, c->stream);   // Gamma + h from the grouped products
+    if (prec != PHF_PREC_FP64 && b.rjk)   // Gamma + h from the grouped products (split-K 1:
+      rjk_unpack(c, b, c->stream, bt->splitk > 1);   // Gamma_ab, Gamma_ba were written directly)
     eval_post(c, b, true);
     const int nel = (c->nso - b.N) * b.N;
     const int blocks = std::max(1, std::min(1024, (nel + 255) / 256));

@@ -82,11 +82,27 @@ struct H16Item {
   const int* colmap;
   long ldP, sP, ldA, sA;
   long ldc;
-  int K;
+  int K;                // 0: an empty slot of the work list (skipped)
   int mt, nt;
   int sbeg, send;
   int atomic;           // 1: add into C (zero-filled by the caller)
+  // epilogue of the spin-free form's exchange sets (epi = 1): C[colmap[r]] = s1 v (+ bias)
+  int epi;
+  double s1;
 };
+
+// The spin-free epilogue of a work item (see H16Item).
+struct Epi {
+  int on = 0;
+  double s1 = 1.0;
+};
+
+__host__ __device__ inline Epi item_epi(const H16Item& it) {
+  Epi e;
+  e.on = it.epi;
+  e.s1 = it.s1;
+  return e;
+}
 
 // XCD-aware remap of a linear workgroup id: workgroups are dispatched
 // round-robin over the 8 XCDs; this gives each XCD a contiguous range of

@@ -30,8 +30,8 @@ int dens_split_max_nso();
 void dens_split(hipStream_t s, int prec, int nso, int nk, const double* C0, const double* X, long sX, int npts,
                 const int* ocol, const int* padcols, int npad, int kout, long rows_total, void* dst, double* inv);
 // The same pass writing the spin-free operand (n <= 64 labels): smap = tile
-// row -> MO [128] then tile column -> MO [128]; rows [0, Mp) P_aa + P_bb,
-// [Mp, Mp + Mk) P_aa - P_bb, P_x1, P_x2 (2 npts rows each); kj columns.
+// row -> MO [128] then tile column -> MO [128]; four row sets of Mp rows
+// (Mk = 3 Mp): P_aa + P_bb, P_aa - P_bb, P_x1, P_x2 (2 npts data rows each); kj columns.
 void dens_split_rjk(hipStream_t s, int prec, int n, int nk, const double* C0, const double* X, long sX, int npts,
                     const int* smap, int kj, long Mp, long Mk, void* dst, double* inv);
 // Staging maps of dens_split_rjk for the pairing mo (label s: alpha MO mo[s],
@@ -119,6 +119,7 @@ struct phf_ctx_s {
   double* dAjinv[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   int* d_rjmo = nullptr;
   int* d_rjsmap = nullptr;
+  int* d_rjcol = nullptr;    // [4][rj_kj]: label pair a n + c -> natural column of (aa, bb, ab, ba) (-1 padding)
   std::vector<int> rj_mo;    // host copy of d_rjmo (the pairing it was built for)
   // workspace arena
   char* ws = nullptr;
@@ -268,10 +269,10 @@ struct EvalBufs {
   bool blk = false;           // the GEMM runs over the spin blocks
   long kP = 0;                // row length (per slice) of the GEMM's density operand
   double* Pperm = nullptr;    // FP64 density rows in permuted column order (blk, FP64 form)
-  // spin-free form: operand rows [0, Mp) P_aa + P_bb of every density (2 Gv
-  // data rows); rows [Mp, Mp + Mk) P_aa - P_bb, P_x1, P_x2 (2 Gv rows each,
-  // in that order), kP = rj_kj columns; Cjk the products in the same row
-  // order (FP64), unpacked into Gam by rjk_unpack
+  // spin-free form: four operand row sets of Mp rows each (Mk = 3 Mp), set s
+  // at rows s Mp: P_aa + P_bb, P_aa - P_bb, P_x1, P_x2 of every density (2 Gv
+  // data rows, then zero padding), kP = rj_kj columns; Cjk the products in the
+  // same row order (FP64), unpacked into Gam by rjk_unpack
   bool rjk = false;
   long Mk = 0;
   double* Cjk = nullptr;      // shares its memory with Pt (dead once the operand is packed)
@@ -293,9 +294,14 @@ int eval_gemm(phf_ctx* ctx, const EvalBufs& b);
 // blk_mask selects the spin blocks of blocked units (bit q: block q).
 void build_h16_items(const std::vector<std::pair<const phf_ctx*, const EvalBufs*>>& units, int prec, int cfg,
                      int sk, int grid, std::vector<H16Item>& out, unsigned blk_mask = 7u);
+// True when build_h16_items makes the spin-free units' exchange sets write
+// Gamma_ab, Gamma_ba + h directly (split-K 1 and at least one spin-free unit).
+bool rjk_direct_items(const std::vector<std::pair<const phf_ctx*, const EvalBufs*>>& units, int sk);
 void eval_post(phf_ctx* ctx, const EvalBufs& b, bool resident);
 // Spin-free form: Gamma + h (b.Gam, natural column order) from the products in
 // b.Cjk; eval_gemm runs it after its launches (the batch after its grouped one).
-void rjk_unpack(phf_ctx* ctx, const EvalBufs& b, hipStream_t s);
+// all = false: Gamma_aa and Gamma_bb only (a grouped launch with split-K 1
+// wrote Gamma_ab and Gamma_ba itself).
+void rjk_unpack(phf_ctx* ctx, const EvalBufs& b, hipStream_t s, bool all);
 // C0 = [I; Z] column-major complex (NSO x N) from Z ((NSO-N) x N, C order)
 void build_c0(const double* z, int nso, int N, int cmplxconj, double* c0);
