@@ -736,8 +736,7 @@ def gemm_launches(mode, st, G, n, prec, report):
     """Every launch of the ERI GEMM stage of one evaluation from the profiled
     stage times st[8:] (HIP events): name, kernel, ms, algorithmic FLOPs (one
     product), achieved TFLOP/s and fractions of the form's peak; plus, for the
-    spin-free form, the unpack into Gamma + h with its algorithmic bytes
-    (read the four product sets, write Gamma: 2G rows x (4 + 4) n^2 doubles)."""
+    spin-free form, the unpack into Gamma + h with its algorithmic bytes."""
     nprod, peak = NPROD[prec], PEAK[prec]
     code = {"fp32": 1, "bf16x3": 2, "f16x2": 3, "i8x4": 4}.get(prec)
 
@@ -760,10 +759,11 @@ def gemm_launches(mode, st, G, n, prec, report):
     if mode == "spin-free":
         a0, a1 = 2.0 * (2 * G) * n2 * n2, 2.0 * (6 * G) * n2 * n2
         c1 = tuned_cfg(6 * G, n2)
-        chosen = None
+        chosen, half = None, False
         for rec in _report(report):
             if rec.get("prec") == code and "spin_free_units" in rec:
                 chosen = rec["spin_free_units"]["chosen"]
+                half = rec["spin_free_units"].get("unpack") == "aa_bb"
         if ms(1) is None:   # one grouped launch of both products
             cfg = (4 if prec == "i8x4" else 5) if chosen == "grouped_256x256" else c1
             kernels = [row("spinfree", "2J-K + K (one grouped launch)", _cfg_kernel(prec, cfg, True), ms(0),
@@ -771,9 +771,12 @@ def gemm_launches(mode, st, G, n, prec, report):
         else:
             kernels = [row("2J-K", "2J-K", _cfg_kernel(prec, tuned_cfg(2 * G, n2)), ms(0), a0),
                        row("K", "K", _cfg_kernel(prec, c1), ms(1), a1)]
-        ub = 8.0 * (2 * G) * (4 + 4) * n2
-        unpack = {"launch": "unpack into Gamma + h", "kernel": "rjk_unpack_kernel", "ms_per_launch": ms(2),
-                  "algorithmic_bytes": ub, "GBps": ub / (ms(2) * 1e-3) / 1e9 if ms(2) else None,
+        # read the product sets, write their Gamma elements: all four (2G rows x (4 + 4) n^2 doubles),
+        # or only Gamma_aa, Gamma_bb when the grouped launch wrote the exchange elements itself
+        ub = 8.0 * (2 * G) * ((2 + 2) if half else (4 + 4)) * n2
+        unpack = {"launch": "unpack into Gamma + h" + (" (aa, bb)" if half else ""), "kernel": "rjk_unpack_kernel",
+                  "ms_per_launch": ms(2), "algorithmic_bytes": ub,
+                  "GBps": ub / (ms(2) * 1e-3) / 1e9 if ms(2) else None,
                   "frac_hbm": ub / (ms(2) * 1e-3) / 8e12 if ms(2) else None}
         return kernels, unpack
     if mode == "spin-blocks":

@@ -2285,9 +2285,10 @@ static int rjk_gemm(phf_ctx* ctx, const EvalBufs& b) {
     if (const char* f = getenv("PHF_XGROUP")) pick = std::max(0, std::min(atoi(f), ncand - 1));
     static const char* names[3] = {"two_launches", "grouped", "grouped_256x256"};
     char line[256];
+    const bool half = pick > 0 && sel(pick).second == 1;   // exchange sets written by the grouped launch
     snprintf(line, sizeof line, "{\"prec\": %d, \"rows\": %ld, \"spin_free_units\": {\"two_launches_ms\": %.4f, "
-             "\"grouped_ms\": %.4f, \"grouped_256x256_ms\": %.4f, \"chosen\": \"%s\"}}\n", prec, 2 * Gv, t[0],
-             t[1], ncand > 2 ? t[2] : -1.0f, names[pick]);
+             "\"grouped_ms\": %.4f, \"grouped_256x256_ms\": %.4f, \"chosen\": \"%s\", \"unpack\": \"%s\"}}\n",
+             prec, 2 * Gv, t[0], t[1], ncand > 2 ? t[2] : -1.0f, names[pick], half ? "aa_bb" : "all");
     ctx->tune_log += line;
     git = ctx->xgroup.emplace(gkey, pick).first;
   }
