@@ -161,6 +161,236 @@ __global__ __launch_bounds__(256) void energy_kernel(int nso, int nocc, const do
   }
 }
 
+// The epilogue of one grid point in one 256-thread workgroup (NSO <= 128,
+// N <= 32, one density per point), fusing the Fock product, Q, the energy and
+// the adjoint of the separate launches:
+//   FX = (h + Gamma_g) X_g,  Q = C0^H FX,  e_g = (tr(Hc X_g) + tr Q) / 2,
+//   Y_g = FX - X_g (Q - e_g I)  (written to Y, the layout of X), E[g] = e_g.
+// Phase 1 forms FX^T = X^T (h + Gamma)^T on the FP64 matrix cores (three
+// real products per complex one, 16-deep k-slabs staged in LDS; wave w owns
+// the 32 x 32 block of FX rows p in [32 w, 32 w + 32)) and, while staging
+// X^T, the partial sums of tr(Hc X).  Phase 2 parks FX^T in LDS and wave w
+// forms the 16 x 16 tile (w / 2, w % 2) of Q, with C0 read from global
+// (shared by every point).  e_g is reduced over the workgroup in a fixed
+// order; Q - e_g I goes to LDS, and phase 3 subtracts X Q' from the FX block
+// each wave still holds.  Gam: row 2g the real, row 2g + 1 the imaginary
+// plane of h + Gamma_g, element (i, k) at i nso + k.  X_g, Y_g: NSO x N
+// column-major complex at g sX; C0 likewise; Hc(a, p) at a nso + p.
+constexpr int FA_LD = 18;          // k-slab row stride in LDS (16 + 2)
+constexpr int FA_FLD = 128 + 1;    // row stride of the parked FX^T planes
+__global__ __launch_bounds__(256, 2) void fock_adjoint_kernel(int nso, int nocc, const double* __restrict__ X,
+                                                              long sX, const double* __restrict__ Gam, long ldg,
+                                                              const double* __restrict__ C0,
+                                                              const double* __restrict__ Hc,
+                                                              double* __restrict__ Y, double* __restrict__ E) {
+  // phase 1: A slab planes [32][FA_LD] and B slab planes [128][FA_LD]; phase 2:
+  // FX^T planes [32][FA_FLD]; then Q' planes [32][33]
+  constexpr int P1 = 2 * 32 * FA_LD + 2 * 128 * FA_LD, P2 = 2 * 32 * FA_FLD;
+  __shared__ double sm[P1 > P2 ? P1 : P2];
+  __shared__ double red[2][4];
+  double* As_re = sm;
+  double* As_im = As_re + 32 * FA_LD;
+  double* Bs_re = As_im + 32 * FA_LD;
+  double* Bs_im = Bs_re + 128 * FA_LD;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const long g = blockIdx.x;
+  const double* Xg = X + g * sX;
+  const double* Gre = Gam + (2 * g) * ldg;
+  const double* Gim = Gre + ldg;
+  // ---- phase 1: FX^T[a][j] = sum_k X(k, a) G[j nso + k], wave w: j in [32 w, 32 w + 32)
+  d4_t p1[2][2], p2[2][2], p3[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) p1[i][j] = p2[i][j] = p3[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+  double hx_re = 0.0, hx_im = 0.0;   // this thread's terms of tr(Hc X)
+  // staging: A slab 32 x 16 (2 elements per thread), B slab 128 x 16 (8 per thread), k fastest
+  double lar[2], lai[2], lbr[8], lbi[8];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int idx = tid + 256 * e, kk = idx & 15, a = idx >> 4, k = k0 + kk;
+      const bool v = a < nocc && k < nso;
+      const long off = v ? 2L * (k + (long)a * nso) : 0;
+      lar[e] = v ? Xg[off] : 0.0;
+      lai[e] = v ? Xg[off + 1] : 0.0;
+      if (v) {   // Hc(a, k) X(k, a)
+        const double hr = Hc[off], hi = Hc[off + 1];
+        hx_re += hr * lar[e] - hi * lai[e];
+        hx_im += hr * lai[e] + hi * lar[e];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int idx = tid + 256 * e, kk = idx & 15, j = idx >> 4, k = k0 + kk;
+      const bool v = j < nso && k < nso;
+      const long off = v ? (long)j * nso + k : 0;
+      lbr[e] = v ? Gre[off] : 0.0;
+      lbi[e] = v ? Gim[off] : 0.0;
+    }
+  };
+  gload(0);
+  for (int k0 = 0; k0 < nso; k0 += 16) {
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int idx = tid + 256 * e, kk = idx & 15, a = idx >> 4;
+      As_re[a * FA_LD + kk] = lar[e];
+      As_im[a * FA_LD + kk] = lai[e];
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int idx = tid + 256 * e, kk = idx & 15, j = idx >> 4;
+      Bs_re[j * FA_LD + kk] = lbr[e];
+      Bs_im[j * FA_LD + kk] = lbi[e];
+    }
+    __syncthreads();
+    if (k0 + 16 < nso) gload(k0 + 16);
+#pragma unroll
+    for (int ks = 0; ks < 16; ks += 4) {
+      double are[2], aim[2], bre[2], bim[2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int a = t * 16 + (lane & 15), j = w * 32 + t * 16 + (lane & 15), kk = ks + (lane >> 4);
+        are[t] = As_re[a * FA_LD + kk];
+        aim[t] = As_im[a * FA_LD + kk];
+        bre[t] = Bs_re[j * FA_LD + kk];
+        bim[t] = Bs_im[j * FA_LD + kk];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          p1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bre[j], p1[i][j], 0, 0, 0);
+          p2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bim[j], p2[i][j], 0, 0, 0);
+          p3[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i] + aim[i], bre[j] + bim[j], p3[i][j], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+  // FX^T in this wave's registers: fxr/fxi[i][j][r] = element (a, p),
+  // a = 16 i + (lane >> 4) + 4 r, p = 32 w + 16 j + (lane & 15)
+  double fxr[2][2][4], fxi[2][2][4];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        fxr[i][j][r] = p1[i][j][r] - p2[i][j][r];
+        fxi[i][j][r] = p3[i][j][r] - p1[i][j][r] - p2[i][j][r];
+      }
+  // ---- phase 2: park FX^T, Q = C0^H FX; wave w: Q tile rows a in [16 (w >> 1), ...), cols b in [16 (w & 1), ...)
+  double* Fre = sm;
+  double* Fim = sm + 32 * FA_FLD;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int a = 16 * i + (lane >> 4) + 4 * r, pp = 32 * w + 16 * j + (lane & 15);
+        Fre[a * FA_FLD + pp] = fxr[i][j][r];
+        Fim[a * FA_FLD + pp] = fxi[i][j][r];
+      }
+  __syncthreads();
+  d4_t q1 = d4_t{0.0, 0.0, 0.0, 0.0}, q2 = q1, q3 = q1;
+  {
+    const int qa = 16 * (w >> 1) + (lane & 15), qb = 16 * (w & 1) + (lane & 15);
+    for (int p0 = 0; p0 < nso; p0 += 4) {
+      const int pp = p0 + (lane >> 4);
+      // A[a][p] = conj(C0(p, a)), B[p][b] = FX(p, b) = FX^T[b][p]
+      const bool v = qa < nocc && pp < nso;
+      const double cr = v ? C0[2 * (pp + (long)qa * nso)] : 0.0, ci = v ? -C0[2 * (pp + (long)qa * nso) + 1] : 0.0;
+      const double fr = Fre[qb * FA_FLD + pp], fi = Fim[qb * FA_FLD + pp];
+      q1 = __builtin_amdgcn_mfma_f64_16x16x4f64(cr, fr, q1, 0, 0, 0);
+      q2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ci, fi, q2, 0, 0, 0);
+      q3 = __builtin_amdgcn_mfma_f64_16x16x4f64(cr + ci, fr + fi, q3, 0, 0, 0);
+    }
+  }
+  // Q tile element (a, b): a = 16 (w >> 1) + (lane >> 4) + 4 r, b = 16 (w & 1) + (lane & 15)
+  double qr[4], qi[4], tr_re = 0.0, tr_im = 0.0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    qr[r] = q1[r] - q2[r];
+    qi[r] = q3[r] - q1[r] - q2[r];
+    const int a = 16 * (w >> 1) + (lane >> 4) + 4 * r, bcol = 16 * (w & 1) + (lane & 15);
+    if (a == bcol && a < nocc) {
+      tr_re += qr[r];
+      tr_im += qi[r];
+    }
+  }
+  // e_g: every wave's terms of tr(Hc X) + tr Q, reduced in a fixed order
+  double sr = hx_re + tr_re, si = hx_im + tr_im;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    sr += __shfl_xor(sr, off);
+    si += __shfl_xor(si, off);
+  }
+  if (lane == 0) {
+    red[0][w] = sr;
+    red[1][w] = si;
+  }
+  __syncthreads();   // also: every wave is done reading the parked FX^T
+  const double er = 0.5 * ((red[0][0] + red[0][1]) + (red[0][2] + red[0][3]));
+  const double ei = 0.5 * ((red[1][0] + red[1][1]) + (red[1][2] + red[1][3]));
+  if (tid == 0) {
+    E[2 * g] = er;
+    E[2 * g + 1] = ei;
+  }
+  // Q' = Q - e I to LDS (over the parked FX^T)
+  double* Qre = sm;
+  double* Qim = sm + 32 * 33;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int a = 16 * (w >> 1) + (lane >> 4) + 4 * r, bcol = 16 * (w & 1) + (lane & 15);
+    const bool d = a == bcol;
+    Qre[a * 33 + bcol] = qr[r] - (d ? er : 0.0);
+    Qim[a * 33 + bcol] = qi[r] - (d ? ei : 0.0);
+  }
+  __syncthreads();
+  // ---- phase 3: Y^T[b][p] = FX^T[b][p] - sum_c Q'(c, b) X(p, c), wave w: p in [32 w, 32 w + 32)
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) p1[i][j] = p2[i][j] = p3[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int c0 = 0; c0 < 32; c0 += 4) {
+    const int c = c0 + (lane >> 4);
+    double are[2], aim[2], bre[2], bim[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int b = 16 * t + (lane & 15), pp = 32 * w + 16 * t + (lane & 15);
+      are[t] = Qre[c * 33 + b];   // A[b][c] = Q'(c, b)
+      aim[t] = Qim[c * 33 + b];
+      const bool v = c < nocc && pp < nso;
+      const long off = v ? 2L * (pp + (long)c * nso) : 0;
+      bre[t] = v ? Xg[off] : 0.0;   // B[c][p] = X(p, c)
+      bim[t] = v ? Xg[off + 1] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        p1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bre[j], p1[i][j], 0, 0, 0);
+        p2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bim[j], p2[i][j], 0, 0, 0);
+        p3[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i] + aim[i], bre[j] + bim[j], p3[i][j], 0, 0, 0);
+      }
+  }
+  double* Yg = Y + g * sX;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int b = 16 * i + (lane >> 4) + 4 * r, pp = 32 * w + 16 * j + (lane & 15);
+        if (b >= nocc || pp >= nso) continue;
+        const double xr = p1[i][j][r] - p2[i][j][r], xi = p3[i][j][r] - p1[i][j][r] - p2[i][j][r];
+        Yg[2 * (pp + (long)b * nso)] = fxr[i][j][r] - xr;
+        Yg[2 * (pp + (long)b * nso) + 1] = fxi[i][j][r] - xi;
+      }
+}
+
 // partial[c][mn][e] = sum_{g in chunk c} Wd[g][mn] * V[g][e]   (complex), for
 // two sources at once: blockIdx.z = 0 takes (V0, part0), 1 takes (V1, part1)
 __global__ void wreduce_partial_kernel(int G, int gchunk, int len, int nmn, const double* __restrict__ Wd,
@@ -1676,6 +1906,13 @@ static bool no_fused_split() {
   return f && atoi(f) == 0;
 }
 
+// PHF_FUSED_EPILOGUE=0 keeps the Fock product, Q, the energy and the adjoint
+// as separate launches (the tests compare both).
+static bool no_fused_epilogue() {
+  const char* f = getenv("PHF_FUSED_EPILOGUE");
+  return f && atoi(f) == 0;
+}
+
 int eval_plan(phf_ctx* ctx, int nocc, int precision, int cmplxconj, EvalBufs& b) {
   if (!ctx->has_ham) return fail(ctx, PHF_ERR_NO_HAMILTONIAN, "phf_buildhsg: no Hamiltonian set");
   if (!ctx->has_grid || ctx->g_nso != ctx->nso)
@@ -2523,17 +2760,30 @@ void eval_post(phf_ctx* ctx, const EvalBufs& b, bool resident) {
   const long sT = 2 * lenX, sNN = 2L * N * N;
   const int nB = (V == 4) ? 2 : 1;
   const long per = (V == 4) ? 2 * G : G;
+  // one density per point, NSO <= 128, N <= 32: the Fock product, Q, the
+  // energy and the adjoint in one kernel per point (Y written over FX)
+  const bool fused = V == 1 && nso <= 128 && N <= 32 && !no_fused_epilogue();
+  if (fused) {
+    if (ctx->side[0])
+      (void)hipStreamWaitEvent(s, ctx->hc_join, 0);
+    else
+      hc_products(ctx, b, s);
+    fock_adjoint_kernel<<<(unsigned)G, 256, 0, s>>>(nso, N, b.X, sT, b.Gam, nrpad, b.C0, b.Hc, b.FX, b.E);
+  }
   // K4: FX = (h + Gamma) X   (the ERI GEMM epilogue added h), formed as
   // FX^T = X^T (h + Gamma)^T so that the stores run along the NSO index
-  zgemm_batched(s, N, nso, nso, 1, 0, zmat_colmajor_t(b.X, nso, sT), ZMat{b.Gam, 1, nso, 2L * nrpad, nrpad, 0}, 0, 0,
-                zout_colmajor_t(b.FX, nso, sT), (int)Gv);
+  if (!fused)
+    zgemm_batched(s, N, nso, nso, 1, 0, zmat_colmajor_t(b.X, nso, sT), ZMat{b.Gam, 1, nso, 2L * nrpad, nrpad, 0}, 0,
+                  0, zout_colmajor_t(b.FX, nso, sT), (int)Gv);
   // per right factor B: Q_v = B^H FX_v, Hc = B^H h (computed by eval_pre on a
   // side stream when the context has one), e_v = Makeh
-  if (ctx->side[0])
-    (void)hipStreamWaitEvent(s, ctx->hc_join, 0);
-  else
-    hc_products(ctx, b, s);
-  for (int bc = 0; bc < nB; ++bc) {
+  if (!fused) {
+    if (ctx->side[0])
+      (void)hipStreamWaitEvent(s, ctx->hc_join, 0);
+    else
+      hc_products(ctx, b, s);
+  }
+  for (int bc = 0; bc < nB && !fused; ++bc) {
     const ZMat bh{b.C0, 2L * nso, 2, 0, 1, bc ? 0 : 1};
     double* Hc = b.Hc + bc * sT;
     zgemm_batched(s, N, N, nso, 1, 0, bh, zmat_colmajor(b.FX + bc * per * sT, nso, sT), 0, 0,
@@ -2544,8 +2794,9 @@ void eval_post(phf_ctx* ctx, const EvalBufs& b, bool resident) {
   // Y_v = FX_v + e_v X_v - X_v Q_v = FX_v - X_v (Q_v - e_v I), in place over FX
   // (energy_kernel shifted the diagonal of Q); Y^T = FX^T - (Q - e I)^T X^T
   double* Y = b.FX;
-  zgemm_batched(s, N, nso, N, -1, 0, zmat_colmajor_t(b.Q, N, sNN), zmat_colmajor_t(b.X, nso, sT), 1, 0,
-                zout_colmajor_t(Y, nso, sT), (int)Gv);
+  if (!fused)
+    zgemm_batched(s, N, nso, N, -1, 0, zmat_colmajor_t(b.Q, N, sNN), zmat_colmajor_t(b.X, nso, sT), 1, 0,
+                  zout_colmajor_t(Y, nso, sT), (int)Gv);
   // K5: weighted grid sums; every variant lands in its own CI block, so one
   // reduction over all virtual points assembles the nci x nci matrices
   const int d = (V == 4) ? nci / 2 : nci;
@@ -2671,10 +2922,10 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
   auto make_key = [&]() {
     char kb[512];
     auto env = [](const char* v) { const char* e = getenv(v); return e ? e : "-"; };
-    snprintf(kb, sizeof kb, "%d|%d|%d|%d|%d|%ld|%p|%p|%p|%p|%zu|%s|%s|%s|%s|%s", precision, nocc, cmplxconj,
+    snprintf(kb, sizeof kb, "%d|%d|%d|%d|%d|%ld|%p|%p|%p|%p|%zu|%s|%s|%s|%s|%s|%s", precision, nocc, cmplxconj,
              (int)resident, (int)(objective != nullptr), ctx->gen, (void*)ctx->ws, (void*)ctx->pin,
              (void*)ctx->small, (void*)ctx->d_xitems, ctx->ws_size, env("PHF_H16_CFG"), env("PHF_XGROUP"),
-             env("PHF_FUSED_SPLIT"), env("PHF_H16_FOLD"), env("PHF_H16_ALLOW_F32FOLD"));
+             env("PHF_FUSED_SPLIT"), env("PHF_H16_FOLD"), env("PHF_H16_ALLOW_F32FOLD"), env("PHF_FUSED_EPILOGUE"));
     return std::string(kb);
   };
   std::string key = graph_ok ? make_key() : std::string();
