@@ -166,109 +166,122 @@ __global__ __launch_bounds__(256) void energy_kernel(int nso, int nocc, const do
 // the adjoint of the separate launches:
 //   FX = (h + Gamma_g) X_g,  Q = C0^H FX,  e_g = (tr(Hc X_g) + tr Q) / 2,
 //   Y_g = FX - X_g (Q - e_g I)  (written to Y, the layout of X), E[g] = e_g.
-// Phase 1 forms FX^T = X^T (h + Gamma)^T on the FP64 matrix cores (three
-// real products per complex one, 16-deep k-slabs staged in LDS; wave w owns
-// the 32 x 32 block of FX rows p in [32 w, 32 w + 32)) and, while staging
-// X^T, the partial sums of tr(Hc X).  Phase 2 parks FX^T in LDS and wave w
-// forms the 16 x 16 tile (w / 2, w % 2) of Q, with C0 read from global
-// (shared by every point).  e_g is reduced over the workgroup in a fixed
-// order; Q - e_g I goes to LDS, and phase 3 subtracts X Q' from the FX block
-// each wave still holds.  Gam: row 2g the real, row 2g + 1 the imaginary
-// plane of h + Gamma_g, element (i, k) at i nso + k.  X_g, Y_g: NSO x N
-// column-major complex at g sX; C0 likewise; Hc(a, p) at a nso + p.
-constexpr int FA_LD = 18;          // k-slab row stride in LDS (16 + 2)
-constexpr int FA_FLD = 128 + 1;    // row stride of the parked FX^T planes
+// X_g is staged once in LDS as [k][a] (with the terms of tr(Hc X) formed on
+// the way).  Wave w owns the FX rows p in [32 w, 32 w + 32) (all N columns) on
+// the FP64 matrix cores, three real products per complex one, with no barrier
+// in the k loop: each lane loads its rows of h + Gamma straight into the MFMA
+// A layout, the k index of a 16-deep slab permuted so that step s of lane
+// group q = lane / 16 takes k = k0 + 4 q + s (16-byte loads of consecutive k),
+// and the next slab is loaded while the current one is multiplied.  The FX
+// block stays in the wave's registers in the MFMA's D layout, which is also
+// its B layout with the row index as the k index: phase 2 forms this wave's
+// share of Q = sum over its rows p of conj(C0(p, a)) FX(p, b), the four
+// shares are summed through LDS in a fixed order, and every wave forms e_g
+// the same way; phase 3 subtracts X (Q - e I) from its FX block.
+// Gam: row 2g the real, row 2g + 1 the imaginary plane of h + Gamma_g, element
+// (i, k) at i nso + k.  X_g, Y_g: NSO x N column-major complex at g sX; C0
+// likewise; Hc(a, p) at a nso + p.
+constexpr int FA_LD = 33;   // row stride (complex elements) of X^T and of the Q shares in LDS
 __global__ __launch_bounds__(256, 2) void fock_adjoint_kernel(int nso, int nocc, const double* __restrict__ X,
                                                               long sX, const double* __restrict__ Gam, long ldg,
                                                               const double* __restrict__ C0,
                                                               const double* __restrict__ Hc,
                                                               double* __restrict__ Y, double* __restrict__ E) {
-  // phase 1: A slab planes [32][FA_LD] and B slab planes [128][FA_LD]; phase 2:
-  // FX^T planes [32][FA_FLD]; then Q' planes [32][33]
-  constexpr int P1 = 2 * 32 * FA_LD + 2 * 128 * FA_LD, P2 = 2 * 32 * FA_FLD;
-  __shared__ double sm[P1 > P2 ? P1 : P2];
+  // X_g as [k][a] (128 rows), later the four Q shares (32 rows each)
+  __shared__ double2 sm[128 * FA_LD];
   __shared__ double red[2][4];
-  double* As_re = sm;
-  double* As_im = As_re + 32 * FA_LD;
-  double* Bs_re = As_im + 32 * FA_LD;
-  double* Bs_im = Bs_re + 128 * FA_LD;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, l16 = lane & 15, kq = lane >> 4;
+  const int pw = 32 * w;
   const long g = blockIdx.x;
   const double* Xg = X + g * sX;
   const double* Gre = Gam + (2 * g) * ldg;
   const double* Gim = Gre + ldg;
-  // ---- phase 1: FX^T[a][j] = sum_k X(k, a) G[j nso + k], wave w: j in [32 w, 32 w + 32)
+  const double2 z2 = make_double2(0.0, 0.0);
+  // stage X^T; this thread's terms of tr(Hc X) (Hc(a, k) and X(k, a) share the flat index)
+  double hx_re = 0.0, hx_im = 0.0;
+  {
+    const double2* H2 = reinterpret_cast<const double2*>(Hc);
+    const double2* X2 = reinterpret_cast<const double2*>(Xg);
+    const int len = nso * nocc;
+#pragma unroll 4
+    for (int idx = tid; idx < len; idx += 256) {
+      const double2 h = H2[idx], x = X2[idx];
+      hx_re += h.x * x.x - h.y * x.y;
+      hx_im += h.x * x.y + h.y * x.x;
+      const int a = idx / nso, k = idx - a * nso;
+      sm[k * FA_LD + a] = x;
+    }
+  }
   d4_t p1[2][2], p2[2][2], p3[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) p1[i][j] = p2[i][j] = p3[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
-  double hx_re = 0.0, hx_im = 0.0;   // this thread's terms of tr(Hc X)
-  // staging: A slab 32 x 16 (2 elements per thread), B slab 128 x 16 (8 per thread), k fastest
-  double lar[2], lai[2], lbr[8], lbi[8];
-  auto gload = [&](int k0) {
+  __syncthreads();
+  // ---- phase 1: FX(p, a) = sum_k F(p, k) X(k, a); A[p][k] = F(p, k), B[k][a] = X(k, a).
+  // Every load is unconditional (rows clamped to nso - 1, columns k >= nso
+  // read at 0): the B operand is zero for k >= nso, and FX rows p >= nso are
+  // neither stored nor used (C0 is zero there in phase 2).
+  if (pw < nso) {
+    const long row0 = (long)min(pw + l16, nso - 1) * nso, row1 = (long)min(pw + 16 + l16, nso - 1) * nso;
+    // a slab's A operands: fr/fi[pi][h] = F(p, k0 + 4 kq + 2 h + {0, 1}), p = pw + 16 pi + l16
+    auto load = [&](int k0, double2 (&ar)[2][2], double2 (&ai)[2][2]) {
 #pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      const int idx = tid + 256 * e, kk = idx & 15, a = idx >> 4, k = k0 + kk;
-      const bool v = a < nocc && k < nso;
-      const long off = v ? 2L * (k + (long)a * nso) : 0;
-      lar[e] = v ? Xg[off] : 0.0;
-      lai[e] = v ? Xg[off + 1] : 0.0;
-      if (v) {   // Hc(a, k) X(k, a)
-        const double hr = Hc[off], hi = Hc[off + 1];
-        hx_re += hr * lar[e] - hi * lai[e];
-        hx_im += hr * lai[e] + hi * lar[e];
+      for (int h = 0; h < 2; ++h) {
+        const int k = k0 + 4 * kq + 2 * h;   // NSO is even: k < nso implies k + 1 < nso
+        const int kc = k < nso ? k : 0;
+        ar[0][h] = *reinterpret_cast<const double2*>(Gre + row0 + kc);
+        ai[0][h] = *reinterpret_cast<const double2*>(Gim + row0 + kc);
+        ar[1][h] = *reinterpret_cast<const double2*>(Gre + row1 + kc);
+        ai[1][h] = *reinterpret_cast<const double2*>(Gim + row1 + kc);
       }
-    }
+    };
+    auto slab = [&](int k0, const double2 (&ar)[2][2], const double2 (&ai)[2][2]) {
+      double2 xb[2][4];   // [aj][s] = X(k0 + 4 kq + s, 16 aj + l16)
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int idx = tid + 256 * e, kk = idx & 15, j = idx >> 4, k = k0 + kk;
-      const bool v = j < nso && k < nso;
-      const long off = v ? (long)j * nso + k : 0;
-      lbr[e] = v ? Gre[off] : 0.0;
-      lbi[e] = v ? Gim[off] : 0.0;
-    }
-  };
-  gload(0);
-  for (int k0 = 0; k0 < nso; k0 += 16) {
+      for (int s = 0; s < 4; ++s) {
+        const int k = k0 + 4 * kq + s;
 #pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      const int idx = tid + 256 * e, kk = idx & 15, a = idx >> 4;
-      As_re[a * FA_LD + kk] = lar[e];
-      As_im[a * FA_LD + kk] = lai[e];
-    }
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int idx = tid + 256 * e, kk = idx & 15, j = idx >> 4;
-      Bs_re[j * FA_LD + kk] = lbr[e];
-      Bs_im[j * FA_LD + kk] = lbi[e];
-    }
-    __syncthreads();
-    if (k0 + 16 < nso) gload(k0 + 16);
-#pragma unroll
-    for (int ks = 0; ks < 16; ks += 4) {
-      double are[2], aim[2], bre[2], bim[2];
-#pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int a = t * 16 + (lane & 15), j = w * 32 + t * 16 + (lane & 15), kk = ks + (lane >> 4);
-        are[t] = As_re[a * FA_LD + kk];
-        aim[t] = As_im[a * FA_LD + kk];
-        bre[t] = Bs_re[j * FA_LD + kk];
-        bim[t] = Bs_im[j * FA_LD + kk];
+        for (int aj = 0; aj < 2; ++aj) xb[aj][s] = sm[(k < nso ? k : 0) * FA_LD + 16 * aj + l16];
       }
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int s = 0; s < 4; ++s) {
+        const bool vk = k0 + 4 * kq + s < nso;
+        double are[2], aim[2], bre[2], bim[2];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          p1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bre[j], p1[i][j], 0, 0, 0);
-          p2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bim[j], p2[i][j], 0, 0, 0);
-          p3[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i] + aim[i], bre[j] + bim[j], p3[i][j], 0, 0, 0);
+        for (int pi = 0; pi < 2; ++pi) {
+          are[pi] = (s & 1) ? ar[pi][s >> 1].y : ar[pi][s >> 1].x;
+          aim[pi] = (s & 1) ? ai[pi][s >> 1].y : ai[pi][s >> 1].x;
         }
+#pragma unroll
+        for (int aj = 0; aj < 2; ++aj) {
+          const bool v = vk && 16 * aj + l16 < nocc;
+          bre[aj] = v ? xb[aj][s].x : 0.0;
+          bim[aj] = v ? xb[aj][s].y : 0.0;
+        }
+#pragma unroll
+        for (int pi = 0; pi < 2; ++pi)
+#pragma unroll
+          for (int aj = 0; aj < 2; ++aj) {
+            p1[pi][aj] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[pi], bre[aj], p1[pi][aj], 0, 0, 0);
+            p2[pi][aj] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[pi], bim[aj], p2[pi][aj], 0, 0, 0);
+            p3[pi][aj] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[pi] + aim[pi], bre[aj] + bim[aj], p3[pi][aj], 0,
+                                                              0, 0);
+          }
+      }
+    };
+    // two register buffers; slabs in pairs (a slab past nso contributes zero)
+    double2 fr0[2][2], fi0[2][2], fr1[2][2], fi1[2][2];
+    load(0, fr0, fi0);
+    for (int k0 = 0; k0 < nso; k0 += 32) {
+      load(k0 + 16, fr1, fi1);
+      slab(k0, fr0, fi0);
+      load(k0 + 32, fr0, fi0);
+      slab(k0 + 16, fr1, fi1);
     }
-    __syncthreads();
   }
-  // FX^T in this wave's registers: fxr/fxi[i][j][r] = element (a, p),
-  // a = 16 i + (lane >> 4) + 4 r, p = 32 w + 16 j + (lane & 15)
+  // FX block (pi, aj) in this wave's registers: fxr/fxi[pi][aj][r] = FX(p, a),
+  // p = pw + 16 pi + kq + 4 r, a = 16 aj + l16
   double fxr[2][2][4], fxi[2][2][4];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -279,115 +292,140 @@ __global__ __launch_bounds__(256, 2) void fock_adjoint_kernel(int nso, int nocc,
         fxr[i][j][r] = p1[i][j][r] - p2[i][j][r];
         fxi[i][j][r] = p3[i][j][r] - p1[i][j][r] - p2[i][j][r];
       }
-  // ---- phase 2: park FX^T, Q = C0^H FX; wave w: Q tile rows a in [16 (w >> 1), ...), cols b in [16 (w & 1), ...)
-  double* Fre = sm;
-  double* Fim = sm + 32 * FA_FLD;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    hx_re += __shfl_xor(hx_re, off);
+    hx_im += __shfl_xor(hx_im, off);
+  }
+  if (lane == 0) {
+    red[0][w] = hx_re;
+    red[1][w] = hx_im;
+  }
+  // ---- phase 2: this wave's share of Q(a, b) = sum_{p in its rows} conj(C0(p, a)) FX(p, b);
+  // k-step (pi, r) takes p = pw + 16 pi + 4 r + kq, whose B operand is fx[pi][bj][r]
+  d4_t q1[2][2], q2[2][2], q3[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < 2; ++j) q1[i][j] = q2[i][j] = q3[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+  if (pw < nso) {
+#pragma unroll
+    for (int pi = 0; pi < 2; ++pi) {
+      double2 c0v[2][4];   // [ai][r]
+#pragma unroll
+      for (int ai = 0; ai < 2; ++ai)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int a = 16 * ai + l16, p = pw + 16 * pi + 4 * r + kq;
+          const bool v = a < nocc && p < nso;
+          const double2 c = *reinterpret_cast<const double2*>(C0 + (v ? 2L * (p + (long)a * nso) : 0));
+          c0v[ai][r] = v ? c : z2;
+        }
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int ai = 0; ai < 2; ++ai) {
+          const double cr = c0v[ai][r].x, ci = -c0v[ai][r].y;
+#pragma unroll
+          for (int bj = 0; bj < 2; ++bj) {
+            const double br = fxr[pi][bj][r], bi = fxi[pi][bj][r];
+            q1[ai][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(cr, br, q1[ai][bj], 0, 0, 0);
+            q2[ai][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(ci, bi, q2[ai][bj], 0, 0, 0);
+            q3[ai][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(cr + ci, br + bi, q3[ai][bj], 0, 0, 0);
+          }
+        }
+    }
+  }
+  __syncthreads();   // every wave is done reading X^T
+  // share element (a, b) at w 32 FA_LD + a FA_LD + b: a = 16 ai + kq + 4 r, b = 16 bj + l16
+  double2* qs = sm;
+#pragma unroll
+  for (int ai = 0; ai < 2; ++ai)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int a = 16 * i + (lane >> 4) + 4 * r, pp = 32 * w + 16 * j + (lane & 15);
-        Fre[a * FA_FLD + pp] = fxr[i][j][r];
-        Fim[a * FA_FLD + pp] = fxi[i][j][r];
+        const int a = 16 * ai + kq + 4 * r, b = 16 * bj + l16;
+        qs[(32 * w + a) * FA_LD + b] =
+            make_double2(q1[ai][bj][r] - q2[ai][bj][r], q3[ai][bj][r] - q1[ai][bj][r] - q2[ai][bj][r]);
       }
   __syncthreads();
-  d4_t q1 = d4_t{0.0, 0.0, 0.0, 0.0}, q2 = q1, q3 = q1;
+  // Q = (Q_0 + Q_1) + (Q_2 + Q_3), in place over share 0
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const int idx = tid + 256 * m, o = (idx >> 5) * FA_LD + (idx & 31), S = 32 * FA_LD;
+    const double2 a0 = qs[o], a1 = qs[S + o], a2 = qs[2 * S + o], a3 = qs[3 * S + o];
+    qs[o] = make_double2((a0.x + a1.x) + (a2.x + a3.x), (a0.y + a1.y) + (a2.y + a3.y));
+  }
+  __syncthreads();
+  // e_g, formed by every wave the same way: tr(Hc X) + tr Q
+  double er, ei;
   {
-    const int qa = 16 * (w >> 1) + (lane & 15), qb = 16 * (w & 1) + (lane & 15);
-    for (int p0 = 0; p0 < nso; p0 += 4) {
-      const int pp = p0 + (lane >> 4);
-      // A[a][p] = conj(C0(p, a)), B[p][b] = FX(p, b) = FX^T[b][p]
-      const bool v = qa < nocc && pp < nso;
-      const double cr = v ? C0[2 * (pp + (long)qa * nso)] : 0.0, ci = v ? -C0[2 * (pp + (long)qa * nso) + 1] : 0.0;
-      const double fr = Fre[qb * FA_FLD + pp], fi = Fim[qb * FA_FLD + pp];
-      q1 = __builtin_amdgcn_mfma_f64_16x16x4f64(cr, fr, q1, 0, 0, 0);
-      q2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ci, fi, q2, 0, 0, 0);
-      q3 = __builtin_amdgcn_mfma_f64_16x16x4f64(cr + ci, fr + fi, q3, 0, 0, 0);
+    double tr_re = 0.0, tr_im = 0.0;
+    if (lane < nocc) {
+      const double2 d = qs[lane * (FA_LD + 1)];
+      tr_re = d.x;
+      tr_im = d.y;
     }
-  }
-  // Q tile element (a, b): a = 16 (w >> 1) + (lane >> 4) + 4 r, b = 16 (w & 1) + (lane & 15)
-  double qr[4], qi[4], tr_re = 0.0, tr_im = 0.0;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    qr[r] = q1[r] - q2[r];
-    qi[r] = q3[r] - q1[r] - q2[r];
-    const int a = 16 * (w >> 1) + (lane >> 4) + 4 * r, bcol = 16 * (w & 1) + (lane & 15);
-    if (a == bcol && a < nocc) {
-      tr_re += qr[r];
-      tr_im += qi[r];
+    for (int off = 32; off > 0; off >>= 1) {
+      tr_re += __shfl_xor(tr_re, off);
+      tr_im += __shfl_xor(tr_im, off);
     }
+    er = 0.5 * (((red[0][0] + red[0][1]) + (red[0][2] + red[0][3])) + tr_re);
+    ei = 0.5 * (((red[1][0] + red[1][1]) + (red[1][2] + red[1][3])) + tr_im);
   }
-  // e_g: every wave's terms of tr(Hc X) + tr Q, reduced in a fixed order
-  double sr = hx_re + tr_re, si = hx_im + tr_im;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    sr += __shfl_xor(sr, off);
-    si += __shfl_xor(si, off);
-  }
-  if (lane == 0) {
-    red[0][w] = sr;
-    red[1][w] = si;
-  }
-  __syncthreads();   // also: every wave is done reading the parked FX^T
-  const double er = 0.5 * ((red[0][0] + red[0][1]) + (red[0][2] + red[0][3]));
-  const double ei = 0.5 * ((red[1][0] + red[1][1]) + (red[1][2] + red[1][3]));
   if (tid == 0) {
     E[2 * g] = er;
     E[2 * g + 1] = ei;
   }
-  // Q' = Q - e I to LDS (over the parked FX^T)
-  double* Qre = sm;
-  double* Qim = sm + 32 * 33;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int a = 16 * (w >> 1) + (lane >> 4) + 4 * r, bcol = 16 * (w & 1) + (lane & 15);
-    const bool d = a == bcol;
-    Qre[a * 33 + bcol] = qr[r] - (d ? er : 0.0);
-    Qim[a * 33 + bcol] = qi[r] - (d ? ei : 0.0);
-  }
-  __syncthreads();
-  // ---- phase 3: Y^T[b][p] = FX^T[b][p] - sum_c Q'(c, b) X(p, c), wave w: p in [32 w, 32 w + 32)
+  if (pw >= nso) return;
+  // ---- phase 3: Y(p, b) = FX(p, b) - sum_c X(p, c) Q'(c, b), Q' = Q - e I;
+  // A[p][c] = X(p, c), B[c][b] = Q'(c, b), k-step t takes c = 4 t + kq
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) p1[i][j] = p2[i][j] = p3[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int c0 = 0; c0 < 32; c0 += 4) {
-    const int c = c0 + (lane >> 4);
+  for (int t = 0; t < 8; ++t) {
+    const int c = 4 * t + kq;
     double are[2], aim[2], bre[2], bim[2];
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      const int b = 16 * t + (lane & 15), pp = 32 * w + 16 * t + (lane & 15);
-      are[t] = Qre[c * 33 + b];   // A[b][c] = Q'(c, b)
-      aim[t] = Qim[c * 33 + b];
-      const bool v = c < nocc && pp < nso;
-      const long off = v ? 2L * (pp + (long)c * nso) : 0;
-      bre[t] = v ? Xg[off] : 0.0;   // B[c][p] = X(p, c)
-      bim[t] = v ? Xg[off + 1] : 0.0;
+    for (int pi = 0; pi < 2; ++pi) {
+      const int p = pw + 16 * pi + l16;
+      const bool v = p < nso && c < nocc;
+      const double2 x = *reinterpret_cast<const double2*>(Xg + (v ? 2L * (p + (long)c * nso) : 0));
+      are[pi] = v ? x.x : 0.0;
+      aim[pi] = v ? x.y : 0.0;
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int bj = 0; bj < 2; ++bj) {
+      const int b = 16 * bj + l16;
+      const double2 q = qs[c * FA_LD + b];
+      bre[bj] = q.x - (c == b ? er : 0.0);
+      bim[bj] = q.y - (c == b ? ei : 0.0);
+    }
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        p1[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i], bre[j], p1[i][j], 0, 0, 0);
-        p2[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[i], bim[j], p2[i][j], 0, 0, 0);
-        p3[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[i] + aim[i], bre[j] + bim[j], p3[i][j], 0, 0, 0);
+    for (int pi = 0; pi < 2; ++pi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) {
+        p1[pi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[pi], bre[bj], p1[pi][bj], 0, 0, 0);
+        p2[pi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim[pi], bim[bj], p2[pi][bj], 0, 0, 0);
+        p3[pi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(are[pi] + aim[pi], bre[bj] + bim[bj], p3[pi][bj], 0, 0, 0);
       }
   }
   double* Yg = Y + g * sX;
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int pi = 0; pi < 2; ++pi)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int b = 16 * i + (lane >> 4) + 4 * r, pp = 32 * w + 16 * j + (lane & 15);
-        if (b >= nocc || pp >= nso) continue;
-        const double xr = p1[i][j][r] - p2[i][j][r], xi = p3[i][j][r] - p1[i][j][r] - p2[i][j][r];
-        Yg[2 * (pp + (long)b * nso)] = fxr[i][j][r] - xr;
-        Yg[2 * (pp + (long)b * nso) + 1] = fxi[i][j][r] - xi;
+        const int p = pw + 16 * pi + kq + 4 * r, b = 16 * bj + l16;
+        if (p >= nso || b >= nocc) continue;
+        const double xr = p1[pi][bj][r] - p2[pi][bj][r], xi = p3[pi][bj][r] - p1[pi][bj][r] - p2[pi][bj][r];
+        *reinterpret_cast<double2*>(Yg + 2L * (p + (long)b * nso)) =
+            make_double2(fxr[pi][bj][r] - xr, fxi[pi][bj][r] - xi);
       }
 }
 
