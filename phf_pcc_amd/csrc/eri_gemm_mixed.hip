@@ -533,23 +533,26 @@ __device__ __forceinline__ void rn16_tile(const T* __restrict__ P, long ldP, lon
     __builtin_amdgcn_s_barrier();
     if (kt + STAGES - 1 < nk) issue(kt + STAGES - 1);
     const unsigned char* st = smem + (kt % STAGES) * STAGE_BYTES;
-    v8 a0[FM], a1[FM];
-#pragma unroll
-    for (int i = 0; i < FM; ++i) {
-      a0[i] = frag(st, wm * WM + i * 16 + (lane & 15), lane >> 4);
-      a1[i] = frag(st, wm * WM + i * 16 + (lane & 15), 4 + (lane >> 4));
-    }
+    // the B fragments of both k-steps stay in registers; A row block by row
+    // block, so that only FN partials are live per block and the adds of one
+    // block overlap the products of the next
+    v8 b0[FN], b1[FN];
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
-      const v8 b0 = frag(st, BM + wn * WN + j * 16 + (lane & 15), lane >> 4);
-      const v8 b1 = frag(st, BM + wn * WN + j * 16 + (lane & 15), 4 + (lane >> 4));
-      f4_t t[FM];
+      b0[j] = frag(st, BM + wn * WN + j * 16 + (lane & 15), lane >> 4);
+      b1[j] = frag(st, BM + wn * WN + j * 16 + (lane & 15), 4 + (lane >> 4));
+    }
 #pragma unroll
-      for (int i = 0; i < FM; ++i) t[i] = MfmaH<T>::mma(a0[i], b0, f4_t{0.f, 0.f, 0.f, 0.f});
+    for (int i = 0; i < FM; ++i) {
+      const v8 a0 = frag(st, wm * WM + i * 16 + (lane & 15), lane >> 4);
+      const v8 a1 = frag(st, wm * WM + i * 16 + (lane & 15), 4 + (lane >> 4));
+      f4_t t[FN];
 #pragma unroll
-      for (int i = 0; i < FM; ++i) {
-        t[i] = MfmaH<T>::mma(a1[i], b1, t[i]);
-        acc[i][j] += t[i];
+      for (int j = 0; j < FN; ++j) t[j] = MfmaH<T>::mma(a0, b0[j], f4_t{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        t[j] = MfmaH<T>::mma(a1, b1[j], t[j]);
+        acc[i][j] += t[j];
       }
     }
   }
