@@ -151,16 +151,18 @@ __global__ __launch_bounds__(256) void lu_inverse_kernel(int n, const double* __
 // i holds row pos_i of B^{-1}, and M^{-1} = B^{-1} P, so storage column m of
 // row i lands in column sigma(m) of M^{-1}, sigma(m) = the row that ended at
 // position m.
-template <int NMAX>
-__global__ __launch_bounds__(64) void lu_inverse_wave_kernel(int n, const double* __restrict__ M, long mbs,
-                                                             double* __restrict__ Minv, long ibs,
-                                                             double* __restrict__ det, int* __restrict__ info) {
+//
+// The routine runs on one wave (lane = threadIdx.x % 64) with LDS scratch
+// s_row[NMAX], s_sig[64]; sync() orders the wave's LDS accesses between the
+// steps (a workgroup barrier in a one-wave workgroup; in a larger workgroup
+// where one wave runs it, a wavefront-scope fence).  put(r, c, v) receives
+// M^{-1}(r, c) for r, c < n (zero when M is singular), once each; the
+// determinant and info go to *det_out (complex) and *info_out from lane 0.
+template <int NMAX, typename Sync, typename Put>
+__device__ __forceinline__ void lu_wave_inverse(int n, const double* __restrict__ Mg, double2* s_row, int* s_sig,
+                                                Sync sync, Put put, double* det_out, int* info_out) {
   constexpr int LPR = 64 / NMAX, CPL = NMAX / LPR;
-  __shared__ double2 s_row[NMAX];
-  __shared__ int s_sig[64];
-  const long g = blockIdx.x;
-  const int lane = threadIdx.x, row = lane % NMAX, part = lane / NMAX;
-  const double* Mg = M + g * mbs;
+  const int lane = threadIdx.x & 63, row = lane % NMAX, part = lane / NMAX;
   const bool real_row = row < n;
   double2 a[CPL];
 #pragma unroll
@@ -207,7 +209,7 @@ __global__ __launch_bounds__(64) void lu_inverse_wave_kernel(int n, const double
       const int src = (part < LPR - 1) ? lane + NMAX : lane;
       nxt = make_double2(__shfl(a[0].x, src), __shfl(a[0].y, src));
     }
-    __syncthreads();
+    sync();
     const double2 pv = s_row[0];
     detv = cmul(detv, pv);
     if (pp != k) detv = make_double2(-detv.x, -detv.y);
@@ -230,26 +232,123 @@ __global__ __launch_bounds__(64) void lu_inverse_wave_kernel(int n, const double
       }
       if (t == 0) search(c, key);
     }
-    __syncthreads();
+    sync();
   }
   if (part == 0) s_sig[pos] = row;
-  __syncthreads();
+  sync();
   if (real_row) {
-    double* Ig = Minv + g * ibs;
 #pragma unroll
     for (int t = 0; t < CPL; ++t) {
       const int m = part * CPL + t + n - NMAX;
-      if (m >= 0) {
-        const long off = 2 * ((long)s_sig[m] * n + pos);
-        Ig[off] = sinfo ? 0.0 : a[t].x;
-        Ig[off + 1] = sinfo ? 0.0 : a[t].y;
-      }
+      if (m >= 0) put(pos, s_sig[m], sinfo ? make_double2(0.0, 0.0) : a[t]);
     }
   }
   if (lane == 0) {
-    det[2 * g] = sinfo ? 0.0 : detv.x;
-    det[2 * g + 1] = sinfo ? 0.0 : detv.y;
-    info[g] = sinfo;
+    det_out[0] = sinfo ? 0.0 : detv.x;
+    det_out[1] = sinfo ? 0.0 : detv.y;
+    *info_out = sinfo;
+  }
+}
+
+template <int NMAX>
+__global__ __launch_bounds__(64) void lu_inverse_wave_kernel(int n, const double* __restrict__ M, long mbs,
+                                                             double* __restrict__ Minv, long ibs,
+                                                             double* __restrict__ det, int* __restrict__ info) {
+  __shared__ double2 s_row[NMAX];
+  __shared__ int s_sig[64];
+  const long g = blockIdx.x;
+  double* Ig = Minv + g * ibs;
+  // M^{-1} column-major
+  lu_wave_inverse<NMAX>(n, M + g * mbs, s_row, s_sig, [] { __syncthreads(); },
+                        [&](int r, int c, double2 v) {
+                          const long off = 2 * ((long)c * n + r);
+                          Ig[off] = v.x;
+                          Ig[off + 1] = v.y;
+                        },
+                        det + 2 * g, info + g);
+}
+
+// Inverse and solve for one grid point per 128-thread workgroup (n <= 32):
+// wave 0 inverts M_g (lu_wave_inverse, M^{-1} kept in LDS), then both waves
+// form X_g = T_g M_g^{-1} (NSO x n, column-major complex) on the FP64 matrix
+// cores, three real products per complex one, wave w taking the 16-row
+// blocks w, w + 2, ...; k-step t takes c = 4 t + lane / 16.  Two waves, not
+// more: with two waves per SIMD a CU holds four workgroups, so up to 1024
+// points' workgroups are resident at once (the inverse is latency-bound; a
+// second round of workgroups would double it).  This replaces the
+// inverse's M^{-1} store and the solve's reads of it.  A singular M_g
+// (info > 0) gives X_g = 0, as the zero inverse does.
+constexpr int LS_LD = 33;   // row stride (complex) of M^{-1} in LDS
+__global__ __launch_bounds__(128, 2) void lu_inverse_solve_kernel(int n, int nso, const double* __restrict__ M, long mbs,
+                                                               const double* __restrict__ T, long sT,
+                                                               double* __restrict__ X, long sX,
+                                                               double* __restrict__ det, int* __restrict__ info) {
+  __shared__ double2 s_row[32];
+  __shared__ int s_sig[64];
+  __shared__ double2 s_inv[32 * LS_LD];
+  const long g = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, l16 = lane & 15, kq = lane >> 4;
+  const double* Tg = T + g * sT;
+  double* Xg = X + g * sX;
+  const int nrb = (nso + 15) >> 4;
+  // A operands of one 16-row block, k-step t: T(16 rb + l16, 4 t + kq), loaded
+  // unconditionally (clamped address; zeroed where used)
+  auto load_t = [&](int rb, double2 (&tv)[8]) {
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int p = 16 * rb + l16, c = 4 * t + kq;
+      const bool v = p < nso && c < n;
+      tv[t] = *reinterpret_cast<const double2*>(Tg + (v ? 2L * (p + (long)c * nso) : 0));
+    }
+  };
+  if (w == 0)
+    lu_wave_inverse<32>(n, M + g * mbs, s_row, s_sig,
+                        [] {
+                          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                          __builtin_amdgcn_wave_barrier();
+                        },
+                        [&](int r, int c, double2 v) { s_inv[r * LS_LD + c] = v; }, det + 2 * g, info + g);
+  __syncthreads();
+  // B operands (the same for every row block): M^{-1}(4 t + kq, 16 j + l16)
+  double2 bq[8][2];
+#pragma unroll
+  for (int t = 0; t < 8; ++t)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int c = 4 * t + kq, b = 16 * j + l16;
+      const double2 q = s_inv[c * LS_LD + b];
+      bq[t][j] = (c < n && b < n) ? q : make_double2(0.0, 0.0);
+    }
+  // row blocks w, w + 2, ...: a block's T loads are issued together, then its products
+#pragma unroll 1
+  for (int rb = w; rb < nrb; rb += 2) {
+    double2 tv[8];
+    load_t(rb, tv);
+    d4_t p1[2], p2[2], p3[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) p1[j] = p2[j] = p3[j] = d4_t{0.0, 0.0, 0.0, 0.0};
+    const int p = 16 * rb + l16;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const bool va = p < nso && 4 * t + kq < n;
+      const double are = va ? tv[t].x : 0.0, aim = va ? tv[t].y : 0.0;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const double bre = bq[t][j].x, bim = bq[t][j].y;
+        p1[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bre, p1[j], 0, 0, 0);
+        p2[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, bim, p2[j], 0, 0, 0);
+        p3[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(are + aim, bre + bim, p3[j], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int pr = 16 * rb + kq + 4 * r, b = 16 * j + l16;
+        if (pr >= nso || b >= n) continue;
+        *reinterpret_cast<double2*>(Xg + 2L * (pr + (long)b * nso)) =
+            make_double2(p1[j][r] - p2[j][r], p3[j][r] - p1[j][r] - p2[j][r]);
+      }
   }
 }
 
@@ -368,7 +467,13 @@ __global__ __launch_bounds__(128) void lu_inverse_wave2_kernel(int n, const doub
 
 int lu_inverse_max_n() { return 96; }
 
-// M: column-major N x N complex matrices with batch stride mbs (doubles).
+// X_g = T_g M_g^{-1} with the inverse's det and info, for n <= 32 (T_g, X_g:
+// NSO x n column-major complex, batch strides sT, sX doubles).
+void lu_inverse_solve_batched(hipStream_t s, int n, int nso, const double* M, long mbs, const double* T, long sT,
+                              double* X, long sX, double* det, int* info, int batch) {
+  lu_inverse_solve_kernel<<<batch, 128, 0, s>>>(n, nso, M, mbs, T, sT, X, sX, det, info);
+}
+
 void lu_inverse_batched(hipStream_t s, int n, const double* M, long mbs, double* Minv, long ibs,
                         double* det, int* info, int batch) {
   if (n <= 16)

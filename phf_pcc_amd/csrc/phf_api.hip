@@ -1950,6 +1950,12 @@ static bool no_fused_epilogue() {
   const char* f = getenv("PHF_FUSED_EPILOGUE");
   return f && atoi(f) == 0;
 }
+// PHF_FUSED_SOLVE=0 keeps the overlap inverse and the solve X = T M^{-1} as
+// separate launches (the tests compare both).
+static bool no_fused_solve() {
+  const char* f = getenv("PHF_FUSED_SOLVE");
+  return f && atoi(f) == 0;
+}
 
 int eval_plan(phf_ctx* ctx, int nocc, int precision, int cmplxconj, EvalBufs& b) {
   if (!ctx->has_ham) return fail(ctx, PHF_ERR_NO_HAMILTONIAN, "phf_buildhsg: no Hamiltonian set");
@@ -2164,13 +2170,19 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
       zgemm_batched(s, N, N, nso, 1, 0, ZMat{b.C0, 2L * nso, 2, 0, 1, variant_right(v) ? 0 : 1},
                     zmat_colmajor(b.T + variant_left(v) * sT * G, nso, sT), 0, 0,
                     zout_colmajor(b.M + v * sNN * G, N, sNN), (int)G);
-  // K2: inverse and determinant of every variant
-  lu_inverse_batched(s, N, b.M, sNN, b.Minv, sNN, b.Det, b.Info, (int)(V * G));
-  // X_v = T_{L_v} M_v^{-1}, formed as X^T = M^{-T} T^T (stores along the NSO index)
-  for (int v = 0; v < V; ++v)
-    zgemm_batched(s, N, nso, N, 1, 0, zmat_colmajor_t(b.Minv + v * sNN * G, N, sNN),
-                  zmat_colmajor_t(b.T + variant_left(v) * sT * G, nso, sT), 0, 0,
-                  zout_colmajor_t(b.X + v * sT * G, nso, sT), (int)G);
+  // K2: inverse and determinant of every variant, and X_v = T_{L_v} M_v^{-1}:
+  // one kernel per point for one density per point and N <= 32 (the inverse
+  // stays in LDS); else the inverse, then X^T = M^{-T} T^T (stores along the
+  // NSO index)
+  if (V == 1 && N <= 32 && !no_fused_solve()) {
+    lu_inverse_solve_batched(s, N, nso, b.M, sNN, b.T, sT, b.X, sT, b.Det, b.Info, (int)G);
+  } else {
+    lu_inverse_batched(s, N, b.M, sNN, b.Minv, sNN, b.Det, b.Info, (int)(V * G));
+    for (int v = 0; v < V; ++v)
+      zgemm_batched(s, N, nso, N, 1, 0, zmat_colmajor_t(b.Minv + v * sNN * G, N, sNN),
+                    zmat_colmajor_t(b.T + variant_left(v) * sT * G, nso, sT), 0, 0,
+                    zout_colmajor_t(b.X + v * sT * G, nso, sT), (int)G);
+  }
   if (marks) (void)hipEventRecord(marks[1], s);
   // density pack: Pt row 2gv = Re rho_gv^T, row 2gv+1 = Im rho_gv^T (flattened
   // j*nso + l); rho^T = conj(B) X^T, one launch per right factor (variants
@@ -2960,10 +2972,11 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
   auto make_key = [&]() {
     char kb[512];
     auto env = [](const char* v) { const char* e = getenv(v); return e ? e : "-"; };
-    snprintf(kb, sizeof kb, "%d|%d|%d|%d|%d|%ld|%p|%p|%p|%p|%zu|%s|%s|%s|%s|%s|%s", precision, nocc, cmplxconj,
+    snprintf(kb, sizeof kb, "%d|%d|%d|%d|%d|%ld|%p|%p|%p|%p|%zu|%s|%s|%s|%s|%s|%s|%s", precision, nocc, cmplxconj,
              (int)resident, (int)(objective != nullptr), ctx->gen, (void*)ctx->ws, (void*)ctx->pin,
              (void*)ctx->small, (void*)ctx->d_xitems, ctx->ws_size, env("PHF_H16_CFG"), env("PHF_XGROUP"),
-             env("PHF_FUSED_SPLIT"), env("PHF_H16_FOLD"), env("PHF_H16_ALLOW_F32FOLD"), env("PHF_FUSED_EPILOGUE"));
+             env("PHF_FUSED_SPLIT"), env("PHF_H16_FOLD"), env("PHF_H16_ALLOW_F32FOLD"), env("PHF_FUSED_EPILOGUE"),
+             env("PHF_FUSED_SOLVE"));
     return std::string(kb);
   };
   std::string key = graph_ok ? make_key() : std::string();
