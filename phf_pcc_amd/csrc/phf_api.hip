@@ -884,6 +884,8 @@ int phf_ctx_init(phf_ctx* ctx, int device, hipStream_t shared_stream) {
     HIPCK(hipEventCreateWithFlags(&ctx->fork_ev, kSyncEventFlags));
     HIPCK(hipEventCreateWithFlags(&ctx->hc_fork, kSyncEventFlags));
     HIPCK(hipEventCreateWithFlags(&ctx->hc_join, kSyncEventFlags));
+    HIPCK(hipEventCreateWithFlags(&ctx->tq_fork, kSyncEventFlags));
+    HIPCK(hipEventCreateWithFlags(&ctx->tq_join, kSyncEventFlags));
   }
   return PHF_OK;
 }
@@ -939,7 +941,7 @@ extern "C" int phf_ctx_destroy(phf_ctx* ctx) {
     }
     if (ctx->join_ev[k]) (void)hipEventDestroy(ctx->join_ev[k]);
   }
-  for (hipEvent_t e : {ctx->fork_ev, ctx->hc_fork, ctx->hc_join})
+  for (hipEvent_t e : {ctx->fork_ev, ctx->hc_fork, ctx->hc_join, ctx->tq_fork, ctx->tq_join})
     if (e) (void)hipEventDestroy(e);
   if (ctx->owns_stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -2127,6 +2129,7 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
   }
   // K1: rotation chain, right to left (Ref^H R1 and Rpg Ref are cached with the
   // grid), once per left factor L = C0 (lc = 0) or C0* (lc = 1)
+  bool t_forked = false;
   for (int lc = 0; lc < nL; ++lc) {
     double* Wpg = b.Wpg + lc * sT * npg;
     double* W2 = b.W2 + lc * sT * npg * ny;
@@ -2137,7 +2140,16 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
       double* Q = W2;
       zgemm_batched(s, nso, N, nso, 1, 0, zmat_colmajor(ctx->dGP, nso, 2 * n2), zmat_colmajor(b.C0, nso, 0, lc), 0,
                     0, zout_colmajor(Q, nso, sT), 4 * npg);
-      spinrot_kernel<<<spinrot_grid(G, lenX), 256, 0, s>>>(G, npg, lenX, ctx->dGu, Q, T);
+      // T_g (needed by the solve) on a side stream, beside the overlaps below
+      if (ctx->side[1]) {
+        (void)hipEventRecord(ctx->tq_fork, s);
+        (void)hipStreamWaitEvent(ctx->side[1], ctx->tq_fork, 0);
+        spinrot_kernel<<<spinrot_grid(G, lenX), 256, 0, ctx->side[1]>>>(G, npg, lenX, ctx->dGu, Q, T);
+        (void)hipEventRecord(ctx->tq_join, ctx->side[1]);
+        t_forked = true;
+      } else {
+        spinrot_kernel<<<spinrot_grid(G, lenX), 256, 0, s>>>(G, npg, lenX, ctx->dGu, Q, T);
+      }
       // the overlaps of the variants with this left factor the same way:
       // M_g = B^H T_g = sum U_g[s][t] (B^H Q[s][t][ipg]), four N x N products
       // per point-group element (staged in the Q buffer of the epilogue)
@@ -2162,6 +2174,7 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
     zgemm_batched(s, nso, N * npg * ny, nso, 1, 0, zmat_colmajor(ctx->dGr1p, nso, 2 * n2), zmat_colmajor(W2, nso, 0),
                   0, 0, zout_colmajor(T, nso, sT * npg * ny), nl);
   }
+  if (t_forked) (void)hipStreamWaitEvent(s, ctx->tq_join, 0);
   if (marks) (void)hipEventRecord(marks[0], s);
   // M_v = B_v^H T_{L_v}   (B = C0: C0^H; B = C0*: C0^T); formed above from
   // the spin-rotation products when the grid has that structure

@@ -66,6 +66,9 @@ struct phf_ctx_s {
   hipEvent_t fork_ev = nullptr, join_ev[2] = {};
   // Hc = C0^H h runs on side[0] during the pre-stages (hc_fork, hc_join)
   hipEvent_t hc_fork = nullptr, hc_join = nullptr;
+  // spin-rotation grids: the T_g combine runs on side[1] beside the overlap
+  // products (tq_fork, tq_join)
+  hipEvent_t tq_fork = nullptr, tq_join = nullptr;
   std::string err;
   // cached Hamiltonian
   int nso = 0;
