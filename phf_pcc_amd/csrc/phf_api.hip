@@ -2137,9 +2137,14 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
     if (ctx->g_spinrot) {
       // Q[s][t][ipg] = Ref_s^H (Rpg[ipg] Ref L)_t = P[s][t][ipg] L in one batched
       // product, then T_g = sum U_g[s][t] Q[s][t][ipg]
+      // (these few products split their contraction over more workgroups; the
+      // epilogue's FX buffer is free scratch until then)
       double* Q = W2;
-      zgemm_batched(s, nso, N, nso, 1, 0, zmat_colmajor(ctx->dGP, nso, 2 * n2), zmat_colmajor(b.C0, nso, 0, lc), 0,
-                    0, zout_colmajor(Q, nso, sT), 4 * npg);
+      const long ws_cplx = lenX * V * G;
+      const int dq = (nso % 4 == 0 && nso >= 64 && 4 * npg <= 16) ? 4 : 1;
+      zgemm_batched_kslices(s, nso, N, nso, 1, 0, zmat_colmajor(ctx->dGP, nso, 2 * n2),
+                            zmat_colmajor(b.C0, nso, 0, lc), 0, 0, zout_colmajor(Q, nso, sT), 4 * npg, dq, b.FX,
+                            ws_cplx);
       // T_g (needed by the solve) on a side stream, beside the overlaps below
       if (ctx->side[1]) {
         (void)hipEventRecord(ctx->tq_fork, s);
@@ -2157,8 +2162,9 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
         for (int v = 0; v < V; ++v) {
           if (variant_left(v) != lc) continue;
           double* MQ = b.Q + v * 4L * npg * sNN;
-          zgemm_batched(s, N, N, nso, 1, 0, ZMat{b.C0, 2L * nso, 2, 0, 1, variant_right(v) ? 0 : 1},
-                        zmat_colmajor(Q, nso, sT), 0, 0, zout_colmajor(MQ, N, sNN), 4 * npg);
+          zgemm_batched_kslices(s, N, N, nso, 1, 0, ZMat{b.C0, 2L * nso, 2, 0, 1, variant_right(v) ? 0 : 1},
+                                zmat_colmajor(Q, nso, sT), 0, 0, zout_colmajor(MQ, N, sNN), 4 * npg, dq, b.FX,
+                                ws_cplx);
           spinrot_kernel<<<spinrot_grid(G, (long)N * N), 256, 0, s>>>(G, npg, (long)N * N, ctx->dGu, MQ,
                                                                       b.M + v * sNN * G);
         }

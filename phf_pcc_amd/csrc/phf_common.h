@@ -65,6 +65,12 @@ void zgemm_batched(hipStream_t s, int m, int n, int k, double ar, double ai, ZMa
 // and are summed in a fixed order; A and B must not use two-level batching.
 void zgemm_batched_splitk(hipStream_t s, int m, int n, int k, double ar, double ai, ZMat A, ZMat B, double br,
                           double bi, ZOut C, int batch, double* ws, long ws_cplx);
+// The same product with the contraction split into d slices (d divides k;
+// batch d m n <= ws_cplx complex elements of ws), for products too few to
+// fill the GPU: d times the workgroups, partials summed in a fixed order.  A
+// and B must not use two-level batching.
+void zgemm_batched_kslices(hipStream_t s, int m, int n, int k, double ar, double ai, ZMat A, ZMat B, double br,
+                           double bi, ZOut C, int batch, int d, double* ws, long ws_cplx);
 
 // One output tile (or one part of a split contraction) of a 16-bit ERI GEMM in
 // a grouped launch: rows mt*BM.. of P against rows nt*BN.. of A over the slabs

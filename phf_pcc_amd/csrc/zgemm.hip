@@ -204,7 +204,13 @@ void zgemm_batched_splitk(hipStream_t s, int m, int n, int k, double ar, double 
         d = q;
         break;
       }
-  if (d == 1) {
+  zgemm_batched_kslices(s, m, n, k, ar, ai, A, B, br, bi, C, batch, d, ws, ws_cplx);
+}
+
+void zgemm_batched_kslices(hipStream_t s, int m, int n, int k, double ar, double ai, ZMat A, ZMat B, double br,
+                           double bi, ZOut C, int batch, int d, double* ws, long ws_cplx) {
+  if (d <= 1 || !ws || k % d != 0 || A.bdiv != 1 || B.bdiv != 1 || (long)batch * d > 65535 ||
+      (long)batch * d * m * n > ws_cplx) {
     zgemm_batched(s, m, n, k, ar, ai, A, B, br, bi, C, batch);
     return;
   }
