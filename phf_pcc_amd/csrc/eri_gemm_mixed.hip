@@ -1250,6 +1250,13 @@ static const int I8_TILE_MAX_K = 33285;
 // the contraction (measured |dE| 3.6e-7 at K = 8192, n = 64; 2.8e-6 at
 // K = 32768, n = 128, where the FP64-fold tiles give 3.0e-7 and 1.3e-6).
 static const int RN16_TILE_MAX_K = 8192;
+// Largest NSO of the rn16 tile: the 16-bit products' truncation bias within
+// each 64-product partial grows with NSO^2 (F16X2: |dE| 4.0e-7 at n = 64,
+// NSO = 128; 1.34e-6 for an n = 80 instance of the 64-instance batch, whose
+// contraction K = 6400 is inside the length limit).
+static const int RN16_TILE_MAX_NSO = 128;
+bool h16_cfg_rn16(int c) { return kH16Cfgs[c].rn16; }
+int h16_rn16_max_nso() { return RN16_TILE_MAX_NSO; }
 int h16_num_configs() { return (int)(sizeof kH16Cfgs / sizeof kH16Cfgs[0]); }
 int h16_cfg_bm(int c) { return kH16Cfgs[c].bm; }
 int h16_cfg_bn(int c) { return kH16Cfgs[c].bn; }

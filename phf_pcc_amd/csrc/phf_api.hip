@@ -1863,6 +1863,7 @@ static int tune_h16(phf_ctx* ctx, const std::tuple<int, long, int, int>& key, lo
   for (int c = 0; c < h16_num_configs(); ++c) {
     if (forced >= 0 ? c != forced : !h16_cfg_fold64(c) && !allow_f32fold) continue;
     if (!h16_cfg_serves(c, std::get<0>(key), std::get<3>(key) / mixed_segments(std::get<0>(key)))) continue;
+    if (h16_cfg_rn16(c) && ctx->nso > h16_rn16_max_nso()) continue;
     if (nrpad % h16_cfg_bn(c) != 0) continue;
     for (int sk : {1, 2})
       if (!h16_forced_splitk() || sk == h16_forced_splitk()) cands.push_back({c, sk});
@@ -2519,7 +2520,8 @@ static int rjk_gemm(phf_ctx* ctx, const EvalBufs& b) {
   int big = -1;
   if (h16_forced_config() < 0)
     for (int c = 0; c < h16_num_configs(); ++c)
-      if (h16_cfg_bm(c) == 256 && h16_cfg_bn(c) == 256 && h16_cfg_serves(c, prec, (int)kj) && kj % 256 == 0)
+      if (h16_cfg_bm(c) == 256 && h16_cfg_bn(c) == 256 && h16_cfg_serves(c, prec, (int)kj) && kj % 256 == 0 &&
+          !(h16_cfg_rn16(c) && ctx->nso > h16_rn16_max_nso()))
         big = c;
   auto sel = [&](int g) { return g == 2 ? std::make_pair(big, 1) : choice[1]; };
   auto upload = [&](int cfg, int sk) -> int {
@@ -2720,7 +2722,7 @@ int eval_gemm(phf_ctx* ctx, const EvalBufs& b) {
     if (can_group && h16_forced_config() < 0)
       for (int c = 0; c < h16_num_configs(); ++c)
         if (h16_cfg_bm(c) == 256 && h16_cfg_bn(c) == 256 && h16_cfg_serves(c, precision, k_of(1)) &&
-            rows_of(1) % 256 == 0)
+            rows_of(1) % 256 == 0 && !(h16_cfg_rn16(c) && ctx->nso > h16_rn16_max_nso()))
           big = c;
     auto sel = [&](int g) { return g == 2 ? std::make_pair(big, 1) : choice[1]; };
     // the work list is rebuilt per call (a few hundred items) and uploaded only when it differs

@@ -173,7 +173,7 @@ int tune_grouped(phf_batch* bt, int prec) {
     if (forced >= 0 ? c != forced : !h16_cfg_fold64(c)) continue;
     bool fits = true;   // A rows (dense or per block) are multiples of 256
     for (phf_ctx* x : bt->inst) {
-      fits = fits && (x->nrpad % h16_cfg_bn(c) == 0);
+      fits = fits && (x->nrpad % h16_cfg_bn(c) == 0) && !(h16_cfg_rn16(c) && x->nso > h16_rn16_max_nso());
       for (int q = 0; q < 3; ++q)   // every contraction length of the instance (dense, each block, spin-free)
         fits = fits && h16_cfg_serves(c, prec, phf_spinfree(x) ? x->rj_kj
                                                                : (x->blocked && x->use_blocks) ? x->blk_k[q] : x->kt);

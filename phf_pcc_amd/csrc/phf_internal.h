@@ -49,6 +49,10 @@ int h16_num_configs();
 int h16_cfg_bm(int c);
 int h16_cfg_bn(int c);
 bool h16_cfg_fold64(int c);
+// whether config c is the round-to-nearest 16-bit tile, and the largest NSO it
+// serves (its accuracy limit; the contraction limit is in h16_cfg_serves)
+bool h16_cfg_rn16(int c);
+int h16_rn16_max_nso();
 bool h16_cfg_serves(int c, int prec, int K);
 int h16_slabs(int prec, int c, int K);
 int h16_forced_config();
