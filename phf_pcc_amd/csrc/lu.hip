@@ -158,6 +158,43 @@ __global__ __launch_bounds__(256) void lu_inverse_kernel(int n, const double* __
 // where one wave runs it, a wavefront-scope fence).  put(r, c, v) receives
 // M^{-1}(r, c) for r, c < n (zero when M is singular), once each; the
 // determinant and info go to *det_out (complex) and *info_out from lane 0.
+// One step of the pivot search's wave reduction: each lane compares its
+// candidate (c, key) with the lane CTRL names (a DPP row shift or row
+// broadcast; lanes without a source, or in rows outside ROWMASK, compare with
+// themselves) and keeps the larger c, ties to the smaller key.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ void pivot_step(double& c, int& key) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, c);
+  const int lo = (int)(unsigned)u, hi = (int)(unsigned)(u >> 32);
+  const int olo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWMASK, 0xf, false);
+  const int ohi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWMASK, 0xf, false);
+  const int ok = __builtin_amdgcn_update_dpp(key, key, CTRL, ROWMASK, 0xf, false);
+  const double oc = __builtin_bit_cast(double, ((unsigned long long)(unsigned)ohi << 32) | (unsigned)olo);
+  if (oc > c || (oc == c && ok < key)) {
+    c = oc;
+    key = ok;
+  }
+}
+
+// The best (c, key) over the wave in every lane: row shifts by 1, 2, 4, 8
+// leave each 16-lane row's best in its lane 15, two row broadcasts carry
+// them to lane 63, whose pair every lane then reads.  The choice (largest c,
+// ties to the smallest key; keys are distinct) does not depend on the order
+// of the comparisons, so it equals the butterfly's.
+__device__ __forceinline__ void wave_best_pivot(double& c, int& key) {
+  pivot_step<0x111, 0xf>(c, key);   // row_shr:1
+  pivot_step<0x112, 0xf>(c, key);   // row_shr:2
+  pivot_step<0x114, 0xf>(c, key);   // row_shr:4
+  pivot_step<0x118, 0xf>(c, key);   // row_shr:8
+  pivot_step<0x142, 0xa>(c, key);   // row_bcast:15 into rows 1 and 3
+  pivot_step<0x143, 0xc>(c, key);   // row_bcast:31 into rows 2 and 3
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, c);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), 63);
+  c = __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+  key = __builtin_amdgcn_readlane(key, 63);
+}
+
 template <int NMAX, typename Sync, typename Put>
 __device__ __forceinline__ void lu_wave_inverse(int n, const double* __restrict__ Mg, double2* s_row, int* s_sig,
                                                 Sync sync, Put put, double* det_out, int* info_out) {
@@ -182,12 +219,7 @@ __device__ __forceinline__ void lu_wave_inverse(int n, const double* __restrict_
     c = (done || part > 0) ? -1.0 : fabs(a[0].x) + fabs(a[0].y);
     if (c != c) c = -1.0;   // a NaN row is never a pivot; keeps the reduction wave-uniform
     key = (pos << 6) | lane;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const double oc = __shfl_xor(c, off);
-      const int ok = __shfl_xor(key, off);
-      if (oc > c || (oc == c && ok < key)) { c = oc; key = ok; }
-    }
+    wave_best_pivot(c, key);
   };
   double c;
   int key;
@@ -204,7 +236,24 @@ __device__ __forceinline__ void lu_wave_inverse(int n, const double* __restrict_
     if (is_p) { pos = k; done = true; }
     // column 0 of this row (part 0's register 0) and the next part's first column
     double2 f = a[0], nxt = a[0];
-    if constexpr (LPR > 1) {
+    if constexpr (LPR == 2) {
+      // f: part 1 takes part 0's column 0 (lane - 32), nxt: part 0 takes part
+      // 1's (lane + 32): one swap of the wave's halves per 32-bit word
+      const unsigned long long ux = __builtin_bit_cast(unsigned long long, a[0].x);
+      const unsigned long long uy = __builtin_bit_cast(unsigned long long, a[0].y);
+      unsigned w[4] = {(unsigned)ux, (unsigned)(ux >> 32), (unsigned)uy, (unsigned)(uy >> 32)};
+      unsigned fw[4], nw[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const auto r = __builtin_amdgcn_permlane32_swap(w[q], w[q], false, false);
+        fw[q] = r[0];   // lanes 32-63: lane - 32's word; lanes 0-31: their own
+        nw[q] = r[1];   // lanes 0-31: lane + 32's word; lanes 32-63: their own
+      }
+      f = make_double2(__builtin_bit_cast(double, ((unsigned long long)fw[1] << 32) | fw[0]),
+                       __builtin_bit_cast(double, ((unsigned long long)fw[3] << 32) | fw[2]));
+      nxt = make_double2(__builtin_bit_cast(double, ((unsigned long long)nw[1] << 32) | nw[0]),
+                         __builtin_bit_cast(double, ((unsigned long long)nw[3] << 32) | nw[2]));
+    } else if constexpr (LPR > 2) {
       f = make_double2(__shfl(a[0].x, row), __shfl(a[0].y, row));
       const int src = (part < LPR - 1) ? lane + NMAX : lane;
       nxt = make_double2(__shfl(a[0].x, src), __shfl(a[0].y, src));
