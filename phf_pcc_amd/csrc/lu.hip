@@ -195,9 +195,10 @@ __device__ __forceinline__ void wave_best_pivot(double& c, int& key) {
   key = __builtin_amdgcn_readlane(key, 63);
 }
 
-template <int NMAX, typename Sync, typename Put>
-__device__ __forceinline__ void lu_wave_inverse(int n, const double* __restrict__ Mg, double2* s_row, int* s_sig,
-                                                Sync sync, Put put, double* det_out, int* info_out) {
+// get(i, j) returns M(i, j) (complex) for i, j < n.
+template <int NMAX, typename Get, typename Sync, typename Put>
+__device__ __forceinline__ void lu_wave_inverse(int n, Get get, double2* s_row, int* s_sig, Sync sync, Put put,
+                                                double* det_out, int* info_out) {
   constexpr int LPR = 64 / NMAX, CPL = NMAX / LPR;
   const int lane = threadIdx.x & 63, row = lane % NMAX, part = lane / NMAX;
   const bool real_row = row < n;
@@ -206,10 +207,7 @@ __device__ __forceinline__ void lu_wave_inverse(int n, const double* __restrict_
   for (int t = 0; t < CPL; ++t) {
     const int j = part * CPL + t;
     a[t] = make_double2(0.0, 0.0);
-    if (real_row && j < n) {
-      const long off = 2 * ((long)j * n + row);   // global is column-major
-      a[t] = make_double2(Mg[off], Mg[off + 1]);
-    }
+    if (real_row && j < n) a[t] = get(row, j);
   }
   int pos = row;
   bool done = !real_row;   // row already used as a pivot, or no row of M
@@ -308,7 +306,12 @@ __global__ __launch_bounds__(64) void lu_inverse_wave_kernel(int n, const double
   const long g = blockIdx.x;
   double* Ig = Minv + g * ibs;
   // M^{-1} column-major
-  lu_wave_inverse<NMAX>(n, M + g * mbs, s_row, s_sig, [] { __syncthreads(); },
+  const double* Mg = M + g * mbs;   // column-major
+  lu_wave_inverse<NMAX>(n, [&](int i, int j) {
+                          const long off = 2 * ((long)j * n + i);
+                          return make_double2(Mg[off], Mg[off + 1]);
+                        },
+                        s_row, s_sig, [] { __syncthreads(); },
                         [&](int r, int c, double2 v) {
                           const long off = 2 * ((long)c * n + r);
                           Ig[off] = v.x;
@@ -331,7 +334,8 @@ constexpr int LS_LD = 33;   // row stride (complex) of M^{-1} in LDS
 __global__ __launch_bounds__(128, 2) void lu_inverse_solve_kernel(int n, int nso, const double* __restrict__ M, long mbs,
                                                                const double* __restrict__ T, long sT,
                                                                double* __restrict__ X, long sX,
-                                                               double* __restrict__ det, int* __restrict__ info) {
+                                                               double* __restrict__ det, int* __restrict__ info,
+                                                               const double* __restrict__ U, int npg) {
   __shared__ double2 s_row[32];
   __shared__ int s_sig[64];
   __shared__ double2 s_inv[32 * LS_LD];
@@ -350,8 +354,26 @@ __global__ __launch_bounds__(128, 2) void lu_inverse_solve_kernel(int n, int nso
       tv[t] = *reinterpret_cast<const double2*>(Tg + (v ? 2L * (p + (long)c * nso) : 0));
     }
   };
+  // M_g itself, or on spin-rotation grids (U != nullptr) the combination
+  // sum_{s,t} U_g[s][t] M_st[g % npg] of the four N x N products M_st (at
+  // (st npg + ipg) mbs), in the order of spinrot_kernel
+  auto get = [&](int i, int j) {
+    const long off = 2 * ((long)j * n + i);
+    if (!U) return make_double2(M[g * mbs + off], M[g * mbs + off + 1]);
+    const double* u = U + 8 * g;
+    const long ipg = g % npg;
+    double re = 0.0, im = 0.0;
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      const double ur = u[2 * st], ui = u[2 * st + 1];
+      const double* q = M + (st * npg + ipg) * mbs + off;
+      re += ur * q[0] - ui * q[1];
+      im += ur * q[1] + ui * q[0];
+    }
+    return make_double2(re, im);
+  };
   if (w == 0)
-    lu_wave_inverse<32>(n, M + g * mbs, s_row, s_sig,
+    lu_wave_inverse<32>(n, get, s_row, s_sig,
                         [] {
                           __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                           __builtin_amdgcn_wave_barrier();
@@ -517,10 +539,11 @@ __global__ __launch_bounds__(128) void lu_inverse_wave2_kernel(int n, const doub
 int lu_inverse_max_n() { return 96; }
 
 // X_g = T_g M_g^{-1} with the inverse's det and info, for n <= 32 (T_g, X_g:
-// NSO x n column-major complex, batch strides sT, sX doubles).
+// NSO x n column-major complex, batch strides sT, sX doubles).  With U, M
+// holds the 4 npg spin-rotation products and M_g is their combination.
 void lu_inverse_solve_batched(hipStream_t s, int n, int nso, const double* M, long mbs, const double* T, long sT,
-                              double* X, long sX, double* det, int* info, int batch) {
-  lu_inverse_solve_kernel<<<batch, 128, 0, s>>>(n, nso, M, mbs, T, sT, X, sX, det, info);
+                              double* X, long sX, double* det, int* info, int batch, const double* U, int npg) {
+  lu_inverse_solve_kernel<<<batch, 128, 0, s>>>(n, nso, M, mbs, T, sT, X, sX, det, info, U, npg > 0 ? npg : 1);
 }
 
 void lu_inverse_batched(hipStream_t s, int n, const double* M, long mbs, double* Minv, long ibs,

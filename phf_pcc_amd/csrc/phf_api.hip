@@ -2120,6 +2120,8 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
   // combine; the products are staged in b.Q (G N^2 per variant, free until
   // the epilogue), which holds them when G >= 4 npg
   const bool m_from_spinrot = ctx->g_spinrot && G >= 4L * npg;
+  // one density per point, N <= 32: inverse and solve in one kernel per point
+  const bool fused_solve = V == 1 && N <= 32 && !no_fused_solve();
   // Hc = B^H h per right factor (used by the energy in eval_post): on a side
   // stream, overlapping the pre-stages, when the context has one
   if (ctx->side[0]) {
@@ -2166,8 +2168,9 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
           zgemm_batched_kslices(s, N, N, nso, 1, 0, ZMat{b.C0, 2L * nso, 2, 0, 1, variant_right(v) ? 0 : 1},
                                 zmat_colmajor(Q, nso, sT), 0, 0, zout_colmajor(MQ, N, sNN), 4 * npg, dq, b.FX,
                                 ws_cplx);
-          spinrot_kernel<<<spinrot_grid(G, (long)N * N), 256, 0, s>>>(G, npg, (long)N * N, ctx->dGu, MQ,
-                                                                      b.M + v * sNN * G);
+          if (!fused_solve)   // else the fused inverse forms M_g from MQ as it loads it
+            spinrot_kernel<<<spinrot_grid(G, (long)N * N), 256, 0, s>>>(G, npg, (long)N * N, ctx->dGu, MQ,
+                                                                        b.M + v * sNN * G);
         }
       continue;
     }
@@ -2194,8 +2197,11 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
   // one kernel per point for one density per point and N <= 32 (the inverse
   // stays in LDS); else the inverse, then X^T = M^{-T} T^T (stores along the
   // NSO index)
-  if (V == 1 && N <= 32 && !no_fused_solve()) {
-    lu_inverse_solve_batched(s, N, nso, b.M, sNN, b.T, sT, b.X, sT, b.Det, b.Info, (int)G);
+  if (fused_solve) {
+    if (m_from_spinrot)   // M_g = sum U_g[s][t] M_st, formed by the kernel from the products in b.Q
+      lu_inverse_solve_batched(s, N, nso, b.Q, sNN, b.T, sT, b.X, sT, b.Det, b.Info, (int)G, ctx->dGu, npg);
+    else
+      lu_inverse_solve_batched(s, N, nso, b.M, sNN, b.T, sT, b.X, sT, b.Det, b.Info, (int)G);
   } else {
     lu_inverse_batched(s, N, b.M, sNN, b.Minv, sNN, b.Det, b.Info, (int)(V * G));
     for (int v = 0; v < V; ++v)

@@ -12,7 +12,8 @@
 #include "../../include/phf_hip.h"
 
 void lu_inverse_solve_batched(hipStream_t s, int n, int nso, const double* M, long mbs, const double* T, long sT,
-                              double* X, long sX, double* det, int* info, int batch);
+                              double* X, long sX, double* det, int* info, int batch, const double* U = nullptr,
+                              int npg = 1);
 void lu_inverse_batched(hipStream_t s, int n, const double* M, long mbs, double* Minv, long ibs,
                         double* det, int* info, int batch);
 int lu_inverse_max_n();
