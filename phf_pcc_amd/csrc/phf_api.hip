@@ -456,9 +456,20 @@ __global__ void wreduce_partial_kernel(int G, int gchunk, int len, int nmn, cons
 
 // out_z[mn][e] = sum_c part_z[c][mn][e] in chunk order, for both sources
 // (blockIdx.y = z)
-__global__ void wreduce_final_kernel(int nchunk, int len, int nmn, const double* __restrict__ part0,
-                                     const double* __restrict__ part1, double* __restrict__ out0,
-                                     double* __restrict__ out1) {
+__device__ void hs_reduce_block(int G, int nmn, const double* Wd, const double* E, double* HS);
+
+// blockIdx.y = 0, 1: out[e] = sum_c part[c][e] for the two partial blocks;
+// blockIdx.y = 2: workgroup 0 forms Hmat/Smat (hs_reduce_block), the rest exit.
+__global__ __launch_bounds__(256) void wreduce_final_kernel(int nchunk, int len, int nmn,
+                                                            const double* __restrict__ part0,
+                                                            const double* __restrict__ part1,
+                                                            double* __restrict__ out0, double* __restrict__ out1,
+                                                            int Ghs, const double* __restrict__ Wd,
+                                                            const double* __restrict__ E, double* __restrict__ HS) {
+  if (blockIdx.y == 2) {
+    if (blockIdx.x == 0) hs_reduce_block(Ghs, nmn, Wd, E, HS);
+    return;
+  }
   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   const double* part = blockIdx.y ? part1 : part0;
   double* out = blockIdx.y ? out1 : out0;
@@ -474,8 +485,7 @@ __global__ void wreduce_final_kernel(int nchunk, int len, int nmn, const double*
 }
 
 // HS[mn] = sum_g Wd[g][mn] E[g],  HS[nmn + mn] = sum_g Wd[g][mn]; one workgroup.
-__global__ __launch_bounds__(256) void hs_reduce_kernel(int G, int nmn, const double* Wd, const double* E,
-                                                        double* HS) {
+__device__ void hs_reduce_block(int G, int nmn, const double* Wd, const double* E, double* HS) {
   __shared__ double s[4][256];
   for (int mn = 0; mn < nmn; ++mn) {
     double hr = 0, hi = 0, sr = 0, si = 0;
@@ -886,6 +896,7 @@ int phf_ctx_init(phf_ctx* ctx, int device, hipStream_t shared_stream) {
     HIPCK(hipEventCreateWithFlags(&ctx->hc_join, kSyncEventFlags));
     HIPCK(hipEventCreateWithFlags(&ctx->tq_fork, kSyncEventFlags));
     HIPCK(hipEventCreateWithFlags(&ctx->tq_join, kSyncEventFlags));
+    HIPCK(hipEventCreateWithFlags(&ctx->wd_fork, kSyncEventFlags));
   }
   return PHF_OK;
 }
@@ -941,7 +952,7 @@ extern "C" int phf_ctx_destroy(phf_ctx* ctx) {
     }
     if (ctx->join_ev[k]) (void)hipEventDestroy(ctx->join_ev[k]);
   }
-  for (hipEvent_t e : {ctx->fork_ev, ctx->hc_fork, ctx->hc_join, ctx->tq_fork, ctx->tq_join})
+  for (hipEvent_t e : {ctx->fork_ev, ctx->hc_fork, ctx->hc_join, ctx->tq_fork, ctx->tq_join, ctx->wd_fork})
     if (e) (void)hipEventDestroy(e);
   if (ctx->owns_stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -2128,7 +2139,7 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
     (void)hipEventRecord(ctx->hc_fork, s);
     (void)hipStreamWaitEvent(ctx->side[0], ctx->hc_fork, 0);
     hc_products(ctx, b, ctx->side[0]);
-    (void)hipEventRecord(ctx->hc_join, ctx->side[0]);
+    (void)hipEventRecord(ctx->hc_join, ctx->side[0]);   // recorded again after the weights below
   }
   // K1: rotation chain, right to left (Ref^H R1 and Rpg Ref are cached with the
   // grid), once per left factor L = C0 (lc = 0) or C0* (lc = 1)
@@ -2208,6 +2219,18 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
       zgemm_batched(s, N, nso, N, 1, 0, zmat_colmajor_t(b.Minv + v * sNN * G, N, sNN),
                     zmat_colmajor_t(b.T + variant_left(v) * sT * G, nso, sT), 0, 0,
                     zout_colmajor_t(b.X + v * sT * G, nso, sT), (int)G);
+  }
+  // the determinant weights of the grid sums on the side stream (they need
+  // only Det), joined through hc_join before the epilogue
+  ctx->wd_forked = false;
+  if (ctx->side[0]) {
+    const int nci = ctx->g_nci, nmn = nci * nci, d = (V == 4) ? nci / 2 : nci;
+    (void)hipEventRecord(ctx->wd_fork, s);
+    (void)hipStreamWaitEvent(ctx->side[0], ctx->wd_fork, 0);
+    wd_kernel<<<(unsigned)((V * G * nmn + 255) / 256), 256, 0, ctx->side[0]>>>((int)G, V, nci, d, ctx->dGw, b.Det,
+                                                                              b.Wd);
+    (void)hipEventRecord(ctx->hc_join, ctx->side[0]);
+    ctx->wd_forked = true;
   }
   if (marks) (void)hipEventRecord(marks[1], s);
   // density pack: Pt row 2gv = Re rho_gv^T, row 2gv+1 = Im rho_gv^T (flattened
@@ -2877,16 +2900,20 @@ void eval_post(phf_ctx* ctx, const EvalBufs& b, bool resident) {
   // K5: weighted grid sums; every variant lands in its own CI block, so one
   // reduction over all virtual points assembles the nci x nci matrices
   const int d = (V == 4) ? nci / 2 : nci;
-  wd_kernel<<<(unsigned)((Gv * nmn + 255) / 256), 256, 0, s>>>((int)G, V, nci, d, ctx->dGw, b.Det, b.Wd);
-  // Hmat/Smat from variants 0 and 1 only (blocks (0,0), (0,1); PHFTools.f90:153-179)
-  hs_reduce_kernel<<<1, 256, 0, s>>>((int)((V == 4) ? 2 * G : G), nmn, b.Wd, b.E, b.HS);
+  if (ctx->wd_forked)
+    ctx->wd_forked = false;   // done on the side stream, joined above (hc_join)
+  else
+    wd_kernel<<<(unsigned)((Gv * nmn + 255) / 256), 256, 0, s>>>((int)G, V, nci, d, ctx->dGw, b.Det, b.Wd);
   {
-    // Ysum and Xsum in one pair of launches (two partial blocks)
+    // Ysum and Xsum in one pair of launches (two partial blocks); the final
+    // launch also forms Hmat/Smat from variants 0 and 1 only (blocks (0,0),
+    // (0,1); PHFTools.f90:153-179) in its third row of workgroups
     double* part1 = b.Part + 2L * b.nchunk * nmn * lenX;
     dim3 grid((unsigned)((lenX + 255) / 256), (unsigned)b.nchunk, 2);
-    dim3 fin((unsigned)((lenX * nmn + 255) / 256), 2);
+    dim3 fin((unsigned)((lenX * nmn + 255) / 256), 3);
     wreduce_partial_kernel<<<grid, 256, 0, s>>>((int)Gv, b.gchunk, (int)lenX, nmn, b.Wd, Y, b.X, b.Part, part1);
-    wreduce_final_kernel<<<fin, 256, 0, s>>>(b.nchunk, (int)lenX, nmn, b.Part, part1, b.Ysum, b.Xsum);
+    wreduce_final_kernel<<<fin, 256, 0, s>>>(b.nchunk, (int)lenX, nmn, b.Part, part1, b.Ysum, b.Xsum,
+                                            (int)((V == 4) ? 2 * G : G), b.Wd, b.E, b.HS);
   }
   // Gradmat[mn] = Ysum[mn] B^H, Rdmmat[mn] = Xsum[mn] B^H (C-order [nso][nso]);
   // CI block rows m < d come from variants with B = C0, rows m >= d (V = 4)

@@ -74,6 +74,10 @@ struct phf_ctx_s {
   // spin-rotation grids: the T_g combine runs on side[1] beside the overlap
   // products (tq_fork, tq_join)
   hipEvent_t tq_fork = nullptr, tq_join = nullptr;
+  // the determinant weights (wd_kernel) run on side[0] once the inverses are
+  // done (wd_fork); hc_join then also covers them
+  hipEvent_t wd_fork = nullptr;
+  bool wd_forked = false;
   std::string err;
   // cached Hamiltonian
   int nso = 0;
