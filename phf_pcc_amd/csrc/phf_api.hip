@@ -636,25 +636,35 @@ __global__ void localg_vo_kernel(int nso, int nocc, int nterm, const int* __rest
 // orthogonalisation gives E0 = Re H / |Re S| and fsp = |Re S|^{-1/2}, so the
 // gradient block is (Ysum - E0 Xsum)[N:, :] f / (f S) with f = 1 / |Re S|.
 // out[0..1] = E0, out[2..] = the (NSO - N) x N block, C order.
+// out, hs_out and info_out are host memory (coherent, mapped): the kernel
+// also copies Hmat/Smat (nhs doubles) and the inverses' info flags there, so
+// the call needs no copies back.
 __global__ void objective_kernel(int nso, int nocc, const double* __restrict__ hs, const double* __restrict__ ysum,
-                                 const double* __restrict__ xsum, double* __restrict__ out) {
+                                 const double* __restrict__ xsum, double* __restrict__ out, int nhs,
+                                 const int* __restrict__ info, long ninfo, double* __restrict__ hs_out,
+                                 int* __restrict__ info_out) {
   const int nv = nso - nocc;
   const double as = fabs(hs[2]);
   const double e0 = hs[0] / as, f = 1.0 / as;
   const double ovr = hs[2] * f, ovi = hs[3] * f;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const long nth = (long)gridDim.x * blockDim.x;
+  for (long i = e; i < nhs; i += nth) hs_out[i] = hs[i];
+  for (long i = e; i < ninfo; i += nth) info_out[i] = info[i];
   if (e == 0) {
     out[0] = e0;
     out[1] = 0.0;
   }
-  if (e >= nv * nocc) return;
-  const int pv = e / nocc, a = e % nocc;
-  const long o = (long)(nocc + pv) + (long)a * nso;
-  const double gr = f * ysum[2 * o], gi = f * ysum[2 * o + 1], rr = f * xsum[2 * o], ri = f * xsum[2 * o + 1];
-  const double nr = gr - e0 * rr, ni = gi - e0 * ri;
-  const double d = ovr * ovr + ovi * ovi;
-  out[2 + 2 * e] = (nr * ovr + ni * ovi) / d;
-  out[2 + 2 * e + 1] = (ni * ovr - nr * ovi) / d;
+  if (e < nv * nocc) {
+    const int pv = e / nocc, a = e % nocc;
+    const long o = (long)(nocc + pv) + (long)a * nso;
+    const double gr = f * ysum[2 * o], gi = f * ysum[2 * o + 1], rr = f * xsum[2 * o], ri = f * xsum[2 * o + 1];
+    const double nr = gr - e0 * rr, ni = gi - e0 * ri;
+    const double d = ovr * ovr + ovi * ovi;
+    out[2 + 2 * e] = (nr * ovr + ni * ovi) / d;
+    out[2 + 2 * e + 1] = (ni * ovr - nr * ovi) / d;
+  }
+  __threadfence_system();
 }
 
 // ---------------------------------------------------------------------------
@@ -1970,6 +1980,12 @@ static bool no_fused_solve() {
   const char* f = getenv("PHF_FUSED_SOLVE");
   return f && atoi(f) == 0;
 }
+// PHF_SPLIT_RJK=0 keeps the direct grouped GEMM of the spin-free form as one
+// launch followed by the unpack (the tests compare both).
+static bool no_split_rjk() {
+  const char* f = getenv("PHF_SPLIT_RJK");
+  return f && atoi(f) == 0;
+}
 
 int eval_plan(phf_ctx* ctx, int nocc, int precision, int cmplxconj, EvalBufs& b) {
   if (!ctx->has_ham) return fail(ctx, PHF_ERR_NO_HAMILTONIAN, "phf_buildhsg: no Hamiltonian set");
@@ -2459,6 +2475,13 @@ void build_h16_items(const std::vector<std::pair<const phf_ctx*, const EvalBufs*
           logical.push_back(it);
         }
   }
+  h16_xcd_order(logical, grid, out);
+}
+
+// Work-list order of a persistent launch of `grid` workgroups: workgroup b
+// runs on XCD b % 8 and takes items b, b + grid, ...; each XCD gets a
+// contiguous run of the logical items in every round.
+void h16_xcd_order(const std::vector<H16Item>& logical, int grid, std::vector<H16Item>& out) {
   const size_t n = logical.size();
   out.resize(n);
   const int per = grid / 8;
@@ -2553,9 +2576,7 @@ static int rjk_gemm(phf_ctx* ctx, const EvalBufs& b) {
           !(h16_cfg_rn16(c) && ctx->nso > h16_rn16_max_nso()))
         big = c;
   auto sel = [&](int g) { return g == 2 ? std::make_pair(big, 1) : choice[1]; };
-  auto upload = [&](int cfg, int sk) -> int {
-    std::vector<H16Item> items;
-    build_h16_items({{ctx, &b}}, prec, cfg, sk, h16_grouped_grid(cfg, ctx->ncu), items, 3u);
+  auto upload_items = [&](std::vector<H16Item>& items) -> int {
     if (items.size() != ctx->h_xitems.size() ||
         std::memcmp(items.data(), ctx->h_xitems.data(), items.size() * sizeof(H16Item)) != 0) {
       if (items.size() > ctx->xitems_cap) {
@@ -2568,6 +2589,11 @@ static int rjk_gemm(phf_ctx* ctx, const EvalBufs& b) {
       ctx->h_xitems.swap(items);
     }
     return PHF_OK;
+  };
+  auto upload = [&](int cfg, int sk) -> int {
+    std::vector<H16Item> items;
+    build_h16_items({{ctx, &b}}, prec, cfg, sk, h16_grouped_grid(cfg, ctx->ncu), items, 3u);
+    return upload_items(items);
   };
   auto run_grouped = [&](int cfg, int sk) {
     if (sk > 1) fill_zero(s, b.Cjk, (Mp + b.Mk) * kj);
@@ -2631,6 +2657,45 @@ static int rjk_gemm(phf_ctx* ctx, const EvalBufs& b) {
   }
   int grouped = git->second;
   if (grouped == 2 && big < 0) grouped = 1;
+  auto cs_split = [&](std::pair<int, int> cs) {
+    return cs.second == 1 && ctx->side[1] && rjk_direct_items({{ctx, &b}}, 1) && !no_split_rjk();
+  };
+  if (grouped && cs_split(sel(grouped))) {
+    // Two launches of the direct grouped GEMM: the tiles of the Gamma_aa /
+    // Gamma_bb product sets on s, the exchange tiles (which write Gamma_ab and
+    // Gamma_ba themselves) on side[1].  The unpack follows the first launch on
+    // s and runs on the CUs the exchange tiles leave idle in their last round.
+    const int cfg = sel(grouped).first, grid = h16_grouped_grid(cfg, ctx->ncu);
+    std::vector<H16Item> logical, part[2], ordered, items;
+    build_h16_items({{ctx, &b}}, prec, cfg, 1, 1, logical, 3u);   // one workgroup: logical order
+    for (const H16Item& it : logical) part[it.epi ? 1 : 0].push_back(it);
+    int gq[2];
+    for (int q = 0; q < 2; ++q) {
+      gq[q] = (int)std::min<long>(grid, round_up((long)part[q].size(), 8));
+      h16_xcd_order(part[q], gq[q], ordered);
+      items.insert(items.end(), ordered.begin(), ordered.end());
+    }
+    int rc = upload_items(items);
+    if (rc) return rc;
+    const int na = (int)part[0].size(), nx = (int)part[1].size();
+    mark(0, 0);
+    (void)hipEventRecord(ctx->fork_ev, s);
+    (void)hipStreamWaitEvent(ctx->side[1], ctx->fork_ev, 0);
+    eri_gemm_mixed_grouped(s, prec, ctx->d_xitems, na, cfg, gq[0], false);
+    eri_gemm_mixed_grouped(ctx->side[1], prec, ctx->d_xitems + na, nx, cfg, gq[1], false);
+    if (ctx->profiling && ctx->bev[0][0]) {   // the GEMM ends with the exchange launch
+      (void)hipEventRecord(ctx->bev[0][1], ctx->side[1]);
+      ctx->nblk_timed = std::max(ctx->nblk_timed, 1);
+    }
+    (void)hipEventRecord(ctx->join_ev[1], ctx->side[1]);
+    HIPCK(hipGetLastError());
+    mark(1, 0);
+    mark(1, 1);
+    ctx->rjk_grouped_last = true;
+    rc = unpack(false);
+    (void)hipStreamWaitEvent(s, ctx->join_ev[1], 0);
+    return rc;
+  }
   if (grouped) {
     const auto cs = sel(grouped);
     int rc = upload(cs.first, cs.second);
@@ -2973,8 +3038,11 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
     if (ctx->pin) HIPCK(hipHostFree(ctx->pin));
     ctx->pin = nullptr;
     ctx->pin_size = 0;
-    HIPCK(hipHostMalloc((void**)&ctx->pin, pin_need, hipHostMallocDefault));
+    HIPCK(hipHostMalloc((void**)&ctx->pin, pin_need, hipHostMallocCoherent | hipHostMallocMapped));
     ctx->pin_size = pin_need;
+    void* dp = nullptr;
+    HIPCK(hipHostGetDevicePointer(&dp, ctx->pin, 0));
+    ctx->pin_dev = static_cast<char*>(dp);
   }
   double* c0 = reinterpret_cast<double*>(ctx->pin);
   double* hs = reinterpret_cast<double*>(ctx->pin + c0_bytes);
@@ -2982,14 +3050,6 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
   int* info = reinterpret_cast<int*>(ctx->pin + c0_bytes + hs_bytes + obj_bytes);
   build_c0(z, nso, N, cmplxconj, c0);
   ctx->pin_busy = true;
-  if (objective && ctx->small_size < sizeof(double) * nobj) {
-    HIPCK(hipStreamSynchronize(s));
-    if (ctx->small) HIPCK(hipFree(ctx->small));
-    ctx->small = nullptr;
-    ctx->small_size = 0;
-    HIPCK(hipMalloc(&ctx->small, sizeof(double) * nobj));
-    ctx->small_size = sizeof(double) * nobj;
-  }
   // one evaluation's launch sequence: eager, or captured for replay
   auto enqueue = [&]() -> int {
     HIPCK(hipMemcpyAsync(b.C0, c0, sizeof(double) * 2 * (size_t)nso * N, hipMemcpyHostToDevice, s));
@@ -3008,15 +3068,17 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
     // the objective kernel goes in before any copy back, and the small results
     // come back through pinned memory, so nothing waits on the host until the
     // one synchronization below
-    if (objective) {
-      double* dob = reinterpret_cast<double*>(ctx->small);
+    if (objective) {   // results, Hmat/Smat and the info flags stored into pin by the kernel
       const int nel = (nso - N) * N;
-      objective_kernel<<<(std::max(nel, 1) + 255) / 256, 256, 0, s>>>(nso, N, b.HS, b.Ysum, b.Xsum, dob);
+      char* pd = ctx->pin_dev;
+      objective_kernel<<<(std::max(nel, 1) + 255) / 256, 256, 0, s>>>(
+          nso, N, b.HS, b.Ysum, b.Xsum, reinterpret_cast<double*>(pd + c0_bytes + hs_bytes), 4 * nmn, b.Info, Gv,
+          reinterpret_cast<double*>(pd + c0_bytes), reinterpret_cast<int*>(pd + c0_bytes + hs_bytes + obj_bytes));
       HIPCK(hipGetLastError());
-      HIPCK(hipMemcpyAsync(pobj, dob, sizeof(double) * nobj, hipMemcpyDeviceToHost, s));
+    } else {
+      HIPCK(hipMemcpyAsync(hs, b.HS, sizeof(double) * 4 * (size_t)nmn, hipMemcpyDeviceToHost, s));
+      HIPCK(hipMemcpyAsync(info, b.Info, sizeof(int) * (size_t)Gv, hipMemcpyDeviceToHost, s));
     }
-    HIPCK(hipMemcpyAsync(hs, b.HS, sizeof(double) * 4 * (size_t)nmn, hipMemcpyDeviceToHost, s));
-    HIPCK(hipMemcpyAsync(info, b.Info, sizeof(int) * (size_t)Gv, hipMemcpyDeviceToHost, s));
     if (!resident) {
       HIPCK(hipMemcpyAsync(gradmat, b.Grad, 2 * (size_t)nmn * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
       HIPCK(hipMemcpyAsync(rdmmat, b.Rdm, 2 * (size_t)nmn * n2 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -3026,11 +3088,11 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
   auto make_key = [&]() {
     char kb[512];
     auto env = [](const char* v) { const char* e = getenv(v); return e ? e : "-"; };
-    snprintf(kb, sizeof kb, "%d|%d|%d|%d|%d|%ld|%p|%p|%p|%p|%zu|%s|%s|%s|%s|%s|%s|%s", precision, nocc, cmplxconj,
+    snprintf(kb, sizeof kb, "%d|%d|%d|%d|%d|%ld|%p|%p|%p|%p|%zu|%s|%s|%s|%s|%s|%s|%s|%s", precision, nocc, cmplxconj,
              (int)resident, (int)(objective != nullptr), ctx->gen, (void*)ctx->ws, (void*)ctx->pin,
              (void*)ctx->small, (void*)ctx->d_xitems, ctx->ws_size, env("PHF_H16_CFG"), env("PHF_XGROUP"),
              env("PHF_FUSED_SPLIT"), env("PHF_H16_FOLD"), env("PHF_H16_ALLOW_F32FOLD"), env("PHF_FUSED_EPILOGUE"),
-             env("PHF_FUSED_SOLVE"));
+             env("PHF_FUSED_SOLVE"), env("PHF_SPLIT_RJK"));
     return std::string(kb);
   };
   std::string key = graph_ok ? make_key() : std::string();

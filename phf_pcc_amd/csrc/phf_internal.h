@@ -163,7 +163,8 @@ struct phf_ctx_s {
   size_t small_size = 0;
   // pinned host staging of one evaluation's C0 upload and small results
   // (asynchronous copies; reused after each call's final synchronization)
-  char* pin = nullptr;
+  char* pin = nullptr;       // coherent, mapped host memory
+  char* pin_dev = nullptr;   // its device address (kernels store the objective's results there)
   size_t pin_size = 0;
   bool pin_busy = false;   // copies from/to pin may be in flight (a call ended before its synchronization)
   // the two exchange blocks X1, X2 as one grouped launch (reduced forms):
@@ -308,6 +309,9 @@ int eval_gemm(phf_ctx* ctx, const EvalBufs& b);
 // blk_mask selects the spin blocks of blocked units (bit q: block q).
 void build_h16_items(const std::vector<std::pair<const phf_ctx*, const EvalBufs*>>& units, int prec, int cfg,
                      int sk, int grid, std::vector<H16Item>& out, unsigned blk_mask = 7u);
+// Order of a logical work list for a persistent launch of `grid` workgroups
+// (each XCD takes a contiguous run of items per round).
+void h16_xcd_order(const std::vector<H16Item>& logical, int grid, std::vector<H16Item>& out);
 // True when build_h16_items makes the spin-free units' exchange sets write
 // Gamma_ab, Gamma_ba + h directly (split-K 1 and at least one spin-free unit).
 bool rjk_direct_items(const std::vector<std::pair<const phf_ctx*, const EvalBufs*>>& units, int sk);

@@ -13,7 +13,10 @@ columns), the spin blocks and the dense A, FP64 and F16X2.
 Inverse + solve: lu_inverse_solve_kernel (N <= 32, one density per point)
 inverts M_g in LDS and forms X_g = T_g M_g^{-1} in the same workgroup,
 against the inverse kernel and the solve GEMM (PHF_FUSED_SOLVE=0), to the
-same bounds; the overlaps' determinants enter Hmat/Smat and the gradient."""
+same bounds; the overlaps' determinants enter Hmat/Smat and the gradient.
+
+Split spin-free GEMM: the direct grouped launch as two launches on two
+streams against one launch (PHF_SPLIT_RJK=0)."""
 import os
 
 import numpy as np
@@ -73,6 +76,31 @@ def test_fused_inverse_solve_equals_launches(n, nocc, ngrid, prec):
     e1, g1, r1 = _eval(m, p.z, True, "PHF_FUSED_SOLVE")
     e0, g0, r0 = _eval(m, p.z, False, "PHF_FUSED_SOLVE")
     m.ctx.close()
+    assert abs(e1 - e0) <= 1e-12 * max(1.0, abs(e0))
+    assert np.abs(g1 - g0).max() <= 1e-12 * max(1.0, np.abs(g0).max())
+    for name, a, b in zip(["Hmat", "Smat", "Gradmat", "Rdmmat"], r1, r0):
+        assert _rel(a, b) <= 1e-12, name
+
+
+@pytest.mark.parametrize("n,nocc,ngrid", [(64, 32, (50, 16)), (24, 12, (14, 8))])
+@pytest.mark.parametrize("prec", ["f16x2", "bf16x3"])
+def test_split_spinfree_gemm_equals_one_launch(n, nocc, ngrid, prec):
+    """The direct grouped GEMM of the spin-free form as two launches on two
+    streams (Gamma_aa / Gamma_bb product tiles, then the unpack, on the main
+    stream; the exchange tiles on a side stream) against one launch followed
+    by the unpack (PHF_SPLIT_RJK=0): the same tiles and the same unpack, so
+    the results agree to the last bit; checked to 1e-12 like the cases above."""
+    from phf_pcc_amd.model import ProjectedHF
+    p = synthetic_problem(n, nocc, seed=970 + n, zscale=0.05)
+    ga = grid_args(n, ngrid, 0)
+    m = ProjectedHF(p.nocc, p.ref, ga, factors=(p.h_spatial, p.b), precision=prec)
+    m.ctx.set_blocked(True)
+    print(f"split spin-free GEMM n={n} N={nocc} G={int(np.prod(ngrid))} {prec}", flush=True)
+    e1, g1, r1 = _eval(m, p.z, True, "PHF_SPLIT_RJK")
+    e0, g0, r0 = _eval(m, p.z, False, "PHF_SPLIT_RJK")
+    e2, g2, _ = _eval(m, p.z, True, "PHF_SPLIT_RJK")
+    m.ctx.close()
+    assert e1 == e2 and np.array_equal(g1, g2)   # repeated split launches
     assert abs(e1 - e0) <= 1e-12 * max(1.0, abs(e0))
     assert np.abs(g1 - g0).max() <= 1e-12 * max(1.0, np.abs(g0).max())
     for name, a, b in zip(["Hmat", "Smat", "Gradmat", "Rdmmat"], r1, r0):
