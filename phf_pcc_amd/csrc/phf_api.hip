@@ -1980,6 +1980,15 @@ static bool no_fused_solve() {
   const char* f = getenv("PHF_FUSED_SOLVE");
   return f && atoi(f) == 0;
 }
+// Side stream k for the small pre-stage and grid-sum work (Hc, the T_g
+// combine, the determinant weights) with PHF_SIDE_PRE=1; by default that work
+// stays on the main stream: each fork or join of a stream costs the main
+// stream more idle time (measured 5-10 us in the replayed launch graph) than
+// running these 5-17 us kernels in line.
+static hipStream_t pre_side(const phf_ctx* ctx, int k) {
+  const char* f = getenv("PHF_SIDE_PRE");
+  return (f && atoi(f) == 1) ? ctx->side[k] : nullptr;
+}
 // PHF_SPLIT_RJK=0 keeps the direct grouped GEMM of the spin-free form as one
 // launch followed by the unpack (the tests compare both).
 static bool no_split_rjk() {
@@ -2150,8 +2159,8 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
   // one density per point, N <= 32: inverse and solve in one kernel per point
   const bool fused_solve = V == 1 && N <= 32 && !no_fused_solve();
   // Hc = B^H h per right factor (used by the energy in eval_post): on a side
-  // stream, overlapping the pre-stages, when the context has one
-  if (ctx->side[0]) {
+  // stream, overlapping the pre-stages, with pre_side; else in eval_post
+  if (pre_side(ctx, 0)) {
     (void)hipEventRecord(ctx->hc_fork, s);
     (void)hipStreamWaitEvent(ctx->side[0], ctx->hc_fork, 0);
     hc_products(ctx, b, ctx->side[0]);
@@ -2175,8 +2184,9 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
       zgemm_batched_kslices(s, nso, N, nso, 1, 0, zmat_colmajor(ctx->dGP, nso, 2 * n2),
                             zmat_colmajor(b.C0, nso, 0, lc), 0, 0, zout_colmajor(Q, nso, sT), 4 * npg, dq, b.FX,
                             ws_cplx);
-      // T_g (needed by the solve) on a side stream, beside the overlaps below
-      if (ctx->side[1]) {
+      // T_g (needed by the solve): with pre_side on a side stream, beside the
+      // overlaps below
+      if (pre_side(ctx, 1)) {
         (void)hipEventRecord(ctx->tq_fork, s);
         (void)hipStreamWaitEvent(ctx->side[1], ctx->tq_fork, 0);
         spinrot_kernel<<<spinrot_grid(G, lenX), 256, 0, ctx->side[1]>>>(G, npg, lenX, ctx->dGu, Q, T);
@@ -2236,10 +2246,11 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
                     zmat_colmajor_t(b.T + variant_left(v) * sT * G, nso, sT), 0, 0,
                     zout_colmajor_t(b.X + v * sT * G, nso, sT), (int)G);
   }
-  // the determinant weights of the grid sums on the side stream (they need
-  // only Det), joined through hc_join before the epilogue
+  // with pre_side, the determinant weights of the grid sums on the side stream
+  // (they need only Det), joined through hc_join before the epilogue; else
+  // eval_post launches them
   ctx->wd_forked = false;
-  if (ctx->side[0]) {
+  if (pre_side(ctx, 0)) {
     const int nci = ctx->g_nci, nmn = nci * nci, d = (V == 4) ? nci / 2 : nci;
     (void)hipEventRecord(ctx->wd_fork, s);
     (void)hipStreamWaitEvent(ctx->side[0], ctx->wd_fork, 0);
@@ -2929,7 +2940,7 @@ void eval_post(phf_ctx* ctx, const EvalBufs& b, bool resident) {
   // energy and the adjoint in one kernel per point (Y written over FX)
   const bool fused = V == 1 && nso <= 128 && N <= 32 && !no_fused_epilogue();
   if (fused) {
-    if (ctx->side[0])
+    if (pre_side(ctx, 0))
       (void)hipStreamWaitEvent(s, ctx->hc_join, 0);
     else
       hc_products(ctx, b, s);
@@ -2943,7 +2954,7 @@ void eval_post(phf_ctx* ctx, const EvalBufs& b, bool resident) {
   // per right factor B: Q_v = B^H FX_v, Hc = B^H h (computed by eval_pre on a
   // side stream when the context has one), e_v = Makeh
   if (!fused) {
-    if (ctx->side[0])
+    if (pre_side(ctx, 0))
       (void)hipStreamWaitEvent(s, ctx->hc_join, 0);
     else
       hc_products(ctx, b, s);
@@ -3088,11 +3099,11 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
   auto make_key = [&]() {
     char kb[512];
     auto env = [](const char* v) { const char* e = getenv(v); return e ? e : "-"; };
-    snprintf(kb, sizeof kb, "%d|%d|%d|%d|%d|%ld|%p|%p|%p|%p|%zu|%s|%s|%s|%s|%s|%s|%s|%s", precision, nocc, cmplxconj,
+    snprintf(kb, sizeof kb, "%d|%d|%d|%d|%d|%ld|%p|%p|%p|%p|%zu|%s|%s|%s|%s|%s|%s|%s|%s|%s", precision, nocc, cmplxconj,
              (int)resident, (int)(objective != nullptr), ctx->gen, (void*)ctx->ws, (void*)ctx->pin,
              (void*)ctx->small, (void*)ctx->d_xitems, ctx->ws_size, env("PHF_H16_CFG"), env("PHF_XGROUP"),
              env("PHF_FUSED_SPLIT"), env("PHF_H16_FOLD"), env("PHF_H16_ALLOW_F32FOLD"), env("PHF_FUSED_EPILOGUE"),
-             env("PHF_FUSED_SOLVE"), env("PHF_SPLIT_RJK"));
+             env("PHF_FUSED_SOLVE"), env("PHF_SPLIT_RJK"), env("PHF_SIDE_PRE"));
     return std::string(kb);
   };
   std::string key = graph_ok ? make_key() : std::string();

@@ -16,7 +16,9 @@ against the inverse kernel and the solve GEMM (PHF_FUSED_SOLVE=0), to the
 same bounds; the overlaps' determinants enter Hmat/Smat and the gradient.
 
 Split spin-free GEMM: the direct grouped launch as two launches on two
-streams against one launch (PHF_SPLIT_RJK=0)."""
+streams against one launch (PHF_SPLIT_RJK=0).  Side-stream pre-stages: Hc,
+the T_g combine and the determinant weights on side streams (PHF_SIDE_PRE=1)
+against the main stream."""
 import os
 
 import numpy as np
@@ -101,6 +103,25 @@ def test_split_spinfree_gemm_equals_one_launch(n, nocc, ngrid, prec):
     e2, g2, _ = _eval(m, p.z, True, "PHF_SPLIT_RJK")
     m.ctx.close()
     assert e1 == e2 and np.array_equal(g1, g2)   # repeated split launches
+    assert abs(e1 - e0) <= 1e-12 * max(1.0, abs(e0))
+    assert np.abs(g1 - g0).max() <= 1e-12 * max(1.0, np.abs(g0).max())
+    for name, a, b in zip(["Hmat", "Smat", "Gradmat", "Rdmmat"], r1, r0):
+        assert _rel(a, b) <= 1e-12, name
+
+
+@pytest.mark.parametrize("n,nocc,ngrid,prec", [(64, 32, (50, 16), "f16x2"), (24, 12, (14, 8), "fp64")])
+def test_side_stream_pre_stages_equal_inline(n, nocc, ngrid, prec):
+    """Hc, the T_g combine and the determinant weights on side streams
+    (PHF_SIDE_PRE=1) against the default, where they run on the main stream:
+    the same kernels on the same inputs, to 1e-12."""
+    from phf_pcc_amd.model import ProjectedHF
+    p = synthetic_problem(n, nocc, seed=990 + n, zscale=0.05)
+    ga = grid_args(n, ngrid, 0)
+    m = ProjectedHF(p.nocc, p.ref, ga, factors=(p.h_spatial, p.b), precision=prec)
+    print(f"side-stream pre-stages n={n} N={nocc} G={int(np.prod(ngrid))} {prec}", flush=True)
+    e1, g1, r1 = _eval(m, p.z, True, "PHF_SIDE_PRE")
+    e0, g0, r0 = _eval(m, p.z, False, "PHF_SIDE_PRE")
+    m.ctx.close()
     assert abs(e1 - e0) <= 1e-12 * max(1.0, abs(e0))
     assert np.abs(g1 - g0).max() <= 1e-12 * max(1.0, np.abs(g0).max())
     for name, a, b in zip(["Hmat", "Smat", "Gradmat", "Rdmmat"], r1, r0):
