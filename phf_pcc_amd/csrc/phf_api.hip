@@ -2159,12 +2159,16 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
   // one density per point, N <= 32: inverse and solve in one kernel per point
   const bool fused_solve = V == 1 && N <= 32 && !no_fused_solve();
   // Hc = B^H h per right factor (used by the energy in eval_post): on a side
-  // stream, overlapping the pre-stages, with pre_side; else in eval_post
+  // stream, overlapping the pre-stages, with pre_side
   if (pre_side(ctx, 0)) {
     (void)hipEventRecord(ctx->hc_fork, s);
     (void)hipStreamWaitEvent(ctx->side[0], ctx->hc_fork, 0);
     hc_products(ctx, b, ctx->side[0]);
     (void)hipEventRecord(ctx->hc_join, ctx->side[0]);   // recorded again after the weights below
+  } else {
+    // first in line: a 16 us launch during which the host's submission of the
+    // short pre-stage launches below gets ahead of the device
+    hc_products(ctx, b, s);
   }
   // K1: rotation chain, right to left (Ref^H R1 and Rpg Ref are cached with the
   // grid), once per left factor L = C0 (lc = 0) or C0* (lc = 1)
@@ -2940,10 +2944,7 @@ void eval_post(phf_ctx* ctx, const EvalBufs& b, bool resident) {
   // energy and the adjoint in one kernel per point (Y written over FX)
   const bool fused = V == 1 && nso <= 128 && N <= 32 && !no_fused_epilogue();
   if (fused) {
-    if (pre_side(ctx, 0))
-      (void)hipStreamWaitEvent(s, ctx->hc_join, 0);
-    else
-      hc_products(ctx, b, s);
+    if (pre_side(ctx, 0)) (void)hipStreamWaitEvent(s, ctx->hc_join, 0);   // else eval_pre formed Hc on s
     fock_adjoint_kernel<<<(unsigned)G, 256, 0, s>>>(nso, N, b.X, sT, b.Gam, nrpad, b.C0, b.Hc, b.FX, b.E);
   }
   // K4: FX = (h + Gamma) X   (the ERI GEMM epilogue added h), formed as
@@ -2951,14 +2952,8 @@ void eval_post(phf_ctx* ctx, const EvalBufs& b, bool resident) {
   if (!fused)
     zgemm_batched(s, N, nso, nso, 1, 0, zmat_colmajor_t(b.X, nso, sT), ZMat{b.Gam, 1, nso, 2L * nrpad, nrpad, 0}, 0,
                   0, zout_colmajor_t(b.FX, nso, sT), (int)Gv);
-  // per right factor B: Q_v = B^H FX_v, Hc = B^H h (computed by eval_pre on a
-  // side stream when the context has one), e_v = Makeh
-  if (!fused) {
-    if (pre_side(ctx, 0))
-      (void)hipStreamWaitEvent(s, ctx->hc_join, 0);
-    else
-      hc_products(ctx, b, s);
-  }
+  // per right factor B: Q_v = B^H FX_v, Hc = B^H h (formed by eval_pre), e_v = Makeh
+  if (!fused && pre_side(ctx, 0)) (void)hipStreamWaitEvent(s, ctx->hc_join, 0);   // else eval_pre formed Hc on s
   for (int bc = 0; bc < nB && !fused; ++bc) {
     const ZMat bh{b.C0, 2L * nso, 2, 0, 1, bc ? 0 : 1};
     double* Hc = b.Hc + bc * sT;
