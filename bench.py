@@ -759,22 +759,27 @@ def gemm_launches(mode, st, G, n, prec, report):
     if mode == "spin-free":
         a0, a1 = 2.0 * (2 * G) * n2 * n2, 2.0 * (6 * G) * n2 * n2
         c1 = tuned_cfg(6 * G, n2)
-        chosen, half = None, False
+        chosen, half, split = None, False, False
         for rec in _report(report):
             if rec.get("prec") == code and "spin_free_units" in rec:
                 chosen = rec["spin_free_units"]["chosen"]
                 half = rec["spin_free_units"].get("unpack") == "aa_bb"
-        if ms(1) is None:   # one grouped launch of both products
+        if ms(1) is None:   # one grouped launch of both products, or its two halves on two streams
             cfg = (4 if prec == "i8x4" else 5) if chosen == "grouped_256x256" else c1
-            kernels = [row("spinfree", "2J-K + K (one grouped launch)", _cfg_kernel(prec, cfg, True), ms(0),
-                           a0 + a1)]
+            split = half and os.environ.get("PHF_SPLIT_RJK", "1").strip() != "0"
+            name = ("2J-K + K (the grouped tiles as two concurrent launches: Gamma_aa/bb product sets on the "
+                    "main stream, exchange sets on a side stream; span from the first's start to the second's "
+                    "end)") if split else "2J-K + K (one grouped launch)"
+            kernels = [row("spinfree", name, _cfg_kernel(prec, cfg, True), ms(0), a0 + a1)]
+            kernels[0]["launches"] = 2 if split else 1
         else:
             kernels = [row("2J-K", "2J-K", _cfg_kernel(prec, tuned_cfg(2 * G, n2)), ms(0), a0),
                        row("K", "K", _cfg_kernel(prec, c1), ms(1), a1)]
         # read the product sets, write their Gamma elements: all four (2G rows x (4 + 4) n^2 doubles),
         # or only Gamma_aa, Gamma_bb when the grouped launch wrote the exchange elements itself
         ub = 8.0 * (2 * G) * ((2 + 2) if half else (4 + 4)) * n2
-        unpack = {"launch": "unpack into Gamma + h" + (" (aa, bb)" if half else ""), "kernel": "rjk_unpack_kernel",
+        unpack = {"launch": "unpack into Gamma + h" + (" (aa, bb)" if half else "")
+                  + (", beside the exchange launch" if ms(1) is None and split else ""), "kernel": "rjk_unpack_kernel",
                   "ms_per_launch": ms(2), "algorithmic_bytes": ub,
                   "GBps": ub / (ms(2) * 1e-3) / 1e9 if ms(2) else None,
                   "frac_hbm": ub / (ms(2) * 1e-3) / 8e12 if ms(2) else None}
@@ -983,12 +988,17 @@ def main():
             "ms_per_launch": dom["ms_per_launch"],
             "algorithmic_flops_per_launch": dom["algorithmic_flops_per_launch"],
             "executed_flops_per_launch": nprod * dom["algorithmic_flops_per_launch"], "products": nprod,
-            "time_source": "HIP events around this launch on the stream it runs on, mean over the timed steps",
+            "time_source": ("HIP events around this launch on the stream it runs on, mean over the timed steps"
+                            if dom.get("launches", 1) == 1 else
+                            "HIP events: start recorded before the first launch (main stream), end after the "
+                            "second (side stream), mean over the timed steps"),
             "kernels": kernels, "unpack": unpack,
             "gemm_stage": {"ms": best["gemm_ms"], "launches": len(kernels) + (1 if unpack else 0),
                            "algorithmic_flops": flops, "achieved_tflops": flops / t_gemm / 1e12,
                            "frac": flops / t_gemm / peak, "frac_executed": nprod * flops / t_gemm / peak,
-                           "note": "the stage includes the unpack of the spin-free products" if unpack else None},
+                           "note": ("the stage includes the unpack of the spin-free products"
+                                    + (" (run beside the exchange launch)" if dom.get("launches", 1) == 2 else ""))
+                           if unpack else None},
             "plain_fp64_dense_frac": flops_dense / (plain["gemm_ms"] * 1e-3) / PEAK["fp64"]}
     value = best["value"]
     ms_step = best["ms_step"]

@@ -759,25 +759,28 @@ __global__ __launch_bounds__(256) void split_rows_kernel(const double* __restric
 // (waves of (16 FM) x (16 FN), WN = 128 / (16 FN) across).  Without maps tile
 // row / column q is MO q (q < nso); with MAPPED, rmap[q] / cmap[q] name the MO
 // (-1: an empty row or column, zero).
-template <int NT, int FM, int FN, bool MAPPED>
+template <int NT, int FM, int FN, bool MAPPED, bool K2>
 __device__ __forceinline__ void dens_tile(int nso, int nk, const double* __restrict__ C0,
                                           const double* __restrict__ Xg, const int* __restrict__ rmap,
                                           const int* __restrict__ cmap, d4_t (&p1)[FM][FN], d4_t (&p2)[FM][FN]) {
-  constexpr int TB = 128, KC = 16, LD = KC + 2, WN = TB / (16 * FN);
-  __shared__ double Ar[TB * LD], Ai[TB * LD], Br[TB * LD], Bi[TB * LD];
+  constexpr int TB = 128, KC = 16, LD = KC + 2, WN = TB / (16 * FN), CH = TB * LD, NB = K2 ? 2 : 1;
+  // K2 (nk <= 2 KC, i.e. N <= 32): both chunks staged at once in two buffers,
+  // one barrier and no second load latency; else one chunk after another
+  __shared__ double Ar[NB * CH], Ai[NB * CH], Br[NB * CH], Bi[NB * CH];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WN, wn = wave % WN;
   d4_t p3[FM][FN];
-  // every load of a 16-deep k-chunk is issued before the first LDS store;
-  // the first chunk is peeled so that its loads are consumed before the
+  // every load of NC 16-deep k-chunks is issued before the first LDS store;
+  // the first stage is peeled so that its loads are consumed before the
   // accumulators come alive
   constexpr int EL = TB * KC / NT;
-  auto stage = [&](int k0) {
-    double lar[EL], lai[EL], lbr[EL], lbi[EL];
-    bool la[EL], lb[EL];
+  auto stage = [&](int k0, auto nc_tag) {
+    constexpr int NC = decltype(nc_tag)::value;
+    double lar[NC * EL], lai[NC * EL], lbr[NC * EL], lbi[NC * EL];
+    bool la[NC * EL], lb[NC * EL];
 #pragma unroll
-    for (int e = 0; e < EL; ++e) {
-      const int idx = e * NT + tid, i = idx % TB, gk = k0 + idx / TB;
+    for (int e = 0; e < NC * EL; ++e) {
+      const int idx = (e % EL) * NT + tid, i = idx % TB, gk = k0 + (e / EL) * KC + idx / TB;
       int ra = i, rb = i;
       if constexpr (MAPPED) {
         ra = rmap[i];
@@ -792,29 +795,30 @@ __device__ __forceinline__ void dens_tile(int nso, int nk, const double* __restr
       lbi[e] = Xg[offb + 1];
     }
 #pragma unroll
-    for (int e = 0; e < EL; ++e) {
-      const int idx = e * NT + tid, i = idx % TB, kk = idx / TB;
-      Ar[i * LD + kk] = la[e] ? lar[e] : 0.0;
-      Ai[i * LD + kk] = la[e] ? -lai[e] : 0.0;
-      Br[i * LD + kk] = lb[e] ? lbr[e] : 0.0;
-      Bi[i * LD + kk] = lb[e] ? lbi[e] : 0.0;
+    for (int e = 0; e < NC * EL; ++e) {
+      const int idx = (e % EL) * NT + tid, i = idx % TB, kk = idx / TB, o = (e / EL) * CH + i * LD + kk;
+      Ar[o] = la[e] ? lar[e] : 0.0;
+      Ai[o] = la[e] ? -lai[e] : 0.0;
+      Br[o] = lb[e] ? lbr[e] : 0.0;
+      Bi[o] = lb[e] ? lbi[e] : 0.0;
     }
   };
-  auto mfma_chunk = [&]() {
+  auto mfma_chunk = [&](int buf) {
+    const double *ar_ = Ar + buf * CH, *ai_ = Ai + buf * CH, *br_ = Br + buf * CH, *bi_ = Bi + buf * CH;
 #pragma unroll
     for (int ks = 0; ks < KC; ks += 4) {
       double are[FM], aim[FM], bre[FN], bim[FN];
 #pragma unroll
       for (int t = 0; t < FM; ++t) {
         const int row = wm * 16 * FM + t * 16 + (lane & 15);
-        are[t] = Ar[row * LD + ks + (lane >> 4)];
-        aim[t] = Ai[row * LD + ks + (lane >> 4)];
+        are[t] = ar_[row * LD + ks + (lane >> 4)];
+        aim[t] = ai_[row * LD + ks + (lane >> 4)];
       }
 #pragma unroll
       for (int t = 0; t < FN; ++t) {
         const int col = wn * 16 * FN + t * 16 + (lane & 15);
-        bre[t] = Br[col * LD + ks + (lane >> 4)];
-        bim[t] = Bi[col * LD + ks + (lane >> 4)];
+        bre[t] = br_[col * LD + ks + (lane >> 4)];
+        bim[t] = bi_[col * LD + ks + (lane >> 4)];
       }
 #pragma unroll
       for (int i = 0; i < FM; ++i)
@@ -826,22 +830,33 @@ __device__ __forceinline__ void dens_tile(int nso, int nk, const double* __restr
         }
     }
   };
-  stage(0);
-  __syncthreads();
+  auto zero = [&]() {
 #pragma unroll
-  for (int i = 0; i < FM; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      p1[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
-      p2[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
-      p3[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+      for (int j = 0; j < FN; ++j) {
+        p1[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+        p2[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+        p3[i][j] = d4_t{0.0, 0.0, 0.0, 0.0};
+      }
+  };
+  if constexpr (K2) {
+    stage(0, std::integral_constant<int, 2>());
+    __syncthreads();
+    zero();
+    mfma_chunk(0);
+    if (nk > KC) mfma_chunk(1);
+  } else {
+    stage(0, std::integral_constant<int, 1>());
+    __syncthreads();
+    zero();
+    mfma_chunk(0);
+    for (int k0 = KC; k0 < nk; k0 += KC) {
+      __syncthreads();
+      stage(k0, std::integral_constant<int, 1>());
+      __syncthreads();
+      mfma_chunk(0);
     }
-  mfma_chunk();
-  for (int k0 = KC; k0 < nk; k0 += KC) {
-    __syncthreads();
-    stage(k0);
-    __syncthreads();
-    mfma_chunk();
   }
 #pragma unroll
   for (int i = 0; i < FM; ++i)
@@ -881,7 +896,7 @@ __device__ __forceinline__ void put_slices(T* __restrict__ d, long kout, long c,
 // without writing and reading those rows.  Columns no element maps to are
 // zeroed (padcols, or [NSO^2, kout) without a map); workgroups past the last
 // point clear the padding rows up to rows_total (zero slices, inverse scale 1).
-template <typename T, int NS, int NT, int FM, int FN>
+template <typename T, int NS, int NT, int FM, int FN, bool K2>
 __global__ __launch_bounds__(NT) void dens_split_kernel(int nso, int nk, const double* __restrict__ C0,
                                                           const double* __restrict__ X, long sX, int npts,
                                                           const int* __restrict__ ocol,
@@ -905,7 +920,7 @@ __global__ __launch_bounds__(NT) void dens_split_kernel(int nso, int nk, const d
   }
   const int wm = wave / WN, wn = wave % WN;
   d4_t p1[FM][FN], p2[FM][FN];
-  dens_tile<NT, FM, FN, false>(nso, nk, C0, X + (long)g * sX, nullptr, nullptr, p1, p2);
+  dens_tile<NT, FM, FN, false, K2>(nso, nk, C0, X + (long)g * sX, nullptr, nullptr, p1, p2);
   // complex results and the point's largest magnitudes
   double mre = 0.0, mim = 0.0;
 #pragma unroll
@@ -995,7 +1010,7 @@ __global__ __launch_bounds__(NT) void dens_split_kernel(int nso, int nk, const d
 // rjk_pack_kernel and split_rows_kernel, bit for bit.  Columns [n^2, kj) are
 // zeroed; workgroups past the last point clear each set's padding rows
 // [2 npts, Mp).
-template <typename T, int NS, int NT, int FM, int FN>
+template <typename T, int NS, int NT, int FM, int FN, bool K2>
 __global__ __launch_bounds__(NT) void dens_split_rjk_kernel(int n, int nso, int nk, const double* __restrict__ C0,
                                                               const double* __restrict__ X, long sX, int npts,
                                                               const int* __restrict__ smap, int kj, long Mp,
@@ -1022,7 +1037,7 @@ __global__ __launch_bounds__(NT) void dens_split_rjk_kernel(int n, int nso, int 
   }
   const int wm = wave / WN, wn = wave % WN;
   d4_t p1[FM][FN], p2[FM][FN];
-  dens_tile<NT, FM, FN, true>(nso, nk, C0, X + (long)g * sX, smap, smap + 128, p1, p2);
+  dens_tile<NT, FM, FN, true, K2>(nso, nk, C0, X + (long)g * sX, smap, smap + 128, p1, p2);
   const int lc = wn * 16 + (lane & 15);
   // the four sets of one (m, r): value q = set * 2 + plane
   auto sets = [&](int m, int r, double (&v)[8]) {
@@ -1169,15 +1184,20 @@ int dens_split_max_nso() { return 128; }
 void dens_split(hipStream_t s, int prec, int nso, int nk, const double* C0, const double* X, long sX, int npts,
                 const int* ocol, const int* padcols, int npad, int kout, long rows_total, void* dst, double* inv) {
   const unsigned grid = (unsigned)(npts + (rows_total - 2L * npts + 1) / 2);
-  if (prec == 3)
-    dens_split_kernel<_Float16, 2, DS_NT, DS_FM, DS_FN><<<grid, DS_NT, 0, s>>>(
-        nso, nk, C0, X, sX, npts, ocol, padcols, npad, kout, rows_total, (_Float16*)dst, 14, inv);
-  else if (prec == 2)
-    dens_split_kernel<__bf16, 3, DS_NT, DS_FM, DS_FN><<<grid, DS_NT, 0, s>>>(
-        nso, nk, C0, X, sX, npts, ocol, padcols, npad, kout, rows_total, (__bf16*)dst, 14, inv);
-  else
-    dens_split_kernel<signed char, 4, DS_NT, DS_FM, DS_FN><<<grid, DS_NT, 0, s>>>(
-        nso, nk, C0, X, sX, npts, ocol, padcols, npad, kout, rows_total, (signed char*)dst, 7, inv);
+  auto go = [&](auto k2) {
+    constexpr bool K2 = decltype(k2)::value;
+    if (prec == 3)
+      dens_split_kernel<_Float16, 2, DS_NT, DS_FM, DS_FN, K2><<<grid, DS_NT, 0, s>>>(
+          nso, nk, C0, X, sX, npts, ocol, padcols, npad, kout, rows_total, (_Float16*)dst, 14, inv);
+    else if (prec == 2)
+      dens_split_kernel<__bf16, 3, DS_NT, DS_FM, DS_FN, K2><<<grid, DS_NT, 0, s>>>(
+          nso, nk, C0, X, sX, npts, ocol, padcols, npad, kout, rows_total, (__bf16*)dst, 14, inv);
+    else
+      dens_split_kernel<signed char, 4, DS_NT, DS_FM, DS_FN, K2><<<grid, DS_NT, 0, s>>>(
+          nso, nk, C0, X, sX, npts, ocol, padcols, npad, kout, rows_total, (signed char*)dst, 7, inv);
+  };
+  if (nk <= 32) go(std::true_type());
+  else go(std::false_type());
 }
 
 void dens_split_rjk_maps(const std::vector<int>& mo, int n, int* smap) {
@@ -1197,15 +1217,20 @@ void dens_split_rjk(hipStream_t s, int prec, int n, int nk, const double* C0, co
   const long pad = 4 * (Mp - 2L * npts);   // Mk = 3 Mp: four sets of Mp rows
   const unsigned grid = (unsigned)(npts + (pad + 1) / 2);
   const int nso = 2 * n;
-  if (prec == 3)
-    dens_split_rjk_kernel<_Float16, 2, DS_NT, DS_FM, DS_FN><<<grid, DS_NT, 0, s>>>(
-        n, nso, nk, C0, X, sX, npts, smap, kj, Mp, Mk, (_Float16*)dst, 14, inv);
-  else if (prec == 2)
-    dens_split_rjk_kernel<__bf16, 3, DS_NT, DS_FM, DS_FN><<<grid, DS_NT, 0, s>>>(
-        n, nso, nk, C0, X, sX, npts, smap, kj, Mp, Mk, (__bf16*)dst, 14, inv);
-  else
-    dens_split_rjk_kernel<signed char, 4, DS_NT, DS_FM, DS_FN><<<grid, DS_NT, 0, s>>>(
-        n, nso, nk, C0, X, sX, npts, smap, kj, Mp, Mk, (signed char*)dst, 7, inv);
+  auto go = [&](auto k2) {
+    constexpr bool K2 = decltype(k2)::value;
+    if (prec == 3)
+      dens_split_rjk_kernel<_Float16, 2, DS_NT, DS_FM, DS_FN, K2><<<grid, DS_NT, 0, s>>>(
+          n, nso, nk, C0, X, sX, npts, smap, kj, Mp, Mk, (_Float16*)dst, 14, inv);
+    else if (prec == 2)
+      dens_split_rjk_kernel<__bf16, 3, DS_NT, DS_FM, DS_FN, K2><<<grid, DS_NT, 0, s>>>(
+          n, nso, nk, C0, X, sX, npts, smap, kj, Mp, Mk, (__bf16*)dst, 14, inv);
+    else
+      dens_split_rjk_kernel<signed char, 4, DS_NT, DS_FM, DS_FN, K2><<<grid, DS_NT, 0, s>>>(
+          n, nso, nk, C0, X, sX, npts, smap, kj, Mp, Mk, (signed char*)dst, 7, inv);
+  };
+  if (nk <= 32) go(std::true_type());
+  else go(std::false_type());
 }
 
 void split_operand(hipStream_t s, int prec, const double* src, long lds, int rows, int kt, void* dst, double* inv) {

@@ -47,3 +47,24 @@ def test_fused_equals_two_pass(n, ngrid, blocked, prec):
     assert a[0] == b[0]
     for x, y in zip(a[1:], b[1:]):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("n,nocc", [(64, 32), (48, 36)])
+@pytest.mark.parametrize("spinfree", [False, True])
+def test_fused_equals_two_pass_chunks(n, nocc, spinfree):
+    """N = 32 stages both 16-deep chunks of the density product at once; N = 36
+    takes the chunk loop (three chunks, the last one partial): both bit for bit
+    against the two-pass split, through the spin blocks and the spin-free form."""
+    from phf_pcc_amd.model import ProjectedHF
+    p = synthetic_problem(n, nocc, seed=310 + n, zscale=0.05)
+    ga = grid_args(n, (6, 5), 0)
+    model = ProjectedHF(p.nocc, p.ref, ga, factors=(p.h_spatial, p.b), precision="f16x2")
+    model.ctx.set_blocked(True)
+    model.ctx.set_spinfree(spinfree)
+    assert model.ctx.spinfree() == spinfree
+    print(f"fused split chunks n={n} N={nocc} spinfree={spinfree}", flush=True)
+    a = _run(model, p.z, True)
+    b = _run(model, p.z, False)
+    assert a[0] == b[0]
+    for x, y in zip(a[1:], b[1:]):
+        assert np.array_equal(x, y)
