@@ -2170,6 +2170,7 @@ void eval_pre(phf_ctx* ctx, const EvalBufs& b, hipEvent_t* marks) {
     // short pre-stage launches below gets ahead of the device
     hc_products(ctx, b, s);
   }
+  if (marks) (void)hipEventRecord(marks[3], s);   // the rotation stage starts after Hc
   // K1: rotation chain, right to left (Ref^H R1 and Rpg Ref are cached with the
   // grid), once per left factor L = C0 (lc = 0) or C0* (lc = 1)
   bool t_forked = false;
@@ -3148,7 +3149,7 @@ static int eval_z(phf_ctx* ctx, const double* z, int cmplxconj, int nocc, int pr
     ctx->stage_ms[2] = t12;   // ERI GEMM
     ctx->stage_ms[3] = t23;   // epilogue and grid sums
     float t04, t45, t56, t61;
-    HIPCK(hipEventElapsedTime(&t04, ctx->ev[0], ctx->ev[4]));
+    HIPCK(hipEventElapsedTime(&t04, ctx->ev[7], ctx->ev[4]));
     HIPCK(hipEventElapsedTime(&t45, ctx->ev[4], ctx->ev[5]));
     HIPCK(hipEventElapsedTime(&t56, ctx->ev[5], ctx->ev[6]));
     HIPCK(hipEventElapsedTime(&t61, ctx->ev[6], ctx->ev[1]));
